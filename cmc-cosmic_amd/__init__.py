@@ -1,0 +1,386 @@
+"""Python binding of the MI355X Henon-step hot path (libcmcg.so / libcmchost.so).
+
+The library is a C ABI (include/cmcg.h); this module only wraps it with ctypes and
+numpy so tests and the benchmark can drive it.  It never falls back to anything:
+if the HIP library is missing, importing raises.
+
+Directory name `cmc-cosmic_amd` is not a Python identifier; load it with
+``load_package()`` from ``tests/conftest.py`` / ``bench.py`` (importlib by path).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcmcg.so")
+HOST_LIB_PATH = os.path.join(HERE, "libcmchost.so")
+
+if not os.path.exists(LIB_PATH) or not os.path.exists(HOST_LIB_PATH):
+    raise ImportError(
+        "cmc-cosmic_amd: libcmcg.so / libcmchost.so not built; run __graft_entry__.build() "
+        "(make -C cmc-cosmic_amd)")
+
+lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+hostlib = C.CDLL(HOST_LIB_PATH)
+
+# ---------------------------------------------------------------- error codes
+OK, E_NAN, E_NTRY, E_BRACKET, E_WZERO, E_WPZERO, E_BRENT, E_TIES = 0, -1, -2, -3, -4, -5, -6, -7
+E_HIP, E_ARG, E_CAPACITY, E_NOMEM, E_STATE = -10, -11, -12, -13, -14
+
+SF_CIRCULAR, SF_REMOVED_E, SF_REMOVED_RT, SF_DESTROYED = 1, 2, 4, 8
+SF_BRENT, SF_LINSCAN, SF_STRIPPED, SF_BRACKET_HIT, SF_SAMPLED = 16, 32, 64, 128, 256
+PF_LEARNED_BRACKETS = 1
+
+lib.cmcg_error_string.restype = C.c_char_p
+lib.cmcg_error_string.argtypes = [C.c_int]
+
+
+class CmcgError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__(f"{where}: {lib.cmcg_error_string(code).decode()} ({code})")
+
+
+def _chk(rc, where):
+    if rc != 0:
+        raise CmcgError(rc, where)
+    return rc
+
+
+# ---------------------------------------------------------------- structs
+class Params(C.Structure):
+    _fields_ = [("n_star", C.c_int64), ("avekernel", C.c_int64), ("theta_se_max", C.c_double),
+                ("gamma", C.c_double), ("orbit_r", C.c_double), ("cenma_m", C.c_double),
+                ("seed", C.c_uint64), ("tidal_strip", C.c_int32), ("flags", C.c_int32)]
+
+
+DFIELDS = ["r", "m", "phi", "vr", "vt", "E", "J"]
+DFIELDS2 = ["rnew", "vrnew", "vtnew", "X", "Y", "r_peri", "r_apo",
+            "rOld", "Uoldrold", "Uoldrnew", "vrold", "vtold", "EI", "sigma", "pot_rnew"]
+FIELD_TYPES = {**{k: np.float64 for k in DFIELDS + DFIELDS2}, "id": np.int64,
+               "ntry": np.int32, "flags": np.int32}
+ALL_FIELDS = DFIELDS + ["id"] + DFIELDS2 + ["ntry", "flags"]
+
+_soa_fields = [("n", C.c_int64)]
+for _k in DFIELDS:
+    _soa_fields.append((_k, C.POINTER(C.c_double)))
+_soa_fields.append(("id", C.POINTER(C.c_int64)))
+for _k in DFIELDS2:
+    _soa_fields.append((_k, C.POINTER(C.c_double)))
+_soa_fields += [("ntry", C.POINTER(C.c_int32)), ("flags", C.POINTER(C.c_int32))]
+
+
+class HostSoa(C.Structure):
+    _fields_ = _soa_fields
+
+
+class Scalars(C.Structure):
+    _fields_ = [("n_max", C.c_int64), ("n_max_new", C.c_int64)] + [
+        (k, C.c_double) for k in ["Mtotal", "Rtidal", "max_r", "Dt", "Prev_Dt", "TotalTime", "TidalMassLoss",
+                                  "TidalMassLoss_old", "OldTidalMassLoss", "Eescaped", "Jescaped", "Etidal",
+                                  "Eoops", "Etot", "Ekin", "Epot"]] + [("step", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class OrbitStats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ["n_circular", "n_removed_E", "n_removed_rt", "n_destroyed", "n_brent",
+                                         "n_linscan", "tries_total", "tries_max", "n_bracket_hit",
+                                         "n_bracket_fallback"]] + [
+        (k, C.c_double) for k in ["max_r", "tidal_mass_loss", "E_escaped", "J_escaped"]] + [
+        ("error", C.c_int32), ("pad", C.c_int32), ("error_index", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+_P = C.POINTER
+_ctx = C.c_void_p
+_sigs = {
+    "cmcg_create": [_P(_ctx), C.c_int, C.c_int64, _P(Params)],
+    "cmcg_destroy": [_ctx],
+    "cmcg_upload": [_ctx, _P(HostSoa)],
+    "cmcg_download": [_ctx, _P(HostSoa)],
+    "cmcg_get_scalars": [_ctx, _P(Scalars)],
+    "cmcg_set_scalars": [_ctx, _P(Scalars)],
+    "cmcg_sync": [_ctx],
+    "cmcg_sort_by_r": [_ctx, _P(C.c_int64)],
+    "cmcg_potential": [_ctx, _P(C.c_int64), _P(C.c_double), _P(C.c_double)],
+    "cmcg_potential_at": [_ctx, C.c_int64, _P(C.c_double), _P(C.c_double)],
+    "cmcg_simul_relax": [_ctx, _P(C.c_double)],
+    "cmcg_calc_timestep": [_ctx, _P(C.c_double)],
+    "cmcg_sigma_r": [_ctx],
+    "cmcg_relax": [_ctx, C.c_double, C.c_uint64],
+    "cmcg_orbits": [_ctx, C.c_uint64, _P(OrbitStats)],
+    "cmcg_calc_pericenter_and_apocenter": [_ctx, C.c_int64, C.c_int64, _P(C.c_double), _P(C.c_double),
+                                           _P(C.c_double), _P(C.c_double), _P(C.c_double), _P(C.c_double),
+                                           _P(C.c_int64), _P(C.c_int64), _P(C.c_int32)],
+    "cmcg_find_zero_Q": [_ctx, C.c_int64, _P(C.c_int64), _P(C.c_int64), _P(C.c_int64), _P(C.c_double),
+                         _P(C.c_double), _P(C.c_int64)],
+    "cmcg_energy_conservation1": [_ctx],
+    "cmcg_energy_conservation2": [_ctx],
+    "cmcg_tidally_strip_stars": [_ctx],
+    "cmcg_compute_intermediate_energy": [_ctx],
+    "cmcg_set_velocities3": [_ctx],
+    "cmcg_compute_energy": [_ctx],
+    "cmcg_init_cluster": [_ctx],
+    "cmcg_step": [_ctx, _P(OrbitStats)],
+    "cmcg_rng_draws": [_ctx, C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int, _P(C.c_double)],
+    "cmcg_timer_enable": [_ctx, C.c_int],
+    "cmcg_timer_get": [_ctx, C.c_char_p, _P(C.c_double), _P(C.c_int64)],
+    "cmcg_timer_reset": [_ctx],
+}
+for _name, _args in _sigs.items():
+    _f = getattr(lib, _name)
+    _f.argtypes = _args
+    _f.restype = C.c_int
+
+EXPORTED = sorted(_sigs) + ["cmcg_error_string"]
+
+hostlib.cmc_ic_plummer.argtypes = [C.c_int64, C.c_uint64, C.c_int, _P(HostSoa), _P(C.c_double)]
+hostlib.cmc_ic_king.argtypes = [C.c_int64, C.c_double, C.c_uint64, C.c_int, C.c_int, _P(HostSoa), _P(C.c_double)]
+hostlib.cmc_ic_kroupa.argtypes = [C.c_int64, C.c_uint64, _P(C.c_double)]
+for _f in (hostlib.cmc_ic_plummer, hostlib.cmc_ic_king, hostlib.cmc_ic_kroupa):
+    _f.restype = C.c_int
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+# ---------------------------------------------------------------- host cluster
+class Cluster:
+    """Host SoA copy of a cluster: arrays of n+2 entries (sentinels at 0, n+1)."""
+
+    def __init__(self, n, fields=None):
+        self.n = int(n)
+        self.a = {}
+        for k in (fields or ALL_FIELDS):
+            self.a[k] = np.zeros(self.n + 2, dtype=FIELD_TYPES[k])
+        self.rtid = None
+
+    def __getitem__(self, k):
+        return self.a[k]
+
+    def __setitem__(self, k, v):
+        self.a[k] = np.ascontiguousarray(v, dtype=FIELD_TYPES[k])
+
+    def soa(self, fields=None):
+        s = HostSoa()
+        s.n = self.n
+        for k in ALL_FIELDS:
+            arr = self.a.get(k)
+            if fields is not None and k not in fields:
+                arr = None
+            if arr is not None:
+                ct = {np.float64: C.c_double, np.int64: C.c_int64, np.int32: C.c_int32}[FIELD_TYPES[k]]
+                setattr(s, k, _ptr(arr, ct))
+        return s
+
+    def copy(self):
+        c = Cluster(self.n, fields=list(self.a))
+        for k, v in self.a.items():
+            c.a[k] = v.copy()
+        c.rtid = self.rtid
+        return c
+
+
+def make_plummer(n, seed=1, kroupa=False):
+    c = Cluster(n)
+    rt = C.c_double()
+    _chk(hostlib.cmc_ic_plummer(n, seed, int(kroupa), C.byref(c.soa()), C.byref(rt)), "cmc_ic_plummer")
+    c.rtid = rt.value
+    return c
+
+
+def make_king(n, w0=7.0, seed=1, kroupa=True, segregate=True):
+    c = Cluster(n)
+    rt = C.c_double()
+    _chk(hostlib.cmc_ic_king(n, float(w0), seed, int(kroupa), int(segregate), C.byref(c.soa()), C.byref(rt)),
+         "cmc_ic_king")
+    c.rtid = rt.value
+    return c
+
+
+def kroupa_masses(n, seed=1):
+    m = np.zeros(n)
+    _chk(hostlib.cmc_ic_kroupa(n, seed, _ptr(m, C.c_double)), "cmc_ic_kroupa")
+    return m
+
+
+def default_params(n_star, orbit_r=1e3, seed=1, theta_se_max=1.412, avekernel=20, gamma=0.1, tidal_strip=1,
+                   flags=0):
+    p = Params()
+    p.n_star = int(n_star)
+    p.avekernel = int(avekernel)
+    p.theta_se_max = float(theta_se_max)
+    p.gamma = float(gamma)
+    p.orbit_r = float(orbit_r)
+    p.cenma_m = 0.0
+    p.seed = int(seed)
+    p.tidal_strip = int(tidal_strip)
+    p.flags = int(flags)
+    return p
+
+
+# ---------------------------------------------------------------- device context
+class Context:
+    def __init__(self, n_cap, params, device=0):
+        self.h = C.c_void_p()
+        self.params = params
+        _chk(lib.cmcg_create(C.byref(self.h), device, int(n_cap), C.byref(params)), "cmcg_create")
+
+    def close(self):
+        if self.h:
+            lib.cmcg_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, cl, fields=None):
+        _chk(lib.cmcg_upload(self.h, C.byref(cl.soa(fields))), "cmcg_upload")
+
+    def download(self, n_hint=None, fields=None):
+        sc = self.scalars()
+        n = sc["n_max_new"] if n_hint is None else n_hint
+        cl = Cluster(n, fields=fields)
+        s = cl.soa()
+        _chk(lib.cmcg_download(self.h, C.byref(s)), "cmcg_download")
+        cl.n = s.n
+        return cl
+
+    def scalars(self):
+        s = Scalars()
+        rc = lib.cmcg_get_scalars(self.h, C.byref(s))
+        if rc == E_HIP:
+            _chk(rc, "cmcg_get_scalars")
+        d = s.as_dict()
+        d["error"] = rc
+        return d
+
+    def set_scalars(self, **kw):
+        s = Scalars()
+        lib.cmcg_get_scalars(self.h, C.byref(s))
+        for k, v in kw.items():
+            setattr(s, k, v)
+        _chk(lib.cmcg_set_scalars(self.h, C.byref(s)), "cmcg_set_scalars")
+
+    def sync(self):
+        return lib.cmcg_sync(self.h)
+
+    def sort_by_r(self):
+        n = C.c_int64()
+        _chk(lib.cmcg_sort_by_r(self.h, C.byref(n)), "cmcg_sort_by_r")
+        return n.value
+
+    def potential(self):
+        n, mt, rt = C.c_int64(), C.c_double(), C.c_double()
+        _chk(lib.cmcg_potential(self.h, C.byref(n), C.byref(mt), C.byref(rt)), "cmcg_potential")
+        return n.value, mt.value, rt.value
+
+    def potential_at(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.zeros_like(r)
+        _chk(lib.cmcg_potential_at(self.h, r.size, _ptr(r, C.c_double), _ptr(out, C.c_double)), "cmcg_potential_at")
+        return out
+
+    def simul_relax(self):
+        d = C.c_double()
+        _chk(lib.cmcg_simul_relax(self.h, C.byref(d)), "cmcg_simul_relax")
+        return d.value
+
+    def calc_timestep(self):
+        d = C.c_double()
+        _chk(lib.cmcg_calc_timestep(self.h, C.byref(d)), "cmcg_calc_timestep")
+        return d.value
+
+    def sigma_r(self):
+        _chk(lib.cmcg_sigma_r(self.h), "cmcg_sigma_r")
+
+    def relax(self, dt, step=0):
+        _chk(lib.cmcg_relax(self.h, float(dt), int(step)), "cmcg_relax")
+
+    def orbits(self, step=0, check=True):
+        st = OrbitStats()
+        rc = lib.cmcg_orbits(self.h, int(step), C.byref(st))
+        if check:
+            _chk(rc, "cmcg_orbits")
+        return st.as_dict()
+
+    def apsides(self, first, E, J):
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        n = E.size
+        rp, ra, dp, da = (np.zeros(n) for _ in range(4))
+        kmin, kmax = np.zeros(n, np.int64), np.zeros(n, np.int64)
+        circ = np.zeros(n, np.int32)
+        _chk(lib.cmcg_calc_pericenter_and_apocenter(
+            self.h, int(first), n, _ptr(E, C.c_double), _ptr(J, C.c_double), _ptr(rp, C.c_double),
+            _ptr(ra, C.c_double), _ptr(dp, C.c_double), _ptr(da, C.c_double), _ptr(kmin, C.c_int64),
+            _ptr(kmax, C.c_int64), _ptr(circ, C.c_int32)), "cmcg_calc_pericenter_and_apocenter")
+        return dict(rp=rp, ra=ra, dQdrp=dp, dQdra=da, kmin=kmin, kmax=kmax, circular=circ)
+
+    def find_zero_Q(self, j, kmin, kmax, E, J):
+        j, kmin, kmax = (np.ascontiguousarray(x, dtype=np.int64) for x in (j, kmin, kmax))
+        E, J = (np.ascontiguousarray(x, dtype=np.float64) for x in (E, J))
+        out = np.zeros(j.size, np.int64)
+        _chk(lib.cmcg_find_zero_Q(self.h, j.size, _ptr(j, C.c_int64), _ptr(kmin, C.c_int64), _ptr(kmax, C.c_int64),
+                                  _ptr(E, C.c_double), _ptr(J, C.c_double), _ptr(out, C.c_int64)),
+             "cmcg_find_zero_Q")
+        return out
+
+    def energy_conservation1(self):
+        _chk(lib.cmcg_energy_conservation1(self.h), "cmcg_energy_conservation1")
+
+    def energy_conservation2(self):
+        _chk(lib.cmcg_energy_conservation2(self.h), "cmcg_energy_conservation2")
+
+    def tidally_strip_stars(self):
+        _chk(lib.cmcg_tidally_strip_stars(self.h), "cmcg_tidally_strip_stars")
+
+    def compute_intermediate_energy(self):
+        _chk(lib.cmcg_compute_intermediate_energy(self.h), "cmcg_compute_intermediate_energy")
+
+    def set_velocities3(self):
+        _chk(lib.cmcg_set_velocities3(self.h), "cmcg_set_velocities3")
+
+    def compute_energy(self):
+        _chk(lib.cmcg_compute_energy(self.h), "cmcg_compute_energy")
+
+    def init_cluster(self):
+        _chk(lib.cmcg_init_cluster(self.h), "cmcg_init_cluster")
+
+    def step(self, check=True):
+        st = OrbitStats()
+        rc = lib.cmcg_step(self.h, C.byref(st))
+        if check:
+            _chk(rc, "cmcg_step")
+        return st.as_dict()
+
+    def rng_draws(self, phase, step, first, count, ndraw):
+        out = np.zeros(count * ndraw)
+        _chk(lib.cmcg_rng_draws(self.h, phase, step, first, count, ndraw, _ptr(out, C.c_double)), "cmcg_rng_draws")
+        return out.reshape(count, ndraw)
+
+    def timer_enable(self, on=True):
+        lib.cmcg_timer_enable(self.h, int(on))
+
+    def timer_get(self, name):
+        ms, calls = C.c_double(), C.c_int64()
+        lib.cmcg_timer_get(self.h, name.encode(), C.byref(ms), C.byref(calls))
+        return ms.value, calls.value
+
+    def timer_reset(self):
+        lib.cmcg_timer_reset(self.h)

@@ -1,0 +1,794 @@
+// C-ABI of the cmcg library: context lifetime, host<->device transfers, the
+// hot-path phase entry points and the whole Henon step (src/cmc/cmc.c:240-407).
+#include "kernels.hpp"
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+void cmcg_report_hip(hipError_t e, const char *expr, const char *file, int line)
+{
+    if (getenv("CMCG_VERBOSE")) fprintf(stderr, "cmcg: HIP error %s (%d) in %s at %s:%d\n", hipGetErrorString(e), (int)e, expr, file, line);
+}
+
+struct cmcg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n_cap = 0;
+    cmcg_params params{};
+    KParams kp{};
+    DevFields f{}, alt{};
+    DevScalars *s = nullptr;
+    RngTable *T = nullptr;
+    Scratch sc{};
+    std::vector<void *> allocs;
+    int64_t n_ub = 0;
+    // timing
+    bool timing = false;
+    std::map<std::string, std::pair<double, int64_t>> timers;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    // host-side copy of the seeded RNG state
+    Rng seeded{};
+    int64_t last_sort_ties = 0;
+};
+
+namespace {
+
+// ------------------------------------------------------------------ host RNG setup
+// taus113 seeding, src/libs/taus113-v2.c:197-216
+Rng host_seed(uint64_t s)
+{
+    auto step = [](Rng &st) {
+        uint32_t b;
+        b = ((st.z0 << 6) ^ st.z0) >> 13;  st.z0 = ((st.z0 & 4294967294u) << 18) ^ b;
+        b = ((st.z1 << 2) ^ st.z1) >> 27;  st.z1 = ((st.z1 & 4294967288u) << 2) ^ b;
+        b = ((st.z2 << 13) ^ st.z2) >> 21; st.z2 = ((st.z2 & 4294967280u) << 7) ^ b;
+        b = ((st.z3 << 3) ^ st.z3) >> 12;  st.z3 = ((st.z3 & 4294967168u) << 13) ^ b;
+    };
+    if (s == 0) s = 1;
+    Rng st;
+    uint64_t z;
+    z = (69069ull * s) & 0xffffffffull;     if (z < 2) z += 2;     st.z0 = (uint32_t)z;
+    z = (69069ull * st.z0) & 0xffffffffull; if (z < 8) z += 8;     st.z1 = (uint32_t)z;
+    z = (69069ull * st.z1) & 0xffffffffull; if (z < 16) z += 16;   st.z2 = (uint32_t)z;
+    z = (69069ull * st.z2) & 0xffffffffull; if (z < 128) z += 128; st.z3 = (uint32_t)z;
+    for (int i = 0; i < 10; i++) step(st);
+    return st;
+}
+
+// Column i of the one-step map of component j applied to e_i.
+uint32_t comp_step(int j, uint32_t z)
+{
+    switch (j) {
+    case 0: return ((z & 4294967294u) << 18) ^ (((z << 6) ^ z) >> 13);
+    case 1: return ((z & 4294967288u) << 2) ^ (((z << 2) ^ z) >> 27);
+    case 2: return ((z & 4294967280u) << 7) ^ (((z << 13) ^ z) >> 21);
+    default: return ((z & 4294967168u) << 13) ^ (((z << 3) ^ z) >> 12);
+    }
+}
+
+uint32_t host_matvec(const uint32_t *col, uint32_t v)
+{
+    uint32_t o = 0;
+    for (int i = 0; i < 32; i++)
+        if ((v >> i) & 1u) o ^= col[i];
+    return o;
+}
+
+void build_table(RngTable &t)
+{
+    for (int j = 0; j < 4; j++)
+        for (int i = 0; i < 32; i++) t.col[0][j][i] = comp_step(j, 1u << i);
+    for (int b = 1; b < 128; b++)
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 32; i++) t.col[b][j][i] = host_matvec(t.col[b - 1][j], t.col[b - 1][j][i]);
+}
+
+template <typename T>
+int dalloc(cmcg_ctx *c, T **p, size_t count)
+{
+    void *q = nullptr;
+    if (hipMalloc(&q, count * sizeof(T) > 0 ? count * sizeof(T) : 8) != hipSuccess) return CMCG_E_NOMEM;
+    if (hipMemset(q, 0, count * sizeof(T) > 0 ? count * sizeof(T) : 8) != hipSuccess) return CMCG_E_HIP;
+    c->allocs.push_back(q);
+    *p = (T *)q;
+    return CMCG_OK;
+}
+
+int alloc_fields(cmcg_ctx *c, DevFields &f, bool full)
+{
+    size_t n = (size_t)c->n_cap + 2;
+    double **dfields_sorted[] = {&f.r, &f.m, &f.vr, &f.vt, &f.E, &f.J, &f.rnew, &f.vrnew, &f.vtnew, &f.X, &f.Y,
+                                 &f.r_peri, &f.r_apo, &f.rOld, &f.Uoldrold, &f.Uoldrnew, &f.vrold, &f.vtold, &f.EI};
+    for (auto pp : dfields_sorted)
+        if (int e = dalloc(c, pp, n)) return e;
+    if (int e = dalloc(c, &f.id, n)) return e;
+    if (full) {
+        if (int e = dalloc(c, &f.phi, n)) return e;
+        if (int e = dalloc(c, &f.sigma, n)) return e;
+        if (int e = dalloc(c, &f.pot_rnew, n)) return e;
+        if (int e = dalloc(c, &f.ntry, n)) return e;
+        if (int e = dalloc(c, &f.flags, n)) return e;
+    }
+    return CMCG_OK;
+}
+
+int check_error(cmcg_ctx *c)
+{
+    int32_t err = 0;
+    if (hipMemcpy(&err, &c->s->error, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess) return CMCG_E_HIP;
+    return err;
+}
+
+hipError_t sync(cmcg_ctx *c) { return hipStreamSynchronize(c->stream); }
+
+hipEvent_t get_event(cmcg_ctx *c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+struct PhaseTimer {
+    cmcg_ctx *c;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    PhaseTimer(cmcg_ctx *c_, const char *n) : c(c_), name(n)
+    {
+        if (c->timing) { a = get_event(c); hipEventRecord(a, c->stream); }
+    }
+    ~PhaseTimer()
+    {
+        if (c->timing) {
+            b = get_event(c);
+            hipEventRecord(b, c->stream);
+            c->pending.push_back({name, {a, b}});
+        }
+    }
+};
+
+void collect_timers(cmcg_ctx *c)
+{
+    for (auto &p : c->pending) {
+        hipEventSynchronize(p.second.second);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p.second.first, p.second.second);
+        auto &acc = c->timers[p.first];
+        acc.first += ms;
+        acc.second += 1;
+        c->event_pool.push_back(p.second.first);
+        c->event_pool.push_back(p.second.second);
+    }
+    c->pending.clear();
+}
+
+__global__ void set_dt_step_kernel(DevScalars *s, double dt, int64_t step)
+{
+    if (dt == dt) s->Dt = dt;
+    if (step >= 0) s->step = (uint64_t)step;
+}
+__global__ void step_end_kernel(DevScalars *s) { s->step += 1; }
+
+// payload field lists for the sort
+struct SortSpec {
+    int nf = 0;
+    const uint64_t *src[24];
+    uint64_t *dst[24];
+    double **swap_a[24];
+    double **swap_b[24];
+    int64_t **swap_ia = nullptr, **swap_ib = nullptr;
+};
+
+void add_field(SortSpec &sp, double *src, double *dst, double **fa, double **fb)
+{
+    sp.src[sp.nf] = (const uint64_t *)src;
+    sp.dst[sp.nf] = (uint64_t *)dst;
+    sp.swap_a[sp.nf] = fa;
+    sp.swap_b[sp.nf] = fb;
+    sp.nf++;
+}
+
+// Runs the sort and swaps every destination field into the live set.
+int run_sort(cmcg_ctx *c, bool fused)
+{
+    DevFields &f = c->f, &a = c->alt;
+    SortSpec sp;
+    const double *key = fused ? f.rnew : f.r;
+    if (fused) {
+        // ComputeIntermediateEnergy's transfer (cmc_utils.c:382-390) fused into the gather:
+        // r <- rnew (the key), rOld <- r, vr <- vrnew, vt <- vtnew.
+        add_field(sp, f.m, a.m, &f.m, &a.m);
+        add_field(sp, f.vrnew, a.vr, &f.vr, &a.vr);
+        add_field(sp, f.vtnew, a.vt, &f.vt, &a.vt);
+        add_field(sp, f.r, a.rOld, &f.rOld, &a.rOld);
+        add_field(sp, f.Uoldrold, a.Uoldrold, &f.Uoldrold, &a.Uoldrold);
+        add_field(sp, f.Uoldrnew, a.Uoldrnew, &f.Uoldrnew, &a.Uoldrnew);
+        add_field(sp, f.vrold, a.vrold, &f.vrold, &a.vrold);
+        add_field(sp, f.vtold, a.vtold, &f.vtold, &a.vtold);
+        add_field(sp, f.r_apo, a.r_apo, &f.r_apo, &a.r_apo);
+        add_field(sp, f.r_peri, a.r_peri, &f.r_peri, &a.r_peri);
+    } else {
+        double *DevFields::*flds[] = {&DevFields::m, &DevFields::vr, &DevFields::vt, &DevFields::E,
+                                      &DevFields::J, &DevFields::rnew, &DevFields::vrnew, &DevFields::vtnew,
+                                      &DevFields::X, &DevFields::Y, &DevFields::r_peri, &DevFields::r_apo,
+                                      &DevFields::rOld, &DevFields::Uoldrold, &DevFields::Uoldrnew,
+                                      &DevFields::vrold, &DevFields::vtold, &DevFields::EI};
+        for (auto fp : flds) add_field(sp, f.*fp, a.*fp, &(f.*fp), &(a.*fp));
+    }
+    // id travels as 8-byte payload too
+    sp.src[sp.nf] = (const uint64_t *)f.id;
+    sp.dst[sp.nf] = (uint64_t *)a.id;
+    sp.nf++;
+    hipError_t e = launch_sort_fields(key, a.r, f.id, sp.nf, sp.src, sp.dst, c->s, c->sc, c->n_ub, c->stream);
+    if (e != hipSuccess) { cmcg_report_hip(e, "launch_sort_fields", __FILE__, __LINE__); return CMCG_E_HIP; }
+    // swap live/alt pointers of every written field
+    std::swap(f.r, a.r);
+    for (int i = 0; i < sp.nf - 1; i++) std::swap(*sp.swap_a[i], *sp.swap_b[i]);
+    std::swap(f.id, a.id);
+    if (fused) {
+        // the old vr/vt buffers are now stale copies; keep vrnew/vtnew as they were
+    }
+    return CMCG_OK;
+}
+
+int read_scalars(cmcg_ctx *c, DevScalars &h)
+{
+    if (sync(c) != hipSuccess) return CMCG_E_HIP;
+    if (hipMemcpy(&h, c->s, sizeof(DevScalars), hipMemcpyDeviceToHost) != hipSuccess) return CMCG_E_HIP;
+    collect_timers(c);
+    return CMCG_OK;
+}
+
+#define LAUNCH(expr)                                                                 \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) { cmcg_report_hip(_e, #expr, __FILE__, __LINE__); return CMCG_E_HIP; } \
+    } while (0)
+
+} // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+const char *cmcg_error_string(int code)
+{
+    switch (code) {
+    case CMCG_OK: return "ok";
+    case CMCG_E_NAN: return "NaN detected";
+    case CMCG_E_NTRY: return "N_TRY exceeded";
+    case CMCG_E_BRACKET: return "bisection bracket failure";
+    case CMCG_E_WZERO: return "W = 0 in encounter";
+    case CMCG_E_WPZERO: return "wp = 0 in relaxation";
+    case CMCG_E_BRENT: return "apsis root finder failed";
+    case CMCG_E_TIES: return "run of equal radii too long";
+    case CMCG_E_HIP: return "HIP runtime error";
+    case CMCG_E_ARG: return "invalid argument";
+    case CMCG_E_CAPACITY: return "capacity exceeded";
+    case CMCG_E_NOMEM: return "device allocation failed";
+    case CMCG_E_STATE: return "call out of order";
+    default: return "unknown error";
+    }
+}
+
+int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *params)
+{
+    if (!out || !params || n_cap < 1 || params->n_star < 1 || params->avekernel < 1) return CMCG_E_ARG;
+    if (n_cap > (int64_t)0x7fffffff) return CMCG_E_CAPACITY;
+    *out = nullptr;
+    cmcg_ctx *c = new cmcg_ctx();
+    c->device = device;
+    c->n_cap = n_cap;
+    c->params = *params;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return CMCG_E_HIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CMCG_E_HIP; }
+    KParams &P = c->kp;
+    P.n_star = params->n_star;
+    P.inv_nstar = 1.0 / params->n_star;
+    P.madhoc = 1.0 / ((double)params->n_star);
+    P.cenma_m = params->cenma_m;
+    P.orbit_r = params->orbit_r;
+    P.theta_se_max = params->theta_se_max;
+    P.gamma = params->gamma;
+    P.avekernel = params->avekernel;
+    int e;
+    if ((e = alloc_fields(c, c->f, true)) || (e = alloc_fields(c, c->alt, false))) { cmcg_destroy(c); return e; }
+    c->alt.phi = c->f.phi; c->alt.sigma = c->f.sigma; c->alt.pot_rnew = c->f.pot_rnew;
+    c->alt.ntry = c->f.ntry; c->alt.flags = c->f.flags;
+    if ((e = dalloc(c, &c->s, 1))) { cmcg_destroy(c); return e; }
+    Scratch &sc = c->sc;
+    sc.max_tiles = (n_cap + 2047) / 2048 + 1;
+    sc.max_blocks = (n_cap + 255) / 256 + 2;
+    sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
+    sc.max_events = 4096;
+    if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 6)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 8)) ||
+        (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 1)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 1)) ||
+        (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 1)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 1)) ||
+        (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
+        (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 2)) || (e = dalloc(c, &sc.excess_list, (size_t)sc.max_events)) ||
+        (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
+        (e = dalloc(c, &c->T, 1))) {
+        cmcg_destroy(c);
+        return e;
+    }
+    RngTable *ht = new RngTable;
+    build_table(*ht);
+    c->seeded = host_seed(params->seed);
+    bool ok = hipMemcpy(c->T, ht, sizeof(RngTable), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(sc.rng_seeded, &c->seeded, sizeof(Rng), hipMemcpyHostToDevice) == hipSuccess;
+    delete ht;
+    if (!ok) { cmcg_destroy(c); return CMCG_E_HIP; }
+    *out = c;
+    return CMCG_OK;
+}
+
+int cmcg_destroy(cmcg_ctx *c)
+{
+    if (!c) return CMCG_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &p : c->pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    for (void *p : c->allocs) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return CMCG_OK;
+}
+
+int cmcg_sync(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    if (sync(c) != hipSuccess) return CMCG_E_HIP;
+    collect_timers(c);
+    return check_error(c);
+}
+
+#define H2D(dst, srcp, count)                                                                     \
+    if (srcp) {                                                                                   \
+        if (hipMemcpyAsync(dst, srcp, (count) * sizeof(*(srcp)), hipMemcpyHostToDevice, c->stream) != hipSuccess) \
+            return CMCG_E_HIP;                                                                    \
+    }
+#define D2H(dstp, src, count)                                                                     \
+    if (dstp) {                                                                                   \
+        if (hipMemcpyAsync(dstp, src, (count) * sizeof(*(dstp)), hipMemcpyDeviceToHost, c->stream) != hipSuccess) \
+            return CMCG_E_HIP;                                                                    \
+    }
+
+int cmcg_upload(cmcg_ctx *c, const cmcg_host_soa *h)
+{
+    if (!c || !h || h->n < 0) return CMCG_E_ARG;
+    if (h->n > c->n_cap) return CMCG_E_CAPACITY;
+    size_t n = (size_t)h->n + 2;
+    DevFields &f = c->f;
+    H2D(f.r, h->r, n); H2D(f.m, h->m, n); H2D(f.phi, h->phi, n);
+    H2D(f.vr, h->vr, n); H2D(f.vt, h->vt, n); H2D(f.E, h->E, n); H2D(f.J, h->J, n);
+    H2D(f.id, h->id, n);
+    H2D(f.rnew, h->rnew, n); H2D(f.vrnew, h->vrnew, n); H2D(f.vtnew, h->vtnew, n);
+    H2D(f.X, h->X, n); H2D(f.Y, h->Y, n); H2D(f.r_peri, h->r_peri, n); H2D(f.r_apo, h->r_apo, n);
+    H2D(f.rOld, h->rOld, n); H2D(f.Uoldrold, h->Uoldrold, n); H2D(f.Uoldrnew, h->Uoldrnew, n);
+    H2D(f.vrold, h->vrold, n); H2D(f.vtold, h->vtold, n); H2D(f.EI, h->EI, n);
+    H2D(f.sigma, h->sigma, n); H2D(f.pot_rnew, h->pot_rnew, n);
+    H2D(f.ntry, h->ntry, n); H2D(f.flags, h->flags, n);
+    DevScalars hs;
+    if (hipMemcpyAsync(&hs, c->s, sizeof(hs), hipMemcpyDeviceToHost, c->stream) != hipSuccess) return CMCG_E_HIP;
+    if (sync(c) != hipSuccess) return CMCG_E_HIP;
+    hs.n_max = h->n;
+    hs.n_max_new = h->n;
+    if (hipMemcpy(c->s, &hs, sizeof(hs), hipMemcpyHostToDevice) != hipSuccess) return CMCG_E_HIP;
+    c->n_ub = h->n;
+    return CMCG_OK;
+}
+
+int cmcg_download(cmcg_ctx *c, cmcg_host_soa *h)
+{
+    if (!c || !h) return CMCG_E_ARG;
+    DevScalars hs;
+    if (int e = read_scalars(c, hs)) return e;
+    size_t n = (size_t)hs.n_max_new + 2;
+    DevFields &f = c->f;
+    D2H(h->r, f.r, n); D2H(h->m, f.m, n); D2H(h->phi, f.phi, n);
+    D2H(h->vr, f.vr, n); D2H(h->vt, f.vt, n); D2H(h->E, f.E, n); D2H(h->J, f.J, n);
+    D2H(h->id, f.id, n);
+    D2H(h->rnew, f.rnew, n); D2H(h->vrnew, f.vrnew, n); D2H(h->vtnew, f.vtnew, n);
+    D2H(h->X, f.X, n); D2H(h->Y, f.Y, n); D2H(h->r_peri, f.r_peri, n); D2H(h->r_apo, f.r_apo, n);
+    D2H(h->rOld, f.rOld, n); D2H(h->Uoldrold, f.Uoldrold, n); D2H(h->Uoldrnew, f.Uoldrnew, n);
+    D2H(h->vrold, f.vrold, n); D2H(h->vtold, f.vtold, n); D2H(h->EI, f.EI, n);
+    D2H(h->sigma, f.sigma, n); D2H(h->pot_rnew, f.pot_rnew, n);
+    D2H(h->ntry, f.ntry, n); D2H(h->flags, f.flags, n);
+    if (sync(c) != hipSuccess) return CMCG_E_HIP;
+    h->n = hs.n_max_new;
+    return CMCG_OK;
+}
+
+int cmcg_get_scalars(cmcg_ctx *c, cmcg_scalars *o)
+{
+    if (!c || !o) return CMCG_E_ARG;
+    DevScalars h;
+    if (int e = read_scalars(c, h)) return e;
+    o->n_max = h.n_max; o->n_max_new = h.n_max_new;
+    o->Mtotal = h.Mtotal; o->Rtidal = h.Rtidal; o->max_r = h.max_r; o->Dt = h.Dt; o->Prev_Dt = h.Prev_Dt;
+    o->TotalTime = h.TotalTime; o->TidalMassLoss = h.TidalMassLoss; o->TidalMassLoss_old = h.TidalMassLoss_old;
+    o->OldTidalMassLoss = h.OldTidalMassLoss; o->Eescaped = h.Eescaped; o->Jescaped = h.Jescaped;
+    o->Etidal = h.Etidal; o->Eoops = h.Eoops; o->Etot = h.Etot; o->Ekin = h.Ekin; o->Epot = h.Epot;
+    o->step = h.step;
+    return h.error;
+}
+
+int cmcg_set_scalars(cmcg_ctx *c, const cmcg_scalars *in)
+{
+    if (!c || !in) return CMCG_E_ARG;
+    if (in->n_max > c->n_cap || in->n_max_new > c->n_cap) return CMCG_E_CAPACITY;
+    DevScalars h;
+    if (int e = read_scalars(c, h)) { if (e < 0 && e == CMCG_E_HIP) return e; }
+    h.n_max = in->n_max; h.n_max_new = in->n_max_new;
+    h.Mtotal = in->Mtotal; h.Rtidal = in->Rtidal; h.max_r = in->max_r; h.Dt = in->Dt; h.Prev_Dt = in->Prev_Dt;
+    h.TotalTime = in->TotalTime; h.TidalMassLoss = in->TidalMassLoss; h.TidalMassLoss_old = in->TidalMassLoss_old;
+    h.OldTidalMassLoss = in->OldTidalMassLoss; h.Eescaped = in->Eescaped; h.Jescaped = in->Jescaped;
+    h.Etidal = in->Etidal; h.Eoops = in->Eoops; h.Etot = in->Etot; h.Ekin = in->Ekin; h.Epot = in->Epot;
+    h.step = in->step;
+    h.error = 0; h.error_index = 0;
+    if (hipMemcpy(c->s, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) return CMCG_E_HIP;
+    c->n_ub = in->n_max_new > in->n_max ? in->n_max_new : in->n_max;
+    return CMCG_OK;
+}
+
+int cmcg_sort_by_r(cmcg_ctx *c, int64_t *n_max)
+{
+    if (!c) return CMCG_E_ARG;
+    {
+        PhaseTimer t(c, "sort");
+        if (int e = run_sort(c, false)) return e;
+    }
+    DevScalars h;
+    if (int e = read_scalars(c, h)) return e;
+    c->n_ub = h.n_max;
+    c->last_sort_ties = h.ntie_runs;
+    if (n_max) *n_max = h.n_max;
+    return h.error;
+}
+
+int cmcg_potential(cmcg_ctx *c, int64_t *n_max, double *mtotal, double *rtidal)
+{
+    if (!c) return CMCG_E_ARG;
+    {
+        PhaseTimer t(c, "potential");
+        LAUNCH(launch_potential(c->f, c->s, c->kp, c->n_ub, c->sc, c->stream));
+    }
+    DevScalars h;
+    if (int e = read_scalars(c, h)) return e;
+    c->n_ub = h.n_max;
+    if (n_max) *n_max = h.n_max;
+    if (mtotal) *mtotal = h.Mtotal;
+    if (rtidal) *rtidal = h.Rtidal;
+    return h.error;
+}
+
+int cmcg_potential_at(cmcg_ctx *c, int64_t count, const double *r, double *out)
+{
+    if (!c || count < 0 || (count > 0 && (!r || !out))) return CMCG_E_ARG;
+    if (count == 0) return CMCG_OK;
+    double *dx = nullptr, *dy = nullptr;
+    if (hipMalloc(&dx, count * sizeof(double)) != hipSuccess) return CMCG_E_NOMEM;
+    if (hipMalloc(&dy, count * sizeof(double)) != hipSuccess) { hipFree(dx); return CMCG_E_NOMEM; }
+    int rc = CMCG_OK;
+    if (hipMemcpyAsync(dx, r, count * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        launch_potential_at(c->f, c->s, c->kp, dx, dy, count, c->stream) != hipSuccess ||
+        hipMemcpyAsync(out, dy, count * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        sync(c) != hipSuccess)
+        rc = CMCG_E_HIP;
+    hipFree(dx);
+    hipFree(dy);
+    if (rc) return rc;
+    return check_error(c);
+}
+
+int cmcg_simul_relax(cmcg_ctx *c, double *dtrel)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_simul_relax(c->f, c->s, c->kp, c->n_ub, c->stream));
+    DevScalars h;
+    if (int e = read_scalars(c, h)) return e;
+    if (dtrel) *dtrel = h.dtrel;
+    return h.error;
+}
+
+int cmcg_calc_timestep(cmcg_ctx *c, double *dt)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_simul_relax(c->f, c->s, c->kp, c->n_ub, c->stream));
+    LAUNCH(launch_timestep_finalize(c->s, c->stream));
+    DevScalars h;
+    if (int e = read_scalars(c, h)) return e;
+    if (dt) *dt = h.Dt;
+    return h.error;
+}
+
+int cmcg_sigma_r(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_sigma(c->f, c->s, c->kp, c->n_ub, c->stream));
+    return cmcg_sync(c);
+}
+
+int cmcg_relax(cmcg_ctx *c, double dt, uint64_t step)
+{
+    if (!c) return CMCG_E_ARG;
+    {
+        PhaseTimer t(c, "relax");
+        hipLaunchKernelGGL(set_dt_step_kernel, dim3(1), dim3(1), 0, c->stream, c->s, dt, (int64_t)step);
+        LAUNCH(launch_rng_bases(c->T, c->s, c->sc, c->stream));
+        LAUNCH(launch_sigma(c->f, c->s, c->kp, c->n_ub, c->stream));
+        LAUNCH(launch_relax(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream));
+    }
+    return cmcg_sync(c);
+}
+
+static void fill_stats(const DevScalars &h, cmcg_orbit_stats *st)
+{
+    if (!st) return;
+    st->n_circular = h.n_circular; st->n_removed_E = h.n_removed_E; st->n_removed_rt = h.n_removed_rt;
+    st->n_destroyed = h.n_destroyed; st->n_brent = h.n_brent; st->n_linscan = h.n_linscan;
+    st->tries_total = h.tries_total; st->tries_max = h.tries_max;
+    st->n_bracket_hit = h.n_bracket_hit; st->n_bracket_fallback = h.n_bracket_fallback;
+    st->max_r = h.max_r; st->tidal_mass_loss = h.TidalMassLoss; st->E_escaped = h.Eescaped; st->J_escaped = h.Jescaped;
+    st->error = h.error; st->error_index = h.error_index;
+}
+
+int cmcg_orbits(cmcg_ctx *c, uint64_t step, cmcg_orbit_stats *stats)
+{
+    if (!c) return CMCG_E_ARG;
+    {
+        PhaseTimer t(c, "orbits");
+        hipLaunchKernelGGL(set_dt_step_kernel, dim3(1), dim3(1), 0, c->stream, c->s, __builtin_nan(""), (int64_t)step);
+        LAUNCH(launch_rng_bases(c->T, c->s, c->sc, c->stream));
+        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream));
+    }
+    DevScalars h;
+    if (int e = read_scalars(c, h)) return e;
+    fill_stats(h, stats);
+    return h.error;
+}
+
+int cmcg_calc_pericenter_and_apocenter(cmcg_ctx *c, int64_t first, int64_t count, const double *E, const double *J,
+                                       double *rp, double *ra, double *dQdrp, double *dQdra, int64_t *kmin,
+                                       int64_t *kmax, int32_t *circular)
+{
+    if (!c || count < 0 || !E || !J) return CMCG_E_ARG;
+    if (count == 0) return CMCG_OK;
+    if (first < 1 || first + count - 1 > c->n_ub) return CMCG_E_ARG;
+    double *dE, *dJ, *d4;
+    int64_t *dk;
+    int32_t *dc;
+    if (hipMalloc(&dE, count * 8) || hipMalloc(&dJ, count * 8) || hipMalloc(&d4, count * 32) ||
+        hipMalloc(&dk, count * 16) || hipMalloc(&dc, count * 4))
+        return CMCG_E_NOMEM;
+    std::vector<double> h4(4 * count);
+    std::vector<int64_t> hk(2 * count);
+    std::vector<int32_t> hc(count);
+    int rc = CMCG_OK;
+    if (hipMemcpyAsync(dE, E, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dJ, J, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        launch_apsides(c->f, c->s, c->kp, first, count, dE, dJ, d4, dk, dc, c->stream) ||
+        hipMemcpyAsync(h4.data(), d4, count * 32, hipMemcpyDeviceToHost, c->stream) ||
+        hipMemcpyAsync(hk.data(), dk, count * 16, hipMemcpyDeviceToHost, c->stream) ||
+        hipMemcpyAsync(hc.data(), dc, count * 4, hipMemcpyDeviceToHost, c->stream) || sync(c))
+        rc = CMCG_E_HIP;
+    hipFree(dE); hipFree(dJ); hipFree(d4); hipFree(dk); hipFree(dc);
+    if (rc) return rc;
+    for (int64_t i = 0; i < count; i++) {
+        if (rp) rp[i] = h4[4 * i];
+        if (ra) ra[i] = h4[4 * i + 1];
+        if (dQdrp) dQdrp[i] = h4[4 * i + 2];
+        if (dQdra) dQdra[i] = h4[4 * i + 3];
+        if (kmin) kmin[i] = hk[2 * i];
+        if (kmax) kmax[i] = hk[2 * i + 1];
+        if (circular) circular[i] = hc[i];
+    }
+    return check_error(c);
+}
+
+int cmcg_find_zero_Q(cmcg_ctx *c, int64_t count, const int64_t *j, const int64_t *kmin, const int64_t *kmax,
+                     const double *E, const double *J, int64_t *out)
+{
+    if (!c || count < 0 || (count > 0 && (!j || !kmin || !kmax || !E || !J || !out))) return CMCG_E_ARG;
+    if (count == 0) return CMCG_OK;
+    for (int64_t i = 0; i < count; i++)
+        if (j[i] < 0 || j[i] > c->n_ub + 1 || kmin[i] < 0 || kmax[i] > c->n_ub + 1 || kmin[i] > kmax[i])
+            return CMCG_E_ARG;
+    int64_t *dj, *dmin, *dmax, *dout;
+    double *dE, *dJ;
+    if (hipMalloc(&dj, count * 8) || hipMalloc(&dmin, count * 8) || hipMalloc(&dmax, count * 8) ||
+        hipMalloc(&dout, count * 8) || hipMalloc(&dE, count * 8) || hipMalloc(&dJ, count * 8))
+        return CMCG_E_NOMEM;
+    int rc = CMCG_OK;
+    if (hipMemcpyAsync(dj, j, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dmin, kmin, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dmax, kmax, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dE, E, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dJ, J, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        launch_find_zero_Q(c->f, c->s, c->kp, count, dj, dmin, dmax, dE, dJ, dout, c->stream) ||
+        hipMemcpyAsync(out, dout, count * 8, hipMemcpyDeviceToHost, c->stream) || sync(c))
+        rc = CMCG_E_HIP;
+    hipFree(dj); hipFree(dmin); hipFree(dmax); hipFree(dout); hipFree(dE); hipFree(dJ);
+    if (rc) return rc;
+    return check_error(c);
+}
+
+int cmcg_energy_conservation1(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_energy_conservation1(c->f, c->s, c->kp, c->n_ub, c->stream));
+    return cmcg_sync(c);
+}
+
+int cmcg_energy_conservation2(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_energy_conservation2(c->f, c->s, c->kp, c->n_ub, c->stream));
+    return cmcg_sync(c);
+}
+
+int cmcg_tidally_strip_stars(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, c->n_ub, 1, true, c->stream));
+    for (int guard = 0; guard < 100000; guard++) {
+        int32_t done = 0;
+        if (sync(c) != hipSuccess) return CMCG_E_HIP;
+        if (hipMemcpy(&done, c->sc.strip_done, 4, hipMemcpyDeviceToHost) != hipSuccess) return CMCG_E_HIP;
+        if (done) break;
+        LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, c->n_ub, 1, false, c->stream));
+    }
+    return cmcg_sync(c);
+}
+
+int cmcg_compute_intermediate_energy(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_intermediate_energy(c->f, c->s, c->kp, c->n_ub, c->stream));
+    LAUNCH(launch_ie_transfer(c->f, c->s, c->n_ub, c->stream));
+    return cmcg_sync(c);
+}
+
+int cmcg_set_velocities3(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_set_velocities3(c->f, c->s, c->kp, c->sc, c->n_ub, c->stream));
+    return cmcg_sync(c);
+}
+
+int cmcg_compute_energy(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_compute_energy(c->f, c->s, c->kp, c->sc, c->n_ub, c->stream));
+    return cmcg_sync(c);
+}
+
+int cmcg_init_cluster(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    LAUNCH(launch_potential(c->f, c->s, c->kp, c->n_ub, c->sc, c->stream));
+    LAUNCH(launch_sigma(c->f, c->s, c->kp, c->n_ub, c->stream));
+    LAUNCH(launch_compute_energy(c->f, c->s, c->kp, c->sc, c->n_ub, c->stream));
+    DevScalars h;
+    if (int e = read_scalars(c, h)) return e;
+    c->n_ub = h.n_max;
+    return h.error;
+}
+
+int cmcg_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
+{
+    if (!c) return CMCG_E_ARG;
+    const int64_t nub = c->n_ub;
+    {
+        PhaseTimer t(c, "timestep");
+        LAUNCH(launch_step_begin(c->s, c->stream));
+        LAUNCH(launch_simul_relax(c->f, c->s, c->kp, nub, c->stream));
+        LAUNCH(launch_timestep_finalize(c->s, c->stream));
+        LAUNCH(launch_rng_bases(c->T, c->s, c->sc, c->stream));
+    }
+    {
+        PhaseTimer t(c, "relax");
+        LAUNCH(launch_sigma(c->f, c->s, c->kp, nub, c->stream));
+        LAUNCH(launch_relax(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream));
+        LAUNCH(launch_after_dynamics(c->s, c->stream));
+    }
+    {
+        PhaseTimer t(c, "ec1");
+        LAUNCH(launch_energy_conservation1(c->f, c->s, c->kp, nub, c->stream));
+    }
+    {
+        PhaseTimer t(c, "orbits");
+        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream));
+    }
+    {
+        PhaseTimer t(c, "glue_pre_sort");
+        LAUNCH(launch_energy_conservation2(c->f, c->s, c->kp, nub, c->stream));
+        if (c->params.tidal_strip) LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, 2, true, c->stream));
+    }
+    if (c->params.tidal_strip) {
+        // the stripping loop normally ends within two passes; otherwise continue it here
+        for (int guard = 0; guard < 100000; guard++) {
+            int32_t done = 0;
+            if (sync(c) != hipSuccess) return CMCG_E_HIP;
+            if (hipMemcpy(&done, c->sc.strip_done, 4, hipMemcpyDeviceToHost) != hipSuccess) return CMCG_E_HIP;
+            if (done) break;
+            LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, 1, false, c->stream));
+        }
+    }
+    {
+        PhaseTimer t(c, "ie");
+        LAUNCH(launch_intermediate_energy(c->f, c->s, c->kp, nub, c->stream));
+    }
+    {
+        PhaseTimer t(c, "sort");
+        if (int e = run_sort(c, true)) return e;
+    }
+    {
+        PhaseTimer t(c, "potential");
+        LAUNCH(launch_potential(c->f, c->s, c->kp, nub, c->sc, c->stream));
+    }
+    {
+        PhaseTimer t(c, "glue_post_sort");
+        LAUNCH(launch_set_velocities3(c->f, c->s, c->kp, c->sc, nub, c->stream));
+        LAUNCH(launch_compute_energy(c->f, c->s, c->kp, c->sc, nub, c->stream));
+        hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->s);
+    }
+    DevScalars h;
+    if (int e = read_scalars(c, h)) return e;
+    c->n_ub = h.n_max;
+    fill_stats(h, stats);
+    return h.error;
+}
+
+int cmcg_rng_draws(cmcg_ctx *c, int phase, uint64_t step, int64_t first, int64_t count, int ndraw, double *out)
+{
+    if (!c || (phase != 0 && phase != 1) || first < 0 || count < 0 || ndraw < 1 || (count > 0 && !out))
+        return CMCG_E_ARG;
+    if (count == 0) return CMCG_OK;
+    double *d = nullptr;
+    if (hipMalloc(&d, (size_t)count * ndraw * 8) != hipSuccess) return CMCG_E_NOMEM;
+    int rc = CMCG_OK;
+    if (launch_rng_bases_step(c->T, c->s, c->sc, (int64_t)step, c->stream) ||
+        launch_rng_draws(c->T, c->sc.rng_base, phase, first, count, ndraw, d, c->stream) ||
+        hipMemcpyAsync(out, d, (size_t)count * ndraw * 8, hipMemcpyDeviceToHost, c->stream) || sync(c))
+        rc = CMCG_E_HIP;
+    hipFree(d);
+    return rc;
+}
+
+int cmcg_timer_enable(cmcg_ctx *c, int on)
+{
+    if (!c) return CMCG_E_ARG;
+    c->timing = on != 0;
+    return CMCG_OK;
+}
+
+int cmcg_timer_get(cmcg_ctx *c, const char *name, double *ms_total, int64_t *calls)
+{
+    if (!c || !name) return CMCG_E_ARG;
+    if (sync(c) != hipSuccess) return CMCG_E_HIP;
+    collect_timers(c);
+    auto it = c->timers.find(name);
+    if (ms_total) *ms_total = it == c->timers.end() ? 0.0 : it->second.first;
+    if (calls) *calls = it == c->timers.end() ? 0 : it->second.second;
+    return CMCG_OK;
+}
+
+int cmcg_timer_reset(cmcg_ctx *c)
+{
+    if (!c) return CMCG_E_ARG;
+    if (sync(c) != hipSuccess) return CMCG_E_HIP;
+    collect_timers(c);
+    c->timers.clear();
+    return CMCG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,372 @@
+// Internal declarations of the cmcg HIP library (gfx950).
+//
+// Device data layout (all in HBM, one allocation per field, SoA):
+//   r, m, phi           : N+2 doubles, the reference's star_r/star_m/star_phi with
+//                         sentinels at 0 and N+1 (src/cmc/cmc_utils.c:713-714,
+//                         src/cmc/cmc_fits.c:128-129)
+//   vr, vt, E, J, ...   : per-star fields at slots 1..N (the AoS star_t fields the
+//                         path touches, include/cmc/cmc.h:409-601)
+// Every double expression below is written in the reference's operand order and the
+// library is compiled with -ffp-contract=off, so the results equal the CPU
+// reference's on the same inputs (except libm sin/cos, ~1 ulp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/cmcg.h"
+
+#define CMCG_SF_INFINITY 1.0e10
+#define CMCG_ZERO 1.0e-20
+#define CMCG_PI 3.14159265358979323
+#define CMCG_N_TRY 50000
+
+// ---------------------------------------------------------------- device scalars
+// Lives in device memory so that a whole step can run without host round trips.
+struct DevScalars {
+    int64_t n_max, n_max_new;
+    double Mtotal, Rtidal, max_r, Dt, Prev_Dt, TotalTime;
+    double TidalMassLoss, TidalMassLoss_old, OldTidalMassLoss;
+    double Eescaped, Jescaped, Etidal, Eoops;
+    double Etot, Ekin, Epot;
+    uint64_t step;
+    double Eescaped_old, Jescaped_old, Etidal_old;
+    // per-call work
+    double DTidalMassLoss;
+    int32_t error; int32_t pad0;
+    int64_t error_index;
+    int64_t ntie_runs;
+    int64_t n_excess;            // set_velocities3 stars with vnew2 <= 0
+    int64_t first_inf;           // potential_calculate: first slot with r >= SF_INFINITY
+    int64_t sort_n;              // sort: live stars after dropping the r == SF_INFINITY tail
+    // orbit statistics
+    int64_t n_circular, n_removed_E, n_removed_rt, n_destroyed, n_brent, n_linscan;
+    int64_t tries_total, tries_max, n_bracket_hit, n_bracket_fallback;
+    double dtrel;
+};
+
+// RNG jump table: g_pow[b][j][i] = column i of T_j^(2^b), b < 128.
+struct RngTable { uint32_t col[128][4][32]; };
+
+struct Rng { uint32_t z0, z1, z2, z3; };
+
+// ---------------------------------------------------------------- device fields
+struct DevFields {
+    double *r, *m, *phi;
+    double *vr, *vt, *E, *J;
+    int64_t *id;
+    double *rnew, *vrnew, *vtnew, *X, *Y, *r_peri, *r_apo;
+    double *rOld, *Uoldrold, *Uoldrnew, *vrold, *vtold, *EI;
+    double *sigma, *pot_rnew;
+    int32_t *ntry, *flags;
+};
+
+struct KParams {
+    int64_t n_star;
+    double inv_nstar;      // 1.0/N_STAR exactly as the reference's (1.0/clus.N_STAR)
+    double madhoc;         // 1.0/((double)N_STAR), cmc_utils.c:1183
+    double cenma_m;
+    double orbit_r;
+    double theta_se_max;
+    double gamma;
+    int64_t avekernel;
+};
+
+__device__ __forceinline__ void dev_set_error(DevScalars *s, int code, int64_t idx)
+{
+    if (atomicCAS(&s->error, 0, code) == 0) s->error_index = idx;
+}
+
+// ---------------------------------------------------------------- taus113
+// One step of src/libs/taus113-v2.c:166-177 (32-bit arithmetic replaces the MASK).
+__device__ __forceinline__ uint32_t rng_int(Rng &s)
+{
+    uint32_t b;
+    b = ((s.z0 << 6) ^ s.z0) >> 13;  s.z0 = ((s.z0 & 4294967294u) << 18) ^ b;
+    b = ((s.z1 << 2) ^ s.z1) >> 27;  s.z1 = ((s.z1 & 4294967288u) << 2) ^ b;
+    b = ((s.z2 << 13) ^ s.z2) >> 21; s.z2 = ((s.z2 & 4294967280u) << 7) ^ b;
+    b = ((s.z3 << 3) ^ s.z3) >> 12;  s.z3 = ((s.z3 & 4294967168u) << 13) ^ b;
+    return s.z0 ^ s.z1 ^ s.z2 ^ s.z3;
+}
+// taus113-v2.c:187-189
+__device__ __forceinline__ double rng_dbl(Rng &s) { return (double)rng_int(s) / 4294967296.0; }
+
+__device__ __forceinline__ uint32_t gf2_matvec(const uint32_t *__restrict__ col, uint32_t v)
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int i = 0; i < 32; i++) out ^= col[i] & (0u - ((v >> i) & 1u));
+    return out;
+}
+
+// Jump the state forward by `off` steps, using bits [b0, b1) of `off` (the caller
+// knows the rest are zero).  Loops are uniform over bit positions, so the matrix
+// columns are wave-uniform loads.
+__device__ __forceinline__ Rng rng_jump(const RngTable *__restrict__ T, Rng s, uint64_t off, int b0, int b1)
+{
+    for (int b = b0; b < b1; b++) {
+        if (!((off >> b) & 1ull)) continue;
+        s.z0 = gf2_matvec(T->col[b][0], s.z0);
+        s.z1 = gf2_matvec(T->col[b][1], s.z1);
+        s.z2 = gf2_matvec(T->col[b][2], s.z2);
+        s.z3 = gf2_matvec(T->col[b][3], s.z3);
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------- potential helpers
+// include/cmc/cmc.h:2175 MPI_PHI_S(rad, j)
+__device__ __forceinline__ double phi_s(const double *__restrict__ r, const double *__restrict__ m,
+                                        double inv_nstar, double rad, int64_t j)
+{
+    return (rad >= r[j] ? m[j] / rad : m[j] / r[j]) * inv_nstar;
+}
+__device__ __forceinline__ double phi_s_own(double rj, double mj, double inv_nstar, double rad)
+{
+    return (rad >= rj ? mj / rad : mj / rj) * inv_nstar;
+}
+
+// cmc_utils.c:1654-1662 function_q evaluated in double (the reference uses x87
+// long double here; the oracle measures how often the sign differs).
+__device__ __forceinline__ double function_q(double rj, double mj, double inv_nstar,
+                                             double r, double pot, double E, double J)
+{
+    double Jor = J / r;
+    double Jr = Jor * Jor;
+    double phis = (r >= rj ? mj / r : mj / rj) * inv_nstar;
+    return 2.0 * (E - (pot + phis)) - Jr;
+}
+
+// cmc_nr.c:62-82 FindZero_r: largest k in [kmin, kmax] with r[k] < x (for sorted r).
+__device__ __forceinline__ int64_t find_zero_r(const double *__restrict__ r, int64_t kmin, int64_t kmax, double x)
+{
+    int guard = 0;
+    do {
+        int64_t ktry = (kmin + kmax + 1) / 2;
+        if (r[ktry] < x) kmin = ktry;
+        else kmax = ktry - 1;
+    } while (kmax != kmin && ++guard < 96);
+    return kmin;
+}
+
+// cmc_utils.c:112-165 potential(r); the bisection bracket [lo, hi] may be narrowed
+// by the caller to any [lo, hi] with r[lo] < x <= r[hi] (lo >= 1): the result is the
+// unique largest k with r[k] < x, identical to the full-range search.
+__device__ __forceinline__ double potential_at(const double *__restrict__ r, const double *__restrict__ phi,
+                                               int64_t n_max, double x, int64_t lo, int64_t hi, bool *bad)
+{
+    if (x == 0.0) return phi[0];
+    double r1 = r[1];
+    if (x < r1) return phi[0] - (phi[0] - phi[1]) * r1 / x;
+    int64_t i;
+    if (lo == hi - 1) i = lo;
+    else i = find_zero_r(r, lo, hi, x);
+    double ri = r[i], ri1 = r[i + 1];
+    if (ri > x || ri1 < x) { *bad = true; return 0.0; }
+    if (i == 0) return phi[1];
+    double pi = phi[i], pi1 = phi[i + 1];
+    return pi + (pi1 - pi) * (1.0 / ri - 1.0 / x) / (1.0 / ri - 1.0 / ri1);
+}
+
+// cmc_nr.c:316-383 find_zero_Q.  Returns -1 if the bracket cannot close.
+__device__ __forceinline__ int64_t find_zero_Q(const double *__restrict__ r, const double *__restrict__ phi,
+                                               double rj, double mj, double inv_nstar,
+                                               int64_t kmin, int64_t kmax, double E, double J)
+{
+    double qmin = function_q(rj, mj, inv_nstar, r[kmin], phi[kmin], E, J);
+    double qmax = function_q(rj, mj, inv_nstar, r[kmax], phi[kmax], E, J);
+    int guard = 0;
+    if (qmin < qmax) {
+        do {
+            int64_t ktry = (kmin + kmax + 1) / 2;
+            double qtry = function_q(rj, mj, inv_nstar, r[ktry], phi[ktry], E, J);
+            if (qtry < 0.e0) kmin = ktry;
+            else kmax = ktry - 1;
+        } while (kmax != kmin && ++guard < 96);
+    } else {
+        do {
+            int64_t ktry = (kmin + kmax + 1) / 2;
+            double qtry = function_q(rj, mj, inv_nstar, r[ktry], phi[ktry], E, J);
+            if (qtry > 0.e0) kmin = ktry;
+            else kmax = ktry - 1;
+        } while (kmax != kmin && ++guard < 96);
+    }
+    return guard >= 96 ? -1 : kmin;
+}
+
+// cmc_orbit.c:722-740 set_a_b
+__device__ __forceinline__ void set_a_b(const double *__restrict__ r, const double *__restrict__ phi,
+                                        double rj, double mj, const KParams &P, int64_t k,
+                                        double *a, double *b)
+{
+    if (k == 0) {
+        *a = -P.cenma_m * P.madhoc;
+        *b = phi[1] + P.cenma_m * P.madhoc / r[1] + mj * P.madhoc / rj;
+    } else {
+        double rk = r[k], rk1 = r[k + 1];
+        double Uk = phi[k] + phi_s_own(rj, mj, P.inv_nstar, rk);
+        double Uk1 = phi[k + 1] + phi_s_own(rj, mj, P.inv_nstar, rk1);
+        *a = (Uk1 - Uk) / (1. / rk1 - 1. / rk);
+        *b = (Uk / rk1 - Uk1 / rk) / (1. / rk1 - 1. / rk);
+    }
+}
+
+// cmc_nr.c:239-269 + :393-403: v_r^2 inside [r_k, r_{k+1}]
+__device__ __forceinline__ double vr2_in_interval(const double *__restrict__ r, const double *__restrict__ phi,
+                                                  double rj, double mj, const KParams &P, int64_t k,
+                                                  double E, double J, double x)
+{
+    const double eps = 2.220446049250313e-16;
+    double pot;
+    if (fabs(x - r[k]) < eps) pot = phi[k];
+    else if (fabs(x - r[k + 1]) < eps) pot = phi[k + 1];
+    else if (x < r[1]) pot = phi[0] - P.cenma_m * P.madhoc / x;
+    else pot = (phi[k] + (phi[k + 1] - phi[k]) * (1.0 / r[k] - 1.0 / x) / (1.0 / r[k] - 1.0 / r[k + 1]));
+    return function_q(rj, mj, P.inv_nstar, x, pot, E, J);
+}
+
+__device__ __forceinline__ bool interval_converged(double lo, double hi, double epsabs, double epsrel)
+{
+    double alo = fabs(lo), ahi = fabs(hi), min_abs;
+    if ((lo > 0.0 && hi > 0.0) || (lo < 0.0 && hi < 0.0)) min_abs = alo < ahi ? alo : ahi;
+    else min_abs = 0.0;
+    return fabs(hi - lo) < epsabs + epsrel * min_abs;
+}
+
+// cmc_nr.c:500-603 find_root_vr by the Brent-Dekker method (Brent 1973, ch. 4),
+// the same iteration as the CPU oracle's.  Rare fallback path.
+__device__ __noinline__ double find_root_vr(const double *__restrict__ r, const double *__restrict__ phi,
+                                            double rj, double mj, const KParams &P, int64_t k,
+                                            double E, double J, bool *bad)
+{
+    const double prec = 1e-11, deps = 2.220446049250313e-16;
+    int iter = 100;
+    double a = r[k], b = r[k + 1];
+    double fa = vr2_in_interval(r, phi, rj, mj, P, k, E, J, a);
+    double fb = vr2_in_interval(r, phi, rj, mj, P, k, E, J, b);
+    if ((fa > 0) == (fb > 0) && (fa < 0) == (fb < 0)) { *bad = true; return 0.0; }
+    double c = a, fc = fa, d = b - a, e = d;
+    double lo = a < b ? a : b, hi = a < b ? b : a;
+    bool converged = false;
+    double prev_apsis = -1.0;
+    while (!converged && iter-- > 0) {
+        if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) { c = a; fc = fa; d = b - a; e = d; }
+        if (fabs(fc) < fabs(fb)) { a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; }
+        double tol = 2.0 * deps * fabs(b);
+        double mm = 0.5 * (c - b);
+        if (fb == 0.0) { lo = hi = b; converged = true; break; }
+        if (fabs(e) < tol || fabs(fa) <= fabs(fb)) {
+            d = mm; e = mm;
+        } else {
+            double p, q, r2, s2 = fb / fa;
+            if (a == c) {
+                p = 2.0 * mm * s2; q = 1.0 - s2;
+            } else {
+                q = fa / fc; r2 = fb / fc;
+                p = s2 * (2.0 * mm * q * (q - r2) - (b - a) * (r2 - 1.0));
+                q = (q - 1.0) * (r2 - 1.0) * (s2 - 1.0);
+            }
+            if (p > 0) q = -q; else p = -p;
+            if (2.0 * p < fmin(3.0 * mm * q - fabs(tol * q), fabs(e * q))) { e = d; d = p / q; }
+            else { d = mm; e = mm; }
+        }
+        a = b; fa = fb;
+        if (fabs(d) > tol) b += d;
+        else b += (mm > 0 ? tol : -tol);
+        fb = vr2_in_interval(r, phi, rj, mj, P, k, E, J, b);
+        lo = b < c ? b : c; hi = b < c ? c : b;
+        if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) { lo = a < b ? a : b; hi = a < b ? b : a; }
+        converged = interval_converged(lo, hi, prec, prec);
+        if (!converged && iter < 10) {
+            if (prev_apsis >= 0.0 && fabs(b - prev_apsis) < 5e-13 + 5e-13 * fabs(b)) converged = true;
+            prev_apsis = b;
+        }
+    }
+    if (!converged) *bad = true;
+    double apsis = b;
+    if (vr2_in_interval(r, phi, rj, mj, P, k, E, J, apsis) < 0.) {
+        if (vr2_in_interval(r, phi, rj, mj, P, k, E, J, lo) < 0.) apsis = hi;
+        if (vr2_in_interval(r, phi, rj, mj, P, k, E, J, hi) < 0.) apsis = lo;
+    }
+    if (vr2_in_interval(r, phi, rj, mj, P, k, E, J, apsis) < 0.) *bad = true;
+    return apsis;
+}
+
+struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; int brent; int linscan; int err; };
+
+// cmc_orbit.c:30-59 get_positive_Q_index + :413-540 calc_orbit_new for star slot j.
+// kmin_hint/kmax_hint: optional narrowed brackets (learned path); when they bracket
+// the sign change the result equals the full-bracket search (monotone Q sides).
+__device__ __forceinline__ OrbitRS calc_orbit(const double *__restrict__ r, const double *__restrict__ phi,
+                                              int64_t n_max, int64_t j, double rj, double mj,
+                                              const KParams &P, double E, double J)
+{
+    OrbitRS o;
+    o.brent = 0; o.linscan = 0; o.err = 0;
+    int64_t ktemp = j;
+    double Qtemp = function_q(rj, mj, P.inv_nstar, r[ktemp], phi[ktemp], E, J);
+    bool circ = false;
+    if (Qtemp <= 0.0) {
+        o.linscan = 1;
+        ktemp = -1;
+        do {
+            ktemp++;
+            Qtemp = function_q(rj, mj, P.inv_nstar, r[ktemp], phi[ktemp], E, J);
+        } while (Qtemp <= 0.0 && ktemp <= n_max);
+        if (ktemp >= n_max) circ = true;
+    }
+    if (circ) {
+        o.rp = o.ra = rj; o.kmin = o.kmax = j; o.dQdrp = o.dQdra = 0.0; o.circular = 1;
+        return o;
+    }
+    int64_t kmin = find_zero_Q(r, phi, rj, mj, P.inv_nstar, 0, ktemp, E, J);
+    int64_t kmax = find_zero_Q(r, phi, rj, mj, P.inv_nstar, ktemp, n_max + 1, E, J);
+    if (kmin < 0 || kmax < 0) {
+        o.err = CMCG_E_BRACKET;
+        o.rp = o.ra = rj; o.kmin = o.kmax = j; o.dQdrp = o.dQdra = 0.0; o.circular = 1;
+        return o;
+    }
+    double a, b, rmin, rmax, inside_sqrt;
+    bool bad = false;
+    set_a_b(r, phi, rj, mj, P, kmin, &a, &b);
+    inside_sqrt = a * a - 2.0 * J * J * (b - E);
+    if (inside_sqrt < 0.0) {
+        rmin = -1.0 * J * J / a;
+        if ((rmin < r[kmin]) || (rmin > r[kmin + 1])) {
+            rmin = find_root_vr(r, phi, rj, mj, P, kmin, E, J, &bad);
+            o.brent = 1;
+        }
+    } else {
+        rmin = J * J / (-a + sqrt(inside_sqrt));
+    }
+    double dQdr_min = 2.0 * J * J / (rmin * rmin * rmin) + 2.0 * a / (rmin * rmin);
+    set_a_b(r, phi, rj, mj, P, kmax, &a, &b);
+    inside_sqrt = a * a - 2.0 * J * J * (b - E);
+    if (inside_sqrt < 0.0) {
+        rmax = -a / (2.0 * (b - E));
+        if ((rmax < r[kmax]) || (rmax > r[kmax + 1])) {
+            rmax = find_root_vr(r, phi, rj, mj, P, kmax, E, J, &bad);
+            o.brent = 1;
+        }
+    } else {
+        rmax = (-a + sqrt(inside_sqrt)) / (2.0 * (b - E));
+    }
+    double dQdr_max = 2.0 * J * J / (rmax * rmax * rmax) + 2.0 * a / (rmax * rmax);
+    if (bad) o.err = CMCG_E_BRENT;
+    if ((rmin > rmax) || (dQdr_min <= 0.) || (dQdr_max >= 0.)) {
+        o.rp = o.ra = rj; o.kmin = o.kmax = j; o.dQdrp = o.dQdra = 0.0; o.circular = 1;
+    } else {
+        o.rp = rmin; o.ra = rmax; o.kmin = kmin; o.kmax = kmax;
+        o.dQdrp = dQdr_min; o.dQdra = dQdr_max; o.circular = 0;
+    }
+    return o;
+}
+
+// ---------------------------------------------------------------- host side
+struct cmcg_ctx_impl;   // defined in cmcg_api.hip
+
+#define CMCG_HIP_CHECK(expr)                                                       \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) { cmcg_report_hip(_e, #expr, __FILE__, __LINE__); return CMCG_E_HIP; } \
+    } while (0)
+
+void cmcg_report_hip(hipError_t e, const char *expr, const char *file, int line);

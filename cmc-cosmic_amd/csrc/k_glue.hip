@@ -1,0 +1,365 @@
+// Henon-step glue around the hot path (src/cmc/cmc.c:240-407 order):
+// energy_conservation1/2 (src/cmc/cmc_utils.c:2153-2227), tidally_strip_stars
+// (src/cmc/cmc_evolution_thr.c:443-646, TIDAL_TREATMENT 0), ComputeIntermediateEnergy
+// (cmc_utils.c:368-391), set_velocities3 (:254-354) and ComputeEnergy (:604-653).
+// All are per-star streaming maps plus deterministic two-level reductions.
+#include "kernels.hpp"
+
+namespace {
+constexpr int GT = 256;
+__device__ __forceinline__ double sqr(double x) { return x * x; }
+
+template <int NV>
+__device__ void block_sum_store(double (&v)[NV], double *__restrict__ out)
+{
+    __shared__ double lds[NV][GT];
+    const int t = threadIdx.x;
+    for (int i = 0; i < NV; i++) lds[i][t] = v[i];
+    __syncthreads();
+    for (int off = GT / 2; off > 0; off >>= 1) {
+        if (t < off)
+            for (int i = 0; i < NV; i++) lds[i][t] += lds[i][t + off];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int i = 0; i < NV; i++) out[i] = lds[i][0];
+}
+
+template <int NV>
+__device__ void final_sum(const double *__restrict__ partials, int64_t nblocks, double (&res)[NV])
+{
+    __shared__ double lds[NV][GT];
+    const int t = threadIdx.x;
+    double acc[NV];
+    for (int i = 0; i < NV; i++) acc[i] = 0.0;
+    for (int64_t b = t; b < nblocks; b += GT)
+        for (int i = 0; i < NV; i++) acc[i] += partials[8 * b + i];
+    for (int i = 0; i < NV; i++) lds[i][t] = acc[i];
+    __syncthreads();
+    for (int off = GT / 2; off > 0; off >>= 1) {
+        if (t < off)
+            for (int i = 0; i < NV; i++) lds[i][t] += lds[i][t + off];
+        __syncthreads();
+    }
+    for (int i = 0; i < NV; i++) res[i] = lds[i][0];
+}
+
+// cmc.c:246-258, 278: per-step resets
+__global__ void step_begin_kernel(DevScalars *s)
+{
+    s->Eescaped_old = s->Eescaped;
+    s->Jescaped_old = s->Jescaped;
+    s->TidalMassLoss_old = s->TidalMassLoss;
+    s->Etidal_old = s->Etidal;
+    s->Eescaped = 0.0; s->Jescaped = 0.0; s->TidalMassLoss = 0.0; s->Etidal = 0.0;
+    s->n_max_new = s->n_max;
+}
+
+__global__ void after_dynamics_kernel(DevScalars *s) { s->Prev_Dt = s->Dt; }   // cmc.c:297
+
+__global__ void ec1_kernel(DevFields f, DevScalars *s, KParams P)
+{
+    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > s->n_max) return;
+    double ri = f.r[i];
+    f.vtold[i] = f.vt[i];
+    f.vrold[i] = f.vr[i];
+    f.Uoldrold[i] = f.phi[i] + phi_s_own(ri, f.m[i], P.inv_nstar, ri);
+}
+
+// energy_conservation2: Uoldrnew = potential(rnew) + MPI_PHI_S(rnew, i).  The orbit
+// phase already evaluated potential(rnew) for sampled stars (flag 256) on the same
+// arrays, so only circular / removed stars search here.
+__global__ void ec2_kernel(DevFields f, DevScalars *s, KParams P)
+{
+    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t n = s->n_max;
+    if (i > n) return;
+    double x = f.rnew[i];
+    double pot;
+    int fl = f.flags[i];
+    if (fl & 256) {
+        pot = f.pot_rnew[i];
+    } else {
+        bool bad = false;
+        pot = potential_at(f.r, f.phi, n, x, 1, n + 1, &bad);
+        if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
+        f.pot_rnew[i] = pot;
+        f.flags[i] = fl | 256;
+    }
+    f.Uoldrnew[i] = pot + phi_s_own(f.r[i], f.m[i], P.inv_nstar, x);
+}
+
+__global__ void strip_begin_kernel(DevScalars *s, int32_t *done)
+{
+    s->Etidal = 0.0;        // cmc_evolution_thr.c:449
+    *done = 0;
+}
+
+__global__ void strip_rtidal_kernel(DevScalars *s, KParams P, const int32_t *done)
+{
+    if (*done) return;
+    s->Rtidal = P.orbit_r * pow(s->Mtotal - (s->TidalMassLoss - s->OldTidalMassLoss), 1.0 / 3.0);
+}
+
+__global__ __launch_bounds__(GT) void strip_pass_kernel(DevFields f, DevScalars *s, KParams P,
+                                                        const int32_t *done, double *__restrict__ partials)
+{
+    if (*done) return;
+    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i <= s->n_max) {
+        double Rt = s->Rtidal;
+        if (f.r_apo[i] > Rt && f.rnew[i] < 1000000) {
+            double m = f.m[i], E = f.E[i], J = f.J[i];
+            f.rnew[i] = CMCG_SF_INFINITY; f.vrnew[i] = 0.0; f.vtnew[i] = 0.0;
+            v[0] = E * m / (double)P.n_star;   // Eescaped
+            v[1] = J * m / (double)P.n_star;   // Jescaped
+            v[2] = m / (double)P.n_star;       // DTidalMassLoss
+            v[3] = E * m / (double)P.n_star;   // Etidal
+            // destroy_obj
+            f.vr[i] = 0.0; f.vt[i] = 0.0; f.E[i] = 0.0; f.J[i] = 0.0; f.id[i] = 0;
+            f.X[i] = 0.0; f.Y[i] = 0.0; f.r_peri[i] = 0.0; f.r_apo[i] = 0.0;
+            f.rOld[i] = 0.0; f.Uoldrold[i] = 0.0; f.Uoldrnew[i] = 0.0; f.vrold[i] = 0.0; f.vtold[i] = 0.0;
+            f.EI[i] = 0.0;
+            f.m[i] = 2.2250738585072014e-308;
+            f.flags[i] |= CMCG_SF_STRIPPED;
+        }
+    }
+    block_sum_store<4>(v, partials + 8 * blockIdx.x);
+}
+
+__global__ void strip_end_kernel(DevScalars *s, int32_t *done, const double *__restrict__ partials, int64_t nblocks)
+{
+    if (*done) return;
+    double res[4];
+    final_sum<4>(partials, nblocks, res);
+    if (threadIdx.x == 0) {
+        s->Eescaped += res[0];
+        s->Jescaped += res[1];
+        s->Etidal += res[3];
+        s->TidalMassLoss += res[2];
+        s->DTidalMassLoss = res[2];
+        if (!(res[2] > 0)) *done = 1;
+    }
+}
+
+// ComputeIntermediateEnergy, first loop (EI).
+__global__ void ie_kernel(DevFields f, DevScalars *s)
+{
+    int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t n = s->n_max;
+    if (j > n) return;
+    double x = f.rnew[j];
+    if (x < 1.0e6) {
+        double pot;
+        if (f.flags[j] & 256) pot = f.pot_rnew[j];
+        else {
+            bool bad = false;
+            pot = potential_at(f.r, f.phi, n, x, 1, n + 1, &bad);
+            if (bad) dev_set_error(s, CMCG_E_BRACKET, j);
+        }
+        f.EI[j] = sqr(f.vr[j]) + sqr(f.vt[j]) + f.phi[j] - pot;
+    }
+}
+
+// ComputeIntermediateEnergy, second loop (transfer of the new orbit variables).
+__global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
+{
+    int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > s->n_max) return;
+    f.rOld[j] = f.r[j];
+    f.r[j] = f.rnew[j];
+    f.vr[j] = f.vrnew[j];
+    f.vt[j] = f.vtnew[j];
+}
+
+// set_velocities3 per-star part.  Stars with vnew2 <= 0 ("events") feed the
+// sequential excess-energy chain; they are listed for the chain kernel.
+__global__ void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
+                           int64_t *__restrict__ events, int64_t max_events)
+{
+    const double q = 0.5;
+    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t n = s->n_max;
+    if (i > n) return;
+    double m = f.m[i], ri = f.r[i];
+    double rO = f.rOld[i];
+    bool bad = false;
+    double Unewrold = potential_at(f.r, f.phi, n, rO, 1, n + 1, &bad) + phi_s_own(ri, m, P.inv_nstar, rO);
+    if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
+    double Unewrnew = f.phi[i] + phi_s_own(ri, m, P.inv_nstar, ri);
+    double vtold = f.vtold[i], vrold = f.vrold[i];
+    double vold2 = vtold * vtold + vrold * vrold;
+    double vnew2 = vold2 + 2.0 * (1.0 - q) * (f.Uoldrold[i] - f.Uoldrnew[i]) + 2.0 * q * (Unewrold - Unewrnew);
+    double vr = f.vr[i], vt = f.vt[i];
+    if (vnew2 <= 0.0) {
+        excess_c[i] = 0.5 * (sqr(vr) + sqr(vt) - vnew2) * m;
+        unsigned long long slot = atomicAdd((unsigned long long *)&s->n_excess, 1ull);
+        if ((int64_t)slot < max_events) events[slot] = i;
+    } else {
+        excess_c[i] = -1.0;
+        double alpha = sqrt(vnew2 / (sqr(vr) + sqr(vt)));
+        vr *= alpha;
+        vt *= alpha;
+        f.vr[i] = vr; f.vt[i] = vt;
+        f.vrnew[i] = vr; f.vtnew[i] = vt;
+    }
+}
+
+__global__ void sv3_begin_kernel(DevScalars *s) { s->n_excess = 0; }
+
+// The excess chain of set_velocities3 (cmc_utils.c:268-310) in exact sequential
+// order: one thread walks from each event (sorted) while excess remains.  Events
+// are rare; with none this kernel returns immediately.
+__global__ void sv3_chain_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
+                                 int64_t *__restrict__ events, int64_t max_events)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t ne = s->n_excess;
+    if (ne == 0) return;
+    const double E_dump_factor = 0.8;
+    int64_t n = s->n_max;
+    int64_t start;
+    bool listed = ne <= max_events;
+    if (listed) {
+        for (int64_t a = 1; a < ne; a++) {          // insertion sort of the event slots
+            int64_t v = events[a], b = a - 1;
+            while (b >= 0 && events[b] > v) { events[b + 1] = events[b]; b--; }
+            events[b + 1] = v;
+        }
+        start = events[0];
+    } else {
+        start = 1;
+    }
+    double Eexcess = 0.0;
+    int64_t ev = 0;
+    int64_t i = start;
+    while (i <= n) {
+        double c = excess_c[i];
+        if (c >= 0.0) {
+            Eexcess += c;
+            if (listed) ev++;
+        } else if (Eexcess > 0) {
+            double m = f.m[i], vr = f.vr[i], vt = f.vt[i];
+            double v2 = sqr(vr) + sqr(vt);
+            double E_dump_capacity = E_dump_factor * 0.5 * m * v2;
+            double E_dump = E_dump_capacity <= Eexcess ? E_dump_capacity : Eexcess;
+            double exc_ratio = sqrt((v2 - 2 * E_dump / m) / v2);
+            f.vr[i] = vr * exc_ratio;
+            f.vt[i] = vt * exc_ratio;
+            Eexcess -= E_dump;
+        }
+        if (listed && !(Eexcess > 0)) {
+            if (ev >= ne) break;
+            i = events[ev];         // skip to the next event
+            continue;
+        }
+        i++;
+    }
+    // at one rank the leftover excess is dropped (cmc_utils.c:313-353, Eoops += 0)
+}
+
+__global__ __launch_bounds__(GT) void energy_kernel(DevFields f, DevScalars *s, KParams P,
+                                                    double *__restrict__ partials)
+{
+    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    int64_t n = s->n_max;
+    if (i <= n) {
+        double vr = f.vr[i], vt = f.vt[i], ph = f.phi[i], m = f.m[i];
+        f.E[i] = ph + 0.5 * (sqr(vr) + sqr(vt));
+        f.J[i] = f.r[i] * vt;
+        v[0] = 0.5 * (sqr(vr) + sqr(vt)) * m * P.madhoc;
+        v[1] = ph * m * P.madhoc + f.phi[0] * P.cenma_m * P.madhoc / n;
+    }
+    block_sum_store<2>(v, partials + 8 * blockIdx.x);
+}
+
+// compute_energy_new (cmc_utils.c:2017-2028) + the totals of ComputeEnergy
+__global__ void energy_end_kernel(DevScalars *s, const double *__restrict__ partials, int64_t nblocks)
+{
+    double res[2];
+    final_sum<2>(partials, nblocks, res);
+    if (threadIdx.x == 0) {
+        s->Eescaped += s->Eescaped_old;
+        s->Jescaped += s->Jescaped_old;
+        s->Etidal += s->Etidal_old;
+        double K = res[0], Pt = res[1] * 0.5;
+        s->Ekin = K;
+        s->Epot = Pt;
+        s->Etot = K + Pt + 0.0 + 0.0;
+        s->Etot += 0.0 + s->Eescaped + 0.0 + 0.0;
+    }
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)((n + GT - 1) / GT > 0 ? (n + GT - 1) / GT : 1); }
+} // namespace
+
+hipError_t launch_step_begin(DevScalars *s, hipStream_t st)
+{
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, st, s);
+    return hipGetLastError();
+}
+hipError_t launch_after_dynamics(DevScalars *s, hipStream_t st)
+{
+    hipLaunchKernelGGL(after_dynamics_kernel, dim3(1), dim3(1), 0, st, s);
+    return hipGetLastError();
+}
+hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
+                                       hipStream_t st)
+{
+    hipLaunchKernelGGL(ec1_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
+    return hipGetLastError();
+}
+hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
+                                       hipStream_t st)
+{
+    hipLaunchKernelGGL(ec2_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
+    return hipGetLastError();
+}
+hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,
+                        int64_t n_ub, int passes, bool begin, hipStream_t st)
+{
+    int64_t nb = grid_for(n_ub);
+    if (nb > sc.max_blocks) return hipErrorInvalidValue;
+    if (begin) hipLaunchKernelGGL(strip_begin_kernel, dim3(1), dim3(1), 0, st, s, done);
+    for (int it = 0; it < passes; it++) {
+        hipLaunchKernelGGL(strip_rtidal_kernel, dim3(1), dim3(1), 0, st, s, P, (const int32_t *)done);
+        hipLaunchKernelGGL(strip_pass_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, (const int32_t *)done,
+                           sc.partials);
+        hipLaunchKernelGGL(strip_end_kernel, dim3(1), dim3(GT), 0, st, s, done, (const double *)sc.partials, nb);
+    }
+    return hipGetLastError();
+}
+hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
+                                      hipStream_t st)
+{
+    (void)P;
+    hipLaunchKernelGGL(ie_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s);
+    return hipGetLastError();
+}
+hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, hipStream_t st)
+{
+    hipLaunchKernelGGL(ie_transfer_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s);
+    return hipGetLastError();
+}
+hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
+                                  int64_t n_ub, hipStream_t st)
+{
+    hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s);
+    hipLaunchKernelGGL(sv3_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.excess_list,
+                       sc.max_events);
+    hipLaunchKernelGGL(sv3_chain_kernel, dim3(1), dim3(64), 0, st, f, s, (const double *)sc.excess_c,
+                       sc.excess_list, sc.max_events);
+    return hipGetLastError();
+}
+hipError_t launch_compute_energy(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
+                                 int64_t n_ub, hipStream_t st)
+{
+    int64_t nb = grid_for(n_ub);
+    if (nb > sc.max_blocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(energy_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.partials);
+    hipLaunchKernelGGL(energy_end_kernel, dim3(1), dim3(GT), 0, st, s, (const double *)sc.partials, nb);
+    return hipGetLastError();
+}

@@ -1,0 +1,249 @@
+// a5-a8: get_positions_loop (src/cmc/cmc_evolution_thr.c:784-937) with
+// calc_orbit_new (src/cmc/cmc_orbit.c:413-540), find_zero_Q (src/cmc/cmc_nr.c:316-383),
+// find_root_vr (:500-603) and potential(r) (src/cmc/cmc_utils.c:112-165).
+//
+// One thread per star.  The rejection loop's potential(r) bisection is narrowed to
+// [kmin, kmax+1] (the bracket the apsides already bound; the result index is the
+// same as the reference's full-range FindZero_r, see potential_at).  Each star draws
+// from its own taus113 substream (offset step*2^72 + 2^70 + slot*2^20).
+//
+// The phase is bound by dependent gathers into the sorted (r, phi) table (16 B per
+// probe, ~(2 + tries) * log2 N probes per star), which at N = 4e6 is 64 MB and lives
+// in L2/MALL; the compulsory HBM traffic is 96 B/star.
+#include "kernels.hpp"
+
+namespace {
+constexpr int ORB_THREADS = 256;
+
+// Deterministic block reduction of NV doubles (fixed tree order) into partials.
+template <int NV, bool MAXFIRST>
+__device__ void block_reduce_store(double (&v)[NV], double *__restrict__ out)
+{
+    __shared__ double lds[NV][ORB_THREADS];
+    const int t = threadIdx.x;
+    for (int i = 0; i < NV; i++) lds[i][t] = v[i];
+    __syncthreads();
+    for (int off = ORB_THREADS / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            for (int i = 0; i < NV; i++) {
+                double a = lds[i][t], b = lds[i][t + off];
+                lds[i][t] = (MAXFIRST && i == 0) ? (a >= b ? a : b) : a + b;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int i = 0; i < NV; i++) out[i] = lds[i][0];
+}
+
+__global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
+                                                             const RngTable *__restrict__ T,
+                                                             const Rng *__restrict__ base, int jb1,
+                                                             double *__restrict__ partials)
+{
+    const int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s->n_max;
+    // [0] max_r, [1] Eescaped, [2] Jescaped, [3] TidalMassLoss, [4] Etidal
+    double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (j <= n) {
+        const double *__restrict__ r = f.r;
+        const double *__restrict__ phi = f.phi;
+        const double rj = r[j], mj = f.m[j];
+        const double Ej = f.E[j], J = f.J[j];
+        const double E = Ej + phi_s_own(rj, mj, P.inv_nstar, rj);    // cmc_evolution_thr.c:806
+        int flags = 0, ntry = 0;
+        if (mj < CMCG_ZERO) {
+            // destroy_obj (cmc_dynamics_helper.c:131-153) + remove_star_center
+            f.vr[j] = 0.0; f.vt[j] = 0.0; f.E[j] = 0.0; f.J[j] = 0.0; f.id[j] = 0;
+            f.X[j] = 0.0; f.Y[j] = 0.0; f.r_peri[j] = 0.0; f.r_apo[j] = 0.0;
+            f.rnew[j] = CMCG_SF_INFINITY; f.m[j] = 2.2250738585072014e-308;
+            f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
+            flags = CMCG_SF_DESTROYED;
+            atomicAdd((unsigned long long *)&s->n_destroyed, 1ull);
+        } else if (E >= 0.0) {
+            // remove_star (cmc_evolution_thr.c:655-716)
+            f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
+            red[1] = Ej * mj / (double)P.n_star;
+            red[2] = J * mj / (double)P.n_star;
+            red[3] = mj / (double)P.n_star;
+            red[4] = Ej * mj / (double)P.n_star;
+            flags = CMCG_SF_REMOVED_E;
+            atomicAdd((unsigned long long *)&s->n_removed_E, 1ull);
+        } else {
+            OrbitRS o = calc_orbit(r, phi, n, j, rj, mj, P, E, J);
+            if (o.err) dev_set_error(s, o.err, j);
+            if (o.brent) { flags |= CMCG_SF_BRENT; atomicAdd((unsigned long long *)&s->n_brent, 1ull); }
+            if (o.linscan) { flags |= CMCG_SF_LINSCAN; atomicAdd((unsigned long long *)&s->n_linscan, 1ull); }
+            if (o.circular) {
+                f.rnew[j] = rj; f.vrnew[j] = f.vr[j]; f.vtnew[j] = f.vt[j];
+                flags |= CMCG_SF_CIRCULAR;
+                atomicAdd((unsigned long long *)&s->n_circular, 1ull);
+            } else if (o.ra >= s->Rtidal) {
+                f.r_apo[j] = o.ra; f.r_peri[j] = o.rp;
+                f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
+                red[1] = Ej * mj / (double)P.n_star;
+                red[2] = J * mj / (double)P.n_star;
+                red[3] = mj / (double)P.n_star;
+                red[4] = Ej * mj / (double)P.n_star;
+                flags |= CMCG_SF_REMOVED_RT;
+                atomicAdd((unsigned long long *)&s->n_removed_rt, 1ull);
+            } else {
+                const double rmin = o.rp, rmax = o.ra;
+                double g1 = sqrt(3.0 * (rmax - rmin) / o.dQdrp);
+                double g2 = sqrt(-3.0 * (rmax - rmin) / o.dQdra);
+                double F = 1.2 * (g1 >= g2 ? g1 : g2);
+                Rng rng = rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1);
+                // narrowed bracket for potential(r): r in [rmin, rmax] within [r_kmin, r_kmax+1]
+                int64_t lo0 = o.kmin < 1 ? 1 : o.kmin;
+                int64_t hi0 = o.kmax + 1 > n + 1 ? n + 1 : o.kmax + 1;
+                double X = 0.0, rr = 0.0, vr = 0.0, potr = 0.0;
+                int k;
+                bool bad = false;
+                for (k = 1; k <= CMCG_N_TRY; k++) {
+                    X = rng_dbl(rng);
+                    double s0 = 2.0 * X - 1.0;
+                    double g0 = F * rng_dbl(rng);
+                    rr = 0.5 * (rmin + rmax) + 0.25 * (rmax - rmin) * (3.0 * s0 - s0 * s0 * s0);
+                    int64_t lo = lo0, hi = hi0;
+                    if (!(r[lo] < rr)) lo = 1;
+                    if (!(rr <= r[hi])) hi = n + 1;
+                    potr = potential_at(r, phi, n, rr, lo, hi, &bad);
+                    double pot = potr + phi_s_own(rj, mj, P.inv_nstar, rr);
+                    double drds = 0.25 * (rmax - rmin) * (3.0 - 3.0 * s0 * s0);
+                    double Q = 2.0 * E - 2.0 * pot - J * J / rr / rr;
+                    if (Q >= 0.0) {
+                        vr = sqrt(Q);
+                    } else {
+                        if (isnan(Q)) { dev_set_error(s, CMCG_E_NAN, j); break; }
+                        vr = 0;
+                    }
+                    if (g0 < 1.0 / vr * drds) break;
+                }
+                if (bad) dev_set_error(s, CMCG_E_BRACKET, j);
+                if (k == CMCG_N_TRY + 1) dev_set_error(s, CMCG_E_NTRY, j);
+                ntry = k;
+                f.X[j] = X;
+                f.r_peri[j] = rmin;
+                f.r_apo[j] = rmax;
+                if (rng_dbl(rng) < 0.5) vr = -vr;
+                f.rnew[j] = rr;
+                f.vrnew[j] = vr;
+                f.vtnew[j] = J / rr;
+                f.pot_rnew[j] = potr;
+                flags |= 256;      // sampled: pot_rnew holds potential(rnew)
+                red[0] = rr;
+                atomicAdd((unsigned long long *)&s->tries_total, (unsigned long long)k);
+                atomicMax((unsigned long long *)&s->tries_max, (unsigned long long)k);
+            }
+        }
+        f.flags[j] = flags;
+        f.ntry[j] = ntry;
+    }
+    block_reduce_store<5, true>(red, partials + 8 * blockIdx.x);
+}
+
+__global__ void orbits_begin_kernel(DevScalars *s)
+{
+    // new_orbits_calculate (cmc_utils.c:2198) and counters of this call
+    s->OldTidalMassLoss = s->TidalMassLoss_old;
+    s->n_circular = s->n_removed_E = s->n_removed_rt = s->n_destroyed = s->n_brent = s->n_linscan = 0;
+    s->tries_total = s->tries_max = s->n_bracket_hit = s->n_bracket_fallback = 0;
+}
+
+// Sums the per-block partials in block order and finishes new_orbits_calculate.
+__global__ void orbits_end_kernel(DevScalars *s, const double *__restrict__ partials, int64_t nblocks)
+{
+    __shared__ double lds[5][256];
+    const int t = threadIdx.x;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t b = t; b < nblocks; b += 256) {
+        const double *p = partials + 8 * b;
+        acc[0] = acc[0] >= p[0] ? acc[0] : p[0];
+        for (int i = 1; i < 5; i++) acc[i] += p[i];
+    }
+    for (int i = 0; i < 5; i++) lds[i][t] = acc[i];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            lds[0][t] = lds[0][t] >= lds[0][t + off] ? lds[0][t] : lds[0][t + off];
+            for (int i = 1; i < 5; i++) lds[i][t] += lds[i][t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        s->max_r = lds[0][0];
+        s->Eescaped += lds[1][0];
+        s->Jescaped += lds[2][0];
+        s->TidalMassLoss += lds[3][0];
+        s->Etidal += lds[4][0];
+        s->TidalMassLoss += s->TidalMassLoss_old;    // cmc_utils.c:2210
+    }
+}
+
+__global__ void apsides_kernel(DevFields f, DevScalars *s, KParams P, int64_t first, int64_t count,
+                               const double *__restrict__ E, const double *__restrict__ J,
+                               double *__restrict__ out6, int64_t *__restrict__ kout, int32_t *__restrict__ circ)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t j = first + i, n = s->n_max;
+    OrbitRS o = calc_orbit(f.r, f.phi, n, j, f.r[j], f.m[j], P, E[i], J[i]);
+    if (o.err) dev_set_error(s, o.err, j);
+    out6[4 * i + 0] = o.rp; out6[4 * i + 1] = o.ra; out6[4 * i + 2] = o.dQdrp; out6[4 * i + 3] = o.dQdra;
+    kout[2 * i + 0] = o.kmin; kout[2 * i + 1] = o.kmax;
+    circ[i] = o.circular;
+}
+
+__global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count, const int64_t *__restrict__ jj,
+                           const int64_t *__restrict__ kmin, const int64_t *__restrict__ kmax,
+                           const double *__restrict__ E, const double *__restrict__ J, int64_t *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t j = jj[i];
+    int64_t k = find_zero_Q(f.r, f.phi, f.r[j], f.m[j], P.inv_nstar, kmin[i], kmax[i], E[i], J[i]);
+    if (k < 0) dev_set_error(s, CMCG_E_BRACKET, j);
+    out[i] = k;
+}
+
+int bits_needed(uint64_t x)
+{
+    int b = 0;
+    while (b < 64 && (x >> b)) b++;
+    return b;
+}
+} // namespace
+
+hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
+                         const Scratch &sc, int64_t n_ub, hipStream_t st)
+{
+    int64_t nblocks = (n_ub + ORB_THREADS - 1) / ORB_THREADS;
+    if (nblocks < 1) nblocks = 1;
+    if (nblocks > sc.max_blocks) return hipErrorInvalidValue;
+    int jb1 = 20 + bits_needed((uint64_t)n_ub);
+    hipLaunchKernelGGL(orbits_begin_kernel, dim3(1), dim3(1), 0, st, s);
+    hipLaunchKernelGGL(orbits_kernel, dim3((unsigned)nblocks), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base,
+                       jb1, sc.partials);
+    hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(256), 0, st, s, sc.partials, nblocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
+                          const double *E, const double *J, double *out6, int64_t *kout, int32_t *circ,
+                          hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(apsides_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
+                       count, E, J, out6, kout, circ);
+    return hipGetLastError();
+}
+
+hipError_t launch_find_zero_Q(const DevFields &f, DevScalars *s, const KParams &P, int64_t count,
+                              const int64_t *j, const int64_t *kmin, const int64_t *kmax,
+                              const double *E, const double *J, int64_t *out, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fzq_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, count, j, kmin,
+                       kmax, E, J, out);
+    return hipGetLastError();
+}

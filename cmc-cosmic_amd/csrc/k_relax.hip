@@ -1,0 +1,250 @@
+// a4 two-body relaxation (src/cmc/cmc_dynamics.c:17-542, relaxation branch
+// :427-498), a10 relaxation time step (src/cmc/cmc_dynamics_helper.c:2526-2579),
+// calc_sigma_r (:2821-2986), calc_timestep (src/cmc/cmc_utils.c:2132-2148) and the
+// per-pair / per-star RNG substream bases.
+//
+// Relaxation is one thread per radially adjacent pair (2p+1, 2p+2): a pure
+// streaming map reading r, m, phi, vr, vt of both stars plus r[k-20], r[k+20]
+// (cache-served) and writing vr, vt, E, J, Y: 72 B/star of compulsory HBM traffic.
+#include "kernels.hpp"
+
+namespace {
+__device__ __forceinline__ double sqr(double x) { return x * x; }
+__device__ __forceinline__ double cub(double x) { return x * x * x; }
+
+// cmc_dynamics_helper.c:768-786 calc_n_local
+__device__ __forceinline__ double calc_n_local(const double *__restrict__ r, int64_t k, int64_t p, int64_t N_LIMIT)
+{
+    int64_t kmin = k - p, kmax = k + p;
+    if (kmin < 0) { kmin = 0; kmax = 2 * p + 1; }
+    else if (kmax > N_LIMIT) { kmax = N_LIMIT; kmin = N_LIMIT - 2 * p - 1; }
+    return (2.0 * ((double)p)) * 3.0 / (4.0 * CMCG_PI * (cub(r[kmax]) - cub(r[kmin])));
+}
+
+__global__ void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s, const Rng *__restrict__ seeded,
+                                 Rng *__restrict__ base, int64_t step_override)
+{
+    uint64_t step = step_override >= 0 ? (uint64_t)step_override : s->step;
+    // offset step*2^72: bits 72.. of the 128-bit count, i.e. bits 8.. of the high word
+    Rng b = *seeded;
+    uint64_t hi = step << 8;
+    for (int bit = 0; bit < 64; bit++) {
+        if (!((hi >> bit) & 1ull)) continue;
+        int row = 64 + bit;
+        if (row >= 128) break;
+        b.z0 = gf2_matvec(T->col[row][0], b.z0);
+        b.z1 = gf2_matvec(T->col[row][1], b.z1);
+        b.z2 = gf2_matvec(T->col[row][2], b.z2);
+        b.z3 = gf2_matvec(T->col[row][3], b.z3);
+    }
+    base[0] = b;
+    // orbits: + 2^70
+    b.z0 = gf2_matvec(T->col[70][0], b.z0);
+    b.z1 = gf2_matvec(T->col[70][1], b.z1);
+    b.z2 = gf2_matvec(T->col[70][2], b.z2);
+    b.z3 = gf2_matvec(T->col[70][3], b.z3);
+    base[1] = b;
+}
+
+__global__ void rng_draws_kernel(const RngTable *__restrict__ T, const Rng *__restrict__ base, int phase,
+                                 int64_t first, int64_t count, int ndraw, double *__restrict__ out, int jb1)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t idx = first + i;
+    Rng st;
+    if (phase == 0) st = rng_jump(T, base[0], 3ull * (uint64_t)idx, 0, jb1);
+    else st = rng_jump(T, base[1], (uint64_t)idx << 20, 20, jb1);
+    for (int d = 0; d < ndraw; d++) out[i * ndraw + d] = rng_dbl(st);
+}
+
+__global__ void dtrel_init_kernel(DevScalars *s) { s->dtrel = __longlong_as_double(0x7ff0000000000000ll); }
+
+// simul_relax_new: one thread per window si = 1+p, 1+3p, ... (si < N - p), the
+// window sums taken in the reference's order, then an order-independent min.
+__global__ void simul_relax_kernel(const double *__restrict__ r, const double *__restrict__ m,
+                                   const double *__restrict__ vr, const double *__restrict__ vt,
+                                   DevScalars *s, KParams P)
+{
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t p = P.avekernel, N = s->n_max;
+    int64_t si = 1 + p + w * 2 * p;
+    if (!(si < N - p)) return;
+    int64_t simin = si - p, simax = simin + (2 * p - 1);
+    double Mv2ave = 0.0, Mave = 0.0, M2ave = 0.0;
+    for (int64_t k = simin; k <= simax; k++) {
+        double tmp = m[k] * P.madhoc;
+        Mv2ave += tmp * (sqr(vr[k]) + sqr(vt[k]));
+        Mave += tmp;
+        M2ave += sqr(tmp);
+    }
+    Mv2ave /= (double)(2 * p);
+    Mave /= (double)(2 * p);
+    M2ave /= (double)(2 * p);
+    double sigma = sqrt(Mv2ave / Mave);
+    double W = 4.0 * sigma / sqrt(3.0 * CMCG_PI);
+    double n_local = calc_n_local(r, si, p, N);
+    double dt = sqr(2.0 * P.theta_se_max / CMCG_PI) * (CMCG_PI / 32.0) * cub(W) /
+                (((double)P.n_star) * n_local * (4.0 * M2ave));
+    if (dt >= 0.0)   // positive doubles order like their bit patterns
+        atomicMin((unsigned long long *)&s->dtrel, (unsigned long long)__double_as_longlong(dt));
+}
+
+// calc_timestep (cmc_utils.c:2132-2148) and GetTimeStep's fallback (cmc_evolution_thr.c:430-432)
+__global__ void timestep_finalize_kernel(DevScalars *s)
+{
+    double Dt = s->dtrel;
+    if (Dt == __longlong_as_double(0x7ff0000000000000ll)) Dt = 0.001;
+    if ((s->TidalMassLoss_old - s->OldTidalMassLoss) > 0.01) {
+        Dt = s->Prev_Dt * 0.8;
+    } else if (Dt > 1.1 * s->Prev_Dt && s->Prev_Dt > 0 && (s->TidalMassLoss - s->OldTidalMassLoss) > 0.02) {
+        Dt = s->Prev_Dt * 1.1;
+    }
+    s->Dt = Dt;
+    s->TotalTime += Dt;
+}
+
+// calc_sigma_r at one rank: sigma over the clamped window [si-p, si+p-1]; the
+// reference slides running sums, here each window is summed directly.
+__global__ void sigma_kernel(const double *__restrict__ m, const double *__restrict__ vr,
+                             const double *__restrict__ vt, double *__restrict__ sigma, DevScalars *s, KParams P)
+{
+    int64_t si = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t N = s->n_max, p = P.avekernel;
+    if (si > N) return;
+    int64_t simin = si - p, simax = simin + (2 * p - 1);
+    if (simin < 1) { simin = 1; simax = simin + (2 * p - 1); }
+    else if (simax > N) { simax = N; simin = simax - (2 * p - 1); }
+    double Mv2ave = 0.0, Mave = 0.0;
+    for (int64_t k = simin; k <= simax; k++) {
+        double mk = m[k] * P.madhoc;
+        Mv2ave += mk * (sqr(vr[k]) + sqr(vt[k]));
+        Mave += mk;
+    }
+    sigma[si] = sqrt(Mv2ave / Mave);
+}
+
+__global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, KParams P,
+                                                    const RngTable *__restrict__ T, const Rng *__restrict__ base,
+                                                    int jb1)
+{
+    int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t N = s->n_max;
+    if (pr >= N / 2) return;            // pairs while si <= N - N%2 - 1 (cmc_dynamics.c:203)
+    const int64_t k = 2 * pr + 1, kp = k + 1;
+    const double dt = s->Dt;
+    Rng rng = rng_jump(T, base[0], 3ull * (uint64_t)pr, 0, jb1);
+
+    // calc_encounter_dyns (cmc_dynamics_helper.c:831-871)
+    double Y = rng_dbl(rng);
+    f.Y[k] = Y;
+    f.Y[kp] = Y;
+    double phi = Y * 2.0 * CMCG_PI;
+    double vrk = f.vr[k], vtk = f.vt[k], vrkp = f.vr[kp], vtkp = f.vt[kp];
+    double v[4], vp[4], w[4], w1[4], w2[4];
+    v[1] = vtk; v[2] = 0.0; v[3] = vrk;
+    vp[1] = vtkp * cos(phi);
+    vp[2] = vtkp * sin(phi);
+    vp[3] = vrkp;
+    for (int j = 1; j <= 3; j++) w[j] = vp[j] - v[j];
+    double W = sqrt(sqr(w[1]) + sqr(w[2]) + sqr(w[3]));
+    if (W == 0.0) { dev_set_error(s, CMCG_E_WZERO, k); return; }
+
+    double n_local = calc_n_local(f.r, k, P.avekernel, N);
+    double mass_k = f.m[k], mass_kp = f.m[kp];
+    (void)rng_dbl(rng);                 // P_enc test (cmc_dynamics.c:406); P_enc = 0 here
+
+    double Trel12 = (CMCG_PI / 32.0) * cub(W) / (((double)P.n_star) * n_local * sqr((mass_k + mass_kp) * P.madhoc));
+    double beta = (CMCG_PI / 2.0) * sqrt(dt / Trel12);
+    if (beta > CMCG_PI / 2.0) beta = CMCG_PI / 2.0;
+    double wp = sqrt(sqr(w[1]) + sqr(w[2]));
+    if (wp == 0.0) { dev_set_error(s, CMCG_E_WPZERO, k); return; }
+    w1[1] = -w[2] * W / wp;
+    w1[2] = w[1] * W / wp;
+    w1[3] = 0.0;
+    w2[1] = -w[1] * w[3] / wp;
+    w2[2] = -w[2] * w[3] / wp;
+    w2[3] = wp;
+    double psi = rng_dbl(rng) * 2 * CMCG_PI;
+    double cb = cos(beta), sb = sin(beta), cp = cos(psi), sp = sin(psi);
+    double v_new[4], vp_new[4];
+    for (int j = 1; j <= 3; j++) {
+        double w_new = w[j] * cb + w1[j] * sb * cp + w2[j] * sb * sp;
+        v_new[j] = v[j] - mass_kp / (mass_k + mass_kp) * (w_new - w[j]);
+        vp_new[j] = vp[j] + mass_k / (mass_k + mass_kp) * (w_new - w[j]);
+    }
+    double nvr_k = v_new[3], nvt_k = sqrt(sqr(v_new[1]) + sqr(v_new[2]));
+    double nvr_kp = vp_new[3], nvt_kp = sqrt(sqr(vp_new[1]) + sqr(vp_new[2]));
+    f.vr[k] = nvr_k; f.vt[k] = nvt_k;
+    f.vr[kp] = nvr_kp; f.vt[kp] = nvt_kp;
+    // set_star_EJ (cmc_dynamics_helper.c:878-882), old phi and r
+    f.E[k] = f.phi[k] + 0.5 * (sqr(nvr_k) + sqr(nvt_k));
+    f.J[k] = f.r[k] * nvt_k;
+    f.E[kp] = f.phi[kp] + 0.5 * (sqr(nvr_kp) + sqr(nvt_kp));
+    f.J[kp] = f.r[kp] * nvt_kp;
+}
+
+int bits_needed(uint64_t x)
+{
+    int b = 0;
+    while (b < 64 && (x >> b)) b++;
+    return b;
+}
+} // namespace
+
+hipError_t launch_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(rng_bases_kernel, dim3(1), dim3(1), 0, st, T, s, sc.rng_seeded, sc.rng_base, (int64_t)-1);
+    return hipGetLastError();
+}
+
+hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch &sc, int64_t step, hipStream_t st)
+{
+    hipLaunchKernelGGL(rng_bases_kernel, dim3(1), dim3(1), 0, st, T, s, sc.rng_seeded, sc.rng_base, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64_t first, int64_t count,
+                            int ndraw, double *out, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    uint64_t last = (uint64_t)(first + count);
+    int jb1 = phase == 0 ? bits_needed(3ull * last) : 20 + bits_needed(last);
+    hipLaunchKernelGGL(rng_draws_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, T, base, phase,
+                       first, count, ndraw, out, jb1);
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st)
+{
+    if (n_ub <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sigma_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f.m, f.vr, f.vt,
+                       f.sigma, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st)
+{
+    hipLaunchKernelGGL(dtrel_init_kernel, dim3(1), dim3(1), 0, st, s);
+    int64_t nw = n_ub / (2 * P.avekernel) + 1;
+    hipLaunchKernelGGL(simul_relax_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, f.r, f.m, f.vr,
+                       f.vt, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st)
+{
+    hipLaunchKernelGGL(timestep_finalize_kernel, dim3(1), dim3(1), 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
+                        const Scratch &sc, int64_t n_ub, hipStream_t st)
+{
+    int64_t npairs = n_ub / 2;
+    if (npairs <= 0) return hipSuccess;
+    int jb1 = bits_needed(3ull * (uint64_t)npairs);
+    hipLaunchKernelGGL(relax_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P, T,
+                       sc.rng_base, jb1);
+    return hipGetLastError();
+}

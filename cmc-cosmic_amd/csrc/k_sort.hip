@@ -1,0 +1,309 @@
+// a1: qsorts_new -> sample_sort at one rank (src/cmc/cmc_sort.c:375-629): order the
+// stars by r ascending (compare_type, :245-250), ties by id (the reference's libc
+// qsort leaves them unordered), drop the r == SF_INFINITY tail
+// (remove_stripped_stars, :343-354), then refresh r and m (post_sort_comm,
+// src/cmc/cmc_utils.c:2259-2288).
+//
+// Device algorithm: LSD radix sort of (64-bit order-preserving key of r, 32-bit
+// slot) with 8-bit digits, 8 passes of three kernels (tile histogram, per-digit
+// tile scan, stable scatter).  The scatter ranks each 4096-key tile with 64-lane
+// ballot multisplit (stable in tile order), stages the sorted tile in LDS and
+// writes it out digit-run by digit-run, so global writes are coalesced.  A tie
+// pass orders runs of equal keys by id, then one gather moves the payload fields.
+#include "kernels.hpp"
+
+namespace {
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_MAX_FIELDS = 24;
+constexpr int TIE_MAX_RUN = 64;
+
+__device__ __forceinline__ uint64_t sortable(double x)
+{
+    uint64_t b = (uint64_t)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unsortable(uint64_t k)
+{
+    uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+struct GatherList {
+    int nf;
+    const uint64_t *src[RS_MAX_FIELDS];
+    uint64_t *dst[RS_MAX_FIELDS];
+};
+
+__global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// keys/vals from slots 1..n plus the eight global digit histograms
+__global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__restrict__ keysrc, DevScalars *s,
+                                                             uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                             uint32_t *__restrict__ digit_totals)
+{
+    __shared__ uint32_t h[8][256];
+    const int t = threadIdx.x;
+    for (int p = 0; p < 8; p++) h[p][t] = 0;
+    __syncthreads();
+    const int64_t n = s->n_max_new;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int u = 0; u < RS_ITEMS; u++) {
+        int64_t i = base + u * RS_THREADS + t;
+        if (i < n) {
+            uint64_t k = sortable(keysrc[i + 1]);
+            keys[i] = k;
+            vals[i] = (uint32_t)(i + 1);
+            for (int p = 0; p < 8; p++) atomicAdd(&h[p][(k >> (8 * p)) & 255], 1u);
+        }
+    }
+    __syncthreads();
+    for (int p = 0; p < 8; p++)
+        if (h[p][t]) atomicAdd(&digit_totals[p * 256 + t], h[p][t]);
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint64_t *__restrict__ keys, DevScalars *s,
+                                                             int shift, uint32_t *__restrict__ block_hist,
+                                                             int64_t ntiles)
+{
+    __shared__ uint32_t h[256];
+    const int t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    const int64_t n = s->n_max_new;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int u = 0; u < RS_ITEMS; u++) {
+        int64_t i = base + u * RS_THREADS + t;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    block_hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
+}
+
+// One block per digit: exclusive scan of that digit's tile counts, plus the digit's
+// global base (sum of the lower digits' totals for this pass).
+__global__ __launch_bounds__(RS_THREADS) void rs_scan_kernel(uint32_t *__restrict__ block_hist, int64_t ntiles,
+                                                             const uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t red[RS_THREADS];
+    const int t = threadIdx.x;
+    const int d = blockIdx.x;
+    red[t] = t < d ? totals[t] : 0u;
+    __syncthreads();
+    for (int off = RS_THREADS / 2; off > 0; off >>= 1) {
+        if (t < off) red[t] += red[t + off];
+        __syncthreads();
+    }
+    const uint32_t dbase = red[0];
+    __syncthreads();
+    uint32_t *row = block_hist + (int64_t)d * ntiles;
+    const int64_t chunk = (ntiles + RS_THREADS - 1) / RS_THREADS;
+    const int64_t b0 = (int64_t)t * chunk;
+    uint32_t sum = 0;
+    for (int64_t b = b0; b < b0 + chunk && b < ntiles; b++) sum += row[b];
+    red[t] = sum;
+    __syncthreads();
+    // inclusive Hillis-Steele scan of the per-thread sums
+    for (int off = 1; off < RS_THREADS; off <<= 1) {
+        uint32_t v = t >= off ? red[t - off] : 0u;
+        __syncthreads();
+        red[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = dbase + red[t] - sum;
+    for (int64_t b = b0; b < b0 + chunk && b < ntiles; b++) {
+        uint32_t c = row[b];
+        row[b] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *__restrict__ keys_in,
+                                                                const uint32_t *__restrict__ vals_in,
+                                                                uint64_t *__restrict__ keys_out,
+                                                                uint32_t *__restrict__ vals_out, DevScalars *s,
+                                                                int shift, const uint32_t *__restrict__ tile_off,
+                                                                int64_t ntiles)
+{
+    __shared__ uint64_t skeys[RS_TILE];
+    __shared__ uint32_t svals[RS_TILE];
+    __shared__ uint32_t whist[4][256];
+    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t dstart[256];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t n = s->n_max_new;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    cnt[t] = 0;
+    whist[0][t] = 0; whist[1][t] = 0; whist[2][t] = 0; whist[3][t] = 0;
+    __syncthreads();
+    uint64_t k[RS_ITEMS];
+    uint32_t v[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    for (int u = 0; u < RS_ITEMS; u++) {
+        int64_t i = base + u * RS_THREADS + t;
+        bool valid = i < n;
+        k[u] = valid ? keys_in[i] : 0ull;
+        v[u] = valid ? vals_in[i] : 0u;
+        uint32_t d = (uint32_t)((k[u] >> shift) & 255);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        if (!valid) peers = 0;
+        uint32_t rw = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && rw == 0) whist[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t r = cnt[d] + rw;
+            for (int w2 = 0; w2 < w; w2++) r += whist[w2][d];
+            rank[u] = r;
+        }
+        __syncthreads();
+        cnt[t] += whist[0][t] + whist[1][t] + whist[2][t] + whist[3][t];
+        whist[0][t] = 0; whist[1][t] = 0; whist[2][t] = 0; whist[3][t] = 0;
+        __syncthreads();
+    }
+    // exclusive scan of the tile's digit counts
+    dstart[t] = cnt[t];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t x = t >= off ? dstart[t - off] : 0u;
+        __syncthreads();
+        dstart[t] += x;
+        __syncthreads();
+    }
+    uint32_t incl = dstart[t];
+    __syncthreads();
+    dstart[t] = incl - cnt[t];
+    __syncthreads();
+    for (int u = 0; u < RS_ITEMS; u++) {
+        int64_t i = base + u * RS_THREADS + t;
+        if (i < n) {
+            uint32_t d = (uint32_t)((k[u] >> shift) & 255);
+            uint32_t p = dstart[d] + rank[u];
+            skeys[p] = k[u];
+            svals[p] = v[u];
+        }
+    }
+    __syncthreads();
+    const int64_t cnt_tile = n - base < RS_TILE ? n - base : RS_TILE;
+    for (int u = 0; u < RS_ITEMS; u++) {
+        int p = u * RS_THREADS + t;
+        if (p < cnt_tile) {
+            uint64_t key = skeys[p];
+            uint32_t d = (uint32_t)((key >> shift) & 255);
+            int64_t g = (int64_t)tile_off[(int64_t)d * ntiles + blockIdx.x] + (p - (int64_t)dstart[d]);
+            keys_out[g] = key;
+            vals_out[g] = svals[p];
+        }
+    }
+}
+
+// Runs of equal keys (below the SF_INFINITY tail) are ordered by id.
+__global__ void rs_ties_kernel(const uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                               const int64_t *__restrict__ id, DevScalars *s, uint64_t kinf)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s->n_max_new;
+    if (i + 1 >= n) return;
+    uint64_t k = keys[i];
+    if (k >= kinf || keys[i + 1] != k) return;
+    if (i > 0 && keys[i - 1] == k) return;
+    int64_t e = i + 1;
+    while (e + 1 < n && keys[e + 1] == k && e - i < TIE_MAX_RUN) e++;
+    if (e + 1 < n && keys[e + 1] == k) { dev_set_error(s, CMCG_E_TIES, i + 1); return; }
+    for (int64_t a = i + 1; a <= e; a++) {
+        uint32_t v = vals[a];
+        int64_t idv = id[v];
+        int64_t b = a - 1;
+        while (b >= i && id[vals[b]] > idv) { vals[b + 1] = vals[b]; b--; }
+        vals[b + 1] = v;
+    }
+    atomicAdd((unsigned long long *)&s->ntie_runs, 1ull);
+}
+
+// number of live stars = keys below the SF_INFINITY key
+__global__ void rs_count_kernel(const uint64_t *__restrict__ keys, DevScalars *s, uint64_t kinf)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s->n_max_new;
+    if (i > n) return;
+    bool below_prev = (i == 0) ? true : keys[i - 1] < kinf;
+    bool below_here = (i == n) ? false : keys[i] < kinf;
+    if (below_prev && !below_here) s->sort_n = i;
+}
+
+__global__ __launch_bounds__(256) void rs_gather_kernel(const uint64_t *__restrict__ keys,
+                                                        const uint32_t *__restrict__ vals, DevScalars *s,
+                                                        double *__restrict__ r_out, GatherList g)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s->sort_n;
+    if (i > n) return;
+    if (i == n) {   // the outer sentinel (potential_calculate sets it again)
+        r_out[n + 1] = CMCG_SF_INFINITY;
+        return;
+    }
+    uint32_t src = vals[i];
+    r_out[i + 1] = unsortable(keys[i]);
+    for (int f = 0; f < g.nf; f++) g.dst[f][i + 1] = g.src[f][src];
+}
+
+__global__ void rs_finish_kernel(DevScalars *s)
+{
+    s->n_max = s->sort_n;
+    s->n_max_new = s->sort_n;
+}
+
+__global__ void rs_begin_kernel(DevScalars *s) { s->sort_n = 0; s->ntie_runs = 0; }
+} // namespace
+
+// Sort slots 1..n_max_new by (key field, id).  src/dst are parallel lists of payload
+// fields; key_src is the field holding the radii to sort by; r_dst receives them sorted.
+hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_t *id_src, int nf,
+                              const uint64_t *const *src, uint64_t *const *dst, DevScalars *s, const Scratch &sc,
+                              int64_t n_ub, hipStream_t st)
+{
+    if (nf > RS_MAX_FIELDS) return hipErrorInvalidValue;
+    int64_t ntiles = (n_ub + RS_TILE - 1) / RS_TILE;
+    if (ntiles < 1) ntiles = 1;
+    if (ntiles > sc.max_sort_tiles) return hipErrorInvalidValue;
+    const uint64_t kinf = 0x8000000000000000ull | 0x4202a05f20000000ull;   // sortable(1.0e10)
+    hipLaunchKernelGGL(rs_begin_kernel, dim3(1), dim3(1), 0, st, s);
+    hipLaunchKernelGGL(rs_zero_kernel, dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256));
+    hipLaunchKernelGGL(rs_prep_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, sc.keys_a,
+                       sc.vals_a, sc.digit_totals);
+    uint64_t *kin = sc.keys_a, *kout = sc.keys_b;
+    uint32_t *vin = sc.vals_a, *vout = sc.vals_b;
+    for (int pass = 0; pass < 8; pass++) {
+        int shift = 8 * pass;
+        hipLaunchKernelGGL(rs_hist_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, s, shift,
+                           sc.block_hist, ntiles);
+        hipLaunchKernelGGL(rs_scan_kernel, dim3(256), dim3(RS_THREADS), 0, st, sc.block_hist, ntiles,
+                           (const uint32_t *)(sc.digit_totals + 256 * pass));
+        hipLaunchKernelGGL(rs_scatter_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, vin, kout, vout,
+                           s, shift, (const uint32_t *)sc.block_hist, ntiles);
+        uint64_t *tk = kin; kin = kout; kout = tk;
+        uint32_t *tv = vin; vin = vout; vout = tv;
+    }
+    unsigned g1 = (unsigned)((n_ub + 256) / 256);
+    hipLaunchKernelGGL(rs_ties_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, vin, id_src, s, kinf);
+    hipLaunchKernelGGL(rs_count_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, kinf);
+    GatherList g;
+    g.nf = nf;
+    for (int f = 0; f < nf; f++) { g.src[f] = src[f]; g.dst[f] = dst[f]; }
+    for (int f = nf; f < RS_MAX_FIELDS; f++) { g.src[f] = nullptr; g.dst[f] = nullptr; }
+    hipLaunchKernelGGL(rs_gather_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, (const uint32_t *)vin, s,
+                       r_dst, g);
+    hipLaunchKernelGGL(rs_finish_kernel, dim3(1), dim3(1), 0, st, s);
+    return hipGetLastError();
+}

@@ -1,0 +1,72 @@
+// Host-side launch entry points of the cmcg kernels (one translation unit per
+// phase).  All launches are asynchronous on the given stream; they read the live
+// star count from DevScalars on the device, and `n_ub` is only an upper bound used
+// to size grids (stars beyond the device-side count exit immediately), so a whole
+// step can be enqueued without host round trips.
+#pragma once
+#include "cmcg_internal.hpp"
+
+struct Scratch {
+    // potential scan: 6 doubles per tile
+    double *tile_sums;
+    int64_t max_tiles;
+    // deterministic block partials: 8 doubles per block
+    double *partials;
+    int64_t max_blocks;
+    // sort
+    uint64_t *keys_a, *keys_b;
+    uint32_t *vals_a, *vals_b;
+    uint32_t *block_hist;       // 256 * max_sort_tiles
+    uint32_t *digit_totals;     // 8 * 256
+    int64_t max_sort_tiles;
+    // set_velocities3 excess chain
+    double *excess_c;           // per star
+    int64_t *excess_list;
+    int64_t max_events;
+    // tidal stripping loop flag
+    int32_t *strip_done;
+    // RNG bases of the current step: [0] relaxation, [1] orbits; seeded state
+    Rng *rng_base;
+    Rng *rng_seeded;
+};
+
+hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
+                            const Scratch &sc, hipStream_t st);
+hipError_t launch_potential_at(const DevFields &f, DevScalars *s, const KParams &P,
+                               const double *x, double *out, int64_t count, hipStream_t st);
+
+hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
+hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
+hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st);
+hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
+                        const Scratch &sc, int64_t n_ub, hipStream_t st);
+hipError_t launch_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st);
+hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch &sc, int64_t step, hipStream_t st);
+hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64_t first, int64_t count,
+                            int ndraw, double *out, hipStream_t st);
+
+hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
+                         const Scratch &sc, int64_t n_ub, hipStream_t st);
+hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
+                          const double *E, const double *J, double *out4, int64_t *kout, int32_t *circ,
+                          hipStream_t st);
+hipError_t launch_find_zero_Q(const DevFields &f, DevScalars *s, const KParams &P, int64_t count,
+                              const int64_t *j, const int64_t *kmin, const int64_t *kmax,
+                              const double *E, const double *J, int64_t *out, hipStream_t st);
+
+hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_t *id_src, int nf,
+                              const uint64_t *const *src, uint64_t *const *dst, DevScalars *s, const Scratch &sc,
+                              int64_t n_ub, hipStream_t st);
+
+hipError_t launch_step_begin(DevScalars *s, hipStream_t st);
+hipError_t launch_after_dynamics(DevScalars *s, hipStream_t st);
+hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
+hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
+hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,
+                        int64_t n_ub, int passes, bool begin, hipStream_t st);
+hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
+hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, hipStream_t st);
+hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
+                                  int64_t n_ub, hipStream_t st);
+hipError_t launch_compute_energy(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
+                                 int64_t n_ub, hipStream_t st);

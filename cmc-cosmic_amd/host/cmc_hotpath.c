@@ -1,0 +1,198 @@
+/*
+ * The reference's hot-path call surface on top of the cmcg C-ABI (one device,
+ * one process-wide context).  Each function keeps the reference name, arguments
+ * and effect on the cluster state; the state itself lives on the device and the
+ * scalar globals are mirrored in cmc_hp after every call.
+ */
+#include "../../include/cmc_hotpath.h"
+#include <stdio.h>
+#include <stdlib.h>
+#include <float.h>
+
+cmc_hotpath_globals cmc_hp;
+
+static cmcg_ctx *g_ctx = NULL;
+static uint64_t g_step = 0;   /* RNG step counter (tcount - 1 in the reference's main loop) */
+
+static void default_handler(int code, int64_t index, const char *where)
+{
+    fprintf(stderr, "cmc hot path: %s in %s (star %lld)\n", cmcg_error_string(code), where, (long long)index);
+    exit(code < 0 ? -code : code);   /* exit_cleanly, src/cmc/cmc_utils.c:208-220 */
+}
+static cmc_hotpath_error_handler g_handler = default_handler;
+
+void cmc_hotpath_set_error_handler(cmc_hotpath_error_handler h) { g_handler = h ? h : default_handler; }
+
+static void refresh(void)
+{
+    cmcg_scalars s;
+    cmcg_get_scalars(g_ctx, &s);
+    cmc_hp.N_MAX = (long)s.n_max;
+    cmc_hp.N_MAX_NEW = (long)s.n_max_new;
+    cmc_hp.Mtotal = s.Mtotal;
+    cmc_hp.Rtidal = s.Rtidal;
+    cmc_hp.max_r = s.max_r;
+    cmc_hp.Dt = s.Dt;
+    cmc_hp.TotalTime = s.TotalTime;
+    cmc_hp.TidalMassLoss = s.TidalMassLoss;
+    cmc_hp.Eescaped = s.Eescaped;
+    cmc_hp.Jescaped = s.Jescaped;
+    cmc_hp.Etidal = s.Etidal;
+    cmc_hp.Etot = s.Etot;
+    cmc_hp.Ekin = s.Ekin;
+    cmc_hp.Epot = s.Epot;
+}
+
+static int check(int rc, const char *where)
+{
+    if (rc < 0) {
+        cmcg_scalars s;
+        (void)s;
+        g_handler(rc, -1, where);
+    }
+    return rc;
+}
+
+int cmc_hotpath_init(const cmcg_params *p, int64_t n_cap, const cmcg_host_soa *stars, int device)
+{
+    if (g_ctx) cmc_hotpath_finalize();
+    int rc = cmcg_create(&g_ctx, device, n_cap, p);
+    if (rc) return rc;
+    rc = cmcg_upload(g_ctx, stars);
+    if (rc) return rc;
+    cmc_hp.N_STAR = (long)p->n_star;
+    cmc_hp.madhoc = 1.0 / ((double)p->n_star);
+    cmc_hp.tcount = 1;
+    g_step = 0;
+    refresh();
+    return CMCG_OK;
+}
+
+void cmc_hotpath_finalize(void)
+{
+    if (g_ctx) cmcg_destroy(g_ctx);
+    g_ctx = NULL;
+}
+
+cmcg_ctx *cmc_hotpath_ctx(void) { return g_ctx; }
+
+int cmc_hotpath_download(cmcg_host_soa *h) { return cmcg_download(g_ctx, h); }
+
+void qsorts_new(void)
+{
+    int64_t n;
+    check(cmcg_sort_by_r(g_ctx, &n), "qsorts_new");
+    refresh();
+}
+
+/* On one device the sorted r and m already are the global arrays. */
+void post_sort_comm(void) {}
+
+long potential_calculate(void)
+{
+    int64_t n;
+    check(cmcg_potential(g_ctx, &n, NULL, NULL), "potential_calculate");
+    refresh();
+    return (long)n;
+}
+
+void calc_potential_new(void) { (void)potential_calculate(); }
+
+double potential(double r)
+{
+    double out = 0.0;
+    check(cmcg_potential_at(g_ctx, 1, &r, &out), "potential");
+    return out;
+}
+
+void calc_sigma_r_all(void) { check(cmcg_sigma_r(g_ctx), "calc_sigma_r"); }
+
+double simul_relax_new(void)
+{
+    double dt = 0.0;
+    check(cmcg_simul_relax(g_ctx, &dt), "simul_relax_new");
+    return dt;
+}
+
+void calc_timestep(void *rng)
+{
+    (void)rng;
+    double dt;
+    check(cmcg_calc_timestep(g_ctx, &dt), "calc_timestep");
+    refresh();
+}
+
+void dynamics_apply(double dt, void *rng)
+{
+    (void)rng;   /* the reference's gsl_rng argument; the path draws from curr_st */
+    check(cmcg_relax(g_ctx, dt, g_step), "dynamics_apply");
+    refresh();
+}
+
+void energy_conservation1(void) { check(cmcg_energy_conservation1(g_ctx), "energy_conservation1"); }
+
+double get_positions(void)
+{
+    cmcg_orbit_stats st;
+    check(cmcg_orbits(g_ctx, g_step, &st), "get_positions");
+    refresh();
+    return st.max_r;
+}
+
+void new_orbits_calculate(void) { (void)get_positions(); }
+
+void energy_conservation2(void) { check(cmcg_energy_conservation2(g_ctx), "energy_conservation2"); }
+
+void tidally_strip_stars(void)
+{
+    check(cmcg_tidally_strip_stars(g_ctx), "tidally_strip_stars");
+    refresh();
+}
+
+void ComputeIntermediateEnergy(void)
+{
+    check(cmcg_compute_intermediate_energy(g_ctx), "ComputeIntermediateEnergy");
+}
+
+void energy_conservation3(void) { check(cmcg_set_velocities3(g_ctx), "energy_conservation3"); }
+
+void compute_energy_new(void)
+{
+    check(cmcg_compute_energy(g_ctx), "compute_energy_new");
+    refresh();
+    g_step++;
+    cmc_hp.tcount++;
+}
+
+cmc_orbit_rs_t calc_orbit_new(long index, double E, double J)
+{
+    cmc_orbit_rs_t o;
+    double rp, ra, dp, da;
+    int64_t kmin, kmax;
+    int32_t circ;
+    check(cmcg_calc_pericenter_and_apocenter(g_ctx, index, 1, &E, &J, &rp, &ra, &dp, &da, &kmin, &kmax, &circ),
+          "calc_orbit_new");
+    o.rp = rp; o.ra = ra; o.dQdrp = dp; o.dQdra = da;
+    o.kmin = (long)kmin; o.kmax = (long)kmax; o.circular_flag = circ;
+    return o;
+}
+
+cmc_orbit_rs_t calc_pericenter_and_apocenter(long index, double E, double J) { return calc_orbit_new(index, E, J); }
+
+long find_zero_Q(long j, long kmin, long kmax, long double E, long double J)
+{
+    int64_t jj = j, a = kmin, b = kmax, out = -1;
+    double e = (double)E, jv = (double)J;
+    check(cmcg_find_zero_Q(g_ctx, 1, &jj, &a, &b, &e, &jv, &out), "find_zero_Q");
+    return (long)out;
+}
+
+int cmc_hotpath_step(cmcg_orbit_stats *stats)
+{
+    int rc = cmcg_step(g_ctx, stats);
+    check(rc, "henon step");
+    refresh();
+    g_step++;
+    cmc_hp.tcount++;
+    return rc;
+}

@@ -1,0 +1,91 @@
+/*
+ * C host side of the MI355X hot path, keeping the reference's call surface.
+ *
+ * The reference's hot-path functions are `void` functions over globals
+ * (include/cmc/cmc_vars.h: star[], star_r/star_m/star_phi, clus, Dt, Mtotal,
+ * Rtidal, curr_st ...).  This header gives the same names and argument meanings;
+ * their bodies call the cmcg_* C-ABI (include/cmcg.h) on one process-wide device
+ * context created by cmc_hotpath_init.  A maintainer links this library in place
+ * of the reference objects that define these names.
+ *
+ * Failure semantics follow exit_cleanly (src/cmc/cmc_utils.c:208-220): on a
+ * negative cmcg code the installed handler is called; the default handler prints
+ * the error and the offending star index and exits with the code.
+ */
+#ifndef CMC_HOTPATH_H
+#define CMC_HOTPATH_H
+#include <stdint.h>
+#include "cmcg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* orbit_rs_t, include/cmc/cmc.h:1621-1650 */
+typedef struct {
+    double rp, ra, dQdrp, dQdra;
+    long kmax, kmin;
+    int circular_flag;
+} cmc_orbit_rs_t;
+
+/* Mirror of the reference globals the path reads or writes (refreshed after every call). */
+typedef struct {
+    long N_STAR, N_MAX, N_MAX_NEW;      /* clus.* (include/cmc/cmc.h:1690-1711) */
+    double Mtotal, Rtidal, max_r, Dt, TotalTime, madhoc;
+    double TidalMassLoss, Eescaped, Jescaped, Etidal;
+    double Etot, Ekin, Epot;
+    long tcount;
+} cmc_hotpath_globals;
+
+extern cmc_hotpath_globals cmc_hp;
+
+typedef void (*cmc_hotpath_error_handler)(int code, int64_t index, const char *where);
+void cmc_hotpath_set_error_handler(cmc_hotpath_error_handler h);
+
+/* Creates the device context and uploads the cluster (replaces get_star_data's
+ * array set-up, src/cmc/cmc_io.c:3384-3440, and set_rng_states). */
+int  cmc_hotpath_init(const cmcg_params *p, int64_t n_cap, const cmcg_host_soa *stars, int device);
+void cmc_hotpath_finalize(void);
+cmcg_ctx *cmc_hotpath_ctx(void);
+/* Copies the device state back into host arrays (for output / checkpoints). */
+int  cmc_hotpath_download(cmcg_host_soa *h);
+
+/* --- the reference's names ------------------------------------------------ */
+void   qsorts_new(void);                                   /* src/cmc/cmc_utils.c:2232 */
+void   post_sort_comm(void);                               /* src/cmc/cmc_utils.c:2259 */
+void   calc_potential_new(void);                           /* src/cmc/cmc_utils.c:2006 */
+long   potential_calculate(void);                          /* src/cmc/cmc_utils.c:675  */
+double potential(double r);                                /* src/cmc/cmc_utils.c:112  */
+void   calc_sigma_r_all(void);                             /* src/cmc/cmc_dynamics_helper.c:2821 */
+double simul_relax_new(void);                              /* src/cmc/cmc_dynamics_helper.c:2526 */
+void   calc_timestep(void *rng);                           /* src/cmc/cmc_utils.c:2132 */
+void   dynamics_apply(double dt, void *rng);               /* src/cmc/cmc_dynamics.c:17 */
+void   energy_conservation1(void);                         /* src/cmc/cmc_utils.c:2153 */
+void   new_orbits_calculate(void);                         /* src/cmc/cmc_utils.c:2194 */
+double get_positions(void);                                /* src/cmc/cmc_evolution_thr.c:944 */
+void   energy_conservation2(void);                         /* src/cmc/cmc_utils.c:2217 */
+void   tidally_strip_stars(void);                          /* src/cmc/cmc_evolution_thr.c:443 */
+void   ComputeIntermediateEnergy(void);                    /* src/cmc/cmc_utils.c:368 */
+void   energy_conservation3(void);                         /* src/cmc/cmc_utils.c:360 */
+void   compute_energy_new(void);                           /* src/cmc/cmc_utils.c:2017 */
+cmc_orbit_rs_t calc_orbit_new(long index, double E, double J);                 /* src/cmc/cmc_orbit.c:413 */
+cmc_orbit_rs_t calc_pericenter_and_apocenter(long index, double E, double J);  /* north_star name of the same */
+long   find_zero_Q(long j, long kmin, long kmax, long double E, long double J); /* src/cmc/cmc_nr.c:316 */
+/* One Henon step in the order of src/cmc/cmc.c:240-407 (configs' switch set). */
+int    cmc_hotpath_step(cmcg_orbit_stats *stats);
+
+/* --- initial conditions (seeded generators; host side) -------------------- */
+/* Outputs are sorted by r with ids 1..n, stored masses m*N_STAR, sentinels at 0 and
+ * n+1, Henon units scaled exactly to PE = -1/2, KE = 1/4.  Arrays need n+2 entries;
+ * r, m, vr, vt and id must be non-NULL.  *rtid receives the tidal radius to use as
+ * cmcg_params.orbit_r. */
+int cmc_ic_plummer(int64_t n, uint64_t seed, int kroupa, cmcg_host_soa *out, double *rtid);
+int cmc_ic_king(int64_t n, double w0, uint64_t seed, int kroupa, int segregate, cmcg_host_soa *out, double *rtid);
+/* Kroupa (2001) two-segment IMF, alpha = 1.3 on [0.08, 0.5] and 2.3 on [0.5, 150]
+ * solar masses; n draws into m[0..n-1] (not normalised). */
+int cmc_ic_kroupa(int64_t n, uint64_t seed, double *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,196 @@
+/*
+ * cmcg — MI355X (gfx950) kernels for the CMC-COSMIC Henon-step hot path, behind a
+ * plain C ABI.  No torch or HIP types appear in these signatures: contexts are
+ * opaque, buffers are host pointers with element counts, errors are negative ints.
+ *
+ * Index conventions follow the reference (ClusterMonteCarlo/CMC-COSMIC):
+ *   - the global arrays star_r / star_m / star_phi (include/cmc/cmc_vars.h:218-226)
+ *     hold N+2 entries; index 0 is the central sentinel r = ZERO = 1e-20 and index
+ *     N+1 the outer sentinel r = SF_INFINITY = 1e10 (include/cmc/cmc.h:58,60);
+ *   - per-star fields (the AoS star_t of include/cmc/cmc.h:409-601) use slots 1..N;
+ *   - stored masses are m * N_STAR and madhoc = 1/N_STAR (src/cmc/cmc_fits.c:99,123,
+ *     src/cmc/cmc_utils.c:1183).
+ *
+ * One context = one device stream.  Phase calls are asynchronous on that stream;
+ * scalars they return are copied back at the end of the call (which synchronises
+ * the stream) unless the *_async variant is used.  A context is not thread-safe.
+ *
+ * Random numbers: each star (relaxation pair, orbit-sampling star) draws from its
+ * own substream of the reference's taus113 generator (src/libs/taus113-v2.c),
+ * reached by GF(2) jump-ahead from the state seeded by reset_rng_t113_new(IDUM):
+ *   relaxation, step s, pair p (stars 2p+1, 2p+2):  offset s*2^72 + 3p
+ *   orbits,     step s, star slot i:                 offset s*2^72 + 2^70 + i*2^20
+ * Step 0's relaxation therefore draws exactly the reference's sequential stream.
+ */
+#ifndef CMCG_H
+#define CMCG_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+/* Error codes map the reference's fail-stop sites (exit_cleanly, src/cmc/cmc_utils.c:208-220). */
+#define CMCG_OK            0
+#define CMCG_E_NAN        -1   /* NaN in potential or Q (cmc_utils.c:688-691,721-733; cmc_evolution_thr.c:880-883) */
+#define CMCG_E_NTRY       -2   /* N_TRY exceeded in rejection sampling (cmc_evolution_thr.c:890-893) */
+#define CMCG_E_BRACKET    -3   /* FindZero_r / find_zero_Q bracket failure (cmc_utils.c:144-152) */
+#define CMCG_E_WZERO      -4   /* W == 0 in calc_encounter_dyns (cmc_dynamics_helper.c:858-863) */
+#define CMCG_E_WPZERO     -5   /* wp == 0 in dynamics_apply (cmc_dynamics.c:450-453) */
+#define CMCG_E_BRENT      -6   /* find_root_vr failed (cmc_nr.c:510-517,577-584,597-600) */
+#define CMCG_E_TIES       -7   /* a run of equal radii too long for the tie-break pass */
+#define CMCG_E_HIP       -10   /* HIP runtime error */
+#define CMCG_E_ARG       -11   /* invalid argument */
+#define CMCG_E_CAPACITY  -12   /* more stars than the context capacity */
+#define CMCG_E_NOMEM     -13   /* device allocation failed */
+#define CMCG_E_STATE     -14   /* call out of order (e.g. no bracket model loaded) */
+
+const char *cmcg_error_string(int code);
+
+/* ------------------------------------------------------------------ types */
+typedef struct cmcg_ctx cmcg_ctx;
+
+/* Run parameters.  Defaults follow the reference parser (src/cmc/cmc_io.c:1659-2022). */
+typedef struct cmcg_params {
+    int64_t n_star;        /* clus.N_STAR; madhoc = 1/n_star */
+    int64_t avekernel;     /* AVEKERNEL (cmc_io.c:1749), default 20 */
+    double  theta_se_max;  /* THETASEMAX (cmc_io.c:1671), default 1.412 */
+    double  gamma;         /* CMC_GAMMA Coulomb-log factor (cmc_io.c:1252-1255) */
+    double  orbit_r;       /* R_MAX = input RTID (cmc_io.c:2071, cmc.c:145) */
+    double  cenma_m;       /* central mass in stored units (0 in these configs) */
+    uint64_t seed;         /* IDUM (cmc_io.c:1736); seed 0 means 1 */
+    int32_t  tidal_strip;  /* TIDALLY_STRIP_STARS (cmc_io.c:1670), default 1 */
+    int32_t  flags;        /* CMCG_PF_* */
+} cmcg_params;
+
+#define CMCG_PF_LEARNED_BRACKETS 1   /* use the loaded bracket model in cmcg_orbits */
+
+/* Host-side SoA view used by upload/download.  Any pointer may be NULL (field
+ * skipped).  Arrays r, m, phi have n+2 entries (0..n+1); per-star arrays have n+2
+ * entries too, slot 0 and n+1 unused.  Borrowed only for the duration of a call. */
+typedef struct cmcg_host_soa {
+    int64_t n;
+    double *r, *m, *phi;
+    double *vr, *vt, *E, *J;
+    int64_t *id;
+    double *rnew, *vrnew, *vtnew, *X, *Y, *r_peri, *r_apo;
+    double *rOld, *Uoldrold, *Uoldrnew, *vrold, *vtold, *EI;
+    double *sigma;
+    double *pot_rnew;      /* potential(rnew) seen by the orbit phase (diagnostic) */
+    int32_t *ntry;         /* rejection tries of the last orbit phase */
+    int32_t *flags;        /* CMCG_SF_* outcome bits of the last orbit phase */
+} cmcg_host_soa;
+
+#define CMCG_SF_CIRCULAR   1
+#define CMCG_SF_REMOVED_E  2
+#define CMCG_SF_REMOVED_RT 4
+#define CMCG_SF_DESTROYED  8
+#define CMCG_SF_BRENT     16
+#define CMCG_SF_LINSCAN   32
+#define CMCG_SF_STRIPPED  64
+#define CMCG_SF_BRACKET_HIT 128   /* learned bracket bracketed both roots */
+
+/* Cluster scalars kept on the device (the reference's globals of the same names). */
+typedef struct cmcg_scalars {
+    int64_t n_max, n_max_new;
+    double Mtotal, Rtidal, max_r, Dt, Prev_Dt, TotalTime;
+    double TidalMassLoss, TidalMassLoss_old, OldTidalMassLoss;
+    double Eescaped, Jescaped, Etidal, Eoops;
+    double Etot, Ekin, Epot;
+    uint64_t step;
+} cmcg_scalars;
+
+typedef struct cmcg_orbit_stats {
+    int64_t n_circular, n_removed_E, n_removed_rt, n_destroyed, n_brent, n_linscan;
+    int64_t tries_total, tries_max;
+    int64_t n_bracket_hit, n_bracket_fallback;   /* learned brackets */
+    double  max_r, tidal_mass_loss, E_escaped, J_escaped;
+    int32_t error; int32_t pad;
+    int64_t error_index;
+} cmcg_orbit_stats;
+
+/* ------------------------------------------------------------------ lifetime */
+/* Replaces the allocation of the star arrays (cmc_io.c:2099-2104,3434-3436). */
+int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *params);
+int cmcg_destroy(cmcg_ctx *ctx);
+int cmcg_upload(cmcg_ctx *ctx, const cmcg_host_soa *h);      /* sets N_MAX = N_MAX_NEW = h->n */
+int cmcg_download(cmcg_ctx *ctx, cmcg_host_soa *h);          /* h->n receives N_MAX_NEW */
+int cmcg_get_scalars(cmcg_ctx *ctx, cmcg_scalars *out);
+int cmcg_set_scalars(cmcg_ctx *ctx, const cmcg_scalars *in);
+int cmcg_sync(cmcg_ctx *ctx);
+
+/* ------------------------------------------------------------------ hot-path phases */
+/* a1: qsorts_new + remove_stripped_stars + post_sort_comm
+ *     (src/cmc/cmc_utils.c:2232-2288, src/cmc/cmc_sort.c:245-250,343-354,375-629).
+ *     Sorts slots 1..N_MAX_NEW by (r, id) ascending, drops the r == SF_INFINITY tail. */
+int cmcg_sort_by_r(cmcg_ctx *ctx, int64_t *n_max);
+
+/* a2: calc_potential_new -> potential_calculate (src/cmc/cmc_utils.c:675-736,2006-2012). */
+int cmcg_potential(cmcg_ctx *ctx, int64_t *n_max, double *mtotal, double *rtidal);
+
+/* a3: potential(r) (src/cmc/cmc_utils.c:112-165) for a batch of radii (host arrays). */
+int cmcg_potential_at(cmcg_ctx *ctx, int64_t count, const double *r, double *out);
+
+/* a10: simul_relax_new (src/cmc/cmc_dynamics_helper.c:2526-2579). */
+int cmcg_simul_relax(cmcg_ctx *ctx, double *dtrel);
+/* calc_timestep (src/cmc/cmc_utils.c:2132-2148 + GetTimeStep cmc_evolution_thr.c:317-438):
+ * sets Dt and TotalTime on the device. */
+int cmcg_calc_timestep(cmcg_ctx *ctx, double *dt);
+
+/* calc_sigma_r (src/cmc/cmc_dynamics_helper.c:2821-2986) into the sigma array. */
+int cmcg_sigma_r(cmcg_ctx *ctx);
+
+/* a4: dynamics_apply relaxation branch (src/cmc/cmc_dynamics.c:17-542). step selects
+ * the random substreams. Includes calc_sigma_r as the reference does (:37). */
+int cmcg_relax(cmcg_ctx *ctx, double dt, uint64_t step);
+
+/* a5: new_orbits_calculate -> get_positions (src/cmc/cmc_utils.c:2194-2211,
+ *     src/cmc/cmc_evolution_thr.c:784-1023). */
+int cmcg_orbits(cmcg_ctx *ctx, uint64_t step, cmcg_orbit_stats *stats);
+
+/* a6: calc_orbit_new (src/cmc/cmc_orbit.c:413-540; the north_star's
+ *     calc_pericenter_and_apocenter) for slots first..first+count-1 with the given
+ *     E (already including the self term, cmc_evolution_thr.c:806) and J.
+ *     Outputs may be NULL. */
+int cmcg_calc_pericenter_and_apocenter(cmcg_ctx *ctx, int64_t first, int64_t count,
+                                       const double *E, const double *J,
+                                       double *rp, double *ra, double *dQdrp, double *dQdra,
+                                       int64_t *kmin, int64_t *kmax, int32_t *circular);
+
+/* a7: find_zero_Q (src/cmc/cmc_nr.c:316-383) for a batch of (j, kmin, kmax, E, J). */
+int cmcg_find_zero_Q(cmcg_ctx *ctx, int64_t count, const int64_t *j, const int64_t *kmin,
+                     const int64_t *kmax, const double *E, const double *J, int64_t *out);
+
+/* ------------------------------------------------------------------ step glue */
+int cmcg_energy_conservation1(cmcg_ctx *ctx);        /* cmc_utils.c:2153-2172 */
+int cmcg_energy_conservation2(cmcg_ctx *ctx);        /* cmc_utils.c:2217-2227 */
+int cmcg_tidally_strip_stars(cmcg_ctx *ctx);         /* cmc_evolution_thr.c:443-646 */
+int cmcg_compute_intermediate_energy(cmcg_ctx *ctx); /* cmc_utils.c:368-391 */
+int cmcg_set_velocities3(cmcg_ctx *ctx);             /* cmc_utils.c:254-354 */
+int cmcg_compute_energy(cmcg_ctx *ctx);              /* cmc_utils.c:604-653, 2017-2028 */
+/* Initial potential, sigma and energies (cmc.c:151-165). */
+int cmcg_init_cluster(cmcg_ctx *ctx);
+/* One whole Henon step in the order of src/cmc/cmc.c:240-407. */
+int cmcg_step(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
+
+/* ------------------------------------------------------------------ RNG access */
+/* Draw `ndraw` consecutive doubles of the substream of each of `count` consecutive
+ * pairs/stars starting at `first` (phase 0 = relaxation pair index, 1 = orbit slot).
+ * out is count*ndraw doubles, pair/star-major. */
+int cmcg_rng_draws(cmcg_ctx *ctx, int phase, uint64_t step, int64_t first, int64_t count,
+                   int ndraw, double *out);
+
+/* ------------------------------------------------------------------ timing */
+/* Average device time (ms) of the named kernel family over the calls since the last
+ * reset, measured with HIP events on the context stream. names: "sort", "potential",
+ * "relax", "orbits", "glue", "step". */
+int cmcg_timer_enable(cmcg_ctx *ctx, int on);
+int cmcg_timer_get(cmcg_ctx *ctx, const char *name, double *ms_total, int64_t *calls);
+int cmcg_timer_reset(cmcg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,199 @@
+"""ctypes binding of the CPU oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this, as
+the checker or the timed CPU baseline.  The state is a numpy SoA mirror of
+cmc_oracle.h's orc_state.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_TAUS = os.path.join(HERE, "_ref", "libtaus113_ref.so")
+
+lib = C.CDLL(LIB_PATH)
+
+DPTR = C.POINTER(C.c_double)
+I64P = C.POINTER(C.c_int64)
+I32P = C.POINTER(C.c_int32)
+
+
+class T113(C.Structure):
+    _fields_ = [("z", C.c_uint32 * 4)]
+
+    def tolist(self):
+        return [int(x) for x in self.z]
+
+
+D_FIELDS_A = ["r", "m", "phi", "vr", "vt", "E", "J"]
+D_FIELDS_B = ["rnew", "vrnew", "vtnew", "X", "Y", "r_peri", "r_apo", "rOld", "Uoldrold", "Uoldrnew",
+              "vrold", "vtold", "EI", "pot_rnew", "sigma"]
+SCALARS = ["madhoc", "cenma_m", "Mtotal", "Rtidal", "orbit_r", "max_r", "Dt", "Prev_Dt", "TotalTime",
+           "TidalMassLoss", "TidalMassLoss_old", "OldTidalMassLoss", "Eescaped", "Jescaped", "Etidal", "Eoops",
+           "Etot", "Ekin", "Epot", "theta_se_max", "gamma"]
+
+
+class State(C.Structure):
+    _fields_ = ([("N_STAR", C.c_int64), ("N_MAX", C.c_int64), ("N_MAX_NEW", C.c_int64), ("cap", C.c_int64)]
+                + [(k, DPTR) for k in D_FIELDS_A] + [("id", I64P)] + [(k, DPTR) for k in D_FIELDS_B]
+                + [("ntry", I32P), ("flags", I32P)] + [(k, C.c_double) for k in SCALARS]
+                + [("avekernel", C.c_int64), ("seeded", T113), ("seq", T113), ("rng_mode", C.c_int32),
+                   ("q_mode", C.c_int32), ("step", C.c_uint64)]
+                + [(k, C.c_int64) for k in ["n_brent", "n_circular", "n_removed_E", "n_removed_tidal",
+                                            "n_removed_mass", "n_linscan"]]
+                + [("error", C.c_int32), ("error_index", C.c_int64)])
+
+
+class OrbitRS(C.Structure):
+    _fields_ = [("rp", C.c_double), ("ra", C.c_double), ("dQdrp", C.c_double), ("dQdra", C.c_double),
+                ("kmax", C.c_int64), ("kmin", C.c_int64), ("circular_flag", C.c_int32)]
+
+
+_SP = C.POINTER(State)
+_sigs = {
+    "orc_t113_seed": (None, [C.c_uint64, C.POINTER(T113)]),
+    "orc_t113_int": (C.c_uint32, [C.POINTER(T113)]),
+    "orc_t113_dbl": (C.c_double, [C.POINTER(T113)]),
+    "orc_t113_jump_poly": (T113, [T113, C.POINTER(C.c_uint32)]),
+    "orc_t113_jump": (T113, [T113, C.c_uint64, C.c_uint64]),
+    "orc_stream_relax": (T113, [C.POINTER(T113), C.c_uint64, C.c_uint64]),
+    "orc_stream_orbit": (T113, [C.POINTER(T113), C.c_uint64, C.c_uint64]),
+    "orc_FindZero_r": (C.c_int64, [_SP, C.c_int64, C.c_int64, C.c_double]),
+    "orc_potential": (C.c_double, [_SP, C.c_double]),
+    "orc_potential_calculate": (C.c_int64, [_SP]),
+    "orc_function_q": (C.c_double, [_SP, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "orc_find_zero_Q": (C.c_int64, [_SP, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double]),
+    "orc_get_positive_Q_index": (C.c_int64, [_SP, C.c_int64, C.c_double, C.c_double]),
+    "orc_calc_orbit_new": (OrbitRS, [_SP, C.c_int64, C.c_double, C.c_double]),
+    "orc_find_root_vr": (C.c_double, [_SP, C.c_int64, C.c_int64, C.c_double, C.c_double]),
+    "orc_get_positions": (C.c_double, [_SP]),
+    "orc_calc_n_local": (C.c_double, [_SP, C.c_int64, C.c_int64, C.c_int64]),
+    "orc_calc_sigma_r": (None, [_SP]),
+    "orc_simul_relax_new": (C.c_double, [_SP]),
+    "orc_dynamics_apply": (None, [_SP, C.c_double]),
+    "orc_sort_by_r": (C.c_int64, [_SP]),
+    "orc_energy_conservation1": (None, [_SP]),
+    "orc_energy_conservation2": (None, [_SP]),
+    "orc_tidally_strip_stars": (None, [_SP]),
+    "orc_compute_intermediate_energy": (None, [_SP]),
+    "orc_set_velocities3": (None, [_SP]),
+    "orc_compute_energy": (None, [_SP]),
+    "orc_calc_timestep": (C.c_double, [_SP]),
+    "orc_init": (None, [_SP]),
+    "orc_step": (None, [_SP]),
+}
+for _n, (_r, _a) in _sigs.items():
+    _f = getattr(lib, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+
+RNG_REF_SEQ, RNG_PER_STAR = 0, 1
+Q_LONG_DOUBLE, Q_FP64 = 0, 1
+
+
+def seed_state(seed):
+    st = T113()
+    lib.orc_t113_seed(seed, C.byref(st))
+    return st
+
+
+class Oracle:
+    """Oracle cluster state with numpy-owned arrays."""
+
+    def __init__(self, cluster_arrays, n, n_star=None, orbit_r=1e3, seed=1, rng_mode=RNG_PER_STAR,
+                 q_mode=Q_LONG_DOUBLE, theta_se_max=1.412, avekernel=20, gamma=0.1):
+        self.n = int(n)
+        cap = self.n + 2
+        self.arr = {}
+        for k in D_FIELDS_A + D_FIELDS_B:
+            src = cluster_arrays.get(k)
+            a = np.zeros(cap, np.float64)
+            if src is not None:
+                a[:] = np.asarray(src, np.float64)[:cap]
+            self.arr[k] = a
+        ids = cluster_arrays.get("id")
+        self.arr["id"] = np.zeros(cap, np.int64) if ids is None else np.array(ids[:cap], np.int64)
+        self.arr["ntry"] = np.zeros(cap, np.int32)
+        self.arr["flags"] = np.zeros(cap, np.int32)
+        s = State()
+        s.N_STAR = int(n_star or n)
+        s.N_MAX = self.n
+        s.N_MAX_NEW = self.n
+        s.cap = cap
+        for k in D_FIELDS_A + D_FIELDS_B:
+            setattr(s, k, self.arr[k].ctypes.data_as(DPTR))
+        s.id = self.arr["id"].ctypes.data_as(I64P)
+        s.ntry = self.arr["ntry"].ctypes.data_as(I32P)
+        s.flags = self.arr["flags"].ctypes.data_as(I32P)
+        s.madhoc = 1.0 / float(s.N_STAR)
+        s.orbit_r = float(orbit_r)
+        s.theta_se_max = float(theta_se_max)
+        s.gamma = float(gamma)
+        s.avekernel = int(avekernel)
+        s.seeded = seed_state(seed)
+        s.seq = seed_state(seed)
+        s.rng_mode = int(rng_mode)
+        s.q_mode = int(q_mode)
+        self.s = s
+
+    def __getitem__(self, k):
+        return self.arr[k]
+
+    @property
+    def N(self):
+        return int(self.s.N_MAX)
+
+    def sc(self, k):
+        return getattr(self.s, k)
+
+    def call(self, name, *args):
+        return getattr(lib, name)(C.byref(self.s), *args)
+
+    def init(self):
+        lib.orc_init(C.byref(self.s))
+
+    def step(self):
+        lib.orc_step(C.byref(self.s))
+        return self.s.error
+
+    def potential_calculate(self):
+        return lib.orc_potential_calculate(C.byref(self.s))
+
+    def potential(self, r):
+        return lib.orc_potential(C.byref(self.s), float(r))
+
+    def sort(self):
+        return lib.orc_sort_by_r(C.byref(self.s))
+
+    def relax(self, dt):
+        lib.orc_dynamics_apply(C.byref(self.s), float(dt))
+
+    def orbits(self):
+        return lib.orc_get_positions(C.byref(self.s))
+
+    def apsides(self, j, E, J):
+        o = lib.orc_calc_orbit_new(C.byref(self.s), int(j), float(E), float(J))
+        return dict(rp=o.rp, ra=o.ra, dQdrp=o.dQdrp, dQdra=o.dQdra, kmin=o.kmin, kmax=o.kmax,
+                    circular=o.circular_flag)
+
+
+def ref_taus_lib():
+    """The reference's own taus113-v2.c compiled from source into oracle/_ref (or None)."""
+    if not os.path.exists(REF_TAUS):
+        return None
+    L = C.CDLL(REF_TAUS)
+
+    class RefState(C.Structure):
+        _fields_ = [("z", C.c_ulong * 4)]
+
+    L.reset_rng_t113_new.argtypes = [C.c_ulong, C.POINTER(RefState)]
+    L.rng_t113_int_new.argtypes = [C.POINTER(RefState)]
+    L.rng_t113_int_new.restype = C.c_ulong
+    L.rng_t113_dbl_new.argtypes = [C.POINTER(RefState)]
+    L.rng_t113_dbl_new.restype = C.c_double
+    L.rng_t113_jump.argtypes = [RefState, C.POINTER(C.c_uint)]
+    L.rng_t113_jump.restype = RefState
+    L.RefState = RefState
+    return L

@@ -100,7 +100,7 @@ int dalloc(cmcg_ctx *c, T **p, size_t count)
 
 int alloc_fields(cmcg_ctx *c, DevFields &f, bool full)
 {
-    size_t n = (size_t)c->n_cap + 2;
+    size_t n = (size_t)c->n_cap + 6;     // slots 0..n_cap+1 plus 4 entries of padding
     double **dfields_sorted[] = {&f.r, &f.m, &f.vr, &f.vt, &f.E, &f.J, &f.rnew, &f.vrnew, &f.vtnew, &f.X, &f.Y,
                                  &f.r_peri, &f.r_apo, &f.rOld, &f.Uoldrold, &f.Uoldrnew, &f.vrold, &f.vtold, &f.EI};
     for (auto pp : dfields_sorted)
@@ -309,10 +309,10 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
     sc.max_events = 4096;
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 6)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 8)) ||
-        (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 1)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 1)) ||
-        (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 1)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 1)) ||
+        (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
+        (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
-        (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 2)) || (e = dalloc(c, &sc.excess_list, (size_t)sc.max_events)) ||
+        (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)sc.max_events)) ||
         (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
         (e = dalloc(c, &c->T, 1))) {
         cmcg_destroy(c);

@@ -159,6 +159,7 @@ __device__ __forceinline__ double potential_at(const double *__restrict__ r, con
     int64_t i;
     if (lo == hi - 1) i = lo;
     else i = find_zero_r(r, lo, hi, x);
+    if (i > n_max) { *bad = true; return 0.0; }   // x beyond the outer sentinel
     double ri = r[i], ri1 = r[i + 1];
     if (ri > x || ri1 < x) { *bad = true; return 0.0; }
     if (i == 0) return phi[1];

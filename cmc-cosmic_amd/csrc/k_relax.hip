@@ -18,6 +18,10 @@ __device__ __forceinline__ double calc_n_local(const double *__restrict__ r, int
     int64_t kmin = k - p, kmax = k + p;
     if (kmin < 0) { kmin = 0; kmax = 2 * p + 1; }
     else if (kmax > N_LIMIT) { kmax = N_LIMIT; kmin = N_LIMIT - 2 * p - 1; }
+    // clusters smaller than 2p+2 stars would index outside [0, N+1] (undefined in
+    // the reference as well); clamp to the array
+    if (kmax > N_LIMIT + 1) kmax = N_LIMIT + 1;
+    if (kmin < 0) kmin = 0;
     return (2.0 * ((double)p)) * 3.0 / (4.0 * CMCG_PI * (cub(r[kmax]) - cub(r[kmin])));
 }
 
@@ -115,6 +119,8 @@ __global__ void sigma_kernel(const double *__restrict__ m, const double *__restr
     int64_t simin = si - p, simax = simin + (2 * p - 1);
     if (simin < 1) { simin = 1; simax = simin + (2 * p - 1); }
     else if (simax > N) { simax = N; simin = simax - (2 * p - 1); }
+    if (simin < 1) simin = 1;      // N < 2p: keep the window inside the array
+    if (simax > N) simax = N;
     double Mv2ave = 0.0, Mave = 0.0;
     for (int64_t k = simin; k <= simax; k++) {
         double mk = m[k] * P.madhoc;

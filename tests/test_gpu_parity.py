@@ -1,0 +1,411 @@
+"""Parity of the HIP hot path (through the C-ABI) with the CPU oracle on the same
+seeded inputs.
+
+Bars (tolerances written in each test):
+  - RNG draws, sort permutation, time step, apsides indices: bit-exact;
+  - potential: <= 1e-12 relative;
+  - relaxation (libm sin/cos on both sides): <= 1e-10 relative on the speed scale;
+  - orbit sampling from identical inputs: bit-exact against the oracle's FP64-Q
+    mode; against the reference's long-double Q, r / v_r / v_t within 1e-10 except
+    for reported branch flips;
+  - whole step from the same state: r, v_r, v_t within 1e-10 except reported flips;
+  - energy drift over 100 steps within 1e-6 of the oracle's.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_for(O, cl, **kw):
+    kw.setdefault("orbit_r", cl.rtid)
+    return O.Oracle(cl.a, cl.n, **kw)
+
+
+def ctx_for(cm, cl, cap=None, seed=1, **kw):
+    p = cm.default_params(cl.n, orbit_r=cl.rtid, seed=seed, **kw)
+    ctx = cm.Context(cap or cl.n, p)
+    return ctx
+
+
+def upload_state(ctx, cl, o, extra=()):
+    """Upload the cluster with the oracle's current phi/E/J and scalars."""
+    h = cl.copy()
+    for k in ("phi", "E", "J", "sigma") + tuple(extra):
+        h[k] = o[k]
+    ctx.upload(h)
+    ctx.set_scalars(n_max=o.N, n_max_new=o.N, Mtotal=o.sc("Mtotal"), Rtidal=o.sc("Rtidal"),
+                    TidalMassLoss=0.0, TidalMassLoss_old=0.0, OldTidalMassLoss=0.0, Eescaped=0.0,
+                    Jescaped=0.0, Etidal=0.0, Dt=0.0, Prev_Dt=0.0, TotalTime=0.0, step=0)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    d = np.abs(a - b)
+    s = np.maximum(np.abs(a), np.abs(b))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(d == 0, 0.0, d / s)
+
+
+# ------------------------------------------------------------------ RNG
+def test_gpu_rng_streams_bit_exact(gpu, O):
+    cm = gpu
+    say("rng streams seed 1234")
+    cl = cm.make_plummer(1000, seed=1)
+    with ctx_for(cm, cl, seed=1234) as ctx:
+        seeded = O.seed_state(1234)
+        for phase, step, first in [(0, 0, 0), (0, 3, 100000), (1, 0, 1), (1, 7, 3999000)]:
+            got = ctx.rng_draws(phase, step, first, 64, 5)
+            for i in range(0, 64, 9):
+                fn = O.lib.orc_stream_relax if phase == 0 else O.lib.orc_stream_orbit
+                st = fn(C.byref(seeded), step, first + i)
+                want = [O.lib.orc_t113_dbl(C.byref(st)) for _ in range(5)]
+                assert list(got[i]) == want, (phase, step, first + i)
+
+
+# ------------------------------------------------------------------ potential
+@pytest.mark.parametrize("case", ["plummer_1e5", "plummer_kroupa_1e6", "king_4e6"])
+def test_gpu_potential_parity(gpu, O, case):
+    cm = gpu
+    say("potential", case)
+    if case == "plummer_1e5":
+        cl = cm.make_plummer(100000, seed=1)
+    elif case == "plummer_kroupa_1e6":
+        cl = cm.make_plummer(1000000, seed=1, kroupa=True)
+    else:
+        cl = cm.make_king(4000000, 7.0, seed=1)
+    o = oracle_for(O, cl)
+    n = o.potential_calculate()
+    with ctx_for(cm, cl) as ctx:
+        ctx.upload(cl)
+        ng, mt, rt = ctx.potential()
+        d = ctx.download()
+    assert ng == n
+    assert mt == pytest.approx(o.sc("Mtotal"), rel=1e-15)
+    assert rt == pytest.approx(o.sc("Rtidal"), rel=1e-14)
+    err = rel(d["phi"][: n + 2], o["phi"][: n + 2])
+    say(case, "max rel phi err", err.max(), "median", np.median(err))
+    assert err.max() <= 1e-12
+    assert d["r"][n + 1] == 1e10 and d["phi"][n + 1] == 0.0
+
+
+def test_gpu_potential_tail_and_count(gpu, O):
+    cm = gpu
+    say("potential with stripped tail")
+    cl = cm.make_plummer(50001, seed=2)
+    cl["r"][49000:50002] = 1e10
+    o = oracle_for(O, cl)
+    n = o.potential_calculate()
+    with ctx_for(cm, cl) as ctx:
+        ctx.upload(cl)
+        ng, mt, rt = ctx.potential()
+        d = ctx.download(n_hint=cl.n)
+    assert ng == n == 48999
+    assert rel(d["phi"][: n + 2], o["phi"][: n + 2]).max() <= 1e-12
+
+
+# ------------------------------------------------------------------ sort
+@pytest.mark.parametrize("n,ties", [(100003, True), (4000000, False)])
+def test_gpu_sort_bit_exact(gpu, O, n, ties):
+    cm = gpu
+    say("sort n", n, "ties", ties)
+    rng = np.random.default_rng(n)
+    cl = cm.Cluster(n)
+    if ties:
+        r = rng.choice(rng.random(n // 3) * 5 + 1e-3, n)
+    else:
+        r = np.exp(rng.normal(size=n) * 2)
+    r[rng.choice(n, n // 200, replace=False)] = 1e10
+    ids = rng.permutation(n).astype(np.int64) + 1
+    cl["r"][1:n + 1] = r
+    cl["r"][0], cl["r"][n + 1] = 1e-20, 1e10
+    cl["id"][1:n + 1] = ids
+    for k in ("m", "vr", "vt", "E", "J", "rOld", "Uoldrold", "Uoldrnew", "vrold", "vtold", "r_apo", "X"):
+        cl[k][1:n + 1] = rng.random(n)
+    cl.rtid = 100.0
+    o = oracle_for(O, cl)
+    nn = o.sort()
+    with ctx_for(cm, cl) as ctx:
+        ctx.upload(cl)
+        ng = ctx.sort_by_r()
+        d = ctx.download()
+    assert ng == nn
+    for k in ("r", "id", "m", "vr", "vt", "E", "J", "rOld", "Uoldrold", "Uoldrnew", "vrold", "vtold", "r_apo", "X"):
+        assert np.array_equal(d[k][1:nn + 1], o[k][1:nn + 1]), k
+    assert d["r"][nn + 1] == 1e10
+
+
+# ------------------------------------------------------------------ relaxation
+@pytest.mark.parametrize("case", ["plummer_1e5", "plummer_kroupa_1e6"])
+def test_gpu_relaxation_parity(gpu, O, case):
+    cm = gpu
+    say("relaxation", case)
+    cl = cm.make_plummer(100000, seed=1) if case == "plummer_1e5" else cm.make_plummer(1000000, seed=2, kroupa=True)
+    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR)
+    o.init()
+    dt = O.lib.orc_simul_relax_new(C.byref(o.s))
+    with ctx_for(cm, cl) as ctx:
+        upload_state(ctx, cl, o)
+        dt_gpu = ctx.simul_relax()
+        assert dt_gpu == dt                      # bit-exact window sums and minimum
+        ctx.relax(dt, step=0)
+        d = ctx.download()
+    o.relax(dt)
+    N = o.N
+    assert np.array_equal(d["Y"][1:N], o["Y"][1:N])
+    scale = np.sqrt(o["vr"][1:N + 1] ** 2 + o["vt"][1:N + 1] ** 2) + 1e-300
+    for k in ("vr", "vt"):
+        err = np.abs(d[k][1:N + 1] - o[k][1:N + 1]) / scale
+        say(case, k, "max err / |v|", err.max())
+        assert err.max() <= 1e-10
+    for k in ("E", "J"):
+        err = rel(d[k][1:N + 1], o[k][1:N + 1])
+        assert err.max() <= 1e-10, k
+    # sigma (direct window sums on the device, sliding sums in the reference)
+    err = rel(d["sigma"][1:N + 1], o["sigma"][1:N + 1])
+    say(case, "sigma max rel", err.max())
+    assert err.max() <= 1e-10
+
+
+# ------------------------------------------------------------------ orbits
+def _orbit_case(cm, O, cl, qmode):
+    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=qmode)
+    o.init()
+    with ctx_for(cm, cl) as ctx:
+        upload_state(ctx, cl, o)
+        st = ctx.orbits(step=0)
+        d = ctx.download()
+    o.s.step = 0
+    o.orbits()
+    return o, d, st
+
+
+@pytest.mark.parametrize("case", ["plummer_1e5", "plummer_kroupa_1e6", "king_1e6"])
+def test_gpu_orbits_bit_exact_vs_fp64_oracle(gpu, O, case):
+    cm = gpu
+    say("orbits fp64-Q parity", case)
+    if case == "plummer_1e5":
+        cl = cm.make_plummer(100000, seed=1)
+    elif case == "plummer_kroupa_1e6":
+        cl = cm.make_plummer(1000000, seed=3, kroupa=True)
+    else:
+        cl = cm.make_king(1000000, 7.0, seed=1)
+    o, d, st = _orbit_case(cm, O, cl, O.Q_FP64)
+    N = o.N
+    say(case, "stats", st)
+    assert st["error"] == 0 and o.s.error == 0
+    for k in ("rnew", "vrnew", "vtnew", "r_peri", "r_apo", "X"):
+        same = np.sum(d[k][1:N + 1] == o[k][1:N + 1])
+        say(case, k, "bit-equal", same, "of", N)
+        assert same == N, k
+    assert np.array_equal(d["ntry"][1:N + 1], o["ntry"][1:N + 1])
+    assert st["n_circular"] == o.sc("n_circular")
+    assert st["max_r"] == o.sc("max_r")
+
+
+@pytest.mark.parametrize("case", ["plummer_1e5", "king_1e6"])
+def test_gpu_orbits_vs_reference_long_double(gpu, O, case):
+    """Against the reference's x87 long-double Q: r, v_r, v_t within 1e-10 except
+    for stars whose apsides bracket or acceptance branch flips (reported)."""
+    cm = gpu
+    say("orbits vs long-double oracle", case)
+    cl = cm.make_plummer(100000, seed=1) if case == "plummer_1e5" else cm.make_king(1000000, 7.0, seed=2)
+    o, d, st = _orbit_case(cm, O, cl, O.Q_LONG_DOUBLE)
+    N = o.N
+    bad = np.zeros(N, bool)
+    for k in ("rnew", "vrnew", "vtnew"):
+        bad |= rel(d[k][1:N + 1], o[k][1:N + 1]) > 1e-10
+    flips = int(bad.sum())
+    say(case, "flipped stars", flips, "fraction", flips / N)
+    assert flips / N < 1e-3
+
+
+def test_gpu_apsides_and_find_zero_Q(gpu, O):
+    cm = gpu
+    say("apsides / find_zero_Q batch")
+    cl = cm.make_king(50000, 7.0, seed=5)
+    o = oracle_for(O, cl, q_mode=O.Q_FP64)
+    o.init()
+    N = o.N
+    js = np.arange(1, N + 1, 37)
+    E = o["E"][js] + o["m"][js] / o["r"][js] / o.s.N_STAR
+    J = o["J"][js]
+    with ctx_for(cm, cl) as ctx:
+        upload_state(ctx, cl, o)
+        res = {k: [] for k in ("rp", "ra", "kmin", "kmax", "circular")}
+        for j, e, jj in zip(js, E, J):
+            r1 = ctx.apsides(int(j), [e], [jj])
+            for k in res:
+                res[k].append(r1[k][0])
+        kq = ctx.find_zero_Q(js, np.zeros_like(js), js, E, J)
+    for i, j in enumerate(js):
+        w = o.apsides(int(j), E[i], J[i])
+        for k in res:
+            assert res[k][i] == w[k], (j, k)
+        assert kq[i] == O.lib.orc_find_zero_Q(C.byref(o.s), int(j), 0, int(j), E[i], J[i])
+
+
+# ------------------------------------------------------------------ glue
+def test_gpu_glue_parity(gpu, O):
+    cm = gpu
+    say("glue: energy conservation, strip, intermediate energy, set_velocities3, energy")
+    cl = cm.make_king(200000, 7.0, seed=4)
+    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64)
+    o.init()
+    with ctx_for(cm, cl) as ctx:
+        upload_state(ctx, cl, o)
+        ctx.energy_conservation1()
+        O.lib.orc_energy_conservation1(C.byref(o.s))
+        ctx.orbits(step=0)
+        o.s.step = 0
+        o.s.OldTidalMassLoss = 0.0
+        o.orbits()
+        ctx.energy_conservation2()
+        O.lib.orc_energy_conservation2(C.byref(o.s))
+        d = ctx.download()
+        N = o.N
+        for k in ("Uoldrold", "Uoldrnew", "vrold", "vtold", "rnew"):
+            assert np.array_equal(d[k][1:N + 1], o[k][1:N + 1]), k
+        ctx.tidally_strip_stars()
+        O.lib.orc_tidally_strip_stars(C.byref(o.s))
+        ctx.compute_intermediate_energy()
+        O.lib.orc_compute_intermediate_energy(C.byref(o.s))
+        d = ctx.download()
+        for k in ("r", "rOld", "vr", "vt", "m"):
+            assert np.array_equal(d[k][1:N + 1], o[k][1:N + 1]), k
+        live = o["rnew"][1:N + 1] < 1e6
+        assert rel(d["EI"][1:N + 1][live], o["EI"][1:N + 1][live]).max() <= 1e-12
+        ng = ctx.sort_by_r()
+        nn = o.sort()
+        assert ng == nn
+        ctx.potential()
+        o.potential_calculate()
+        # continue from the oracle's potential so set_velocities3 sees identical inputs
+        d = ctx.download()
+        assert rel(d["phi"][1:nn + 1], o["phi"][1:nn + 1]).max() <= 1e-12
+        h = cm.Cluster(nn)
+        for k in h.a:
+            if k in o.arr:
+                h[k] = o[k][: nn + 2]
+        ctx.upload(h)
+        ctx.set_velocities3()
+        O.lib.orc_set_velocities3(C.byref(o.s))
+        ctx.compute_energy()
+        O.lib.orc_compute_energy(C.byref(o.s))
+        d = ctx.download()
+        sc = ctx.scalars()
+    for k in ("vr", "vt", "E", "J"):
+        assert np.array_equal(d[k][1:nn + 1], o[k][1:nn + 1]), k
+    assert sc["Ekin"] == pytest.approx(o.sc("Ekin"), rel=1e-12)
+    assert sc["Epot"] == pytest.approx(o.sc("Epot"), rel=1e-12)
+
+
+# ------------------------------------------------------------------ whole step
+@pytest.mark.parametrize("case", ["plummer_1e5", "plummer_kroupa_1e6"])
+def test_gpu_step_vs_oracle(gpu, O, case):
+    """One Henon step from the same state (config 1 is Plummer 1e5, seed 1, one
+    step): the device potential differs from the sequential recurrence by ~1e-13,
+    so a few acceptance branches flip; the rest agree within 1e-10."""
+    cm = gpu
+    say("whole step", case)
+    cl = cm.make_plummer(100000, seed=1) if case == "plummer_1e5" else cm.make_plummer(1000000, seed=1, kroupa=True)
+    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_LONG_DOUBLE)
+    o.init()
+    with ctx_for(cm, cl) as ctx:
+        ctx.upload(cl)
+        ctx.init_cluster()
+        st = ctx.step()
+        d = ctx.download()
+        sc = ctx.scalars()
+    assert o.step() == 0
+    say(case, "gpu stats", st)
+    assert sc["n_max"] == o.N
+    assert sc["Dt"] == pytest.approx(o.sc("Dt"), rel=1e-12)
+    N = o.N
+    gi = {int(i): k for k, i in enumerate(d["id"][1:N + 1])}
+    idx = np.array([gi[int(i)] for i in o["id"][1:N + 1]]) + 1
+    bad = np.zeros(N, bool)
+    for k in ("r", "vr", "vt"):
+        bad |= rel(d[k][idx], o[k][1:N + 1]) > 1e-10
+    flips = int(bad.sum())
+    say(case, "stars outside 1e-10:", flips, "fraction", flips / N)
+    assert flips / N < 2e-3
+    assert sc["Etot"] == pytest.approx(o.sc("Etot"), rel=1e-8)
+
+
+def test_gpu_energy_drift_100_steps(gpu, O):
+    cm = gpu
+    say("energy drift, 100 steps, Plummer 2e4")
+    cl = cm.make_plummer(20000, seed=1)
+    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_LONG_DOUBLE)
+    o.init()
+    e0 = o.sc("Etot")
+    with ctx_for(cm, cl) as ctx:
+        ctx.upload(cl)
+        ctx.init_cluster()
+        g0 = ctx.scalars()["Etot"]
+        for _ in range(100):
+            ctx.step()
+        sc = ctx.scalars()
+    for _ in range(100):
+        assert o.step() == 0
+    drift_gpu = (sc["Etot"] + sc["Eoops"] - g0) / abs(g0)
+    drift_ref = (o.sc("Etot") + o.sc("Eoops") - e0) / abs(e0)
+    say("drift gpu", drift_gpu, "oracle", drift_ref, "N", sc["n_max"], o.N)
+    assert abs(drift_gpu - drift_ref) < 1e-6
+
+
+# ------------------------------------------------------------------ edge cases
+def test_gpu_edge_cases(gpu, O):
+    cm = gpu
+    say("edge cases: odd N, unbound, destroyed, small N, capacity")
+    cl = cm.make_plummer(2001, seed=9)
+    o0 = oracle_for(O, cl)
+    o0.init()
+    # make some stars unbound and one massless
+    cl2 = cl.copy()
+    cl2["vr"][10:15] *= 20.0
+    cl2["m"][30] = 1e-30
+    o = oracle_for(O, cl2, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64)
+    o.init()
+    with ctx_for(cm, cl2) as ctx:
+        upload_state(ctx, cl2, o)
+        st = ctx.orbits(step=0)
+        d = ctx.download()
+    o.orbits()
+    N = o.N
+    assert st["n_removed_E"] == o.sc("n_removed_E") >= 5
+    assert st["n_destroyed"] == 1
+    assert np.array_equal(d["rnew"][1:N + 1], o["rnew"][1:N + 1])
+    # capacity and argument errors are reported, not raised inside the library
+    with pytest.raises(cm.CmcgError) as ei:
+        with ctx_for(cm, cl, cap=100) as ctx:
+            ctx.upload(cl)
+    assert ei.value.code == cm.E_CAPACITY
+
+
+def test_gpu_host_shim_reference_names(gpu, O):
+    """The reference-named C entry points drive the same kernels."""
+    cm = gpu
+    say("host shim")
+    cl = cm.make_plummer(30000, seed=4)
+    h = cm.hostlib
+    p = cm.default_params(cl.n, orbit_r=cl.rtid, seed=1)
+    assert h.cmc_hotpath_init(C.byref(p), cl.n, C.byref(cl.soa()), 0) == 0
+    h.calc_potential_new.restype = None
+    h.calc_potential_new()
+    h.potential.restype = C.c_double
+    h.potential.argtypes = [C.c_double]
+    o = oracle_for(O, cl)
+    o.potential_calculate()
+    for x in [0.0, 1e-3, 0.5, 1.0, 3.0]:
+        assert h.potential(x) == pytest.approx(o.potential(x), rel=1e-12)
+    h.cmc_hotpath_finalize()

@@ -313,7 +313,10 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)sc.max_events)) ||
-        (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
+        (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.chain_x, (size_t)n_cap + 8)) ||
+        (e = dalloc(c, &sc.chain_g, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.chain_mk, (size_t)n_cap + 8)) ||
+        (e = dalloc(c, &sc.defer_list, 65536)) || (e = dalloc(c, &sc.defer_kfirst, 65536)) ||
+        (e = dalloc(c, &sc.defer_partials, 65536 * 8)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
         (e = dalloc(c, &c->T, 1))) {
         cmcg_destroy(c);
         return e;

@@ -36,6 +36,7 @@ struct DevScalars {
     int64_t ntie_runs;
     int64_t n_excess;            // set_velocities3 stars with vnew2 <= 0
     int64_t first_inf;           // potential_calculate: first slot with r >= SF_INFINITY
+    int64_t n_defer;             // orbit phase: stars whose own-index Q <= 0 (linear-scan path)
     int64_t sort_n;              // sort: live stars after dropping the r == SF_INFINITY tail
     // orbit statistics
     int64_t n_circular, n_removed_E, n_removed_rt, n_destroyed, n_brent, n_linscan;
@@ -293,28 +294,15 @@ __device__ __noinline__ double find_root_vr(const double *__restrict__ r, const 
 
 struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; int brent; int linscan; int err; };
 
-// cmc_orbit.c:30-59 get_positive_Q_index + :413-540 calc_orbit_new for star slot j.
-// kmin_hint/kmax_hint: optional narrowed brackets (learned path); when they bracket
-// the sign change the result equals the full-bracket search (monotone Q sides).
-__device__ __forceinline__ OrbitRS calc_orbit(const double *__restrict__ r, const double *__restrict__ phi,
-                                              int64_t n_max, int64_t j, double rj, double mj,
-                                              const KParams &P, double E, double J)
+// cmc_orbit.c:413-540 calc_orbit_new after get_positive_Q_index has produced ktemp
+// (an index with Q > 0; ktemp < 0 means "no positive Q": circular orbit).
+__device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r, const double *__restrict__ phi,
+                                                   int64_t n_max, int64_t j, double rj, double mj,
+                                                   const KParams &P, double E, double J, int64_t ktemp)
 {
     OrbitRS o;
     o.brent = 0; o.linscan = 0; o.err = 0;
-    int64_t ktemp = j;
-    double Qtemp = function_q(rj, mj, P.inv_nstar, r[ktemp], phi[ktemp], E, J);
-    bool circ = false;
-    if (Qtemp <= 0.0) {
-        o.linscan = 1;
-        ktemp = -1;
-        do {
-            ktemp++;
-            Qtemp = function_q(rj, mj, P.inv_nstar, r[ktemp], phi[ktemp], E, J);
-        } while (Qtemp <= 0.0 && ktemp <= n_max);
-        if (ktemp >= n_max) circ = true;
-    }
-    if (circ) {
+    if (ktemp < 0) {
         o.rp = o.ra = rj; o.kmin = o.kmax = j; o.dQdrp = o.dQdra = 0.0; o.circular = 1;
         return o;
     }
@@ -358,6 +346,29 @@ __device__ __forceinline__ OrbitRS calc_orbit(const double *__restrict__ r, cons
         o.rp = rmin; o.ra = rmax; o.kmin = kmin; o.kmax = kmax;
         o.dQdrp = dQdr_min; o.dQdra = dQdr_max; o.circular = 0;
     }
+    return o;
+}
+
+// get_positive_Q_index (cmc_orbit.c:30-59) with its linear scan, then calc_orbit_from.
+// Used by the small batch entry points; the orbit phase defers the scan instead.
+__device__ __forceinline__ OrbitRS calc_orbit(const double *__restrict__ r, const double *__restrict__ phi,
+                                              int64_t n_max, int64_t j, double rj, double mj,
+                                              const KParams &P, double E, double J)
+{
+    int64_t ktemp = j;
+    double Qtemp = function_q(rj, mj, P.inv_nstar, r[ktemp], phi[ktemp], E, J);
+    int lin = 0;
+    if (Qtemp <= 0.0) {
+        lin = 1;
+        ktemp = -1;
+        do {
+            ktemp++;
+            Qtemp = function_q(rj, mj, P.inv_nstar, r[ktemp], phi[ktemp], E, J);
+        } while (Qtemp <= 0.0 && ktemp <= n_max);
+        if (ktemp >= n_max) ktemp = -1;            // cmc_orbit.c:52-56
+    }
+    OrbitRS o = calc_orbit_from(r, phi, n_max, j, rj, mj, P, E, J, ktemp);
+    o.linscan = lin;
     return o;
 }
 

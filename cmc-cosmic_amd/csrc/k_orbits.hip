@@ -7,6 +7,11 @@
 // same as the reference's full-range FindZero_r, see potential_at).  Each star draws
 // from its own taus113 substream (offset step*2^72 + 2^70 + slot*2^20).
 //
+// get_positive_Q_index (cmc_orbit.c:30-59) scans linearly from the centre when Q at
+// the star's own index is not positive (v_r rounded to ~0).  That O(N) walk by one
+// thread would stall the whole phase, so such stars are deferred: a grid-wide
+// search finds the same first positive index, and a tail kernel finishes them.
+//
 // The phase is bound by dependent gathers into the sorted (r, phi) table (16 B per
 // probe, ~(2 + tries) * log2 N probes per star), which at N = 4e6 is 64 MB and lives
 // in L2/MALL; the compulsory HBM traffic is 96 B/star.
@@ -14,6 +19,7 @@
 
 namespace {
 constexpr int ORB_THREADS = 256;
+constexpr int64_t MAX_DEFER = 65536;
 
 // Deterministic block reduction of NV doubles (fixed tree order) into partials.
 template <int NV, bool MAXFIRST>
@@ -36,19 +42,101 @@ __device__ void block_reduce_store(double (&v)[NV], double *__restrict__ out)
         for (int i = 0; i < NV; i++) out[i] = lds[i][0];
 }
 
+// remove_star (cmc_evolution_thr.c:655-716) contributions: Eescaped, Jescaped,
+// TidalMassLoss, Etidal
+__device__ __forceinline__ void removed(double (&red)[5], double Ej, double J, double mj, const KParams &P)
+{
+    red[1] = Ej * mj / (double)P.n_star;
+    red[2] = J * mj / (double)P.n_star;
+    red[3] = mj / (double)P.n_star;
+    red[4] = Ej * mj / (double)P.n_star;
+}
+
+// The part of get_positions_loop after the positive-Q index is known
+// (cmc_evolution_thr.c:832-934).
+__device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *__restrict__ T,
+                            const Rng &base, int jb1, int64_t n, int64_t j, double rj, double mj, double E,
+                            double Ej, double J, int64_t ktemp, int &flags, int &ntry, double (&red)[5])
+{
+    const double *__restrict__ r = f.r;
+    const double *__restrict__ phi = f.phi;
+    OrbitRS o = calc_orbit_from(r, phi, n, j, rj, mj, P, E, J, ktemp);
+    if (o.err) dev_set_error(s, o.err, j);
+    if (o.brent) { flags |= CMCG_SF_BRENT; atomicAdd((unsigned long long *)&s->n_brent, 1ull); }
+    if (o.circular) {
+        f.rnew[j] = rj; f.vrnew[j] = f.vr[j]; f.vtnew[j] = f.vt[j];
+        flags |= CMCG_SF_CIRCULAR;
+        atomicAdd((unsigned long long *)&s->n_circular, 1ull);
+        return;
+    }
+    if (o.ra >= s->Rtidal) {
+        f.r_apo[j] = o.ra; f.r_peri[j] = o.rp;
+        f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
+        removed(red, Ej, J, mj, P);
+        flags |= CMCG_SF_REMOVED_RT;
+        atomicAdd((unsigned long long *)&s->n_removed_rt, 1ull);
+        return;
+    }
+    const double rmin = o.rp, rmax = o.ra;
+    double g1 = sqrt(3.0 * (rmax - rmin) / o.dQdrp);
+    double g2 = sqrt(-3.0 * (rmax - rmin) / o.dQdra);
+    double F = 1.2 * (g1 >= g2 ? g1 : g2);
+    Rng rng = rng_jump(T, base, (uint64_t)j << 20, 20, jb1);
+    // narrowed bracket for potential(r): r in [rmin, rmax] within [r_kmin, r_kmax+1]
+    int64_t lo0 = o.kmin < 1 ? 1 : o.kmin;
+    int64_t hi0 = o.kmax + 1 > n + 1 ? n + 1 : o.kmax + 1;
+    double X = 0.0, rr = 0.0, vr = 0.0, potr = 0.0;
+    int k;
+    bool bad = false;
+    for (k = 1; k <= CMCG_N_TRY; k++) {
+        X = rng_dbl(rng);
+        double s0 = 2.0 * X - 1.0;
+        double g0 = F * rng_dbl(rng);
+        rr = 0.5 * (rmin + rmax) + 0.25 * (rmax - rmin) * (3.0 * s0 - s0 * s0 * s0);
+        int64_t lo = lo0, hi = hi0;
+        if (!(r[lo] < rr)) lo = 1;
+        if (!(rr <= r[hi])) hi = n + 1;
+        potr = potential_at(r, phi, n, rr, lo, hi, &bad);
+        double pot = potr + phi_s_own(rj, mj, P.inv_nstar, rr);
+        double drds = 0.25 * (rmax - rmin) * (3.0 - 3.0 * s0 * s0);
+        double Q = 2.0 * E - 2.0 * pot - J * J / rr / rr;
+        if (Q >= 0.0) {
+            vr = sqrt(Q);
+        } else {
+            if (isnan(Q)) { dev_set_error(s, CMCG_E_NAN, j); break; }
+            vr = 0;
+        }
+        if (g0 < 1.0 / vr * drds) break;
+    }
+    if (bad) dev_set_error(s, CMCG_E_BRACKET, j);
+    if (k == CMCG_N_TRY + 1) dev_set_error(s, CMCG_E_NTRY, j);
+    ntry = k;
+    f.X[j] = X;
+    f.r_peri[j] = rmin;
+    f.r_apo[j] = rmax;
+    if (rng_dbl(rng) < 0.5) vr = -vr;
+    f.rnew[j] = rr;
+    f.vrnew[j] = vr;
+    f.vtnew[j] = J / rr;
+    f.pot_rnew[j] = potr;
+    flags |= 256;      // sampled: pot_rnew holds potential(rnew)
+    red[0] = rr;
+    atomicAdd((unsigned long long *)&s->tries_total, (unsigned long long)k);
+    atomicMax((unsigned long long *)&s->tries_max, (unsigned long long)k);
+}
+
 __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
                                                              const RngTable *__restrict__ T,
                                                              const Rng *__restrict__ base, int jb1,
-                                                             double *__restrict__ partials)
+                                                             double *__restrict__ partials,
+                                                             int64_t *__restrict__ defer_list)
 {
     const int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max;
     // [0] max_r, [1] Eescaped, [2] Jescaped, [3] TidalMassLoss, [4] Etidal
     double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (j <= n) {
-        const double *__restrict__ r = f.r;
-        const double *__restrict__ phi = f.phi;
-        const double rj = r[j], mj = f.m[j];
+        const double rj = f.r[j], mj = f.m[j];
         const double Ej = f.E[j], J = f.J[j];
         const double E = Ej + phi_s_own(rj, mj, P.inv_nstar, rj);    // cmc_evolution_thr.c:806
         int flags = 0, ntry = 0;
@@ -61,79 +149,21 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
             flags = CMCG_SF_DESTROYED;
             atomicAdd((unsigned long long *)&s->n_destroyed, 1ull);
         } else if (E >= 0.0) {
-            // remove_star (cmc_evolution_thr.c:655-716)
             f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
-            red[1] = Ej * mj / (double)P.n_star;
-            red[2] = J * mj / (double)P.n_star;
-            red[3] = mj / (double)P.n_star;
-            red[4] = Ej * mj / (double)P.n_star;
+            removed(red, Ej, J, mj, P);
             flags = CMCG_SF_REMOVED_E;
             atomicAdd((unsigned long long *)&s->n_removed_E, 1ull);
         } else {
-            OrbitRS o = calc_orbit(r, phi, n, j, rj, mj, P, E, J);
-            if (o.err) dev_set_error(s, o.err, j);
-            if (o.brent) { flags |= CMCG_SF_BRENT; atomicAdd((unsigned long long *)&s->n_brent, 1ull); }
-            if (o.linscan) { flags |= CMCG_SF_LINSCAN; atomicAdd((unsigned long long *)&s->n_linscan, 1ull); }
-            if (o.circular) {
-                f.rnew[j] = rj; f.vrnew[j] = f.vr[j]; f.vtnew[j] = f.vt[j];
-                flags |= CMCG_SF_CIRCULAR;
-                atomicAdd((unsigned long long *)&s->n_circular, 1ull);
-            } else if (o.ra >= s->Rtidal) {
-                f.r_apo[j] = o.ra; f.r_peri[j] = o.rp;
-                f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
-                red[1] = Ej * mj / (double)P.n_star;
-                red[2] = J * mj / (double)P.n_star;
-                red[3] = mj / (double)P.n_star;
-                red[4] = Ej * mj / (double)P.n_star;
-                flags |= CMCG_SF_REMOVED_RT;
-                atomicAdd((unsigned long long *)&s->n_removed_rt, 1ull);
+            double Qown = function_q(rj, mj, P.inv_nstar, rj, f.phi[j], E, J);
+            if (Qown <= 0.0) {
+                // linear-scan case: finished by the deferred path
+                flags = CMCG_SF_LINSCAN;
+                unsigned long long slot = atomicAdd((unsigned long long *)&s->n_defer, 1ull);
+                if ((int64_t)slot < MAX_DEFER) defer_list[slot] = j;
+                else dev_set_error(s, CMCG_E_STATE, j);
+                atomicAdd((unsigned long long *)&s->n_linscan, 1ull);
             } else {
-                const double rmin = o.rp, rmax = o.ra;
-                double g1 = sqrt(3.0 * (rmax - rmin) / o.dQdrp);
-                double g2 = sqrt(-3.0 * (rmax - rmin) / o.dQdra);
-                double F = 1.2 * (g1 >= g2 ? g1 : g2);
-                Rng rng = rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1);
-                // narrowed bracket for potential(r): r in [rmin, rmax] within [r_kmin, r_kmax+1]
-                int64_t lo0 = o.kmin < 1 ? 1 : o.kmin;
-                int64_t hi0 = o.kmax + 1 > n + 1 ? n + 1 : o.kmax + 1;
-                double X = 0.0, rr = 0.0, vr = 0.0, potr = 0.0;
-                int k;
-                bool bad = false;
-                for (k = 1; k <= CMCG_N_TRY; k++) {
-                    X = rng_dbl(rng);
-                    double s0 = 2.0 * X - 1.0;
-                    double g0 = F * rng_dbl(rng);
-                    rr = 0.5 * (rmin + rmax) + 0.25 * (rmax - rmin) * (3.0 * s0 - s0 * s0 * s0);
-                    int64_t lo = lo0, hi = hi0;
-                    if (!(r[lo] < rr)) lo = 1;
-                    if (!(rr <= r[hi])) hi = n + 1;
-                    potr = potential_at(r, phi, n, rr, lo, hi, &bad);
-                    double pot = potr + phi_s_own(rj, mj, P.inv_nstar, rr);
-                    double drds = 0.25 * (rmax - rmin) * (3.0 - 3.0 * s0 * s0);
-                    double Q = 2.0 * E - 2.0 * pot - J * J / rr / rr;
-                    if (Q >= 0.0) {
-                        vr = sqrt(Q);
-                    } else {
-                        if (isnan(Q)) { dev_set_error(s, CMCG_E_NAN, j); break; }
-                        vr = 0;
-                    }
-                    if (g0 < 1.0 / vr * drds) break;
-                }
-                if (bad) dev_set_error(s, CMCG_E_BRACKET, j);
-                if (k == CMCG_N_TRY + 1) dev_set_error(s, CMCG_E_NTRY, j);
-                ntry = k;
-                f.X[j] = X;
-                f.r_peri[j] = rmin;
-                f.r_apo[j] = rmax;
-                if (rng_dbl(rng) < 0.5) vr = -vr;
-                f.rnew[j] = rr;
-                f.vrnew[j] = vr;
-                f.vtnew[j] = J / rr;
-                f.pot_rnew[j] = potr;
-                flags |= 256;      // sampled: pot_rnew holds potential(rnew)
-                red[0] = rr;
-                atomicAdd((unsigned long long *)&s->tries_total, (unsigned long long)k);
-                atomicMax((unsigned long long *)&s->tries_max, (unsigned long long)k);
+                sample_star(f, s, P, T, base[1], jb1, n, j, rj, mj, E, Ej, J, j, flags, ntry, red);
             }
         }
         f.flags[j] = flags;
@@ -142,16 +172,82 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
     block_reduce_store<5, true>(red, partials + 8 * blockIdx.x);
 }
 
+// Deferred stars: order the list (for a deterministic reduction order) and reset
+// the first-positive-index slots.
+__global__ void defer_prepare_kernel(DevScalars *s, int64_t *__restrict__ list, int64_t *__restrict__ kfirst)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
+    // (more than 4096 deferred stars: keep append order; only the order of the
+    // escaped-energy sums depends on it)
+    for (int64_t a = 1; a < (nd <= 4096 ? nd : 0); a++) {
+        int64_t v = list[a], b = a - 1;
+        while (b >= 0 && list[b] > v) { list[b + 1] = list[b]; b--; }
+        list[b + 1] = v;
+    }
+    for (int64_t d = 0; d < nd; d++) kfirst[d] = s->n_max + 2;
+}
+
+// Grid-wide equivalent of the linear scan: the first k in [0, N_MAX+1] with Q > 0.
+__global__ __launch_bounds__(256) void defer_search_kernel(DevFields f, DevScalars *s, KParams P,
+                                                           const int64_t *__restrict__ list,
+                                                           int64_t *__restrict__ kfirst)
+{
+    const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
+    if (nd == 0) return;
+    const int64_t n = s->n_max;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t d = blockIdx.y; d < nd; d += gridDim.y) {
+        const int64_t j = list[d];
+        const double rj = f.r[j], mj = f.m[j];
+        const double E = f.E[j] + phi_s_own(rj, mj, P.inv_nstar, rj);
+        const double J = f.J[j];
+        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n + 1; k += stride) {
+            if (k > (int64_t)kfirst[d]) break;
+            if (function_q(rj, mj, P.inv_nstar, f.r[k], f.phi[k], E, J) > 0.0)
+                atomicMin((unsigned long long *)&kfirst[d], (unsigned long long)k);
+        }
+    }
+}
+
+__global__ __launch_bounds__(ORB_THREADS) void defer_tail_kernel(DevFields f, DevScalars *s, KParams P,
+                                                                 const RngTable *__restrict__ T,
+                                                                 const Rng *__restrict__ base, int jb1,
+                                                                 const int64_t *__restrict__ list,
+                                                                 const int64_t *__restrict__ kfirst,
+                                                                 double *__restrict__ dpartials)
+{
+    const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
+    const int64_t n = s->n_max;
+    for (int64_t d = threadIdx.x; d < nd; d += blockDim.x) {
+        const int64_t j = list[d];
+        const double rj = f.r[j], mj = f.m[j];
+        const double Ej = f.E[j], J = f.J[j];
+        const double E = Ej + phi_s_own(rj, mj, P.inv_nstar, rj);
+        int64_t kt = kfirst[d];
+        if (kt >= n) kt = -1;                      // cmc_orbit.c:52-56: circular
+        double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        int flags = CMCG_SF_LINSCAN, ntry = 0;
+        sample_star(f, s, P, T, base[1], jb1, n, j, rj, mj, E, Ej, J, kt, flags, ntry, red);
+        f.flags[j] = flags;
+        f.ntry[j] = ntry;
+        for (int i = 0; i < 5; i++) dpartials[8 * d + i] = red[i];
+    }
+}
+
 __global__ void orbits_begin_kernel(DevScalars *s)
 {
     // new_orbits_calculate (cmc_utils.c:2198) and counters of this call
     s->OldTidalMassLoss = s->TidalMassLoss_old;
     s->n_circular = s->n_removed_E = s->n_removed_rt = s->n_destroyed = s->n_brent = s->n_linscan = 0;
     s->tries_total = s->tries_max = s->n_bracket_hit = s->n_bracket_fallback = 0;
+    s->n_defer = 0;
 }
 
-// Sums the per-block partials in block order and finishes new_orbits_calculate.
-__global__ void orbits_end_kernel(DevScalars *s, const double *__restrict__ partials, int64_t nblocks)
+// Sums the per-block partials in block order, then the deferred stars' in star
+// order, and finishes new_orbits_calculate.
+__global__ void orbits_end_kernel(DevScalars *s, const double *__restrict__ partials, int64_t nblocks,
+                                  const double *__restrict__ dpartials)
 {
     __shared__ double lds[5][256];
     const int t = threadIdx.x;
@@ -171,25 +267,32 @@ __global__ void orbits_end_kernel(DevScalars *s, const double *__restrict__ part
         __syncthreads();
     }
     if (t == 0) {
-        s->max_r = lds[0][0];
-        s->Eescaped += lds[1][0];
-        s->Jescaped += lds[2][0];
-        s->TidalMassLoss += lds[3][0];
-        s->Etidal += lds[4][0];
+        double m = lds[0][0], e1 = lds[1][0], e2 = lds[2][0], e3 = lds[3][0], e4 = lds[4][0];
+        int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
+        for (int64_t d = 0; d < nd; d++) {
+            const double *p = dpartials + 8 * d;
+            m = m >= p[0] ? m : p[0];
+            e1 += p[1]; e2 += p[2]; e3 += p[3]; e4 += p[4];
+        }
+        s->max_r = m;
+        s->Eescaped += e1;
+        s->Jescaped += e2;
+        s->TidalMassLoss += e3;
+        s->Etidal += e4;
         s->TidalMassLoss += s->TidalMassLoss_old;    // cmc_utils.c:2210
     }
 }
 
 __global__ void apsides_kernel(DevFields f, DevScalars *s, KParams P, int64_t first, int64_t count,
                                const double *__restrict__ E, const double *__restrict__ J,
-                               double *__restrict__ out6, int64_t *__restrict__ kout, int32_t *__restrict__ circ)
+                               double *__restrict__ out4, int64_t *__restrict__ kout, int32_t *__restrict__ circ)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     int64_t j = first + i, n = s->n_max;
     OrbitRS o = calc_orbit(f.r, f.phi, n, j, f.r[j], f.m[j], P, E[i], J[i]);
     if (o.err) dev_set_error(s, o.err, j);
-    out6[4 * i + 0] = o.rp; out6[4 * i + 1] = o.ra; out6[4 * i + 2] = o.dQdrp; out6[4 * i + 3] = o.dQdra;
+    out4[4 * i + 0] = o.rp; out4[4 * i + 1] = o.ra; out4[4 * i + 2] = o.dQdrp; out4[4 * i + 3] = o.dQdra;
     kout[2 * i + 0] = o.kmin; kout[2 * i + 1] = o.kmax;
     circ[i] = o.circular;
 }
@@ -223,18 +326,24 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     int jb1 = 20 + bits_needed((uint64_t)n_ub);
     hipLaunchKernelGGL(orbits_begin_kernel, dim3(1), dim3(1), 0, st, s);
     hipLaunchKernelGGL(orbits_kernel, dim3((unsigned)nblocks), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base,
-                       jb1, sc.partials);
-    hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(256), 0, st, s, sc.partials, nblocks);
+                       jb1, sc.partials, sc.defer_list);
+    hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
+    hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
+                       sc.defer_kfirst);
+    hipLaunchKernelGGL(defer_tail_kernel, dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,
+                       (const int64_t *)sc.defer_list, (const int64_t *)sc.defer_kfirst, sc.defer_partials);
+    hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(256), 0, st, s, (const double *)sc.partials, nblocks,
+                       (const double *)sc.defer_partials);
     return hipGetLastError();
 }
 
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
-                          const double *E, const double *J, double *out6, int64_t *kout, int32_t *circ,
+                          const double *E, const double *J, double *out4, int64_t *kout, int32_t *circ,
                           hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(apsides_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
-                       count, E, J, out6, kout, circ);
+                       count, E, J, out4, kout, circ);
     return hipGetLastError();
 }
 

@@ -7,9 +7,14 @@
 #include "cmcg_internal.hpp"
 
 struct Scratch {
-    // potential scan: 6 doubles per tile
+    // potential chains: per-tile int128 sums and prefixes plus a boundary value
+    // (48 B per tile), chain increments / iterates / enclosed masses per star
     double *tile_sums;
     int64_t max_tiles;
+    double *chain_x, *chain_g, *chain_mk;
+    double *chain_scal;         // 4 doubles
+    int64_t *chain_len;         // 4
+    int64_t *chain_fbad;        // 6
     // deterministic block partials: 8 doubles per block
     double *partials;
     int64_t max_blocks;
@@ -25,6 +30,9 @@ struct Scratch {
     int64_t max_events;
     // tidal stripping loop flag
     int32_t *strip_done;
+    // orbit stars deferred to the linear-scan path (65536 slots)
+    int64_t *defer_list, *defer_kfirst;
+    double *defer_partials;
     // RNG bases of the current step: [0] relaxation, [1] orbits; seeded state
     Rng *rng_base;
     Rng *rng_seeded;

@@ -92,8 +92,12 @@ def test_gpu_potential_parity(gpu, O, case):
     assert mt == pytest.approx(o.sc("Mtotal"), rel=1e-15)
     assert rt == pytest.approx(o.sc("Rtidal"), rel=1e-14)
     err = rel(d["phi"][: n + 2], o["phi"][: n + 2])
-    say(case, "max rel phi err", err.max(), "median", np.median(err))
-    assert err.max() <= 1e-12
+    neq = int(np.sum(d["phi"][: n + 2] != o["phi"][: n + 2]))
+    say(case, "max rel phi err", err.max(), "unequal", neq)
+    # the device reproduces the reference's sequential rounding: bit-exact
+    # (the stated bar is 1e-12 relative)
+    assert neq == 0
+    assert mt == o.sc("Mtotal")
     assert d["r"][n + 1] == 1e10 and d["phi"][n + 1] == 0.0
 
 
@@ -109,7 +113,7 @@ def test_gpu_potential_tail_and_count(gpu, O):
         ng, mt, rt = ctx.potential()
         d = ctx.download(n_hint=cl.n)
     assert ng == n == 48999
-    assert rel(d["phi"][: n + 2], o["phi"][: n + 2]).max() <= 1e-12
+    assert np.array_equal(d["phi"][: n + 2], o["phi"][: n + 2])
 
 
 # ------------------------------------------------------------------ sort
@@ -374,6 +378,7 @@ def test_gpu_edge_cases(gpu, O):
     cl2 = cl.copy()
     cl2["vr"][10:15] *= 20.0
     cl2["m"][30] = 1e-30
+    cl2["vr"][[40, 700, 1500]] = 0.0          # own-index Q <= 0: the linear-scan path
     o = oracle_for(O, cl2, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64)
     o.init()
     with ctx_for(cm, cl2) as ctx:
@@ -384,7 +389,10 @@ def test_gpu_edge_cases(gpu, O):
     N = o.N
     assert st["n_removed_E"] == o.sc("n_removed_E") >= 5
     assert st["n_destroyed"] == 1
-    assert np.array_equal(d["rnew"][1:N + 1], o["rnew"][1:N + 1])
+    say("linear-scan stars gpu", st["n_linscan"], "oracle", o.sc("n_linscan"))
+    assert st["n_linscan"] == o.sc("n_linscan")
+    for k in ("rnew", "vrnew", "vtnew", "r_apo", "r_peri"):
+        assert np.array_equal(d[k][1:N + 1], o[k][1:N + 1]), k
     # capacity and argument errors are reported, not raised inside the library
     with pytest.raises(cm.CmcgError) as ei:
         with ctx_for(cm, cl, cap=100) as ctx:

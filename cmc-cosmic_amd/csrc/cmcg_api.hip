@@ -305,18 +305,22 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     if ((e = dalloc(c, &c->s, 1))) { cmcg_destroy(c); return e; }
     Scratch &sc = c->sc;
     sc.max_tiles = (n_cap + 2047) / 2048 + 1;
+    sc.esa_max = 1 << 16;
     sc.max_blocks = (n_cap + 255) / 256 + 2;
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
-    sc.max_events = 4096;
-    if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 6)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 8)) ||
+    if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 8)) ||
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
-        (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)sc.max_events)) ||
+        (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
+        (e = dalloc(c, &sc.sv3_wend, (size_t)n_cap + 2)) || (e = dalloc(c, &sc.sv3_merged, (size_t)n_cap + 2)) ||
+        (e = dalloc(c, &sc.sv3_blk, (size_t)sc.max_blocks)) || (e = dalloc(c, &sc.sv3_fallback, 1)) ||
         (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.chain_x, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.chain_g, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.chain_mk, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.defer_list, 65536)) || (e = dalloc(c, &sc.defer_kfirst, 65536)) ||
-        (e = dalloc(c, &sc.defer_partials, 65536 * 8)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
+        (e = dalloc(c, &sc.defer_partials, 65536 * 8)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
+        (e = dalloc(c, &sc.esa_idx, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_racc, (size_t)sc.esa_max * 2)) ||
+        (e = dalloc(c, &sc.esa_v, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_n, 4)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
         (e = dalloc(c, &c->T, 1))) {
         cmcg_destroy(c);
         return e;

@@ -174,91 +174,185 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
     f.vt[j] = f.vtnew[j];
 }
 
-// set_velocities3 per-star part.  Stars with vnew2 <= 0 ("events") feed the
-// sequential excess-energy chain; they are listed for the chain kernel.
-__global__ void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
-                           int64_t *__restrict__ events, int64_t max_events)
+// set_velocities3 per-star part.  Stars with vnew2 <= 0 ("events") carry excess
+// energy that the following stars absorb (cmc_utils.c:268-310); they are counted
+// per block for an ordered compaction.
+__global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
+                                                 int64_t *__restrict__ blk_cnt)
 {
+    __shared__ int cnt;
     const double q = 0.5;
     int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
-    if (i > n) return;
-    double m = f.m[i], ri = f.r[i];
-    double rO = f.rOld[i];
-    bool bad = false;
-    double Unewrold = potential_at(f.r, f.phi, n, rO, 1, n + 1, &bad) + phi_s_own(ri, m, P.inv_nstar, rO);
-    if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
-    double Unewrnew = f.phi[i] + phi_s_own(ri, m, P.inv_nstar, ri);
-    double vtold = f.vtold[i], vrold = f.vrold[i];
-    double vold2 = vtold * vtold + vrold * vrold;
-    double vnew2 = vold2 + 2.0 * (1.0 - q) * (f.Uoldrold[i] - f.Uoldrnew[i]) + 2.0 * q * (Unewrold - Unewrnew);
-    double vr = f.vr[i], vt = f.vt[i];
-    if (vnew2 <= 0.0) {
-        excess_c[i] = 0.5 * (sqr(vr) + sqr(vt) - vnew2) * m;
-        unsigned long long slot = atomicAdd((unsigned long long *)&s->n_excess, 1ull);
-        if ((int64_t)slot < max_events) events[slot] = i;
-    } else {
-        excess_c[i] = -1.0;
-        double alpha = sqrt(vnew2 / (sqr(vr) + sqr(vt)));
-        vr *= alpha;
-        vt *= alpha;
-        f.vr[i] = vr; f.vt[i] = vt;
-        f.vrnew[i] = vr; f.vtnew[i] = vt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    if (i <= n) {
+        double m = f.m[i], ri = f.r[i];
+        double rO = f.rOld[i];
+        bool bad = false;
+        double Unewrold = potential_at(f.r, f.phi, n, rO, 1, n + 1, &bad) + phi_s_own(ri, m, P.inv_nstar, rO);
+        if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
+        double Unewrnew = f.phi[i] + phi_s_own(ri, m, P.inv_nstar, ri);
+        double vtold = f.vtold[i], vrold = f.vrold[i];
+        double vold2 = vtold * vtold + vrold * vrold;
+        double vnew2 = vold2 + 2.0 * (1.0 - q) * (f.Uoldrold[i] - f.Uoldrnew[i]) + 2.0 * q * (Unewrold - Unewrnew);
+        double vr = f.vr[i], vt = f.vt[i];
+        if (vnew2 <= 0.0) {
+            excess_c[i] = 0.5 * (sqr(vr) + sqr(vt) - vnew2) * m;
+            atomicAdd(&cnt, 1);
+        } else {
+            excess_c[i] = -1.0;
+            double alpha = sqrt(vnew2 / (sqr(vr) + sqr(vt)));
+            vr *= alpha;
+            vt *= alpha;
+            f.vr[i] = vr; f.vt[i] = vt;
+            f.vrnew[i] = vr; f.vtnew[i] = vt;      // post-scaling values, read by the chain
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) blk_cnt[blockIdx.x] = cnt;
+}
+
+__global__ __launch_bounds__(1024) void sv3_scan_kernel(DevScalars *s, int64_t *__restrict__ blk_cnt, int64_t nb)
+{
+    __shared__ int64_t lds[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (nb + 1023) / 1024;
+    const int64_t b0 = (int64_t)t * chunk;
+    int64_t acc = 0;
+    for (int64_t b = b0; b < b0 + chunk && b < nb; b++) acc += blk_cnt[b];
+    lds[t] = acc;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = t >= off ? lds[t - off] : 0;
+        __syncthreads();
+        lds[t] += v;
+        __syncthreads();
+    }
+    int64_t run = lds[t] - acc;
+    if (t == 1023) s->n_excess = lds[1023];
+    for (int64_t b = b0; b < b0 + chunk && b < nb; b++) {
+        int64_t c = blk_cnt[b];
+        blk_cnt[b] = run;
+        run += c;
     }
 }
 
-__global__ void sv3_begin_kernel(DevScalars *s) { s->n_excess = 0; }
-
-// The excess chain of set_velocities3 (cmc_utils.c:268-310) in exact sequential
-// order: one thread walks from each event (sorted) while excess remains.  Events
-// are rare; with none this kernel returns immediately.
-__global__ void sv3_chain_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
-                                 int64_t *__restrict__ events, int64_t max_events)
+// ordered list of the event stars
+__global__ __launch_bounds__(GT) void sv3_compact_kernel(DevScalars *s, const double *__restrict__ excess_c,
+                                                         const int64_t *__restrict__ blk_off,
+                                                         int64_t *__restrict__ events)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t ne = s->n_excess;
-    if (ne == 0) return;
+    __shared__ int wcnt[GT / 64];
+    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool ev = i <= s->n_max && excess_c[i] >= 0.0;
+    uint64_t bal = __ballot(ev);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    int64_t pos = blk_off[blockIdx.x];
+    for (int w2 = 0; w2 < w; w2++) pos += wcnt[w2];
+    pos += __popcll(bal & ((1ull << lane) - 1ull));
+    if (ev) events[pos] = i;
+}
+
+constexpr int64_t SPEC_WALK_MAX = 4096;
+
+// One episode of the excess chain starting at event star `i` with no excess
+// carried into it: event stars add their excess, the other stars absorb
+// min(0.8 KE, excess) each (cmc_utils.c:268-310).  Returns the star at which the
+// excess reaches 0, n + 1 when excess is left at the end (dropped at one rank,
+// cmc_utils.c:313-353), or -1 after `max_steps` stars.
+__device__ int64_t excess_walk(const DevFields &f, const double *__restrict__ excess_c, int64_t i, int64_t n,
+                               bool write, int64_t max_steps)
+{
     const double E_dump_factor = 0.8;
-    int64_t n = s->n_max;
-    int64_t start;
-    bool listed = ne <= max_events;
-    if (listed) {
-        for (int64_t a = 1; a < ne; a++) {          // insertion sort of the event slots
-            int64_t v = events[a], b = a - 1;
-            while (b >= 0 && events[b] > v) { events[b + 1] = events[b]; b--; }
-            events[b + 1] = v;
-        }
-        start = events[0];
-    } else {
-        start = 1;
-    }
     double Eexcess = 0.0;
-    int64_t ev = 0;
-    int64_t i = start;
-    while (i <= n) {
+    for (int64_t steps = 0; i <= n; i++, steps++) {
+        if (steps >= max_steps) return -1;
         double c = excess_c[i];
         if (c >= 0.0) {
             Eexcess += c;
-            if (listed) ev++;
         } else if (Eexcess > 0) {
-            double m = f.m[i], vr = f.vr[i], vt = f.vt[i];
+            double m = f.m[i], vr = f.vrnew[i], vt = f.vtnew[i];
             double v2 = sqr(vr) + sqr(vt);
             double E_dump_capacity = E_dump_factor * 0.5 * m * v2;
             double E_dump = E_dump_capacity <= Eexcess ? E_dump_capacity : Eexcess;
-            double exc_ratio = sqrt((v2 - 2 * E_dump / m) / v2);
-            f.vr[i] = vr * exc_ratio;
-            f.vt[i] = vt * exc_ratio;
+            if (write) {
+                double exc_ratio = sqrt((v2 - 2 * E_dump / m) / v2);
+                f.vr[i] = vr * exc_ratio;
+                f.vt[i] = vt * exc_ratio;
+            }
             Eexcess -= E_dump;
         }
-        if (listed && !(Eexcess > 0)) {
-            if (ev >= ne) break;
-            i = events[ev];         // skip to the next event
-            continue;
-        }
-        i++;
+        if (!(Eexcess > 0)) return i;
     }
-    // at one rank the leftover excess is dropped (cmc_utils.c:313-353, Eoops += 0)
+    return n + 1;
 }
+
+// Fresh (no carried excess) bounded walk of every event.  joins[d] = 1 when that
+// walk reaches event d + 1 or was cut off, so event d + 1 is presumed to lie inside
+// the same episode.  An episode starting at a true head is exactly its fresh walk.
+__global__ void sv3_spec_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
+                                const int64_t *__restrict__ events, int64_t *__restrict__ wend,
+                                int32_t *__restrict__ joins)
+{
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ne = s->n_excess;
+    if (d >= ne) return;
+    int64_t e = excess_walk(f, excess_c, events[d], s->n_max, false, SPEC_WALK_MAX);
+    wend[d] = e;
+    joins[d] = (e < 0) || (d + 1 < ne && e >= events[d + 1]);
+}
+
+// Presumed heads are d = 0 and every d with joins[d-1] = 0; the presumed group of
+// head d runs to the first q >= d with joins[q] = 0.  If every presumed head's walk
+// ends inside its group's span [events[q], events[q+1]), then by induction from
+// event 0 (which starts with no carried excess) each presumed head is a true head
+// and its walk is the true episode.  Otherwise the heads are relinked exactly.
+__global__ void sv3_verify_kernel(DevScalars *s, const int64_t *__restrict__ events,
+                                  const int32_t *__restrict__ joins, const int64_t *__restrict__ wend,
+                                  int32_t *__restrict__ relink)
+{
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ne = s->n_excess;
+    if (d >= ne || (d > 0 && joins[d - 1])) return;
+    int64_t q = d;
+    while (q + 1 < ne && joins[q]) q++;
+    int64_t e = wend[d];
+    if (e < 0 || e < events[q] || (q + 1 < ne && e >= events[q + 1])) *relink = 1;
+}
+
+// Exact head chain in event order: the next head is the first event after the end
+// of the current episode.  Runs only when the presumed heads were wrong.
+__global__ void sv3_relink_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
+                                  const int64_t *__restrict__ events, const int64_t *__restrict__ wend,
+                                  int32_t *__restrict__ joins, const int32_t *__restrict__ relink)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || !*relink) return;
+    int64_t n = s->n_max, ne = s->n_excess;
+    int64_t d = 0;
+    while (d < ne) {
+        int64_t end = wend[d];
+        if (end < 0) end = excess_walk(f, excess_c, events[d], n, false, INT64_MAX);
+        int64_t d2 = d + 1;
+        while (d2 < ne && events[d2] <= end) joins[d2 - 1] = 1, d2++;
+        if (d2 < ne) joins[d2 - 1] = 0;
+        d = d2;
+    }
+}
+
+// every head walks its episode and writes the dumped velocities
+__global__ void sv3_heads_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
+                                 const int64_t *__restrict__ events, const int32_t *__restrict__ joins)
+{
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ne = s->n_excess;
+    if (d >= ne || (d > 0 && joins[d - 1])) return;
+    excess_walk(f, excess_c, events[d], s->n_max, true, INT64_MAX);
+}
+
+__global__ void sv3_begin_kernel(DevScalars *s, int32_t *relink) { s->n_excess = 0; *relink = 0; }
 
 __global__ __launch_bounds__(GT) void energy_kernel(DevFields f, DevScalars *s, KParams P,
                                                     double *__restrict__ partials)
@@ -347,11 +441,23 @@ hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, h
 hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
                                   int64_t n_ub, hipStream_t st)
 {
-    hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s);
-    hipLaunchKernelGGL(sv3_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.excess_list,
-                       sc.max_events);
-    hipLaunchKernelGGL(sv3_chain_kernel, dim3(1), dim3(64), 0, st, f, s, (const double *)sc.excess_c,
-                       sc.excess_list, sc.max_events);
+    int64_t nb = grid_for(n_ub);
+    if (nb > sc.max_blocks) return hipErrorInvalidValue;
+    unsigned ge = grid_for(n_ub);
+    hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
+    hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk);
+    hipLaunchKernelGGL(sv3_scan_kernel, dim3(1), dim3(1024), 0, st, s, sc.sv3_blk, nb);
+    hipLaunchKernelGGL(sv3_compact_kernel, dim3((unsigned)nb), dim3(GT), 0, st, s, (const double *)sc.excess_c,
+                       (const int64_t *)sc.sv3_blk, sc.excess_list);
+    hipLaunchKernelGGL(sv3_spec_kernel, dim3(ge), dim3(GT), 0, st, f, s, (const double *)sc.excess_c,
+                       (const int64_t *)sc.excess_list, sc.sv3_wend, sc.sv3_merged);
+    hipLaunchKernelGGL(sv3_verify_kernel, dim3(ge), dim3(GT), 0, st, s, (const int64_t *)sc.excess_list,
+                       (const int32_t *)sc.sv3_merged, (const int64_t *)sc.sv3_wend, sc.sv3_fallback);
+    hipLaunchKernelGGL(sv3_relink_kernel, dim3(1), dim3(64), 0, st, f, s, (const double *)sc.excess_c,
+                       (const int64_t *)sc.excess_list, (const int64_t *)sc.sv3_wend, sc.sv3_merged,
+                       (const int32_t *)sc.sv3_fallback);
+    hipLaunchKernelGGL(sv3_heads_kernel, dim3(ge), dim3(GT), 0, st, f, s, (const double *)sc.excess_c,
+                       (const int64_t *)sc.excess_list, (const int32_t *)sc.sv3_merged);
     return hipGetLastError();
 }
 hipError_t launch_compute_energy(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,

@@ -5,19 +5,27 @@
 //   A  mprev += m_k                  k = 1..N          (total mass, :685-696)
 //   B  mprev -= m_k / N_STAR         k = N..2          (enclosed mass M_k, :723)
 //   C  phi_k = phi_{k+1} - M_k (1/r_k - 1/r_{k+1})     (:722)
-// Each chain is v_j = fl(v_{j-1} + s*x_j).  Repeated rounding makes the result
-// drift from the exact sums (6e-11 relative at N = 1e5), so a correctly rounded
-// parallel scan does not reproduce it.  Here each chain is solved as the fixed
-// point of
-//     v_j = v_{-1} + sum_{i<=j} [ fl(g_{i-1} + s*x_i) - g_{i-1} ],   g = previous iterate,
-// where every bracket is an exact integer in 2^-100 units (int128), so each
-// iteration is an exact parallel prefix sum.  An element's increment depends on
-// its predecessor only through the predecessor's binade (and last bit at binade
-// crossings and ties), so the iteration converges in a few rounds; every round
-// restarts at the first element that does not yet satisfy the recurrence
-// (everything before it is final), and a bounded sequential pass finishes any
-// remainder after the round limit.  The result equals the sequential chain
-// exactly: it is verified element by element against fl(v_{j-1} + s*x_j).
+// Each chain is v_j = fl(v_{j-1} + s*x_j) and is monotone.  Repeated rounding makes
+// the result drift from the exact sums (6e-11 relative at N = 1e5), so a correctly
+// rounded parallel scan does not reproduce it.  Exact reproduction uses:
+//
+// * If v_{j-1} and v_j lie in one binade with ulp u, then fl(v_{j-1} + s x_j) =
+//   v_{j-1} + RN_u(s x_j) unless s x_j / u is a tie: the increment does not depend
+//   on v_{j-1}.  Such "regular" elements are recognised from the exact prefix sum
+//   of the x (2^-K fixed point, int128) and a rigorous bound on the accumulated
+//   rounding error; the rest ("sensitive": binade crossings, ties, tiny values) are
+//   few, about one per binade.
+// * Segment and walk: an exact prefix sum of the regular increments, then one
+//   thread walks the sensitive elements in order, each getting
+//   fl(v_prev + s x) from the exact value before it (the run of regular elements
+//   in between stays in one binade, so adding the run's sum is exact), then every
+//   element is rebuilt in parallel from the last sensitive value before it.
+// * Every element is verified against fl(v_{j-1} + s x_j).  A violation (only when
+//   the error bound or the sensitive-list capacity is exceeded) is repaired by
+//   Jacobi rounds in which each bracket fl(g_{i-1} + s x_i) - g_{i-1} of the
+//   previous iterate g is an exact int128 and each round restarts at the first
+//   violation, and finally by a sequential pass.  The result equals the sequential
+//   chain exactly.
 #include "kernels.hpp"
 
 namespace {
@@ -25,6 +33,7 @@ constexpr int CT = 256;             // threads per tile
 constexpr int CI = 8;               // items per thread
 constexpr int CTILE = CT * CI;
 constexpr int SCAN_T = 1024;
+constexpr int REPAIR_ROUNDS = 3;      // Jacobi rounds after the segment-and-walk pass
 
 typedef __int128 i128;
 
@@ -90,29 +99,6 @@ __device__ i128 block_excl_scan(i128 v, i128 *lds, i128 *total)
     *total = lds[NT - 1];
     __syncthreads();
     return incl - v;
-}
-
-// Round 0: zero iterate (every predecessor 0 except v_{-1} = v0): its increments are
-// the exact-arithmetic partial sums.
-__global__ __launch_bounds__(CT) void chain_seed_kernel(ChainArgs a)
-{
-    __shared__ i128 lds[CT];
-    const int64_t L = *a.Lp;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { a.first_bad[0] = 0; a.first_bad[1] = 0; }
-    const int64_t base = (int64_t)blockIdx.x * CTILE;
-    if (base >= L) return;
-    const double v0 = *a.v0p;
-    i128 acc = 0;
-    for (int u = 0; u < CI; u++) {
-        int64_t j = base + (int64_t)threadIdx.x * CI + u;
-        if (j < L) {
-            acc += incr(j == 0 ? v0 : 0.0, a.x[j], a.sign, a.K);
-            a.g[j] = 0.0;
-        }
-    }
-    i128 tot;
-    block_excl_scan<CT>(acc, lds, &tot);
-    if (threadIdx.x == 0) a.tsum[blockIdx.x] = tot;
 }
 
 // One block: takes the restart index found by the previous round, snapshots the
@@ -237,6 +223,311 @@ __global__ void chain_finish_kernel(ChainArgs a)
     a.first_bad[0] = L;
 }
 
+// ---------------------------------------------------------------- segment and walk
+struct EsaArgs {
+    const double *x;
+    double *g;
+    double sign;
+    const double *v0p;
+    int64_t *first_bad;         // as in ChainArgs, initialised here
+    i128 *tsumP, *tpreP;        // per tile: fixed-point sums of s*x and exclusive prefixes
+    i128 *tsumR, *tpreR;        // per tile: regular-increment sums and prefixes
+    i128 *tcnt, *toff;          // per tile: sensitive counts and prefixes
+    i128 *tsum;                 // per tile: next Jacobi increments (for the repair rounds)
+    int64_t *s_idx;             // sensitive list: element index
+    i128 *s_racc;               //   regular-increment prefix before the element
+    double *s_v;                //   exact chain value at the element
+    int64_t *s_n;               // number of sensitive elements (uncapped)
+    int64_t smax;               // list capacity
+    const int64_t *Lp;
+    int K;
+};
+
+__device__ __forceinline__ int binade_of(double v)
+{
+    // exponent of a normal double, or INT_MIN for zero, subnormal, inf, nan
+    uint64_t b = (uint64_t)__double_as_longlong(v);
+    int e = (int)((b >> 52) & 0x7ff);
+    return (e == 0 || e == 0x7ff) ? INT32_MIN : e - 1023;
+}
+
+// binade of every value within e of the estimate v, or INT32_MIN if they differ
+__device__ __forceinline__ int interval_binade(double v, double e)
+{
+    double lo = v - e, hi = v + e;
+    if (!(lo > 0.0) && !(hi < 0.0)) return INT32_MIN;
+    int b1 = binade_of(lo), b2 = binade_of(hi);
+    return b1 == b2 ? b1 : INT32_MIN;
+}
+
+// Bound on |v - estimate| after n additions: n roundings of at most half an ulp of
+// values below 2 vmax (monotone chain), n truncations to 2^-K, and the conversion of
+// the fixed-point estimate to double.
+__device__ __forceinline__ double err_bound(int64_t n, double vmax, double a, int K)
+{
+    return ((double)n * (ldexp(vmax, -51) + ldexp(1.0, 1 - K)) + ldexp(a, -50)) * 1.001;
+}
+
+// regular increment of element j in 2^-K units, or sensitive
+__device__ __forceinline__ bool classify(int64_t j, double x, double sign, double v0, i128 Pp, i128 Pq, int K,
+                                         i128 *R)
+{
+    double av0 = fabs(v0);
+    double p = j == 0 ? v0 : unfxk(Pp, K);
+    double q = unfxk(Pq, K);
+    double ep = j == 0 ? 0.0 : err_bound(j, fmax(av0, fabs(p)), fabs(p), K);
+    double eq = err_bound(j + 1, fmax(av0, fabs(q)), fabs(q), K);
+    int bp = interval_binade(p, ep), bq = interval_binade(q, eq);
+    if (bp == INT32_MIN || bp != bq || bp < 52 - K || bp > 52 + 60 - K) return true;
+    double y = ldexp(sign * x, 52 - bp);         // exact: power-of-two scaling
+    double fy = floor(y);
+    if (y - fy == 0.5 || !(fabs(y) <= 9007199254740992.0)) return true;
+    double ry = rint(y);
+    *R = (i128)(int64_t)ry * ((i128)1 << (bp - 52 + K));
+    return false;
+}
+
+// Classification of one tile's items (CI consecutive per thread).  Needs tpreP.
+struct TileItems {
+    double xv[CI];
+    i128 R[CI];
+    uint32_t sens;      // bit u: item u sensitive (or past the end)
+};
+
+__device__ void tile_classify(const EsaArgs &a, int64_t tile, int64_t L, double v0, TileItems &it, i128 *lds)
+{
+    const int t = threadIdx.x;
+    const int64_t j0 = tile * CTILE + (int64_t)t * CI;
+    i128 fxs[CI];
+    i128 acc = 0;
+    for (int u = 0; u < CI; u++) {
+        int64_t j = j0 + u;
+        it.xv[u] = j < L ? a.x[j] : 0.0;
+        fxs[u] = j < L ? fxk(a.sign * it.xv[u], a.K) : (i128)0;
+        acc += fxs[u];
+    }
+    i128 tot;
+    i128 P = fxk(v0, a.K) + a.tpreP[tile] + block_excl_scan<CT>(acc, lds, &tot);
+    it.sens = 0;
+    for (int u = 0; u < CI; u++) {
+        int64_t j = j0 + u;
+        i128 Pq = P + fxs[u];
+        it.R[u] = 0;
+        if (j >= L || classify(j, it.xv[u], a.sign, v0, P, Pq, a.K, &it.R[u])) {
+            it.sens |= 1u << u;
+            it.R[u] = 0;
+        }
+        P = Pq;
+    }
+}
+
+__global__ __launch_bounds__(CT) void esa_sum_kernel(EsaArgs a)
+{
+    __shared__ i128 lds[CT];
+    const int64_t L = *a.Lp;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { a.first_bad[0] = 0; a.first_bad[1] = L; }
+    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    if (base >= L) return;
+    i128 acc = 0;
+    for (int u = 0; u < CI; u++) {
+        int64_t j = base + (int64_t)threadIdx.x * CI + u;
+        if (j < L) acc += fxk(a.sign * a.x[j], a.K);
+    }
+    i128 tot;
+    block_excl_scan<CT>(acc, lds, &tot);
+    if (threadIdx.x == 0) a.tsumP[blockIdx.x] = tot;
+}
+
+// exclusive scan of up to two per-tile int128 arrays (one block); the total of the
+// second goes to *total
+__global__ __launch_bounds__(SCAN_T) void esa_tile_scan_kernel(const i128 *in1, i128 *out1, const i128 *in2,
+                                                               i128 *out2, int64_t *total, const int64_t *Lp)
+{
+    __shared__ i128 lds[SCAN_T];
+    const int64_t L = *Lp;
+    const int64_t ntiles = (L + CTILE - 1) / CTILE;
+    const int64_t chunk = (ntiles + SCAN_T - 1) / SCAN_T;
+    const int64_t b0 = (int64_t)threadIdx.x * chunk;
+    for (int pass = 0; pass < (in2 ? 2 : 1); pass++) {
+        const i128 *in = pass ? in2 : in1;
+        i128 *out = pass ? out2 : out1;
+        i128 acc = 0;
+        for (int64_t b = b0; b < b0 + chunk && b < ntiles; b++) acc += in[b];
+        i128 tot;
+        i128 run = block_excl_scan<SCAN_T>(acc, lds, &tot);
+        for (int64_t b = b0; b < b0 + chunk && b < ntiles; b++) {
+            out[b] = run;
+            run += in[b];
+        }
+        if (pass == 1 && threadIdx.x == 0) *total = (int64_t)tot;
+    }
+}
+
+__global__ __launch_bounds__(CT) void esa_classify_kernel(EsaArgs a)
+{
+    __shared__ i128 lds[CT];
+    const int64_t L = *a.Lp;
+    const int64_t tile = blockIdx.x;
+    if (tile * CTILE >= L) return;
+    const double v0 = *a.v0p;
+    TileItems it;
+    tile_classify(a, tile, L, v0, it, lds);
+    i128 racc = 0, cnt = 0;
+    for (int u = 0; u < CI; u++) {
+        racc += it.R[u];
+        int64_t j = tile * CTILE + (int64_t)threadIdx.x * CI + u;
+        if (j < L && ((it.sens >> u) & 1)) cnt += 1;
+    }
+    i128 tr, tc;
+    block_excl_scan<CT>(racc, lds, &tr);
+    block_excl_scan<CT>(cnt, lds, &tc);
+    if (threadIdx.x == 0) { a.tsumR[tile] = tr; a.tcnt[tile] = tc; }
+}
+
+// in-tile exclusive prefixes of the regular increments and sensitive counts
+__device__ void tile_prefixes(const EsaArgs &a, int64_t tile, int64_t L, const TileItems &it, i128 *lds,
+                              i128 *Racc, int64_t *kpos)
+{
+    i128 racc = 0, cnt = 0;
+    const int64_t j0 = tile * CTILE + (int64_t)threadIdx.x * CI;
+    for (int u = 0; u < CI; u++) {
+        racc += it.R[u];
+        if (j0 + u < L && ((it.sens >> u) & 1)) cnt += 1;
+    }
+    i128 tr, tc;
+    *Racc = a.tpreR[tile] + block_excl_scan<CT>(racc, lds, &tr);
+    *kpos = (int64_t)(a.toff[tile] + block_excl_scan<CT>(cnt, lds, &tc));
+}
+
+__global__ __launch_bounds__(CT) void esa_compact_kernel(EsaArgs a)
+{
+    __shared__ i128 lds[CT];
+    const int64_t L = *a.Lp;
+    const int64_t tile = blockIdx.x;
+    if (tile * CTILE >= L) return;
+    const double v0 = *a.v0p;
+    TileItems it;
+    tile_classify(a, tile, L, v0, it, lds);
+    i128 R;
+    int64_t k;
+    tile_prefixes(a, tile, L, it, lds, &R, &k);
+    const int64_t j0 = tile * CTILE + (int64_t)threadIdx.x * CI;
+    for (int u = 0; u < CI; u++) {
+        int64_t j = j0 + u;
+        if (j >= L) break;
+        if ((it.sens >> u) & 1) {
+            if (k < a.smax) { a.s_idx[k] = j; a.s_racc[k] = R; }
+            k++;
+        }
+        R += it.R[u];
+    }
+}
+
+// One thread walks the sensitive elements in order; the block stages the list
+// through LDS.  Between two sensitive elements the regular run stays in one binade,
+// so v_prev + (run sum) is exact in double.
+__global__ __launch_bounds__(256) void esa_walk_kernel(EsaArgs a)
+{
+    __shared__ i128 sr[256];
+    __shared__ double sx[256], sv[256];
+    __shared__ double Vd;
+    __shared__ i128 Rl;
+    const int t = threadIdx.x;
+    int64_t ns = *a.s_n;
+    if (ns > a.smax) ns = a.smax;
+    if (t == 0) { Vd = *a.v0p; Rl = 0; }
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < ns; c0 += 256) {
+        int64_t k = c0 + t;
+        if (k < ns) { sr[t] = a.s_racc[k]; sx[t] = a.x[a.s_idx[k]]; }
+        __syncthreads();
+        if (t == 0) {
+            double vd = Vd;
+            i128 rl = Rl;
+            int cnt = (int)(ns - c0 < 256 ? ns - c0 : 256);
+            for (int u = 0; u < cnt; u++) {
+                i128 ra = sr[u];
+                double pred = ra == rl ? vd : vd + unfxk(ra - rl, a.K);
+                vd = pred + a.sign * sx[u];
+                rl = ra;
+                sv[u] = vd;
+            }
+            Vd = vd;
+            Rl = rl;
+        }
+        __syncthreads();
+        if (k < ns) a.s_v[k] = sv[t];
+        __syncthreads();
+    }
+}
+
+// value of an element from the last sensitive element kk at or before it
+__device__ __forceinline__ double esa_value(const EsaArgs &a, int64_t kk, int64_t ns_eff, double v0, i128 Rincl)
+{
+    if (kk >= ns_eff) kk = ns_eff - 1;
+    if (kk < 0) return v0 + unfxk(Rincl, a.K);
+    return a.s_v[kk] + unfxk(Rincl - a.s_racc[kk], a.K);
+}
+
+// Parallel rebuild of every element, verification of the recurrence, and the
+// tile sums of the next Jacobi increments.
+__global__ __launch_bounds__(CT) void esa_rebuild_kernel(EsaArgs a)
+{
+    __shared__ i128 lds[CT];
+    __shared__ double lastv[CT];
+    const int64_t L = *a.Lp;
+    const int64_t tile = blockIdx.x;
+    const int64_t base = tile * CTILE;
+    if (base >= L) return;
+    const double v0 = *a.v0p;
+    const int t = threadIdx.x;
+    int64_t ns_eff = *a.s_n;
+    if (ns_eff > a.smax) ns_eff = a.smax;
+    TileItems it;
+    tile_classify(a, tile, L, v0, it, lds);
+    i128 R;
+    int64_t k;
+    tile_prefixes(a, tile, L, it, lds, &R, &k);
+    const int64_t j0 = base + (int64_t)t * CI;
+    double nv[CI];
+    for (int u = 0; u < CI; u++) {
+        int64_t j = j0 + u;
+        nv[u] = 0.0;
+        if (j >= L) continue;
+        bool sens = (it.sens >> u) & 1;
+        R += it.R[u];
+        if (sens && k < ns_eff) nv[u] = a.s_v[k];
+        else nv[u] = esa_value(a, sens ? k : k - 1, ns_eff, v0, R);
+        if (sens) k++;
+    }
+    lastv[t] = nv[CI - 1];
+    __syncthreads();
+    double before;
+    if (t > 0) before = lastv[t - 1];
+    else if (tile == 0) before = v0;
+    else {
+        int64_t kk = (int64_t)a.toff[tile] - 1;
+        if (kk >= 0 && kk < ns_eff && a.s_idx[kk] == base - 1) before = a.s_v[kk];
+        else before = esa_value(a, kk, ns_eff, v0, a.tpreR[tile]);
+    }
+    i128 nacc = 0;
+    int64_t bad = L;
+    for (int u = 0; u < CI; u++) {
+        int64_t j = j0 + u;
+        if (j < L) {
+            double want = before + a.sign * it.xv[u];
+            if (want != nv[u] && j < bad) bad = j;
+            nacc += incr(before, it.xv[u], a.sign, a.K);
+            a.g[j] = nv[u];
+        }
+        before = nv[u];
+    }
+    if (bad < L) atomicMin((unsigned long long *)&a.first_bad[1], (unsigned long long)bad);
+    i128 tot;
+    block_excl_scan<CT>(nacc, lds, &tot);
+    if (t == 0) a.tsum[tile] = tot;
+}
+
 // ---------------------------------------------------------------- the three chains
 __global__ void pot_init_kernel(DevScalars *s) { s->first_inf = s->n_max + 1; }
 
@@ -350,14 +641,23 @@ __global__ void pot_at_kernel(const double *__restrict__ r, const double *__rest
     out[i] = v;
 }
 
-hipError_t run_chain(ChainArgs a, int rounds, int64_t n_ub, hipStream_t st)
+hipError_t run_chain(ChainArgs a, EsaArgs e, int rounds, int64_t n_ub, hipStream_t st)
 {
     int64_t ntiles = (n_ub + CTILE - 1) / CTILE;
     if (ntiles < 1) ntiles = 1;
-    hipLaunchKernelGGL(chain_seed_kernel, dim3((unsigned)ntiles), dim3(CT), 0, st, a);
+    dim3 gt((unsigned)ntiles), bt(CT);
+    hipLaunchKernelGGL(esa_sum_kernel, gt, bt, 0, st, e);
+    hipLaunchKernelGGL(esa_tile_scan_kernel, dim3(1), dim3(SCAN_T), 0, st, (const i128 *)e.tsumP, e.tpreP,
+                       (const i128 *)nullptr, (i128 *)nullptr, (int64_t *)nullptr, e.Lp);
+    hipLaunchKernelGGL(esa_classify_kernel, gt, bt, 0, st, e);
+    hipLaunchKernelGGL(esa_tile_scan_kernel, dim3(1), dim3(SCAN_T), 0, st, (const i128 *)e.tsumR, e.tpreR,
+                       (const i128 *)e.tcnt, e.toff, e.s_n, e.Lp);
+    hipLaunchKernelGGL(esa_compact_kernel, gt, bt, 0, st, e);
+    hipLaunchKernelGGL(esa_walk_kernel, dim3(1), dim3(256), 0, st, e);
+    hipLaunchKernelGGL(esa_rebuild_kernel, gt, bt, 0, st, e);
     for (int it = 0; it < rounds; it++) {
         hipLaunchKernelGGL(chain_scan_kernel, dim3(1), dim3(SCAN_T), 0, st, a);
-        hipLaunchKernelGGL(chain_round_kernel, dim3((unsigned)ntiles), dim3(CT), 0, st, a);
+        hipLaunchKernelGGL(chain_round_kernel, gt, bt, 0, st, a);
     }
     hipLaunchKernelGGL(chain_finish_kernel, dim3(1), dim3(1), 0, st, a);
     return hipGetLastError();
@@ -377,9 +677,21 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     w.scal = sc.chain_scal;
     w.len = sc.chain_len;
     w.fbad = sc.chain_fbad;
-    w.tsum = (i128 *)sc.tile_sums;          // 3 x 16 B per tile
-    w.tpre = w.tsum + sc.max_tiles;
-    w.bnd = (double *)(w.tpre + sc.max_tiles);
+    // per tile: tsum tpre tsumP tpreP tsumR tpreR tcnt toff (int128) and bnd (double)
+    i128 *tb = (i128 *)sc.tile_sums;
+    const int64_t T = sc.max_tiles;
+    w.tsum = tb;
+    w.tpre = tb + T;
+    w.bnd = (double *)(tb + 8 * T);
+    EsaArgs e{};
+    e.tsumP = tb + 2 * T; e.tpreP = tb + 3 * T; e.tsumR = tb + 4 * T; e.tpreR = tb + 5 * T;
+    e.tcnt = tb + 6 * T; e.toff = tb + 7 * T;
+    e.tsum = w.tsum;
+    e.s_idx = sc.esa_idx;
+    e.s_racc = (i128 *)sc.esa_racc;
+    e.s_v = sc.esa_v;
+    e.s_n = sc.esa_n;
+    e.smax = sc.esa_max;
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
     hipLaunchKernelGGL(pot_init_kernel, dim3(1), dim3(1), 0, st, s);
     if (n_ub > 0) hipLaunchKernelGGL(pot_count_kernel, dim3(g1), dim3(256), 0, st, f.r, s);
@@ -388,17 +700,23 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     // chain A: total mass (stored masses are O(1): 2^-70 resolution)
     hipLaunchKernelGGL(prep_a_kernel, dim3(g1), dim3(256), 0, st, f.m, s, w.xa);
     ChainArgs a{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 0, w.tsum, w.tpre, w.bnd, w.len + 0, 70};
-    if (hipError_t e = run_chain(a, 16, n_ub, st)) return e;
+    EsaArgs ae = e;
+    ae.x = w.xa; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0; ae.K = 70;
+    if (hipError_t err = run_chain(a, ae, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(total_kernel, dim3(1), dim3(1), 0, st, s, P, (const double *)w.ga, w);
     // chain B: enclosed mass
     hipLaunchKernelGGL(prep_b_kernel, dim3(g1), dim3(256), 0, st, f.m, s, (double)P.n_star, w.xa);
     ChainArgs b{w.xa, w.ga, -1.0, w.scal + 1, w.fbad + 2, w.tsum, w.tpre, w.bnd, w.len + 1, 100};
-    if (hipError_t e = run_chain(b, 8, n_ub, st)) return e;
+    EsaArgs be = e;
+    be.x = w.xa; be.g = w.ga; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1; be.K = 100;
+    if (hipError_t err = run_chain(b, be, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(save_mk_kernel, dim3(g1), dim3(256), 0, st, s, (const double *)w.ga, w.Mk);
     // chain C: w = -phi
     hipLaunchKernelGGL(prep_c_kernel, dim3(g1), dim3(256), 0, st, f.r, s, (const double *)w.Mk, w.xa);
     ChainArgs c{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 4, w.tsum, w.tpre, w.bnd, w.len + 2, 100};
-    if (hipError_t e = run_chain(c, 16, n_ub, st)) return e;
+    EsaArgs ce = e;
+    ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2; ce.K = 100;
+    if (hipError_t err = run_chain(c, ce, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(phi_out_kernel, dim3(g1), dim3(256), 0, st, f.r, f.phi, s, P, (const double *)w.ga);
     hipLaunchKernelGGL(phi0_empty_kernel, dim3(1), dim3(1), 0, st, f.r, f.phi, s, P);
     return hipGetLastError();

@@ -9,12 +9,18 @@
 struct Scratch {
     // potential chains: per-tile int128 sums and prefixes plus a boundary value
     // (48 B per tile), chain increments / iterates / enclosed masses per star
-    double *tile_sums;
+    double *tile_sums;         // 18 doubles per tile (potential chains)
     int64_t max_tiles;
     double *chain_x, *chain_g, *chain_mk;
     double *chain_scal;         // 4 doubles
     int64_t *chain_len;         // 4
     int64_t *chain_fbad;        // 6
+    // potential chains: list of sensitive elements (index, int128 prefix, value)
+    int64_t *esa_idx;
+    double *esa_racc;           // int128 per entry
+    double *esa_v;
+    int64_t *esa_n;
+    int64_t esa_max;
     // deterministic block partials: 8 doubles per block
     double *partials;
     int64_t max_blocks;
@@ -24,10 +30,14 @@ struct Scratch {
     uint32_t *block_hist;       // 256 * max_sort_tiles
     uint32_t *digit_totals;     // 8 * 256
     int64_t max_sort_tiles;
-    // set_velocities3 excess chain
-    double *excess_c;           // per star
+    // set_velocities3 excess chain: per-star excess, ordered event list, per-event
+    // walk ends and merge flags, per-block counts, fallback flag
+    double *excess_c;
     int64_t *excess_list;
-    int64_t max_events;
+    int64_t *sv3_wend;
+    int32_t *sv3_merged;
+    int64_t *sv3_blk;
+    int32_t *sv3_fallback;
     // tidal stripping loop flag
     int32_t *strip_done;
     // orbit stars deferred to the linear-scan path (65536 slots)

@@ -90,6 +90,8 @@ typedef struct orc_state {
     int64_t n_linear_scan;
     int32_t error;               /* first error code (ORC_E_*), 0 if none */
     int64_t error_index;
+    int64_t n_excess_events;     /* set_velocities3 stars with vnew2 <= 0 (last call) */
+    int64_t n_excess_dumps;      /* stars that absorbed excess energy (last call) */
 } orc_state;
 
 enum {

@@ -43,7 +43,8 @@ class State(C.Structure):
                    ("q_mode", C.c_int32), ("step", C.c_uint64)]
                 + [(k, C.c_int64) for k in ["n_brent", "n_circular", "n_removed_E", "n_removed_tidal",
                                             "n_removed_mass", "n_linscan"]]
-                + [("error", C.c_int32), ("error_index", C.c_int64)])
+                + [("error", C.c_int32), ("error_index", C.c_int64), ("n_excess_events", C.c_int64),
+                   ("n_excess_dumps", C.c_int64)])
 
 
 class OrbitRS(C.Structure):

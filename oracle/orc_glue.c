@@ -92,6 +92,8 @@ void orc_set_velocities3(orc_state *s)
 {
     const double q = 0.5, E_dump_factor = 0.8;
     double Eexcess = 0.0;
+    s->n_excess_events = 0;
+    s->n_excess_dumps = 0;
     for (int64_t i = 1; i <= s->N_MAX_NEW; i++) {
         double m = s->m[i];
         double Unewrold = orc_potential(s, s->rOld[i]) + phi_s(s, s->rOld[i], i);
@@ -100,6 +102,7 @@ void orc_set_velocities3(orc_state *s)
         double vnew2 = vold2 + 2.0 * (1.0 - q) * (s->Uoldrold[i] - s->Uoldrnew[i])
                        + 2.0 * q * (Unewrold - Unewrnew);
         if (vnew2 <= 0.0) {
+            s->n_excess_events++;
             Eexcess += 0.5 * (orc_sqr(s->vr[i]) + orc_sqr(s->vt[i]) - vnew2) * m;
         } else {
             double alpha = sqrt(vnew2 / (orc_sqr(s->vr[i]) + orc_sqr(s->vt[i])));
@@ -111,6 +114,7 @@ void orc_set_velocities3(orc_state *s)
             double E_dump_capacity = E_dump_factor * 0.5 * m * v2;
             double E_dump = OMIN(E_dump_capacity, Eexcess);
             if (Eexcess > 0) {
+                s->n_excess_dumps++;
                 double exc_ratio = sqrt((v2 - 2 * E_dump / m) / v2);
                 s->vr[i] *= exc_ratio;
                 s->vt[i] *= exc_ratio;

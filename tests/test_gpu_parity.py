@@ -312,6 +312,45 @@ def test_gpu_glue_parity(gpu, O):
     assert sc["Epot"] == pytest.approx(o.sc("Epot"), rel=1e-12)
 
 
+@pytest.mark.parametrize("case", ["sparse", "dense", "huge_excess"])
+def test_gpu_set_velocities3_excess_chain(gpu, O, case):
+    """Excess-energy chain of set_velocities3 (cmc_utils.c:268-310) with planted
+    events (stars whose new kinetic energy would be negative): isolated, merging
+    and very long episodes must give bit-identical velocities."""
+    cm = gpu
+    say("set_velocities3 excess chain", case)
+    n = 300000
+    cl = cm.make_plummer(n, seed=11)
+    o = oracle_for(O, cl)
+    o.potential_calculate()
+    rng = np.random.default_rng(5)
+    N = o.N
+    p_ev, scale = {"sparse": (0.002, 2.0), "dense": (0.3, 1.5), "huge_excess": (2e-5, 2e5)}[case]
+    v2 = o["vr"][1:N + 1] ** 2 + o["vt"][1:N + 1] ** 2
+    ev = rng.random(N) < p_ev
+    delta = np.where(ev, v2 * (1.0 + scale * rng.random(N)), v2 * 0.2 * (rng.random(N) - 0.5))
+    o["rOld"][1:N + 1] = o["r"][1:N + 1]
+    o["vrold"][1:N + 1] = o["vr"][1:N + 1]
+    o["vtold"][1:N + 1] = o["vt"][1:N + 1]
+    o["Uoldrold"][1:N + 1] = 0.0
+    o["Uoldrnew"][1:N + 1] = delta
+    h = cm.Cluster(N)
+    for k in h.a:
+        if k in o.arr:
+            h[k] = o[k][: N + 2]
+    with ctx_for(cm, cl) as ctx:
+        ctx.upload(h)
+        ctx.set_scalars(n_max=N, n_max_new=N, Mtotal=o.sc("Mtotal"))
+        ctx.set_velocities3()
+        d = ctx.download()
+        assert ctx.scalars()["error"] == 0
+    O.lib.orc_set_velocities3(C.byref(o.s))
+    say(case, "events", o.s.n_excess_events, "dumps", o.s.n_excess_dumps)
+    assert o.s.n_excess_events > 0
+    for k in ("vr", "vt", "vrnew", "vtnew"):
+        assert np.array_equal(d[k][1:N + 1], o[k][1:N + 1]), k
+
+
 # ------------------------------------------------------------------ whole step
 @pytest.mark.parametrize("case", ["plummer_1e5", "plummer_kroupa_1e6"])
 def test_gpu_step_vs_oracle(gpu, O, case):

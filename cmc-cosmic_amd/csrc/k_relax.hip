@@ -6,7 +6,10 @@
 // Relaxation is one thread per radially adjacent pair (2p+1, 2p+2): a pure
 // streaming map reading r, m, phi, vr, vt of both stars plus r[k-20], r[k+20]
 // (cache-served) and writing vr, vt, E, J, Y: 72 B/star of compulsory HBM traffic.
+// The scattering angles' sin/cos are correctly rounded double-double evaluations
+// (dd_trig.hpp), so the kicks do not depend on the device math library.
 #include "kernels.hpp"
+#include "dd_trig.hpp"
 
 namespace {
 __device__ __forceinline__ double sqr(double x) { return x * x; }
@@ -149,8 +152,10 @@ __global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, 
     double vrk = f.vr[k], vtk = f.vt[k], vrkp = f.vr[kp], vtkp = f.vt[kp];
     double v[4], vp[4], w[4], w1[4], w2[4];
     v[1] = vtk; v[2] = 0.0; v[3] = vrk;
-    vp[1] = vtkp * cos(phi);
-    vp[2] = vtkp * sin(phi);
+    double sphi, cphi;
+    ddtrig::sincos(phi, &sphi, &cphi);
+    vp[1] = vtkp * cphi;
+    vp[2] = vtkp * sphi;
     vp[3] = vrkp;
     for (int j = 1; j <= 3; j++) w[j] = vp[j] - v[j];
     double W = sqrt(sqr(w[1]) + sqr(w[2]) + sqr(w[3]));
@@ -172,7 +177,9 @@ __global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, 
     w2[2] = -w[2] * w[3] / wp;
     w2[3] = wp;
     double psi = rng_dbl(rng) * 2 * CMCG_PI;
-    double cb = cos(beta), sb = sin(beta), cp = cos(psi), sp = sin(psi);
+    double cb, sb, cp, sp;
+    ddtrig::sincos(beta, &sb, &cb);
+    ddtrig::sincos(psi, &sp, &cp);
     double v_new[4], vp_new[4];
     for (int j = 1; j <= 3; j++) {
         double w_new = w[j] * cb + w1[j] * sb * cp + w2[j] * sb * sp;

@@ -51,6 +51,7 @@ orc_t113 orc_stream_orbit(const orc_t113 *seeded, uint64_t step, uint64_t slot);
 
 enum { ORC_RNG_REF_SEQ = 0, ORC_RNG_PER_STAR = 1 };
 enum { ORC_Q_LONG_DOUBLE = 0, ORC_Q_FP64 = 1 };
+enum { ORC_LIBM_SYSTEM = 0, ORC_LIBM_LONG_DOUBLE = 1, ORC_LIBM_DD = 2 };
 
 /* ---------------------------------------------------------------- state */
 typedef struct orc_state {
@@ -92,6 +93,7 @@ typedef struct orc_state {
     int64_t error_index;
     int64_t n_excess_events;     /* set_velocities3 stars with vnew2 <= 0 (last call) */
     int64_t n_excess_dumps;      /* stars that absorbed excess energy (last call) */
+    int32_t libm_mode;           /* ORC_LIBM_*: sin/cos of the relaxation */
 } orc_state;
 
 enum {
@@ -100,6 +102,9 @@ enum {
 };
 enum { ORC_E_NAN = -1, ORC_E_NTRY = -2, ORC_E_BRACKET = -3, ORC_E_WZERO = -4, ORC_E_WPZERO = -5,
        ORC_E_BRENT = -6 };
+
+/* double-double sin/cos shared (operation for operation) with the device relaxation */
+void orc_dd_sincos(double x, double *s, double *c);
 
 /* ---------------------------------------------------------------- kernels of the path */
 double  orc_sqr(double x);                                     /* cmc_utils.c:27-30 */

@@ -44,7 +44,7 @@ class State(C.Structure):
                 + [(k, C.c_int64) for k in ["n_brent", "n_circular", "n_removed_E", "n_removed_tidal",
                                             "n_removed_mass", "n_linscan"]]
                 + [("error", C.c_int32), ("error_index", C.c_int64), ("n_excess_events", C.c_int64),
-                   ("n_excess_dumps", C.c_int64)])
+                   ("n_excess_dumps", C.c_int64), ("libm_mode", C.c_int32)])
 
 
 class OrbitRS(C.Structure):
@@ -83,6 +83,7 @@ _sigs = {
     "orc_compute_energy": (None, [_SP]),
     "orc_calc_timestep": (C.c_double, [_SP]),
     "orc_init": (None, [_SP]),
+    "orc_dd_sincos": (None, [C.c_double, DPTR, DPTR]),
     "orc_step": (None, [_SP]),
 }
 for _n, (_r, _a) in _sigs.items():
@@ -92,6 +93,7 @@ for _n, (_r, _a) in _sigs.items():
 
 RNG_REF_SEQ, RNG_PER_STAR = 0, 1
 Q_LONG_DOUBLE, Q_FP64 = 0, 1
+LIBM_SYSTEM, LIBM_LONG_DOUBLE, LIBM_DD = 0, 1, 2
 
 
 def seed_state(seed):
@@ -104,7 +106,7 @@ class Oracle:
     """Oracle cluster state with numpy-owned arrays."""
 
     def __init__(self, cluster_arrays, n, n_star=None, orbit_r=1e3, seed=1, rng_mode=RNG_PER_STAR,
-                 q_mode=Q_LONG_DOUBLE, theta_se_max=1.412, avekernel=20, gamma=0.1):
+                 q_mode=Q_LONG_DOUBLE, theta_se_max=1.412, avekernel=20, gamma=0.1, libm_mode=LIBM_SYSTEM):
         self.n = int(n)
         cap = self.n + 2
         self.arr = {}
@@ -137,6 +139,7 @@ class Oracle:
         s.seq = seed_state(seed)
         s.rng_mode = int(rng_mode)
         s.q_mode = int(q_mode)
+        s.libm_mode = int(libm_mode)
         self.s = s
 
     def __getitem__(self, k):
@@ -178,6 +181,12 @@ class Oracle:
         o = lib.orc_calc_orbit_new(C.byref(self.s), int(j), float(E), float(J))
         return dict(rp=o.rp, ra=o.ra, dQdrp=o.dQdrp, dQdra=o.dQdra, kmin=o.kmin, kmax=o.kmax,
                     circular=o.circular_flag)
+
+
+def dd_sincos(x):
+    s, c = C.c_double(), C.c_double()
+    lib.orc_dd_sincos(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
 
 
 def ref_taus_lib():

@@ -526,6 +526,29 @@ double orc_simul_relax_new(const orc_state *s)
     return dtmin;
 }
 
+/* sin/cos of the relaxation.  ORC_LIBM_SYSTEM calls the C library, as the
+ * reference does; ORC_LIBM_DD uses the double-double functions that the device
+ * uses (orc_trig.c); ORC_LIBM_LONG_DOUBLE rounds the long double functions to
+ * double, a third last-bit variant used only to measure how sensitive the
+ * evolution is to the math library.  The library functions are called through
+ * volatile pointers so that an optimising compiler cannot merge a same-argument
+ * pair into sincos(), which rounds differently; the reference's documented build
+ * is unoptimised and calls them separately. */
+static double (*volatile libm_sin)(double) = sin;
+static double (*volatile libm_cos)(double) = cos;
+static double o_sin(const orc_state *s, double x)
+{
+    double sn, cs;
+    if (s->libm_mode == ORC_LIBM_DD) { orc_dd_sincos(x, &sn, &cs); return sn; }
+    return s->libm_mode == ORC_LIBM_LONG_DOUBLE ? (double)sinl((long double)x) : libm_sin(x);
+}
+static double o_cos(const orc_state *s, double x)
+{
+    double sn, cs;
+    if (s->libm_mode == ORC_LIBM_DD) { orc_dd_sincos(x, &sn, &cs); return cs; }
+    return s->libm_mode == ORC_LIBM_LONG_DOUBLE ? (double)cosl((long double)x) : libm_cos(x);
+}
+
 /* cmc_dynamics.c:17-542, relaxation branch (:427-498) with the configs' switches:
  * no binaries, no collisions (S = 0 so P_enc = 0), no three-body binaries, no
  * central mass.  Three draws per pair: Y (cmc_dynamics_helper.c:839), the P_enc test
@@ -550,8 +573,8 @@ void orc_dynamics_apply(orc_state *s, double dt)
         s->Y[kp] = s->Y[k];
         phi = s->Y[k] * 2.0 * ORC_PI;
         v[1] = s->vt[k]; v[2] = 0.0; v[3] = s->vr[k];
-        vp[1] = s->vt[kp] * cos(phi);
-        vp[2] = s->vt[kp] * sin(phi);
+        vp[1] = s->vt[kp] * o_cos(s, phi);
+        vp[2] = s->vt[kp] * o_sin(s, phi);
         vp[3] = s->vr[kp];
         for (j = 1; j <= 3; j++) w[j] = vp[j] - v[j];
         W = sqrt(orc_sqr(w[1]) + orc_sqr(w[2]) + orc_sqr(w[3]));
@@ -577,7 +600,7 @@ void orc_dynamics_apply(orc_state *s, double dt)
         w2[3] = wp;
         psi = orc_t113_dbl(rng) * 2 * ORC_PI;
         for (j = 1; j <= 3; j++)
-            w_new[j] = w[j] * cos(beta) + w1[j] * sin(beta) * cos(psi) + w2[j] * sin(beta) * sin(psi);
+            w_new[j] = w[j] * o_cos(s, beta) + w1[j] * o_sin(s, beta) * o_cos(s, psi) + w2[j] * o_sin(s, beta) * o_sin(s, psi);
         for (j = 1; j <= 3; j++) {
             v_new[j] = v[j] - mass_kp / (mass_k + mass_kp) * (w_new[j] - w[j]);
             vp_new[j] = vp[j] + mass_k / (mass_k + mass_kp) * (w_new[j] - w[j]);

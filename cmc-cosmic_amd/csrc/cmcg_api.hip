@@ -306,6 +306,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     Scratch &sc = c->sc;
     sc.max_tiles = (n_cap + 2047) / 2048 + 1;
     sc.esa_max = 1 << 16;
+    sc.max_waves = (n_cap + 2) / 64 + 2;
     sc.max_blocks = (n_cap + 255) / 256 + 2;
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 8)) ||
@@ -320,7 +321,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.defer_list, 65536)) || (e = dalloc(c, &sc.defer_kfirst, 65536)) ||
         (e = dalloc(c, &sc.defer_partials, 65536 * 8)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
         (e = dalloc(c, &sc.esa_idx, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_racc, (size_t)sc.esa_max * 2)) ||
-        (e = dalloc(c, &sc.esa_v, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_n, 4)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
+        (e = dalloc(c, &sc.esa_v, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_n, 4)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_wave, (size_t)sc.max_waves)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
         (e = dalloc(c, &c->T, 1))) {
         cmcg_destroy(c);
         return e;

@@ -113,6 +113,49 @@ __device__ __forceinline__ Rng rng_jump(const RngTable *__restrict__ T, Rng s, u
     return s;
 }
 
+// In-wave jumps through LDS: for the NB matrix powers T^(2^b), b = b0..b0+NB-1, and
+// each state word, tab[b][c][q][v] is the product of the matrix with the nibble v
+// placed at bits 4q..4q+3, so a 32x32 GF(2) product is 8 table lookups.  Lanes of a
+// wave read 16 consecutive words per lookup: no bank conflicts.
+template <int NB>
+struct NibbleTables { uint32_t t[NB][4][8][16]; };
+
+template <int NB>
+__device__ void build_nibble_tables(const RngTable *__restrict__ T, int b0, NibbleTables<NB> &lt)
+{
+    for (int idx = threadIdx.x; idx < NB * 4 * 8 * 16; idx += blockDim.x) {
+        const int v = idx & 15, q = (idx >> 4) & 7, c = (idx >> 7) & 3, b = idx >> 9;
+        const uint32_t *col = T->col[b0 + b][c];
+        uint32_t o = 0;
+        for (int i = 0; i < 4; i++)
+            if ((v >> i) & 1) o ^= col[4 * q + i];
+        lt.t[b][c][q][v] = o;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t nib_matvec(const uint32_t (&tab)[8][16], uint32_t v)
+{
+    uint32_t o = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) o ^= tab[q][(v >> (4 * q)) & 15u];
+    return o;
+}
+
+// advance by off * 2^b0 steps, off < 2^NB
+template <int NB>
+__device__ __forceinline__ Rng rng_jump_lds(const NibbleTables<NB> &lt, Rng s, uint32_t off)
+{
+    for (int b = 0; b < NB; b++) {
+        if (!((off >> b) & 1u)) continue;
+        s.z0 = nib_matvec(lt.t[b][0], s.z0);
+        s.z1 = nib_matvec(lt.t[b][1], s.z1);
+        s.z2 = nib_matvec(lt.t[b][2], s.z2);
+        s.z3 = nib_matvec(lt.t[b][3], s.z3);
+    }
+    return s;
+}
+
 // ---------------------------------------------------------------- potential helpers
 // include/cmc/cmc.h:2175 MPI_PHI_S(rad, j)
 __device__ __forceinline__ double phi_s(const double *__restrict__ r, const double *__restrict__ m,

@@ -54,8 +54,8 @@ __device__ __forceinline__ void removed(double (&red)[5], double Ej, double J, d
 
 // The part of get_positions_loop after the positive-Q index is known
 // (cmc_evolution_thr.c:832-934).
-__device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *__restrict__ T,
-                            const Rng &base, int jb1, int64_t n, int64_t j, double rj, double mj, double E,
+__device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P, Rng rng,
+                            int64_t n, int64_t j, double rj, double mj, double E,
                             double Ej, double J, int64_t ktemp, int &flags, int &ntry, double (&red)[5])
 {
     const double *__restrict__ r = f.r;
@@ -81,7 +81,6 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
     double g1 = sqrt(3.0 * (rmax - rmin) / o.dQdrp);
     double g2 = sqrt(-3.0 * (rmax - rmin) / o.dQdra);
     double F = 1.2 * (g1 >= g2 ? g1 : g2);
-    Rng rng = rng_jump(T, base, (uint64_t)j << 20, 20, jb1);
     // narrowed bracket for potential(r): r in [rmin, rmax] within [r_kmin, r_kmax+1]
     int64_t lo0 = o.kmin < 1 ? 1 : o.kmin;
     int64_t hi0 = o.kmax + 1 > n + 1 ? n + 1 : o.kmax + 1;
@@ -125,12 +124,16 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
     atomicMax((unsigned long long *)&s->tries_max, (unsigned long long)k);
 }
 
+// Slot j draws from offset j 2^20; the wave's first slot starts at the per-wave state
+// and lane l jumps l 2^20 (l < 2^6) through the LDS tables.
 __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
                                                              const RngTable *__restrict__ T,
-                                                             const Rng *__restrict__ base, int jb1,
+                                                             const Rng *__restrict__ wave_base,
                                                              double *__restrict__ partials,
                                                              int64_t *__restrict__ defer_list)
 {
+    __shared__ NibbleTables<6> lt;
+    build_nibble_tables<6>(T, 20, lt);
     const int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max;
     // [0] max_r, [1] Eescaped, [2] Jescaped, [3] TidalMassLoss, [4] Etidal
@@ -163,7 +166,8 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
                 else dev_set_error(s, CMCG_E_STATE, j);
                 atomicAdd((unsigned long long *)&s->n_linscan, 1ull);
             } else {
-                sample_star(f, s, P, T, base[1], jb1, n, j, rj, mj, E, Ej, J, j, flags, ntry, red);
+                Rng rng = rng_jump_lds<6>(lt, wave_base[(j - 1) >> 6], (uint32_t)((j - 1) & 63));
+                sample_star(f, s, P, rng, n, j, rj, mj, E, Ej, J, j, flags, ntry, red);
             }
         }
         f.flags[j] = flags;
@@ -228,7 +232,8 @@ __global__ __launch_bounds__(ORB_THREADS) void defer_tail_kernel(DevFields f, De
         if (kt >= n) kt = -1;                      // cmc_orbit.c:52-56: circular
         double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         int flags = CMCG_SF_LINSCAN, ntry = 0;
-        sample_star(f, s, P, T, base[1], jb1, n, j, rj, mj, E, Ej, J, kt, flags, ntry, red);
+        sample_star(f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), n, j, rj, mj, E, Ej, J, kt, flags,
+                    ntry, red);
         f.flags[j] = flags;
         f.ntry[j] = ntry;
         for (int i = 0; i < 5; i++) dpartials[8 * d + i] = red[i];
@@ -325,8 +330,9 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     if (nblocks > sc.max_blocks) return hipErrorInvalidValue;
     int jb1 = 20 + bits_needed((uint64_t)n_ub);
     hipLaunchKernelGGL(orbits_begin_kernel, dim3(1), dim3(1), 0, st, s);
-    hipLaunchKernelGGL(orbits_kernel, dim3((unsigned)nblocks), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base,
-                       jb1, sc.partials, sc.defer_list);
+    if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
+    hipLaunchKernelGGL(orbits_kernel, dim3((unsigned)nblocks), dim3(ORB_THREADS), 0, st, f, s, P, T,
+                       (const Rng *)sc.rng_wave, sc.partials, sc.defer_list);
     hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
                        sc.defer_kfirst);

@@ -53,6 +53,17 @@ __global__ void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s, 
     base[1] = b;
 }
 
+// state at the first pair (phase 0: offset 3 * 64 w) or slot (phase 1: offset
+// (1 + 64 w) 2^20) of wave w
+__global__ void rng_wave_bases_kernel(const RngTable *__restrict__ T, const Rng *__restrict__ base, int phase,
+                                      int64_t nwaves, Rng *__restrict__ out, int jb1)
+{
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwaves) return;
+    if (phase == 0) out[w] = rng_jump(T, base[0], 192ull * (uint64_t)w, 0, jb1);
+    else out[w] = rng_jump(T, base[1], (uint64_t)(1 + 64 * w) << 20, 20, jb1);
+}
+
 __global__ void rng_draws_kernel(const RngTable *__restrict__ T, const Rng *__restrict__ base, int phase,
                                  int64_t first, int64_t count, int ndraw, double *__restrict__ out, int jb1)
 {
@@ -133,16 +144,19 @@ __global__ void sigma_kernel(const double *__restrict__ m, const double *__restr
     sigma[si] = sqrt(Mv2ave / Mave);
 }
 
+// Pair pr draws from offset 3 pr; a wave's first pair starts at the per-wave state
+// and lane l jumps 3 l < 2^8 steps through the LDS tables.
 __global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, KParams P,
-                                                    const RngTable *__restrict__ T, const Rng *__restrict__ base,
-                                                    int jb1)
+                                                    const RngTable *__restrict__ T, const Rng *__restrict__ wave_base)
 {
+    __shared__ NibbleTables<8> lt;
+    build_nibble_tables<8>(T, 0, lt);
     int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t N = s->n_max;
     if (pr >= N / 2) return;            // pairs while si <= N - N%2 - 1 (cmc_dynamics.c:203)
     const int64_t k = 2 * pr + 1, kp = k + 1;
     const double dt = s->Dt;
-    Rng rng = rng_jump(T, base[0], 3ull * (uint64_t)pr, 0, jb1);
+    Rng rng = rng_jump_lds<8>(lt, wave_base[pr >> 6], 3u * (uint32_t)(pr & 63));
 
     // calc_encounter_dyns (cmc_dynamics_helper.c:831-871)
     double Y = rng_dbl(rng);
@@ -217,6 +231,17 @@ hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch
     return hipGetLastError();
 }
 
+hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase, int64_t count, hipStream_t st)
+{
+    int64_t nwaves = (count + 63) / 64;
+    if (nwaves <= 0) return hipSuccess;
+    if (nwaves > sc.max_waves) return hipErrorInvalidValue;
+    int jb1 = phase == 0 ? bits_needed(192ull * (uint64_t)nwaves) : 20 + bits_needed((uint64_t)(64 * nwaves + 1));
+    hipLaunchKernelGGL(rng_wave_bases_kernel, dim3((unsigned)((nwaves + 255) / 256)), dim3(256), 0, st, T,
+                       (const Rng *)sc.rng_base, phase, nwaves, sc.rng_wave, jb1);
+    return hipGetLastError();
+}
+
 hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64_t first, int64_t count,
                             int ndraw, double *out, hipStream_t st)
 {
@@ -256,8 +281,8 @@ hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, con
 {
     int64_t npairs = n_ub / 2;
     if (npairs <= 0) return hipSuccess;
-    int jb1 = bits_needed(3ull * (uint64_t)npairs);
+    if (hipError_t e = launch_rng_wave_bases(T, sc, 0, npairs, st)) return e;
     hipLaunchKernelGGL(relax_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P, T,
-                       sc.rng_base, jb1);
+                       (const Rng *)sc.rng_wave);
     return hipGetLastError();
 }

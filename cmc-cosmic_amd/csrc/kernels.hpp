@@ -46,6 +46,8 @@ struct Scratch {
     // RNG bases of the current step: [0] relaxation, [1] orbits; seeded state
     Rng *rng_base;
     Rng *rng_seeded;
+    Rng *rng_wave;              // per-wave substream starts (relaxation pairs / orbit slots)
+    int64_t max_waves;
 };
 
 hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
@@ -60,6 +62,9 @@ hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, con
                         const Scratch &sc, int64_t n_ub, hipStream_t st);
 hipError_t launch_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st);
 hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch &sc, int64_t step, hipStream_t st);
+// substream start of every 64-lane wave: relaxation (phase 0, pairs 64w..) or orbits
+// (phase 1, slots 1 + 64w..), for `count` pairs / slots
+hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase, int64_t count, hipStream_t st);
 hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64_t first, int64_t count,
                             int ndraw, double *out, hipStream_t st);
 

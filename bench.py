@@ -6,10 +6,14 @@ masses placed in the inner 10% of radii, FP64, plain GPU step (sort, potential,
 relaxation, orbit sampling and the step glue between them), seeded synthetic data.
 
 Prints ONE JSON line.  value = live stars processed per second of wall time over
-the timed steps (inputs resident in HBM).  roofline: the phase with the largest
-device time, its algorithmic bytes per launch over its HIP-event time.  cpu_baseline:
-the CPU oracle (a restatement of the reference's serial path, reference RNG order,
-x87 long-double Q) timed on a bounded sample on this host.
+the timed plain steps (inputs resident in HBM).  learned: the same for steps with
+learned find_zero_Q brackets, after training the network on the GPU on rows
+harvested during the warm-up steps.  roofline: the dominant kernel (the orbit
+sampler), its algorithmic bytes per launch over its HIP-event time on the
+launching stream; traffic = measured HBM bytes per launch when a counter summary
+exists in profiles/.  cpu_baseline: the CPU oracle (a restatement of the
+reference's serial path, reference RNG order, x87 long-double Q) timed on a
+bounded sample on this host.
 """
 import argparse
 import importlib.util
@@ -66,6 +70,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--train-iters", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported "
@@ -80,43 +85,83 @@ def main():
     ctx = cm.Context(cl.n, params)
     ctx.upload(cl)
     ctx.init_cluster()
+    # warm-up steps also harvest training rows for the learned brackets
+    ctx.mlp_init(seed=1, max_rows=1 << 24)
+    ctx.mlp_harvest(True)
     for i in range(args.warmup):
         ctx.step()
-    log(f"warmup {args.warmup} steps done, N={ctx.scalars()['n_max']}")
-    ctx.timer_reset()
-    ctx.timer_enable(True)
-    live = []
-    times = []
-    ctx.sync()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        n_live = ctx.scalars()["n_max"] if i == 0 else live_next
-        ts = time.perf_counter()
-        st = ctx.step()
-        te = time.perf_counter()
-        times.append(te - ts)
-        live.append(n_live)
-        live_next = ctx.scalars()["n_max"]
-    ctx.sync()
-    total = time.perf_counter() - t_start
-    ctx.timer_enable(False)
-    log(f"timed {args.steps} steps in {total:.3f} s")
+    ctx.mlp_harvest(False)
+    log(f"warmup {args.warmup} steps done, N={ctx.scalars()['n_max']}, rows harvested {ctx.mlp_rows(copy=False)}")
+
+    def timed(n_steps):
+        ctx.timer_reset()
+        ctx.timer_enable(True)
+        live, times, st = [], [], None
+        ctx.sync()
+        n_live = ctx.scalars()["n_max"]
+        t_start = time.perf_counter()
+        for i in range(n_steps):
+            ts = time.perf_counter()
+            st = ctx.step()
+            times.append(time.perf_counter() - ts)
+            live.append(n_live)
+            n_live = ctx.scalars()["n_max"]
+        ctx.sync()
+        total = time.perf_counter() - t_start
+        ctx.timer_enable(False)
+        phases = {}
+        for name in ["timestep", "relax", "ec1", "orbits", "orbits_kernel", "glue_pre_sort", "ie", "sort",
+                     "potential", "glue_post_sort"]:
+            ms, calls = ctx.timer_get(name)
+            if calls:
+                per = ms / calls
+                ent = {"ms": per}
+                key = "orbits" if name == "orbits_kernel" else name
+                if key in ALG_BYTES:
+                    gbs = ALG_BYTES[key] * np.mean(live) / (per * 1e-3) / 1e9
+                    ent.update({"alg_bytes_per_star": ALG_BYTES[key], "alg_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS})
+                phases[name] = ent
+        return total, live, times, phases, st
+
+    total, live, times, phases, st = timed(args.steps)
+    log(f"timed {args.steps} plain steps in {total:.3f} s")
     stars = float(np.sum(live))
-    phases = {}
-    for name in ["timestep", "relax", "ec1", "orbits", "glue_pre_sort", "ie", "sort", "potential", "glue_post_sort"]:
-        ms, calls = ctx.timer_get(name)
-        if calls:
-            per = ms / calls
-            ent = {"ms": per}
-            if name in ALG_BYTES:
-                gbs = ALG_BYTES[name] * np.mean(live) / (per * 1e-3) / 1e9
-                ent.update({"alg_bytes_per_star": ALG_BYTES[name], "alg_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS})
-            phases[name] = ent
-    dom = max((k for k in phases if k in ALG_BYTES), key=lambda k: phases[k]["ms"])
+
+    # learned brackets: train on the harvested rows (device kernels), then time
+    t_tr = time.perf_counter()
+    rows = ctx.mlp_rows()
+    ctx.mlp_fit_norm()
+    val = rows[:: max(1, rows.shape[0] // 20000)][:20000]
+    loss = ctx.mlp_train(args.train_iters, batch=4096, lr=2e-3, seed=3)
+    ctx.sync()
+    train_s = time.perf_counter() - t_tr
+    err = float(np.sqrt(np.mean((ctx.mlp_predict(val[:, :6]) - val[:, 6:]) ** 2)))
+    margin = max(3.0 * err, 0.005)
+    log(f"trained {args.train_iters} Adam steps on {rows.shape[0]} rows in {train_s:.2f} s: "
+        f"loss {loss:.3e}, rms log-r error {err:.4f}, margin {margin:.4f}")
+    ctx.set_learned_brackets(True, margin=margin)
+    ltotal, llive, ltimes, lphases, lst = timed(args.steps)
+    ctx.set_learned_brackets(False)
+    lstars = float(np.sum(llive))
+    log(f"timed {args.steps} learned steps in {ltotal:.3f} s")
+    learned = {"value": lstars / ltotal, "unit": "stars/s", "ms_per_step": ltotal / args.steps * 1e3,
+               "median_ms_per_step": float(np.median(ltimes) * 1e3),
+               "speedup_vs_plain": (lstars / ltotal) / (stars / total),
+               "orbits_kernel_ms": lphases.get("orbits_kernel", {}).get("ms"),
+               "brackets_accepted": lst["n_bracket_hit"], "brackets_rejected": lst["n_bracket_fallback"],
+               "train": {"rows": int(rows.shape[0]), "adam_steps": args.train_iters, "batch": 4096, "seconds": train_s,
+                         "rms_log_r_error": err, "margin_log_r": margin}}
+
+    dom = "orbits_kernel"
     d = phases[dom]
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": None,
-                "alg_bytes_per_launch": ALG_BYTES[dom] * float(np.mean(live))}
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", "orbits_kernel_king4e6_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"kernel": "orbits_kernel (orbit sampling, k_orbits.hip)", "bound": "hbm", "achieved": d["alg_GBs"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                "alg_bytes_per_launch": ALG_BYTES["orbits"] * float(np.mean(live)), "avg_launch_ms": d["ms"]}
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle step on the same {cl.n} stars")
@@ -141,6 +186,7 @@ def main():
                    "n_stars": args.n, "n_live_mean": float(np.mean(live)), "seed": 1, "step": "plain"},
         "roofline": roofline,
         "phases": phases,
+        "learned": learned,
         "cpu_baseline": cpu,
         "last_step_stats": st,
     }

@@ -131,7 +131,18 @@ _sigs = {
     "cmcg_timer_enable": [_ctx, C.c_int],
     "cmcg_timer_get": [_ctx, C.c_char_p, _P(C.c_double), _P(C.c_int64)],
     "cmcg_timer_reset": [_ctx],
+    "cmcg_mlp_init": [_ctx, C.c_uint64, C.c_int64],
+    "cmcg_mlp_get_params": [_ctx, _P(C.c_float), _P(C.c_float)],
+    "cmcg_mlp_set_params": [_ctx, _P(C.c_float), _P(C.c_float)],
+    "cmcg_mlp_harvest": [_ctx, C.c_int],
+    "cmcg_mlp_rows": [_ctx, _P(C.c_int64), _P(C.c_float)],
+    "cmcg_mlp_fit_norm": [_ctx],
+    "cmcg_mlp_loss_grad": [_ctx, _P(C.c_float), C.c_int64, _P(C.c_double), _P(C.c_float)],
+    "cmcg_mlp_train": [_ctx, C.c_int64, C.c_int64, C.c_float, C.c_uint64, _P(C.c_double)],
+    "cmcg_mlp_predict": [_ctx, _P(C.c_float), C.c_int64, _P(C.c_float)],
+    "cmcg_set_learned_brackets": [_ctx, C.c_int, C.c_double],
 }
+MLP_NPARAM, MLP_NNORM = 4738, 16
 for _name, _args in _sigs.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args
@@ -384,3 +395,59 @@ class Context:
 
     def timer_reset(self):
         lib.cmcg_timer_reset(self.h)
+
+    # ---------------------------------------------------------- learned brackets
+    def mlp_init(self, seed=1, max_rows=1 << 22):
+        _chk(lib.cmcg_mlp_init(self.h, int(seed), int(max_rows)), "cmcg_mlp_init")
+
+    def mlp_params(self):
+        th = np.zeros(MLP_NPARAM, np.float32)
+        nr = np.zeros(MLP_NNORM, np.float32)
+        _chk(lib.cmcg_mlp_get_params(self.h, _ptr(th, C.c_float), _ptr(nr, C.c_float)), "cmcg_mlp_get_params")
+        return th, nr
+
+    def mlp_set_params(self, theta, norm):
+        th = np.ascontiguousarray(theta, np.float32)
+        nr = np.ascontiguousarray(norm, np.float32)
+        assert th.size == MLP_NPARAM and nr.size == MLP_NNORM
+        _chk(lib.cmcg_mlp_set_params(self.h, _ptr(th, C.c_float), _ptr(nr, C.c_float)), "cmcg_mlp_set_params")
+
+    def mlp_harvest(self, on=True):
+        _chk(lib.cmcg_mlp_harvest(self.h, int(on)), "cmcg_mlp_harvest")
+
+    def mlp_rows(self, copy=True):
+        n = C.c_int64()
+        _chk(lib.cmcg_mlp_rows(self.h, C.byref(n), None), "cmcg_mlp_rows")
+        if not copy:
+            return n.value
+        rows = np.zeros((n.value, 8), np.float32)
+        if n.value:
+            _chk(lib.cmcg_mlp_rows(self.h, C.byref(n), _ptr(rows, C.c_float)), "cmcg_mlp_rows")
+        return rows
+
+    def mlp_fit_norm(self):
+        _chk(lib.cmcg_mlp_fit_norm(self.h), "cmcg_mlp_fit_norm")
+
+    def mlp_loss_grad(self, rows):
+        rows = np.ascontiguousarray(rows, np.float32)
+        assert rows.ndim == 2 and rows.shape[1] == 8
+        loss = C.c_double()
+        g = np.zeros(MLP_NPARAM, np.float32)
+        _chk(lib.cmcg_mlp_loss_grad(self.h, _ptr(rows, C.c_float), rows.shape[0], C.byref(loss), _ptr(g, C.c_float)),
+             "cmcg_mlp_loss_grad")
+        return loss.value, g
+
+    def mlp_train(self, iters, batch=4096, lr=1e-3, seed=1):
+        loss = C.c_double()
+        _chk(lib.cmcg_mlp_train(self.h, int(iters), int(batch), float(lr), int(seed), C.byref(loss)), "cmcg_mlp_train")
+        return loss.value
+
+    def mlp_predict(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        assert x.ndim == 2 and x.shape[1] == 6
+        y = np.zeros((x.shape[0], 2), np.float32)
+        _chk(lib.cmcg_mlp_predict(self.h, _ptr(x, C.c_float), x.shape[0], _ptr(y, C.c_float)), "cmcg_mlp_predict")
+        return y
+
+    def set_learned_brackets(self, on=True, margin=0.05):
+        _chk(lib.cmcg_set_learned_brackets(self.h, int(on), float(margin)), "cmcg_set_learned_brackets")

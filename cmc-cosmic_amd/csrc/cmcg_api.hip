@@ -33,6 +33,13 @@ struct cmcg_ctx {
     // host-side copy of the seeded RNG state
     Rng seeded{};
     int64_t last_sort_ties = 0;
+    // learned brackets
+    MlpState mlp{};
+    bool mlp_ready = false, harvest = false, learned = false;
+    float margin = 0.05f;        // half-width of the learned brackets in log r
+    int64_t adam_t = 0;
+    float *io = nullptr;
+    int64_t io_cap = 0;          // floats
 };
 
 namespace {
@@ -154,6 +161,16 @@ struct PhaseTimer {
         }
     }
 };
+
+// a pair of events bracketing one kernel launch, collected like a phase timer
+void kernel_events(cmcg_ctx *c, const char *name, hipEvent_t &a, hipEvent_t &b)
+{
+    a = b = nullptr;
+    if (!c->timing) return;
+    a = get_event(c);
+    b = get_event(c);
+    c->pending.push_back({name, {a, b}});
+}
 
 void collect_timers(cmcg_ctx *c)
 {
@@ -309,7 +326,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.max_waves = (n_cap + 2) / 64 + 2;
     sc.max_blocks = (n_cap + 255) / 256 + 2;
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
-    if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 8)) ||
+    if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
@@ -319,7 +336,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.chain_x, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.chain_g, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.chain_mk, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.defer_list, 65536)) || (e = dalloc(c, &sc.defer_kfirst, 65536)) ||
-        (e = dalloc(c, &sc.defer_partials, 65536 * 8)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
+        (e = dalloc(c, &sc.defer_partials, 65536 * 16)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
         (e = dalloc(c, &sc.esa_idx, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_racc, (size_t)sc.esa_max * 2)) ||
         (e = dalloc(c, &sc.esa_v, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_n, 4)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_wave, (size_t)sc.max_waves)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
         (e = dalloc(c, &c->T, 1))) {
@@ -345,6 +362,7 @@ int cmcg_destroy(cmcg_ctx *c)
     for (auto &p : c->pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
     for (auto e : c->event_pool) hipEventDestroy(e);
     for (void *p : c->allocs) hipFree(p);
+    if (c->io) hipFree(c->io);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return CMCG_OK;
@@ -556,7 +574,11 @@ int cmcg_orbits(cmcg_ctx *c, uint64_t step, cmcg_orbit_stats *stats)
         PhaseTimer t(c, "orbits");
         hipLaunchKernelGGL(set_dt_step_kernel, dim3(1), dim3(1), 0, c->stream, c->s, __builtin_nan(""), (int64_t)step);
         LAUNCH(launch_rng_bases(c->T, c->s, c->sc, c->stream));
-        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream));
+        hipEvent_t ka = nullptr, kb = nullptr;
+        kernel_events(c, "orbits_kernel", ka, kb);
+        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream, c->learned ? c->mlp.theta : nullptr,
+                             c->mlp.norm, c->margin, ka, kb));
+        if (c->harvest) LAUNCH(launch_mlp_harvest(c->f, c->s, c->kp, c->mlp, c->n_ub, c->stream));
     }
     DevScalars h;
     if (int e = read_scalars(c, h)) return e;
@@ -714,7 +736,11 @@ int cmcg_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
     }
     {
         PhaseTimer t(c, "orbits");
-        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream));
+        hipEvent_t ka = nullptr, kb = nullptr;
+        kernel_events(c, "orbits_kernel", ka, kb);
+        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream, c->learned ? c->mlp.theta : nullptr,
+                             c->mlp.norm, c->margin, ka, kb));
+        if (c->harvest) LAUNCH(launch_mlp_harvest(c->f, c->s, c->kp, c->mlp, nub, c->stream));
     }
     {
         PhaseTimer t(c, "glue_pre_sort");
@@ -770,6 +796,176 @@ int cmcg_rng_draws(cmcg_ctx *c, int phase, uint64_t step, int64_t first, int64_t
         rc = CMCG_E_HIP;
     hipFree(d);
     return rc;
+}
+
+// ------------------------------------------------------------------ learned brackets
+static uint64_t splitmix64(uint64_t &x)
+{
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+static int io_reserve(cmcg_ctx *c, int64_t floats)
+{
+    if (floats <= c->io_cap) return CMCG_OK;
+    if (c->io) { hipFree(c->io); c->io = nullptr; c->io_cap = 0; }
+    if (hipMalloc((void **)&c->io, (size_t)floats * sizeof(float)) != hipSuccess) return CMCG_E_NOMEM;
+    c->io_cap = floats;
+    return CMCG_OK;
+}
+
+int cmcg_mlp_init(cmcg_ctx *c, uint64_t seed, int64_t max_rows)
+{
+    if (!c || max_rows < 1) return CMCG_E_ARG;
+    MlpState &m = c->mlp;
+    if (!c->mlp_ready) {
+        int e;
+        if ((e = dalloc(c, &m.theta, MLP_NPARAM)) || (e = dalloc(c, &m.grad, MLP_NPARAM)) ||
+            (e = dalloc(c, &m.m, MLP_NPARAM)) || (e = dalloc(c, &m.v, MLP_NPARAM)) || (e = dalloc(c, &m.norm, 1)) ||
+            (e = dalloc(c, &m.gpart, (size_t)MLP_MAX_BLOCKS * MLP_NPARAM)) || (e = dalloc(c, &m.lpart, MLP_MAX_BLOCKS)) ||
+            (e = dalloc(c, &m.loss, 1)) || (e = dalloc(c, &m.rows, (size_t)max_rows * 8)) ||
+            (e = dalloc(c, &m.counter, 1)) || (e = dalloc(c, &m.colsum, 256 * 16)))
+            return e;
+        m.cap = max_rows;
+        c->mlp_ready = true;
+    } else if (max_rows > m.cap) {
+        return CMCG_E_CAPACITY;
+    }
+    std::vector<float> th(MLP_NPARAM);
+    uint64_t st = seed;
+    auto uni = [&](double a) { return (float)(((splitmix64(st) >> 11) * 0x1.0p-53 * 2.0 - 1.0) * a); };
+    const int fan[6][2] = {{MLP_NIN, MLP_NH * MLP_NIN}, {MLP_NIN, MLP_NH}, {MLP_NH, MLP_NH * MLP_NH},
+                           {MLP_NH, MLP_NH}, {MLP_NH, MLP_NOUT * MLP_NH}, {MLP_NH, MLP_NOUT}};
+    int p = 0;
+    for (auto &fb : fan)
+        for (int i = 0; i < fb[1]; i++) th[p++] = uni(1.0 / sqrt((double)fb[0]));
+    MlpNorm nrm;
+    for (int i = 0; i < MLP_NIN; i++) { nrm.xm[i] = 0.f; nrm.xs[i] = 1.f; }
+    for (int i = 0; i < MLP_NOUT; i++) { nrm.ym[i] = 0.f; nrm.ys[i] = 1.f; }
+    std::vector<float> zero(MLP_NPARAM, 0.f);
+    unsigned long long z0 = 0;
+    H2D(m.theta, th.data(), MLP_NPARAM);
+    H2D(m.m, zero.data(), MLP_NPARAM);
+    H2D(m.v, zero.data(), MLP_NPARAM);
+    H2D(m.norm, &nrm, 1);
+    H2D(m.counter, &z0, 1);
+    c->adam_t = 0;
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+int cmcg_mlp_get_params(cmcg_ctx *c, float *theta, float *norm)
+{
+    if (!c || !c->mlp_ready) return CMCG_E_STATE;
+    D2H(theta, c->mlp.theta, MLP_NPARAM);
+    D2H(norm, (const float *)c->mlp.norm, CMCG_MLP_NNORM);
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+int cmcg_mlp_set_params(cmcg_ctx *c, const float *theta, const float *norm)
+{
+    if (!c || !c->mlp_ready) return CMCG_E_STATE;
+    H2D(c->mlp.theta, theta, MLP_NPARAM);
+    H2D((float *)c->mlp.norm, norm, CMCG_MLP_NNORM);
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+int cmcg_mlp_harvest(cmcg_ctx *c, int on)
+{
+    if (!c) return CMCG_E_ARG;
+    if (on && !c->mlp_ready) return CMCG_E_STATE;
+    c->harvest = on != 0;
+    return CMCG_OK;
+}
+
+static int mlp_count(cmcg_ctx *c, int64_t *count)
+{
+    unsigned long long n = 0;
+    if (hipMemcpyAsync(&n, c->mlp.counter, sizeof(n), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        sync(c) != hipSuccess)
+        return CMCG_E_HIP;
+    *count = (int64_t)n < c->mlp.cap ? (int64_t)n : c->mlp.cap;
+    return CMCG_OK;
+}
+
+int cmcg_mlp_rows(cmcg_ctx *c, int64_t *count, float *rows)
+{
+    if (!c || !count) return CMCG_E_ARG;
+    if (!c->mlp_ready) return CMCG_E_STATE;
+    if (int e = mlp_count(c, count)) return e;
+    D2H(rows, c->mlp.rows, *count * 8);
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+int cmcg_mlp_fit_norm(cmcg_ctx *c)
+{
+    if (!c || !c->mlp_ready) return CMCG_E_STATE;
+    int64_t n;
+    if (int e = mlp_count(c, &n)) return e;
+    if (n < 2) return CMCG_E_STATE;
+    LAUNCH(launch_mlp_fit_norm(c->mlp, n, c->stream));
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+int cmcg_mlp_loss_grad(cmcg_ctx *c, const float *rows, int64_t n, double *loss, float *grad)
+{
+    if (!c || !rows || n < 1) return CMCG_E_ARG;
+    if (!c->mlp_ready) return CMCG_E_STATE;
+    if (int e = io_reserve(c, 8 * n)) return e;
+    H2D(c->io, rows, 8 * n);
+    LAUNCH(launch_mlp_loss_grad(c->mlp, c->io, n, 0, n, 1, 0, c->stream));
+    D2H(loss, c->mlp.loss, 1);
+    D2H(grad, c->mlp.grad, MLP_NPARAM);
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+static uint64_t gcd64(uint64_t a, uint64_t b)
+{
+    while (b) { uint64_t t = a % b; a = b; b = t; }
+    return a;
+}
+
+int cmcg_mlp_train(cmcg_ctx *c, int64_t iters, int64_t batch, float lr, uint64_t seed, double *loss)
+{
+    if (!c || iters < 0 || batch < 1 || !(lr > 0.f)) return CMCG_E_ARG;
+    if (!c->mlp_ready) return CMCG_E_STATE;
+    int64_t n;
+    if (int e = mlp_count(c, &n)) return e;
+    if (n < 1) return CMCG_E_STATE;
+    uint64_t st = seed;
+    uint64_t mult = splitmix64(st) % (uint64_t)n;
+    while (mult == 0 || gcd64(mult, (uint64_t)n) != 1) mult = (mult + 1) % (uint64_t)n + (n == 1);
+    uint64_t off = splitmix64(st) % (uint64_t)n;
+    {
+        PhaseTimer t(c, "mlp_train");
+        for (int64_t it = 0; it < iters; it++) {
+            LAUNCH(launch_mlp_loss_grad(c->mlp, c->mlp.rows, n, (it * batch) % n, batch, mult, off, c->stream));
+            LAUNCH(launch_mlp_adam(c->mlp, ++c->adam_t, lr, c->stream));
+        }
+    }
+    D2H(loss, c->mlp.loss, 1);
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+int cmcg_mlp_predict(cmcg_ctx *c, const float *x, int64_t n, float *y)
+{
+    if (!c || !x || !y || n < 1) return CMCG_E_ARG;
+    if (!c->mlp_ready) return CMCG_E_STATE;
+    if (int e = io_reserve(c, 8 * n)) return e;
+    H2D(c->io, x, MLP_NIN * n);
+    LAUNCH(launch_mlp_predict(c->mlp, c->io, n, c->io + MLP_NIN * n, c->stream));
+    D2H(y, c->io + MLP_NIN * n, MLP_NOUT * n);
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+int cmcg_set_learned_brackets(cmcg_ctx *c, int on, double margin)
+{
+    if (!c || (on && !(margin > 0.0 && margin < 10.0))) return CMCG_E_ARG;
+    if (on && !c->mlp_ready) return CMCG_E_STATE;
+    c->learned = on != 0;
+    if (on) c->margin = (float)margin;
+    return CMCG_OK;
 }
 
 int cmcg_timer_enable(cmcg_ctx *c, int on)

@@ -335,22 +335,142 @@ __device__ __noinline__ double find_root_vr(const double *__restrict__ r, const 
     return apsis;
 }
 
-struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; int brent; int linscan; int err; };
+// Top levels of the sorted-potential search tree staged in LDS: cr[s] = r[k(s)],
+// cphi[s] = phi[k(s)] with k(s) = min(s * stride, n + 1), s = 0..cs (k(cs) = n + 1).
+struct Coarse {
+    const double *cr, *cphi;
+    int64_t stride, n1;
+    int cs;
+    // learned bracket proposal: predicted log r_peri, log r_apo and the half-width
+    bool hint;
+    float lrp, lra, margin;
+};
+
+__device__ __forceinline__ int64_t coarse_k(const Coarse &c, int s)
+{
+    int64_t k = (int64_t)s * c.stride;
+    return k < c.n1 ? k : c.n1;
+}
+
+// Index near radius x: the LDS sample interval holding x, then linear interpolation
+// in log r inside it (r is sorted; only an estimate, every use is verified).
+__device__ __forceinline__ int64_t coarse_index_estimate(const Coarse &c, double x)
+{
+    int a = 0, b = c.cs;
+    while (a < b) {
+        int m = (a + b + 1) >> 1;
+        if (c.cr[m] < x) a = m;
+        else b = m - 1;
+    }
+    if (a >= c.cs) return c.n1;
+    int64_t ka = coarse_k(c, a), kb = coarse_k(c, a + 1);
+    float la = __logf((float)c.cr[a]), lb = __logf((float)c.cr[a + 1]), lx = __logf((float)x);
+    float fr = lb > la ? (lx - la) / (lb - la) : 0.f;
+    fr = fr >= 0.f ? (fr > 1.f ? 1.f : fr) : 0.f;      // also maps NaN to 0
+    return ka + (int64_t)(fr * (float)(kb - ka));
+}
+
+// potential(x) with the index search started from the LDS samples: the result is the
+// unique largest k with r[k] < x, so any bracket [lo, hi] with r[lo] < x <= r[hi]
+// gives the same index as the full-range search.
+__device__ __forceinline__ double potential_at_coarse(const double *__restrict__ r, const double *__restrict__ phi,
+                                                      int64_t n_max, double x, int64_t lo, int64_t hi,
+                                                      const Coarse &c, bool *bad)
+{
+    if (x == 0.0 || x < r[1]) return potential_at(r, phi, n_max, x, lo, hi, bad);
+    int a = 0, b = c.cs;                    // largest s with cr[s] < x (cr[0] = r[0] < x)
+    while (a < b) {
+        int m = (a + b + 1) >> 1;
+        if (c.cr[m] < x) a = m;
+        else b = m - 1;
+    }
+    if (a < c.cs) {
+        int64_t ka = coarse_k(c, a), kb = coarse_k(c, a + 1);
+        if (ka > lo) lo = ka;
+        if (kb < hi) hi = kb;
+    }
+    return potential_at(r, phi, n_max, x, lo, hi, bad);
+}
+
+struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; int brent; int linscan; int err;
+                 int hint_hits, hint_misses; };
 
 // cmc_orbit.c:413-540 calc_orbit_new after get_positive_Q_index has produced ktemp
 // (an index with Q > 0; ktemp < 0 means "no positive Q": circular orbit).
 __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r, const double *__restrict__ phi,
                                                    int64_t n_max, int64_t j, double rj, double mj,
-                                                   const KParams &P, double E, double J, int64_t ktemp)
+                                                   const KParams &P, double E, double J, int64_t ktemp,
+                                                   const Coarse *C = nullptr)
 {
     OrbitRS o;
-    o.brent = 0; o.linscan = 0; o.err = 0;
+    o.brent = 0; o.linscan = 0; o.err = 0; o.hint_hits = 0; o.hint_misses = 0;
     if (ktemp < 0) {
         o.rp = o.ra = rj; o.kmin = o.kmax = j; o.dQdrp = o.dQdra = 0.0; o.circular = 1;
         return o;
     }
-    int64_t kmin = find_zero_Q(r, phi, rj, mj, P.inv_nstar, 0, ktemp, E, J);
-    int64_t kmax = find_zero_Q(r, phi, rj, mj, P.inv_nstar, ktemp, n_max + 1, E, J);
+    // Q is negative at the centre, positive on one interval around ktemp and negative
+    // far out (r^3 dPhi/dr = r M(r) increases), so the index bisection's result is
+    // the unique sign change of its bracket; the LDS samples narrow both brackets
+    // to one sample interval before the bisection in global memory.
+    int64_t kp_lo = 0, kp_hi = ktemp, ka_lo = ktemp, ka_hi = n_max + 1;
+    bool got_p = false, got_a = false;
+    if (C && C->hint) {
+        // learned proposals: the index range of the predicted apsis +- margin (in
+        // log r), padded by 2, accepted only when Q has the bisection's signs at its ends
+        const double inv = P.inv_nstar;
+        const float ep = __expf(C->margin);
+        int64_t lo = coarse_index_estimate(*C, (double)(__expf(C->lrp) / ep)) - 2;
+        int64_t hi = coarse_index_estimate(*C, (double)(__expf(C->lrp) * ep)) + 2;
+        lo = lo < 0 ? 0 : lo;
+        hi = hi > ktemp ? ktemp : hi;
+        if (lo < hi && function_q(rj, mj, inv, r[lo], phi[lo], E, J) < 0.0 &&
+            (hi == ktemp || function_q(rj, mj, inv, r[hi], phi[hi], E, J) >= 0.0)) {
+            kp_lo = lo; kp_hi = hi; got_p = true; o.hint_hits++;
+        } else {
+            o.hint_misses++;
+        }
+        lo = coarse_index_estimate(*C, (double)(__expf(C->lra) / ep)) - 2;
+        hi = coarse_index_estimate(*C, (double)(__expf(C->lra) * ep)) + 2;
+        lo = lo < ktemp ? ktemp : lo;
+        hi = hi > n_max + 1 ? n_max + 1 : hi;
+        if (lo < hi && (lo == ktemp || function_q(rj, mj, inv, r[lo], phi[lo], E, J) > 0.0) &&
+            !(function_q(rj, mj, inv, r[hi], phi[hi], E, J) > 0.0)) {
+            ka_lo = lo; ka_hi = hi; got_a = true; o.hint_hits++;
+        } else {
+            o.hint_misses++;
+        }
+    }
+    if (C) {
+        const double inv = P.inv_nstar;
+        if (!got_p && ktemp >= 1 && function_q(rj, mj, inv, C->cr[0], C->cphi[0], E, J) < 0.0) {
+            int a = 0, b = (int)((ktemp - 1) / C->stride);
+            const int top = b;
+            while (a < b) {
+                int m = (a + b + 1) >> 1;
+                if (function_q(rj, mj, inv, C->cr[m], C->cphi[m], E, J) < 0.0) a = m;
+                else b = m - 1;
+            }
+            kp_lo = coarse_k(*C, a);
+            kp_hi = a < top ? coarse_k(*C, a + 1) : ktemp;
+        }
+        const int sb = (int)(ktemp / C->stride) + 1;
+        if (!got_a && sb <= C->cs && !(function_q(rj, mj, inv, C->cr[C->cs], C->cphi[C->cs], E, J) > 0.0)) {
+            if (!(function_q(rj, mj, inv, C->cr[sb], C->cphi[sb], E, J) > 0.0)) {
+                ka_hi = coarse_k(*C, sb);
+            } else {
+                int a = sb, b = C->cs - 1;
+                while (a < b) {
+                    int m = (a + b + 1) >> 1;
+                    if (function_q(rj, mj, inv, C->cr[m], C->cphi[m], E, J) > 0.0) a = m;
+                    else b = m - 1;
+                }
+                ka_lo = coarse_k(*C, a);
+                ka_hi = coarse_k(*C, a + 1);
+            }
+        }
+    }
+    int64_t kmin = find_zero_Q(r, phi, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
+    int64_t kmax = find_zero_Q(r, phi, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
     if (kmin < 0 || kmax < 0) {
         o.err = CMCG_E_BRACKET;
         o.rp = o.ra = rj; o.kmin = o.kmax = j; o.dQdrp = o.dQdra = 0.0; o.circular = 1;

@@ -1,0 +1,342 @@
+// Learned apsides brackets: a 6 -> 64 -> 64 -> 2 SiLU network (FP32) that predicts
+// (log r_peri, log r_apo) of a star from (E, J, log r, phi(r), log(r^2 dphi/dr), m).
+// The prediction only proposes the starting bracket of find_zero_Q
+// (src/cmc/cmc_nr.c:316-383); the exact bisection decides the index and the full
+// bracket is used whenever the proposal does not bracket the sign change.
+//
+// Training data are harvested from the method itself after each orbit phase
+// (every sampled star's features and its exact apsides); training is minibatch
+// Adam with the forward and backward passes below.
+//
+// Parameters: one flat float vector W1[64][6], b1[64], W2[64][64], b2[64],
+// W3[2][64], b3[2]; inputs and targets are standardised with per-column mean and
+// standard deviation fitted on the harvested rows (MlpNorm).
+#include "kernels.hpp"
+
+namespace {
+constexpr int NIN = MLP_NIN, NH = MLP_NH, NOUT = MLP_NOUT;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NIN, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NOUT * NH;
+static_assert(OFF_B3 + NOUT == MLP_NPARAM, "parameter layout");
+constexpr int TILE = 64;            // samples per tile
+constexpr int FT = 256;             // threads per training block
+
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z)
+{
+    float s = sigm(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// row r of the training set for minibatch position i: a multiplicative shuffle
+__device__ __forceinline__ int64_t shuffled(int64_t i, int64_t count, uint64_t mult, uint64_t off)
+{
+    return (int64_t)(((unsigned __int128)(uint64_t)i * mult + off) % (uint64_t)count);
+}
+
+// One block: tiles blockIdx.x, blockIdx.x + gridDim.x, ... of the batch.  Writes the
+// block's gradient partial (MLP_NPARAM floats) and its loss partial.
+__global__ __launch_bounds__(FT) void mlp_fwdbwd_kernel(const float *__restrict__ theta, const MlpNorm *__restrict__ nrm,
+                                                        const float *__restrict__ rows, int64_t count, int64_t first,
+                                                        int64_t batch, uint64_t mult, uint64_t off, float inv_b,
+                                                        float *__restrict__ gpart, double *__restrict__ lpart)
+{
+    __shared__ float W1T[NIN][NH], B1[NH], W2[NH][NH], W2T[NH][NH], B2[NH], W3[NOUT][NH], B3[NOUT];
+    __shared__ float X[TILE][NIN], T[TILE][NOUT], D3[TILE][NOUT];
+    __shared__ float Z1[TILE][NH], H1[TILE][NH], Z2[TILE][NH], H2[TILE][NH];
+    __shared__ float xm[NIN], xs[NIN], ym[NOUT], ys[NOUT];
+    const int t = threadIdx.x;
+    for (int e = t; e < NH * NIN; e += FT) W1T[e % NIN][e / NIN] = theta[OFF_W1 + e];
+    for (int e = t; e < NH * NH; e += FT) {
+        float w = theta[OFF_W2 + e];
+        W2[e / NH][e % NH] = w;
+        W2T[e % NH][e / NH] = w;
+    }
+    for (int e = t; e < NOUT * NH; e += FT) W3[e / NH][e % NH] = theta[OFF_W3 + e];
+    if (t < NH) { B1[t] = theta[OFF_B1 + t]; B2[t] = theta[OFF_B2 + t]; }
+    if (t < NOUT) { B3[t] = theta[OFF_B3 + t]; ym[t] = nrm->ym[t]; ys[t] = nrm->ys[t]; }
+    if (t < NIN) { xm[t] = nrm->xm[t]; xs[t] = nrm->xs[t]; }
+    // gradient accumulators owned by this thread
+    float gW2[16], gW1[2] = {0.f, 0.f}, gB = 0.f, gW3 = 0.f, gB3 = 0.f;
+    for (int q = 0; q < 16; q++) gW2[q] = 0.f;
+    double loss = 0.0;
+    __syncthreads();
+    const int64_t ntiles = (batch + TILE - 1) / TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        // 1. inputs and targets (standardised); samples past the batch get zero weight
+        if (t < TILE) {
+            int64_t i = tile * TILE + t;
+            bool ok = i < batch;
+            const float *row = rows + 8 * shuffled(first + (ok ? i : 0), count, mult, off);
+            for (int c = 0; c < NIN; c++) X[t][c] = ok ? (row[c] - xm[c]) / xs[c] : 0.f;
+            for (int c = 0; c < NOUT; c++) T[t][c] = ok ? (row[NIN + c] - ym[c]) / ys[c] : 0.f;
+            D3[t][0] = ok ? 1.f : 0.f;     // validity, replaced by the output error below
+        }
+        __syncthreads();
+        // 2. layer 1
+        for (int q = 0; q < 16; q++) {
+            int e = t + FT * q, s = e / NH, u = e % NH;
+            float z = B1[u];
+            for (int c = 0; c < NIN; c++) z += W1T[c][u] * X[s][c];
+            Z1[s][u] = z;
+            H1[s][u] = silu(z);
+        }
+        __syncthreads();
+        // 3. layer 2
+        for (int q = 0; q < 16; q++) {
+            int e = t + FT * q, s = e / NH, u = e % NH;
+            float z = B2[u];
+            for (int v = 0; v < NH; v++) z += W2T[v][u] * H1[s][v];
+            Z2[s][u] = z;
+            H2[s][u] = silu(z);
+        }
+        __syncthreads();
+        // 4. output, loss, output error
+        float err = 0.f;
+        if (t < TILE * NOUT) {
+            int s = t / NOUT, o = t % NOUT;
+            float y = B3[o];
+            for (int u = 0; u < NH; u++) y += W3[o][u] * H2[s][u];
+            float r = (y - T[s][o]) * D3[s][0];
+            loss += (double)r * (double)r;
+            err = 2.f * r * inv_b;
+        }
+        __syncthreads();
+        if (t < TILE * NOUT) D3[t / NOUT][t % NOUT] = err;
+        __syncthreads();
+        // 5. layer 3 gradients; error at layer 2 (in place of Z2)
+        if (t >= 128) {
+            int idx = t - 128, o = idx / NH, u = idx % NH;
+            float g = 0.f;
+            for (int s = 0; s < TILE; s++) g += D3[s][o] * H2[s][u];
+            gW3 += g;
+        }
+        if (t < NOUT) {
+            float g = 0.f;
+            for (int s = 0; s < TILE; s++) g += D3[s][t];
+            gB3 += g;
+        }
+        for (int q = 0; q < 16; q++) {
+            int e = t + FT * q, s = e / NH, u = e % NH;
+            float d = 0.f;
+            for (int o = 0; o < NOUT; o++) d += D3[s][o] * W3[o][u];
+            Z2[s][u] = d * dsilu(Z2[s][u]);
+        }
+        __syncthreads();
+        // 6. layer 2 gradients; error at layer 1 (in place of Z1)
+        for (int q = 0; q < 16; q++) {
+            int e = t + FT * q, u = e / NH, v = e % NH;
+            float g = 0.f;
+            for (int s = 0; s < TILE; s++) g += Z2[s][u] * H1[s][v];
+            gW2[q] += g;
+        }
+        if (t >= NH && t < 2 * NH) {
+            float g = 0.f;
+            for (int s = 0; s < TILE; s++) g += Z2[s][t - NH];
+            gB += g;
+        }
+        for (int q = 0; q < 16; q++) {
+            int e = t + FT * q, s = e / NH, v = e % NH;
+            float d = 0.f;
+            for (int u = 0; u < NH; u++) d += Z2[s][u] * W2[u][v];
+            Z1[s][v] = d * dsilu(Z1[s][v]);
+        }
+        __syncthreads();
+        // 7. layer 1 gradients
+        for (int k = 0; k < 2; k++) {
+            int e = t + FT * k;
+            if (e < NH * NIN) {
+                int u = e / NIN, c = e % NIN;
+                float g = 0.f;
+                for (int s = 0; s < TILE; s++) g += Z1[s][u] * X[s][c];
+                gW1[k] += g;
+            }
+        }
+        if (t < NH) {
+            float g = 0.f;
+            for (int s = 0; s < TILE; s++) g += Z1[s][t];
+            gB += g;
+        }
+        __syncthreads();
+    }
+    float *gp = gpart + (int64_t)blockIdx.x * MLP_NPARAM;
+    for (int q = 0; q < 16; q++) gp[OFF_W2 + t + FT * q] = gW2[q];
+    for (int k = 0; k < 2; k++)
+        if (t + FT * k < NH * NIN) gp[OFF_W1 + t + FT * k] = gW1[k];
+    if (t < NH) gp[OFF_B1 + t] = gB;
+    else if (t < 2 * NH) gp[OFF_B2 + t - NH] = gB;
+    if (t >= 128) gp[OFF_W3 + t - 128] = gW3;
+    if (t < NOUT) gp[OFF_B3 + t] = gB3;
+    // block loss: fixed-order reduction
+    __shared__ double lred[FT];
+    lred[t] = loss;
+    __syncthreads();
+    if (t == 0) {
+        double a = 0.0;
+        for (int i = 0; i < FT; i++) a += lred[i];
+        lpart[blockIdx.x] = a;
+    }
+}
+
+// gradient = sum of the block partials in block order; loss likewise (mean over batch)
+__global__ void mlp_reduce_kernel(const float *__restrict__ gpart, const double *__restrict__ lpart, int nblocks,
+                                  float *__restrict__ grad, double *__restrict__ loss, double inv_b)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < MLP_NPARAM) {
+        float a = 0.f;
+        for (int b = 0; b < nblocks; b++) a += gpart[(int64_t)b * MLP_NPARAM + p];
+        grad[p] = a;
+    }
+    if (p == 0) {
+        double a = 0.0;
+        for (int b = 0; b < nblocks; b++) a += lpart[b];
+        *loss = a * inv_b;
+    }
+}
+
+// Adam (Kingma and Ba 2015, Algorithm 1), step t counted from 1
+__global__ void mlp_adam_kernel(float *__restrict__ theta, const float *__restrict__ grad, float *__restrict__ m,
+                                float *__restrict__ v, int64_t step, float lr)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= MLP_NPARAM) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float g = grad[p];
+    float mm = b1 * m[p] + (1.f - b1) * g;
+    float vv = b2 * v[p] + (1.f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    float mh = mm / (1.f - powf(b1, (float)step));
+    float vh = vv / (1.f - powf(b2, (float)step));
+    theta[p] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// per-thread inference; weights are wave-uniform reads
+__global__ void mlp_predict_kernel(const float *__restrict__ theta, const MlpNorm *__restrict__ nrm,
+                                   const float *__restrict__ x, int64_t n, float *__restrict__ y)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float in[NIN], out[NOUT];
+    for (int c = 0; c < NIN; c++) in[c] = x[NIN * i + c];
+    mlp_forward(theta, nrm, in, out);
+    for (int o = 0; o < NOUT; o++) y[NOUT * i + o] = out[o];
+}
+
+// column sums of the harvested rows (mean and variance of every input and target)
+__global__ __launch_bounds__(256) void mlp_colsum_kernel(const float *__restrict__ rows, int64_t count,
+                                                         double *__restrict__ part)
+{
+    __shared__ double lds[16][256];
+    double acc[16];
+    for (int c = 0; c < 16; c++) acc[c] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 8; c++) {
+            double v = rows[8 * i + c];
+            acc[c] += v;
+            acc[8 + c] += v * v;
+        }
+    for (int c = 0; c < 16; c++) lds[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double a = 0.0;
+        for (int k = 0; k < 256; k++) a += lds[threadIdx.x][k];
+        part[16 * blockIdx.x + threadIdx.x] = a;
+    }
+}
+
+__global__ void mlp_norm_kernel(const double *__restrict__ part, int nblocks, int64_t count, MlpNorm *__restrict__ nrm)
+{
+    int c = threadIdx.x;
+    if (c >= 8) return;
+    double s = 0.0, s2 = 0.0;
+    for (int b = 0; b < nblocks; b++) { s += part[16 * b + c]; s2 += part[16 * b + 8 + c]; }
+    double mean = s / (double)count;
+    double var = s2 / (double)count - mean * mean;
+    float sd = var > 1e-24 ? (float)sqrt(var) : 1.f;
+    if (c < NIN) { nrm->xm[c] = (float)mean; nrm->xs[c] = sd; }
+    else { nrm->ym[c - NIN] = (float)mean; nrm->ys[c - NIN] = sd; }
+}
+
+// Training rows from the orbit phase just run: every sampled star (flag 256, set
+// only by the orbit sampler before energy_conservation2) appends its features and
+// exact apsides to the ring buffer.
+__global__ __launch_bounds__(256) void mlp_harvest_kernel(DevFields f, DevScalars *s, KParams P,
+                                                          float *__restrict__ rows, int64_t cap,
+                                                          unsigned long long *__restrict__ counter)
+{
+    int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t n = s->n_max;
+    bool take = j <= n && (f.flags[j] & 256) && !(f.flags[j] & (CMCG_SF_LINSCAN | CMCG_SF_BRENT));
+    float x[NIN];
+    if (take) {
+        double rj = f.r[j], mj = f.m[j];
+        double E = f.E[j] + phi_s_own(rj, mj, P.inv_nstar, rj);
+        mlp_features(f.r, f.phi, n, j, rj, mj, E, f.J[j], x);
+    }
+    uint64_t bal = __ballot(take);
+    if (!bal) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == __ffsll((long long)bal) - 1) base = atomicAdd(counter, (unsigned long long)__popcll(bal));
+    base = __shfl(base, __ffsll((long long)bal) - 1);
+    if (!take) return;
+    int64_t pos = (int64_t)((base + __popcll(bal & ((1ull << lane) - 1ull))) % (unsigned long long)cap);
+    float *row = rows + 8 * pos;
+    for (int c = 0; c < NIN; c++) row[c] = x[c];
+    row[NIN] = logf((float)f.r_peri[j]);
+    row[NIN + 1] = logf((float)f.r_apo[j]);
+}
+} // namespace
+
+int mlp_grid_blocks(int64_t batch)
+{
+    int64_t nt = (batch + TILE - 1) / TILE;
+    return (int)(nt < MLP_MAX_BLOCKS ? (nt > 0 ? nt : 1) : MLP_MAX_BLOCKS);
+}
+
+hipError_t launch_mlp_loss_grad(const MlpState &m, const float *rows, int64_t count, int64_t first, int64_t batch,
+                                uint64_t mult, uint64_t off, hipStream_t st)
+{
+    int nb = mlp_grid_blocks(batch);
+    hipLaunchKernelGGL(mlp_fwdbwd_kernel, dim3(nb), dim3(FT), 0, st, (const float *)m.theta, (const MlpNorm *)m.norm,
+                       rows, count, first, batch, mult, off, 1.0f / (float)batch, m.gpart, m.lpart);
+    hipLaunchKernelGGL(mlp_reduce_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, (const float *)m.gpart,
+                       (const double *)m.lpart, nb, m.grad, m.loss, 1.0 / (double)batch);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(const MlpState &m, int64_t step, float lr, hipStream_t st)
+{
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, m.theta,
+                       (const float *)m.grad, m.m, m.v, step, lr);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_predict(const MlpState &m, const float *x, int64_t n, float *y, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_predict_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       (const float *)m.theta, (const MlpNorm *)m.norm, x, n, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_fit_norm(const MlpState &m, int64_t count, hipStream_t st)
+{
+    if (count <= 0) return hipErrorInvalidValue;
+    int nb = 256;
+    hipLaunchKernelGGL(mlp_colsum_kernel, dim3(nb), dim3(256), 0, st, (const float *)m.rows, count, m.colsum);
+    hipLaunchKernelGGL(mlp_norm_kernel, dim3(1), dim3(64), 0, st, (const double *)m.colsum, nb, count, m.norm);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_harvest(const DevFields &f, DevScalars *s, const KParams &P, const MlpState &m, int64_t n_ub,
+                              hipStream_t st)
+{
+    if (n_ub <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_harvest_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f, s, P, m.rows,
+                       m.cap, m.counter);
+    return hipGetLastError();
+}

@@ -19,32 +19,57 @@
 
 namespace {
 constexpr int ORB_THREADS = 256;
+constexpr int ORB_COARSE = 1024;     // LDS samples of the (r, phi) table per block
 constexpr int64_t MAX_DEFER = 65536;
 
-// Deterministic block reduction of NV doubles (fixed tree order) into partials.
-template <int NV, bool MAXFIRST>
-__device__ void block_reduce_store(double (&v)[NV], double *__restrict__ out)
+// Per-block partials of the orbit phase (fixed-order wave-shuffle tree, then the
+// four waves in order): [0] max_r (max), [1] Eescaped, [2] Jescaped,
+// [3] TidalMassLoss, [4] Etidal, [5..10] star counts by outcome (circular,
+// removed E >= 0, removed r_apo >= Rtidal, destroyed, Brent, linear scan),
+// [11] tries, [12] max tries (max), [13] learned brackets used, [14] learned
+// brackets rejected.
+constexpr int NRED = 15, RED_STRIDE = 16;
+__device__ __forceinline__ bool red_is_max(int i) { return i == 0 || i == 12; }
+
+__device__ void block_reduce_store(double (&v)[NRED], double *__restrict__ out)
 {
-    __shared__ double lds[NV][ORB_THREADS];
-    const int t = threadIdx.x;
-    for (int i = 0; i < NV; i++) lds[i][t] = v[i];
-    __syncthreads();
-    for (int off = ORB_THREADS / 2; off > 0; off >>= 1) {
-        if (t < off) {
-            for (int i = 0; i < NV; i++) {
-                double a = lds[i][t], b = lds[i][t + off];
-                lds[i][t] = (MAXFIRST && i == 0) ? (a >= b ? a : b) : a + b;
-            }
+    __shared__ double wsum[ORB_THREADS / 64][NRED];
+    for (int off = 32; off > 0; off >>= 1)
+        for (int i = 0; i < NRED; i++) {
+            double o = __shfl_xor(v[i], off);
+            v[i] = red_is_max(i) ? (v[i] >= o ? v[i] : o) : v[i] + o;
         }
-        __syncthreads();
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int i = 0; i < NRED; i++) wsum[w][i] = v[i];
+    __syncthreads();
+    if (threadIdx.x < NRED) {
+        const int i = threadIdx.x;
+        double a = wsum[0][i];
+        for (int k = 1; k < ORB_THREADS / 64; k++) a = red_is_max(i) ? (a >= wsum[k][i] ? a : wsum[k][i]) : a + wsum[k][i];
+        out[i] = a;
     }
-    if (t == 0)
-        for (int i = 0; i < NV; i++) out[i] = lds[i][0];
+}
+
+// outcome counters of one star into the partials
+__device__ __forceinline__ void count_star(double (&red)[NRED], int flags, int ntry, int hits, int misses,
+                                           bool count_linscan)
+{
+    red[5] += (flags & CMCG_SF_CIRCULAR) ? 1.0 : 0.0;
+    red[6] += (flags & CMCG_SF_REMOVED_E) ? 1.0 : 0.0;
+    red[7] += (flags & CMCG_SF_REMOVED_RT) ? 1.0 : 0.0;
+    red[8] += (flags & CMCG_SF_DESTROYED) ? 1.0 : 0.0;
+    red[9] += (flags & CMCG_SF_BRENT) ? 1.0 : 0.0;
+    red[10] += (count_linscan && (flags & CMCG_SF_LINSCAN)) ? 1.0 : 0.0;
+    red[11] += (double)ntry;
+    red[12] = red[12] >= (double)ntry ? red[12] : (double)ntry;
+    red[13] += (double)hits;
+    red[14] += (double)misses;
 }
 
 // remove_star (cmc_evolution_thr.c:655-716) contributions: Eescaped, Jescaped,
 // TidalMassLoss, Etidal
-__device__ __forceinline__ void removed(double (&red)[5], double Ej, double J, double mj, const KParams &P)
+__device__ __forceinline__ void removed(double (&red)[NRED], double Ej, double J, double mj, const KParams &P)
 {
     red[1] = Ej * mj / (double)P.n_star;
     red[2] = J * mj / (double)P.n_star;
@@ -54,19 +79,21 @@ __device__ __forceinline__ void removed(double (&red)[5], double Ej, double J, d
 
 // The part of get_positions_loop after the positive-Q index is known
 // (cmc_evolution_thr.c:832-934).
-__device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P, Rng rng,
+__device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P, Rng rng, const Coarse *C,
                             int64_t n, int64_t j, double rj, double mj, double E,
-                            double Ej, double J, int64_t ktemp, int &flags, int &ntry, double (&red)[5])
+                            double Ej, double J, int64_t ktemp, int &flags, int &ntry, int &hits, int &misses,
+                            double (&red)[NRED])
 {
     const double *__restrict__ r = f.r;
     const double *__restrict__ phi = f.phi;
-    OrbitRS o = calc_orbit_from(r, phi, n, j, rj, mj, P, E, J, ktemp);
+    OrbitRS o = calc_orbit_from(r, phi, n, j, rj, mj, P, E, J, ktemp, C);
     if (o.err) dev_set_error(s, o.err, j);
-    if (o.brent) { flags |= CMCG_SF_BRENT; atomicAdd((unsigned long long *)&s->n_brent, 1ull); }
+    hits = o.hint_hits;
+    misses = o.hint_misses;
+    if (o.brent) flags |= CMCG_SF_BRENT;
     if (o.circular) {
         f.rnew[j] = rj; f.vrnew[j] = f.vr[j]; f.vtnew[j] = f.vt[j];
         flags |= CMCG_SF_CIRCULAR;
-        atomicAdd((unsigned long long *)&s->n_circular, 1ull);
         return;
     }
     if (o.ra >= s->Rtidal) {
@@ -74,7 +101,6 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
         f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
         removed(red, Ej, J, mj, P);
         flags |= CMCG_SF_REMOVED_RT;
-        atomicAdd((unsigned long long *)&s->n_removed_rt, 1ull);
         return;
     }
     const double rmin = o.rp, rmax = o.ra;
@@ -95,7 +121,7 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
         int64_t lo = lo0, hi = hi0;
         if (!(r[lo] < rr)) lo = 1;
         if (!(rr <= r[hi])) hi = n + 1;
-        potr = potential_at(r, phi, n, rr, lo, hi, &bad);
+        potr = C ? potential_at_coarse(r, phi, n, rr, lo, hi, *C, &bad) : potential_at(r, phi, n, rr, lo, hi, &bad);
         double pot = potr + phi_s_own(rj, mj, P.inv_nstar, rr);
         double drds = 0.25 * (rmax - rmin) * (3.0 - 3.0 * s0 * s0);
         double Q = 2.0 * E - 2.0 * pot - J * J / rr / rr;
@@ -120,29 +146,41 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
     f.pot_rnew[j] = potr;
     flags |= 256;      // sampled: pot_rnew holds potential(rnew)
     red[0] = rr;
-    atomicAdd((unsigned long long *)&s->tries_total, (unsigned long long)k);
-    atomicMax((unsigned long long *)&s->tries_max, (unsigned long long)k);
 }
 
 // Slot j draws from offset j 2^20; the wave's first slot starts at the per-wave state
-// and lane l jumps l 2^20 (l < 2^6) through the LDS tables.
+// and lane l jumps l 2^20 (l < 2^6) through the LDS tables.  With learned brackets
+// the network's prediction proposes both find_zero_Q brackets.
 __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
                                                              const RngTable *__restrict__ T,
                                                              const Rng *__restrict__ wave_base,
                                                              double *__restrict__ partials,
-                                                             int64_t *__restrict__ defer_list)
+                                                             int64_t *__restrict__ defer_list,
+                                                             const float *__restrict__ theta,
+                                                             const MlpNorm *__restrict__ nrm, float margin)
 {
     __shared__ NibbleTables<6> lt;
-    build_nibble_tables<6>(T, 20, lt);
-    const int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double cr[ORB_COARSE + 1], cphi[ORB_COARSE + 1];
     const int64_t n = s->n_max;
-    // [0] max_r, [1] Eescaped, [2] Jescaped, [3] TidalMassLoss, [4] Etidal
-    double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    Coarse C;
+    C.cr = cr; C.cphi = cphi; C.cs = ORB_COARSE; C.n1 = n + 1;
+    C.stride = (n + 1 + ORB_COARSE - 1) / ORB_COARSE;
+    if (C.stride < 1) C.stride = 1;
+    C.hint = false; C.lrp = C.lra = 0.f; C.margin = margin;
+    for (int sI = threadIdx.x; sI <= ORB_COARSE; sI += blockDim.x) {
+        int64_t k = coarse_k(C, sI);
+        cr[sI] = f.r[k];
+        cphi[sI] = f.phi[k];
+    }
+    build_nibble_tables<6>(T, 20, lt);      // ends with a block barrier
+    const int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double red[NRED];
+    for (int i = 0; i < NRED; i++) red[i] = 0.0;
     if (j <= n) {
         const double rj = f.r[j], mj = f.m[j];
         const double Ej = f.E[j], J = f.J[j];
         const double E = Ej + phi_s_own(rj, mj, P.inv_nstar, rj);    // cmc_evolution_thr.c:806
-        int flags = 0, ntry = 0;
+        int flags = 0, ntry = 0, hits = 0, misses = 0;
         if (mj < CMCG_ZERO) {
             // destroy_obj (cmc_dynamics_helper.c:131-153) + remove_star_center
             f.vr[j] = 0.0; f.vt[j] = 0.0; f.E[j] = 0.0; f.J[j] = 0.0; f.id[j] = 0;
@@ -150,12 +188,10 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
             f.rnew[j] = CMCG_SF_INFINITY; f.m[j] = 2.2250738585072014e-308;
             f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
             flags = CMCG_SF_DESTROYED;
-            atomicAdd((unsigned long long *)&s->n_destroyed, 1ull);
         } else if (E >= 0.0) {
             f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
             removed(red, Ej, J, mj, P);
             flags = CMCG_SF_REMOVED_E;
-            atomicAdd((unsigned long long *)&s->n_removed_E, 1ull);
         } else {
             double Qown = function_q(rj, mj, P.inv_nstar, rj, f.phi[j], E, J);
             if (Qown <= 0.0) {
@@ -164,16 +200,24 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
                 unsigned long long slot = atomicAdd((unsigned long long *)&s->n_defer, 1ull);
                 if ((int64_t)slot < MAX_DEFER) defer_list[slot] = j;
                 else dev_set_error(s, CMCG_E_STATE, j);
-                atomicAdd((unsigned long long *)&s->n_linscan, 1ull);
             } else {
+                if (theta) {
+                    float x[MLP_NIN], y[MLP_NOUT];
+                    mlp_features(f.r, f.phi, n, j, rj, mj, E, J, x);
+                    mlp_forward(theta, nrm, x, y);
+                    C.hint = true;
+                    C.lrp = y[0];
+                    C.lra = y[1];
+                }
                 Rng rng = rng_jump_lds<6>(lt, wave_base[(j - 1) >> 6], (uint32_t)((j - 1) & 63));
-                sample_star(f, s, P, rng, n, j, rj, mj, E, Ej, J, j, flags, ntry, red);
+                sample_star(f, s, P, rng, &C, n, j, rj, mj, E, Ej, J, j, flags, ntry, hits, misses, red);
             }
         }
         f.flags[j] = flags;
         f.ntry[j] = ntry;
+        count_star(red, flags, ntry, hits, misses, true);
     }
-    block_reduce_store<5, true>(red, partials + 8 * blockIdx.x);
+    block_reduce_store(red, partials + RED_STRIDE * blockIdx.x);
 }
 
 // Deferred stars: order the list (for a deterministic reduction order) and reset
@@ -230,13 +274,15 @@ __global__ __launch_bounds__(ORB_THREADS) void defer_tail_kernel(DevFields f, De
         const double E = Ej + phi_s_own(rj, mj, P.inv_nstar, rj);
         int64_t kt = kfirst[d];
         if (kt >= n) kt = -1;                      // cmc_orbit.c:52-56: circular
-        double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        int flags = CMCG_SF_LINSCAN, ntry = 0;
-        sample_star(f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), n, j, rj, mj, E, Ej, J, kt, flags,
-                    ntry, red);
+        double red[NRED];
+        for (int i = 0; i < NRED; i++) red[i] = 0.0;
+        int flags = CMCG_SF_LINSCAN, ntry = 0, hits = 0, misses = 0;
+        sample_star(f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), nullptr, n, j, rj, mj, E, Ej, J, kt,
+                    flags, ntry, hits, misses, red);
         f.flags[j] = flags;
         f.ntry[j] = ntry;
-        for (int i = 0; i < 5; i++) dpartials[8 * d + i] = red[i];
+        count_star(red, flags, ntry, hits, misses, false);
+        for (int i = 0; i < NRED; i++) dpartials[RED_STRIDE * d + i] = red[i];
     }
 }
 
@@ -244,8 +290,6 @@ __global__ void orbits_begin_kernel(DevScalars *s)
 {
     // new_orbits_calculate (cmc_utils.c:2198) and counters of this call
     s->OldTidalMassLoss = s->TidalMassLoss_old;
-    s->n_circular = s->n_removed_E = s->n_removed_rt = s->n_destroyed = s->n_brent = s->n_linscan = 0;
-    s->tries_total = s->tries_max = s->n_bracket_hit = s->n_bracket_fallback = 0;
     s->n_defer = 0;
 }
 
@@ -254,37 +298,48 @@ __global__ void orbits_begin_kernel(DevScalars *s)
 __global__ void orbits_end_kernel(DevScalars *s, const double *__restrict__ partials, int64_t nblocks,
                                   const double *__restrict__ dpartials)
 {
-    __shared__ double lds[5][256];
+    __shared__ double lds[NRED][256];
     const int t = threadIdx.x;
-    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[NRED];
+    for (int i = 0; i < NRED; i++) acc[i] = 0.0;
     for (int64_t b = t; b < nblocks; b += 256) {
-        const double *p = partials + 8 * b;
-        acc[0] = acc[0] >= p[0] ? acc[0] : p[0];
-        for (int i = 1; i < 5; i++) acc[i] += p[i];
+        const double *p = partials + RED_STRIDE * b;
+        for (int i = 0; i < NRED; i++) acc[i] = red_is_max(i) ? (acc[i] >= p[i] ? acc[i] : p[i]) : acc[i] + p[i];
     }
-    for (int i = 0; i < 5; i++) lds[i][t] = acc[i];
+    for (int i = 0; i < NRED; i++) lds[i][t] = acc[i];
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if (t < off) {
-            lds[0][t] = lds[0][t] >= lds[0][t + off] ? lds[0][t] : lds[0][t + off];
-            for (int i = 1; i < 5; i++) lds[i][t] += lds[i][t + off];
-        }
+        if (t < off)
+            for (int i = 0; i < NRED; i++) {
+                double a = lds[i][t], b = lds[i][t + off];
+                lds[i][t] = red_is_max(i) ? (a >= b ? a : b) : a + b;
+            }
         __syncthreads();
     }
     if (t == 0) {
-        double m = lds[0][0], e1 = lds[1][0], e2 = lds[2][0], e3 = lds[3][0], e4 = lds[4][0];
+        double v[NRED];
+        for (int i = 0; i < NRED; i++) v[i] = lds[i][0];
         int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
         for (int64_t d = 0; d < nd; d++) {
-            const double *p = dpartials + 8 * d;
-            m = m >= p[0] ? m : p[0];
-            e1 += p[1]; e2 += p[2]; e3 += p[3]; e4 += p[4];
+            const double *p = dpartials + RED_STRIDE * d;
+            for (int i = 0; i < NRED; i++) v[i] = red_is_max(i) ? (v[i] >= p[i] ? v[i] : p[i]) : v[i] + p[i];
         }
-        s->max_r = m;
-        s->Eescaped += e1;
-        s->Jescaped += e2;
-        s->TidalMassLoss += e3;
-        s->Etidal += e4;
+        s->max_r = v[0];
+        s->Eescaped += v[1];
+        s->Jescaped += v[2];
+        s->TidalMassLoss += v[3];
+        s->Etidal += v[4];
         s->TidalMassLoss += s->TidalMassLoss_old;    // cmc_utils.c:2210
+        s->n_circular = (int64_t)v[5];
+        s->n_removed_E = (int64_t)v[6];
+        s->n_removed_rt = (int64_t)v[7];
+        s->n_destroyed = (int64_t)v[8];
+        s->n_brent = (int64_t)v[9];
+        s->n_linscan = (int64_t)v[10];
+        s->tries_total = (int64_t)v[11];
+        s->tries_max = (int64_t)v[12];
+        s->n_bracket_hit = (int64_t)v[13];
+        s->n_bracket_fallback = (int64_t)v[14];
     }
 }
 
@@ -323,7 +378,8 @@ int bits_needed(uint64_t x)
 } // namespace
 
 hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
-                         const Scratch &sc, int64_t n_ub, hipStream_t st)
+                         const Scratch &sc, int64_t n_ub, hipStream_t st, const float *theta, const MlpNorm *nrm,
+                         float margin, hipEvent_t ev_a, hipEvent_t ev_b)
 {
     int64_t nblocks = (n_ub + ORB_THREADS - 1) / ORB_THREADS;
     if (nblocks < 1) nblocks = 1;
@@ -331,8 +387,10 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     int jb1 = 20 + bits_needed((uint64_t)n_ub);
     hipLaunchKernelGGL(orbits_begin_kernel, dim3(1), dim3(1), 0, st, s);
     if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
+    if (ev_a) hipEventRecord(ev_a, st);
     hipLaunchKernelGGL(orbits_kernel, dim3((unsigned)nblocks), dim3(ORB_THREADS), 0, st, f, s, P, T,
-                       (const Rng *)sc.rng_wave, sc.partials, sc.defer_list);
+                       (const Rng *)sc.rng_wave, sc.partials, sc.defer_list, theta, nrm, margin);
+    if (ev_b) hipEventRecord(ev_b, st);
     hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
                        sc.defer_kfirst);

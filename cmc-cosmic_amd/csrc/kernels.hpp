@@ -5,6 +5,7 @@
 // step can be enqueued without host round trips.
 #pragma once
 #include "cmcg_internal.hpp"
+#include "mlp.hpp"
 
 struct Scratch {
     // potential chains: per-tile int128 sums and prefixes plus a boundary value
@@ -69,7 +70,9 @@ hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64
                             int ndraw, double *out, hipStream_t st);
 
 hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
-                         const Scratch &sc, int64_t n_ub, hipStream_t st);
+                         const Scratch &sc, int64_t n_ub, hipStream_t st, const float *theta = nullptr,
+                         const MlpNorm *nrm = nullptr, float margin = 0.f, hipEvent_t ev_a = nullptr,
+                         hipEvent_t ev_b = nullptr);
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
                           const double *E, const double *J, double *out4, int64_t *kout, int32_t *circ,
                           hipStream_t st);
@@ -93,3 +96,13 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
                                   int64_t n_ub, hipStream_t st);
 hipError_t launch_compute_energy(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
                                  int64_t n_ub, hipStream_t st);
+
+// learned apsides brackets (k_mlp.hip)
+int mlp_grid_blocks(int64_t batch);
+hipError_t launch_mlp_loss_grad(const MlpState &m, const float *rows, int64_t count, int64_t first, int64_t batch,
+                                uint64_t mult, uint64_t off, hipStream_t st);
+hipError_t launch_mlp_adam(const MlpState &m, int64_t step, float lr, hipStream_t st);
+hipError_t launch_mlp_predict(const MlpState &m, const float *x, int64_t n, float *y, hipStream_t st);
+hipError_t launch_mlp_fit_norm(const MlpState &m, int64_t count, hipStream_t st);
+hipError_t launch_mlp_harvest(const DevFields &f, DevScalars *s, const KParams &P, const MlpState &m, int64_t n_ub,
+                              hipStream_t st);

@@ -190,6 +190,41 @@ int cmcg_timer_enable(cmcg_ctx *ctx, int on);
 int cmcg_timer_get(cmcg_ctx *ctx, const char *name, double *ms_total, int64_t *calls);
 int cmcg_timer_reset(cmcg_ctx *ctx);
 
+/* ------------------------------------------------------------------ learned brackets */
+/* A 6 -> 64 -> 64 -> 2 SiLU network (FP32) predicting (log r_peri, log r_apo) of a star
+ * from (E, J, log r, phi(r), log(r^2 dphi/dr), m).  No counterpart in the reference:
+ * it proposes the starting bracket of find_zero_Q (cmc_nr.c:316-383); the exact index
+ * bisection decides the result and the full bracket is used whenever the proposal
+ * does not bracket the sign change.
+ * Parameters: W1[64][6], b1[64], W2[64][64], b2[64], W3[2][64], b3[2] (4738 floats);
+ * norm: input means[6], input std[6], target means[2], target std[2] (16 floats).
+ * Training rows: 6 inputs then 2 targets (8 floats). */
+#define CMCG_MLP_NPARAM 4738
+#define CMCG_MLP_NNORM 16
+/* Allocate the network (uniform(+-1/sqrt(fan_in)) initial weights from `seed`), its
+ * Adam state and a ring buffer of `max_rows` training rows. */
+int cmcg_mlp_init(cmcg_ctx *ctx, uint64_t seed, int64_t max_rows);
+int cmcg_mlp_get_params(cmcg_ctx *ctx, float *theta, float *norm);
+int cmcg_mlp_set_params(cmcg_ctx *ctx, const float *theta, const float *norm);
+/* on != 0: after every orbit phase, append each sampled star's inputs and exact
+ * apsides to the ring buffer. */
+int cmcg_mlp_harvest(cmcg_ctx *ctx, int on);
+/* Number of stored rows; when rows != NULL also copies them (count x 8 floats). */
+int cmcg_mlp_rows(cmcg_ctx *ctx, int64_t *count, float *rows);
+/* Fit the input/target standardisation to the stored rows. */
+int cmcg_mlp_fit_norm(cmcg_ctx *ctx);
+/* Mean squared error (standardised targets, summed over the two outputs) of the n
+ * given rows and its gradient with respect to the parameters (one batch). */
+int cmcg_mlp_loss_grad(cmcg_ctx *ctx, const float *rows, int64_t n, double *loss, float *grad);
+/* `iters` Adam steps (lr, beta1 0.9, beta2 0.999) on minibatches of `batch` stored rows
+ * in a shuffled order drawn from `seed`; *loss = loss of the last minibatch. */
+int cmcg_mlp_train(cmcg_ctx *ctx, int64_t iters, int64_t batch, float lr, uint64_t seed, double *loss);
+/* y[2n] = predicted (log r_peri, log r_apo) for x[6n] raw inputs. */
+int cmcg_mlp_predict(cmcg_ctx *ctx, const float *x, int64_t n, float *y);
+/* on != 0: the orbit phase starts find_zero_Q from the predicted apsides widened by
+ * +-margin in log r (0 < margin < 10). */
+int cmcg_set_learned_brackets(cmcg_ctx *ctx, int on, double margin);
+
 #ifdef __cplusplus
 }
 #endif

@@ -177,20 +177,28 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
 // set_velocities3 per-star part.  Stars with vnew2 <= 0 ("events") carry excess
 // energy that the following stars absorb (cmc_utils.c:268-310); they are counted
 // per block for an ordered compaction.
+constexpr int SV3_COARSE = 1024;      // LDS samples of r for potential(rOld)
+
 __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
                                                  int64_t *__restrict__ blk_cnt)
 {
     __shared__ int cnt;
+    __shared__ double cr[SV3_COARSE + 1];
     const double q = 0.5;
     int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
+    Coarse C;
+    C.cr = cr; C.cphi = nullptr; C.cs = SV3_COARSE; C.n1 = n + 1; C.hint = false;
+    C.stride = (n + 1 + SV3_COARSE - 1) / SV3_COARSE;
+    if (C.stride < 1) C.stride = 1;
+    for (int sI = threadIdx.x; sI <= SV3_COARSE; sI += blockDim.x) cr[sI] = f.r[coarse_k(C, sI)];
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     if (i <= n) {
         double m = f.m[i], ri = f.r[i];
         double rO = f.rOld[i];
         bool bad = false;
-        double Unewrold = potential_at(f.r, f.phi, n, rO, 1, n + 1, &bad) + phi_s_own(ri, m, P.inv_nstar, rO);
+        double Unewrold = potential_at_coarse(f.r, f.phi, n, rO, 1, n + 1, C, &bad) + phi_s_own(ri, m, P.inv_nstar, rO);
         if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
         double Unewrnew = f.phi[i] + phi_s_own(ri, m, P.inv_nstar, ri);
         double vtold = f.vtold[i], vrold = f.vrold[i];
@@ -312,7 +320,7 @@ __global__ void sv3_spec_kernel(DevFields f, DevScalars *s, const double *__rest
 // and its walk is the true episode.  Otherwise the heads are relinked exactly.
 __global__ void sv3_verify_kernel(DevScalars *s, const int64_t *__restrict__ events,
                                   const int32_t *__restrict__ joins, const int64_t *__restrict__ wend,
-                                  int32_t *__restrict__ relink)
+                                  int64_t *__restrict__ relink)
 {
     int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t ne = s->n_excess;
@@ -320,18 +328,22 @@ __global__ void sv3_verify_kernel(DevScalars *s, const int64_t *__restrict__ eve
     int64_t q = d;
     while (q + 1 < ne && joins[q]) q++;
     int64_t e = wend[d];
-    if (e < 0 || e < events[q] || (q + 1 < ne && e >= events[q + 1])) *relink = 1;
+    if (e < 0 || e < events[q] || (q + 1 < ne && e >= events[q + 1]))
+        atomicMin((unsigned long long *)relink, (unsigned long long)d);
 }
 
 // Exact head chain in event order: the next head is the first event after the end
-// of the current episode.  Runs only when the presumed heads were wrong.
+// of the current episode.  Runs only when a presumed head failed the check, and
+// starts at the first one that did: every presumed head before it passed, so by
+// induction it is a true head and the groups before it are exact.
 __global__ void sv3_relink_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
                                   const int64_t *__restrict__ events, const int64_t *__restrict__ wend,
-                                  int32_t *__restrict__ joins, const int32_t *__restrict__ relink)
+                                  int32_t *__restrict__ joins, const int64_t *__restrict__ relink)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || !*relink) return;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int64_t n = s->n_max, ne = s->n_excess;
-    int64_t d = 0;
+    int64_t d = *relink;
+    if (d >= ne) return;
     while (d < ne) {
         int64_t end = wend[d];
         if (end < 0) end = excess_walk(f, excess_c, events[d], n, false, INT64_MAX);
@@ -352,7 +364,7 @@ __global__ void sv3_heads_kernel(DevFields f, DevScalars *s, const double *__res
     excess_walk(f, excess_c, events[d], s->n_max, true, INT64_MAX);
 }
 
-__global__ void sv3_begin_kernel(DevScalars *s, int32_t *relink) { s->n_excess = 0; *relink = 0; }
+__global__ void sv3_begin_kernel(DevScalars *s, int64_t *relink) { s->n_excess = 0; *relink = INT64_MAX; }
 
 __global__ __launch_bounds__(GT) void energy_kernel(DevFields f, DevScalars *s, KParams P,
                                                     double *__restrict__ partials)
@@ -455,7 +467,7 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
                        (const int32_t *)sc.sv3_merged, (const int64_t *)sc.sv3_wend, sc.sv3_fallback);
     hipLaunchKernelGGL(sv3_relink_kernel, dim3(1), dim3(64), 0, st, f, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.excess_list, (const int64_t *)sc.sv3_wend, sc.sv3_merged,
-                       (const int32_t *)sc.sv3_fallback);
+                       (const int64_t *)sc.sv3_fallback);
     hipLaunchKernelGGL(sv3_heads_kernel, dim3(ge), dim3(GT), 0, st, f, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.excess_list, (const int32_t *)sc.sv3_merged);
     return hipGetLastError();

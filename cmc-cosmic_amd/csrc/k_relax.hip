@@ -124,6 +124,43 @@ __global__ void timestep_finalize_kernel(DevScalars *s)
 
 // calc_sigma_r at one rank: sigma over the clamped window [si-p, si+p-1]; the
 // reference slides running sums, here each window is summed directly.
+// The same window sums with the block's range [first - 2p, last + 2p] staged in
+// LDS (every window, including the shifted ones at both ends, lies inside it), in
+// the same order and expression as sigma_kernel.
+constexpr int SIG_T = 256, SIG_MAXP = 448, SIG_LDS = SIG_T + 4 * SIG_MAXP + 1;
+
+__global__ __launch_bounds__(SIG_T) void sigma_lds_kernel(const double *__restrict__ m, const double *__restrict__ vr,
+                                                          const double *__restrict__ vt, double *__restrict__ sigma,
+                                                          DevScalars *s, KParams P)
+{
+    __shared__ double a[SIG_LDS], b[SIG_LDS];
+    const int64_t N = s->n_max, p = P.avekernel;
+    const int64_t first = 1 + (int64_t)blockIdx.x * SIG_T;
+    if (first > N) return;
+    const int64_t last = first + SIG_T - 1 < N ? first + SIG_T - 1 : N;
+    const int64_t lo = first - 2 * p > 1 ? first - 2 * p : 1;
+    const int64_t hi = last + 2 * p < N ? last + 2 * p : N;
+    for (int64_t k = lo + threadIdx.x; k <= hi; k += SIG_T) {
+        double mk = m[k] * P.madhoc;
+        a[k - lo] = mk;
+        b[k - lo] = mk * (sqr(vr[k]) + sqr(vt[k]));
+    }
+    __syncthreads();
+    const int64_t si = first + threadIdx.x;
+    if (si > N) return;
+    int64_t simin = si - p, simax = simin + (2 * p - 1);
+    if (simin < 1) { simin = 1; simax = simin + (2 * p - 1); }
+    else if (simax > N) { simax = N; simin = simax - (2 * p - 1); }
+    if (simin < 1) simin = 1;
+    if (simax > N) simax = N;
+    double Mv2ave = 0.0, Mave = 0.0;
+    for (int64_t k = simin; k <= simax; k++) {
+        Mv2ave += b[k - lo];
+        Mave += a[k - lo];
+    }
+    sigma[si] = sqrt(Mv2ave / Mave);
+}
+
 __global__ void sigma_kernel(const double *__restrict__ m, const double *__restrict__ vr,
                              const double *__restrict__ vt, double *__restrict__ sigma, DevScalars *s, KParams P)
 {
@@ -256,8 +293,12 @@ hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64
 hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st)
 {
     if (n_ub <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sigma_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f.m, f.vr, f.vt,
-                       f.sigma, s, P);
+    if (P.avekernel <= SIG_MAXP)
+        hipLaunchKernelGGL(sigma_lds_kernel, dim3((unsigned)((n_ub + SIG_T - 1) / SIG_T)), dim3(SIG_T), 0, st, f.m,
+                           f.vr, f.vt, f.sigma, s, P);
+    else
+        hipLaunchKernelGGL(sigma_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f.m, f.vr, f.vt,
+                           f.sigma, s, P);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@ constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_MAX_FIELDS = 24;
 constexpr int TIE_MAX_RUN = 64;
+constexpr int AOS_W = 12;          // words per packed payload record (96 B)
 
 __device__ __forceinline__ uint64_t sortable(double x)
 {
@@ -42,10 +43,13 @@ __global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n)
     if (i < n) a[i] = 0;
 }
 
-// keys/vals from slots 1..n plus the eight global digit histograms
+// keys/vals from slots 1..n plus the eight global digit histograms; with `aos` the
+// payload fields of each slot are also packed into one 96-byte record, so the
+// final gather reads one record per star instead of one cache line per field.
 __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__restrict__ keysrc, DevScalars *s,
                                                              uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                             uint32_t *__restrict__ digit_totals)
+                                                             uint32_t *__restrict__ digit_totals,
+                                                             uint64_t *__restrict__ aos, GatherList g)
 {
     __shared__ uint32_t h[8][256];
     const int t = threadIdx.x;
@@ -60,6 +64,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
             keys[i] = k;
             vals[i] = (uint32_t)(i + 1);
             for (int p = 0; p < 8; p++) atomicAdd(&h[p][(k >> (8 * p)) & 255], 1u);
+            if (aos) {
+                uint64_t *rec = aos + (int64_t)AOS_W * i;
+                for (int f = 0; f < g.nf; f++) rec[f] = g.src[f][i + 1];
+            }
         }
     }
     __syncthreads();
@@ -258,6 +266,33 @@ __global__ __launch_bounds__(256) void rs_gather_kernel(const uint64_t *__restri
     for (int f = 0; f < g.nf; f++) g.dst[f][i + 1] = g.src[f][src];
 }
 
+__global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__restrict__ keys,
+                                                            const uint32_t *__restrict__ vals, DevScalars *s,
+                                                            double *__restrict__ r_out,
+                                                            const uint64_t *__restrict__ aos, GatherList g)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s->sort_n;
+    if (i > n) return;
+    if (i == n) {
+        r_out[n + 1] = CMCG_SF_INFINITY;
+        return;
+    }
+    uint32_t src = vals[i];
+    r_out[i + 1] = unsortable(keys[i]);
+    const ulonglong2 *rec = (const ulonglong2 *)(aos + (int64_t)AOS_W * (src - 1));
+    uint64_t w[AOS_W];
+#pragma unroll
+    for (int q = 0; q < AOS_W / 2; q++) {
+        ulonglong2 v = rec[q];
+        w[2 * q] = v.x;
+        w[2 * q + 1] = v.y;
+    }
+#pragma unroll
+    for (int f = 0; f < AOS_W; f++)
+        if (f < g.nf) g.dst[f][i + 1] = w[f];
+}
+
 __global__ void rs_finish_kernel(DevScalars *s)
 {
     s->n_max = s->sort_n;
@@ -274,6 +309,11 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
                               int64_t n_ub, hipStream_t st)
 {
     if (nf > RS_MAX_FIELDS) return hipErrorInvalidValue;
+    GatherList g;
+    g.nf = nf;
+    for (int f = 0; f < nf; f++) { g.src[f] = src[f]; g.dst[f] = dst[f]; }
+    for (int f = nf; f < RS_MAX_FIELDS; f++) { g.src[f] = nullptr; g.dst[f] = nullptr; }
+    uint64_t *aos = (nf <= AOS_W && sc.sort_aos && n_ub <= sc.sort_aos_cap) ? sc.sort_aos : nullptr;
     int64_t ntiles = (n_ub + RS_TILE - 1) / RS_TILE;
     if (ntiles < 1) ntiles = 1;
     if (ntiles > sc.max_sort_tiles) return hipErrorInvalidValue;
@@ -281,7 +321,7 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     hipLaunchKernelGGL(rs_begin_kernel, dim3(1), dim3(1), 0, st, s);
     hipLaunchKernelGGL(rs_zero_kernel, dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256));
     hipLaunchKernelGGL(rs_prep_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, sc.keys_a,
-                       sc.vals_a, sc.digit_totals);
+                       sc.vals_a, sc.digit_totals, aos, g);
     uint64_t *kin = sc.keys_a, *kout = sc.keys_b;
     uint32_t *vin = sc.vals_a, *vout = sc.vals_b;
     for (int pass = 0; pass < 8; pass++) {
@@ -298,12 +338,12 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
     hipLaunchKernelGGL(rs_ties_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, vin, id_src, s, kinf);
     hipLaunchKernelGGL(rs_count_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, kinf);
-    GatherList g;
-    g.nf = nf;
-    for (int f = 0; f < nf; f++) { g.src[f] = src[f]; g.dst[f] = dst[f]; }
-    for (int f = nf; f < RS_MAX_FIELDS; f++) { g.src[f] = nullptr; g.dst[f] = nullptr; }
-    hipLaunchKernelGGL(rs_gather_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, (const uint32_t *)vin, s,
-                       r_dst, g);
+    if (aos)
+        hipLaunchKernelGGL(rs_gather_aos_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin,
+                           (const uint32_t *)vin, s, r_dst, (const uint64_t *)aos, g);
+    else
+        hipLaunchKernelGGL(rs_gather_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, (const uint32_t *)vin,
+                           s, r_dst, g);
     hipLaunchKernelGGL(rs_finish_kernel, dim3(1), dim3(1), 0, st, s);
     return hipGetLastError();
 }

@@ -31,6 +31,8 @@ struct Scratch {
     uint32_t *block_hist;       // 256 * max_sort_tiles
     uint32_t *digit_totals;     // 8 * 256
     int64_t max_sort_tiles;
+    uint64_t *sort_aos;         // packed payload records, 12 words per star
+    int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event
     // walk ends and merge flags, per-block counts, fallback flag
     double *excess_c;
@@ -38,7 +40,7 @@ struct Scratch {
     int64_t *sv3_wend;
     int32_t *sv3_merged;
     int64_t *sv3_blk;
-    int32_t *sv3_fallback;
+    int64_t *sv3_fallback;      // first presumed head that failed the check
     // tidal stripping loop flag
     int32_t *strip_done;
     // orbit stars deferred to the linear-scan path (65536 slots)

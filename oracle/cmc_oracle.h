@@ -51,7 +51,7 @@ orc_t113 orc_stream_orbit(const orc_t113 *seeded, uint64_t step, uint64_t slot);
 
 enum { ORC_RNG_REF_SEQ = 0, ORC_RNG_PER_STAR = 1 };
 enum { ORC_Q_LONG_DOUBLE = 0, ORC_Q_FP64 = 1 };
-enum { ORC_LIBM_SYSTEM = 0, ORC_LIBM_LONG_DOUBLE = 1, ORC_LIBM_DD = 2 };
+enum { ORC_LIBM_SYSTEM = 0, ORC_LIBM_LONG_DOUBLE = 1, ORC_LIBM_DD = 2, ORC_LIBM_SINCOS = 3 };
 
 /* ---------------------------------------------------------------- state */
 typedef struct orc_state {

@@ -93,7 +93,7 @@ for _n, (_r, _a) in _sigs.items():
 
 RNG_REF_SEQ, RNG_PER_STAR = 0, 1
 Q_LONG_DOUBLE, Q_FP64 = 0, 1
-LIBM_SYSTEM, LIBM_LONG_DOUBLE, LIBM_DD = 0, 1, 2
+LIBM_SYSTEM, LIBM_LONG_DOUBLE, LIBM_DD, LIBM_SINCOS = 0, 1, 2, 3
 
 
 def seed_state(seed):

@@ -1,3 +1,4 @@
+#define _GNU_SOURCE
 /*
  * Oracle restatement of the reference hot-path functions — TEST INFRASTRUCTURE ONLY.
  *
@@ -526,26 +527,30 @@ double orc_simul_relax_new(const orc_state *s)
     return dtmin;
 }
 
-/* sin/cos of the relaxation.  ORC_LIBM_SYSTEM calls the C library, as the
- * reference does; ORC_LIBM_DD uses the double-double functions that the device
- * uses (orc_trig.c); ORC_LIBM_LONG_DOUBLE rounds the long double functions to
- * double, a third last-bit variant used only to measure how sensitive the
- * evolution is to the math library.  The library functions are called through
- * volatile pointers so that an optimising compiler cannot merge a same-argument
- * pair into sincos(), which rounds differently; the reference's documented build
- * is unoptimised and calls them separately. */
+/* sin/cos of the relaxation.  ORC_LIBM_SYSTEM calls the C library's sin() and cos()
+ * separately, as the reference's documented (unoptimised) build does;
+ * ORC_LIBM_SINCOS calls sincos() for each same-argument pair, as an optimising gcc
+ * compiles the reference's source; ORC_LIBM_DD uses the double-double functions
+ * that the device uses (orc_trig.c); ORC_LIBM_LONG_DOUBLE rounds the long double
+ * functions to double.  The last two and SINCOS are last-bit variants used to
+ * measure how sensitive the evolution is to the math library.  The library
+ * functions are called through volatile pointers so that the compiler cannot merge
+ * them. */
 static double (*volatile libm_sin)(double) = sin;
 static double (*volatile libm_cos)(double) = cos;
+static void (*volatile libm_sincos)(double, double *, double *) = sincos;
 static double o_sin(const orc_state *s, double x)
 {
     double sn, cs;
     if (s->libm_mode == ORC_LIBM_DD) { orc_dd_sincos(x, &sn, &cs); return sn; }
+    if (s->libm_mode == ORC_LIBM_SINCOS) { libm_sincos(x, &sn, &cs); return sn; }
     return s->libm_mode == ORC_LIBM_LONG_DOUBLE ? (double)sinl((long double)x) : libm_sin(x);
 }
 static double o_cos(const orc_state *s, double x)
 {
     double sn, cs;
     if (s->libm_mode == ORC_LIBM_DD) { orc_dd_sincos(x, &sn, &cs); return cs; }
+    if (s->libm_mode == ORC_LIBM_SINCOS) { libm_sincos(x, &sn, &cs); return cs; }
     return s->libm_mode == ORC_LIBM_LONG_DOUBLE ? (double)cosl((long double)x) : libm_cos(x);
 }
 

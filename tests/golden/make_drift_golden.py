@@ -1,8 +1,11 @@
 """Energy drift of the oracle over 100 Henon steps (Plummer N=2e4, seed 1, per-star
-RNG streams, long-double Q) under the C library's sin/cos, which the reference calls,
-and under a second last-bit variant (long double sin/cos rounded to double).  The
-difference between the two is the drift's sensitivity to the math library, the
-noise floor against which the device's drift is compared with the reference's.
+RNG streams, long-double Q) under the reference's own math-library variants: the C
+library's sin() and cos() (the documented, unoptimised build), sincos() for
+same-argument pairs (what an optimising gcc makes of the same source) and long
+double sin/cos rounded to double; and under the device's double-double sin/cos.
+The largest difference among the reference's variants is the drift's sensitivity
+to last-bit libm differences, the floor against which the device's drift is
+compared with the reference's.
 
     python tests/golden/make_drift_golden.py   ->  tests/golden/drift_golden.json
 """
@@ -35,9 +38,12 @@ def drift(libm):
 
 if __name__ == "__main__":
     out = {"n": N, "steps": STEPS, "seed": SEED}
-    for name, lm in (("system", O.LIBM_SYSTEM), ("long_double", O.LIBM_LONG_DOUBLE), ("dd", O.LIBM_DD)):
+    for name, lm in (("system", O.LIBM_SYSTEM), ("sincos", O.LIBM_SINCOS), ("long_double", O.LIBM_LONG_DOUBLE),
+                     ("dd", O.LIBM_DD)):
         d, n = drift(lm)
         out[name] = {"drift": d, "n_live": n}
         print(name, d, n, flush=True)
+    ref = [out[k]["drift"] for k in ("system", "sincos", "long_double")]
+    out["libm_spread"] = max(ref) - min(ref)
     with open(os.path.join(HERE, "drift_golden.json"), "w") as f:
         json.dump(out, f, indent=1)

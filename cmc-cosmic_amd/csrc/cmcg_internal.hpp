@@ -339,12 +339,23 @@ __device__ __noinline__ double find_root_vr(const double *__restrict__ r, const 
 // cphi[s] = phi[k(s)] with k(s) = min(s * stride, n + 1), s = 0..cs (k(cs) = n + 1).
 struct Coarse {
     const double *cr, *cphi;
-    int64_t stride, n1;
+    int64_t stride, n1;          // stride is a multiple of MID_STEP
     int cs;
+    // middle level in global memory (L2-resident): r and phi at every MID_STEP-th node
+    const double *mr, *mphi;
     // learned bracket proposal: predicted log r_peri, log r_apo and the half-width
     bool hint;
     float lrp, lra, margin;
 };
+
+constexpr int MID_STEP = 16;
+
+__device__ __forceinline__ int64_t coarse_stride(int64_t n, int cs)
+{
+    int64_t st = (n + 1 + cs - 1) / cs;
+    st = (st + MID_STEP - 1) / MID_STEP * MID_STEP;
+    return st < MID_STEP ? MID_STEP : st;
+}
 
 __device__ __forceinline__ int64_t coarse_k(const Coarse &c, int s)
 {
@@ -389,7 +400,47 @@ __device__ __forceinline__ double potential_at_coarse(const double *__restrict__
         if (ka > lo) lo = ka;
         if (kb < hi) hi = kb;
     }
+    if (c.mr) {
+        // largest mid node strictly inside (lo, hi) with r < x
+        int64_t t0 = (lo + MID_STEP) / MID_STEP, t1 = (hi - 1) / MID_STEP;
+        if (t0 <= t1) {
+            if (!(c.mr[t0] < x)) {
+                hi = MID_STEP * t0;
+            } else {
+                while (t0 < t1) {
+                    int64_t m = (t0 + t1 + 1) >> 1;
+                    if (c.mr[m] < x) t0 = m;
+                    else t1 = m - 1;
+                }
+                lo = MID_STEP * t0;
+                if (MID_STEP * (t0 + 1) < hi) hi = MID_STEP * (t0 + 1);
+            }
+        }
+    }
     return potential_at(r, phi, n_max, x, lo, hi, bad);
+}
+
+// Narrow a find_zero_Q bracket [lo, hi] with pred(Q(lo)) true and pred(Q(hi)) false
+// (pred monotone over the bracket) to the mid-node interval holding the sign change.
+template <bool POSITIVE>
+__device__ __forceinline__ void mid_narrow_q(const Coarse &c, double rj, double mj, double inv, double E, double J,
+                                             int64_t &lo, int64_t &hi)
+{
+    if (!c.mr) return;
+    int64_t t0 = (lo + MID_STEP) / MID_STEP, t1 = (hi - 1) / MID_STEP;
+    if (t0 > t1) return;
+    auto pred = [&](int64_t t) {
+        double q = function_q(rj, mj, inv, c.mr[t], c.mphi[t], E, J);
+        return POSITIVE ? q > 0.0 : q < 0.0;
+    };
+    if (!pred(t0)) { hi = MID_STEP * t0; return; }
+    while (t0 < t1) {
+        int64_t m = (t0 + t1 + 1) >> 1;
+        if (pred(m)) t0 = m;
+        else t1 = m - 1;
+    }
+    lo = MID_STEP * t0;
+    if (MID_STEP * (t0 + 1) < hi) hi = MID_STEP * (t0 + 1);
 }
 
 struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; int brent; int linscan; int err;
@@ -414,6 +465,7 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
     // to one sample interval before the bisection in global memory.
     int64_t kp_lo = 0, kp_hi = ktemp, ka_lo = ktemp, ka_hi = n_max + 1;
     bool got_p = false, got_a = false;
+    bool vp = false, va = false;       // bracket ends checked: Q(lo), Q(hi) have the bisection's signs
     if (C && C->hint) {
         // learned proposals: the index range of the predicted apsis +- margin (in
         // log r), padded by 2, accepted only when Q has the bisection's signs at its ends
@@ -425,7 +477,7 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
         hi = hi > ktemp ? ktemp : hi;
         if (lo < hi && function_q(rj, mj, inv, r[lo], phi[lo], E, J) < 0.0 &&
             (hi == ktemp || function_q(rj, mj, inv, r[hi], phi[hi], E, J) >= 0.0)) {
-            kp_lo = lo; kp_hi = hi; got_p = true; o.hint_hits++;
+            kp_lo = lo; kp_hi = hi; got_p = vp = true; o.hint_hits++;
         } else {
             o.hint_misses++;
         }
@@ -435,7 +487,7 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
         hi = hi > n_max + 1 ? n_max + 1 : hi;
         if (lo < hi && (lo == ktemp || function_q(rj, mj, inv, r[lo], phi[lo], E, J) > 0.0) &&
             !(function_q(rj, mj, inv, r[hi], phi[hi], E, J) > 0.0)) {
-            ka_lo = lo; ka_hi = hi; got_a = true; o.hint_hits++;
+            ka_lo = lo; ka_hi = hi; got_a = va = true; o.hint_hits++;
         } else {
             o.hint_misses++;
         }
@@ -452,9 +504,11 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
             }
             kp_lo = coarse_k(*C, a);
             kp_hi = a < top ? coarse_k(*C, a + 1) : ktemp;
+            vp = true;
         }
         const int sb = (int)(ktemp / C->stride) + 1;
         if (!got_a && sb <= C->cs && !(function_q(rj, mj, inv, C->cr[C->cs], C->cphi[C->cs], E, J) > 0.0)) {
+            va = true;
             if (!(function_q(rj, mj, inv, C->cr[sb], C->cphi[sb], E, J) > 0.0)) {
                 ka_hi = coarse_k(*C, sb);
             } else {
@@ -469,6 +523,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
             }
         }
     }
+    if (vp) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi);
+    if (va) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi);
     int64_t kmin = find_zero_Q(r, phi, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
     int64_t kmax = find_zero_Q(r, phi, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
     if (kmin < 0 || kmax < 0) {

@@ -180,7 +180,7 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
 constexpr int SV3_COARSE = 1024;      // LDS samples of r for potential(rOld)
 
 __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
-                                                 int64_t *__restrict__ blk_cnt)
+                                                 int64_t *__restrict__ blk_cnt, const double *__restrict__ mid_r)
 {
     __shared__ int cnt;
     __shared__ double cr[SV3_COARSE + 1];
@@ -189,8 +189,8 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
     int64_t n = s->n_max;
     Coarse C;
     C.cr = cr; C.cphi = nullptr; C.cs = SV3_COARSE; C.n1 = n + 1; C.hint = false;
-    C.stride = (n + 1 + SV3_COARSE - 1) / SV3_COARSE;
-    if (C.stride < 1) C.stride = 1;
+    C.mr = mid_r; C.mphi = nullptr;
+    C.stride = coarse_stride(n, SV3_COARSE);
     for (int sI = threadIdx.x; sI <= SV3_COARSE; sI += blockDim.x) cr[sI] = f.r[coarse_k(C, sI)];
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
@@ -457,7 +457,9 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     if (nb > sc.max_blocks) return hipErrorInvalidValue;
     unsigned ge = grid_for(n_ub);
     hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
-    hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk);
+    if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
+    hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
+                       (const double *)sc.mid_r);
     hipLaunchKernelGGL(sv3_scan_kernel, dim3(1), dim3(1024), 0, st, s, sc.sv3_blk, nb);
     hipLaunchKernelGGL(sv3_compact_kernel, dim3((unsigned)nb), dim3(GT), 0, st, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.sv3_blk, sc.excess_list);

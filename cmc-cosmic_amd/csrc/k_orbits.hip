@@ -110,6 +110,7 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
     // narrowed bracket for potential(r): r in [rmin, rmax] within [r_kmin, r_kmax+1]
     int64_t lo0 = o.kmin < 1 ? 1 : o.kmin;
     int64_t hi0 = o.kmax + 1 > n + 1 ? n + 1 : o.kmax + 1;
+    const double r_lo0 = r[lo0], r_hi0 = r[hi0];
     double X = 0.0, rr = 0.0, vr = 0.0, potr = 0.0;
     int k;
     bool bad = false;
@@ -119,8 +120,8 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
         double g0 = F * rng_dbl(rng);
         rr = 0.5 * (rmin + rmax) + 0.25 * (rmax - rmin) * (3.0 * s0 - s0 * s0 * s0);
         int64_t lo = lo0, hi = hi0;
-        if (!(r[lo] < rr)) lo = 1;
-        if (!(rr <= r[hi])) hi = n + 1;
+        if (!(r_lo0 < rr)) lo = 1;
+        if (!(rr <= r_hi0)) hi = n + 1;
         potr = C ? potential_at_coarse(r, phi, n, rr, lo, hi, *C, &bad) : potential_at(r, phi, n, rr, lo, hi, &bad);
         double pot = potr + phi_s_own(rj, mj, P.inv_nstar, rr);
         double drds = 0.25 * (rmax - rmin) * (3.0 - 3.0 * s0 * s0);
@@ -157,16 +158,18 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
                                                              double *__restrict__ partials,
                                                              int64_t *__restrict__ defer_list,
                                                              const float *__restrict__ theta,
-                                                             const MlpNorm *__restrict__ nrm, float margin)
+                                                             const MlpNorm *__restrict__ nrm, float margin,
+                                                             const double *__restrict__ mid_r,
+                                                             const double *__restrict__ mid_phi)
 {
     __shared__ NibbleTables<6> lt;
     __shared__ double cr[ORB_COARSE + 1], cphi[ORB_COARSE + 1];
     const int64_t n = s->n_max;
     Coarse C;
     C.cr = cr; C.cphi = cphi; C.cs = ORB_COARSE; C.n1 = n + 1;
-    C.stride = (n + 1 + ORB_COARSE - 1) / ORB_COARSE;
-    if (C.stride < 1) C.stride = 1;
+    C.stride = coarse_stride(n, ORB_COARSE);
     C.hint = false; C.lrp = C.lra = 0.f; C.margin = margin;
+    C.mr = mid_r; C.mphi = mid_phi;
     for (int sI = threadIdx.x; sI <= ORB_COARSE; sI += blockDim.x) {
         int64_t k = coarse_k(C, sI);
         cr[sI] = f.r[k];
@@ -387,9 +390,11 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     int jb1 = 20 + bits_needed((uint64_t)n_ub);
     hipLaunchKernelGGL(orbits_begin_kernel, dim3(1), dim3(1), 0, st, s);
     if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
+    if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     if (ev_a) hipEventRecord(ev_a, st);
     hipLaunchKernelGGL(orbits_kernel, dim3((unsigned)nblocks), dim3(ORB_THREADS), 0, st, f, s, P, T,
-                       (const Rng *)sc.rng_wave, sc.partials, sc.defer_list, theta, nrm, margin);
+                       (const Rng *)sc.rng_wave, sc.partials, sc.defer_list, theta, nrm, margin,
+                       (const double *)sc.mid_r, (const double *)sc.mid_phi);
     if (ev_b) hipEventRecord(ev_b, st);
     hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,

@@ -32,6 +32,7 @@ struct Scratch {
     uint32_t *digit_totals;     // 8 * 256
     int64_t max_sort_tiles;
     uint64_t *sort_aos;         // packed payload records, 12 words per star
+    double *mid_r, *mid_phi;    // r and phi at every 16th node (search middle level)
     int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event
     // walk ends and merge flags, per-block counts, fallback flag
@@ -68,6 +69,8 @@ hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch
 // substream start of every 64-lane wave: relaxation (phase 0, pairs 64w..) or orbits
 // (phase 1, slots 1 + 64w..), for `count` pairs / slots
 hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase, int64_t count, hipStream_t st);
+// middle level of the (r, phi) search tables, from the current arrays
+hipError_t launch_mid_build(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st);
 hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64_t first, int64_t count,
                             int ndraw, double *out, hipStream_t st);
 

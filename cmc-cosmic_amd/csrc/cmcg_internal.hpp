@@ -156,6 +156,16 @@ __device__ __forceinline__ Rng rng_jump_lds(const NibbleTables<NB> &lt, Rng s, u
     return s;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  This
+// bijection gives XCD x a contiguous range of logical blocks, so the blocks that
+// share an L2 work on neighbouring stars and their search paths overlap.
+__device__ __forceinline__ int64_t xcd_block()
+{
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t x = b & 7, local = b >> 3, q = nb >> 3, rem = nb & 7;
+    return x * q + (x < rem ? x : rem) + local;
+}
+
 // ---------------------------------------------------------------- potential helpers
 // include/cmc/cmc.h:2175 MPI_PHI_S(rad, j)
 __device__ __forceinline__ double phi_s(const double *__restrict__ r, const double *__restrict__ m,
@@ -341,8 +351,8 @@ struct Coarse {
     const double *cr, *cphi;
     int64_t stride, n1;          // stride is a multiple of MID_STEP
     int cs;
-    // middle level in global memory (L2-resident): r and phi at every MID_STEP-th node
-    const double *mr, *mphi;
+    // middle level in global memory (L2-resident): (r, phi) at every MID_STEP-th node
+    const double2 *mid;
     // learned bracket proposal: predicted log r_peri, log r_apo and the half-width
     bool hint;
     float lrp, lra, margin;
@@ -400,16 +410,16 @@ __device__ __forceinline__ double potential_at_coarse(const double *__restrict__
         if (ka > lo) lo = ka;
         if (kb < hi) hi = kb;
     }
-    if (c.mr) {
+    if (c.mid) {
         // largest mid node strictly inside (lo, hi) with r < x
         int64_t t0 = (lo + MID_STEP) / MID_STEP, t1 = (hi - 1) / MID_STEP;
         if (t0 <= t1) {
-            if (!(c.mr[t0] < x)) {
+            if (!(c.mid[t0].x < x)) {
                 hi = MID_STEP * t0;
             } else {
                 while (t0 < t1) {
                     int64_t m = (t0 + t1 + 1) >> 1;
-                    if (c.mr[m] < x) t0 = m;
+                    if (c.mid[m].x < x) t0 = m;
                     else t1 = m - 1;
                 }
                 lo = MID_STEP * t0;
@@ -426,11 +436,12 @@ template <bool POSITIVE>
 __device__ __forceinline__ void mid_narrow_q(const Coarse &c, double rj, double mj, double inv, double E, double J,
                                              int64_t &lo, int64_t &hi)
 {
-    if (!c.mr) return;
+    if (!c.mid) return;
     int64_t t0 = (lo + MID_STEP) / MID_STEP, t1 = (hi - 1) / MID_STEP;
     if (t0 > t1) return;
     auto pred = [&](int64_t t) {
-        double q = function_q(rj, mj, inv, c.mr[t], c.mphi[t], E, J);
+        double2 nd = c.mid[t];
+        double q = function_q(rj, mj, inv, nd.x, nd.y, E, J);
         return POSITIVE ? q > 0.0 : q < 0.0;
     };
     if (!pred(t0)) { hi = MID_STEP * t0; return; }
@@ -523,8 +534,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
             }
         }
     }
-    if (vp) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi);
-    if (va) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi);
+    if (vp && C) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi);
+    if (va && C) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi);
     int64_t kmin = find_zero_Q(r, phi, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
     int64_t kmax = find_zero_Q(r, phi, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
     if (kmin < 0 || kmax < 0) {

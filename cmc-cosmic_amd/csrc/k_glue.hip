@@ -180,7 +180,7 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
 constexpr int SV3_COARSE = 1024;      // LDS samples of r for potential(rOld)
 
 __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
-                                                 int64_t *__restrict__ blk_cnt, const double *__restrict__ mid_r)
+                                                 int64_t *__restrict__ blk_cnt, const double2 *__restrict__ mid)
 {
     __shared__ int cnt;
     __shared__ double cr[SV3_COARSE + 1];
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
     int64_t n = s->n_max;
     Coarse C;
     C.cr = cr; C.cphi = nullptr; C.cs = SV3_COARSE; C.n1 = n + 1; C.hint = false;
-    C.mr = mid_r; C.mphi = nullptr;
+    C.mid = mid;
     C.stride = coarse_stride(n, SV3_COARSE);
     for (int sI = threadIdx.x; sI <= SV3_COARSE; sI += blockDim.x) cr[sI] = f.r[coarse_k(C, sI)];
     if (threadIdx.x == 0) cnt = 0;
@@ -459,7 +459,7 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
-                       (const double *)sc.mid_r);
+                       (const double2 *)sc.mid);
     hipLaunchKernelGGL(sv3_scan_kernel, dim3(1), dim3(1024), 0, st, s, sc.sv3_blk, nb);
     hipLaunchKernelGGL(sv3_compact_kernel, dim3((unsigned)nb), dim3(GT), 0, st, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.sv3_blk, sc.excess_list);

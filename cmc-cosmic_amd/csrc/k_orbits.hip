@@ -159,8 +159,7 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
                                                              int64_t *__restrict__ defer_list,
                                                              const float *__restrict__ theta,
                                                              const MlpNorm *__restrict__ nrm, float margin,
-                                                             const double *__restrict__ mid_r,
-                                                             const double *__restrict__ mid_phi)
+                                                             const double2 *__restrict__ mid)
 {
     __shared__ NibbleTables<6> lt;
     __shared__ double cr[ORB_COARSE + 1], cphi[ORB_COARSE + 1];
@@ -169,14 +168,14 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
     C.cr = cr; C.cphi = cphi; C.cs = ORB_COARSE; C.n1 = n + 1;
     C.stride = coarse_stride(n, ORB_COARSE);
     C.hint = false; C.lrp = C.lra = 0.f; C.margin = margin;
-    C.mr = mid_r; C.mphi = mid_phi;
+    C.mid = mid;
     for (int sI = threadIdx.x; sI <= ORB_COARSE; sI += blockDim.x) {
         int64_t k = coarse_k(C, sI);
         cr[sI] = f.r[k];
         cphi[sI] = f.phi[k];
     }
     build_nibble_tables<6>(T, 20, lt);      // ends with a block barrier
-    const int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = 1 + xcd_block() * blockDim.x + threadIdx.x;
     double red[NRED];
     for (int i = 0; i < NRED; i++) red[i] = 0.0;
     if (j <= n) {
@@ -220,7 +219,7 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
         f.ntry[j] = ntry;
         count_star(red, flags, ntry, hits, misses, true);
     }
-    block_reduce_store(red, partials + RED_STRIDE * blockIdx.x);
+    block_reduce_store(red, partials + RED_STRIDE * xcd_block());
 }
 
 // Deferred stars: order the list (for a deterministic reduction order) and reset
@@ -394,7 +393,7 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     if (ev_a) hipEventRecord(ev_a, st);
     hipLaunchKernelGGL(orbits_kernel, dim3((unsigned)nblocks), dim3(ORB_THREADS), 0, st, f, s, P, T,
                        (const Rng *)sc.rng_wave, sc.partials, sc.defer_list, theta, nrm, margin,
-                       (const double *)sc.mid_r, (const double *)sc.mid_phi);
+                       (const double2 *)sc.mid);
     if (ev_b) hipEventRecord(ev_b, st);
     hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,

@@ -280,20 +280,18 @@ hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase
 }
 
 __global__ void mid_build_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
-                                 double *__restrict__ mr, double *__restrict__ mphi)
+                                 double2 *__restrict__ mid)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t k = 16 * t;
     if (k > s->n_max + 1) return;
-    mr[t] = r[k];
-    mphi[t] = phi[k];
+    mid[t] = make_double2(r[k], phi[k]);
 }
 
 hipError_t launch_mid_build(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st)
 {
     int64_t nt = (n_ub + 1) / 16 + 1;
-    hipLaunchKernelGGL(mid_build_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s, sc.mid_r,
-                       sc.mid_phi);
+    hipLaunchKernelGGL(mid_build_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s, sc.mid);
     return hipGetLastError();
 }
 

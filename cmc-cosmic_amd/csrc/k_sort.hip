@@ -52,22 +52,29 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
                                                              uint64_t *__restrict__ aos, GatherList g)
 {
     __shared__ uint32_t h[8][256];
+    __shared__ uint64_t recs[RS_THREADS * AOS_W];      // one chunk of packed records
     const int t = threadIdx.x;
     for (int p = 0; p < 8; p++) h[p][t] = 0;
     __syncthreads();
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     for (int u = 0; u < RS_ITEMS; u++) {
-        int64_t i = base + u * RS_THREADS + t;
+        const int64_t i0 = base + u * RS_THREADS;
+        const int64_t i = i0 + t;
         if (i < n) {
             uint64_t k = sortable(keysrc[i + 1]);
             keys[i] = k;
             vals[i] = (uint32_t)(i + 1);
             for (int p = 0; p < 8; p++) atomicAdd(&h[p][(k >> (8 * p)) & 255], 1u);
-            if (aos) {
-                uint64_t *rec = aos + (int64_t)AOS_W * i;
-                for (int f = 0; f < g.nf; f++) rec[f] = g.src[f][i + 1];
-            }
+        }
+        if (aos && i0 < n) {        // block-uniform: pack the chunk through LDS, write it contiguously
+            if (i < n)
+                for (int f = 0; f < g.nf; f++) recs[AOS_W * t + f] = g.src[f][i + 1];
+            __syncthreads();
+            const int64_t cnt = n - i0 < RS_THREADS ? n - i0 : RS_THREADS;
+            uint64_t *dst = aos + (int64_t)AOS_W * i0;
+            for (int w = t; w < cnt * AOS_W; w += RS_THREADS) dst[w] = recs[w];
+            __syncthreads();
         }
     }
     __syncthreads();

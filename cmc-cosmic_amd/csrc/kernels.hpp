@@ -32,7 +32,7 @@ struct Scratch {
     uint32_t *digit_totals;     // 8 * 256
     int64_t max_sort_tiles;
     uint64_t *sort_aos;         // packed payload records, 12 words per star
-    double *mid_r, *mid_phi;    // r and phi at every 16th node (search middle level)
+    double2 *mid;               // (r, phi) at every 16th node (search middle level)
     int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event
     // walk ends and merge flags, per-block counts, fallback flag

@@ -127,6 +127,9 @@ _sigs = {
     "cmcg_compute_energy": [_ctx],
     "cmcg_init_cluster": [_ctx],
     "cmcg_step": [_ctx, _P(OrbitStats)],
+    "cmcg_step_graph": [_ctx, _P(OrbitStats)],
+    "cmcg_autotune": [_ctx, C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_double), _P(C.c_double)],
+    "cmcg_ensemble_step": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
     "cmcg_rng_draws": [_ctx, C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int, _P(C.c_double)],
     "cmcg_timer_enable": [_ctx, C.c_int],
     "cmcg_timer_get": [_ctx, C.c_char_p, _P(C.c_double), _P(C.c_int64)],
@@ -380,6 +383,21 @@ class Context:
             _chk(rc, "cmcg_step")
         return st.as_dict()
 
+    def autotune(self, reps=3):
+        v, th, cs = C.c_int32(), C.c_int32(), C.c_int32()
+        best, default = C.c_double(), C.c_double()
+        _chk(lib.cmcg_autotune(self.h, int(reps), C.byref(v), C.byref(th), C.byref(cs), C.byref(best),
+                               C.byref(default)), "cmcg_autotune")
+        return {"variant": v.value, "threads": th.value, "coarse": cs.value, "ms_best": best.value,
+                "ms_default": default.value}
+
+    def step_graph(self, check=True):
+        st = OrbitStats()
+        rc = lib.cmcg_step_graph(self.h, C.byref(st))
+        if check:
+            _chk(rc, "cmcg_step_graph")
+        return st.as_dict()
+
     def rng_draws(self, phase, step, first, count, ndraw):
         out = np.zeros(count * ndraw)
         _chk(lib.cmcg_rng_draws(self.h, phase, step, first, count, ndraw, _ptr(out, C.c_double)), "cmcg_rng_draws")
@@ -451,3 +469,15 @@ class Context:
 
     def set_learned_brackets(self, on=True, margin=0.05):
         _chk(lib.cmcg_set_learned_brackets(self.h, int(on), float(margin)), "cmcg_set_learned_brackets")
+
+
+def ensemble_step(ctxs, check=True):
+    """One step of every context's cluster, all launched before any wait."""
+    n = len(ctxs)
+    arr = (_ctx * n)(*[c.h for c in ctxs])
+    stats = (OrbitStats * n)()
+    errs = (C.c_int32 * n)()
+    rc = lib.cmcg_ensemble_step(arr, n, stats, errs)
+    if check:
+        _chk(rc, "cmcg_ensemble_step")
+    return [stats[i].as_dict() for i in range(n)], [int(errs[i]) for i in range(n)]

@@ -40,7 +40,19 @@ struct cmcg_ctx {
     int64_t adam_t = 0;
     float *io = nullptr;
     int64_t io_cap = 0;          // floats
+    // step graphs: the fused sort swaps buffer pointers, so steps alternate between
+    // two pointer states; graph[p] is the step captured in state p
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int parity = 0;
+    int orb_variant = 0;         // launch shape of the orbit sampler (autotuned)
 };
+
+static void invalidate_graphs(cmcg_ctx *c)
+{
+    for (auto &g : c->graph)
+        if (g) { hipGraphExecDestroy(g); g = nullptr; }
+    c->parity = 0;
+}
 
 namespace {
 
@@ -213,12 +225,10 @@ void add_field(SortSpec &sp, double *src, double *dst, double **fa, double **fb)
     sp.nf++;
 }
 
-// Runs the sort and swaps every destination field into the live set.
-int run_sort(cmcg_ctx *c, bool fused)
+// The payload fields of the sort and the live/alternate pointer pairs they swap.
+static void sort_spec(cmcg_ctx *c, bool fused, SortSpec &sp)
 {
     DevFields &f = c->f, &a = c->alt;
-    SortSpec sp;
-    const double *key = fused ? f.rnew : f.r;
     if (fused) {
         // ComputeIntermediateEnergy's transfer (cmc_utils.c:382-390) fused into the gather:
         // r <- rnew (the key), rOld <- r, vr <- vrnew, vt <- vtnew.
@@ -244,15 +254,25 @@ int run_sort(cmcg_ctx *c, bool fused)
     sp.src[sp.nf] = (const uint64_t *)f.id;
     sp.dst[sp.nf] = (uint64_t *)a.id;
     sp.nf++;
-    hipError_t e = launch_sort_fields(key, a.r, f.id, sp.nf, sp.src, sp.dst, c->s, c->sc, c->n_ub, c->stream);
-    if (e != hipSuccess) { cmcg_report_hip(e, "launch_sort_fields", __FILE__, __LINE__); return CMCG_E_HIP; }
-    // swap live/alt pointers of every written field
-    std::swap(f.r, a.r);
+}
+
+static void swap_sorted(cmcg_ctx *c, const SortSpec &sp)
+{
+    std::swap(c->f.r, c->alt.r);
     for (int i = 0; i < sp.nf - 1; i++) std::swap(*sp.swap_a[i], *sp.swap_b[i]);
-    std::swap(f.id, a.id);
-    if (fused) {
-        // the old vr/vt buffers are now stale copies; keep vrnew/vtnew as they were
-    }
+    std::swap(c->f.id, c->alt.id);
+}
+
+// Runs the sort and swaps every destination field into the live set.
+int run_sort(cmcg_ctx *c, bool fused)
+{
+    SortSpec sp;
+    sort_spec(c, fused, sp);
+    const double *key = fused ? c->f.rnew : c->f.r;
+    hipError_t e = launch_sort_fields(key, c->alt.r, c->f.id, sp.nf, sp.src, sp.dst, c->s, c->sc, c->n_ub, c->stream);
+    if (e != hipSuccess) { cmcg_report_hip(e, "launch_sort_fields", __FILE__, __LINE__); return CMCG_E_HIP; }
+    swap_sorted(c, sp);
+    // the old vr/vt buffers are now stale copies; vrnew/vtnew keep their values
     return CMCG_OK;
 }
 
@@ -291,6 +311,7 @@ const char *cmcg_error_string(int code)
     case CMCG_E_CAPACITY: return "capacity exceeded";
     case CMCG_E_NOMEM: return "device allocation failed";
     case CMCG_E_STATE: return "call out of order";
+    case CMCG_E_STRIP: return "tidal stripping did not end within the captured passes";
     default: return "unknown error";
     }
 }
@@ -335,7 +356,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
         (e = dalloc(c, &sc.sv3_wend, (size_t)n_cap + 2)) || (e = dalloc(c, &sc.sv3_merged, (size_t)n_cap + 2)) ||
         (e = dalloc(c, &sc.sv3_blk, (size_t)sc.max_blocks)) || (e = dalloc(c, &sc.sv3_fallback, 1)) ||
-        (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.chain_x, (size_t)n_cap + 8)) ||
+        (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.strip_arrive, 1)) || (e = dalloc(c, &sc.chain_x, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.chain_g, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.chain_mk, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.defer_list, 65536)) || (e = dalloc(c, &sc.defer_kfirst, 65536)) ||
         (e = dalloc(c, &sc.defer_partials, 65536 * 16)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
@@ -361,6 +382,7 @@ int cmcg_destroy(cmcg_ctx *c)
     if (!c) return CMCG_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    invalidate_graphs(c);
     for (auto &p : c->pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
     for (auto e : c->event_pool) hipEventDestroy(e);
     for (void *p : c->allocs) hipFree(p);
@@ -472,6 +494,7 @@ int cmcg_sort_by_r(cmcg_ctx *c, int64_t *n_max)
     if (!c) return CMCG_E_ARG;
     {
         PhaseTimer t(c, "sort");
+        invalidate_graphs(c);
         if (int e = run_sort(c, false)) return e;
     }
     DevScalars h;
@@ -579,7 +602,7 @@ int cmcg_orbits(cmcg_ctx *c, uint64_t step, cmcg_orbit_stats *stats)
         hipEvent_t ka = nullptr, kb = nullptr;
         kernel_events(c, "orbits_kernel", ka, kb);
         LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream, c->learned ? c->mlp.theta : nullptr,
-                             c->mlp.norm, c->margin, ka, kb));
+                             c->mlp.norm, c->margin, ka, kb, c->orb_variant));
         if (c->harvest) LAUNCH(launch_mlp_harvest(c->f, c->s, c->kp, c->mlp, c->n_ub, c->stream));
     }
     DevScalars h;
@@ -715,10 +738,14 @@ int cmcg_init_cluster(cmcg_ctx *c)
     return h.error;
 }
 
-int cmcg_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
+// Enqueues one Henon step (src/cmc/cmc.c:240-407 order).  With `graph` the tidal
+// stripping loop is a fixed number of guarded passes (no host round trip) so the
+// whole step can be captured; nub bounds the grids (the kernels read N_MAX on the
+// device).
+constexpr int STRIP_GRAPH_PASSES = 16;
+
+static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
 {
-    if (!c) return CMCG_E_ARG;
-    const int64_t nub = c->n_ub;
     {
         PhaseTimer t(c, "timestep");
         LAUNCH(launch_step_begin(c->s, c->stream));
@@ -741,22 +768,28 @@ int cmcg_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
         hipEvent_t ka = nullptr, kb = nullptr;
         kernel_events(c, "orbits_kernel", ka, kb);
         LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream, c->learned ? c->mlp.theta : nullptr,
-                             c->mlp.norm, c->margin, ka, kb));
+                             c->mlp.norm, c->margin, ka, kb, c->orb_variant));
         if (c->harvest) LAUNCH(launch_mlp_harvest(c->f, c->s, c->kp, c->mlp, nub, c->stream));
     }
     {
         PhaseTimer t(c, "glue_pre_sort");
         LAUNCH(launch_energy_conservation2(c->f, c->s, c->kp, nub, c->stream));
-        if (c->params.tidal_strip) LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, 2, true, c->stream));
+        if (c->params.tidal_strip)
+            LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, graph ? STRIP_GRAPH_PASSES : 2, true,
+                                c->stream));
     }
     if (c->params.tidal_strip) {
-        // the stripping loop normally ends within two passes; otherwise continue it here
-        for (int guard = 0; guard < 100000; guard++) {
-            int32_t done = 0;
-            if (sync(c) != hipSuccess) return CMCG_E_HIP;
-            if (hipMemcpy(&done, c->sc.strip_done, 4, hipMemcpyDeviceToHost) != hipSuccess) return CMCG_E_HIP;
-            if (done) break;
-            LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, 1, false, c->stream));
+        if (graph) {
+            LAUNCH(launch_strip_check(c->s, c->sc, c->stream));
+        } else {
+            // the stripping loop normally ends within two passes; otherwise continue it here
+            for (int guard = 0; guard < 100000; guard++) {
+                int32_t done = 0;
+                if (sync(c) != hipSuccess) return CMCG_E_HIP;
+                if (hipMemcpy(&done, c->sc.strip_done, 4, hipMemcpyDeviceToHost) != hipSuccess) return CMCG_E_HIP;
+                if (done) break;
+                LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, 1, false, c->stream));
+            }
         }
     }
     {
@@ -765,7 +798,12 @@ int cmcg_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
     }
     {
         PhaseTimer t(c, "sort");
-        if (int e = run_sort(c, true)) return e;
+        SortSpec sp;
+        sort_spec(c, true, sp);
+        hipError_t e = launch_sort_fields(c->f.rnew, c->alt.r, c->f.id, sp.nf, sp.src, sp.dst, c->s, c->sc, nub,
+                                          c->stream);
+        if (e != hipSuccess) { cmcg_report_hip(e, "launch_sort_fields", __FILE__, __LINE__); return CMCG_E_HIP; }
+        swap_sorted(c, sp);
     }
     {
         PhaseTimer t(c, "potential");
@@ -777,11 +815,87 @@ int cmcg_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
         LAUNCH(launch_compute_energy(c->f, c->s, c->kp, c->sc, nub, c->stream));
         hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->s);
     }
+    c->parity ^= 1;
+    return CMCG_OK;
+}
+
+static int finish_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
+{
     DevScalars h;
     if (int e = read_scalars(c, h)) return e;
     c->n_ub = h.n_max;
     fill_stats(h, stats);
     return h.error;
+}
+
+int cmcg_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
+{
+    if (!c) return CMCG_E_ARG;
+    if (int e = enqueue_step(c, c->n_ub, false)) return e;
+    return finish_step(c, stats);
+}
+
+// Enqueue one step as a graph launch (captured on first use in each pointer state).
+static int launch_step_graph(cmcg_ctx *c)
+{
+    const int p = c->parity;
+    if (!c->graph[p]) {
+        const bool timing = c->timing;
+        c->timing = false;
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            c->timing = timing;
+            return CMCG_E_HIP;
+        }
+        int e = enqueue_step(c, c->n_cap, true);       // swaps the host pointers, flips parity
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        c->timing = timing;
+        if (e || ce != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            invalidate_graphs(c);
+            return e ? e : CMCG_E_HIP;
+        }
+        hipError_t ie = hipGraphInstantiate(&c->graph[p], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ie != hipSuccess) { c->graph[p] = nullptr; invalidate_graphs(c); return CMCG_E_HIP; }
+    } else {
+        SortSpec sp;
+        sort_spec(c, true, sp);
+        swap_sorted(c, sp);                             // the pointer state the graph leaves
+        c->parity ^= 1;
+    }
+    if (hipGraphLaunch(c->graph[p], c->stream) != hipSuccess) return CMCG_E_HIP;
+    return CMCG_OK;
+}
+
+int cmcg_step_graph(cmcg_ctx *c, cmcg_orbit_stats *stats)
+{
+    if (!c) return CMCG_E_ARG;
+    if (int e = launch_step_graph(c)) return e;
+    return finish_step(c, stats);
+}
+
+int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors)
+{
+    if (!ctxs || count < 1) return CMCG_E_ARG;
+    for (int i = 0; i < count; i++)
+        if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) return CMCG_E_ARG;
+    int first_err = CMCG_OK;
+    // every cluster's step graph goes to its own stream before any wait
+    std::vector<int> launched(count, 0);
+    for (int i = 0; i < count; i++) {
+        int e = launch_step_graph(ctxs[i]);
+        launched[i] = e == CMCG_OK;
+        if (errors) errors[i] = e;
+        if (e && !first_err) first_err = e;
+    }
+    for (int i = 0; i < count; i++) {
+        if (!launched[i]) continue;
+        int e = finish_step(ctxs[i], stats ? &stats[i] : nullptr);
+        if (errors) errors[i] = e;
+        if (e && !first_err) first_err = e;
+    }
+    return first_err;
 }
 
 int cmcg_rng_draws(cmcg_ctx *c, int phase, uint64_t step, int64_t first, int64_t count, int ndraw, double *out)
@@ -877,6 +991,7 @@ int cmcg_mlp_harvest(cmcg_ctx *c, int on)
 {
     if (!c) return CMCG_E_ARG;
     if (on && !c->mlp_ready) return CMCG_E_STATE;
+    if (c->harvest != (on != 0)) invalidate_graphs(c);
     c->harvest = on != 0;
     return CMCG_OK;
 }
@@ -965,8 +1080,58 @@ int cmcg_set_learned_brackets(cmcg_ctx *c, int on, double margin)
 {
     if (!c || (on && !(margin > 0.0 && margin < 10.0))) return CMCG_E_ARG;
     if (on && !c->mlp_ready) return CMCG_E_STATE;
+    if (c->learned != (on != 0) || (on && c->margin != (float)margin)) invalidate_graphs(c);
     c->learned = on != 0;
     if (on) c->margin = (float)margin;
+    return CMCG_OK;
+}
+
+// ------------------------------------------------------------------ autotuning
+int cmcg_autotune(cmcg_ctx *c, int reps, int32_t *variant, int32_t *threads, int32_t *coarse, double *ms_best,
+                  double *ms_default)
+{
+    if (!c || reps < 1) return CMCG_E_ARG;
+    DevScalars saved;
+    if (sync(c) != hipSuccess || hipMemcpy(&saved, c->s, sizeof(saved), hipMemcpyDeviceToHost) != hipSuccess)
+        return CMCG_E_HIP;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return CMCG_E_HIP;
+    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return CMCG_E_HIP; }
+    int best = 0;
+    double best_ms = 1e300, def_ms = 0.0;
+    int rc = CMCG_OK;
+    for (int v = 0; v < orbit_variant_count() && rc == CMCG_OK; v++) {
+        double total = 0.0;
+        for (int it = 0; it <= reps; it++) {      // the first run warms up and is not timed
+            hipEvent_t ea = it ? a : nullptr, eb = it ? b : nullptr;
+            if (launch_rng_bases(c->T, c->s, c->sc, c->stream) != hipSuccess ||
+                launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream, c->learned ? c->mlp.theta : nullptr,
+                              c->mlp.norm, c->margin, ea, eb, v) != hipSuccess ||
+                sync(c) != hipSuccess) { rc = CMCG_E_HIP; break; }
+            if (it) {
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, a, b);
+                total += ms;
+            }
+            // the phase is idempotent on the stars; put back the scalars it accumulates
+            if (hipMemcpy(c->s, &saved, sizeof(saved), hipMemcpyHostToDevice) != hipSuccess) { rc = CMCG_E_HIP; break; }
+        }
+        double avg = total / reps;
+        if (v == 0) def_ms = avg;
+        if (avg < best_ms) { best_ms = avg; best = v; }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) return rc;
+    if (best != c->orb_variant) invalidate_graphs(c);
+    c->orb_variant = best;
+    int th, cs;
+    orbit_variant_shape(best, &th, &cs);
+    if (variant) *variant = best;
+    if (threads) *threads = th;
+    if (coarse) *coarse = cs;
+    if (ms_best) *ms_best = best_ms;
+    if (ms_default) *ms_default = def_ms;
     return CMCG_OK;
 }
 

@@ -96,27 +96,30 @@ __global__ void strip_begin_kernel(DevScalars *s, int32_t *done)
     *done = 0;
 }
 
-__global__ void strip_rtidal_kernel(DevScalars *s, KParams P, const int32_t *done)
-{
-    if (*done) return;
-    s->Rtidal = P.orbit_r * pow(s->Mtotal - (s->TidalMassLoss - s->OldTidalMassLoss), 1.0 / 3.0);
-}
+// One pass of the stripping loop (cmc_evolution_thr.c:447-645) in one kernel: every
+// block derives Rtidal from the scalars left by the previous pass, strips its stars
+// (grid-stride), and publishes its partial sums; the last block to arrive reduces
+// them in block order, updates the scalars, decides whether the loop ends and
+// resets the arrival counter.  Passes after the end return at once, so a fixed
+// number of passes can be enqueued (or captured in a graph) without a host round
+// trip.
+constexpr int STRIP_BLOCKS = 1024;
 
-__global__ __launch_bounds__(GT) void strip_pass_kernel(DevFields f, DevScalars *s, KParams P,
-                                                        const int32_t *done, double *__restrict__ partials)
+__global__ __launch_bounds__(GT) void strip_fused_kernel(DevFields f, DevScalars *s, KParams P, int32_t *done,
+                                                         double *__restrict__ partials, unsigned *__restrict__ arrive)
 {
     if (*done) return;
-    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double Rt = P.orbit_r * pow(s->Mtotal - (s->TidalMassLoss - s->OldTidalMassLoss), 1.0 / 3.0);
+    const int64_t n = s->n_max;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    if (i <= s->n_max) {
-        double Rt = s->Rtidal;
+    for (int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
         if (f.r_apo[i] > Rt && f.rnew[i] < 1000000) {
             double m = f.m[i], E = f.E[i], J = f.J[i];
             f.rnew[i] = CMCG_SF_INFINITY; f.vrnew[i] = 0.0; f.vtnew[i] = 0.0;
-            v[0] = E * m / (double)P.n_star;   // Eescaped
-            v[1] = J * m / (double)P.n_star;   // Jescaped
-            v[2] = m / (double)P.n_star;       // DTidalMassLoss
-            v[3] = E * m / (double)P.n_star;   // Etidal
+            v[0] += E * m / (double)P.n_star;   // Eescaped
+            v[1] += J * m / (double)P.n_star;   // Jescaped
+            v[2] += m / (double)P.n_star;       // DTidalMassLoss
+            v[3] += E * m / (double)P.n_star;   // Etidal
             // destroy_obj
             f.vr[i] = 0.0; f.vt[i] = 0.0; f.E[i] = 0.0; f.J[i] = 0.0; f.id[i] = 0;
             f.X[i] = 0.0; f.Y[i] = 0.0; f.r_peri[i] = 0.0; f.r_apo[i] = 0.0;
@@ -127,21 +130,32 @@ __global__ __launch_bounds__(GT) void strip_pass_kernel(DevFields f, DevScalars 
         }
     }
     block_sum_store<4>(v, partials + 8 * blockIdx.x);
-}
-
-__global__ void strip_end_kernel(DevScalars *s, int32_t *done, const double *__restrict__ partials, int64_t nblocks)
-{
-    if (*done) return;
-    double res[4];
-    final_sum<4>(partials, nblocks, res);
+    __shared__ bool last;
     if (threadIdx.x == 0) {
+        __threadfence();                                   // publish this block's partials
+        last = atomicAdd(arrive, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();                                       // see every block's partials
+    double res[4];
+    final_sum<4>(partials, gridDim.x, res);
+    if (threadIdx.x == 0) {
+        s->Rtidal = Rt;
         s->Eescaped += res[0];
         s->Jescaped += res[1];
         s->Etidal += res[3];
         s->TidalMassLoss += res[2];
         s->DTidalMassLoss = res[2];
         if (!(res[2] > 0)) *done = 1;
+        *arrive = 0;
     }
+}
+
+// after a fixed number of passes: the loop must have ended
+__global__ void strip_check_kernel(DevScalars *s, const int32_t *done)
+{
+    if (!*done) dev_set_error(s, CMCG_E_STRIP, 0);
 }
 
 // ComputeIntermediateEnergy, first loop (EI).
@@ -427,15 +441,19 @@ hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const 
 hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,
                         int64_t n_ub, int passes, bool begin, hipStream_t st)
 {
-    int64_t nb = grid_for(n_ub);
-    if (nb > sc.max_blocks) return hipErrorInvalidValue;
+    (void)n_ub;
+    // a fixed grid (grid-stride over the stars) that the per-block partials hold
+    const unsigned nb = (unsigned)(STRIP_BLOCKS < sc.max_blocks ? STRIP_BLOCKS : sc.max_blocks);
     if (begin) hipLaunchKernelGGL(strip_begin_kernel, dim3(1), dim3(1), 0, st, s, done);
-    for (int it = 0; it < passes; it++) {
-        hipLaunchKernelGGL(strip_rtidal_kernel, dim3(1), dim3(1), 0, st, s, P, (const int32_t *)done);
-        hipLaunchKernelGGL(strip_pass_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, (const int32_t *)done,
-                           sc.partials);
-        hipLaunchKernelGGL(strip_end_kernel, dim3(1), dim3(GT), 0, st, s, done, (const double *)sc.partials, nb);
-    }
+    for (int it = 0; it < passes; it++)
+        hipLaunchKernelGGL(strip_fused_kernel, dim3(nb), dim3(GT), 0, st, f, s, P, done, sc.partials,
+                           sc.strip_arrive);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip_check(DevScalars *s, const Scratch &sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(strip_check_kernel, dim3(1), dim3(1), 0, st, s, (const int32_t *)sc.strip_done);
     return hipGetLastError();
 }
 hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,

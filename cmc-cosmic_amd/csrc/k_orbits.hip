@@ -31,9 +31,10 @@ constexpr int64_t MAX_DEFER = 65536;
 constexpr int NRED = 15, RED_STRIDE = 16;
 __device__ __forceinline__ bool red_is_max(int i) { return i == 0 || i == 12; }
 
+template <int THREADS>
 __device__ void block_reduce_store(double (&v)[NRED], double *__restrict__ out)
 {
-    __shared__ double wsum[ORB_THREADS / 64][NRED];
+    __shared__ double wsum[THREADS / 64][NRED];
     for (int off = 32; off > 0; off >>= 1)
         for (int i = 0; i < NRED; i++) {
             double o = __shfl_xor(v[i], off);
@@ -46,7 +47,7 @@ __device__ void block_reduce_store(double (&v)[NRED], double *__restrict__ out)
     if (threadIdx.x < NRED) {
         const int i = threadIdx.x;
         double a = wsum[0][i];
-        for (int k = 1; k < ORB_THREADS / 64; k++) a = red_is_max(i) ? (a >= wsum[k][i] ? a : wsum[k][i]) : a + wsum[k][i];
+        for (int k = 1; k < THREADS / 64; k++) a = red_is_max(i) ? (a >= wsum[k][i] ? a : wsum[k][i]) : a + wsum[k][i];
         out[i] = a;
     }
 }
@@ -152,7 +153,8 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
 // Slot j draws from offset j 2^20; the wave's first slot starts at the per-wave state
 // and lane l jumps l 2^20 (l < 2^6) through the LDS tables.  With learned brackets
 // the network's prediction proposes both find_zero_Q brackets.
-__global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
+template <int THREADS, int COARSE>
+__global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
                                                              const RngTable *__restrict__ T,
                                                              const Rng *__restrict__ wave_base,
                                                              double *__restrict__ partials,
@@ -162,14 +164,14 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
                                                              const double2 *__restrict__ mid)
 {
     __shared__ NibbleTables<6> lt;
-    __shared__ double cr[ORB_COARSE + 1], cphi[ORB_COARSE + 1];
+    __shared__ double cr[COARSE + 1], cphi[COARSE + 1];
     const int64_t n = s->n_max;
     Coarse C;
-    C.cr = cr; C.cphi = cphi; C.cs = ORB_COARSE; C.n1 = n + 1;
-    C.stride = coarse_stride(n, ORB_COARSE);
+    C.cr = cr; C.cphi = cphi; C.cs = COARSE; C.n1 = n + 1;
+    C.stride = coarse_stride(n, COARSE);
     C.hint = false; C.lrp = C.lra = 0.f; C.margin = margin;
     C.mid = mid;
-    for (int sI = threadIdx.x; sI <= ORB_COARSE; sI += blockDim.x) {
+    for (int sI = threadIdx.x; sI <= COARSE; sI += blockDim.x) {
         int64_t k = coarse_k(C, sI);
         cr[sI] = f.r[k];
         cphi[sI] = f.phi[k];
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(ORB_THREADS) void orbits_kernel(DevFields f, DevSca
         f.ntry[j] = ntry;
         count_star(red, flags, ntry, hits, misses, true);
     }
-    block_reduce_store(red, partials + RED_STRIDE * xcd_block());
+    block_reduce_store<THREADS>(red, partials + RED_STRIDE * xcd_block());
 }
 
 // Deferred stars: order the list (for a deterministic reduction order) and reset
@@ -371,6 +373,17 @@ __global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count,
     out[i] = k;
 }
 
+// Launch shapes of the orbit sampler that the autotuner chooses from: threads per
+// block and LDS samples of the (r, phi) table.  Every variant gives the same stars;
+// only the block order of the escape sums differs.
+using OrbFn = void (*)(DevFields, DevScalars *, KParams, const RngTable *, const Rng *, double *, int64_t *,
+                       const float *, const MlpNorm *, float, const double2 *);
+struct OrbVariant { OrbFn fn; int threads, coarse; };
+const OrbVariant kOrbVariants[] = {
+    {orbits_kernel<256, 1024>, 256, 1024}, {orbits_kernel<256, 512>, 256, 512}, {orbits_kernel<256, 2048>, 256, 2048},
+    {orbits_kernel<128, 1024>, 128, 1024}, {orbits_kernel<512, 1024>, 512, 1024}, {orbits_kernel<128, 512>, 128, 512}};
+constexpr int ORB_NVARIANTS = sizeof(kOrbVariants) / sizeof(kOrbVariants[0]);
+
 int bits_needed(uint64_t x)
 {
     int b = 0;
@@ -381,9 +394,10 @@ int bits_needed(uint64_t x)
 
 hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
                          const Scratch &sc, int64_t n_ub, hipStream_t st, const float *theta, const MlpNorm *nrm,
-                         float margin, hipEvent_t ev_a, hipEvent_t ev_b)
+                         float margin, hipEvent_t ev_a, hipEvent_t ev_b, int variant)
 {
-    int64_t nblocks = (n_ub + ORB_THREADS - 1) / ORB_THREADS;
+    const OrbVariant &V = kOrbVariants[variant >= 0 && variant < ORB_NVARIANTS ? variant : 0];
+    int64_t nblocks = (n_ub + V.threads - 1) / V.threads;
     if (nblocks < 1) nblocks = 1;
     if (nblocks > sc.max_blocks) return hipErrorInvalidValue;
     int jb1 = 20 + bits_needed((uint64_t)n_ub);
@@ -391,9 +405,8 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     if (ev_a) hipEventRecord(ev_a, st);
-    hipLaunchKernelGGL(orbits_kernel, dim3((unsigned)nblocks), dim3(ORB_THREADS), 0, st, f, s, P, T,
-                       (const Rng *)sc.rng_wave, sc.partials, sc.defer_list, theta, nrm, margin,
-                       (const double2 *)sc.mid);
+    hipLaunchKernelGGL(V.fn, dim3((unsigned)nblocks), dim3(V.threads), 0, st, f, s, P, T, (const Rng *)sc.rng_wave,
+                       sc.partials, sc.defer_list, theta, nrm, margin, (const double2 *)sc.mid);
     if (ev_b) hipEventRecord(ev_b, st);
     hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
@@ -403,6 +416,13 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(256), 0, st, s, (const double *)sc.partials, nblocks,
                        (const double *)sc.defer_partials);
     return hipGetLastError();
+}
+
+int orbit_variant_count() { return ORB_NVARIANTS; }
+void orbit_variant_shape(int v, int *threads, int *coarse)
+{
+    *threads = kOrbVariants[v].threads;
+    *coarse = kOrbVariants[v].coarse;
 }
 
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,

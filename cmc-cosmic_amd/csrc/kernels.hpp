@@ -44,6 +44,7 @@ struct Scratch {
     int64_t *sv3_fallback;      // first presumed head that failed the check
     // tidal stripping loop flag
     int32_t *strip_done;
+    unsigned *strip_arrive;     // arrival counter of the fused stripping pass
     // orbit stars deferred to the linear-scan path (65536 slots)
     int64_t *defer_list, *defer_kfirst;
     double *defer_partials;
@@ -77,7 +78,9 @@ hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64
 hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
                          const Scratch &sc, int64_t n_ub, hipStream_t st, const float *theta = nullptr,
                          const MlpNorm *nrm = nullptr, float margin = 0.f, hipEvent_t ev_a = nullptr,
-                         hipEvent_t ev_b = nullptr);
+                         hipEvent_t ev_b = nullptr, int variant = 0);
+int orbit_variant_count();
+void orbit_variant_shape(int v, int *threads, int *coarse);
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
                           const double *E, const double *J, double *out4, int64_t *kout, int32_t *circ,
                           hipStream_t st);
@@ -95,6 +98,7 @@ hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const 
 hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
 hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,
                         int64_t n_ub, int passes, bool begin, hipStream_t st);
+hipError_t launch_strip_check(DevScalars *s, const Scratch &sc, hipStream_t st);
 hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
 hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, hipStream_t st);
 hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,

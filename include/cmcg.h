@@ -46,6 +46,7 @@ extern "C" {
 #define CMCG_E_CAPACITY  -12   /* more stars than the context capacity */
 #define CMCG_E_NOMEM     -13   /* device allocation failed */
 #define CMCG_E_STATE     -14   /* call out of order (e.g. no bracket model loaded) */
+#define CMCG_E_STRIP     -15   /* tidal stripping did not end within the passes of a captured step */
 
 const char *cmcg_error_string(int code);
 
@@ -174,6 +175,17 @@ int cmcg_compute_energy(cmcg_ctx *ctx);              /* cmc_utils.c:604-653, 201
 int cmcg_init_cluster(cmcg_ctx *ctx);
 /* One whole Henon step in the order of src/cmc/cmc.c:240-407. */
 int cmcg_step(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
+/* The same step replayed from a HIP graph (captured on first use; the step
+ * alternates between two buffer states, so two graphs are kept).  The tidal
+ * stripping loop runs as 16 guarded passes inside the graph (CMCG_E_STRIP if it had
+ * not ended).  Results are identical to cmcg_step.  No counterpart in the reference:
+ * for callers that step many small clusters, where launch overhead dominates. */
+int cmcg_step_graph(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
+/* One step of each of `count` clusters (contexts on one device): every cluster's
+ * step graph is launched on its own stream before any wait, so the clusters run
+ * concurrently.  stats/errors (optional) receive one entry per cluster; returns the
+ * first error.  No counterpart in the reference (an ensemble driver). */
+int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors);
 
 /* ------------------------------------------------------------------ RNG access */
 /* Draw `ndraw` consecutive doubles of the substream of each of `count` consecutive
@@ -181,6 +193,15 @@ int cmcg_step(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
  * out is count*ndraw doubles, pair/star-major. */
 int cmcg_rng_draws(cmcg_ctx *ctx, int phase, uint64_t step, int64_t first, int64_t count,
                    int ndraw, double *out);
+
+/* ------------------------------------------------------------------ autotuning */
+/* Time every launch shape of the orbit sampler (threads per block x LDS samples of
+ * the (r, phi) search table) `reps` times on the context's current state and keep
+ * the fastest for later steps.  The stars are unchanged (the phase is rerun on the
+ * same inputs and its scalars are restored); per-star results do not depend on the
+ * shape.  No counterpart in the reference. */
+int cmcg_autotune(cmcg_ctx *ctx, int reps, int32_t *variant, int32_t *threads, int32_t *coarse,
+                  double *ms_best, double *ms_default);
 
 /* ------------------------------------------------------------------ timing */
 /* Average device time (ms) of the named kernel family over the calls since the last

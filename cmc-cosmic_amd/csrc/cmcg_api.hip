@@ -345,7 +345,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.max_tiles = (n_cap + 2047) / 2048 + 1;
     sc.esa_max = 1 << 16;
     sc.max_waves = (n_cap + 2) / 64 + 2;
-    sc.max_blocks = (n_cap + 255) / 256 + 2;
+    sc.max_blocks = (n_cap + 127) / 128 + 2;      // the smallest block of any launch shape: 128 threads
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
     sc.sort_aos_cap = n_cap + 4;
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||

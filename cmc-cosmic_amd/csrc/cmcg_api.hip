@@ -350,7 +350,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.sort_aos_cap = n_cap + 4;
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * 12)) ||
-        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / 16 + 4)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
+        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / 16 + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
@@ -361,7 +361,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.defer_list, 65536)) || (e = dalloc(c, &sc.defer_kfirst, 65536)) ||
         (e = dalloc(c, &sc.defer_partials, 65536 * 16)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
         (e = dalloc(c, &sc.esa_idx, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_racc, (size_t)sc.esa_max * 2)) ||
-        (e = dalloc(c, &sc.esa_v, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_n, 4)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_wave, (size_t)sc.max_waves)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
+        (e = dalloc(c, &sc.esa_v, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_n, 4)) || (e = dalloc(c, &sc.esa_raw_idx, 2048)) || (e = dalloc(c, &sc.esa_raw_part, 2 * 2048)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_wave, (size_t)sc.max_waves)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
         (e = dalloc(c, &c->T, 1))) {
         cmcg_destroy(c);
         return e;

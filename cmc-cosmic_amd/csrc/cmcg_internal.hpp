@@ -189,79 +189,137 @@ __device__ __forceinline__ double function_q(double rj, double mj, double inv_ns
     return 2.0 * (E - (pot + phis)) - Jr;
 }
 
+// Access to the (r, phi) nodes: the two global arrays, or a packed copy of 16-byte
+// (r, phi) nodes, whose Q evaluations and interpolations touch one cache line per
+// node instead of two.  Both hold the same values.
+struct SplitNodes {
+    const double *__restrict__ r;
+    const double *__restrict__ phi;
+    __device__ __forceinline__ double R(int64_t k) const { return r[k]; }
+    __device__ __forceinline__ double P(int64_t k) const { return phi[k]; }
+    __device__ __forceinline__ void RP(int64_t k, double &a, double &b) const { a = r[k]; b = phi[k]; }
+};
+struct PackedNodes {
+    const double2 *__restrict__ rp;
+    __device__ __forceinline__ double R(int64_t k) const { return rp[k].x; }
+    __device__ __forceinline__ double P(int64_t k) const { return rp[k].y; }
+    __device__ __forceinline__ void RP(int64_t k, double &a, double &b) const
+    {
+        double2 v = rp[k];
+        a = v.x;
+        b = v.y;
+    }
+};
+
 // cmc_nr.c:62-82 FindZero_r: largest k in [kmin, kmax] with r[k] < x (for sorted r).
-__device__ __forceinline__ int64_t find_zero_r(const double *__restrict__ r, int64_t kmin, int64_t kmax, double x)
+template <class N>
+__device__ __forceinline__ int64_t find_zero_r_n(const N &nd, int64_t kmin, int64_t kmax, double x)
 {
     int guard = 0;
     do {
         int64_t ktry = (kmin + kmax + 1) / 2;
-        if (r[ktry] < x) kmin = ktry;
+        if (nd.R(ktry) < x) kmin = ktry;
         else kmax = ktry - 1;
     } while (kmax != kmin && ++guard < 96);
     return kmin;
+}
+__device__ __forceinline__ int64_t find_zero_r(const double *__restrict__ r, int64_t kmin, int64_t kmax, double x)
+{
+    return find_zero_r_n(SplitNodes{r, nullptr}, kmin, kmax, x);
 }
 
 // cmc_utils.c:112-165 potential(r); the bisection bracket [lo, hi] may be narrowed
 // by the caller to any [lo, hi] with r[lo] < x <= r[hi] (lo >= 1): the result is the
 // unique largest k with r[k] < x, identical to the full-range search.
+template <class N>
+__device__ __forceinline__ double potential_at_n(const N &nd, int64_t n_max, double x, int64_t lo, int64_t hi,
+                                                 bool *bad)
+{
+    if (x == 0.0) return nd.P(0);
+    double r1 = nd.R(1);
+    if (x < r1) {
+        double p0 = nd.P(0);
+        return p0 - (p0 - nd.P(1)) * r1 / x;
+    }
+    int64_t i;
+    if (lo == hi - 1) i = lo;
+    else i = find_zero_r_n(nd, lo, hi, x);
+    if (i > n_max) { *bad = true; return 0.0; }   // x beyond the outer sentinel
+    double ri, pi, ri1, pi1;
+    nd.RP(i, ri, pi);
+    nd.RP(i + 1, ri1, pi1);
+    if (ri > x || ri1 < x) { *bad = true; return 0.0; }
+    if (i == 0) return pi1;
+    return pi + (pi1 - pi) * (1.0 / ri - 1.0 / x) / (1.0 / ri - 1.0 / ri1);
+}
 __device__ __forceinline__ double potential_at(const double *__restrict__ r, const double *__restrict__ phi,
                                                int64_t n_max, double x, int64_t lo, int64_t hi, bool *bad)
 {
-    if (x == 0.0) return phi[0];
-    double r1 = r[1];
-    if (x < r1) return phi[0] - (phi[0] - phi[1]) * r1 / x;
-    int64_t i;
-    if (lo == hi - 1) i = lo;
-    else i = find_zero_r(r, lo, hi, x);
-    if (i > n_max) { *bad = true; return 0.0; }   // x beyond the outer sentinel
-    double ri = r[i], ri1 = r[i + 1];
-    if (ri > x || ri1 < x) { *bad = true; return 0.0; }
-    if (i == 0) return phi[1];
-    double pi = phi[i], pi1 = phi[i + 1];
-    return pi + (pi1 - pi) * (1.0 / ri - 1.0 / x) / (1.0 / ri - 1.0 / ri1);
+    return potential_at_n(SplitNodes{r, phi}, n_max, x, lo, hi, bad);
 }
 
 // cmc_nr.c:316-383 find_zero_Q.  Returns -1 if the bracket cannot close.
-__device__ __forceinline__ int64_t find_zero_Q(const double *__restrict__ r, const double *__restrict__ phi,
-                                               double rj, double mj, double inv_nstar,
-                                               int64_t kmin, int64_t kmax, double E, double J)
+template <class N>
+__device__ __forceinline__ int64_t find_zero_Q_n(const N &nd, double rj, double mj, double inv_nstar,
+                                                 int64_t kmin, int64_t kmax, double E, double J)
 {
-    double qmin = function_q(rj, mj, inv_nstar, r[kmin], phi[kmin], E, J);
-    double qmax = function_q(rj, mj, inv_nstar, r[kmax], phi[kmax], E, J);
+    double a, b;
+    nd.RP(kmin, a, b);
+    double qmin = function_q(rj, mj, inv_nstar, a, b, E, J);
+    nd.RP(kmax, a, b);
+    double qmax = function_q(rj, mj, inv_nstar, a, b, E, J);
     int guard = 0;
     if (qmin < qmax) {
         do {
             int64_t ktry = (kmin + kmax + 1) / 2;
-            double qtry = function_q(rj, mj, inv_nstar, r[ktry], phi[ktry], E, J);
+            nd.RP(ktry, a, b);
+            double qtry = function_q(rj, mj, inv_nstar, a, b, E, J);
             if (qtry < 0.e0) kmin = ktry;
             else kmax = ktry - 1;
         } while (kmax != kmin && ++guard < 96);
     } else {
         do {
             int64_t ktry = (kmin + kmax + 1) / 2;
-            double qtry = function_q(rj, mj, inv_nstar, r[ktry], phi[ktry], E, J);
+            nd.RP(ktry, a, b);
+            double qtry = function_q(rj, mj, inv_nstar, a, b, E, J);
             if (qtry > 0.e0) kmin = ktry;
             else kmax = ktry - 1;
         } while (kmax != kmin && ++guard < 96);
     }
     return guard >= 96 ? -1 : kmin;
 }
+__device__ __forceinline__ int64_t find_zero_Q(const double *__restrict__ r, const double *__restrict__ phi,
+                                               double rj, double mj, double inv_nstar,
+                                               int64_t kmin, int64_t kmax, double E, double J)
+{
+    return find_zero_Q_n(SplitNodes{r, phi}, rj, mj, inv_nstar, kmin, kmax, E, J);
+}
 
 // cmc_orbit.c:722-740 set_a_b
+template <class N>
+__device__ __forceinline__ void set_a_b_n(const N &nd, double rj, double mj, const KParams &P, int64_t k,
+                                          double *a, double *b)
+{
+    if (k == 0) {
+        double r1, p1;
+        nd.RP(1, r1, p1);
+        *a = -P.cenma_m * P.madhoc;
+        *b = p1 + P.cenma_m * P.madhoc / r1 + mj * P.madhoc / rj;
+    } else {
+        double rk, pk, rk1, pk1;
+        nd.RP(k, rk, pk);
+        nd.RP(k + 1, rk1, pk1);
+        double Uk = pk + phi_s_own(rj, mj, P.inv_nstar, rk);
+        double Uk1 = pk1 + phi_s_own(rj, mj, P.inv_nstar, rk1);
+        *a = (Uk1 - Uk) / (1. / rk1 - 1. / rk);
+        *b = (Uk / rk1 - Uk1 / rk) / (1. / rk1 - 1. / rk);
+    }
+}
 __device__ __forceinline__ void set_a_b(const double *__restrict__ r, const double *__restrict__ phi,
                                         double rj, double mj, const KParams &P, int64_t k,
                                         double *a, double *b)
 {
-    if (k == 0) {
-        *a = -P.cenma_m * P.madhoc;
-        *b = phi[1] + P.cenma_m * P.madhoc / r[1] + mj * P.madhoc / rj;
-    } else {
-        double rk = r[k], rk1 = r[k + 1];
-        double Uk = phi[k] + phi_s_own(rj, mj, P.inv_nstar, rk);
-        double Uk1 = phi[k + 1] + phi_s_own(rj, mj, P.inv_nstar, rk1);
-        *a = (Uk1 - Uk) / (1. / rk1 - 1. / rk);
-        *b = (Uk / rk1 - Uk1 / rk) / (1. / rk1 - 1. / rk);
-    }
+    set_a_b_n(SplitNodes{r, phi}, rj, mj, P, k, a, b);
 }
 
 // cmc_nr.c:239-269 + :393-403: v_r^2 inside [r_k, r_{k+1}]
@@ -394,11 +452,11 @@ __device__ __forceinline__ int64_t coarse_index_estimate(const Coarse &c, double
 // potential(x) with the index search started from the LDS samples: the result is the
 // unique largest k with r[k] < x, so any bracket [lo, hi] with r[lo] < x <= r[hi]
 // gives the same index as the full-range search.
-__device__ __forceinline__ double potential_at_coarse(const double *__restrict__ r, const double *__restrict__ phi,
-                                                      int64_t n_max, double x, int64_t lo, int64_t hi,
-                                                      const Coarse &c, bool *bad)
+template <class N>
+__device__ __forceinline__ double potential_at_coarse_n(const N &nd, int64_t n_max, double x, int64_t lo, int64_t hi,
+                                                        const Coarse &c, bool *bad)
 {
-    if (x == 0.0 || x < r[1]) return potential_at(r, phi, n_max, x, lo, hi, bad);
+    if (x == 0.0 || x < nd.R(1)) return potential_at_n(nd, n_max, x, lo, hi, bad);
     int a = 0, b = c.cs;                    // largest s with cr[s] < x (cr[0] = r[0] < x)
     while (a < b) {
         int m = (a + b + 1) >> 1;
@@ -427,7 +485,13 @@ __device__ __forceinline__ double potential_at_coarse(const double *__restrict__
             }
         }
     }
-    return potential_at(r, phi, n_max, x, lo, hi, bad);
+    return potential_at_n(nd, n_max, x, lo, hi, bad);
+}
+__device__ __forceinline__ double potential_at_coarse(const double *__restrict__ r, const double *__restrict__ phi,
+                                                      int64_t n_max, double x, int64_t lo, int64_t hi,
+                                                      const Coarse &c, bool *bad)
+{
+    return potential_at_coarse_n(SplitNodes{r, phi}, n_max, x, lo, hi, c, bad);
 }
 
 // Narrow a find_zero_Q bracket [lo, hi] with pred(Q(lo)) true and pred(Q(hi)) false
@@ -459,10 +523,11 @@ struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; 
 
 // cmc_orbit.c:413-540 calc_orbit_new after get_positive_Q_index has produced ktemp
 // (an index with Q > 0; ktemp < 0 means "no positive Q": circular orbit).
-__device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r, const double *__restrict__ phi,
-                                                   int64_t n_max, int64_t j, double rj, double mj,
-                                                   const KParams &P, double E, double J, int64_t ktemp,
-                                                   const Coarse *C = nullptr)
+template <class N>
+__device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *__restrict__ r,
+                                                     const double *__restrict__ phi, int64_t n_max, int64_t j,
+                                                     double rj, double mj, const KParams &P, double E, double J,
+                                                     int64_t ktemp, const Coarse *C)
 {
     OrbitRS o;
     o.brent = 0; o.linscan = 0; o.err = 0; o.hint_hits = 0; o.hint_misses = 0;
@@ -486,8 +551,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
         int64_t hi = coarse_index_estimate(*C, (double)(__expf(C->lrp) * ep)) + 2;
         lo = lo < 0 ? 0 : lo;
         hi = hi > ktemp ? ktemp : hi;
-        if (lo < hi && function_q(rj, mj, inv, r[lo], phi[lo], E, J) < 0.0 &&
-            (hi == ktemp || function_q(rj, mj, inv, r[hi], phi[hi], E, J) >= 0.0)) {
+        if (lo < hi && function_q(rj, mj, inv, nd.R(lo), nd.P(lo), E, J) < 0.0 &&
+            (hi == ktemp || function_q(rj, mj, inv, nd.R(hi), nd.P(hi), E, J) >= 0.0)) {
             kp_lo = lo; kp_hi = hi; got_p = vp = true; o.hint_hits++;
         } else {
             o.hint_misses++;
@@ -496,8 +561,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
         hi = coarse_index_estimate(*C, (double)(__expf(C->lra) * ep)) + 2;
         lo = lo < ktemp ? ktemp : lo;
         hi = hi > n_max + 1 ? n_max + 1 : hi;
-        if (lo < hi && (lo == ktemp || function_q(rj, mj, inv, r[lo], phi[lo], E, J) > 0.0) &&
-            !(function_q(rj, mj, inv, r[hi], phi[hi], E, J) > 0.0)) {
+        if (lo < hi && (lo == ktemp || function_q(rj, mj, inv, nd.R(lo), nd.P(lo), E, J) > 0.0) &&
+            !(function_q(rj, mj, inv, nd.R(hi), nd.P(hi), E, J) > 0.0)) {
             ka_lo = lo; ka_hi = hi; got_a = va = true; o.hint_hits++;
         } else {
             o.hint_misses++;
@@ -536,8 +601,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
     }
     if (vp && C) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi);
     if (va && C) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi);
-    int64_t kmin = find_zero_Q(r, phi, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
-    int64_t kmax = find_zero_Q(r, phi, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
+    int64_t kmin = find_zero_Q_n(nd, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
+    int64_t kmax = find_zero_Q_n(nd, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
     if (kmin < 0 || kmax < 0) {
         o.err = CMCG_E_BRACKET;
         o.rp = o.ra = rj; o.kmin = o.kmax = j; o.dQdrp = o.dQdra = 0.0; o.circular = 1;
@@ -545,11 +610,11 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
     }
     double a, b, rmin, rmax, inside_sqrt;
     bool bad = false;
-    set_a_b(r, phi, rj, mj, P, kmin, &a, &b);
+    set_a_b_n(nd, rj, mj, P, kmin, &a, &b);
     inside_sqrt = a * a - 2.0 * J * J * (b - E);
     if (inside_sqrt < 0.0) {
         rmin = -1.0 * J * J / a;
-        if ((rmin < r[kmin]) || (rmin > r[kmin + 1])) {
+        if ((rmin < nd.R(kmin)) || (rmin > nd.R(kmin + 1))) {
             rmin = find_root_vr(r, phi, rj, mj, P, kmin, E, J, &bad);
             o.brent = 1;
         }
@@ -557,11 +622,11 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
         rmin = J * J / (-a + sqrt(inside_sqrt));
     }
     double dQdr_min = 2.0 * J * J / (rmin * rmin * rmin) + 2.0 * a / (rmin * rmin);
-    set_a_b(r, phi, rj, mj, P, kmax, &a, &b);
+    set_a_b_n(nd, rj, mj, P, kmax, &a, &b);
     inside_sqrt = a * a - 2.0 * J * J * (b - E);
     if (inside_sqrt < 0.0) {
         rmax = -a / (2.0 * (b - E));
-        if ((rmax < r[kmax]) || (rmax > r[kmax + 1])) {
+        if ((rmax < nd.R(kmax)) || (rmax > nd.R(kmax + 1))) {
             rmax = find_root_vr(r, phi, rj, mj, P, kmax, E, J, &bad);
             o.brent = 1;
         }
@@ -577,6 +642,14 @@ __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r,
         o.dQdrp = dQdr_min; o.dQdra = dQdr_max; o.circular = 0;
     }
     return o;
+}
+
+__device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r, const double *__restrict__ phi,
+                                                   int64_t n_max, int64_t j, double rj, double mj,
+                                                   const KParams &P, double E, double J, int64_t ktemp,
+                                                   const Coarse *C = nullptr)
+{
+    return calc_orbit_from_n(SplitNodes{r, phi}, r, phi, n_max, j, rj, mj, P, E, J, ktemp, C);
 }
 
 // get_positive_Q_index (cmc_orbit.c:30-59) with its linear scan, then calc_orbit_from.

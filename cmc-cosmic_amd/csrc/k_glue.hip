@@ -194,7 +194,8 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
 constexpr int SV3_COARSE = 1024;      // LDS samples of r for potential(rOld)
 
 __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
-                                                 int64_t *__restrict__ blk_cnt, const double2 *__restrict__ mid)
+                                                 int64_t *__restrict__ blk_cnt, const double2 *__restrict__ mid,
+                                                 const double2 *__restrict__ nodes)
 {
     __shared__ int cnt;
     __shared__ double cr[SV3_COARSE + 1];
@@ -212,7 +213,8 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
         double m = f.m[i], ri = f.r[i];
         double rO = f.rOld[i];
         bool bad = false;
-        double Unewrold = potential_at_coarse(f.r, f.phi, n, rO, 1, n + 1, C, &bad) + phi_s_own(ri, m, P.inv_nstar, rO);
+        double Unewrold = potential_at_coarse_n(PackedNodes{nodes}, n, rO, 1, n + 1, C, &bad) +
+                          phi_s_own(ri, m, P.inv_nstar, rO);
         if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
         double Unewrnew = f.phi[i] + phi_s_own(ri, m, P.inv_nstar, ri);
         double vtold = f.vtold[i], vrold = f.vrold[i];
@@ -477,7 +479,7 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
-                       (const double2 *)sc.mid);
+                       (const double2 *)sc.mid, (const double2 *)sc.nodes);
     hipLaunchKernelGGL(sv3_scan_kernel, dim3(1), dim3(1024), 0, st, s, sc.sv3_blk, nb);
     hipLaunchKernelGGL(sv3_compact_kernel, dim3((unsigned)nb), dim3(GT), 0, st, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.sv3_blk, sc.excess_list);

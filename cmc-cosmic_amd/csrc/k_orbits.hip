@@ -80,14 +80,15 @@ __device__ __forceinline__ void removed(double (&red)[NRED], double Ej, double J
 
 // The part of get_positions_loop after the positive-Q index is known
 // (cmc_evolution_thr.c:832-934).
-__device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P, Rng rng, const Coarse *C,
+template <class N>
+__device__ void sample_star(const N &nd, const DevFields &f, DevScalars *s, const KParams &P, Rng rng, const Coarse *C,
                             int64_t n, int64_t j, double rj, double mj, double E,
                             double Ej, double J, int64_t ktemp, int &flags, int &ntry, int &hits, int &misses,
                             double (&red)[NRED])
 {
     const double *__restrict__ r = f.r;
     const double *__restrict__ phi = f.phi;
-    OrbitRS o = calc_orbit_from(r, phi, n, j, rj, mj, P, E, J, ktemp, C);
+    OrbitRS o = calc_orbit_from_n(nd, r, phi, n, j, rj, mj, P, E, J, ktemp, C);
     if (o.err) dev_set_error(s, o.err, j);
     hits = o.hint_hits;
     misses = o.hint_misses;
@@ -111,7 +112,7 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
     // narrowed bracket for potential(r): r in [rmin, rmax] within [r_kmin, r_kmax+1]
     int64_t lo0 = o.kmin < 1 ? 1 : o.kmin;
     int64_t hi0 = o.kmax + 1 > n + 1 ? n + 1 : o.kmax + 1;
-    const double r_lo0 = r[lo0], r_hi0 = r[hi0];
+    const double r_lo0 = nd.R(lo0), r_hi0 = nd.R(hi0);
     double X = 0.0, rr = 0.0, vr = 0.0, potr = 0.0;
     int k;
     bool bad = false;
@@ -123,7 +124,7 @@ __device__ void sample_star(const DevFields &f, DevScalars *s, const KParams &P,
         int64_t lo = lo0, hi = hi0;
         if (!(r_lo0 < rr)) lo = 1;
         if (!(rr <= r_hi0)) hi = n + 1;
-        potr = C ? potential_at_coarse(r, phi, n, rr, lo, hi, *C, &bad) : potential_at(r, phi, n, rr, lo, hi, &bad);
+        potr = C ? potential_at_coarse_n(nd, n, rr, lo, hi, *C, &bad) : potential_at_n(nd, n, rr, lo, hi, &bad);
         double pot = potr + phi_s_own(rj, mj, P.inv_nstar, rr);
         double drds = 0.25 * (rmax - rmin) * (3.0 - 3.0 * s0 * s0);
         double Q = 2.0 * E - 2.0 * pot - J * J / rr / rr;
@@ -161,7 +162,8 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
                                                              int64_t *__restrict__ defer_list,
                                                              const float *__restrict__ theta,
                                                              const MlpNorm *__restrict__ nrm, float margin,
-                                                             const double2 *__restrict__ mid)
+                                                             const double2 *__restrict__ mid,
+                                                             const double2 *__restrict__ nodes)
 {
     __shared__ NibbleTables<6> lt;
     __shared__ double cr[COARSE + 1], cphi[COARSE + 1];
@@ -214,7 +216,8 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
                     C.lra = y[1];
                 }
                 Rng rng = rng_jump_lds<6>(lt, wave_base[(j - 1) >> 6], (uint32_t)((j - 1) & 63));
-                sample_star(f, s, P, rng, &C, n, j, rj, mj, E, Ej, J, j, flags, ntry, hits, misses, red);
+                sample_star(PackedNodes{nodes}, f, s, P, rng, &C, n, j, rj, mj, E, Ej, J, j, flags, ntry, hits,
+                            misses, red);
             }
         }
         f.flags[j] = flags;
@@ -281,8 +284,8 @@ __global__ __launch_bounds__(ORB_THREADS) void defer_tail_kernel(DevFields f, De
         double red[NRED];
         for (int i = 0; i < NRED; i++) red[i] = 0.0;
         int flags = CMCG_SF_LINSCAN, ntry = 0, hits = 0, misses = 0;
-        sample_star(f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), nullptr, n, j, rj, mj, E, Ej, J, kt,
-                    flags, ntry, hits, misses, red);
+        sample_star(SplitNodes{f.r, f.phi}, f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), nullptr, n, j,
+                    rj, mj, E, Ej, J, kt, flags, ntry, hits, misses, red);
         f.flags[j] = flags;
         f.ntry[j] = ntry;
         count_star(red, flags, ntry, hits, misses, false);
@@ -377,7 +380,7 @@ __global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count,
 // block and LDS samples of the (r, phi) table.  Every variant gives the same stars;
 // only the block order of the escape sums differs.
 using OrbFn = void (*)(DevFields, DevScalars *, KParams, const RngTable *, const Rng *, double *, int64_t *,
-                       const float *, const MlpNorm *, float, const double2 *);
+                       const float *, const MlpNorm *, float, const double2 *, const double2 *);
 struct OrbVariant { OrbFn fn; int threads, coarse; };
 const OrbVariant kOrbVariants[] = {
     {orbits_kernel<256, 1024>, 256, 1024}, {orbits_kernel<256, 512>, 256, 512}, {orbits_kernel<256, 2048>, 256, 2048},
@@ -406,7 +409,8 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     if (ev_a) hipEventRecord(ev_a, st);
     hipLaunchKernelGGL(V.fn, dim3((unsigned)nblocks), dim3(V.threads), 0, st, f, s, P, T, (const Rng *)sc.rng_wave,
-                       sc.partials, sc.defer_list, theta, nrm, margin, (const double2 *)sc.mid);
+                       sc.partials, sc.defer_list, theta, nrm, margin, (const double2 *)sc.mid,
+                       (const double2 *)sc.nodes);
     if (ev_b) hipEventRecord(ev_b, st);
     hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,

@@ -33,7 +33,8 @@ constexpr int CT = 256;             // threads per tile
 constexpr int CI = 8;               // items per thread
 constexpr int CTILE = CT * CI;
 constexpr int SCAN_T = 1024;
-constexpr int REPAIR_ROUNDS = 3;      // Jacobi rounds after the segment-and-walk pass
+constexpr int REPAIR_ROUNDS = 1;      // Jacobi rounds after the segment-and-walk pass (then sequential)
+constexpr int RAW_CAP = 2048;         // sensitive elements sorted in LDS by the walk
 
 typedef __int128 i128;
 
@@ -239,6 +240,9 @@ struct EsaArgs {
     double *s_v;                //   exact chain value at the element
     int64_t *s_n;               // number of sensitive elements (uncapped)
     int64_t smax;               // list capacity
+    int64_t *raw_n;             // sensitive elements appended by the classification, unordered:
+    int64_t *raw_idx;           //   index
+    i128 *raw_part;             //   regular-increment prefix inside its tile
     const int64_t *Lp;
     int K;
 };
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(CT) void esa_sum_kernel(EsaArgs a)
 {
     __shared__ i128 lds[CT];
     const int64_t L = *a.Lp;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { a.first_bad[0] = 0; a.first_bad[1] = L; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { a.first_bad[0] = 0; a.first_bad[1] = L; *a.raw_n = 0; }
     const int64_t base = (int64_t)blockIdx.x * CTILE;
     if (base >= L) return;
     i128 acc = 0;
@@ -379,9 +383,19 @@ __global__ __launch_bounds__(CT) void esa_classify_kernel(EsaArgs a)
         if (j < L && ((it.sens >> u) & 1)) cnt += 1;
     }
     i128 tr, tc;
-    block_excl_scan<CT>(racc, lds, &tr);
+    i128 R = block_excl_scan<CT>(racc, lds, &tr);
     block_excl_scan<CT>(cnt, lds, &tc);
     if (threadIdx.x == 0) { a.tsumR[tile] = tr; a.tcnt[tile] = tc; }
+    // the few sensitive elements, unordered, with their in-tile regular prefix
+    for (int u = 0; u < CI; u++) {
+        int64_t j = tile * CTILE + (int64_t)threadIdx.x * CI + u;
+        if (j >= L) break;
+        if ((it.sens >> u) & 1) {
+            unsigned long long slot = atomicAdd((unsigned long long *)a.raw_n, 1ull);
+            if (slot < (unsigned long long)RAW_CAP) { a.raw_idx[slot] = j; a.raw_part[slot] = R; }
+        }
+        R += it.R[u];
+    }
 }
 
 // in-tile exclusive prefixes of the regular increments and sensitive counts
@@ -399,9 +413,11 @@ __device__ void tile_prefixes(const EsaArgs &a, int64_t tile, int64_t L, const T
     *kpos = (int64_t)(a.toff[tile] + block_excl_scan<CT>(cnt, lds, &tc));
 }
 
+// Ordered compaction, used only when more than RAW_CAP elements are sensitive.
 __global__ __launch_bounds__(CT) void esa_compact_kernel(EsaArgs a)
 {
     __shared__ i128 lds[CT];
+    if (*a.s_n <= RAW_CAP) return;
     const int64_t L = *a.Lp;
     const int64_t tile = blockIdx.x;
     if (tile * CTILE >= L) return;
@@ -434,6 +450,22 @@ __global__ __launch_bounds__(256) void esa_walk_kernel(EsaArgs a)
     __shared__ i128 Rl;
     const int t = threadIdx.x;
     int64_t ns = *a.s_n;
+    if (ns <= RAW_CAP) {
+        // rank sort of the classification's list by index (indices are distinct);
+        // the regular prefix gains its tile's offset
+        __shared__ int64_t ridx[RAW_CAP];
+        for (int64_t e = t; e < ns; e += blockDim.x) ridx[e] = a.raw_idx[e];
+        __syncthreads();
+        for (int64_t e = t; e < ns; e += blockDim.x) {
+            const int64_t j = ridx[e];
+            int64_t rank = 0;
+            for (int64_t q = 0; q < ns; q++) rank += ridx[q] < j;
+            a.s_idx[rank] = j;
+            a.s_racc[rank] = a.raw_part[e] + a.tpreR[j / CTILE];
+        }
+        __threadfence();
+        __syncthreads();
+    }
     if (ns > a.smax) ns = a.smax;
     if (t == 0) { Vd = *a.v0p; Rl = 0; }
     __syncthreads();
@@ -510,22 +542,37 @@ __global__ __launch_bounds__(CT) void esa_rebuild_kernel(EsaArgs a)
         if (kk >= 0 && kk < ns_eff && a.s_idx[kk] == base - 1) before = a.s_v[kk];
         else before = esa_value(a, kk, ns_eff, v0, a.tpreR[tile]);
     }
-    i128 nacc = 0;
     int64_t bad = L;
     for (int u = 0; u < CI; u++) {
         int64_t j = j0 + u;
         if (j < L) {
             double want = before + a.sign * it.xv[u];
             if (want != nv[u] && j < bad) bad = j;
-            nacc += incr(before, it.xv[u], a.sign, a.K);
             a.g[j] = nv[u];
         }
         before = nv[u];
     }
     if (bad < L) atomicMin((unsigned long long *)&a.first_bad[1], (unsigned long long)bad);
+}
+
+// Only when the rebuild found a violation: per-tile sums of the Jacobi increments
+// fl(g_{j-1} + s x_j) - g_{j-1} of the rebuilt chain, which the repair rounds scan.
+__global__ __launch_bounds__(CT) void chain_seed_from_g_kernel(ChainArgs a)
+{
+    __shared__ i128 lds[CT];
+    const int64_t L = *a.Lp;
+    if (a.first_bad[1] >= L) return;
+    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    if (base >= L) return;
+    const double v0 = *a.v0p;
+    i128 acc = 0;
+    for (int u = 0; u < CI; u++) {
+        int64_t j = base + (int64_t)threadIdx.x * CI + u;
+        if (j < L) acc += incr(j == 0 ? v0 : a.g[j - 1], a.x[j], a.sign, a.K);
+    }
     i128 tot;
-    block_excl_scan<CT>(nacc, lds, &tot);
-    if (t == 0) a.tsum[tile] = tot;
+    block_excl_scan<CT>(acc, lds, &tot);
+    if (threadIdx.x == 0) a.tsum[blockIdx.x] = tot;
 }
 
 // ---------------------------------------------------------------- the three chains
@@ -655,6 +702,7 @@ hipError_t run_chain(ChainArgs a, EsaArgs e, int rounds, int64_t n_ub, hipStream
     hipLaunchKernelGGL(esa_compact_kernel, gt, bt, 0, st, e);
     hipLaunchKernelGGL(esa_walk_kernel, dim3(1), dim3(256), 0, st, e);
     hipLaunchKernelGGL(esa_rebuild_kernel, gt, bt, 0, st, e);
+    hipLaunchKernelGGL(chain_seed_from_g_kernel, gt, bt, 0, st, a);
     for (int it = 0; it < rounds; it++) {
         hipLaunchKernelGGL(chain_scan_kernel, dim3(1), dim3(SCAN_T), 0, st, a);
         hipLaunchKernelGGL(chain_round_kernel, gt, bt, 0, st, a);
@@ -692,6 +740,9 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     e.s_v = sc.esa_v;
     e.s_n = sc.esa_n;
     e.smax = sc.esa_max;
+    e.raw_n = sc.esa_n + 1;
+    e.raw_idx = sc.esa_raw_idx;
+    e.raw_part = (i128 *)sc.esa_raw_part;
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
     hipLaunchKernelGGL(pot_init_kernel, dim3(1), dim3(1), 0, st, s);
     if (n_ub > 0) hipLaunchKernelGGL(pot_count_kernel, dim3(g1), dim3(256), 0, st, f.r, s);

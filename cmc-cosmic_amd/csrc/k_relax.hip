@@ -279,19 +279,22 @@ hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase
     return hipGetLastError();
 }
 
+// Packed (r, phi) nodes for the searches and the middle table (every 16th node).
 __global__ void mid_build_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
-                                 double2 *__restrict__ mid)
+                                 double2 *__restrict__ nodes, double2 *__restrict__ mid)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t k = 16 * t;
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k > s->n_max + 1) return;
-    mid[t] = make_double2(r[k], phi[k]);
+    double2 v = make_double2(r[k], phi[k]);
+    nodes[k] = v;
+    if ((k & 15) == 0) mid[k >> 4] = v;
 }
 
 hipError_t launch_mid_build(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st)
 {
-    int64_t nt = (n_ub + 1) / 16 + 1;
-    hipLaunchKernelGGL(mid_build_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s, sc.mid);
+    int64_t nn = n_ub + 2;
+    hipLaunchKernelGGL(mid_build_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s,
+                       sc.nodes, sc.mid);
     return hipGetLastError();
 }
 

@@ -20,7 +20,9 @@ struct Scratch {
     int64_t *esa_idx;
     double *esa_racc;           // int128 per entry
     double *esa_v;
-    int64_t *esa_n;
+    int64_t *esa_n;             // [0] sensitive count, [1] appended count
+    int64_t *esa_raw_idx;       // appended sensitive elements (unordered), 2048
+    double *esa_raw_part;       // their in-tile prefixes (int128 each)
     int64_t esa_max;
     // deterministic block partials: 8 doubles per block
     double *partials;
@@ -33,6 +35,7 @@ struct Scratch {
     int64_t max_sort_tiles;
     uint64_t *sort_aos;         // packed payload records, 12 words per star
     double2 *mid;               // (r, phi) at every 16th node (search middle level)
+    double2 *nodes;             // packed (r, phi) of every node, for the searches
     int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event
     // walk ends and merge flags, per-block counts, fallback flag

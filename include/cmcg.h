@@ -197,9 +197,11 @@ int cmcg_rng_draws(cmcg_ctx *ctx, int phase, uint64_t step, int64_t first, int64
 /* ------------------------------------------------------------------ autotuning */
 /* Time every launch shape of the orbit sampler (threads per block x LDS samples of
  * the (r, phi) search table) `reps` times on the context's current state and keep
- * the fastest for later steps.  The stars are unchanged (the phase is rerun on the
- * same inputs and its scalars are restored); per-star results do not depend on the
- * shape.  No counterpart in the reference. */
+ * the fastest for later steps.  The state going into the next step is unchanged
+ * (the phase is rerun on the same inputs, its scalars are restored, and only its own
+ * outputs - r_peri, r_apo, X, rnew, ... - are left from the last trial, which the next
+ * step recomputes); per-star results do not depend on the shape.  No counterpart in
+ * the reference. */
 int cmcg_autotune(cmcg_ctx *ctx, int reps, int32_t *variant, int32_t *threads, int32_t *coarse,
                   double *ms_best, double *ms_default);
 

@@ -350,7 +350,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.sort_aos_cap = n_cap + 4;
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * 12)) ||
-        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / 16 + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
+        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / 16 + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
@@ -811,7 +811,8 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
     }
     {
         PhaseTimer t(c, "glue_post_sort");
-        LAUNCH(launch_set_velocities3(c->f, c->s, c->kp, c->sc, nub, c->stream));
+        // the sort above read the old sorted radii from what is now the alternate r
+        LAUNCH(launch_set_velocities3(c->f, c->s, c->kp, c->sc, nub, c->stream, c->alt.r));
         LAUNCH(launch_compute_energy(c->f, c->s, c->kp, c->sc, nub, c->stream));
         hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->s);
     }

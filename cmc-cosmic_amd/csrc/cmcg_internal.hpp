@@ -38,6 +38,7 @@ struct DevScalars {
     int64_t first_inf;           // potential_calculate: first slot with r >= SF_INFINITY
     int64_t n_defer;             // orbit phase: stars whose own-index Q <= 0 (linear-scan path)
     int64_t sort_n;              // sort: live stars after dropping the r == SF_INFINITY tail
+    int64_t sort_in;             // sort: slots sorted (N_MAX_NEW before the sort)
     // orbit statistics
     int64_t n_circular, n_removed_E, n_removed_rt, n_destroyed, n_brent, n_linscan;
     int64_t tries_total, tries_max, n_bracket_hit, n_bracket_fallback;

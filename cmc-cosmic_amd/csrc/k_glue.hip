@@ -191,11 +191,52 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
 // set_velocities3 per-star part.  Stars with vnew2 <= 0 ("events") carry excess
 // energy that the following stars absorb (cmc_utils.c:268-310); they are counted
 // per block for an ordered compaction.
+// potential(r) of the new potential at every old node, by merging the old sorted
+// radii (the array the sort read, left intact in the alternate buffer) with the new
+// ones: each thread takes MERGE_CH consecutive old nodes, finds the first one's
+// interval by bisection and walks forward for the rest.  The interval is the largest
+// k with r_k < x, as in potential_at, and the interpolation is the same expression,
+// so the values equal potential_at(x) bit for bit.
+constexpr int MERGE_CH = 16;
+
+__global__ __launch_bounds__(GT) void old_node_potential_kernel(const double *__restrict__ r_old,
+                                                                const double2 *__restrict__ nodes, DevScalars *s,
+                                                                double *__restrict__ u_old)
+{
+    const int64_t n_old = s->sort_in, n = s->n_max;
+    const int64_t s0 = 1 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * MERGE_CH;
+    if (s0 > n_old) return;
+    const PackedNodes nd{nodes};
+    const double r1 = nd.R(1);
+    int64_t i = -1;
+    for (int64_t sl = s0; sl < s0 + MERGE_CH && sl <= n_old; sl++) {
+        const double x = r_old[sl];
+        double u;
+        if (x == 0.0 || x < r1) {
+            bool bad = false;
+            u = potential_at_n(nd, n, x, 1, n + 1, &bad);
+        } else {
+            if (i < 0) i = find_zero_r_n(nd, 1, n + 1, x);
+            else
+                while (i + 1 <= n + 1 && nd.R(i + 1) < x) i++;
+            if (i > n) { dev_set_error(s, CMCG_E_BRACKET, sl); u = 0.0; }
+            else {
+                double ri, pi, ri1, pi1;
+                nd.RP(i, ri, pi);
+                nd.RP(i + 1, ri1, pi1);
+                u = i == 0 ? pi1 : pi + (pi1 - pi) * (1.0 / ri - 1.0 / x) / (1.0 / ri - 1.0 / ri1);
+            }
+        }
+        u_old[sl] = u;
+    }
+}
+
 constexpr int SV3_COARSE = 1024;      // LDS samples of r for potential(rOld)
 
 __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
                                                  int64_t *__restrict__ blk_cnt, const double2 *__restrict__ mid,
-                                                 const double2 *__restrict__ nodes)
+                                                 const double2 *__restrict__ nodes, const double *__restrict__ u_old,
+                                                 const uint32_t *__restrict__ perm)
 {
     __shared__ int cnt;
     __shared__ double cr[SV3_COARSE + 1];
@@ -213,7 +254,9 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
         double m = f.m[i], ri = f.r[i];
         double rO = f.rOld[i];
         bool bad = false;
-        double Unewrold = potential_at_coarse_n(PackedNodes{nodes}, n, rO, 1, n + 1, C, &bad) +
+        // potential(rOld): merged values through the sort's permutation when this step
+        // sorted the stars, otherwise a search
+        double Unewrold = (u_old ? u_old[perm[i - 1]] : potential_at_coarse_n(PackedNodes{nodes}, n, rO, 1, n + 1, C, &bad)) +
                           phi_s_own(ri, m, P.inv_nstar, rO);
         if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
         double Unewrnew = f.phi[i] + phi_s_own(ri, m, P.inv_nstar, ri);
@@ -471,15 +514,21 @@ hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, h
     return hipGetLastError();
 }
 hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
-                                  int64_t n_ub, hipStream_t st)
+                                  int64_t n_ub, hipStream_t st, const double *r_old_sorted)
 {
     int64_t nb = grid_for(n_ub);
     if (nb > sc.max_blocks) return hipErrorInvalidValue;
     unsigned ge = grid_for(n_ub);
     hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
+    if (r_old_sorted) {
+        int64_t nt = (n_ub + MERGE_CH - 1) / MERGE_CH;
+        hipLaunchKernelGGL(old_node_potential_kernel, dim3((unsigned)((nt + GT - 1) / GT)), dim3(GT), 0, st,
+                           r_old_sorted, (const double2 *)sc.nodes, s, sc.u_old);
+    }
     hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
-                       (const double2 *)sc.mid, (const double2 *)sc.nodes);
+                       (const double2 *)sc.mid, (const double2 *)sc.nodes,
+                       r_old_sorted ? (const double *)sc.u_old : nullptr, (const uint32_t *)sc.vals_a);
     hipLaunchKernelGGL(sv3_scan_kernel, dim3(1), dim3(1024), 0, st, s, sc.sv3_blk, nb);
     hipLaunchKernelGGL(sv3_compact_kernel, dim3((unsigned)nb), dim3(GT), 0, st, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.sv3_blk, sc.excess_list);

@@ -138,6 +138,12 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scan_kernel(uint32_t *__restric
     }
 }
 
+// Stable scatter of one 4096-key tile.  Wave w ranks the contiguous sub-tile
+// [w*1024, (w+1)*1024) on its own (ballot matching of the 8 digit bits, a per-wave
+// digit counter in LDS updated by the highest lane of each match group), so the
+// item loop needs no block barrier; the (digit, wave) counts then give each wave's
+// offset inside the tile's digit runs.  Index order (wave, item, lane) is the tile's
+// index order, so equal digits keep their order.
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *__restrict__ keys_in,
                                                                 const uint32_t *__restrict__ vals_in,
                                                                 uint64_t *__restrict__ keys_out,
@@ -145,23 +151,23 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
                                                                 int shift, const uint32_t *__restrict__ tile_off,
                                                                 int64_t ntiles)
 {
+    constexpr int NW = RS_THREADS / 64;
     __shared__ uint64_t skeys[RS_TILE];
     __shared__ uint32_t svals[RS_TILE];
-    __shared__ uint32_t whist[4][256];
+    __shared__ uint32_t wcnt[NW][256];
     __shared__ uint32_t cnt[256];
     __shared__ uint32_t dstart[256];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-    cnt[t] = 0;
-    whist[0][t] = 0; whist[1][t] = 0; whist[2][t] = 0; whist[3][t] = 0;
-    __syncthreads();
+    const int64_t wbase = base + (int64_t)w * (64 * RS_ITEMS);
+    for (int d = lane; d < 256; d += 64) wcnt[w][d] = 0;
     uint64_t k[RS_ITEMS];
     uint32_t v[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     for (int u = 0; u < RS_ITEMS; u++) {
-        int64_t i = base + u * RS_THREADS + t;
+        int64_t i = wbase + u * 64 + lane;
         bool valid = i < n;
         k[u] = valid ? keys_in[i] : 0ull;
         v[u] = valid ? vals_in[i] : 0u;
@@ -173,20 +179,22 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
             uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        if (!valid) peers = 0;
-        uint32_t rw = (uint32_t)__popcll(peers & lt_mask);
-        if (valid && rw == 0) whist[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t r = cnt[d] + rw;
-            for (int w2 = 0; w2 < w; w2++) r += whist[w2][d];
-            rank[u] = r;
-        }
-        __syncthreads();
-        cnt[t] += whist[0][t] + whist[1][t] + whist[2][t] + whist[3][t];
-        whist[0][t] = 0; whist[1][t] = 0; whist[2][t] = 0; whist[3][t] = 0;
-        __syncthreads();
+        uint32_t before = valid ? wcnt[w][d] : 0u;
+        rank[u] = before + (uint32_t)__popcll(peers & lt_mask);
+        if (valid && (peers >> lane) == 1ull) wcnt[w][d] = before + (uint32_t)__popcll(peers);
     }
+    __syncthreads();
+    // per digit (thread t = digit): the waves' offsets and the tile total
+    {
+        uint32_t tot = 0;
+        for (int w2 = 0; w2 < NW; w2++) {
+            uint32_t c = wcnt[w2][t];
+            wcnt[w2][t] = tot;
+            tot += c;
+        }
+        cnt[t] = tot;
+    }
+    __syncthreads();
     // exclusive scan of the tile's digit counts
     dstart[t] = cnt[t];
     __syncthreads();
@@ -201,10 +209,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
     dstart[t] = incl - cnt[t];
     __syncthreads();
     for (int u = 0; u < RS_ITEMS; u++) {
-        int64_t i = base + u * RS_THREADS + t;
+        int64_t i = wbase + u * 64 + lane;
         if (i < n) {
             uint32_t d = (uint32_t)((k[u] >> shift) & 255);
-            uint32_t p = dstart[d] + rank[u];
+            uint32_t p = dstart[d] + wcnt[w][d] + rank[u];
             skeys[p] = k[u];
             svals[p] = v[u];
         }
@@ -306,7 +314,7 @@ __global__ void rs_finish_kernel(DevScalars *s)
     s->n_max_new = s->sort_n;
 }
 
-__global__ void rs_begin_kernel(DevScalars *s) { s->sort_n = 0; s->ntie_runs = 0; }
+__global__ void rs_begin_kernel(DevScalars *s) { s->sort_n = 0; s->ntie_runs = 0; s->sort_in = s->n_max_new; }
 } // namespace
 
 // Sort slots 1..n_max_new by (key field, id).  src/dst are parallel lists of payload

@@ -36,6 +36,7 @@ struct Scratch {
     uint64_t *sort_aos;         // packed payload records, 12 words per star
     double2 *mid;               // (r, phi) at every 16th node (search middle level)
     double2 *nodes;             // packed (r, phi) of every node, for the searches
+    double *u_old;              // new potential at the old sorted radii (set_velocities3)
     int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event
     // walk ends and merge flags, per-block counts, fallback flag
@@ -104,8 +105,9 @@ hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, con
 hipError_t launch_strip_check(DevScalars *s, const Scratch &sc, hipStream_t st);
 hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
 hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, hipStream_t st);
+// r_old_sorted: the radii the sort of this step read (old sorted order), or nullptr
 hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
-                                  int64_t n_ub, hipStream_t st);
+                                  int64_t n_ub, hipStream_t st, const double *r_old_sorted = nullptr);
 hipError_t launch_compute_energy(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
                                  int64_t n_ub, hipStream_t st);
 

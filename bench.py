@@ -92,6 +92,9 @@ def main():
         ctx.step()
     ctx.mlp_harvest(False)
     log(f"warmup {args.warmup} steps done, N={ctx.scalars()['n_max']}, rows harvested {ctx.mlp_rows(copy=False)}")
+    tune = ctx.autotune(reps=3)
+    log(f"autotune: orbit sampler {tune['threads']} threads x {tune['coarse']} LDS samples "
+        f"({tune['ms_best']:.3f} ms vs default {tune['ms_default']:.3f} ms)")
 
     def timed(n_steps):
         ctx.timer_reset()
@@ -187,6 +190,7 @@ def main():
         "roofline": roofline,
         "phases": phases,
         "learned": learned,
+        "autotune": tune,
         "cpu_baseline": cpu,
         "last_step_stats": st,
     }

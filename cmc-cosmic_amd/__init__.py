@@ -106,6 +106,10 @@ _sigs = {
     "cmcg_get_scalars": [_ctx, _P(Scalars)],
     "cmcg_set_scalars": [_ctx, _P(Scalars)],
     "cmcg_sync": [_ctx],
+    "cmcg_checkpoint_size": [_ctx, _P(C.c_int64)],
+    "cmcg_checkpoint_save": [_ctx, C.c_void_p, C.c_int64],
+    "cmcg_checkpoint_info": [C.c_void_p, C.c_int64, _P(Params), _P(C.c_int64)],
+    "cmcg_checkpoint_load": [_ctx, C.c_void_p, C.c_int64],
     "cmcg_sort_by_r": [_ctx, _P(C.c_int64)],
     "cmcg_potential": [_ctx, _P(C.c_int64), _P(C.c_double), _P(C.c_double)],
     "cmcg_potential_at": [_ctx, C.c_int64, _P(C.c_double), _P(C.c_double)],
@@ -162,6 +166,14 @@ for _f in (hostlib.cmc_ic_plummer, hostlib.cmc_ic_king, hostlib.cmc_ic_kroupa):
 
 def _ptr(a, ct):
     return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+def checkpoint_info(buf):
+    """(Params, N) stored in a checkpoint buffer; raises CmcgError if malformed."""
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    p, n = Params(), C.c_int64()
+    _chk(lib.cmcg_checkpoint_info(buf.ctypes.data, buf.size, C.byref(p), C.byref(n)), "cmcg_checkpoint_info")
+    return p, n.value
 
 
 # ---------------------------------------------------------------- host cluster
@@ -293,6 +305,18 @@ class Context:
 
     def sync(self):
         return lib.cmcg_sync(self.h)
+
+    def checkpoint(self):
+        """The device state as a byte array (cmcg_checkpoint_save)."""
+        n = C.c_int64()
+        _chk(lib.cmcg_checkpoint_size(self.h, C.byref(n)), "cmcg_checkpoint_size")
+        buf = np.empty(n.value, dtype=np.uint8)
+        _chk(lib.cmcg_checkpoint_save(self.h, buf.ctypes.data, n.value), "cmcg_checkpoint_save")
+        return buf
+
+    def restore(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        _chk(lib.cmcg_checkpoint_load(self.h, buf.ctypes.data, buf.size), "cmcg_checkpoint_load")
 
     def sort_by_r(self):
         n = C.c_int64()

@@ -489,6 +489,125 @@ int cmcg_set_scalars(cmcg_ctx *c, const cmcg_scalars *in)
     return CMCG_OK;
 }
 
+// ------------------------------------------------------------ checkpoints
+// Layout of a checkpoint buffer (little-endian, this library's struct layouts):
+//   CkptHeader | DevScalars | per-star field 0 .. CKPT_NFIELDS-1, each (slots) entries
+// where slots = max(N_MAX, N_MAX_NEW) + 2 (sentinels included).  The step counter in
+// DevScalars selects the RNG substreams of the next step, so a context loaded from a
+// checkpoint continues exactly as the saved one would have.
+namespace {
+constexpr uint64_t CKPT_MAGIC = 0x31544b50474d43ull;    // "CMGPKT1"
+constexpr int CKPT_NFIELDS = 25;
+struct CkptHeader {
+    uint64_t magic;
+    uint32_t version, scalars_bytes;
+    int64_t slots;
+    int64_t bytes;
+    cmcg_params params;
+};
+
+struct CkptField { void *dev; size_t elem; };
+
+void ckpt_fields(const DevFields &f, CkptField *o)
+{
+    void *p[CKPT_NFIELDS] = {f.r, f.m, f.phi, f.vr, f.vt, f.E, f.J, f.id, f.rnew, f.vrnew, f.vtnew, f.X, f.Y,
+                             f.r_peri, f.r_apo, f.rOld, f.Uoldrold, f.Uoldrnew, f.vrold, f.vtold, f.EI, f.sigma,
+                             f.pot_rnew, f.ntry, f.flags};
+    for (int k = 0; k < CKPT_NFIELDS; k++) o[k] = {p[k], k >= CKPT_NFIELDS - 2 ? 4u : 8u};
+}
+
+int64_t ckpt_bytes(int64_t slots)
+{
+    return (int64_t)sizeof(CkptHeader) + (int64_t)sizeof(DevScalars) + slots * (23 * 8 + 2 * 4);
+}
+
+int64_t ckpt_slots(const DevScalars &h) { return (h.n_max > h.n_max_new ? h.n_max : h.n_max_new) + 2; }
+}  // namespace
+
+int cmcg_checkpoint_size(cmcg_ctx *c, int64_t *bytes)
+{
+    if (!c || !bytes) return CMCG_E_ARG;
+    DevScalars h;
+    if (hipMemcpyAsync(&h, c->s, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess || sync(c) != hipSuccess)
+        return CMCG_E_HIP;
+    *bytes = ckpt_bytes(ckpt_slots(h));
+    return CMCG_OK;
+}
+
+int cmcg_checkpoint_save(cmcg_ctx *c, void *buf, int64_t cap)
+{
+    if (!c || !buf) return CMCG_E_ARG;
+    DevScalars h;
+    if (hipMemcpyAsync(&h, c->s, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess || sync(c) != hipSuccess)
+        return CMCG_E_HIP;
+    const int64_t slots = ckpt_slots(h), bytes = ckpt_bytes(slots);
+    if (cap < bytes) return CMCG_E_CAPACITY;
+    CkptHeader hd{};
+    hd.magic = CKPT_MAGIC;
+    hd.version = 1;
+    hd.scalars_bytes = (uint32_t)sizeof(DevScalars);
+    hd.slots = slots;
+    hd.bytes = bytes;
+    hd.params = c->params;
+    char *o = (char *)buf;
+    memcpy(o, &hd, sizeof(hd));
+    memcpy(o + sizeof(hd), &h, sizeof(h));
+    o += sizeof(hd) + sizeof(h);
+    CkptField fl[CKPT_NFIELDS];
+    ckpt_fields(c->f, fl);
+    for (int k = 0; k < CKPT_NFIELDS; k++) {
+        if (hipMemcpyAsync(o, fl[k].dev, slots * fl[k].elem, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            return CMCG_E_HIP;
+        o += slots * fl[k].elem;
+    }
+    return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+}
+
+int cmcg_checkpoint_info(const void *buf, int64_t size, cmcg_params *params, int64_t *n_max)
+{
+    if (!buf || size < (int64_t)(sizeof(CkptHeader) + sizeof(DevScalars))) return CMCG_E_ARG;
+    CkptHeader hd;
+    DevScalars h;
+    memcpy(&hd, buf, sizeof(hd));
+    memcpy(&h, (const char *)buf + sizeof(hd), sizeof(h));
+    if (hd.magic != CKPT_MAGIC || hd.version != 1 || hd.scalars_bytes != sizeof(DevScalars) || hd.slots < 2 ||
+        hd.bytes != ckpt_bytes(hd.slots) || size < hd.bytes || hd.slots != ckpt_slots(h))
+        return CMCG_E_ARG;
+    if (params) *params = hd.params;
+    if (n_max) *n_max = hd.slots - 2;
+    return CMCG_OK;
+}
+
+int cmcg_checkpoint_load(cmcg_ctx *c, const void *buf, int64_t size)
+{
+    if (!c) return CMCG_E_ARG;
+    if (int e = cmcg_checkpoint_info(buf, size, nullptr, nullptr)) return e;
+    CkptHeader hd;
+    DevScalars h;
+    memcpy(&hd, buf, sizeof(hd));
+    memcpy(&h, (const char *)buf + sizeof(hd), sizeof(h));
+    const cmcg_params &a = hd.params, &b = c->params;    // flags do not change results
+    if (a.n_star != b.n_star || a.avekernel != b.avekernel || a.theta_se_max != b.theta_se_max ||
+        a.gamma != b.gamma || a.orbit_r != b.orbit_r || a.cenma_m != b.cenma_m || a.seed != b.seed ||
+        a.tidal_strip != b.tidal_strip)
+        return CMCG_E_ARG;
+    if (hd.slots - 2 > c->n_cap) return CMCG_E_CAPACITY;
+    h.error = 0;
+    h.error_index = 0;
+    const char *o = (const char *)buf + sizeof(hd) + sizeof(h);
+    CkptField fl[CKPT_NFIELDS];
+    ckpt_fields(c->f, fl);
+    for (int k = 0; k < CKPT_NFIELDS; k++) {
+        if (hipMemcpyAsync(fl[k].dev, o, hd.slots * fl[k].elem, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return CMCG_E_HIP;
+        o += hd.slots * fl[k].elem;
+    }
+    if (hipMemcpyAsync(c->s, &h, sizeof(h), hipMemcpyHostToDevice, c->stream) != hipSuccess || sync(c) != hipSuccess)
+        return CMCG_E_HIP;
+    c->n_ub = hd.slots - 2;
+    return CMCG_OK;
+}
+
 int cmcg_sort_by_r(cmcg_ctx *c, int64_t *n_max)
 {
     if (!c) return CMCG_E_ARG;

@@ -300,36 +300,34 @@ __global__ void orbits_begin_kernel(DevScalars *s)
     s->n_defer = 0;
 }
 
-// Sums the per-block partials in block order, then the deferred stars' in star
-// order, and finishes new_orbits_calculate.
-__global__ void orbits_end_kernel(DevScalars *s, const double *__restrict__ partials, int64_t nblocks,
-                                  const double *__restrict__ dpartials)
+// Sums the per-block partials, then the deferred stars' in star order, and finishes
+// new_orbits_calculate.  1024 threads: 16 consecutive threads read one partial record
+// (coalesced), 64 groups take blocks g, g + 64, ..., and the 64 group results are
+// combined in group order (a fixed order, so the sums are deterministic).
+__global__ __launch_bounds__(1024) void orbits_end_kernel(DevScalars *s, const double *__restrict__ partials,
+                                                          int64_t nblocks, const double *__restrict__ dpartials)
 {
-    __shared__ double lds[NRED][256];
-    const int t = threadIdx.x;
-    double acc[NRED];
-    for (int i = 0; i < NRED; i++) acc[i] = 0.0;
-    for (int64_t b = t; b < nblocks; b += 256) {
-        const double *p = partials + RED_STRIDE * b;
-        for (int i = 0; i < NRED; i++) acc[i] = red_is_max(i) ? (acc[i] >= p[i] ? acc[i] : p[i]) : acc[i] + p[i];
-    }
-    for (int i = 0; i < NRED; i++) lds[i][t] = acc[i];
+    __shared__ double gsum[64][RED_STRIDE];
+    const int t = threadIdx.x, i = t & 15, g = t >> 4;
+    double acc = 0.0;
+    if (i < NRED)
+        for (int64_t b = g; b < nblocks; b += 64) {
+            double p = partials[RED_STRIDE * b + i];
+            acc = red_is_max(i) ? (acc >= p ? acc : p) : acc + p;
+        }
+    gsum[g][i] = acc;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (t < off)
-            for (int i = 0; i < NRED; i++) {
-                double a = lds[i][t], b = lds[i][t + off];
-                lds[i][t] = red_is_max(i) ? (a >= b ? a : b) : a + b;
-            }
-        __syncthreads();
-    }
     if (t == 0) {
         double v[NRED];
-        for (int i = 0; i < NRED; i++) v[i] = lds[i][0];
+        for (int k = 0; k < NRED; k++) {
+            double a = gsum[0][k];
+            for (int g2 = 1; g2 < 64; g2++) a = red_is_max(k) ? (a >= gsum[g2][k] ? a : gsum[g2][k]) : a + gsum[g2][k];
+            v[k] = a;
+        }
         int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
         for (int64_t d = 0; d < nd; d++) {
             const double *p = dpartials + RED_STRIDE * d;
-            for (int i = 0; i < NRED; i++) v[i] = red_is_max(i) ? (v[i] >= p[i] ? v[i] : p[i]) : v[i] + p[i];
+            for (int k = 0; k < NRED; k++) v[k] = red_is_max(k) ? (v[k] >= p[k] ? v[k] : p[k]) : v[k] + p[k];
         }
         s->max_r = v[0];
         s->Eescaped += v[1];
@@ -417,7 +415,7 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
                        sc.defer_kfirst);
     hipLaunchKernelGGL(defer_tail_kernel, dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,
                        (const int64_t *)sc.defer_list, (const int64_t *)sc.defer_kfirst, sc.defer_partials);
-    hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(256), 0, st, s, (const double *)sc.partials, nblocks,
+    hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(1024), 0, st, s, (const double *)sc.partials, nblocks,
                        (const double *)sc.defer_partials);
     return hipGetLastError();
 }

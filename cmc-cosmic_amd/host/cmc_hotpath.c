@@ -78,6 +78,61 @@ cmcg_ctx *cmc_hotpath_ctx(void) { return g_ctx; }
 
 int cmc_hotpath_download(cmcg_host_soa *h) { return cmcg_download(g_ctx, h); }
 
+/* save_restart_file (src/cmc/cmc_io.c:3829-3884) for the hot path's state: one file
+ * holding the device checkpoint (fields, scalars, step counter, parameters). */
+int cmc_hotpath_save_restart(const char *path)
+{
+    int64_t bytes = 0;
+    int rc = cmcg_checkpoint_size(g_ctx, &bytes);
+    if (rc) return rc;
+    void *buf = malloc((size_t)bytes);
+    if (!buf) return CMCG_E_NOMEM;
+    rc = cmcg_checkpoint_save(g_ctx, buf, bytes);
+    if (!rc) {
+        FILE *f = fopen(path, "wb");
+        if (!f) rc = CMCG_E_ARG;
+        else {
+            if (fwrite(buf, 1, (size_t)bytes, f) != (size_t)bytes) rc = CMCG_E_ARG;
+            if (fclose(f) != 0) rc = CMCG_E_ARG;
+        }
+    }
+    free(buf);
+    return rc;
+}
+
+/* load_restart_file (src/cmc/cmc_io.c:3886-4031): creates the context from the
+ * parameters stored in the file (capacity max(n_cap, N)) and restores the state;
+ * the next step continues exactly where the saved run stopped. */
+int cmc_hotpath_load_restart(const char *path, int64_t n_cap, int device)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return CMCG_E_ARG;
+    int rc = CMCG_OK;
+    long size = -1;
+    if (fseek(f, 0, SEEK_END) == 0) size = ftell(f);
+    void *buf = size > 0 ? malloc((size_t)size) : NULL;
+    if (!buf || fseek(f, 0, SEEK_SET) != 0 || fread(buf, 1, (size_t)size, f) != (size_t)size) rc = buf ? CMCG_E_ARG : CMCG_E_NOMEM;
+    fclose(f);
+    cmcg_params p;
+    int64_t n = 0;
+    if (!rc) rc = cmcg_checkpoint_info(buf, size, &p, &n);
+    if (!rc) {
+        if (g_ctx) cmc_hotpath_finalize();
+        rc = cmcg_create(&g_ctx, device, n_cap > n ? n_cap : n, &p);
+    }
+    if (!rc) rc = cmcg_checkpoint_load(g_ctx, buf, size);
+    free(buf);
+    if (rc) return rc;
+    cmcg_scalars s;
+    cmcg_get_scalars(g_ctx, &s);
+    cmc_hp.N_STAR = (long)p.n_star;
+    cmc_hp.madhoc = 1.0 / ((double)p.n_star);
+    g_step = s.step;
+    cmc_hp.tcount = (long)s.step + 1;
+    refresh();
+    return CMCG_OK;
+}
+
 void qsorts_new(void)
 {
     int64_t n;

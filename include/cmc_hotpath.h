@@ -50,6 +50,13 @@ cmcg_ctx *cmc_hotpath_ctx(void);
 /* Copies the device state back into host arrays (for output / checkpoints). */
 int  cmc_hotpath_download(cmcg_host_soa *h);
 
+/* Restart files of the hot path's state (the counterpart of save_restart_file and
+ * load_restart_file, src/cmc/cmc_io.c:3829-3884, 3886-4031): per-star fields, device
+ * scalars, step counter and parameters.  _load creates the context from the file's
+ * parameters with capacity max(n_cap, N); the run then continues bit for bit. */
+int  cmc_hotpath_save_restart(const char *path);
+int  cmc_hotpath_load_restart(const char *path, int64_t n_cap, int device);
+
 /* --- the reference's names ------------------------------------------------ */
 void   qsorts_new(void);                                   /* src/cmc/cmc_utils.c:2232 */
 void   post_sort_comm(void);                               /* src/cmc/cmc_utils.c:2259 */

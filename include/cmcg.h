@@ -122,6 +122,20 @@ int cmcg_get_scalars(cmcg_ctx *ctx, cmcg_scalars *out);
 int cmcg_set_scalars(cmcg_ctx *ctx, const cmcg_scalars *in);
 int cmcg_sync(cmcg_ctx *ctx);
 
+/* Checkpoints of the device state; replace save_restart_file / load_restart_file
+ * (src/cmc/cmc_io.c:3829-3884, 3886-4031) for the hot path's state: every per-star
+ * field of cmcg_host_soa, the device scalars and the step counter (which selects the
+ * RNG substreams, the counterpart of saving curr_st), and the context parameters.
+ * A context created with the same cmcg_params and loaded from a checkpoint continues
+ * bit for bit as the saved one.  The buffer is host memory owned by the caller:
+ * cmcg_checkpoint_size gives the bytes _save writes (CMCG_E_CAPACITY if cap is
+ * smaller); _info reads the parameters and N from a buffer without a context; _load
+ * returns CMCG_E_ARG for a malformed buffer or different parameters (flags aside). */
+int cmcg_checkpoint_size(cmcg_ctx *ctx, int64_t *bytes);
+int cmcg_checkpoint_save(cmcg_ctx *ctx, void *buf, int64_t cap);
+int cmcg_checkpoint_info(const void *buf, int64_t size, cmcg_params *params, int64_t *n_max);
+int cmcg_checkpoint_load(cmcg_ctx *ctx, const void *buf, int64_t size);
+
 /* ------------------------------------------------------------------ hot-path phases */
 /* a1: qsorts_new + remove_stripped_stars + post_sort_comm
  *     (src/cmc/cmc_utils.c:2232-2288, src/cmc/cmc_sort.c:245-250,343-354,375-629).

@@ -133,6 +133,8 @@ _sigs = {
     "cmcg_step": [_ctx, _P(OrbitStats)],
     "cmcg_step_graph": [_ctx, _P(OrbitStats)],
     "cmcg_autotune": [_ctx, C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_double), _P(C.c_double)],
+    "cmcg_orbit_variants": [_ctx, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32),
+                            _P(C.c_double)],
     "cmcg_ensemble_step": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
     "cmcg_rng_draws": [_ctx, C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int, _P(C.c_double)],
     "cmcg_timer_enable": [_ctx, C.c_int],
@@ -412,8 +414,21 @@ class Context:
         best, default = C.c_double(), C.c_double()
         _chk(lib.cmcg_autotune(self.h, int(reps), C.byref(v), C.byref(th), C.byref(cs), C.byref(best),
                                C.byref(default)), "cmcg_autotune")
-        return {"variant": v.value, "threads": th.value, "coarse": cs.value, "ms_best": best.value,
-                "ms_default": default.value}
+        out = {"variant": v.value, "threads": th.value, "coarse": cs.value, "ms_best": best.value,
+               "ms_default": default.value}
+        out["variants"] = self.orbit_variants()
+        return out
+
+    def orbit_variants(self):
+        """Launch variants of the orbit sampler with their last autotune times."""
+        cnt = C.c_int32()
+        _chk(lib.cmcg_orbit_variants(self.h, 0, C.byref(cnt), None, None, None, None), "cmcg_orbit_variants")
+        n = cnt.value
+        th, cs, cp = (np.zeros(n, np.int32) for _ in range(3))
+        ms = np.zeros(n, np.float64)
+        _chk(lib.cmcg_orbit_variants(self.h, n, C.byref(cnt), _ptr(th, C.c_int32), _ptr(cs, C.c_int32),
+                                     _ptr(cp, C.c_int32), _ptr(ms, C.c_double)), "cmcg_orbit_variants")
+        return [{"threads": int(th[i]), "coarse": int(cs[i]), "cap": int(cp[i]), "ms": float(ms[i])} for i in range(n)]
 
     def step_graph(self, check=True):
         st = OrbitStats()

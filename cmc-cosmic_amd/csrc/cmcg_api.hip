@@ -45,6 +45,7 @@ struct cmcg_ctx {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int parity = 0;
     int orb_variant = 0;         // launch shape of the orbit sampler (autotuned)
+    std::vector<double> tune_ms; // per-variant time of the last cmcg_autotune
 };
 
 static void invalidate_graphs(cmcg_ctx *c)
@@ -1220,6 +1221,7 @@ int cmcg_autotune(cmcg_ctx *c, int reps, int32_t *variant, int32_t *threads, int
     int best = 0;
     double best_ms = 1e300, def_ms = 0.0;
     int rc = CMCG_OK;
+    c->tune_ms.assign(orbit_variant_count(), 0.0);
     for (int v = 0; v < orbit_variant_count() && rc == CMCG_OK; v++) {
         double total = 0.0;
         for (int it = 0; it <= reps; it++) {      // the first run warms up and is not timed
@@ -1237,6 +1239,7 @@ int cmcg_autotune(cmcg_ctx *c, int reps, int32_t *variant, int32_t *threads, int
             if (hipMemcpy(c->s, &saved, sizeof(saved), hipMemcpyHostToDevice) != hipSuccess) { rc = CMCG_E_HIP; break; }
         }
         double avg = total / reps;
+        c->tune_ms[v] = avg;
         if (v == 0) def_ms = avg;
         if (avg < best_ms) { best_ms = avg; best = v; }
     }
@@ -1252,6 +1255,23 @@ int cmcg_autotune(cmcg_ctx *c, int reps, int32_t *variant, int32_t *threads, int
     if (coarse) *coarse = cs;
     if (ms_best) *ms_best = best_ms;
     if (ms_default) *ms_default = def_ms;
+    return CMCG_OK;
+}
+
+int cmcg_orbit_variants(cmcg_ctx *c, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse, int32_t *cap,
+                        double *ms_last)
+{
+    if (!c || !count || max < 0) return CMCG_E_ARG;
+    const int nv = orbit_variant_count();
+    *count = nv;
+    for (int v = 0; v < nv && v < max; v++) {
+        int th, cs, cp;
+        orbit_variant_shape(v, &th, &cs, &cp);
+        if (threads) threads[v] = th;
+        if (coarse) coarse[v] = cs;
+        if (cap) cap[v] = cp;
+        if (ms_last) ms_last[v] = v < (int)c->tune_ms.size() ? c->tune_ms[v] : 0.0;
+    }
     return CMCG_OK;
 }
 

@@ -450,6 +450,53 @@ __device__ __forceinline__ int64_t coarse_index_estimate(const Coarse &c, double
     return ka + (int64_t)(fr * (float)(kb - ka));
 }
 
+// Largest t in [t0, t1] with eval(mid[t]) true, or t0 - 1 if there is none, for a
+// predicate that is true then false over [t0, t1].  eval also returns a key, negative
+// where the predicate holds, that varies smoothly with t; k_lo and k_hi are the keys
+// just outside the range (at the bracket ends, k_lo < 0 <= k_hi).  Up to two probes
+// read a whole 128-byte line of the middle table (8 entries) at the index where the
+// keys interpolate to zero; the rest is bisection.  The answer is the same for any
+// guesses, since the predicate alone decides it.
+template <class Eval>
+__device__ __forceinline__ int64_t mid_search(const double2 *__restrict__ mid, int64_t t0, int64_t t1, double k_lo,
+                                              double k_hi, Eval eval)
+{
+    int64_t lo = t0 - 1, hi = t1 + 1;         // eval true at lo, false at hi (virtually)
+    for (int probe = 0; probe < 2 && hi - lo > 1; probe++) {
+        double f = k_lo / (k_lo - k_hi);
+        if (!(f > 0.0 && f < 1.0)) f = 0.5;   // also NaN
+        int64_t g = lo + 1 + (int64_t)(f * (double)(hi - lo - 1));
+        if (g >= hi) g = hi - 1;
+        const int64_t ls = g & ~(int64_t)7;
+        int64_t last_true = lo, first_false = hi;
+        double kt = k_lo, kf = k_hi;
+        double2 e[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int64_t t = ls + u;
+            if (t > lo && t < hi) e[u] = mid[t];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int64_t t = ls + u;
+            if (t > lo && t < hi && first_false == hi) {
+                double k;
+                if (eval(e[u], k)) { last_true = t; kt = k; }
+                else { first_false = t; kf = k; }
+            }
+        }
+        lo = last_true; k_lo = kt;
+        hi = first_false; k_hi = kf;
+    }
+    while (hi - lo > 1) {
+        const int64_t m = lo + ((hi - lo) >> 1);
+        double k;
+        if (eval(mid[m], k)) lo = m;
+        else hi = m;
+    }
+    return lo;
+}
+
 // potential(x) with the index search started from the LDS samples: the result is the
 // unique largest k with r[k] < x, so any bracket [lo, hi] with r[lo] < x <= r[hi]
 // gives the same index as the full-range search.
@@ -464,25 +511,28 @@ __device__ __forceinline__ double potential_at_coarse_n(const N &nd, int64_t n_m
         if (c.cr[m] < x) a = m;
         else b = m - 1;
     }
+    const float lx = __logf((float)x);
+    double k_lo = -1.0, k_hi = 1.0;
     if (a < c.cs) {
         int64_t ka = coarse_k(c, a), kb = coarse_k(c, a + 1);
         if (ka > lo) lo = ka;
         if (kb < hi) hi = kb;
+        k_lo = (double)(__logf((float)c.cr[a]) - lx);
+        k_hi = (double)(__logf((float)c.cr[a + 1]) - lx);
     }
     if (c.mid) {
         // largest mid node strictly inside (lo, hi) with r < x
-        int64_t t0 = (lo + MID_STEP) / MID_STEP, t1 = (hi - 1) / MID_STEP;
+        const int64_t t0 = (lo + MID_STEP) / MID_STEP, t1 = (hi - 1) / MID_STEP;
         if (t0 <= t1) {
-            if (!(c.mid[t0].x < x)) {
+            const int64_t t = mid_search(c.mid, t0, t1, k_lo, k_hi, [&](double2 e, double &k) {
+                k = (double)(__logf((float)e.x) - lx);
+                return e.x < x;
+            });
+            if (t < t0) {
                 hi = MID_STEP * t0;
             } else {
-                while (t0 < t1) {
-                    int64_t m = (t0 + t1 + 1) >> 1;
-                    if (c.mid[m].x < x) t0 = m;
-                    else t1 = m - 1;
-                }
-                lo = MID_STEP * t0;
-                if (MID_STEP * (t0 + 1) < hi) hi = MID_STEP * (t0 + 1);
+                lo = MID_STEP * t;
+                if (MID_STEP * (t + 1) < hi) hi = MID_STEP * (t + 1);
             }
         }
     }
@@ -497,26 +547,25 @@ __device__ __forceinline__ double potential_at_coarse(const double *__restrict__
 
 // Narrow a find_zero_Q bracket [lo, hi] with pred(Q(lo)) true and pred(Q(hi)) false
 // (pred monotone over the bracket) to the mid-node interval holding the sign change.
+// q_lo, q_hi: Q at the bracket ends when known (they guide the first probes only).
 template <bool POSITIVE>
 __device__ __forceinline__ void mid_narrow_q(const Coarse &c, double rj, double mj, double inv, double E, double J,
-                                             int64_t &lo, int64_t &hi)
+                                             int64_t &lo, int64_t &hi, double q_lo = 0.0, double q_hi = 0.0)
 {
     if (!c.mid) return;
-    int64_t t0 = (lo + MID_STEP) / MID_STEP, t1 = (hi - 1) / MID_STEP;
+    const int64_t t0 = (lo + MID_STEP) / MID_STEP, t1 = (hi - 1) / MID_STEP;
     if (t0 > t1) return;
-    auto pred = [&](int64_t t) {
-        double2 nd = c.mid[t];
-        double q = function_q(rj, mj, inv, nd.x, nd.y, E, J);
+    // key negative where the predicate holds: Q for Q < 0, -Q for Q > 0
+    double k_lo = POSITIVE ? -q_lo : q_lo, k_hi = POSITIVE ? -q_hi : q_hi;
+    if (!(k_lo < 0.0 && k_hi >= 0.0)) { k_lo = -1.0; k_hi = 1.0; }
+    const int64_t t = mid_search(c.mid, t0, t1, k_lo, k_hi, [&](double2 e, double &k) {
+        const double q = function_q(rj, mj, inv, e.x, e.y, E, J);
+        k = POSITIVE ? -q : q;
         return POSITIVE ? q > 0.0 : q < 0.0;
-    };
-    if (!pred(t0)) { hi = MID_STEP * t0; return; }
-    while (t0 < t1) {
-        int64_t m = (t0 + t1 + 1) >> 1;
-        if (pred(m)) t0 = m;
-        else t1 = m - 1;
-    }
-    lo = MID_STEP * t0;
-    if (MID_STEP * (t0 + 1) < hi) hi = MID_STEP * (t0 + 1);
+    });
+    if (t < t0) { hi = MID_STEP * t0; return; }
+    lo = MID_STEP * t;
+    if (MID_STEP * (t + 1) < hi) hi = MID_STEP * (t + 1);
 }
 
 struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; int brent; int linscan; int err;
@@ -543,6 +592,7 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
     int64_t kp_lo = 0, kp_hi = ktemp, ka_lo = ktemp, ka_hi = n_max + 1;
     bool got_p = false, got_a = false;
     bool vp = false, va = false;       // bracket ends checked: Q(lo), Q(hi) have the bisection's signs
+    double qp_lo = 0.0, qp_hi = 0.0, qa_lo = 0.0, qa_hi = 0.0;   // Q at sample ends (search guidance only)
     if (C && C->hint) {
         // learned proposals: the index range of the predicted apsis +- margin (in
         // log r), padded by 2, accepted only when Q has the bisection's signs at its ends
@@ -582,6 +632,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
             kp_lo = coarse_k(*C, a);
             kp_hi = a < top ? coarse_k(*C, a + 1) : ktemp;
             vp = true;
+            qp_lo = function_q(rj, mj, inv, C->cr[a], C->cphi[a], E, J);
+            if (a < top) qp_hi = function_q(rj, mj, inv, C->cr[a + 1], C->cphi[a + 1], E, J);
         }
         const int sb = (int)(ktemp / C->stride) + 1;
         if (!got_a && sb <= C->cs && !(function_q(rj, mj, inv, C->cr[C->cs], C->cphi[C->cs], E, J) > 0.0)) {
@@ -597,11 +649,13 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
                 }
                 ka_lo = coarse_k(*C, a);
                 ka_hi = coarse_k(*C, a + 1);
+                qa_lo = function_q(rj, mj, inv, C->cr[a], C->cphi[a], E, J);
+                qa_hi = function_q(rj, mj, inv, C->cr[a + 1], C->cphi[a + 1], E, J);
             }
         }
     }
-    if (vp && C) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi);
-    if (va && C) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi);
+    if (vp && C) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi, qp_lo, qp_hi);
+    if (va && C) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi, qa_lo, qa_hi);
     int64_t kmin = find_zero_Q_n(nd, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
     int64_t kmax = find_zero_Q_n(nd, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
     if (kmin < 0 || kmax < 0) {

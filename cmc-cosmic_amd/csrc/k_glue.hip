@@ -191,43 +191,76 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
 // set_velocities3 per-star part.  Stars with vnew2 <= 0 ("events") carry excess
 // energy that the following stars absorb (cmc_utils.c:268-310); they are counted
 // per block for an ordered compaction.
-// potential(r) of the new potential at every old node, by merging the old sorted
-// radii (the array the sort read, left intact in the alternate buffer) with the new
-// ones: each thread takes MERGE_CH consecutive old nodes, finds the first one's
-// interval by bisection and walks forward for the rest.  The interval is the largest
-// k with r_k < x, as in potential_at, and the interpolation is the same expression,
-// so the values equal potential_at(x) bit for bit.
-constexpr int MERGE_CH = 16;
+// potential(r) of the new potential at every old node.  The old radii (the array the
+// sort read, left intact in the alternate buffer) are sorted, so a block's MERGE_BLK
+// consecutive old radii [x_first, x_last] fall into one contiguous run of new nodes:
+// two lanes find its ends by bisection, the block stages the run in LDS (coalesced)
+// and every radius is placed by bisection in LDS.  A run longer than MERGE_LDS nodes
+// is searched in global memory within the same ends.  The interval is the largest
+// k with r_k < x, as in potential_at (the search predicate is monotone, so any bracket
+// holding that k gives the same k), and the interpolation is the same expression, so
+// the values equal potential_at(x) bit for bit.
+constexpr int MERGE_T = 256, MERGE_PER = 4, MERGE_BLK = MERGE_T * MERGE_PER, MERGE_LDS = 2048;
 
-__global__ __launch_bounds__(GT) void old_node_potential_kernel(const double *__restrict__ r_old,
-                                                                const double2 *__restrict__ nodes, DevScalars *s,
-                                                                double *__restrict__ u_old)
+__global__ __launch_bounds__(MERGE_T) void old_node_potential_kernel(const double *__restrict__ r_old,
+                                                                     const double2 *__restrict__ nodes,
+                                                                     DevScalars *s, double *__restrict__ u_old)
 {
+    __shared__ double2 L[MERGE_LDS];
+    __shared__ int64_t ends[2];
+    __shared__ double xends[2];
     const int64_t n_old = s->sort_in, n = s->n_max;
-    const int64_t s0 = 1 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * MERGE_CH;
-    if (s0 > n_old) return;
+    const int64_t first = 1 + (int64_t)blockIdx.x * MERGE_BLK;
+    if (first > n_old) return;
+    const int64_t last = first + MERGE_BLK - 1 <= n_old ? first + MERGE_BLK - 1 : n_old;
     const PackedNodes nd{nodes};
     const double r1 = nd.R(1);
-    int64_t i = -1;
-    for (int64_t sl = s0; sl < s0 + MERGE_CH && sl <= n_old; sl++) {
+    const int t = threadIdx.x;
+    if (t < 2) {
+        const double x = r_old[t == 0 ? first : last];
+        // largest k in [1, n+1] with r_k < x (1 if none); monotone in x
+        ends[t] = (x == 0.0 || x < r1) ? 1 : find_zero_r_n(nd, 1, n + 1, x);
+        xends[t] = x;
+    }
+    __syncthreads();
+    const int64_t lo = ends[0], hi = ends[1];
+    const double xf = xends[0], xl = xends[1];
+    const int64_t span = hi + 2 - lo;             // nodes lo .. hi+1
+    const bool staged = span <= MERGE_LDS && hi + 1 <= n + 1;
+    if (staged)
+        for (int64_t k = t; k < span; k += MERGE_T) L[k] = nodes[lo + k];
+    __syncthreads();
+    for (int u = 0; u < MERGE_PER; u++) {
+        const int64_t sl = first + t + (int64_t)u * MERGE_T;
+        if (sl > last) break;
         const double x = r_old[sl];
-        double u;
-        if (x == 0.0 || x < r1) {
+        double v;
+        if (x == 0.0 || x < r1 || !(x >= xf && x <= xl)) {    // the latter only if unsorted
             bool bad = false;
-            u = potential_at_n(nd, n, x, 1, n + 1, &bad);
+            v = potential_at_n(nd, n, x, 1, n + 1, &bad);
+            if (bad) dev_set_error(s, CMCG_E_BRACKET, sl);
         } else {
-            if (i < 0) i = find_zero_r_n(nd, 1, n + 1, x);
-            else
-                while (i + 1 <= n + 1 && nd.R(i + 1) < x) i++;
-            if (i > n) { dev_set_error(s, CMCG_E_BRACKET, sl); u = 0.0; }
-            else {
-                double ri, pi, ri1, pi1;
+            int64_t i;
+            double ri, pi, ri1, pi1;
+            if (staged) {
+                int a = 0, b = (int)(hi - lo);
+                while (a < b) {
+                    const int m = (a + b + 1) >> 1;
+                    if (L[m].x < x) a = m;
+                    else b = m - 1;
+                }
+                i = lo + a;
+                ri = L[a].x; pi = L[a].y; ri1 = L[a + 1].x; pi1 = L[a + 1].y;
+            } else {
+                i = lo == hi ? lo : find_zero_r_n(nd, lo, hi, x);
                 nd.RP(i, ri, pi);
-                nd.RP(i + 1, ri1, pi1);
-                u = i == 0 ? pi1 : pi + (pi1 - pi) * (1.0 / ri - 1.0 / x) / (1.0 / ri - 1.0 / ri1);
+                if (i + 1 <= n + 1) nd.RP(i + 1, ri1, pi1);
+                else { ri1 = ri; pi1 = pi; }
             }
+            if (i > n) { dev_set_error(s, CMCG_E_BRACKET, sl); v = 0.0; }
+            else v = pi + (pi1 - pi) * (1.0 / ri - 1.0 / x) / (1.0 / ri - 1.0 / ri1);
         }
-        u_old[sl] = u;
+        u_old[sl] = v;
     }
 }
 
@@ -247,7 +280,8 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
     C.cr = cr; C.cphi = nullptr; C.cs = SV3_COARSE; C.n1 = n + 1; C.hint = false;
     C.mid = mid;
     C.stride = coarse_stride(n, SV3_COARSE);
-    for (int sI = threadIdx.x; sI <= SV3_COARSE; sI += blockDim.x) cr[sI] = f.r[coarse_k(C, sI)];
+    if (!u_old)      // the samples serve only the search for potential(rOld)
+        for (int sI = threadIdx.x; sI <= SV3_COARSE; sI += blockDim.x) cr[sI] = f.r[coarse_k(C, sI)];
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     if (i <= n) {
@@ -522,9 +556,8 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     if (r_old_sorted) {
-        int64_t nt = (n_ub + MERGE_CH - 1) / MERGE_CH;
-        hipLaunchKernelGGL(old_node_potential_kernel, dim3((unsigned)((nt + GT - 1) / GT)), dim3(GT), 0, st,
-                           r_old_sorted, (const double2 *)sc.nodes, s, sc.u_old);
+        hipLaunchKernelGGL(old_node_potential_kernel, dim3((unsigned)((n_ub + MERGE_BLK - 1) / MERGE_BLK)),
+                           dim3(MERGE_T), 0, st, r_old_sorted, (const double2 *)sc.nodes, s, sc.u_old);
     }
     hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
                        (const double2 *)sc.mid, (const double2 *)sc.nodes,

@@ -151,10 +151,192 @@ __device__ void sample_star(const N &nd, const DevFields &f, DevScalars *s, cons
     red[0] = rr;
 }
 
+// ------------------------------------------------------------------ wave-parallel tries
+// The rejection loop of get_positions_loop (cmc_evolution_thr.c:858-906) takes a
+// geometric number of tries (mean ~2), so a wave that ran one star per lane would
+// idle most lanes while its slowest star (~8 tries) finishes.  Instead the wave
+// evaluates tries in rounds: the stars still trying share the 64 lanes, each lane
+// taking the next untried try number of one star, stepped to that try's draws in the
+// star's own substream (try t uses draws 2t-2 and 2t-1, the sign draw follows the
+// accepted try).  A star's result is its lowest-numbered terminating try, exactly as
+// the sequential loop, so outputs do not depend on the grouping.
+
+// what a star's tries need (set by prepare_star)
+struct TryCtx {
+    double rmin, rmax, F, E, J, rj, mj, r_lo0, r_hi0;
+    int64_t lo0, hi0;
+};
+
+// result of a star's tries
+struct TryOut {
+    double X, rr, vr, potr;
+    int k;             // tries taken; CMCG_N_TRY + 1 when none was accepted
+    bool bad, nan;
+};
+
+// q-th (from 0) set bit of m
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int q)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if (q >= c) { q -= c; m >>= w; pos += w; }
+        else m = low;
+    }
+    return pos;
+}
+
+// The part of get_positions_loop between the positive-Q index and the rejection loop
+// (cmc_evolution_thr.c:832-857): apsides, removal at r_apo >= Rtidal, circular
+// orbits, and the loop's constants.  Returns true if the star samples a new radius.
+template <class N>
+__device__ __forceinline__ bool prepare_star(const N &nd, const DevFields &f, DevScalars *s, const KParams &P,
+                                             const Coarse *C, int64_t n, int64_t j, double rj, double mj,
+                                             double E, double J, int64_t ktemp, int &flags, int &hits, int &misses,
+                                             TryCtx &tc)
+{
+    OrbitRS o = calc_orbit_from_n(nd, f.r, f.phi, n, j, rj, mj, P, E, J, ktemp, C);
+    if (o.err) dev_set_error(s, o.err, j);
+    hits = o.hint_hits;
+    misses = o.hint_misses;
+    if (o.brent) flags |= CMCG_SF_BRENT;
+    if (o.circular) {
+        f.rnew[j] = rj; f.vrnew[j] = f.vr[j]; f.vtnew[j] = f.vt[j];
+        flags |= CMCG_SF_CIRCULAR;
+        return false;
+    }
+    if (o.ra >= s->Rtidal) {
+        f.r_apo[j] = o.ra; f.r_peri[j] = o.rp;
+        f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
+        flags |= CMCG_SF_REMOVED_RT;      // its removal sums are added by the caller
+        return false;
+    }
+    tc.rmin = o.rp;
+    tc.rmax = o.ra;
+    const double g1 = sqrt(3.0 * (tc.rmax - tc.rmin) / o.dQdrp);
+    const double g2 = sqrt(-3.0 * (tc.rmax - tc.rmin) / o.dQdra);
+    tc.F = 1.2 * (g1 >= g2 ? g1 : g2);
+    tc.E = E; tc.J = J; tc.rj = rj; tc.mj = mj;
+    tc.lo0 = o.kmin < 1 ? 1 : o.kmin;
+    tc.hi0 = o.kmax + 1 > n + 1 ? n + 1 : o.kmax + 1;
+    tc.r_lo0 = nd.R(tc.lo0);
+    tc.r_hi0 = nd.R(tc.hi0);
+    return true;
+}
+
+// One try (cmc_evolution_thr.c:862-886) from the state at its first draw; on return
+// st is the state after the try's two draws.
+template <class N>
+__device__ __forceinline__ bool one_try(const N &nd, const Coarse *C, const KParams &P, int64_t n, const TryCtx &tc,
+                                        Rng &st, double &X, double &rr, double &vr, double &potr, bool &bad,
+                                        bool &nan)
+{
+    X = rng_dbl(st);
+    const double s0 = 2.0 * X - 1.0;
+    const double g0 = tc.F * rng_dbl(st);
+    rr = 0.5 * (tc.rmin + tc.rmax) + 0.25 * (tc.rmax - tc.rmin) * (3.0 * s0 - s0 * s0 * s0);
+    int64_t lo = tc.lo0, hi = tc.hi0;
+    if (!(tc.r_lo0 < rr)) lo = 1;
+    if (!(rr <= tc.r_hi0)) hi = n + 1;
+    potr = C ? potential_at_coarse_n(nd, n, rr, lo, hi, *C, &bad) : potential_at_n(nd, n, rr, lo, hi, &bad);
+    const double pot = potr + phi_s_own(tc.rj, tc.mj, P.inv_nstar, rr);
+    const double drds = 0.25 * (tc.rmax - tc.rmin) * (3.0 - 3.0 * s0 * s0);
+    const double Q = 2.0 * tc.E - 2.0 * pot - tc.J * tc.J / rr / rr;
+    if (Q >= 0.0) {
+        vr = sqrt(Q);
+    } else {
+        if (isnan(Q)) { nan = true; vr = 0.0; return true; }
+        vr = 0;
+    }
+    return g0 < 1.0 / vr * drds;
+}
+
+// All lanes of the wave call this together (convergent).  need: this lane's star
+// samples; wctx: the wave's 64 try constants in LDS (slot = lane, written before a
+// block barrier); rng: this lane's substream state before its star's first try.
+template <int CAP, class N>
+__device__ void wave_tries(const N &nd, const Coarse *C, const KParams &P, int64_t n, bool need,
+                           const TryCtx *__restrict__ wctx, Rng rng, TryOut &out)
+{
+    const int lane = threadIdx.x & 63;
+    int next = 1;                         // owner: first try not yet evaluated
+    bool bad_acc = false;
+    out.k = 0; out.bad = false; out.nan = false;
+    out.X = out.rr = out.vr = out.potr = 0.0;
+    uint64_t pending = __ballot(need);
+    while (pending) {
+        const int R = __popcll(pending);
+        int c = 64 / R, rem = 64 - c * R;
+        if (c >= CAP) { c = CAP; rem = 0; }              // at most CAP lanes per star: fewer wasted tries
+        const bool idle = lane >= c * R + rem;
+        int q, o, cs;
+        if (lane < (c + 1) * rem) { q = lane / (c + 1); o = lane % (c + 1); cs = c + 1; }
+        else { const int l2 = lane - (c + 1) * rem; q = rem + l2 / c; o = l2 % c; cs = c; }
+        const int sl = idle ? nth_set_bit(pending, 0) : nth_set_bit(pending, q);   // owner lane of the star helped
+        const TryCtx t = wctx[sl];
+        Rng st;
+        st.z0 = __shfl(rng.z0, sl); st.z1 = __shfl(rng.z1, sl); st.z2 = __shfl(rng.z2, sl); st.z3 = __shfl(rng.z3, sl);
+        const int k = __shfl(next, sl) + o;
+        const bool eval = !idle && k <= CMCG_N_TRY;
+        if (!idle)
+            for (int d = 0; d < 2 * o; d++) (void)rng_int(st);
+        double X = 0.0, rr = 0.0, vr = 0.0, potr = 0.0;
+        bool bad = false, nan = false, term = false;
+        if (eval) {
+            term = one_try(nd, C, P, n, t, st, X, rr, vr, potr, bad, nan);
+            Rng sg = st;
+            if (term && !nan && rng_dbl(sg) < 0.5) vr = -vr;    // sign draw after the accepted try
+        }
+        // the last try of a round that ends without acceptance also signs its v_r:
+        // it is the final try when the star runs out of tries
+        if (eval && !term && k == CMCG_N_TRY) {
+            Rng sg = st;
+            if (rng_dbl(sg) < 0.5) vr = -vr;
+        }
+        const uint64_t tmask = __ballot(eval && term), bmask = __ballot(eval && bad);
+        // owners collect their star's lanes [base, base + cs)
+        const int my_q = __popcll(pending & ((1ull << lane) - 1ull));
+        const bool owner = (pending >> lane) & 1ull;
+        const int my_cs = my_q < rem ? c + 1 : c;
+        const int base = my_q < rem ? my_q * (c + 1) : rem * (c + 1) + (my_q - rem) * c;
+        const uint64_t span = (my_cs >= 64 ? ~0ull : ((1ull << my_cs) - 1ull)) << base;
+        const uint64_t tw = tmask & span;
+        const int last_k = next + my_cs - 1;               // owner: its last try this round
+        int src = -1;
+        bool done = false;
+        if (owner) {
+            if (tw) { src = __ffsll((unsigned long long)tw) - 1; done = true; }
+            else if (last_k >= CMCG_N_TRY) { src = base + (CMCG_N_TRY - next); done = true; }
+            else src = base + my_cs - 1;        // continue from the round's last try
+        }
+        const int srcl = src < 0 ? lane : src;
+        const double sX = __shfl(X, srcl), srr = __shfl(rr, srcl), svr = __shfl(vr, srcl), spot = __shfl(potr, srcl);
+        const int snan = __shfl((int)nan, srcl);
+        Rng ns;
+        ns.z0 = __shfl(st.z0, srcl); ns.z1 = __shfl(st.z1, srcl); ns.z2 = __shfl(st.z2, srcl); ns.z3 = __shfl(st.z3, srcl);
+        if (owner) {
+            // bad of the tries up to the final one (or all of this round's if none ended)
+            const uint64_t upto = (src >= 63 ? ~0ull : ((1ull << (src + 1)) - 1ull)) & span;
+            bad_acc |= (bmask & upto) != 0ull;
+            if (done) {
+                out.X = sX; out.rr = srr; out.vr = svr; out.potr = spot; out.nan = snan != 0;
+                out.k = tw ? next + (src - base) : CMCG_N_TRY + 1;
+                out.bad = bad_acc;
+            } else {
+                next += my_cs;
+                rng = ns;
+            }
+        }
+        pending &= ~__ballot(owner && done);
+    }
+}
+
 // Slot j draws from offset j 2^20; the wave's first slot starts at the per-wave state
 // and lane l jumps l 2^20 (l < 2^6) through the LDS tables.  With learned brackets
 // the network's prediction proposes both find_zero_Q brackets.
-template <int THREADS, int COARSE>
+template <int THREADS, int COARSE, int CAP>
 __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
                                                              const RngTable *__restrict__ T,
                                                              const Rng *__restrict__ wave_base,
@@ -165,7 +347,12 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
                                                              const double2 *__restrict__ mid,
                                                              const double2 *__restrict__ nodes)
 {
-    __shared__ NibbleTables<6> lt;
+    // the jump tables are needed only for the lanes' first states, the try constants
+    // only after the apsides: one LDS area serves both
+    __shared__ union Share {
+        NibbleTables<6> lt;
+        TryCtx ctx[THREADS];
+    } sh;
     __shared__ double cr[COARSE + 1], cphi[COARSE + 1];
     const int64_t n = s->n_max;
     Coarse C;
@@ -178,15 +365,17 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
         cr[sI] = f.r[k];
         cphi[sI] = f.phi[k];
     }
-    build_nibble_tables<6>(T, 20, lt);      // ends with a block barrier
+    build_nibble_tables<6>(T, 20, sh.lt);      // ends with a block barrier
     const int64_t j = 1 + xcd_block() * blockDim.x + threadIdx.x;
-    double red[NRED];
-    for (int i = 0; i < NRED; i++) red[i] = 0.0;
+    Rng rng{};
+    if (j <= n) rng = rng_jump_lds<6>(sh.lt, wave_base[(j - 1) >> 6], (uint32_t)((j - 1) & 63));
+    __syncthreads();                           // the tables are dead from here
+    bool need = false;
+    int flags = 0, ntry = 0, hits = 0, misses = 0;
     if (j <= n) {
         const double rj = f.r[j], mj = f.m[j];
         const double Ej = f.E[j], J = f.J[j];
         const double E = Ej + phi_s_own(rj, mj, P.inv_nstar, rj);    // cmc_evolution_thr.c:806
-        int flags = 0, ntry = 0, hits = 0, misses = 0;
         if (mj < CMCG_ZERO) {
             // destroy_obj (cmc_dynamics_helper.c:131-153) + remove_star_center
             f.vr[j] = 0.0; f.vt[j] = 0.0; f.E[j] = 0.0; f.J[j] = 0.0; f.id[j] = 0;
@@ -196,7 +385,6 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
             flags = CMCG_SF_DESTROYED;
         } else if (E >= 0.0) {
             f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
-            removed(red, Ej, J, mj, P);
             flags = CMCG_SF_REMOVED_E;
         } else {
             double Qown = function_q(rj, mj, P.inv_nstar, rj, f.phi[j], E, J);
@@ -215,11 +403,36 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
                     C.lrp = y[0];
                     C.lra = y[1];
                 }
-                Rng rng = rng_jump_lds<6>(lt, wave_base[(j - 1) >> 6], (uint32_t)((j - 1) & 63));
-                sample_star(PackedNodes{nodes}, f, s, P, rng, &C, n, j, rj, mj, E, Ej, J, j, flags, ntry, hits,
-                            misses, red);
+                need = prepare_star(PackedNodes{nodes}, f, s, P, &C, n, j, rj, mj, E, J, j, flags, hits, misses,
+                                    sh.ctx[threadIdx.x]);
             }
         }
+    }
+    __syncthreads();                           // try constants visible to the wave
+    // every lane of the wave takes part, stars or not
+    TryOut to;
+    wave_tries<CAP>(PackedNodes{nodes}, &C, P, n, need, sh.ctx + (threadIdx.x & ~63), rng, to);
+    double red[NRED];
+    for (int i = 0; i < NRED; i++) red[i] = 0.0;
+    if (j <= n) {
+        if (need) {
+            const TryCtx &tc = sh.ctx[threadIdx.x];
+            if (to.bad) dev_set_error(s, CMCG_E_BRACKET, j);
+            if (to.nan) dev_set_error(s, CMCG_E_NAN, j);
+            else if (to.k == CMCG_N_TRY + 1) dev_set_error(s, CMCG_E_NTRY, j);
+            ntry = to.k;
+            f.X[j] = to.X;
+            f.r_peri[j] = tc.rmin;
+            f.r_apo[j] = tc.rmax;
+            f.rnew[j] = to.rr;
+            f.vrnew[j] = to.vr;
+            f.vtnew[j] = tc.J / to.rr;
+            f.pot_rnew[j] = to.potr;
+            flags |= 256;      // sampled: pot_rnew holds potential(rnew)
+            red[0] = to.rr;
+        }
+        if (flags & (CMCG_SF_REMOVED_E | CMCG_SF_REMOVED_RT))
+            removed(red, f.E[j], f.J[j], f.m[j], P);     // fields the removal left unchanged
         f.flags[j] = flags;
         f.ntry[j] = ntry;
         count_star(red, flags, ntry, hits, misses, true);
@@ -375,14 +588,18 @@ __global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count,
 }
 
 // Launch shapes of the orbit sampler that the autotuner chooses from: threads per
-// block and LDS samples of the (r, phi) table.  Every variant gives the same stars;
-// only the block order of the escape sums differs.
+// block, LDS samples of the (r, phi) table, and the most lanes one star's tries get
+// in a round (64: all idle lanes; fewer: less speculative work).  Every variant gives
+// the same stars; only the block order of the escape sums differs.
 using OrbFn = void (*)(DevFields, DevScalars *, KParams, const RngTable *, const Rng *, double *, int64_t *,
                        const float *, const MlpNorm *, float, const double2 *, const double2 *);
-struct OrbVariant { OrbFn fn; int threads, coarse; };
+struct OrbVariant { OrbFn fn; int threads, coarse, cap; };
 const OrbVariant kOrbVariants[] = {
-    {orbits_kernel<256, 1024>, 256, 1024}, {orbits_kernel<256, 512>, 256, 512}, {orbits_kernel<256, 2048>, 256, 2048},
-    {orbits_kernel<128, 1024>, 128, 1024}, {orbits_kernel<512, 1024>, 512, 1024}, {orbits_kernel<128, 512>, 128, 512}};
+    {orbits_kernel<256, 1024, 64>, 256, 1024, 64}, {orbits_kernel<256, 512, 64>, 256, 512, 64},
+    {orbits_kernel<256, 2048, 64>, 256, 2048, 64}, {orbits_kernel<128, 1024, 64>, 128, 1024, 64},
+    {orbits_kernel<512, 1024, 64>, 512, 1024, 64}, {orbits_kernel<128, 512, 64>, 128, 512, 64},
+    {orbits_kernel<256, 1024, 2>, 256, 1024, 2},   {orbits_kernel<256, 1024, 4>, 256, 1024, 4},
+    {orbits_kernel<256, 1024, 8>, 256, 1024, 8},   {orbits_kernel<256, 512, 4>, 256, 512, 4}};
 constexpr int ORB_NVARIANTS = sizeof(kOrbVariants) / sizeof(kOrbVariants[0]);
 
 int bits_needed(uint64_t x)
@@ -421,10 +638,11 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
 }
 
 int orbit_variant_count() { return ORB_NVARIANTS; }
-void orbit_variant_shape(int v, int *threads, int *coarse)
+void orbit_variant_shape(int v, int *threads, int *coarse, int *cap)
 {
     *threads = kOrbVariants[v].threads;
     *coarse = kOrbVariants[v].coarse;
+    if (cap) *cap = kOrbVariants[v].cap;
 }
 
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,

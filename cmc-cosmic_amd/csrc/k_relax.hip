@@ -54,14 +54,27 @@ __global__ void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s, 
 }
 
 // state at the first pair (phase 0: offset 3 * 64 w) or slot (phase 1: offset
-// (1 + 64 w) 2^20) of wave w
-__global__ void rng_wave_bases_kernel(const RngTable *__restrict__ T, const Rng *__restrict__ base, int phase,
-                                      int64_t nwaves, Rng *__restrict__ out, int jb1)
+// (1 + 64 w) 2^20) of wave w.  With w = 256 b + t (block b, thread t) the offset
+// splits into a block part, jumped once per block from the global tables, and
+// 3 t 2^6 (phase 0) or t 2^26 (phase 1), jumped through LDS nibble tables of
+// WB_NB consecutive powers; jumps by powers of one matrix commute.
+constexpr int WB_T = 256, WB_NB = 10;
+__global__ __launch_bounds__(WB_T) void rng_wave_bases_kernel(const RngTable *__restrict__ T,
+                                                              const Rng *__restrict__ base, int phase,
+                                                              int64_t nwaves, Rng *__restrict__ out, int jb1)
 {
-    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ NibbleTables<WB_NB> lt;
+    __shared__ Rng bb;
+    const int t = threadIdx.x;
+    const uint64_t b = blockIdx.x;
+    build_nibble_tables<WB_NB>(T, phase == 0 ? 6 : 26, lt);
+    if (t == 0)
+        bb = phase == 0 ? rng_jump(T, base[0], 49152ull * b, 0, jb1)
+                        : rng_jump(T, base[1], (1ull << 20) + (b << 34), 20, jb1);
+    __syncthreads();
+    const int64_t w = (int64_t)b * WB_T + t;
     if (w >= nwaves) return;
-    if (phase == 0) out[w] = rng_jump(T, base[0], 192ull * (uint64_t)w, 0, jb1);
-    else out[w] = rng_jump(T, base[1], (uint64_t)(1 + 64 * w) << 20, 20, jb1);
+    out[w] = rng_jump_lds<WB_NB>(lt, bb, phase == 0 ? 3u * (uint32_t)t : (uint32_t)t);
 }
 
 __global__ void rng_draws_kernel(const RngTable *__restrict__ T, const Rng *__restrict__ base, int phase,
@@ -79,33 +92,51 @@ __global__ void rng_draws_kernel(const RngTable *__restrict__ T, const Rng *__re
 __global__ void dtrel_init_kernel(DevScalars *s) { s->dtrel = __longlong_as_double(0x7ff0000000000000ll); }
 
 // simul_relax_new: one thread per window si = 1+p, 1+3p, ... (si < N - p), the
-// window sums taken in the reference's order, then an order-independent min.
-__global__ void simul_relax_kernel(const double *__restrict__ r, const double *__restrict__ m,
-                                   const double *__restrict__ vr, const double *__restrict__ vt,
-                                   DevScalars *s, KParams P)
+// window sums taken in the reference's order, then an order-independent min: the
+// bit patterns of non-negative doubles order like the values, so the block takes the
+// minimum pattern with wave shuffles and LDS and publishes it with one atomicMin.
+constexpr int SR_T = 256;
+__global__ __launch_bounds__(SR_T) void simul_relax_kernel(const double *__restrict__ r, const double *__restrict__ m,
+                                                           const double *__restrict__ vr,
+                                                           const double *__restrict__ vt, DevScalars *s, KParams P)
 {
+    __shared__ unsigned long long wmin[SR_T / 64];
+    const unsigned long long kInf = 0x7ff0000000000000ull;
+    unsigned long long mine = kInf;
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t p = P.avekernel, N = s->n_max;
     int64_t si = 1 + p + w * 2 * p;
-    if (!(si < N - p)) return;
-    int64_t simin = si - p, simax = simin + (2 * p - 1);
-    double Mv2ave = 0.0, Mave = 0.0, M2ave = 0.0;
-    for (int64_t k = simin; k <= simax; k++) {
-        double tmp = m[k] * P.madhoc;
-        Mv2ave += tmp * (sqr(vr[k]) + sqr(vt[k]));
-        Mave += tmp;
-        M2ave += sqr(tmp);
+    if (si < N - p) {
+        int64_t simin = si - p, simax = simin + (2 * p - 1);
+        double Mv2ave = 0.0, Mave = 0.0, M2ave = 0.0;
+        for (int64_t k = simin; k <= simax; k++) {
+            double tmp = m[k] * P.madhoc;
+            Mv2ave += tmp * (sqr(vr[k]) + sqr(vt[k]));
+            Mave += tmp;
+            M2ave += sqr(tmp);
+        }
+        Mv2ave /= (double)(2 * p);
+        Mave /= (double)(2 * p);
+        M2ave /= (double)(2 * p);
+        double sigma = sqrt(Mv2ave / Mave);
+        double W = 4.0 * sigma / sqrt(3.0 * CMCG_PI);
+        double n_local = calc_n_local(r, si, p, N);
+        double dt = sqr(2.0 * P.theta_se_max / CMCG_PI) * (CMCG_PI / 32.0) * cub(W) /
+                    (((double)P.n_star) * n_local * (4.0 * M2ave));
+        if (dt >= 0.0) mine = (unsigned long long)__double_as_longlong(dt);
     }
-    Mv2ave /= (double)(2 * p);
-    Mave /= (double)(2 * p);
-    M2ave /= (double)(2 * p);
-    double sigma = sqrt(Mv2ave / Mave);
-    double W = 4.0 * sigma / sqrt(3.0 * CMCG_PI);
-    double n_local = calc_n_local(r, si, p, N);
-    double dt = sqr(2.0 * P.theta_se_max / CMCG_PI) * (CMCG_PI / 32.0) * cub(W) /
-                (((double)P.n_star) * n_local * (4.0 * M2ave));
-    if (dt >= 0.0)   // positive doubles order like their bit patterns
-        atomicMin((unsigned long long *)&s->dtrel, (unsigned long long)__double_as_longlong(dt));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        unsigned long long other = __shfl_xor(mine, o, 64);
+        mine = other < mine ? other : mine;
+    }
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = wmin[0];
+        for (int k = 1; k < SR_T / 64; k++) b = wmin[k] < b ? wmin[k] : b;
+        if (b != kInf) atomicMin((unsigned long long *)&s->dtrel, b);
+    }
 }
 
 // calc_timestep (cmc_utils.c:2132-2148) and GetTimeStep's fallback (cmc_evolution_thr.c:430-432)
@@ -274,7 +305,7 @@ hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase
     if (nwaves <= 0) return hipSuccess;
     if (nwaves > sc.max_waves) return hipErrorInvalidValue;
     int jb1 = phase == 0 ? bits_needed(192ull * (uint64_t)nwaves) : 20 + bits_needed((uint64_t)(64 * nwaves + 1));
-    hipLaunchKernelGGL(rng_wave_bases_kernel, dim3((unsigned)((nwaves + 255) / 256)), dim3(256), 0, st, T,
+    hipLaunchKernelGGL(rng_wave_bases_kernel, dim3((unsigned)((nwaves + WB_T - 1) / WB_T)), dim3(WB_T), 0, st, T,
                        (const Rng *)sc.rng_base, phase, nwaves, sc.rng_wave, jb1);
     return hipGetLastError();
 }
@@ -325,8 +356,8 @@ hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &
 {
     hipLaunchKernelGGL(dtrel_init_kernel, dim3(1), dim3(1), 0, st, s);
     int64_t nw = n_ub / (2 * P.avekernel) + 1;
-    hipLaunchKernelGGL(simul_relax_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, f.r, f.m, f.vr,
-                       f.vt, s, P);
+    hipLaunchKernelGGL(simul_relax_kernel, dim3((unsigned)((nw + SR_T - 1) / SR_T)), dim3(SR_T), 0, st, f.r, f.m,
+                       f.vr, f.vt, s, P);
     return hipGetLastError();
 }
 

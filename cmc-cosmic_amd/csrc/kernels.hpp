@@ -84,7 +84,7 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
                          const MlpNorm *nrm = nullptr, float margin = 0.f, hipEvent_t ev_a = nullptr,
                          hipEvent_t ev_b = nullptr, int variant = 0);
 int orbit_variant_count();
-void orbit_variant_shape(int v, int *threads, int *coarse);
+void orbit_variant_shape(int v, int *threads, int *coarse, int *cap = nullptr);
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
                           const double *E, const double *J, double *out4, int64_t *kout, int32_t *circ,
                           hipStream_t st);

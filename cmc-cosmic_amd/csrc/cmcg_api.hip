@@ -351,7 +351,8 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.sort_aos_cap = n_cap + 4;
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * 12)) ||
-        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / 16 + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
+        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) ||
+        (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
@@ -370,8 +371,22 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     RngTable *ht = new RngTable;
     build_table(*ht);
     c->seeded = host_seed(params->seed);
+    // the orbit sampler's per-lane jumps by l 2^20 (l < 64): nibble tables of the
+    // matrix powers 2^20 .. 2^25, t[b][c][q][v] = column XOR for nibble v at bits 4q..
+    auto *nib = new NibbleTables<ORB_NIB_POWERS>;
+    for (int b = 0; b < ORB_NIB_POWERS; b++)
+        for (int cc = 0; cc < 4; cc++)
+            for (int q = 0; q < 8; q++)
+                for (int v = 0; v < 16; v++) {
+                    uint32_t o = 0;
+                    for (int i = 0; i < 4; i++)
+                        if ((v >> i) & 1) o ^= ht->col[ORB_NIB_B0 + b][cc][4 * q + i];
+                    nib->t[b][cc][q][v] = o;
+                }
     bool ok = hipMemcpy(c->T, ht, sizeof(RngTable), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(sc.rng_seeded, &c->seeded, sizeof(Rng), hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(sc.rng_seeded, &c->seeded, sizeof(Rng), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(sc.orb_nib, nib, sizeof(*nib), hipMemcpyHostToDevice) == hipSuccess;
+    delete nib;
     delete ht;
     if (!ok) { cmcg_destroy(c); return CMCG_E_HIP; }
     *out = c;

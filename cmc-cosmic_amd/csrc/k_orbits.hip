@@ -345,12 +345,14 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
                                                              const float *__restrict__ theta,
                                                              const MlpNorm *__restrict__ nrm, float margin,
                                                              const double2 *__restrict__ mid,
-                                                             const double2 *__restrict__ nodes)
+                                                             const double2 *__restrict__ nodes,
+                                                             const double2 *__restrict__ coarse,
+                                                             const NibbleTables<ORB_NIB_POWERS> *__restrict__ nib)
 {
     // the jump tables are needed only for the lanes' first states, the try constants
     // only after the apsides: one LDS area serves both
-    __shared__ union Share {
-        NibbleTables<6> lt;
+    __shared__ union alignas(16) Share {
+        NibbleTables<ORB_NIB_POWERS> lt;
         TryCtx ctx[THREADS];
     } sh;
     __shared__ double cr[COARSE + 1], cphi[COARSE + 1];
@@ -360,15 +362,22 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
     C.stride = coarse_stride(n, COARSE);
     C.hint = false; C.lrp = C.lra = 0.f; C.margin = margin;
     C.mid = mid;
+    // both tables are built once per phase: contiguous copies
     for (int sI = threadIdx.x; sI <= COARSE; sI += blockDim.x) {
-        int64_t k = coarse_k(C, sI);
-        cr[sI] = f.r[k];
-        cphi[sI] = f.phi[k];
+        const double2 v = coarse[sI];
+        cr[sI] = v.x;
+        cphi[sI] = v.y;
     }
-    build_nibble_tables<6>(T, 20, sh.lt);      // ends with a block barrier
+    {
+        constexpr int NQ = (int)(sizeof(NibbleTables<ORB_NIB_POWERS>) / sizeof(uint4));
+        const uint4 *src = reinterpret_cast<const uint4 *>(nib);
+        uint4 *dst = reinterpret_cast<uint4 *>(&sh.lt);
+        for (int q = threadIdx.x; q < NQ; q += blockDim.x) dst[q] = src[q];
+    }
+    __syncthreads();
     const int64_t j = 1 + xcd_block() * blockDim.x + threadIdx.x;
     Rng rng{};
-    if (j <= n) rng = rng_jump_lds<6>(sh.lt, wave_base[(j - 1) >> 6], (uint32_t)((j - 1) & 63));
+    if (j <= n) rng = rng_jump_lds<ORB_NIB_POWERS>(sh.lt, wave_base[(j - 1) >> 6], (uint32_t)((j - 1) & 63));
     __syncthreads();                           // the tables are dead from here
     bool need = false;
     int flags = 0, ntry = 0, hits = 0, misses = 0;
@@ -587,12 +596,26 @@ __global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count,
     out[i] = k;
 }
 
+// The LDS samples of the (r, phi) table for a COARSE-sample launch: node
+// min(s * stride, n + 1) for s = 0..cs, from the packed nodes.
+__global__ void coarse_build_kernel(const double2 *__restrict__ nodes, const DevScalars *s, int cs,
+                                    double2 *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > cs) return;
+    const int64_t n = s->n_max;
+    const int64_t st = coarse_stride(n, cs);
+    const int64_t k = (int64_t)i * st;
+    out[i] = nodes[k < n + 1 ? k : n + 1];
+}
+
 // Launch shapes of the orbit sampler that the autotuner chooses from: threads per
 // block, LDS samples of the (r, phi) table, and the most lanes one star's tries get
 // in a round (64: all idle lanes; fewer: less speculative work).  Every variant gives
 // the same stars; only the block order of the escape sums differs.
 using OrbFn = void (*)(DevFields, DevScalars *, KParams, const RngTable *, const Rng *, double *, int64_t *,
-                       const float *, const MlpNorm *, float, const double2 *, const double2 *);
+                       const float *, const MlpNorm *, float, const double2 *, const double2 *, const double2 *,
+                       const NibbleTables<ORB_NIB_POWERS> *);
 struct OrbVariant { OrbFn fn; int threads, coarse, cap; };
 const OrbVariant kOrbVariants[] = {
     {orbits_kernel<256, 1024, 64>, 256, 1024, 64}, {orbits_kernel<256, 512, 64>, 256, 512, 64},
@@ -622,10 +645,14 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     hipLaunchKernelGGL(orbits_begin_kernel, dim3(1), dim3(1), 0, st, s);
     if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
+    if (V.coarse > ORB_MAX_COARSE) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(coarse_build_kernel, dim3((unsigned)((V.coarse + 256) / 256)), dim3(256), 0, st,
+                       (const double2 *)sc.nodes, (const DevScalars *)s, V.coarse, sc.coarse);
     if (ev_a) hipEventRecord(ev_a, st);
     hipLaunchKernelGGL(V.fn, dim3((unsigned)nblocks), dim3(V.threads), 0, st, f, s, P, T, (const Rng *)sc.rng_wave,
                        sc.partials, sc.defer_list, theta, nrm, margin, (const double2 *)sc.mid,
-                       (const double2 *)sc.nodes);
+                       (const double2 *)sc.nodes, (const double2 *)sc.coarse,
+                       (const NibbleTables<ORB_NIB_POWERS> *)sc.orb_nib);
     if (ev_b) hipEventRecord(ev_b, st);
     hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,

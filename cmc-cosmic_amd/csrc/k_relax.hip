@@ -310,7 +310,7 @@ hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase
     return hipGetLastError();
 }
 
-// Packed (r, phi) nodes for the searches and the middle table (every 16th node).
+// Packed (r, phi) nodes for the searches and the middle table (every MID_STEP-th node).
 __global__ void mid_build_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
                                  double2 *__restrict__ nodes, double2 *__restrict__ mid)
 {
@@ -318,7 +318,7 @@ __global__ void mid_build_kernel(const double *__restrict__ r, const double *__r
     if (k > s->n_max + 1) return;
     double2 v = make_double2(r[k], phi[k]);
     nodes[k] = v;
-    if ((k & 15) == 0) mid[k >> 4] = v;
+    if (k % MID_STEP == 0) mid[k / MID_STEP] = v;
 }
 
 hipError_t launch_mid_build(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st)

@@ -155,8 +155,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
     __shared__ uint64_t skeys[RS_TILE];
     __shared__ uint32_t svals[RS_TILE];
     __shared__ uint32_t wcnt[NW][256];
-    __shared__ uint32_t cnt[256];
-    __shared__ uint32_t dstart[256];
+    __shared__ uint32_t dstart[256];       // 53 KB in all: three blocks per CU
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -185,18 +184,14 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
     }
     __syncthreads();
     // per digit (thread t = digit): the waves' offsets and the tile total
-    {
-        uint32_t tot = 0;
-        for (int w2 = 0; w2 < NW; w2++) {
-            uint32_t c = wcnt[w2][t];
-            wcnt[w2][t] = tot;
-            tot += c;
-        }
-        cnt[t] = tot;
+    uint32_t tot = 0;
+    for (int w2 = 0; w2 < NW; w2++) {
+        uint32_t c = wcnt[w2][t];
+        wcnt[w2][t] = tot;
+        tot += c;
     }
-    __syncthreads();
     // exclusive scan of the tile's digit counts
-    dstart[t] = cnt[t];
+    dstart[t] = tot;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
         uint32_t x = t >= off ? dstart[t - off] : 0u;
@@ -206,7 +201,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
     }
     uint32_t incl = dstart[t];
     __syncthreads();
-    dstart[t] = incl - cnt[t];
+    dstart[t] = incl - tot;
     __syncthreads();
     for (int u = 0; u < RS_ITEMS; u++) {
         int64_t i = wbase + u * 64 + lane;

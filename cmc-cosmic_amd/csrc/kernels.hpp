@@ -7,6 +7,11 @@
 #include "cmcg_internal.hpp"
 #include "mlp.hpp"
 
+// orbit sampler: most LDS samples of any launch variant; lane jumps l 2^20 by nibble
+// tables of the powers 2^ORB_NIB_B0 .. 2^(ORB_NIB_B0 + ORB_NIB_POWERS - 1)
+constexpr int ORB_MAX_COARSE = 2048;
+constexpr int ORB_NIB_B0 = 20, ORB_NIB_POWERS = 6;
+
 struct Scratch {
     // potential chains: per-tile int128 sums and prefixes plus a boundary value
     // (48 B per tile), chain increments / iterates / enclosed masses per star
@@ -37,6 +42,8 @@ struct Scratch {
     double2 *mid;               // (r, phi) at every 16th node (search middle level)
     double2 *nodes;             // packed (r, phi) of every node, for the searches
     double *u_old;              // new potential at the old sorted radii (set_velocities3)
+    double2 *coarse;            // orbit sampler's (r, phi) samples, copied into LDS by every block
+    void *orb_nib;              // orbit sampler's GF(2) nibble tables (constant per context)
     int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event
     // walk ends and merge flags, per-block counts, fallback flag

@@ -350,8 +350,9 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
     sc.sort_aos_cap = n_cap + 4;
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
-        (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * 12)) ||
-        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) ||
+        (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)SORT_LONG_RUNS)) ||
+        (e = dalloc(c, &sc.sort_nlong, 1)) ||
+        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
         (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.vals_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||

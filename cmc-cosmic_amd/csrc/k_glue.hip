@@ -32,7 +32,8 @@ __device__ void final_sum(const double *__restrict__ partials, int64_t nblocks, 
     const int t = threadIdx.x;
     double acc[NV];
     for (int i = 0; i < NV; i++) acc[i] = 0.0;
-    for (int64_t b = t; b < nblocks; b += GT)
+#pragma unroll 8
+    for (int64_t b = t; b < nblocks; b += GT)      // unrolled: independent loads in flight
         for (int i = 0; i < NV; i++) acc[i] += partials[8 * b + i];
     for (int i = 0; i < NV; i++) lds[i][t] = acc[i];
     __syncthreads();
@@ -321,6 +322,7 @@ __global__ __launch_bounds__(1024) void sv3_scan_kernel(DevScalars *s, int64_t *
     const int64_t chunk = (nb + 1023) / 1024;
     const int64_t b0 = (int64_t)t * chunk;
     int64_t acc = 0;
+#pragma unroll 8
     for (int64_t b = b0; b < b0 + chunk && b < nb; b++) acc += blk_cnt[b];
     lds[t] = acc;
     __syncthreads();

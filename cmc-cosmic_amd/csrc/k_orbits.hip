@@ -522,28 +522,48 @@ __global__ void orbits_begin_kernel(DevScalars *s)
     s->n_defer = 0;
 }
 
-// Sums the per-block partials, then the deferred stars' in star order, and finishes
-// new_orbits_calculate.  1024 threads: 16 consecutive threads read one partial record
-// (coalesced), 64 groups take blocks g, g + 64, ..., and the 64 group results are
-// combined in group order (a fixed order, so the sums are deterministic).
-__global__ __launch_bounds__(1024) void orbits_end_kernel(DevScalars *s, const double *__restrict__ partials,
-                                                          int64_t nblocks, const double *__restrict__ dpartials)
+// First level of the orbit phase's reduction: block B of RED2_BLOCKS sums the partial
+// records of sampler blocks [B C, (B + 1) C), C = ceil(nblocks / RED2_BLOCKS); 16
+// consecutive threads read one record (coalesced), its 64 groups take records
+// g, g + 64, ... of the range, and the groups are combined in group order.
+__global__ __launch_bounds__(1024) void orbits_part_kernel(const double *__restrict__ partials, int64_t nblocks,
+                                                           double *__restrict__ out)
 {
     __shared__ double gsum[64][RED_STRIDE];
     const int t = threadIdx.x, i = t & 15, g = t >> 4;
+    const int64_t C = (nblocks + RED2_BLOCKS - 1) / RED2_BLOCKS;
+    const int64_t b0 = (int64_t)blockIdx.x * C, b1 = b0 + C < nblocks ? b0 + C : nblocks;
     double acc = 0.0;
     if (i < NRED)
-        for (int64_t b = g; b < nblocks; b += 64) {
+        for (int64_t b = b0 + g; b < b1; b += 64) {
             double p = partials[RED_STRIDE * b + i];
             acc = red_is_max(i) ? (acc >= p ? acc : p) : acc + p;
         }
     gsum[g][i] = acc;
     __syncthreads();
-    if (t == 0) {
+    if (t < NRED) {
+        double a = gsum[0][t];
+        for (int g2 = 1; g2 < 64; g2++) a = red_is_max(t) ? (a >= gsum[g2][t] ? a : gsum[g2][t]) : a + gsum[g2][t];
+        out[RED_STRIDE * blockIdx.x + t] = a;
+    }
+}
+
+// Sums the first level's RED2_BLOCKS records in block order, then the deferred stars'
+// in star order, and finishes new_orbits_calculate (a fixed order: deterministic sums).
+__global__ __launch_bounds__(RED2_BLOCKS * RED_STRIDE) void orbits_end_kernel(DevScalars *s,
+                                                                              const double *__restrict__ red2,
+                                                                              const double *__restrict__ dpartials)
+{
+    __shared__ double sv[RED2_BLOCKS * RED_STRIDE];
+    sv[threadIdx.x] = red2[threadIdx.x];       // all loads at once
+    __syncthreads();
+    if (threadIdx.x == 0) {
         double v[NRED];
         for (int k = 0; k < NRED; k++) {
-            double a = gsum[0][k];
-            for (int g2 = 1; g2 < 64; g2++) a = red_is_max(k) ? (a >= gsum[g2][k] ? a : gsum[g2][k]) : a + gsum[g2][k];
+            double a = sv[k];
+            for (int b = 1; b < RED2_BLOCKS; b++)
+                a = red_is_max(k) ? (a >= sv[RED_STRIDE * b + k] ? a : sv[RED_STRIDE * b + k])
+                                  : a + sv[RED_STRIDE * b + k];
             v[k] = a;
         }
         int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
@@ -659,7 +679,9 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
                        sc.defer_kfirst);
     hipLaunchKernelGGL(defer_tail_kernel, dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,
                        (const int64_t *)sc.defer_list, (const int64_t *)sc.defer_kfirst, sc.defer_partials);
-    hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(1024), 0, st, s, (const double *)sc.partials, nblocks,
+    hipLaunchKernelGGL(orbits_part_kernel, dim3(RED2_BLOCKS), dim3(1024), 0, st, (const double *)sc.partials, nblocks,
+                       sc.red2);
+    hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(RED2_BLOCKS * RED_STRIDE), 0, st, s, (const double *)sc.red2,
                        (const double *)sc.defer_partials);
     return hipGetLastError();
 }

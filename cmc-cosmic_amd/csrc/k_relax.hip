@@ -93,27 +93,55 @@ __global__ void dtrel_init_kernel(DevScalars *s) { s->dtrel = __longlong_as_doub
 
 // simul_relax_new: one thread per window si = 1+p, 1+3p, ... (si < N - p), the
 // window sums taken in the reference's order, then an order-independent min: the
-// bit patterns of non-negative doubles order like the values, so the block takes the
-// minimum pattern with wave shuffles and LDS and publishes it with one atomicMin.
-constexpr int SR_T = 256;
+// bit patterns of non-negative doubles order like the values, so the wave takes the
+// minimum pattern with shuffles and publishes it with one atomicMin.  A block (one
+// wave) stages its windows' contiguous range in LDS with coalesced loads, as the
+// per-element terms m*madhoc and m*madhoc*(vr^2 + vt^2) (the same roundings as the
+// reference's loop); windows of more than SR_LDS elements read global memory.
+constexpr int SR_T = 64, SR_LDS = 2560;
 __global__ __launch_bounds__(SR_T) void simul_relax_kernel(const double *__restrict__ r, const double *__restrict__ m,
                                                            const double *__restrict__ vr,
                                                            const double *__restrict__ vt, DevScalars *s, KParams P)
 {
-    __shared__ unsigned long long wmin[SR_T / 64];
+    __shared__ double stmp[SR_LDS], sq[SR_LDS];
     const unsigned long long kInf = 0x7ff0000000000000ull;
     unsigned long long mine = kInf;
-    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t p = P.avekernel, N = s->n_max;
-    int64_t si = 1 + p + w * 2 * p;
-    if (si < N - p) {
-        int64_t simin = si - p, simax = simin + (2 * p - 1);
+    const int64_t p = P.avekernel, N = s->n_max, len = 2 * p;
+    const bool staged = len <= SR_LDS;
+    const int G = staged ? (SR_LDS / len < SR_T ? (int)(SR_LDS / len) : SR_T) : SR_T;   // windows per block
+    const int64_t w0 = (int64_t)blockIdx.x * G;
+    const int64_t e0 = 1 + w0 * len;
+    if (staged) {
+        for (int64_t q = threadIdx.x; q < G * len; q += SR_T) {
+            const int64_t k = e0 + q;
+            if (k <= N) {
+                const double tmp = m[k] * P.madhoc;
+                stmp[q] = tmp;
+                sq[q] = tmp * (sqr(vr[k]) + sqr(vt[k]));
+            }
+        }
+        __syncthreads();
+    }
+    const int64_t w = w0 + threadIdx.x;
+    const int64_t si = 1 + p + w * 2 * p;
+    if (threadIdx.x < G && si < N - p) {
+        const int64_t simin = si - p;
         double Mv2ave = 0.0, Mave = 0.0, M2ave = 0.0;
-        for (int64_t k = simin; k <= simax; k++) {
-            double tmp = m[k] * P.madhoc;
-            Mv2ave += tmp * (sqr(vr[k]) + sqr(vt[k]));
-            Mave += tmp;
-            M2ave += sqr(tmp);
+        if (staged) {
+            const int64_t q0 = simin - e0;
+            for (int64_t q = q0; q < q0 + len; q++) {
+                const double tmp = stmp[q];
+                Mv2ave += sq[q];
+                Mave += tmp;
+                M2ave += sqr(tmp);
+            }
+        } else {
+            for (int64_t k = simin; k < simin + len; k++) {
+                double tmp = m[k] * P.madhoc;
+                Mv2ave += tmp * (sqr(vr[k]) + sqr(vt[k]));
+                Mave += tmp;
+                M2ave += sqr(tmp);
+            }
         }
         Mv2ave /= (double)(2 * p);
         Mave /= (double)(2 * p);
@@ -130,13 +158,7 @@ __global__ __launch_bounds__(SR_T) void simul_relax_kernel(const double *__restr
         unsigned long long other = __shfl_xor(mine, o, 64);
         mine = other < mine ? other : mine;
     }
-    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = mine;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long b = wmin[0];
-        for (int k = 1; k < SR_T / 64; k++) b = wmin[k] < b ? wmin[k] : b;
-        if (b != kInf) atomicMin((unsigned long long *)&s->dtrel, b);
-    }
+    if (threadIdx.x == 0 && mine != kInf) atomicMin((unsigned long long *)&s->dtrel, mine);
 }
 
 // calc_timestep (cmc_utils.c:2132-2148) and GetTimeStep's fallback (cmc_evolution_thr.c:430-432)
@@ -355,9 +377,11 @@ hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int
 hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st)
 {
     hipLaunchKernelGGL(dtrel_init_kernel, dim3(1), dim3(1), 0, st, s);
-    int64_t nw = n_ub / (2 * P.avekernel) + 1;
-    hipLaunchKernelGGL(simul_relax_kernel, dim3((unsigned)((nw + SR_T - 1) / SR_T)), dim3(SR_T), 0, st, f.r, f.m,
-                       f.vr, f.vt, s, P);
+    const int64_t nw = n_ub / (2 * P.avekernel) + 1;
+    const int64_t len = 2 * P.avekernel;
+    const int64_t G = len <= SR_LDS ? (SR_LDS / len < SR_T ? SR_LDS / len : SR_T) : SR_T;   // as in the kernel
+    hipLaunchKernelGGL(simul_relax_kernel, dim3((unsigned)((nw + G - 1) / G)), dim3(SR_T), 0, st, f.r, f.m, f.vr,
+                       f.vt, s, P);
     return hipGetLastError();
 }
 

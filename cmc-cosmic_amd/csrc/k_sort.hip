@@ -17,8 +17,15 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_MAX_FIELDS = 24;
-constexpr int TIE_MAX_RUN = 64;
-constexpr int AOS_W = 12;          // words per packed payload record (96 B)
+// LSD passes over the top 40 bits of the key only; keys equal in those bits are then
+// put in (key, id) order by the run pass (one thread per run up to RUN_SHORT keys,
+// one block sorting in LDS up to RUN_LONG keys).
+constexpr int RS_PASSES = 5;
+constexpr int RS_SHIFT0 = 64 - 8 * RS_PASSES;
+constexpr int RUN_SHORT = 64;
+constexpr int RUN_LONG = 4096;
+constexpr int AOS_W = SORT_AOS_WORDS;   // words per packed payload record (96 B)
+constexpr int AOS_LD = AOS_W + 1;        // odd LDS stride per record: no bank conflicts
 
 __device__ __forceinline__ uint64_t sortable(double x)
 {
@@ -49,10 +56,11 @@ __global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n)
 __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__restrict__ keysrc, DevScalars *s,
                                                              uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                              uint32_t *__restrict__ digit_totals,
-                                                             uint64_t *__restrict__ aos, GatherList g)
+                                                             uint64_t *__restrict__ aos, GatherList g, uint64_t kinf)
 {
+    constexpr int P0 = 8 - RS_PASSES;      // first byte sorted on
     __shared__ uint32_t h[8][256];
-    __shared__ uint64_t recs[RS_THREADS * AOS_W];      // one chunk of packed records
+    __shared__ uint64_t recs[RS_THREADS * AOS_LD];     // one chunk of packed records
     const int t = threadIdx.x;
     for (int p = 0; p < 8; p++) h[p][t] = 0;
     __syncthreads();
@@ -63,22 +71,23 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
         const int64_t i = i0 + t;
         if (i < n) {
             uint64_t k = sortable(keysrc[i + 1]);
+            if (k >= kinf) k = ~0ull;      // the SF_INFINITY tail: one key, no prefix shared with a live star
             keys[i] = k;
             vals[i] = (uint32_t)(i + 1);
-            for (int p = 0; p < 8; p++) atomicAdd(&h[p][(k >> (8 * p)) & 255], 1u);
+            for (int p = P0; p < 8; p++) atomicAdd(&h[p][(k >> (8 * p)) & 255], 1u);
         }
         if (aos && i0 < n) {        // block-uniform: pack the chunk through LDS, write it contiguously
             if (i < n)
-                for (int f = 0; f < g.nf; f++) recs[AOS_W * t + f] = g.src[f][i + 1];
+                for (int f = 0; f < g.nf; f++) recs[AOS_LD * t + f] = g.src[f][i + 1];
             __syncthreads();
             const int64_t cnt = n - i0 < RS_THREADS ? n - i0 : RS_THREADS;
             uint64_t *dst = aos + (int64_t)AOS_W * i0;
-            for (int w = t; w < cnt * AOS_W; w += RS_THREADS) dst[w] = recs[w];
+            for (int w = t; w < cnt * AOS_W; w += RS_THREADS) dst[w] = recs[AOS_LD * (w / AOS_W) + w % AOS_W];
             __syncthreads();
         }
     }
     __syncthreads();
-    for (int p = 0; p < 8; p++)
+    for (int p = P0; p < 8; p++)
         if (h[p][t]) atomicAdd(&digit_totals[p * 256 + t], h[p][t]);
 }
 
@@ -226,27 +235,135 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
     }
 }
 
-// Runs of equal keys (below the SF_INFINITY tail) are ordered by id.
-__global__ void rs_ties_kernel(const uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                               const int64_t *__restrict__ id, DevScalars *s, uint64_t kinf)
+// Runs of keys equal in the top 40 bits (live stars only: the tail keys are all ones)
+// are put in (key, id) order.  Run starts sort runs of up to RUN_SHORT keys
+// themselves (insertion sort) and list longer ones for rs_long_runs_kernel.
+__device__ __forceinline__ bool key_id_less(uint64_t ka, int64_t ia, uint64_t kb, int64_t ib)
+{
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+__global__ void rs_runs_kernel(uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                               const int64_t *__restrict__ id, DevScalars *s, int64_t *__restrict__ long_runs,
+                               int64_t *__restrict__ n_long)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max_new;
     if (i + 1 >= n) return;
-    uint64_t k = keys[i];
-    if (k >= kinf || keys[i + 1] != k) return;
-    if (i > 0 && keys[i - 1] == k) return;
+    const uint64_t k = keys[i];
+    if (k == ~0ull) return;
+    const uint64_t p = k >> RS_SHIFT0;
+    if ((keys[i + 1] >> RS_SHIFT0) != p) return;
+    if (i > 0 && (keys[i - 1] >> RS_SHIFT0) == p) return;
     int64_t e = i + 1;
-    while (e + 1 < n && keys[e + 1] == k && e - i < TIE_MAX_RUN) e++;
-    if (e + 1 < n && keys[e + 1] == k) { dev_set_error(s, CMCG_E_TIES, i + 1); return; }
+    while (e + 1 < n && (keys[e + 1] >> RS_SHIFT0) == p && e - i < RUN_SHORT) e++;
+    if (e + 1 < n && (keys[e + 1] >> RS_SHIFT0) == p) {
+        unsigned long long slot = atomicAdd((unsigned long long *)n_long, 1ull);
+        if ((int64_t)slot < SORT_LONG_RUNS) long_runs[slot] = i;
+        else dev_set_error(s, CMCG_E_TIES, i + 1);
+        return;
+    }
     for (int64_t a = i + 1; a <= e; a++) {
-        uint32_t v = vals[a];
-        int64_t idv = id[v];
+        const uint64_t ka = keys[a];
+        const uint32_t v = vals[a];
+        int64_t idv = 0;
+        bool have_id = false;            // ids are gathered only for equal keys
         int64_t b = a - 1;
-        while (b >= i && id[vals[b]] > idv) { vals[b + 1] = vals[b]; b--; }
+        while (b >= i) {
+            const uint64_t kb = keys[b];
+            const uint32_t vb = vals[b];
+            bool less;
+            if (ka != kb) {
+                less = ka < kb;
+            } else {
+                if (!have_id) { idv = id[v]; have_id = true; }
+                less = idv < id[vb];
+            }
+            if (!less) break;
+            keys[b + 1] = kb;
+            vals[b + 1] = vb;
+            b--;
+        }
+        keys[b + 1] = ka;
         vals[b + 1] = v;
     }
     atomicAdd((unsigned long long *)&s->ntie_runs, 1ull);
+}
+
+// Long runs: one block per run (grid-stride over the list), bitonic sort by (key, id)
+// in LDS, padded to a power of two with all-ones entries.
+constexpr int LR_T = 256;
+__global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                            const int64_t *__restrict__ id, DevScalars *s,
+                                                            const int64_t *__restrict__ long_runs,
+                                                            const int64_t *__restrict__ n_long)
+{
+    __shared__ uint64_t sk[RUN_LONG];
+    __shared__ int64_t sid[RUN_LONG];
+    __shared__ uint32_t sv[RUN_LONG];
+    __shared__ int64_t len_s;
+    const int64_t nl = *n_long < SORT_LONG_RUNS ? *n_long : SORT_LONG_RUNS;
+    const int64_t n = s->n_max_new;
+    const int t = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < nl; r += gridDim.x) {
+        const int64_t i = long_runs[r];
+        if (t == 0) {
+            // keys are ordered by prefix: the last index with the run's prefix by bisection
+            const uint64_t p = keys[i] >> RS_SHIFT0;
+            int64_t lo = i, hi = i + RUN_LONG < n ? i + RUN_LONG : n - 1;
+            if ((keys[hi] >> RS_SHIFT0) == p) {
+                len_s = hi == i + RUN_LONG ? -1 : hi - i + 1;     // RUN_LONG + 1 keys do not fit
+            } else {
+                while (hi - lo > 1) {
+                    const int64_t m = lo + ((hi - lo) >> 1);
+                    if ((keys[m] >> RS_SHIFT0) == p) lo = m;
+                    else hi = m;
+                }
+                len_s = lo - i + 1;
+            }
+            if (len_s < 0) dev_set_error(s, CMCG_E_TIES, i + 1);
+        }
+        __syncthreads();
+        const int64_t len = len_s;
+        __syncthreads();
+        if (len < 0) continue;
+        int P = 1;
+        while (P < len) P <<= 1;
+        for (int q = t; q < P; q += LR_T) {
+            if (q < len) {
+                const uint32_t v = vals[i + q];
+                sk[q] = keys[i + q];
+                sv[q] = v;
+                sid[q] = id[v];
+            } else {
+                sk[q] = ~0ull;
+                sid[q] = INT64_MAX;
+                sv[q] = 0u;
+            }
+        }
+        __syncthreads();
+        for (int kk = 2; kk <= P; kk <<= 1)
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                for (int q = t; q < P; q += LR_T) {
+                    const int x = q ^ j;
+                    if (x > q) {
+                        const bool up = (q & kk) == 0;
+                        const bool gt = key_id_less(sk[x], sid[x], sk[q], sid[q]);
+                        if (gt == up) {
+                            uint64_t tk = sk[q]; sk[q] = sk[x]; sk[x] = tk;
+                            int64_t ti = sid[q]; sid[q] = sid[x]; sid[x] = ti;
+                            uint32_t tv = sv[q]; sv[q] = sv[x]; sv[x] = tv;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int q = t; q < len; q += LR_T) {
+            keys[i + q] = sk[q];
+            vals[i + q] = sv[q];
+        }
+        __syncthreads();
+    }
 }
 
 // number of live stars = keys below the SF_INFINITY key
@@ -294,7 +411,7 @@ __global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__re
     uint64_t w[AOS_W];
 #pragma unroll
     for (int q = 0; q < AOS_W / 2; q++) {
-        ulonglong2 v = rec[q];
+        ulonglong2 v = 2 * q < g.nf ? rec[q] : ulonglong2{0ull, 0ull};
         w[2 * q] = v.x;
         w[2 * q + 1] = v.y;
     }
@@ -309,7 +426,13 @@ __global__ void rs_finish_kernel(DevScalars *s)
     s->n_max_new = s->sort_n;
 }
 
-__global__ void rs_begin_kernel(DevScalars *s) { s->sort_n = 0; s->ntie_runs = 0; s->sort_in = s->n_max_new; }
+__global__ void rs_begin_kernel(DevScalars *s, int64_t *n_long)
+{
+    s->sort_n = 0;
+    s->ntie_runs = 0;
+    s->sort_in = s->n_max_new;
+    *n_long = 0;
+}
 } // namespace
 
 // Sort slots 1..n_max_new by (key field, id).  src/dst are parallel lists of payload
@@ -328,13 +451,16 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     if (ntiles < 1) ntiles = 1;
     if (ntiles > sc.max_sort_tiles) return hipErrorInvalidValue;
     const uint64_t kinf = 0x8000000000000000ull | 0x4202a05f20000000ull;   // sortable(1.0e10)
-    hipLaunchKernelGGL(rs_begin_kernel, dim3(1), dim3(1), 0, st, s);
+    hipLaunchKernelGGL(rs_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sort_nlong);
     hipLaunchKernelGGL(rs_zero_kernel, dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256));
-    hipLaunchKernelGGL(rs_prep_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, sc.keys_a,
-                       sc.vals_a, sc.digit_totals, aos, g);
-    uint64_t *kin = sc.keys_a, *kout = sc.keys_b;
-    uint32_t *vin = sc.vals_a, *vout = sc.vals_b;
-    for (int pass = 0; pass < 8; pass++) {
+    // the passes alternate between the two buffer pairs; start so that the sorted keys
+    // and the permutation end in keys_a / vals_a (set_velocities3 reads vals_a)
+    constexpr bool odd = RS_PASSES % 2 == 1;
+    uint64_t *kin = odd ? sc.keys_b : sc.keys_a, *kout = odd ? sc.keys_a : sc.keys_b;
+    uint32_t *vin = odd ? sc.vals_b : sc.vals_a, *vout = odd ? sc.vals_a : sc.vals_b;
+    hipLaunchKernelGGL(rs_prep_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin, vin,
+                       sc.digit_totals, aos, g, kinf);
+    for (int pass = 8 - RS_PASSES; pass < 8; pass++) {
         int shift = 8 * pass;
         hipLaunchKernelGGL(rs_hist_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, s, shift,
                            sc.block_hist, ntiles);
@@ -346,7 +472,9 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
         uint32_t *tv = vin; vin = vout; vout = tv;
     }
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
-    hipLaunchKernelGGL(rs_ties_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, vin, id_src, s, kinf);
+    hipLaunchKernelGGL(rs_runs_kernel, dim3(g1), dim3(256), 0, st, kin, vin, id_src, s, sc.sort_long, sc.sort_nlong);
+    hipLaunchKernelGGL(rs_long_runs_kernel, dim3(64), dim3(LR_T), 0, st, kin, vin, id_src, s,
+                       (const int64_t *)sc.sort_long, (const int64_t *)sc.sort_nlong);
     hipLaunchKernelGGL(rs_count_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, kinf);
     if (aos)
         hipLaunchKernelGGL(rs_gather_aos_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin,

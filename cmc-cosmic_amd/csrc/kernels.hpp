@@ -10,6 +10,10 @@
 // orbit sampler: most LDS samples of any launch variant; lane jumps l 2^20 by nibble
 // tables of the powers 2^ORB_NIB_B0 .. 2^(ORB_NIB_B0 + ORB_NIB_POWERS - 1)
 constexpr int ORB_MAX_COARSE = 2048;
+// sort payload records: 12 words (96 B) per star
+constexpr int SORT_AOS_WORDS = 12;
+constexpr int SORT_LONG_RUNS = 4096;
+constexpr int RED2_BLOCKS = 64;
 constexpr int ORB_NIB_B0 = 20, ORB_NIB_POWERS = 6;
 
 struct Scratch {
@@ -38,12 +42,15 @@ struct Scratch {
     uint32_t *block_hist;       // 256 * max_sort_tiles
     uint32_t *digit_totals;     // 8 * 256
     int64_t max_sort_tiles;
-    uint64_t *sort_aos;         // packed payload records, 12 words per star
+    uint64_t *sort_aos;         // packed payload records, SORT_AOS_WORDS words per star
+    int64_t *sort_long;         // starts of long runs of equal key prefixes (SORT_LONG_RUNS)
+    int64_t *sort_nlong;        // their count
     double2 *mid;               // (r, phi) at every 16th node (search middle level)
     double2 *nodes;             // packed (r, phi) of every node, for the searches
     double *u_old;              // new potential at the old sorted radii (set_velocities3)
     double2 *coarse;            // orbit sampler's (r, phi) samples, copied into LDS by every block
     void *orb_nib;              // orbit sampler's GF(2) nibble tables (constant per context)
+    double *red2;               // second level of the block-partial reductions (RED2_BLOCKS x 16)
     int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event
     // walk ends and merge flags, per-block counts, fallback flag

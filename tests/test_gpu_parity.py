@@ -123,14 +123,24 @@ def test_gpu_potential_tail_and_count(gpu, O):
 
 
 # ------------------------------------------------------------------ sort
-@pytest.mark.parametrize("n,ties", [(100003, True), (4000000, False)])
+@pytest.mark.parametrize("n,ties", [(100003, "short"), (300007, "long"), (4000000, None)])
 def test_gpu_sort_bit_exact(gpu, O, n, ties):
+    """Bit-exact (r, id) order against the oracle.  The device sorts on the top 40 bits
+    of the key and orders runs of equal prefixes by (key, id): "short" has many runs of
+    equal radii; "long" has runs of 3000 and 500 equal radii and a run of 1200 radii
+    that agree in the top 40 bits but differ below."""
     cm = gpu
     say("sort n", n, "ties", ties)
     rng = np.random.default_rng(n)
     cl = cm.Cluster(n)
-    if ties:
+    if ties == "short":
         r = rng.choice(rng.random(n // 3) * 5 + 1e-3, n)
+    elif ties == "long":
+        r = np.exp(rng.normal(size=n))
+        r[rng.choice(n, 3000, replace=False)] = 0.75
+        r[rng.choice(n, 500, replace=False)] = 2.5
+        near = rng.choice(n, 1200, replace=False)
+        r[near] = 1.3 * (1.0 + rng.integers(0, 4000, 1200) * 2.0 ** -52)
     else:
         r = np.exp(rng.normal(size=n) * 2)
     r[rng.choice(n, n // 200, replace=False)] = 1e10

@@ -65,6 +65,56 @@ def cpu_baseline(cl):
                       f"(CPU oracle: serial, reference RNG order, long-double Q), {dt:.1f} s"}
 
 
+def ensemble_bench(cm, steps, seed=7):
+    """64 independent clusters stepped together: N log-uniform on [1e4, 2e6] (seeded),
+    32 Kroupa Plummer and 32 Kroupa King models with W0 = 3, 5, 7, 9, one context each.
+    Times the ensemble step (every cluster's step graph launched on its own stream
+    before any wait) against stepping the same clusters one after another (eager)."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(1e4), np.log(2e6), 64)).astype(np.int64)
+    ctxs = []
+    t0 = time.perf_counter()
+    for i, n in enumerate(ns):
+        n = int(n)
+        if i < 32:
+            cl = cm.make_plummer(n, seed=100 + i, kroupa=True)
+        else:
+            cl = cm.make_king(n, (3.0, 5.0, 7.0, 9.0)[i % 4], seed=100 + i, kroupa=True, segregate=False)
+        ctx = cm.Context(n, cm.default_params(n, orbit_r=cl.rtid, seed=100 + i))
+        ctx.upload(cl)
+        ctx.init_cluster()
+        ctxs.append(ctx)
+        del cl
+    log(f"ensemble: 64 clusters, {int(ns.sum())} stars, set up in {time.perf_counter() - t0:.1f} s")
+    for _ in range(2):                   # captures both pointer states' graphs
+        cm.ensemble_step(ctxs)
+    live = lambda: sum(c.scalars()["n_max"] for c in ctxs)
+    stars, t_ens = 0, 0.0
+    for _ in range(steps):
+        nl = live()
+        t = time.perf_counter()
+        cm.ensemble_step(ctxs)
+        t_ens += time.perf_counter() - t
+        stars += nl
+    seq_stars, t_seq = 0, 0.0
+    for _ in range(max(1, steps // 2)):
+        nl = live()
+        t = time.perf_counter()
+        for c in ctxs:
+            c.step()
+        t_seq += time.perf_counter() - t
+        seq_stars += nl
+    for c in ctxs:
+        c.close()
+    out = {"clusters": 64, "stars": int(ns.sum()), "n_min": int(ns.min()), "n_max": int(ns.max()),
+           "steps": steps, "value": stars / t_ens, "unit": "stars/s", "ms_per_step": t_ens / steps * 1e3,
+           "sequential_eager": {"value": seq_stars / t_seq, "ms_per_step": t_seq / max(1, steps // 2) * 1e3},
+           "note": "R_vir sets physical units only and does not enter the Henon-unit step"}
+    log(f"ensemble: {out['value']:.3e} stars/s ({out['ms_per_step']:.2f} ms per step of all 64), "
+        f"one by one {out['sequential_eager']['value']:.3e} stars/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +123,7 @@ def main():
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--train-iters", type=int, default=1500)
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported "
@@ -194,6 +245,7 @@ def main():
     roofline = {"kernel": "orbits_kernel (orbit sampling, k_orbits.hip)", "bound": "hbm", "achieved": d["alg_GBs"],
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                 "alg_bytes_per_launch": ALG_BYTES["orbits"] * float(np.mean(live)), "avg_launch_ms": d["ms"]}
+    ens = None if args.no_ensemble else ensemble_bench(cm, 10)
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle step on the same {cl.n} stars")
@@ -221,6 +273,7 @@ def main():
         "phases": phases,
         "learned": learned,
         "autotune": tune,
+        "ensemble": ens,
         "cpu_baseline": cpu,
         "last_step_stats": st,
     }

@@ -190,6 +190,35 @@ __device__ __forceinline__ double function_q(double rj, double mj, double inv_ns
     return 2.0 * (E - (pot + phis)) - Jr;
 }
 
+// Sign of function_q(rj, mj, inv_nstar, r, pot, E, J) as that function computes it
+// in double: -1 (Q < 0), 0 (Q == 0), 1 (Q > 0), 2 (NaN).  A division-free value q~
+// (1/r from the hardware reciprocal refined by two Newton steps, accurate to 2^-45
+// for any seed accurate to 12 bits) differs from the computed Q by less than
+// (2 2^-45 + 10 2^-53) S, S = 2(|E| + |pot| + |phis|) + (J/r)^2, since each of the two
+// evaluations rounds at most six times; when |q~| exceeds 2^-42 S the signs agree,
+// otherwise function_q itself decides.  qa receives q~ (a guide for searches only).
+__device__ __forceinline__ int q_sign(double rj, double mj, double inv_nstar, double r, double pot, double E,
+                                      double J, double *qa = nullptr)
+{
+    double y = __builtin_amdgcn_rcp(r);
+    double e = __builtin_fma(-r, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-r, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double a = J * y, T = a * a;
+    const double ph = (r >= rj ? mj * y : mj / rj) * inv_nstar;
+    const double q = 2.0 * (E - (pot + ph)) - T;
+    if (qa) *qa = q;
+    const double S = 2.0 * (__builtin_fabs(E) + __builtin_fabs(pot) + __builtin_fabs(ph)) + T;
+    if (__builtin_fabs(q) > 0x1p-42 * S) return q < 0.0 ? -1 : 1;
+    const double qe = function_q(rj, mj, inv_nstar, r, pot, E, J);
+    return qe < 0.0 ? -1 : (qe > 0.0 ? 1 : (qe == 0.0 ? 0 : 2));
+}
+__device__ __forceinline__ bool q_lt0(int sg) { return sg == -1; }
+__device__ __forceinline__ bool q_gt0(int sg) { return sg == 1; }
+__device__ __forceinline__ bool q_ge0(int sg) { return sg == 0 || sg == 1; }
+__device__ __forceinline__ bool q_le0(int sg) { return sg == -1 || sg == 0; }
+
 // Access to the (r, phi) nodes: the two global arrays, or a packed copy of 16-byte
 // (r, phi) nodes, whose Q evaluations and interpolations touch one cache line per
 // node instead of two.  Both hold the same values.
@@ -264,26 +293,27 @@ template <class N>
 __device__ __forceinline__ int64_t find_zero_Q_n(const N &nd, double rj, double mj, double inv_nstar,
                                                  int64_t kmin, int64_t kmax, double E, double J)
 {
-    double a, b;
+    double a, b, a2, b2;
     nd.RP(kmin, a, b);
-    double qmin = function_q(rj, mj, inv_nstar, a, b, E, J);
-    nd.RP(kmax, a, b);
-    double qmax = function_q(rj, mj, inv_nstar, a, b, E, J);
+    nd.RP(kmax, a2, b2);
+    // the direction test qmin < qmax: decided by the signs when they differ
+    const int smin = q_sign(rj, mj, inv_nstar, a, b, E, J), smax = q_sign(rj, mj, inv_nstar, a2, b2, E, J);
+    bool up;
+    if ((smin == -1 || smin == 1) && (smax == -1 || smax == 1) && smin != smax) up = smin < smax;
+    else up = function_q(rj, mj, inv_nstar, a, b, E, J) < function_q(rj, mj, inv_nstar, a2, b2, E, J);
     int guard = 0;
-    if (qmin < qmax) {
+    if (up) {
         do {
             int64_t ktry = (kmin + kmax + 1) / 2;
             nd.RP(ktry, a, b);
-            double qtry = function_q(rj, mj, inv_nstar, a, b, E, J);
-            if (qtry < 0.e0) kmin = ktry;
+            if (q_lt0(q_sign(rj, mj, inv_nstar, a, b, E, J))) kmin = ktry;
             else kmax = ktry - 1;
         } while (kmax != kmin && ++guard < 96);
     } else {
         do {
             int64_t ktry = (kmin + kmax + 1) / 2;
             nd.RP(ktry, a, b);
-            double qtry = function_q(rj, mj, inv_nstar, a, b, E, J);
-            if (qtry > 0.e0) kmin = ktry;
+            if (q_gt0(q_sign(rj, mj, inv_nstar, a, b, E, J))) kmin = ktry;
             else kmax = ktry - 1;
         } while (kmax != kmin && ++guard < 96);
     }
@@ -559,9 +589,10 @@ __device__ __forceinline__ void mid_narrow_q(const Coarse &c, double rj, double 
     double k_lo = POSITIVE ? -q_lo : q_lo, k_hi = POSITIVE ? -q_hi : q_hi;
     if (!(k_lo < 0.0 && k_hi >= 0.0)) { k_lo = -1.0; k_hi = 1.0; }
     const int64_t t = mid_search(c.mid, t0, t1, k_lo, k_hi, [&](double2 e, double &k) {
-        const double q = function_q(rj, mj, inv, e.x, e.y, E, J);
+        double q;
+        const int sg = q_sign(rj, mj, inv, e.x, e.y, E, J, &q);
         k = POSITIVE ? -q : q;
-        return POSITIVE ? q > 0.0 : q < 0.0;
+        return POSITIVE ? q_gt0(sg) : q_lt0(sg);
     });
     if (t < t0) { hi = MID_STEP * t0; return; }
     lo = MID_STEP * t;
@@ -602,8 +633,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
         int64_t hi = coarse_index_estimate(*C, (double)(__expf(C->lrp) * ep)) + 2;
         lo = lo < 0 ? 0 : lo;
         hi = hi > ktemp ? ktemp : hi;
-        if (lo < hi && function_q(rj, mj, inv, nd.R(lo), nd.P(lo), E, J) < 0.0 &&
-            (hi == ktemp || function_q(rj, mj, inv, nd.R(hi), nd.P(hi), E, J) >= 0.0)) {
+        if (lo < hi && q_lt0(q_sign(rj, mj, inv, nd.R(lo), nd.P(lo), E, J)) &&
+            (hi == ktemp || q_ge0(q_sign(rj, mj, inv, nd.R(hi), nd.P(hi), E, J)))) {
             kp_lo = lo; kp_hi = hi; got_p = vp = true; o.hint_hits++;
         } else {
             o.hint_misses++;
@@ -612,8 +643,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
         hi = coarse_index_estimate(*C, (double)(__expf(C->lra) * ep)) + 2;
         lo = lo < ktemp ? ktemp : lo;
         hi = hi > n_max + 1 ? n_max + 1 : hi;
-        if (lo < hi && (lo == ktemp || function_q(rj, mj, inv, nd.R(lo), nd.P(lo), E, J) > 0.0) &&
-            !(function_q(rj, mj, inv, nd.R(hi), nd.P(hi), E, J) > 0.0)) {
+        if (lo < hi && (lo == ktemp || q_gt0(q_sign(rj, mj, inv, nd.R(lo), nd.P(lo), E, J))) &&
+            !q_gt0(q_sign(rj, mj, inv, nd.R(hi), nd.P(hi), E, J))) {
             ka_lo = lo; ka_hi = hi; got_a = va = true; o.hint_hits++;
         } else {
             o.hint_misses++;
@@ -621,36 +652,36 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
     }
     if (C) {
         const double inv = P.inv_nstar;
-        if (!got_p && ktemp >= 1 && function_q(rj, mj, inv, C->cr[0], C->cphi[0], E, J) < 0.0) {
+        if (!got_p && ktemp >= 1 && q_lt0(q_sign(rj, mj, inv, C->cr[0], C->cphi[0], E, J))) {
             int a = 0, b = (int)((ktemp - 1) / C->stride);
             const int top = b;
             while (a < b) {
                 int m = (a + b + 1) >> 1;
-                if (function_q(rj, mj, inv, C->cr[m], C->cphi[m], E, J) < 0.0) a = m;
+                if (q_lt0(q_sign(rj, mj, inv, C->cr[m], C->cphi[m], E, J))) a = m;
                 else b = m - 1;
             }
             kp_lo = coarse_k(*C, a);
             kp_hi = a < top ? coarse_k(*C, a + 1) : ktemp;
             vp = true;
-            qp_lo = function_q(rj, mj, inv, C->cr[a], C->cphi[a], E, J);
-            if (a < top) qp_hi = function_q(rj, mj, inv, C->cr[a + 1], C->cphi[a + 1], E, J);
+            (void)q_sign(rj, mj, inv, C->cr[a], C->cphi[a], E, J, &qp_lo);
+            if (a < top) (void)q_sign(rj, mj, inv, C->cr[a + 1], C->cphi[a + 1], E, J, &qp_hi);
         }
         const int sb = (int)(ktemp / C->stride) + 1;
-        if (!got_a && sb <= C->cs && !(function_q(rj, mj, inv, C->cr[C->cs], C->cphi[C->cs], E, J) > 0.0)) {
+        if (!got_a && sb <= C->cs && !q_gt0(q_sign(rj, mj, inv, C->cr[C->cs], C->cphi[C->cs], E, J))) {
             va = true;
-            if (!(function_q(rj, mj, inv, C->cr[sb], C->cphi[sb], E, J) > 0.0)) {
+            if (!q_gt0(q_sign(rj, mj, inv, C->cr[sb], C->cphi[sb], E, J))) {
                 ka_hi = coarse_k(*C, sb);
             } else {
                 int a = sb, b = C->cs - 1;
                 while (a < b) {
                     int m = (a + b + 1) >> 1;
-                    if (function_q(rj, mj, inv, C->cr[m], C->cphi[m], E, J) > 0.0) a = m;
+                    if (q_gt0(q_sign(rj, mj, inv, C->cr[m], C->cphi[m], E, J))) a = m;
                     else b = m - 1;
                 }
                 ka_lo = coarse_k(*C, a);
                 ka_hi = coarse_k(*C, a + 1);
-                qa_lo = function_q(rj, mj, inv, C->cr[a], C->cphi[a], E, J);
-                qa_hi = function_q(rj, mj, inv, C->cr[a + 1], C->cphi[a + 1], E, J);
+                (void)q_sign(rj, mj, inv, C->cr[a], C->cphi[a], E, J, &qa_lo);
+                (void)q_sign(rj, mj, inv, C->cr[a + 1], C->cphi[a + 1], E, J, &qa_hi);
             }
         }
     }

@@ -396,8 +396,7 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
             f.rnew[j] = CMCG_SF_INFINITY; f.vrnew[j] = 0.0; f.vtnew[j] = 0.0;
             flags = CMCG_SF_REMOVED_E;
         } else {
-            double Qown = function_q(rj, mj, P.inv_nstar, rj, f.phi[j], E, J);
-            if (Qown <= 0.0) {
+            if (q_le0(q_sign(rj, mj, P.inv_nstar, rj, f.phi[j], E, J))) {
                 // linear-scan case: finished by the deferred path
                 flags = CMCG_SF_LINSCAN;
                 unsigned long long slot = atomicAdd((unsigned long long *)&s->n_defer, 1ull);

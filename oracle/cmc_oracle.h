@@ -105,6 +105,7 @@ enum { ORC_E_NAN = -1, ORC_E_NTRY = -2, ORC_E_BRACKET = -3, ORC_E_WZERO = -4, OR
 
 /* double-double sin/cos shared (operation for operation) with the device relaxation */
 void orc_dd_sincos(double x, double *s, double *c);
+void orc_dd_trig_table(int j, double *out4);   /* sin/cos of j pi/128 (hi, lo), j = 0..32 */
 
 /* ---------------------------------------------------------------- kernels of the path */
 double  orc_sqr(double x);                                     /* cmc_utils.c:27-30 */

@@ -84,6 +84,7 @@ _sigs = {
     "orc_calc_timestep": (C.c_double, [_SP]),
     "orc_init": (None, [_SP]),
     "orc_dd_sincos": (None, [C.c_double, DPTR, DPTR]),
+    "orc_dd_trig_table": (None, [C.c_int, DPTR]),
     "orc_step": (None, [_SP]),
 }
 for _n, (_r, _a) in _sigs.items():
@@ -187,6 +188,13 @@ def dd_sincos(x):
     s, c = C.c_double(), C.c_double()
     lib.orc_dd_sincos(float(x), C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def dd_trig_table(j):
+    """(sin hi, sin lo, cos hi, cos lo) of j pi/128 from the double-double sin/cos table."""
+    out = (C.c_double * 4)()
+    lib.orc_dd_trig_table(int(j), out)
+    return tuple(out)
 
 
 def ref_taus_lib():

@@ -133,6 +133,7 @@ _sigs = {
     "cmcg_step": [_ctx, _P(OrbitStats)],
     "cmcg_step_graph": [_ctx, _P(OrbitStats)],
     "cmcg_autotune": [_ctx, C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_double), _P(C.c_double)],
+    "cmcg_test_hook": [_ctx, C.c_int32, C.c_int64],
     "cmcg_orbit_variants": [_ctx, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32),
                             _P(C.c_double)],
     "cmcg_ensemble_step": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
@@ -152,6 +153,7 @@ _sigs = {
     "cmcg_set_learned_brackets": [_ctx, C.c_int, C.c_double],
 }
 MLP_NPARAM, MLP_NNORM = 4738, 16
+HOOK_CHAIN_K_DROP = 1
 for _name, _args in _sigs.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args
@@ -418,6 +420,9 @@ class Context:
                "ms_default": default.value}
         out["variants"] = self.orbit_variants()
         return out
+
+    def test_hook(self, hook, value):
+        _chk(lib.cmcg_test_hook(self.h, int(hook), int(value)), "cmcg_test_hook")
 
     def orbit_variants(self):
         """Launch variants of the orbit sampler with their last autotune times."""

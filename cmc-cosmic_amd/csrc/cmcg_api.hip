@@ -32,7 +32,6 @@ struct cmcg_ctx {
     std::vector<hipEvent_t> event_pool;
     // host-side copy of the seeded RNG state
     Rng seeded{};
-    int64_t last_sort_ties = 0;
     // learned brackets
     MlpState mlp{};
     bool mlp_ready = false, harvest = false, learned = false;
@@ -46,6 +45,7 @@ struct cmcg_ctx {
     int parity = 0;
     int orb_variant = 0;         // launch shape of the orbit sampler (autotuned)
     std::vector<double> tune_ms; // per-variant time of the last cmcg_autotune
+    void *mpc_buf = nullptr;     // cmcg_comp_mass_percent results (allocated on first use)
 };
 
 static void invalidate_graphs(cmcg_ctx *c)
@@ -404,6 +404,7 @@ int cmcg_destroy(cmcg_ctx *c)
     for (auto e : c->event_pool) hipEventDestroy(e);
     for (void *p : c->allocs) hipFree(p);
     if (c->io) hipFree(c->io);
+    if (c->mpc_buf) hipFree(c->mpc_buf);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return CMCG_OK;
@@ -636,7 +637,6 @@ int cmcg_sort_by_r(cmcg_ctx *c, int64_t *n_max)
     DevScalars h;
     if (int e = read_scalars(c, h)) return e;
     c->n_ub = h.n_max;
-    c->last_sort_ties = h.ntie_runs;
     if (n_max) *n_max = h.n_max;
     return h.error;
 }
@@ -1272,6 +1272,58 @@ int cmcg_autotune(cmcg_ctx *c, int reps, int32_t *variant, int32_t *threads, int
     if (ms_best) *ms_best = best_ms;
     if (ms_default) *ms_default = def_ms;
     return CMCG_OK;
+}
+
+int cmcg_comp_mass_percent(cmcg_ctx *c, const double *mass_pc, int32_t count, int32_t bh_include,
+                           double initial_total_mass, double minimum_r, cmcg_mass_pc *out)
+{
+    if (!c || !out || count < 0 || count > 256 || (count > 0 && !mass_pc)) return CMCG_E_ARG;
+    for (int i = 1; i < count; i++)
+        if (!(mass_pc[i] >= mass_pc[i - 1])) return CMCG_E_ARG;
+    if (!c->mpc_buf) {
+        if (hipMalloc(&c->mpc_buf, mass_percent_out_bytes() + 256 * sizeof(double)) != hipSuccess) {
+            c->mpc_buf = nullptr;
+            return CMCG_E_NOMEM;
+        }
+    }
+    char *dev = (char *)c->mpc_buf;
+    double *dpc = (double *)(dev + mass_percent_out_bytes());
+    if (count > 0 && hipMemcpyAsync(dpc, mass_pc, count * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return CMCG_E_HIP;
+    if (hipMemsetAsync(dev, 0, mass_percent_out_bytes(), c->stream) != hipSuccess) return CMCG_E_HIP;
+    LAUNCH(launch_mass_percent(c->f, c->s, c->kp, c->n_ub, c->sc, dpc, count, bh_include, initial_total_mass,
+                               minimum_r, dev, c->stream));
+    std::vector<char> host(mass_percent_out_bytes());
+    if (hipMemcpyAsync(host.data(), dev, host.size(), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        sync(c) != hipSuccess)
+        return CMCG_E_HIP;
+    // layout: double v[8][256], int64 k[256], int64 reached
+    const double *v = (const double *)host.data();
+    const int64_t *k = (const int64_t *)(host.data() + 8 * 256 * sizeof(double));
+    const int64_t reached = k[256];
+    double *dst[7] = {out->mass_r, out->ave_mass_r, out->densities_r, out->ke_rad_r, out->ke_tan_r, out->v2_rad_r,
+                      out->v2_tan_r};
+    for (int64_t i = 0; i < reached && i < count; i++) {
+        for (int q = 0; q < 7; q++)
+            if (dst[q]) dst[q][i] = v[q * 256 + i];
+        if (out->no_star_r) out->no_star_r[i] = k[i];
+    }
+    out->reached = (int32_t)reached;
+    return check_error(c);
+}
+
+int cmcg_test_hook(cmcg_ctx *c, int32_t hook, int64_t value)
+{
+    if (!c) return CMCG_E_ARG;
+    switch (hook) {
+    case CMCG_HOOK_CHAIN_K_DROP:
+        if (value < 0 || value > 92) return CMCG_E_ARG;
+        c->kp.chain_k_drop = (int32_t)value;
+        invalidate_graphs(c);
+        return CMCG_OK;
+    default:
+        return CMCG_E_ARG;
+    }
 }
 
 int cmcg_orbit_variants(cmcg_ctx *c, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse, int32_t *cap,

@@ -33,7 +33,7 @@ struct DevScalars {
     double DTidalMassLoss;
     int32_t error; int32_t pad0;
     int64_t error_index;
-    int64_t ntie_runs;
+    int64_t reserved0;
     int64_t n_excess;            // set_velocities3 stars with vnew2 <= 0
     int64_t first_inf;           // potential_calculate: first slot with r >= SF_INFINITY
     int64_t n_defer;             // orbit phase: stars whose own-index Q <= 0 (linear-scan path)
@@ -70,6 +70,8 @@ struct KParams {
     double theta_se_max;
     double gamma;
     int64_t avekernel;
+    int32_t chain_k_drop;  // test hook: coarser fixed-point scale in the potential chains
+    int32_t pad0;
 };
 
 __device__ __forceinline__ void dev_set_error(DevScalars *s, int code, int64_t idx)

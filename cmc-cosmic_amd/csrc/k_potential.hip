@@ -712,6 +712,109 @@ hipError_t run_chain(ChainArgs a, EsaArgs e, int rounds, int64_t n_ub, hipStream
 }
 } // namespace
 
+// ---------------------------------------------------------------- comp_mass_percent
+// cmc_utils.c:872-982 for one rank.  Chain M: mprev_k = mprev_{k-1} + m_k / N_STAR
+// (k = 1..N_MAX, from cenma.m madhoc or 0); chains T_r, T_t, V_r, V_t: the running
+// sums of 0.5 m madhoc vr vr, 0.5 m madhoc vt vt, vr vr, vt vt (k = 1..N_MAX_NEW).
+// Each is the reference's sequential recurrence, solved by the same exact emulation
+// as the potential's chains.
+constexpr int MPC_MAX = 256;
+struct MassPcOut {
+    double v[8][MPC_MAX];        // mass_r, ave_mass_r, densities_r, ke_rad, ke_tan, v2_rad, v2_tan, (unused)
+    int64_t k[MPC_MAX];          // no_star_r (the index k)
+    int64_t reached;             // fractions assigned
+};
+
+__global__ void mpc_prep_m_kernel(const double *__restrict__ m, const DevScalars *s, double nstar,
+                                  double *__restrict__ x, int64_t *__restrict__ len)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s->n_max;
+    if (j == 0) len[3] = n;
+    if (j < n) x[j] = m[j + 1] / nstar;                   // star_m[k] / clus.N_STAR, k = j + 1
+}
+
+// term 0: 0.5 m madhoc vr vr, 1: 0.5 m madhoc vt vt, 2: vr vr, 3: vt vt (left to right)
+__global__ void mpc_prep_v_kernel(const DevFields f, const DevScalars *s, double madhoc, int term,
+                                  double *__restrict__ x, int64_t *__restrict__ len)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s->n_max_new;
+    if (j == 0) len[3] = n;
+    if (j >= n) return;
+    const int64_t k = j + 1;
+    double v;
+    switch (term) {
+    case 0: v = 0.5 * f.m[k] * madhoc * f.vr[k] * f.vr[k]; break;
+    case 1: v = 0.5 * f.m[k] * madhoc * f.vt[k] * f.vt[k]; break;
+    case 2: v = f.vr[k] * f.vr[k]; break;
+    default: v = f.vt[k] * f.vt[k]; break;
+    }
+    x[j] = v;
+}
+
+// x^3 from x*x exactly as a double-double, times x: within 2^-50 ulp of the correctly
+// rounded cube (pow(x, 3) of a correctly rounding libm)
+__device__ __forceinline__ double cube_rn(double x)
+{
+    const double p = x * x, e = __builtin_fma(x, x, -p);
+    const double q = p * x, f1 = __builtin_fma(p, x, -q);
+    return q + (f1 + e * x);
+}
+
+// the fractions' indices (one fraction per star at most, as the reference's loop),
+// from the enclosed-mass chain g[k-1] = mprev after star k
+__global__ void mpc_assign_kernel(const double *__restrict__ r, const DevScalars *s, const double *__restrict__ g,
+                                  const double *__restrict__ pc, int count, int bh_include, double mprev0,
+                                  double itm, double minimum_r, MassPcOut *o)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double PI = 3.14159265358979323;                  // include/cmc/cmc.h:62
+    const double Mtotal = s->Mtotal;
+    const int64_t n = s->n_max;
+    int mc = 0;
+    if (bh_include)
+        for (; mc < count && mprev0 / Mtotal > pc[mc]; mc++) {
+            o->v[0][mc] = minimum_r;
+            for (int q = 1; q < 7; q++) o->v[q][mc] = 0.0;
+            o->k[mc] = 0;
+        }
+    int64_t kprev = 0;
+    for (; mc < count; mc++) {
+        const double thr = pc[mc] * Mtotal;
+        // smallest k in (kprev, n] with g[k-1] > thr; g is non-decreasing
+        int64_t lo = kprev + 1, hi = n + 1;
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (g[mid - 1] > thr) hi = mid;
+            else lo = mid + 1;
+        }
+        if (lo > n) break;
+        const int64_t k = lo;
+        const double mprev = g[k - 1];
+        o->v[0][mc] = r[k];
+        o->v[1][mc] = mprev / Mtotal / k * itm;
+        o->v[2][mc] = mprev / (4.0 / 3.0 * PI * cube_rn(r[k]));
+        o->k[mc] = k;
+        kprev = k;
+    }
+    o->reached = mc;
+}
+
+__global__ void mpc_init_kernel(double *v0, double mprev0)
+{
+    v0[0] = mprev0;
+    v0[1] = 0.0;
+}
+
+__global__ void mpc_take_kernel(const double *__restrict__ g, MassPcOut *o, int term, int count)
+{
+    const int i = threadIdx.x;
+    if (i >= count || i >= o->reached) return;
+    const int64_t k = o->k[i];
+    o->v[3 + term][i] = k > 0 ? g[k - 1] : 0.0;
+}
+
 hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
                             const Scratch &sc, hipStream_t st)
 {
@@ -748,25 +851,29 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     if (n_ub > 0) hipLaunchKernelGGL(pot_count_kernel, dim3(g1), dim3(256), 0, st, f.r, s);
     hipLaunchKernelGGL(pot_setup_kernel, dim3(1), dim3(1), 0, st, s, w);
     hipLaunchKernelGGL(sentinel_kernel, dim3(1), dim3(1), 0, st, f.r, f.phi, s);
-    // chain A: total mass (stored masses are O(1): 2^-70 resolution)
+    // chain A: total mass
     hipLaunchKernelGGL(prep_a_kernel, dim3(g1), dim3(256), 0, st, f.m, s, w.xa);
-    ChainArgs a{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 0, w.tsum, w.tpre, w.bnd, w.len + 0, 70};
+    // fixed-point scales 2^-70 (stored masses are O(1)) and 2^-100; a test hook may
+    // coarsen them, which sends most elements through the fallback paths
+    const int KA = 70 - P.chain_k_drop < 8 ? 8 : 70 - P.chain_k_drop;
+    const int KB = 100 - P.chain_k_drop < 8 ? 8 : 100 - P.chain_k_drop;
+    ChainArgs a{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 0, w.tsum, w.tpre, w.bnd, w.len + 0, KA};
     EsaArgs ae = e;
-    ae.x = w.xa; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0; ae.K = 70;
+    ae.x = w.xa; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0; ae.K = KA;
     if (hipError_t err = run_chain(a, ae, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(total_kernel, dim3(1), dim3(1), 0, st, s, P, (const double *)w.ga, w);
     // chain B: enclosed mass
     hipLaunchKernelGGL(prep_b_kernel, dim3(g1), dim3(256), 0, st, f.m, s, (double)P.n_star, w.xa);
-    ChainArgs b{w.xa, w.ga, -1.0, w.scal + 1, w.fbad + 2, w.tsum, w.tpre, w.bnd, w.len + 1, 100};
+    ChainArgs b{w.xa, w.ga, -1.0, w.scal + 1, w.fbad + 2, w.tsum, w.tpre, w.bnd, w.len + 1, KB};
     EsaArgs be = e;
-    be.x = w.xa; be.g = w.ga; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1; be.K = 100;
+    be.x = w.xa; be.g = w.ga; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1; be.K = KB;
     if (hipError_t err = run_chain(b, be, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(save_mk_kernel, dim3(g1), dim3(256), 0, st, s, (const double *)w.ga, w.Mk);
     // chain C: w = -phi
     hipLaunchKernelGGL(prep_c_kernel, dim3(g1), dim3(256), 0, st, f.r, s, (const double *)w.Mk, w.xa);
-    ChainArgs c{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 4, w.tsum, w.tpre, w.bnd, w.len + 2, 100};
+    ChainArgs c{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 4, w.tsum, w.tpre, w.bnd, w.len + 2, KB};
     EsaArgs ce = e;
-    ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2; ce.K = 100;
+    ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2; ce.K = KB;
     if (hipError_t err = run_chain(c, ce, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(phi_out_kernel, dim3(g1), dim3(256), 0, st, f.r, f.phi, s, P, (const double *)w.ga);
     hipLaunchKernelGGL(phi0_empty_kernel, dim3(1), dim3(1), 0, st, f.r, f.phi, s, P);
@@ -782,3 +889,58 @@ hipError_t launch_potential_at(const DevFields &f, DevScalars *s, const KParams 
                        out, count);
     return hipGetLastError();
 }
+
+// comp_mass_percent: pc (device, count <= MPC_MAX, ascending), results into out
+hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, const Scratch &sc,
+                               const double *pc, int count, int bh_include, double itm, double minimum_r,
+                               void *out, hipStream_t st)
+{
+    if (count < 0 || count > MPC_MAX) return hipErrorInvalidValue;
+    int64_t ntiles = (n_ub + CTILE - 1) / CTILE + 1;
+    if (ntiles > sc.max_tiles) return hipErrorInvalidValue;
+    i128 *tb = (i128 *)sc.tile_sums;
+    const int64_t T = sc.max_tiles;
+    EsaArgs e{};
+    e.tsumP = tb + 2 * T; e.tpreP = tb + 3 * T; e.tsumR = tb + 4 * T; e.tpreR = tb + 5 * T;
+    e.tcnt = tb + 6 * T; e.toff = tb + 7 * T;
+    e.tsum = tb;
+    e.s_idx = sc.esa_idx;
+    e.s_racc = (i128 *)sc.esa_racc;
+    e.s_v = sc.esa_v;
+    e.s_n = sc.esa_n;
+    e.smax = sc.esa_max;
+    e.raw_n = sc.esa_n + 1;
+    e.raw_idx = sc.esa_raw_idx;
+    e.raw_part = (i128 *)sc.esa_raw_part;
+    e.x = sc.chain_x; e.g = sc.chain_g; e.sign = 1.0; e.first_bad = sc.chain_fbad + 6;
+    e.Lp = sc.chain_len + 3;
+    double *v0 = sc.chain_scal + 2;              // start values: [2] mprev0, [3] 0
+    const double mprev0 = bh_include ? P.cenma_m * P.madhoc : 0.0;   // cmc_utils.c:878
+    hipLaunchKernelGGL(mpc_init_kernel, dim3(1), dim3(1), 0, st, v0, mprev0);
+    MassPcOut *o = (MassPcOut *)out;
+    unsigned g1 = (unsigned)((n_ub + 256) / 256);
+    const int KD = P.chain_k_drop;
+    auto Kof = [&](int k) { return k - KD < 8 ? 8 : k - KD; };
+    // chain M
+    hipLaunchKernelGGL(mpc_prep_m_kernel, dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
+                       sc.chain_x, sc.chain_len);
+    EsaArgs em = e;
+    em.v0p = v0; em.K = Kof(100);
+    ChainArgs cm{sc.chain_x, sc.chain_g, 1.0, v0, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, em.K};
+    if (hipError_t err = run_chain(cm, em, REPAIR_ROUNDS, n_ub, st)) return err;
+    hipLaunchKernelGGL(mpc_assign_kernel, dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
+                       (const double *)sc.chain_g, pc, count, bh_include, mprev0, itm, minimum_r, o);
+    // the four running sums
+    for (int term = 0; term < 4; term++) {
+        hipLaunchKernelGGL(mpc_prep_v_kernel, dim3(g1), dim3(256), 0, st, f, (const DevScalars *)s, P.madhoc, term,
+                           sc.chain_x, sc.chain_len);
+        EsaArgs ev = e;
+        ev.v0p = v0 + 1; ev.K = Kof(term < 2 ? 100 : 80);
+        ChainArgs cv{sc.chain_x, sc.chain_g, 1.0, v0 + 1, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, ev.K};
+        if (hipError_t err = run_chain(cv, ev, REPAIR_ROUNDS, n_ub, st)) return err;
+        hipLaunchKernelGGL(mpc_take_kernel, dim3(1), dim3(MPC_MAX), 0, st, (const double *)sc.chain_g, o, term, count);
+    }
+    return hipGetLastError();
+}
+
+size_t mass_percent_out_bytes() { return sizeof(MassPcOut); }

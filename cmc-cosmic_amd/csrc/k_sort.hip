@@ -24,7 +24,7 @@ constexpr int RS_PASSES = 5;
 constexpr int RS_SHIFT0 = 64 - 8 * RS_PASSES;
 constexpr int RUN_SHORT = 64;
 constexpr int RUN_LONG = 4096;
-constexpr int AOS_W = SORT_AOS_WORDS;   // words per packed payload record (96 B)
+constexpr int AOS_W = SORT_AOS_WORDS;   // words per packed payload record (one 128-B line)
 constexpr int AOS_LD = AOS_W + 1;        // odd LDS stride per record: no bank conflicts
 
 __device__ __forceinline__ uint64_t sortable(double x)
@@ -59,10 +59,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
                                                              uint64_t *__restrict__ aos, GatherList g, uint64_t kinf)
 {
     constexpr int P0 = 8 - RS_PASSES;      // first byte sorted on
-    __shared__ uint32_t h[8][256];
+    __shared__ uint32_t h[RS_PASSES][256]; // 40 KB of LDS in all: four blocks per CU
     __shared__ uint64_t recs[RS_THREADS * AOS_LD];     // one chunk of packed records
     const int t = threadIdx.x;
-    for (int p = 0; p < 8; p++) h[p][t] = 0;
+    for (int p = 0; p < RS_PASSES; p++) h[p][t] = 0;
     __syncthreads();
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
             if (k >= kinf) k = ~0ull;      // the SF_INFINITY tail: one key, no prefix shared with a live star
             keys[i] = k;
             vals[i] = (uint32_t)(i + 1);
-            for (int p = P0; p < 8; p++) atomicAdd(&h[p][(k >> (8 * p)) & 255], 1u);
+            for (int p = P0; p < 8; p++) atomicAdd(&h[p - P0][(k >> (8 * p)) & 255], 1u);
         }
         if (aos && i0 < n) {        // block-uniform: pack the chunk through LDS, write it contiguously
             if (i < n)
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
     }
     __syncthreads();
     for (int p = P0; p < 8; p++)
-        if (h[p][t]) atomicAdd(&digit_totals[p * 256 + t], h[p][t]);
+        if (h[p - P0][t]) atomicAdd(&digit_totals[p * 256 + t], h[p - P0][t]);
 }
 
 __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint64_t *__restrict__ keys, DevScalars *s,
@@ -287,7 +287,6 @@ __global__ void rs_runs_kernel(uint64_t *__restrict__ keys, uint32_t *__restrict
         keys[b + 1] = ka;
         vals[b + 1] = v;
     }
-    atomicAdd((unsigned long long *)&s->ntie_runs, 1ull);
 }
 
 // Long runs: one block per run (grid-stride over the list), bitonic sort by (key, id)
@@ -429,7 +428,6 @@ __global__ void rs_finish_kernel(DevScalars *s)
 __global__ void rs_begin_kernel(DevScalars *s, int64_t *n_long)
 {
     s->sort_n = 0;
-    s->ntie_runs = 0;
     s->sort_in = s->n_max_new;
     *n_long = 0;
 }

@@ -10,8 +10,8 @@
 // orbit sampler: most LDS samples of any launch variant; lane jumps l 2^20 by nibble
 // tables of the powers 2^ORB_NIB_B0 .. 2^(ORB_NIB_B0 + ORB_NIB_POWERS - 1)
 constexpr int ORB_MAX_COARSE = 2048;
-// sort payload records: 12 words (96 B) per star
-constexpr int SORT_AOS_WORDS = 12;
+// sort payload records: 16 words (128 B, one cache line) per star
+constexpr int SORT_AOS_WORDS = 16;
 constexpr int SORT_LONG_RUNS = 4096;
 constexpr int RED2_BLOCKS = 64;
 constexpr int ORB_NIB_B0 = 20, ORB_NIB_POWERS = 6;
@@ -75,6 +75,10 @@ struct Scratch {
 
 hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
                             const Scratch &sc, hipStream_t st);
+hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, const Scratch &sc,
+                               const double *pc, int count, int bh_include, double itm, double minimum_r,
+                               void *out, hipStream_t st);
+size_t mass_percent_out_bytes();
 hipError_t launch_potential_at(const DevFields &f, DevScalars *s, const KParams &P,
                                const double *x, double *out, int64_t count, hipStream_t st);
 

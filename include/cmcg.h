@@ -222,6 +222,31 @@ int cmcg_autotune(cmcg_ctx *ctx, int reps, int32_t *variant, int32_t *threads, i
  * star's rejection tries get per round) and their times in the last cmcg_autotune
  * (0 before one ran); fills at most max entries, *count = number of variants.
  * No counterpart in the reference. */
+/* comp_mass_percent (src/cmc/cmc_utils.c:872-982) on one device: for the mass
+ * fractions mass_pc[0..count-1] (ascending, count <= 256) the Lagrange radius, mean
+ * mass, star count, density and the kinetic-energy and v^2 sums inside it, with
+ * the reference's sequential sums and its rule of at most one fraction per star.
+ * Host arrays of count entries (any may be NULL); entries the reference's loop would
+ * not reach are left unchanged, and `reached` tells how many were written.
+ * bh_include is MASS_PC_BH_INCLUDE, minimum_r is MINIMUM_R (both only matter with a
+ * central mass), initial_total_mass scales ave_mass_r.  densities_r uses x^3 rounded
+ * once where the reference calls pow(r, 3). */
+typedef struct cmcg_mass_pc {
+    double *mass_r, *ave_mass_r, *densities_r, *ke_rad_r, *ke_tan_r, *v2_rad_r, *v2_tan_r;
+    int64_t *no_star_r;
+    int32_t reached, pad;
+} cmcg_mass_pc;
+int cmcg_comp_mass_percent(cmcg_ctx *ctx, const double *mass_pc, int32_t count, int32_t bh_include,
+                           double initial_total_mass, double minimum_r, cmcg_mass_pc *out);
+
+/* Test hooks (no counterpart in the reference; for tests that force rare paths).
+ * CMCG_HOOK_CHAIN_K_DROP: coarsen the fixed-point scale of the potential chains by
+ * `value` bits (0..92; 0 = normal).  Nearly every element then takes the fallback
+ * paths (list overflow, repair round, sequential completion); the potential must
+ * still equal the reference's recurrence bit for bit, only slower. */
+#define CMCG_HOOK_CHAIN_K_DROP 1
+int cmcg_test_hook(cmcg_ctx *ctx, int32_t hook, int64_t value);
+
 int cmcg_orbit_variants(cmcg_ctx *ctx, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse,
                         int32_t *cap, double *ms_last);
 

@@ -103,6 +103,17 @@ enum {
 enum { ORC_E_NAN = -1, ORC_E_NTRY = -2, ORC_E_BRACKET = -3, ORC_E_WZERO = -4, ORC_E_WPZERO = -5,
        ORC_E_BRENT = -6 };
 
+/* comp_mass_percent (cmc_utils.c:872-982) for one rank: Lagrange radii and the
+ * kinetic-energy and v^2 sums inside them, for the mass fractions pc[0..count-1]
+ * (ascending).  Outputs per fraction; entries the loop does not reach keep the
+ * values the caller put there. */
+typedef struct {
+    double *mass_r, *ave_mass_r, *densities_r, *ke_rad_r, *ke_tan_r, *v2_rad_r, *v2_tan_r;
+    int64_t *no_star_r;
+} orc_mass_pc_out;
+void orc_comp_mass_percent(const orc_state *s, const double *pc, int count, int bh_include,
+                           double initial_total_mass, double minimum_r, orc_mass_pc_out *o);
+
 /* double-double sin/cos shared (operation for operation) with the device relaxation */
 void orc_dd_sincos(double x, double *s, double *c);
 void orc_dd_trig_table(int j, double *out4);   /* sin/cos of j pi/128 (hi, lo), j = 0..32 */

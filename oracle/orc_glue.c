@@ -14,6 +14,7 @@
 #include <math.h>
 #include <float.h>
 #include <string.h>
+#include <stdlib.h>
 
 static inline double phi_s(const orc_state *s, double rad, int64_t j)
 {
@@ -200,4 +201,61 @@ void orc_step(orc_state *s)
     s->Etidal += Etidal_old;
     orc_compute_energy(s);
     s->step++;
+}
+
+/* cmc_utils.c:872-982 comp_mass_percent with one rank (get_global_idx(k) = k, the
+ * MPI_Exscan prefixes are zero): running sums in the reference's order. */
+void orc_comp_mass_percent(const orc_state *s, const double *pc, int count, int bh_include,
+                           double initial_total_mass, double minimum_r, orc_mass_pc_out *o)
+{
+    const double PI = 3.14159265358979323;             /* include/cmc/cmc.h:62 */
+    double mprev;
+    int mcount = 0;
+    if (bh_include) {
+        mprev = s->cenma_m * s->madhoc;
+        for (mcount = 0; mcount < count; mcount++) {
+            if (mprev / s->Mtotal > pc[mcount]) {          /* cmc_utils.c:879-890 */
+                o->mass_r[mcount] = minimum_r;
+                o->ave_mass_r[mcount] = 0.0;
+                o->no_star_r[mcount] = 0;
+                o->densities_r[mcount] = 0.0;
+                o->ke_rad_r[mcount] = 0.0;
+                o->ke_tan_r[mcount] = 0.0;
+                o->v2_rad_r[mcount] = 0.0;
+                o->v2_tan_r[mcount] = 0.0;
+            } else {
+                break;
+            }
+        }
+    } else {
+        mprev = 0.0;
+        mcount = 0;
+    }
+    const int64_t nn = s->N_MAX_NEW;
+    double *kr = calloc((size_t)nn + 1, sizeof(double)), *kt = calloc((size_t)nn + 1, sizeof(double));
+    double *vr2 = calloc((size_t)nn + 1, sizeof(double)), *vt2 = calloc((size_t)nn + 1, sizeof(double));
+    for (int64_t k = 1; k <= nn; k++) {                     /* cmc_utils.c:906-911 */
+        kr[k] = kr[k - 1] + 0.5 * s->m[k] * s->madhoc * s->vr[k] * s->vr[k];
+        kt[k] = kt[k - 1] + 0.5 * s->m[k] * s->madhoc * s->vt[k] * s->vt[k];
+        vr2[k] = vr2[k - 1] + s->vr[k] * s->vr[k];
+        vt2[k] = vt2[k - 1] + s->vt[k] * s->vt[k];
+    }
+    if (mcount < count) {
+        for (int64_t k = 1; k <= s->N_MAX; k++) {          /* cmc_utils.c:931-971 */
+            mprev += s->m[k] / s->N_STAR;
+            if (mprev > pc[mcount] * s->Mtotal) {
+                o->mass_r[mcount] = s->r[k];
+                o->ave_mass_r[mcount] = mprev / s->Mtotal / k * initial_total_mass;
+                o->no_star_r[mcount] = k;
+                o->densities_r[mcount] = mprev / (4.0 / 3.0 * PI * pow(s->r[k], 3));
+                o->ke_rad_r[mcount] = kr[k];
+                o->ke_tan_r[mcount] = kt[k];
+                o->v2_rad_r[mcount] = vr2[k];
+                o->v2_tan_r[mcount] = vt2[k];
+                mcount++;
+                if (mcount == count) break;
+            }
+        }
+    }
+    free(kr); free(kt); free(vr2); free(vt2);
 }

@@ -107,6 +107,34 @@ def test_gpu_potential_parity(gpu, O, case):
     assert d["r"][n + 1] == 1e10 and d["phi"][n + 1] == 0.0
 
 
+@pytest.mark.parametrize("drop", [30, 60])
+def test_gpu_potential_fallback_paths(gpu, O, drop):
+    """A test hook coarsens the chains' fixed-point scale by `drop` bits, so that most
+    elements are classified sensitive: the raw list overflows (ordered compaction),
+    at drop 60 the compacted list overflows too, the verification fails and the
+    repair round and the sequential completion finish the chains.  The potential
+    must still be the reference's recurrence bit for bit."""
+    cm = gpu
+    say("potential fallback paths, scale drop", drop)
+    cl = cm.make_plummer(100000, seed=3, kroupa=True)
+    o = oracle_for(O, cl)
+    n = o.potential_calculate()
+    with ctx_for(cm, cl) as ctx:
+        ctx.upload(cl)
+        ctx.test_hook(cm.HOOK_CHAIN_K_DROP, drop)
+        ng, mt, rt = ctx.potential()
+        d = ctx.download()
+        ctx.test_hook(cm.HOOK_CHAIN_K_DROP, 0)
+        ng2, mt2, _ = ctx.potential()
+        d2 = ctx.download()
+    assert ng == n == ng2
+    assert mt == o.sc("Mtotal") == mt2
+    neq = int(np.sum(d["phi"][: n + 2] != o["phi"][: n + 2]))
+    say("drop", drop, "unequal", neq)
+    assert neq == 0
+    assert np.array_equal(d2["phi"][: n + 2], d["phi"][: n + 2])
+
+
 def test_gpu_potential_tail_and_count(gpu, O):
     cm = gpu
     say("potential with stripped tail")

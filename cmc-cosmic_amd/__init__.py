@@ -96,6 +96,14 @@ class OrbitStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class MassPc(C.Structure):
+    _fields_ = [(k, C.POINTER(C.c_double)) for k in ("mass_r", "ave_mass_r", "densities_r", "ke_rad_r", "ke_tan_r",
+                                                    "v2_rad_r", "v2_tan_r")] + [
+        ("no_star_r", C.POINTER(C.c_int64)), ("reached", C.c_int32), ("pad", C.c_int32)]
+
+
+MASS_PC_KEYS = ("mass_r", "ave_mass_r", "densities_r", "ke_rad_r", "ke_tan_r", "v2_rad_r", "v2_tan_r")
+
 _P = C.POINTER
 _ctx = C.c_void_p
 _sigs = {
@@ -134,6 +142,7 @@ _sigs = {
     "cmcg_step_graph": [_ctx, _P(OrbitStats)],
     "cmcg_autotune": [_ctx, C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_double), _P(C.c_double)],
     "cmcg_test_hook": [_ctx, C.c_int32, C.c_int64],
+    "cmcg_comp_mass_percent": [_ctx, _P(C.c_double), C.c_int32, C.c_int32, C.c_double, C.c_double, _P(MassPc)],
     "cmcg_orbit_variants": [_ctx, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32),
                             _P(C.c_double)],
     "cmcg_ensemble_step": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
@@ -238,6 +247,73 @@ def kroupa_masses(n, seed=1):
     m = np.zeros(n)
     _chk(hostlib.cmc_ic_kroupa(n, seed, _ptr(m, C.c_double)), "cmc_ic_kroupa")
     return m
+
+
+# ---------------------------------------------------------------- HDF5 input (libcmcio.so)
+IO_LIB_PATH = os.path.join(HERE, "libcmcio.so")
+_iolib = None
+
+
+class Hdf5Header(C.Structure):
+    _fields_ = [("nobj", C.c_int64), ("nbinary", C.c_int64), ("rows", C.c_int64), ("cols", C.c_int64),
+                ("rvir", C.c_double), ("rtid", C.c_double), ("mclus", C.c_double), ("z", C.c_double),
+                ("cenma_m", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CmcIoError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__(f"{where}: {_io().cmc_io_error_string(code).decode()} ({code})")
+
+
+def _io():
+    """The HDF5 reader library, loaded on first use (it links the HDF5 C library,
+    which nothing else here needs)."""
+    global _iolib
+    if _iolib is None:
+        if not os.path.exists(IO_LIB_PATH):
+            raise ImportError("cmc-cosmic_amd: libcmcio.so not built (needs the HDF5 C library; "
+                              "make -C cmc-cosmic_amd HDF5_DIR=...)")
+        io = C.CDLL(IO_LIB_PATH)
+        io.cmc_io_error_string.restype = C.c_char_p
+        io.cmc_io_error_string.argtypes = [C.c_int]
+        io.cmc_hdf5_read_header.argtypes = [C.c_char_p, C.POINTER(Hdf5Header)]
+        io.cmc_hdf5_read_objects.argtypes = [C.c_char_p, C.POINTER(Hdf5Header), C.POINTER(HostSoa),
+                                             C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int64)]
+        _iolib = io
+    return _iolib
+
+
+def read_hdf5_header(path):
+    h = Hdf5Header()
+    rc = _io().cmc_hdf5_read_header(os.fsencode(path), C.byref(h))
+    if rc:
+        raise CmcIoError(rc, "cmc_hdf5_read_header")
+    return h
+
+
+def read_hdf5(path, columns=False):
+    """Cluster from the reference's HDF5 input file: id, m (scaled by NOBJ), r, vr, vt
+    in slots 0..n+1, `rtid` = RTID (the initial orbit_r), `cenma_m` = the central
+    mass and `header` = the file's attributes.  With columns=True also returns a
+    dict of the stellar type `k`, radius `Reff` and binary index `binind` columns."""
+    h = read_hdf5_header(path)
+    cl = Cluster(h.nobj)
+    extra = {"k": np.zeros(h.nobj + 2, np.int32), "Reff": np.zeros(h.nobj + 2),
+             "binind": np.zeros(h.nobj + 2, np.int64)}
+    rc = _io().cmc_hdf5_read_objects(os.fsencode(path), C.byref(h), C.byref(cl.soa()),
+                                     _ptr(extra["k"], C.c_int32), _ptr(extra["Reff"], C.c_double),
+                                     _ptr(extra["binind"], C.c_int64))
+    if rc:
+        raise CmcIoError(rc, "cmc_hdf5_read_objects")
+    cl.rtid = h.rtid
+    cl.cenma_m = h.cenma_m
+    cl.header = h.as_dict()
+    return (cl, extra) if columns else cl
 
 
 def default_params(n_star, orbit_r=1e3, seed=1, theta_se_max=1.412, avekernel=20, gamma=0.1, tidal_strip=1,
@@ -419,6 +495,18 @@ class Context:
         out = {"variant": v.value, "threads": th.value, "coarse": cs.value, "ms_best": best.value,
                "ms_default": default.value}
         out["variants"] = self.orbit_variants()
+        return out
+
+    def comp_mass_percent(self, pc, bh_include=0, initial_total_mass=1.0, minimum_r=0.0, fill=np.nan):
+        """cmcg_comp_mass_percent: dict of arrays (entries not reached = fill) and the count reached."""
+        pc = np.ascontiguousarray(pc, dtype=np.float64)
+        out = {k: np.full(pc.size, fill) for k in MASS_PC_KEYS}
+        out["no_star_r"] = np.full(pc.size, -1, dtype=np.int64)
+        o = MassPc(*[_ptr(out[k], C.c_double) for k in MASS_PC_KEYS], _ptr(out["no_star_r"], C.c_int64), 0, 0)
+        _chk(lib.cmcg_comp_mass_percent(self.h, _ptr(pc, C.c_double), int(pc.size), int(bh_include),
+                                        float(initial_total_mass), float(minimum_r), C.byref(o)),
+             "cmcg_comp_mass_percent")
+        out["reached"] = o.reached
         return out
 
     def test_hook(self, hook, value):

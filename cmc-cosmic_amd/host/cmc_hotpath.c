@@ -10,6 +10,7 @@
 #include <float.h>
 
 cmc_hotpath_globals cmc_hp;
+cmc_hotpath_lagrange cmc_lag = {0, 0, 0.0, 1.0, {0}, {0}, {0}, {0}, {0}, {0}, {0}, {0}, {0}};
 
 static cmcg_ctx *g_ctx = NULL;
 static uint64_t g_step = 0;   /* RNG step counter (tcount - 1 in the reference's main loop) */
@@ -217,6 +218,17 @@ void compute_energy_new(void)
     refresh();
     g_step++;
     cmc_hp.tcount++;
+}
+
+void comp_mass_percent(void)
+{
+    int64_t k[256];
+    cmcg_mass_pc o = {cmc_lag.mass_r, cmc_lag.ave_mass_r, cmc_lag.densities_r, cmc_lag.ke_rad_r, cmc_lag.ke_tan_r,
+                      cmc_lag.v2_rad_r, cmc_lag.v2_tan_r, k, 0, 0};
+    int n = cmc_lag.MASS_PC_COUNT < 256 ? cmc_lag.MASS_PC_COUNT : 256;
+    check(cmcg_comp_mass_percent(g_ctx, cmc_lag.mass_pc, n, cmc_lag.MASS_PC_BH_INCLUDE, cmc_lag.initial_total_mass,
+                                 cmc_lag.MINIMUM_R, &o), "comp_mass_percent");
+    for (int i = 0; i < o.reached; i++) cmc_lag.no_star_r[i] = (long)k[i];
 }
 
 cmc_orbit_rs_t calc_orbit_new(long index, double E, double J)

@@ -39,6 +39,20 @@ typedef struct {
 
 extern cmc_hotpath_globals cmc_hp;
 
+/* Inputs and outputs of comp_mass_percent (the reference's globals mass_pc,
+ * MASS_PC_COUNT, MASS_PC_BH_INCLUDE, MINIMUM_R, initial_total_mass and mass_r,
+ * ave_mass_r, no_star_r, densities_r, ke_rad_r, ke_tan_r, v2_rad_r, v2_tan_r,
+ * include/cmc/cmc_vars.h).  At most 256 fractions, ascending. */
+typedef struct {
+    int MASS_PC_COUNT, MASS_PC_BH_INCLUDE;
+    double MINIMUM_R, initial_total_mass;
+    double mass_pc[256];
+    double mass_r[256], ave_mass_r[256], densities_r[256];
+    double ke_rad_r[256], ke_tan_r[256], v2_rad_r[256], v2_tan_r[256];
+    long no_star_r[256];
+} cmc_hotpath_lagrange;
+extern cmc_hotpath_lagrange cmc_lag;
+
 typedef void (*cmc_hotpath_error_handler)(int code, int64_t index, const char *where);
 void cmc_hotpath_set_error_handler(cmc_hotpath_error_handler h);
 
@@ -75,6 +89,7 @@ void   tidally_strip_stars(void);                          /* src/cmc/cmc_evolut
 void   ComputeIntermediateEnergy(void);                    /* src/cmc/cmc_utils.c:368 */
 void   energy_conservation3(void);                         /* src/cmc/cmc_utils.c:360 */
 void   compute_energy_new(void);                           /* src/cmc/cmc_utils.c:2017 */
+void   comp_mass_percent(void);                            /* src/cmc/cmc_utils.c:872  */
 cmc_orbit_rs_t calc_orbit_new(long index, double E, double J);                 /* src/cmc/cmc_orbit.c:413 */
 cmc_orbit_rs_t calc_pericenter_and_apocenter(long index, double E, double J);  /* north_star name of the same */
 long   find_zero_Q(long j, long kmin, long kmax, long double E, long double J); /* src/cmc/cmc_nr.c:316 */

@@ -85,6 +85,7 @@ _sigs = {
     "orc_init": (None, [_SP]),
     "orc_dd_sincos": (None, [C.c_double, DPTR, DPTR]),
     "orc_dd_trig_table": (None, [C.c_int, DPTR]),
+    "orc_comp_mass_percent": (None, [C.c_void_p, DPTR, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]),
     "orc_step": (None, [_SP]),
 }
 for _n, (_r, _a) in _sigs.items():
@@ -101,6 +102,14 @@ def seed_state(seed):
     st = T113()
     lib.orc_t113_seed(seed, C.byref(st))
     return st
+
+
+class MassPcOut(C.Structure):
+    _fields_ = [(k, DPTR) for k in ("mass_r", "ave_mass_r", "densities_r", "ke_rad_r", "ke_tan_r", "v2_rad_r",
+                                    "v2_tan_r")] + [("no_star_r", C.POINTER(C.c_int64))]
+
+
+MASS_PC_KEYS = ("mass_r", "ave_mass_r", "densities_r", "ke_rad_r", "ke_tan_r", "v2_rad_r", "v2_tan_r")
 
 
 class Oracle:
@@ -165,6 +174,17 @@ class Oracle:
 
     def potential_calculate(self):
         return lib.orc_potential_calculate(C.byref(self.s))
+
+    def comp_mass_percent(self, pc, bh_include=0, initial_total_mass=1.0, minimum_r=0.0, fill=np.nan):
+        """comp_mass_percent (cmc_utils.c:872-982): dict of arrays, entries not reached = fill."""
+        pc = np.ascontiguousarray(pc, dtype=np.float64)
+        out = {k: np.full(pc.size, fill) for k in MASS_PC_KEYS}
+        out["no_star_r"] = np.full(pc.size, -1, dtype=np.int64)
+        o = MassPcOut(*[out[k].ctypes.data_as(DPTR) for k in MASS_PC_KEYS],
+                      out["no_star_r"].ctypes.data_as(C.POINTER(C.c_int64)))
+        lib.orc_comp_mass_percent(C.byref(self.s), pc.ctypes.data_as(DPTR), int(pc.size), int(bh_include),
+                                  float(initial_total_mass), float(minimum_r), C.byref(o))
+        return out
 
     def potential(self, r):
         return lib.orc_potential(C.byref(self.s), float(r))

@@ -55,3 +55,14 @@ def test_struct_layouts(cm):
     assert C.sizeof(cm.HostSoa) == 8 + 8 * 25
     assert C.sizeof(cm.Scalars) == 8 * 2 + 8 * 16 + 8
     assert C.sizeof(cm.OrbitStats) == 8 * 10 + 8 * 4 + 8 + 8
+
+
+def test_io_exports_declared_symbols(cm):
+    import pytest
+    if not os.path.exists(cm.IO_LIB_PATH):
+        pytest.skip("libcmcio.so not built (no HDF5 C library)")
+    names = declared("cmc_io.h")
+    assert {"cmc_hdf5_read_header", "cmc_hdf5_read_objects", "cmc_io_error_string"} <= names
+    for n in sorted(names):
+        assert hasattr(cm._io(), n), n
+    assert C.sizeof(cm.Hdf5Header) == 8 * 4 + 8 * 5

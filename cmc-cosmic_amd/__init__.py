@@ -104,6 +104,14 @@ class MassPc(C.Structure):
 
 MASS_PC_KEYS = ("mass_r", "ave_mass_r", "densities_r", "ke_rad_r", "ke_tan_r", "v2_rad_r", "v2_tan_r")
 
+CENTRAL_KEYS = ("rh", "rhojsum", "rhoj2sum", "rho", "v_rms", "rc", "rc_nb", "m_ave", "n", "rc_spitzer",
+                "N_core", "N_core_nb", "Trc")
+
+
+class Central(C.Structure):
+    _fields_ = [("nave", C.c_int64)] + [(k, C.c_double) for k in CENTRAL_KEYS]
+
+
 _P = C.POINTER
 _ctx = C.c_void_p
 _sigs = {
@@ -143,6 +151,7 @@ _sigs = {
     "cmcg_autotune": [_ctx, C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_double), _P(C.c_double)],
     "cmcg_test_hook": [_ctx, C.c_int32, C.c_int64],
     "cmcg_comp_mass_percent": [_ctx, _P(C.c_double), C.c_int32, C.c_int32, C.c_double, C.c_double, _P(MassPc)],
+    "cmcg_central_calculate": [_ctx, _P(Central)],
     "cmcg_orbit_variants": [_ctx, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32),
                             _P(C.c_double)],
     "cmcg_ensemble_step": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
@@ -508,6 +517,15 @@ class Context:
              "cmcg_comp_mass_percent")
         out["reached"] = o.reached
         return out
+
+    def central_calculate(self):
+        """cmcg_central_calculate: dict of the core quantities, the half-mass radius rh
+        and nave (central_calculate + clusdyn_calculate)."""
+        o = Central()
+        _chk(lib.cmcg_central_calculate(self.h, C.byref(o)), "cmcg_central_calculate")
+        d = {k: getattr(o, k) for k in CENTRAL_KEYS}
+        d["nave"] = o.nave
+        return d
 
     def test_hook(self, hook, value):
         _chk(lib.cmcg_test_hook(self.h, int(hook), int(value)), "cmcg_test_hook")

@@ -46,6 +46,7 @@ struct cmcg_ctx {
     int orb_variant = 0;         // launch shape of the orbit sampler (autotuned)
     std::vector<double> tune_ms; // per-variant time of the last cmcg_autotune
     void *mpc_buf = nullptr;     // cmcg_comp_mass_percent results (allocated on first use)
+    void *cen_buf = nullptr;     // cmcg_central_calculate results (allocated on first use)
 };
 
 static void invalidate_graphs(cmcg_ctx *c)
@@ -405,6 +406,7 @@ int cmcg_destroy(cmcg_ctx *c)
     for (void *p : c->allocs) hipFree(p);
     if (c->io) hipFree(c->io);
     if (c->mpc_buf) hipFree(c->mpc_buf);
+    if (c->cen_buf) hipFree(c->cen_buf);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return CMCG_OK;
@@ -1310,6 +1312,29 @@ int cmcg_comp_mass_percent(cmcg_ctx *c, const double *mass_pc, int32_t count, in
     }
     out->reached = (int32_t)reached;
     return check_error(c);
+}
+
+int cmcg_central_calculate(cmcg_ctx *c, cmcg_central *out)
+{
+    if (!c || !out) return CMCG_E_ARG;
+    if (!c->cen_buf) {
+        if (hipMalloc(&c->cen_buf, central_out_bytes()) != hipSuccess) {
+            c->cen_buf = nullptr;
+            return CMCG_E_NOMEM;
+        }
+    }
+    LAUNCH(launch_central(c->f, c->s, c->kp, c->n_ub, c->sc, c->cen_buf, c->stream));
+    // layout: int64 nave, int32 error, int32 pad, then 13 doubles in cmcg_central's order
+    std::vector<char> host(central_out_bytes());
+    if (hipMemcpyAsync(host.data(), c->cen_buf, host.size(), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        sync(c) != hipSuccess)
+        return CMCG_E_HIP;
+    int32_t err;
+    std::memcpy(&out->nave, host.data(), sizeof(int64_t));
+    std::memcpy(&err, host.data() + 8, sizeof(int32_t));
+    std::memcpy(&out->rh, host.data() + 16, 13 * sizeof(double));
+    if (int e = check_error(c)) return e;
+    return err == 0 ? CMCG_OK : err == 1 ? CMCG_E_STATE : CMCG_E_NAN;
 }
 
 int cmcg_test_hook(cmcg_ctx *c, int32_t hook, int64_t value)

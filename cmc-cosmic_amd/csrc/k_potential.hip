@@ -944,3 +944,240 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
 }
 
 size_t mass_percent_out_bytes() { return sizeof(MassPcOut); }
+
+// ---------------------------------------------------------------- central_calculate
+// cmc_utils.c:1239-1345 (J = 6) and clusdyn_calculate cmc_utils.c:1614-1624.  The
+// half-mass index nave comes from the enclosed-mass chain of comp_mass_percent
+// (star_m[k] / N_STAR, from 0); the six density-weighted sums over i = 1..nave are
+// sequential recurrences solved by the exact chain emulation.  Each sum's terms are
+// first scaled by a power of two that brings nave * max(term) to at most 1: binary
+// rounding is invariant under such a scaling, so the scaled chain rounds exactly as
+// the unscaled one and fits the chain's fixed-point range.
+constexpr int CEN_TERMS = 6;     // rhoj, rhoj^2, rhoj v^2, rhoj r, (rhoj r)^2, rhoj m madhoc
+enum { CEN_OK = 0, CEN_E_STATE = 1, CEN_E_RANGE = 2 };
+
+struct CentralOut {
+    int64_t nave;
+    int32_t error, pad;
+    double rh, rhojsum, rhoj2sum, rho, v_rms, rc, rc_nb, m_ave, n, rc_spitzer, N_core, N_core_nb, Trc;
+    double S[CEN_TERMS];
+    unsigned long long mx[CEN_TERMS];     // bit patterns of the terms' maxima (non-negative doubles)
+    double scale[CEN_TERMS], unscale[CEN_TERMS];
+};
+
+// rhoj of cmc_utils.c:1263-1274 for star i: the 5 masses between jmin and jmax = jmin + 6
+// over the shell volume between r[jmin] and r[jmax]
+__device__ __forceinline__ double cen_rhoj(const double *__restrict__ r, const double *__restrict__ m,
+                                           double madhoc, int64_t i)
+{
+    const double PI = 3.14159265358979323;                  // include/cmc/cmc.h:62
+    const int64_t jmin = i - 3 > 1 ? i - 3 : 1, jmax = jmin + 6;
+    double mrho = 0.0;
+#pragma unroll
+    for (int q = 1; q <= 5; q++) mrho += m[jmin + q] * madhoc;
+    const double a = r[jmax], b = r[jmin];
+    const double Vrj = 4.0 / 3.0 * PI * (a * a * a - b * b * b);
+    return mrho / Vrj;
+}
+
+__device__ __forceinline__ double cen_term(const DevFields &f, double madhoc, int64_t i, int t)
+{
+    const double rj = cen_rhoj(f.r, f.m, madhoc, i);
+    switch (t) {
+    case 0: return rj;
+    case 1: return rj * rj;
+    case 2: return rj * (f.vr[i] * f.vr[i] + f.vt[i] * f.vt[i]);
+    case 3: return rj * f.r[i];
+    case 4: { const double q = rj * f.r[i]; return q * q; }
+    default: return rj * f.m[i] * madhoc;
+    }
+}
+
+__global__ void cen_init_kernel(CentralOut *o, double *v0)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    o->nave = 0;
+    o->error = CEN_OK;
+    for (int t = 0; t < CEN_TERMS; t++) {
+        o->mx[t] = 0ull;
+        o->S[t] = 0.0;
+        o->scale[t] = o->unscale[t] = 1.0;
+    }
+    v0[0] = 0.0;
+    v0[1] = 0.0;
+}
+
+// nave and rh from the chain g[k-1] = m after star k (cmc_utils.c:1247-1257, 1620-1623);
+// the chain length for the sums becomes nave
+__global__ void cen_nave_kernel(const double *__restrict__ r, const DevScalars *s, const double *__restrict__ g,
+                                int64_t n_star, CentralOut *o, int64_t *len)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int64_t n = s->n_max;
+    const double thr = 0.5 * s->Mtotal;
+    int64_t lo = 1, hi = n + 1;                 // smallest k with !(g[k-1] < thr); g non-decreasing
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (!(g[mid - 1] < thr)) hi = mid;
+        else lo = mid + 1;
+    }
+    const int64_t nave = lo + 1;
+    const int64_t jmax_last = (nave - 3 > 1 ? nave - 3 : 1) + 6;
+    if (lo > n || n_star <= 12 || nave >= n_star - 6 || jmax_last > n + 1) {
+        o->error = CEN_E_STATE;
+        o->nave = 0;
+        len[3] = 1;                             // the sums then run over one zero term
+        return;
+    }
+    o->nave = nave;
+    o->rh = r[nave];
+    len[3] = nave;
+}
+
+__global__ __launch_bounds__(256) void cen_max_kernel(const DevFields f, double madhoc, CentralOut *o)
+{
+    __shared__ unsigned long long red[CEN_TERMS][256 / 64];
+    const int64_t nave = o->nave;
+    unsigned long long mx[CEN_TERMS] = {0, 0, 0, 0, 0, 0};
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nave; j += (int64_t)gridDim.x * blockDim.x)
+        for (int t = 0; t < CEN_TERMS; t++) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(cen_term(f, madhoc, j + 1, t));
+            mx[t] = b > mx[t] ? b : mx[t];      // a negative or NaN term has the largest patterns
+        }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int t = 0; t < CEN_TERMS; t++) {
+        unsigned long long v = mx[t];
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long u = __shfl_down(v, off, 64);
+            v = u > v ? u : v;
+        }
+        if (lane == 0) red[t][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < CEN_TERMS) {
+        unsigned long long v = 0;
+        for (int q = 0; q < 256 / 64; q++) v = red[threadIdx.x][q] > v ? red[threadIdx.x][q] : v;
+        if (v) atomicMax(&o->mx[threadIdx.x], v);
+    }
+}
+
+__global__ void cen_scale_kernel(CentralOut *o)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || o->error) return;
+    for (int t = 0; t < CEN_TERMS; t++) {
+        if (o->mx[t] >= 0x7ff0000000000000ull) {   // infinite, NaN or negative
+            o->error = CEN_E_RANGE;
+            return;
+        }
+        const double bound = (double)o->nave * __longlong_as_double((long long)o->mx[t]);
+        if (bound > 0.0) {
+            int e;
+            frexp(bound, &e);                      // bound <= 2^e
+            o->scale[t] = ldexp(1.0, -e);
+            o->unscale[t] = ldexp(1.0, e);
+        }
+    }
+}
+
+__global__ void cen_prep_kernel(const DevFields f, double madhoc, int t, CentralOut *o, double *__restrict__ x)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nave = o->nave;
+    if (o->error) {
+        if (j == 0) x[0] = 0.0;
+        return;
+    }
+    if (j >= nave) return;
+    const double v = cen_term(f, madhoc, j + 1, t) * o->scale[t];
+    if (v != 0.0 && v < 2.2250738585072014e-308) o->error = CEN_E_RANGE;   // scaled into subnormals
+    x[j] = v;
+}
+
+__global__ void cen_take_kernel(const double *__restrict__ g, CentralOut *o, int t)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || o->error) return;
+    o->S[t] = g[o->nave - 1] * o->unscale[t];
+}
+
+// the averages and derived quantities, cmc_utils.c:1323-1345
+__global__ void cen_final_kernel(CentralOut *o)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || o->error) return;
+    const double PI = 3.14159265358979323;
+    const double rhojsum = o->S[0], rhoj2sum = o->S[1];
+    double rho = o->S[1], v_rms = o->S[2], rc = o->S[3], rc_nb = o->S[4], m_ave = o->S[5];
+    rho /= rhojsum;
+    rho *= 4.0 / 5.0;
+    rho *= 2.0;
+    v_rms /= rhojsum;
+    v_rms = sqrt(v_rms);
+    rc /= rhojsum;
+    rc_nb = sqrt(rc_nb / rhoj2sum);
+    m_ave /= rhojsum;
+    const double n = rho / m_ave;
+    o->rc_spitzer = sqrt(3.0 * (v_rms * v_rms) / (4.0 * PI * rho));
+    o->N_core = 4.0 / 3.0 * PI * (rc * rc * rc) * (n / 2.0);
+    o->N_core_nb = 4.0 / 3.0 * PI * (rc_nb * rc_nb * rc_nb) * (n / 2.0);
+    o->Trc = 0.065 * (v_rms * v_rms * v_rms) / (rho * m_ave);
+    o->rhojsum = rhojsum;
+    o->rhoj2sum = rhoj2sum;
+    o->rho = rho;
+    o->v_rms = v_rms;
+    o->rc = rc;
+    o->rc_nb = rc_nb;
+    o->m_ave = m_ave;
+    o->n = n;
+}
+
+hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, const Scratch &sc,
+                          void *out, hipStream_t st)
+{
+    int64_t ntiles = (n_ub + CTILE - 1) / CTILE + 1;
+    if (ntiles > sc.max_tiles) return hipErrorInvalidValue;
+    i128 *tb = (i128 *)sc.tile_sums;
+    const int64_t T = sc.max_tiles;
+    EsaArgs e{};
+    e.tsumP = tb + 2 * T; e.tpreP = tb + 3 * T; e.tsumR = tb + 4 * T; e.tpreR = tb + 5 * T;
+    e.tcnt = tb + 6 * T; e.toff = tb + 7 * T;
+    e.tsum = tb;
+    e.s_idx = sc.esa_idx;
+    e.s_racc = (i128 *)sc.esa_racc;
+    e.s_v = sc.esa_v;
+    e.s_n = sc.esa_n;
+    e.smax = sc.esa_max;
+    e.raw_n = sc.esa_n + 1;
+    e.raw_idx = sc.esa_raw_idx;
+    e.raw_part = (i128 *)sc.esa_raw_part;
+    e.x = sc.chain_x; e.g = sc.chain_g; e.sign = 1.0; e.first_bad = sc.chain_fbad + 6;
+    e.Lp = sc.chain_len + 3;
+    double *v0 = sc.chain_scal + 2;              // both start values 0
+    CentralOut *o = (CentralOut *)out;
+    const unsigned g1 = (unsigned)((n_ub + 256) / 256);
+    const int KD = P.chain_k_drop;
+    const int K = 100 - KD < 8 ? 8 : 100 - KD;
+    hipLaunchKernelGGL(cen_init_kernel, dim3(1), dim3(1), 0, st, o, v0);
+    // enclosed mass, star_m[k] / N_STAR from 0
+    hipLaunchKernelGGL(mpc_prep_m_kernel, dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
+                       sc.chain_x, sc.chain_len);
+    EsaArgs em = e;
+    em.v0p = v0; em.K = K;
+    ChainArgs cmass{sc.chain_x, sc.chain_g, 1.0, v0, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, K};
+    if (hipError_t err = run_chain(cmass, em, REPAIR_ROUNDS, n_ub, st)) return err;
+    hipLaunchKernelGGL(cen_nave_kernel, dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
+                       (const double *)sc.chain_g, (int64_t)P.n_star, o, sc.chain_len);
+    const unsigned gmax = g1 < 1024u ? g1 : 1024u;
+    hipLaunchKernelGGL(cen_max_kernel, dim3(gmax), dim3(256), 0, st, f, P.madhoc, o);
+    hipLaunchKernelGGL(cen_scale_kernel, dim3(1), dim3(1), 0, st, o);
+    for (int t = 0; t < CEN_TERMS; t++) {
+        hipLaunchKernelGGL(cen_prep_kernel, dim3(g1), dim3(256), 0, st, f, P.madhoc, t, o, sc.chain_x);
+        EsaArgs es = e;
+        es.v0p = v0 + 1; es.K = K;
+        ChainArgs cs{sc.chain_x, sc.chain_g, 1.0, v0 + 1, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, K};
+        if (hipError_t err = run_chain(cs, es, REPAIR_ROUNDS, n_ub, st)) return err;
+        hipLaunchKernelGGL(cen_take_kernel, dim3(1), dim3(1), 0, st, (const double *)sc.chain_g, o, t);
+    }
+    hipLaunchKernelGGL(cen_final_kernel, dim3(1), dim3(1), 0, st, o);
+    return hipGetLastError();
+}
+
+size_t central_out_bytes() { return sizeof(CentralOut); }

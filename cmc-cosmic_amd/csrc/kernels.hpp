@@ -79,6 +79,9 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
                                const double *pc, int count, int bh_include, double itm, double minimum_r,
                                void *out, hipStream_t st);
 size_t mass_percent_out_bytes();
+hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, const Scratch &sc,
+                          void *out, hipStream_t st);
+size_t central_out_bytes();
 hipError_t launch_potential_at(const DevFields &f, DevScalars *s, const KParams &P,
                                const double *x, double *out, int64_t count, hipStream_t st);
 

@@ -11,6 +11,7 @@
 
 cmc_hotpath_globals cmc_hp;
 cmc_hotpath_lagrange cmc_lag = {0, 0, 0.0, 1.0, {0}, {0}, {0}, {0}, {0}, {0}, {0}, {0}, {0}};
+cmc_hotpath_central cmc_cen;
 
 static cmcg_ctx *g_ctx = NULL;
 static uint64_t g_step = 0;   /* RNG step counter (tcount - 1 in the reference's main loop) */
@@ -42,6 +43,7 @@ static void refresh(void)
     cmc_hp.Etot = s.Etot;
     cmc_hp.Ekin = s.Ekin;
     cmc_hp.Epot = s.Epot;
+    cmc_hp.Eoops = s.Eoops;
 }
 
 static int check(int rc, const char *where)
@@ -229,6 +231,48 @@ void comp_mass_percent(void)
     check(cmcg_comp_mass_percent(g_ctx, cmc_lag.mass_pc, n, cmc_lag.MASS_PC_BH_INCLUDE, cmc_lag.initial_total_mass,
                                  cmc_lag.MINIMUM_R, &o), "comp_mass_percent");
     for (int i = 0; i < o.reached; i++) cmc_lag.no_star_r[i] = (long)k[i];
+}
+
+void central_calculate(void)
+{
+    cmcg_central c;
+    check(cmcg_central_calculate(g_ctx, &c), "central_calculate");
+    cmc_cen.rho = cmc_cen.rho_core = c.rho;
+    cmc_cen.v_rms = cmc_cen.v_core = c.v_rms;
+    cmc_cen.rc = cmc_cen.core_radius = c.rc;
+    cmc_cen.m_ave = c.m_ave;
+    cmc_cen.n = c.n;
+    cmc_cen.rc_spitzer = c.rc_spitzer;
+    cmc_cen.N_core = c.N_core;
+    cmc_cen.N_core_nb = c.N_core_nb;
+    cmc_cen.rc_nb = c.rc_nb;
+    cmc_cen.Trc = c.Trc;
+    cmc_cen.rh = c.rh;
+    cmc_cen.nave = (long)c.nave;
+}
+
+/* clusdyn.rh comes from the same half-mass loop; one device call serves both. */
+void clusdyn_calculate(void) { central_calculate(); }
+
+const char *cmc_hotpath_dyn_header(void)
+{
+    return "# Dynamical information [code units]\n"
+           "#1:t #2:Dt #3:tcount #4:N #5:M #6:VR #7:N_c #8:r_c #9:r_max #10:Etot #11:KE #12:PE #13:Etot_int "
+           "#14:Etot_bin #15:E_cenma #16:Eesc #17:Ebesc #18:Eintesc #19:Eoops #20:Etot+Eoops #21:r_h #22:rho_0 "
+           "#23:rc_spitzer #24:v0_rms #25:rc_nb #26.DMse(MSUN) #27.DMrejuv(MSUN) #28.N_c_nb\n";
+}
+
+int cmc_hotpath_dyn_line(char *buf, size_t len)
+{
+    const double Eint = 0.0, Eb = 0.0, cenma_E = 0.0, Ebesc = 0.0, Eintesc = 0.0, DMse = 0.0, DMrejuv = 0.0;
+    return snprintf(buf, len,
+                    "%.8g %.8g %ld %ld %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g "
+                    "%.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g %.8g\n",
+                    cmc_hp.TotalTime, cmc_hp.Dt, cmc_hp.tcount, cmc_hp.N_MAX, cmc_hp.Mtotal,
+                    -2.0 * cmc_hp.Ekin / cmc_hp.Epot, cmc_cen.N_core, cmc_cen.core_radius, cmc_hp.max_r,
+                    cmc_hp.Etot, cmc_hp.Ekin, cmc_hp.Epot, Eint, Eb, cenma_E, cmc_hp.Eescaped, Ebesc, Eintesc,
+                    cmc_hp.Eoops, cmc_hp.Etot + cmc_hp.Eoops, cmc_cen.rh, cmc_cen.rho, cmc_cen.rc_spitzer,
+                    cmc_cen.v_rms, cmc_cen.rc_nb, DMse, DMrejuv, cmc_cen.N_core_nb);
 }
 
 cmc_orbit_rs_t calc_orbit_new(long index, double E, double J)

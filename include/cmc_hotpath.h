@@ -35,6 +35,7 @@ typedef struct {
     double TidalMassLoss, Eescaped, Jescaped, Etidal;
     double Etot, Ekin, Epot;
     long tcount;
+    double Eoops;
 } cmc_hotpath_globals;
 
 extern cmc_hotpath_globals cmc_hp;
@@ -52,6 +53,17 @@ typedef struct {
     long no_star_r[256];
 } cmc_hotpath_lagrange;
 extern cmc_hotpath_lagrange cmc_lag;
+
+/* Outputs of central_calculate and clusdyn_calculate (the reference's globals
+ * central.rho/.v_rms/.rc/.m_ave/.n/.rc_spitzer, rho_core, v_core, core_radius,
+ * N_core, N_core_nb, rc_nb, Trc and clusdyn.rh; include/cmc/cmc_vars.h). */
+typedef struct {
+    double rho, v_rms, rc, m_ave, n, rc_spitzer;     /* central.* */
+    double rho_core, v_core, core_radius, N_core, N_core_nb, rc_nb, Trc;
+    double rh;                                       /* clusdyn.rh */
+    long nave;
+} cmc_hotpath_central;
+extern cmc_hotpath_central cmc_cen;
 
 typedef void (*cmc_hotpath_error_handler)(int code, int64_t index, const char *where);
 void cmc_hotpath_set_error_handler(cmc_hotpath_error_handler h);
@@ -90,6 +102,17 @@ void   ComputeIntermediateEnergy(void);                    /* src/cmc/cmc_utils.
 void   energy_conservation3(void);                         /* src/cmc/cmc_utils.c:360 */
 void   compute_energy_new(void);                           /* src/cmc/cmc_utils.c:2017 */
 void   comp_mass_percent(void);                            /* src/cmc/cmc_utils.c:872  */
+void   central_calculate(void);                            /* src/cmc/cmc_utils.c:1239 */
+void   clusdyn_calculate(void);                            /* src/cmc/cmc_utils.c:1614 */
+/* One row of the reference's <output>.dyn.dat (the fprintf of src/cmc/cmc_io.c:320-323,
+ * same columns and %.8g format, newline included) for the current state, from cmc_hp and
+ * cmc_cen: call central_calculate() and compute_energy_new() first.  Columns of physics
+ * outside the hot path (internal and binary energies, cenma.E, binary escapes, DMse,
+ * DMrejuv) are 0, as they are in the reference with those modules off.  Returns what
+ * snprintf returns. */
+int    cmc_hotpath_dyn_line(char *buf, size_t len);
+/* The two header lines of that file (src/cmc/cmc_io.c:2346-2347). */
+const char *cmc_hotpath_dyn_header(void);
 cmc_orbit_rs_t calc_orbit_new(long index, double E, double J);                 /* src/cmc/cmc_orbit.c:413 */
 cmc_orbit_rs_t calc_pericenter_and_apocenter(long index, double E, double J);  /* north_star name of the same */
 long   find_zero_Q(long j, long kmin, long kmax, long double E, long double J); /* src/cmc/cmc_nr.c:316 */

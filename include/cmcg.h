@@ -239,6 +239,28 @@ typedef struct cmcg_mass_pc {
 int cmcg_comp_mass_percent(cmcg_ctx *ctx, const double *mass_pc, int32_t count, int32_t bh_include,
                            double initial_total_mass, double minimum_r, cmcg_mass_pc *out);
 
+/* Core quantities; replaces central_calculate (src/cmc/cmc_utils.c:1239-1345, the
+ * Casertano & Hut (1985) density-weighted averages over the stars out to the
+ * half-mass radius, J = 6) and clusdyn_calculate (:1614-1624, rh) for one rank.
+ * Every sum is the reference's sequential recurrence, bit for bit.  The block after
+ * them (NUM_CENTRAL_STARS averages for collision and binary rates) belongs to the
+ * physics modules outside the hot path and is not computed.  Returns CMCG_E_STATE
+ * where the reference exits (N_STAR <= 12 or nave >= N_STAR - 6, or the half-mass
+ * star's window passes the live stars), CMCG_E_NAN for an infinite, NaN or negative
+ * density term (r not ascending). */
+typedef struct cmcg_central {
+    int64_t nave;          /* stars averaged: the one just beyond the half-mass radius */
+    double rh;             /* clusdyn.rh = star_r[nave] */
+    double rhojsum, rhoj2sum;
+    double rho;            /* central.rho (rho_core) */
+    double v_rms;          /* central.v_rms (v_core) */
+    double rc;             /* central.rc (core_radius) */
+    double rc_nb;          /* rc_nb */
+    double m_ave, n, rc_spitzer;
+    double N_core, N_core_nb, Trc;
+} cmcg_central;
+int cmcg_central_calculate(cmcg_ctx *ctx, cmcg_central *out);
+
 /* Test hooks (no counterpart in the reference; for tests that force rare paths).
  * CMCG_HOOK_CHAIN_K_DROP: coarsen the fixed-point scale of the potential chains by
  * `value` bits (0..92; 0 = normal).  Nearly every element then takes the fallback

@@ -114,6 +114,20 @@ typedef struct {
 void orc_comp_mass_percent(const orc_state *s, const double *pc, int count, int bh_include,
                            double initial_total_mass, double minimum_r, orc_mass_pc_out *o);
 
+/* central_calculate (cmc_utils.c:1239-1345, the Casertano & Hut density-weighted
+ * core quantities) and clusdyn_calculate (cmc_utils.c:1614-1624, the half-mass
+ * radius) for one rank; the NUM_CENTRAL_STARS block after them (collision and
+ * binary rates) is not restated.  Returns 0, or -1 where the reference exits
+ * (N_STAR <= 12 or nave >= N_STAR - 6) or the half-mass loop would leave the live
+ * stars. */
+typedef struct {
+    int64_t nave;          /* stars averaged over: the one just beyond the half-mass radius */
+    double rh;             /* clusdyn.rh = star_r[nave] */
+    double rhojsum, rhoj2sum;
+    double rho, v_rms, rc, rc_nb, m_ave, n, rc_spitzer, N_core, N_core_nb, Trc;
+} orc_central_out;
+int orc_central_calculate(const orc_state *s, orc_central_out *o);
+
 /* double-double sin/cos shared (operation for operation) with the device relaxation */
 void orc_dd_sincos(double x, double *s, double *c);
 void orc_dd_trig_table(int j, double *out4);   /* sin/cos of j pi/128 (hi, lo), j = 0..32 */

@@ -86,6 +86,7 @@ _sigs = {
     "orc_dd_sincos": (None, [C.c_double, DPTR, DPTR]),
     "orc_dd_trig_table": (None, [C.c_int, DPTR]),
     "orc_comp_mass_percent": (None, [C.c_void_p, DPTR, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]),
+    "orc_central_calculate": (C.c_int, [C.c_void_p, C.c_void_p]),
     "orc_step": (None, [_SP]),
 }
 for _n, (_r, _a) in _sigs.items():
@@ -107,6 +108,14 @@ def seed_state(seed):
 class MassPcOut(C.Structure):
     _fields_ = [(k, DPTR) for k in ("mass_r", "ave_mass_r", "densities_r", "ke_rad_r", "ke_tan_r", "v2_rad_r",
                                     "v2_tan_r")] + [("no_star_r", C.POINTER(C.c_int64))]
+
+
+CENTRAL_KEYS = ("rh", "rhojsum", "rhoj2sum", "rho", "v_rms", "rc", "rc_nb", "m_ave", "n", "rc_spitzer",
+                "N_core", "N_core_nb", "Trc")
+
+
+class CentralOut(C.Structure):
+    _fields_ = [("nave", C.c_int64)] + [(k, C.c_double) for k in CENTRAL_KEYS]
 
 
 MASS_PC_KEYS = ("mass_r", "ave_mass_r", "densities_r", "ke_rad_r", "ke_tan_r", "v2_rad_r", "v2_tan_r")
@@ -185,6 +194,16 @@ class Oracle:
         lib.orc_comp_mass_percent(C.byref(self.s), pc.ctypes.data_as(DPTR), int(pc.size), int(bh_include),
                                   float(initial_total_mass), float(minimum_r), C.byref(o))
         return out
+
+    def central_calculate(self):
+        """central_calculate + clusdyn_calculate (cmc_utils.c:1239-1345, 1614-1624):
+        dict of the core quantities and rh; raises where the reference exits."""
+        o = CentralOut()
+        if lib.orc_central_calculate(C.byref(self.s), C.byref(o)) != 0:
+            raise RuntimeError("central_calculate: N_STAR <= 12 or nave >= N_STAR - 6")
+        d = {k: getattr(o, k) for k in CENTRAL_KEYS}
+        d["nave"] = o.nave
+        return d
 
     def potential(self, r):
         return lib.orc_potential(C.byref(self.s), float(r))

@@ -259,3 +259,67 @@ void orc_comp_mass_percent(const orc_state *s, const double *pc, int count, int 
     }
     free(kr); free(kt); free(vr2); free(vt2);
 }
+
+/* central_calculate, cmc_utils.c:1239-1345 (first part; J = 6), and
+ * clusdyn_calculate, cmc_utils.c:1614-1624.  sqr/cub are the reference's
+ * x*x and x*x*x (cmc_utils.c:27-42), fb_cub the same (fewbody_utils.c:72-75). */
+int orc_central_calculate(const orc_state *s, orc_central_out *o)
+{
+    const double PI = 3.14159265358979323;             /* include/cmc/cmc.h:62 */
+    const long J = 6;
+    double m = 0.0;
+    int64_t nave = 1;
+    while (m < 0.5 * s->Mtotal) {                        /* cmc_utils.c:1247-1251 */
+        if (nave > s->N_MAX) return -1;
+        m += s->m[nave] / s->N_STAR;
+        nave++;
+    }
+    if (s->N_STAR <= 2 * J || nave >= s->N_STAR - 6) return -1;   /* :1254-1257 */
+    if ((nave - J / 2 > 1 ? nave - J / 2 : 1) + J > s->N_MAX + 1) return -1;   /* last jmax past the live stars */
+    double *rhoj = malloc((size_t)(nave + 1) * sizeof(double));
+    for (int64_t i = 1; i <= nave; i++) {                /* :1263-1274 */
+        int64_t jmin = i - J / 2 > 1 ? i - J / 2 : 1;
+        int64_t jmax = jmin + J;
+        double mrho = 0.0;
+        for (int64_t j = jmin + 1; j <= jmax - 1; j++) mrho += s->m[j] * s->madhoc;
+        double a = s->r[jmax], b = s->r[jmin];
+        double Vrj = 4.0 / 3.0 * PI * (a * a * a - b * b * b);
+        rhoj[i] = mrho / Vrj;
+    }
+    double rhojsum = 0.0, rhoj2sum = 0.0, rho = 0.0, v_rms = 0.0, rc = 0.0, rc_nb = 0.0, m_ave = 0.0;
+    for (int64_t i = 1; i <= nave; i++) {                /* :1283-1294 */
+        rhojsum += rhoj[i];
+        rhoj2sum += rhoj[i] * rhoj[i];
+        rho += rhoj[i] * rhoj[i];
+        v_rms += rhoj[i] * (s->vr[i] * s->vr[i] + s->vt[i] * s->vt[i]);
+        rc += rhoj[i] * s->r[i];
+        double q = rhoj[i] * s->r[i];
+        rc_nb += q * q;
+        m_ave += rhoj[i] * s->m[i] * s->madhoc;
+    }
+    free(rhoj);
+    rho /= rhojsum;                                      /* :1323-1345 */
+    rho *= 4.0 / 5.0;
+    rho *= 2.0;
+    v_rms /= rhojsum;
+    v_rms = sqrt(v_rms);
+    rc /= rhojsum;
+    rc_nb = sqrt(rc_nb / rhoj2sum);
+    m_ave /= rhojsum;
+    double n = rho / m_ave;
+    o->rc_spitzer = sqrt(3.0 * (v_rms * v_rms) / (4.0 * PI * rho));
+    o->N_core = 4.0 / 3.0 * PI * (rc * rc * rc) * (n / 2.0);
+    o->N_core_nb = 4.0 / 3.0 * PI * (rc_nb * rc_nb * rc_nb) * (n / 2.0);
+    o->Trc = 0.065 * (v_rms * v_rms * v_rms) / (rho * m_ave);
+    o->nave = nave;
+    o->rh = s->r[nave];                                  /* clusdyn_calculate: same loop, rh = star_r[k] */
+    o->rhojsum = rhojsum;
+    o->rhoj2sum = rhoj2sum;
+    o->rho = rho;
+    o->v_rms = v_rms;
+    o->rc = rc;
+    o->rc_nb = rc_nb;
+    o->m_ave = m_ave;
+    o->n = n;
+    return 0;
+}

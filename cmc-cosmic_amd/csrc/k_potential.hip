@@ -83,23 +83,41 @@ __device__ __forceinline__ i128 incr(double prev, double x, double sign, int K)
     return fxk(s, K) - fxk(prev, K);
 }
 
-// block-wide exclusive scan of int128 through LDS (Hillis-Steele)
+__device__ __forceinline__ i128 shfl_up_i128(i128 v, int d)
+{
+    const unsigned __int128 u = (unsigned __int128)v;
+    unsigned long long lo = (unsigned long long)u, hi = (unsigned long long)(u >> 64);
+    lo = __shfl_up(lo, d, 64);
+    hi = __shfl_up(hi, d, 64);
+    return (i128)(((unsigned __int128)hi << 64) | lo);
+}
+
+// block-wide exclusive scan of int128: inclusive scans inside each 64-lane wave by
+// shuffles, then the wave totals through LDS (lds needs NT / 64 entries); every
+// thread of the block must call it
 template <int NT>
 __device__ i128 block_excl_scan(i128 v, i128 *lds, i128 *total)
 {
-    const int t = threadIdx.x;
-    lds[t] = v;
-    __syncthreads();
-    for (int off = 1; off < NT; off <<= 1) {
-        i128 add = t >= off ? lds[t - off] : (i128)0;
-        __syncthreads();
-        lds[t] += add;
-        __syncthreads();
+    constexpr int NW = NT / 64;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    i128 x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const i128 y = shfl_up_i128(x, d);
+        if (lane >= d) x += y;
     }
-    i128 incl = lds[t];
-    *total = lds[NT - 1];
+    if (lane == 63) lds[w] = x;
     __syncthreads();
-    return incl - v;
+    i128 off = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) {
+        const i128 s = lds[q];
+        if (q < w) off += s;
+        tot += s;
+    }
+    *total = tot;
+    __syncthreads();
+    return off + x - v;
 }
 
 // One block: takes the restart index found by the previous round, snapshots the

@@ -472,6 +472,10 @@ double orc_calc_n_local(const orc_state *s, int64_t k, int64_t p, int64_t N_LIMI
     int64_t kmin = k - p, kmax = k + p;
     if (kmin < 0) { kmin = 0; kmax = 2 * p + 1; }
     else if (kmax > N_LIMIT) { kmax = N_LIMIT; kmin = N_LIMIT - 2 * p - 1; }
+    /* below 2p+2 stars the reference indexes outside [0, N+1] (undefined); both this
+     * restatement and the device clamp to the array there */
+    if (kmax > N_LIMIT + 1) kmax = N_LIMIT + 1;
+    if (kmin < 0) kmin = 0;
     return (2.0 * ((double)p)) * 3.0 / (4.0 * ORC_PI * (orc_cub(s->r[kmax]) - orc_cub(s->r[kmin])));
 }
 
@@ -485,6 +489,8 @@ void orc_calc_sigma_r(orc_state *s)
         simax = simin + (2 * p - 1);
         if (simin < 1) { simin = 1; simax = simin + (2 * p - 1); }
         else if (simax > s->N_MAX) { simax = N; simin = simax - (2 * p - 1); }
+        if (simin < 1) simin = 1;           /* N < 2p: window clamped to the live stars, as on the device */
+        if (simax > N) simax = N;
         for (k = siminlast; k < simin; k++) {
             Mv2ave -= s->m[k] * s->madhoc * (orc_sqr(s->vr[k]) + orc_sqr(s->vt[k]));
             Mave -= s->m[k] * s->madhoc;

@@ -4,12 +4,14 @@
 // (remove_stripped_stars, :343-354), then refresh r and m (post_sort_comm,
 // src/cmc/cmc_utils.c:2259-2288).
 //
-// Device algorithm: LSD radix sort of (64-bit order-preserving key of r, 32-bit
-// slot) with 8-bit digits, 8 passes of three kernels (tile histogram, per-digit
-// tile scan, stable scatter).  The scatter ranks each 4096-key tile with 64-lane
-// ballot multisplit (stable in tile order), stages the sorted tile in LDS and
-// writes it out digit-run by digit-run, so global writes are coalesced.  A tie
-// pass orders runs of equal keys by id, then one gather moves the payload fields.
+// Device algorithm: LSD radix sort of one 64-bit word per star: a 32-bit key z on
+// top (the order-preserving bits of r, shifted by the live minimum and scaled so the
+// live range fills 31 bits; the SF_INFINITY tail gets z = 2^32 - 1) and the star's
+// slot below.  Four passes of 8-bit digits over z, each three kernels (tile
+// histogram, per-digit tile scan, stable scatter).  The scatter ranks each 4096-word
+// tile with 64-lane ballot multisplit (stable in tile order), stages the sorted tile
+// in LDS and writes it out digit-run by digit-run, so global writes are coalesced.
+// Runs of equal z are then put in (r, id) order, and one gather moves the payload.
 #include "kernels.hpp"
 
 namespace {
@@ -17,11 +19,13 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_MAX_FIELDS = 24;
-// LSD passes over the top 40 bits of the key only; keys equal in those bits are then
-// put in (key, id) order by the run pass (one thread per run up to RUN_SHORT keys,
-// one block sorting in LDS up to RUN_LONG keys).
-constexpr int RS_PASSES = 5;
-constexpr int RS_SHIFT0 = 64 - 8 * RS_PASSES;
+// LSD passes over z, the top 32 bits of each word; words equal in z are then put in
+// (r, id) order by the run pass (one thread per run up to RUN_SHORT words, one block
+// sorting in LDS up to RUN_LONG words).
+constexpr int RS_PASSES = 4;
+constexpr int RS_SHIFT0 = 64 - 8 * RS_PASSES;     // z = word >> RS_SHIFT0
+constexpr uint32_t Z_TAIL = 0xffffffffu;          // z of the SF_INFINITY tail; live z < 2^31
+static_assert(RS_PASSES % 2 == 0, "the sorted words must end in keys_a");
 constexpr int RUN_SHORT = 64;
 constexpr int RUN_LONG = 4096;
 constexpr int AOS_W = SORT_AOS_WORDS;   // words per packed payload record (one 128-B line)
@@ -31,11 +35,6 @@ __device__ __forceinline__ uint64_t sortable(double x)
 {
     uint64_t b = (uint64_t)__double_as_longlong(x);
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-__device__ __forceinline__ double unsortable(uint64_t k)
-{
-    uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-    return __longlong_as_double((long long)b);
 }
 
 struct GatherList {
@@ -50,19 +49,67 @@ __global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n)
     if (i < n) a[i] = 0;
 }
 
-// keys/vals from slots 1..n plus the eight global digit histograms; with `aos` the
-// payload fields of each slot are also packed into one 96-byte record, so the
-// final gather reads one record per star instead of one cache line per field.
+__device__ __forceinline__ uint32_t word_slot(uint64_t w) { return (uint32_t)w; }
+__device__ __forceinline__ uint32_t word_z(uint64_t w) { return (uint32_t)(w >> RS_SHIFT0); }
+
+// Minimum and maximum order-preserving key of the live stars (r below SF_INFINITY):
+// mm[0] = min, mm[1] = max (set to ~0 and 0 by rs_begin_kernel).
+__global__ __launch_bounds__(256) void rs_minmax_kernel(const double *__restrict__ keysrc, DevScalars *s,
+                                                        uint64_t kinf, unsigned long long *__restrict__ mm)
+{
+    __shared__ unsigned long long rmin[4], rmax[4];
+    const int64_t n = s->n_max_new;
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = sortable(keysrc[i + 1]);
+        if (k < kinf) {
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long a = __shfl_down(lo, off, 64), b = __shfl_down(hi, off, 64);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { rmin[w] = lo; rmax[w] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; q++) {
+            lo = rmin[q] < lo ? rmin[q] : lo;
+            hi = rmax[q] > hi ? rmax[q] : hi;
+        }
+        if (lo != ~0ull) atomicMin(&mm[0], lo);
+        if (hi != 0ull) atomicMax(&mm[1], hi);
+    }
+}
+
+// Words from slots 1..n plus the four global digit histograms; with `aos` the payload
+// fields of each slot, and r itself after them, are packed into one 128-byte record,
+// so the final gather reads one record per star instead of one cache line per field.
 __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__restrict__ keysrc, DevScalars *s,
-                                                             uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                             uint64_t *__restrict__ words,
                                                              uint32_t *__restrict__ digit_totals,
-                                                             uint64_t *__restrict__ aos, GatherList g, uint64_t kinf)
+                                                             uint64_t *__restrict__ aos, GatherList g, uint64_t kinf,
+                                                             const unsigned long long *__restrict__ mm)
 {
     constexpr int P0 = 8 - RS_PASSES;      // first byte sorted on
-    __shared__ uint32_t h[RS_PASSES][256]; // 40 KB of LDS in all: four blocks per CU
+    __shared__ uint32_t h[RS_PASSES][256];
     __shared__ uint64_t recs[RS_THREADS * AOS_LD];     // one chunk of packed records
     const int t = threadIdx.x;
     for (int p = 0; p < RS_PASSES; p++) h[p][t] = 0;
+    // z = (key - min) >> S with S chosen so the live range fits 31 bits
+    uint64_t kmin = mm[0];
+    const uint64_t kmax = mm[1];
+    int S = 0;
+    if (kmax >= kmin) {
+        const uint64_t range = kmax - kmin;
+        const int bits = range ? 64 - __clzll((long long)range) : 0;
+        S = bits > 31 ? bits - 31 : 0;
+    } else {
+        kmin = 0;      // no live star
+    }
     __syncthreads();
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -70,15 +117,17 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
         const int64_t i0 = base + u * RS_THREADS;
         const int64_t i = i0 + t;
         if (i < n) {
-            uint64_t k = sortable(keysrc[i + 1]);
-            if (k >= kinf) k = ~0ull;      // the SF_INFINITY tail: one key, no prefix shared with a live star
-            keys[i] = k;
-            vals[i] = (uint32_t)(i + 1);
-            for (int p = P0; p < 8; p++) atomicAdd(&h[p - P0][(k >> (8 * p)) & 255], 1u);
+            const uint64_t k = sortable(keysrc[i + 1]);
+            const uint32_t z = k >= kinf ? Z_TAIL : (uint32_t)((k - kmin) >> S);
+            const uint64_t w = ((uint64_t)z << RS_SHIFT0) | (uint64_t)(uint32_t)(i + 1);
+            words[i] = w;
+            for (int p = P0; p < 8; p++) atomicAdd(&h[p - P0][(w >> (8 * p)) & 255], 1u);
         }
         if (aos && i0 < n) {        // block-uniform: pack the chunk through LDS, write it contiguously
-            if (i < n)
+            if (i < n) {
                 for (int f = 0; f < g.nf; f++) recs[AOS_LD * t + f] = g.src[f][i + 1];
+                recs[AOS_LD * t + g.nf] = (uint64_t)__double_as_longlong(keysrc[i + 1]);
+            }
             __syncthreads();
             const int64_t cnt = n - i0 < RS_THREADS ? n - i0 : RS_THREADS;
             uint64_t *dst = aos + (int64_t)AOS_W * i0;
@@ -147,38 +196,33 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scan_kernel(uint32_t *__restric
     }
 }
 
-// Stable scatter of one 4096-key tile.  Wave w ranks the contiguous sub-tile
+// Stable scatter of one 4096-word tile.  Wave w ranks the contiguous sub-tile
 // [w*1024, (w+1)*1024) on its own (ballot matching of the 8 digit bits, a per-wave
 // digit counter in LDS updated by the highest lane of each match group), so the
 // item loop needs no block barrier; the (digit, wave) counts then give each wave's
 // offset inside the tile's digit runs.  Index order (wave, item, lane) is the tile's
 // index order, so equal digits keep their order.
-__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *__restrict__ keys_in,
-                                                                const uint32_t *__restrict__ vals_in,
-                                                                uint64_t *__restrict__ keys_out,
-                                                                uint32_t *__restrict__ vals_out, DevScalars *s,
+__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *__restrict__ words_in,
+                                                                uint64_t *__restrict__ words_out, DevScalars *s,
                                                                 int shift, const uint32_t *__restrict__ tile_off,
                                                                 int64_t ntiles)
 {
     constexpr int NW = RS_THREADS / 64;
-    __shared__ uint64_t skeys[RS_TILE];
-    __shared__ uint32_t svals[RS_TILE];
+    __shared__ uint64_t swords[RS_TILE];
     __shared__ uint32_t wcnt[NW][256];
-    __shared__ uint32_t dstart[256];       // 53 KB in all: three blocks per CU
+    __shared__ uint32_t dstart[256];       // 37 KB in all: four blocks per CU
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     const int64_t wbase = base + (int64_t)w * (64 * RS_ITEMS);
     for (int d = lane; d < 256; d += 64) wcnt[w][d] = 0;
     uint64_t k[RS_ITEMS];
-    uint32_t v[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     for (int u = 0; u < RS_ITEMS; u++) {
         int64_t i = wbase + u * 64 + lane;
         bool valid = i < n;
-        k[u] = valid ? keys_in[i] : 0ull;
-        v[u] = valid ? vals_in[i] : 0u;
+        k[u] = valid ? words_in[i] : 0ull;
         uint32_t d = (uint32_t)((k[u] >> shift) & 255);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -217,8 +261,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
         if (i < n) {
             uint32_t d = (uint32_t)((k[u] >> shift) & 255);
             uint32_t p = dstart[d] + wcnt[w][d] + rank[u];
-            skeys[p] = k[u];
-            svals[p] = v[u];
+            swords[p] = k[u];
         }
     }
     __syncthreads();
@@ -226,73 +269,65 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
     for (int u = 0; u < RS_ITEMS; u++) {
         int p = u * RS_THREADS + t;
         if (p < cnt_tile) {
-            uint64_t key = skeys[p];
-            uint32_t d = (uint32_t)((key >> shift) & 255);
+            uint64_t word = swords[p];
+            uint32_t d = (uint32_t)((word >> shift) & 255);
             int64_t g = (int64_t)tile_off[(int64_t)d * ntiles + blockIdx.x] + (p - (int64_t)dstart[d]);
-            keys_out[g] = key;
-            vals_out[g] = svals[p];
+            words_out[g] = word;
         }
     }
 }
 
-// Runs of keys equal in the top 40 bits (live stars only: the tail keys are all ones)
-// are put in (key, id) order.  Run starts sort runs of up to RUN_SHORT keys
-// themselves (insertion sort) and list longer ones for rs_long_runs_kernel.
+// Runs of words equal in z (live stars only: the tail has z = Z_TAIL) are put in
+// (r, id) order, r read back through the slot.  Run starts sort runs of up to
+// RUN_SHORT words themselves (insertion sort) and list longer ones for
+// rs_long_runs_kernel.
 __device__ __forceinline__ bool key_id_less(uint64_t ka, int64_t ia, uint64_t kb, int64_t ib)
 {
     return ka < kb || (ka == kb && ia < ib);
 }
 
-__global__ void rs_runs_kernel(uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+__global__ void rs_runs_kernel(uint64_t *__restrict__ words, const double *__restrict__ keysrc,
                                const int64_t *__restrict__ id, DevScalars *s, int64_t *__restrict__ long_runs,
                                int64_t *__restrict__ n_long)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max_new;
     if (i + 1 >= n) return;
-    const uint64_t k = keys[i];
-    if (k == ~0ull) return;
-    const uint64_t p = k >> RS_SHIFT0;
-    if ((keys[i + 1] >> RS_SHIFT0) != p) return;
-    if (i > 0 && (keys[i - 1] >> RS_SHIFT0) == p) return;
+    const uint32_t z = word_z(words[i]);
+    if (z == Z_TAIL) return;
+    if (word_z(words[i + 1]) != z) return;
+    if (i > 0 && word_z(words[i - 1]) == z) return;
     int64_t e = i + 1;
-    while (e + 1 < n && (keys[e + 1] >> RS_SHIFT0) == p && e - i < RUN_SHORT) e++;
-    if (e + 1 < n && (keys[e + 1] >> RS_SHIFT0) == p) {
+    while (e + 1 < n && word_z(words[e + 1]) == z && e - i < RUN_SHORT) e++;
+    if (e + 1 < n && word_z(words[e + 1]) == z) {
         unsigned long long slot = atomicAdd((unsigned long long *)n_long, 1ull);
         if ((int64_t)slot < SORT_LONG_RUNS) long_runs[slot] = i;
         else dev_set_error(s, CMCG_E_TIES, i + 1);
         return;
     }
     for (int64_t a = i + 1; a <= e; a++) {
-        const uint64_t ka = keys[a];
-        const uint32_t v = vals[a];
-        int64_t idv = 0;
-        bool have_id = false;            // ids are gathered only for equal keys
+        const uint64_t wa = words[a];
+        const uint32_t va = word_slot(wa);
+        const uint64_t ka = sortable(keysrc[va]);
+        const int64_t ida = id[va];
         int64_t b = a - 1;
         while (b >= i) {
-            const uint64_t kb = keys[b];
-            const uint32_t vb = vals[b];
-            bool less;
-            if (ka != kb) {
-                less = ka < kb;
-            } else {
-                if (!have_id) { idv = id[v]; have_id = true; }
-                less = idv < id[vb];
-            }
-            if (!less) break;
-            keys[b + 1] = kb;
-            vals[b + 1] = vb;
+            const uint64_t wb = words[b];
+            const uint32_t vb = word_slot(wb);
+            if (!key_id_less(ka, ida, sortable(keysrc[vb]), id[vb])) break;
+            words[b + 1] = wb;
             b--;
         }
-        keys[b + 1] = ka;
-        vals[b + 1] = v;
+        words[b + 1] = wa;
     }
 }
 
-// Long runs: one block per run (grid-stride over the list), bitonic sort by (key, id)
-// in LDS, padded to a power of two with all-ones entries.
+// Long runs: one block per run (grid-stride over the list), bitonic sort by (r, id)
+// in LDS, padded to a power of two with all-ones entries.  Every word of a run has
+// the run's z, so only the slots travel.
 constexpr int LR_T = 256;
-__global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+__global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict__ words,
+                                                            const double *__restrict__ keysrc,
                                                             const int64_t *__restrict__ id, DevScalars *s,
                                                             const int64_t *__restrict__ long_runs,
                                                             const int64_t *__restrict__ n_long)
@@ -306,16 +341,16 @@ __global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict
     const int t = threadIdx.x;
     for (int64_t r = blockIdx.x; r < nl; r += gridDim.x) {
         const int64_t i = long_runs[r];
+        const uint32_t z = word_z(words[i]);
         if (t == 0) {
-            // keys are ordered by prefix: the last index with the run's prefix by bisection
-            const uint64_t p = keys[i] >> RS_SHIFT0;
+            // words are ordered by z: the last index with the run's z by bisection
             int64_t lo = i, hi = i + RUN_LONG < n ? i + RUN_LONG : n - 1;
-            if ((keys[hi] >> RS_SHIFT0) == p) {
-                len_s = hi == i + RUN_LONG ? -1 : hi - i + 1;     // RUN_LONG + 1 keys do not fit
+            if (word_z(words[hi]) == z) {
+                len_s = hi == i + RUN_LONG ? -1 : hi - i + 1;     // RUN_LONG + 1 words do not fit
             } else {
                 while (hi - lo > 1) {
                     const int64_t m = lo + ((hi - lo) >> 1);
-                    if ((keys[m] >> RS_SHIFT0) == p) lo = m;
+                    if (word_z(words[m]) == z) lo = m;
                     else hi = m;
                 }
                 len_s = lo - i + 1;
@@ -330,8 +365,8 @@ __global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict
         while (P < len) P <<= 1;
         for (int q = t; q < P; q += LR_T) {
             if (q < len) {
-                const uint32_t v = vals[i + q];
-                sk[q] = keys[i + q];
+                const uint32_t v = word_slot(words[i + q]);
+                sk[q] = sortable(keysrc[v]);
                 sv[q] = v;
                 sid[q] = id[v];
             } else {
@@ -357,28 +392,27 @@ __global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict
                 }
                 __syncthreads();
             }
-        for (int q = t; q < len; q += LR_T) {
-            keys[i + q] = sk[q];
-            vals[i + q] = sv[q];
-        }
+        for (int q = t; q < len; q += LR_T) words[i + q] = ((uint64_t)z << RS_SHIFT0) | sv[q];
         __syncthreads();
     }
 }
 
-// number of live stars = keys below the SF_INFINITY key
-__global__ void rs_count_kernel(const uint64_t *__restrict__ keys, DevScalars *s, uint64_t kinf)
+// number of live stars = words below the tail
+__global__ void rs_count_kernel(const uint64_t *__restrict__ words, DevScalars *s)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max_new;
     if (i > n) return;
-    bool below_prev = (i == 0) ? true : keys[i - 1] < kinf;
-    bool below_here = (i == n) ? false : keys[i] < kinf;
+    bool below_prev = (i == 0) ? true : word_z(words[i - 1]) != Z_TAIL;
+    bool below_here = (i == n) ? false : word_z(words[i]) != Z_TAIL;
     if (below_prev && !below_here) s->sort_n = i;
 }
 
-__global__ __launch_bounds__(256) void rs_gather_kernel(const uint64_t *__restrict__ keys,
-                                                        const uint32_t *__restrict__ vals, DevScalars *s,
-                                                        double *__restrict__ r_out, GatherList g)
+// The sorted permutation goes to perm (slot of the star now at position i), r and the
+// payload fields to their destinations.
+__global__ __launch_bounds__(256) void rs_gather_kernel(const uint64_t *__restrict__ words, DevScalars *s,
+                                                        const double *__restrict__ keysrc, double *__restrict__ r_out,
+                                                        uint32_t *__restrict__ perm, GatherList g)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = s->sort_n;
@@ -387,14 +421,14 @@ __global__ __launch_bounds__(256) void rs_gather_kernel(const uint64_t *__restri
         r_out[n + 1] = CMCG_SF_INFINITY;
         return;
     }
-    uint32_t src = vals[i];
-    r_out[i + 1] = unsortable(keys[i]);
+    const uint32_t src = word_slot(words[i]);
+    perm[i] = src;
+    r_out[i + 1] = keysrc[src];
     for (int f = 0; f < g.nf; f++) g.dst[f][i + 1] = g.src[f][src];
 }
 
-__global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__restrict__ keys,
-                                                            const uint32_t *__restrict__ vals, DevScalars *s,
-                                                            double *__restrict__ r_out,
+__global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__restrict__ words, DevScalars *s,
+                                                            double *__restrict__ r_out, uint32_t *__restrict__ perm,
                                                             const uint64_t *__restrict__ aos, GatherList g)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -404,19 +438,21 @@ __global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__re
         r_out[n + 1] = CMCG_SF_INFINITY;
         return;
     }
-    uint32_t src = vals[i];
-    r_out[i + 1] = unsortable(keys[i]);
+    const uint32_t src = word_slot(words[i]);
+    perm[i] = src;
     const ulonglong2 *rec = (const ulonglong2 *)(aos + (int64_t)AOS_W * (src - 1));
     uint64_t w[AOS_W];
 #pragma unroll
     for (int q = 0; q < AOS_W / 2; q++) {
-        ulonglong2 v = 2 * q < g.nf ? rec[q] : ulonglong2{0ull, 0ull};
+        ulonglong2 v = 2 * q <= g.nf ? rec[q] : ulonglong2{0ull, 0ull};    // fields, then r in word nf
         w[2 * q] = v.x;
         w[2 * q + 1] = v.y;
     }
 #pragma unroll
-    for (int f = 0; f < AOS_W; f++)
+    for (int f = 0; f < AOS_W; f++) {
         if (f < g.nf) g.dst[f][i + 1] = w[f];
+        if (f == g.nf) r_out[i + 1] = __longlong_as_double((long long)w[f]);
+    }
 }
 
 __global__ void rs_finish_kernel(DevScalars *s)
@@ -425,11 +461,13 @@ __global__ void rs_finish_kernel(DevScalars *s)
     s->n_max_new = s->sort_n;
 }
 
-__global__ void rs_begin_kernel(DevScalars *s, int64_t *n_long)
+__global__ void rs_begin_kernel(DevScalars *s, int64_t *n_long, unsigned long long *mm)
 {
     s->sort_n = 0;
     s->sort_in = s->n_max_new;
     *n_long = 0;
+    mm[0] = ~0ull;
+    mm[1] = 0ull;
 }
 } // namespace
 
@@ -444,42 +482,43 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     g.nf = nf;
     for (int f = 0; f < nf; f++) { g.src[f] = src[f]; g.dst[f] = dst[f]; }
     for (int f = nf; f < RS_MAX_FIELDS; f++) { g.src[f] = nullptr; g.dst[f] = nullptr; }
-    uint64_t *aos = (nf <= AOS_W && sc.sort_aos && n_ub <= sc.sort_aos_cap) ? sc.sort_aos : nullptr;
+    // packed records hold the fields and r after them
+    uint64_t *aos = (nf < AOS_W && sc.sort_aos && n_ub <= sc.sort_aos_cap) ? sc.sort_aos : nullptr;
     int64_t ntiles = (n_ub + RS_TILE - 1) / RS_TILE;
     if (ntiles < 1) ntiles = 1;
     if (ntiles > sc.max_sort_tiles) return hipErrorInvalidValue;
     const uint64_t kinf = 0x8000000000000000ull | 0x4202a05f20000000ull;   // sortable(1.0e10)
-    hipLaunchKernelGGL(rs_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sort_nlong);
+    unsigned long long *mm = (unsigned long long *)sc.sort_mm;
+    hipLaunchKernelGGL(rs_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sort_nlong, mm);
     hipLaunchKernelGGL(rs_zero_kernel, dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256));
-    // the passes alternate between the two buffer pairs; start so that the sorted keys
-    // and the permutation end in keys_a / vals_a (set_velocities3 reads vals_a)
-    constexpr bool odd = RS_PASSES % 2 == 1;
-    uint64_t *kin = odd ? sc.keys_b : sc.keys_a, *kout = odd ? sc.keys_a : sc.keys_b;
-    uint32_t *vin = odd ? sc.vals_b : sc.vals_a, *vout = odd ? sc.vals_a : sc.vals_b;
-    hipLaunchKernelGGL(rs_prep_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin, vin,
-                       sc.digit_totals, aos, g, kinf);
+    const unsigned gm = (unsigned)(ntiles * 4 < 2048 ? ntiles * 4 : 2048);
+    hipLaunchKernelGGL(rs_minmax_kernel, dim3(gm), dim3(256), 0, st, key_src, s, kinf, mm);
+    // an even number of passes: the sorted words end in keys_a
+    uint64_t *kin = sc.keys_a, *kout = sc.keys_b;
+    hipLaunchKernelGGL(rs_prep_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
+                       sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm);
     for (int pass = 8 - RS_PASSES; pass < 8; pass++) {
         int shift = 8 * pass;
         hipLaunchKernelGGL(rs_hist_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, s, shift,
                            sc.block_hist, ntiles);
         hipLaunchKernelGGL(rs_scan_kernel, dim3(256), dim3(RS_THREADS), 0, st, sc.block_hist, ntiles,
                            (const uint32_t *)(sc.digit_totals + 256 * pass));
-        hipLaunchKernelGGL(rs_scatter_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, vin, kout, vout,
-                           s, shift, (const uint32_t *)sc.block_hist, ntiles);
+        hipLaunchKernelGGL(rs_scatter_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, kout, s, shift,
+                           (const uint32_t *)sc.block_hist, ntiles);
         uint64_t *tk = kin; kin = kout; kout = tk;
-        uint32_t *tv = vin; vin = vout; vout = tv;
     }
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
-    hipLaunchKernelGGL(rs_runs_kernel, dim3(g1), dim3(256), 0, st, kin, vin, id_src, s, sc.sort_long, sc.sort_nlong);
-    hipLaunchKernelGGL(rs_long_runs_kernel, dim3(64), dim3(LR_T), 0, st, kin, vin, id_src, s,
+    hipLaunchKernelGGL(rs_runs_kernel, dim3(g1), dim3(256), 0, st, kin, key_src, id_src, s, sc.sort_long,
+                       sc.sort_nlong);
+    hipLaunchKernelGGL(rs_long_runs_kernel, dim3(64), dim3(LR_T), 0, st, kin, key_src, id_src, s,
                        (const int64_t *)sc.sort_long, (const int64_t *)sc.sort_nlong);
-    hipLaunchKernelGGL(rs_count_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, kinf);
+    hipLaunchKernelGGL(rs_count_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s);
     if (aos)
-        hipLaunchKernelGGL(rs_gather_aos_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin,
-                           (const uint32_t *)vin, s, r_dst, (const uint64_t *)aos, g);
+        hipLaunchKernelGGL(rs_gather_aos_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
+                           sc.vals_a, (const uint64_t *)aos, g);
     else
-        hipLaunchKernelGGL(rs_gather_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, (const uint32_t *)vin,
-                           s, r_dst, g);
+        hipLaunchKernelGGL(rs_gather_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, key_src, r_dst,
+                           sc.vals_a, g);
     hipLaunchKernelGGL(rs_finish_kernel, dim3(1), dim3(1), 0, st, s);
     return hipGetLastError();
 }

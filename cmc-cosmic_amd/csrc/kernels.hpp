@@ -37,8 +37,9 @@ struct Scratch {
     double *partials;
     int64_t max_blocks;
     // sort
-    uint64_t *keys_a, *keys_b;
-    uint32_t *vals_a, *vals_b;
+    uint64_t *keys_a, *keys_b;  // sort words: 32-bit normalised key of r above the 32-bit slot
+    uint32_t *vals_a;           // sorted permutation: slot of the star now at each position
+    uint64_t *sort_mm;          // live key minimum and maximum
     uint32_t *block_hist;       // 256 * max_sort_tiles
     uint32_t *digit_totals;     // 8 * 256
     int64_t max_sort_tiles;

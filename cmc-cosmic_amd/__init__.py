@@ -172,6 +172,7 @@ _sigs = {
 }
 MLP_NPARAM, MLP_NNORM = 4738, 16
 HOOK_CHAIN_K_DROP = 1
+HOOK_ORBIT_VARIANT = 2
 for _name, _args in _sigs.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args

@@ -1346,6 +1346,11 @@ int cmcg_test_hook(cmcg_ctx *c, int32_t hook, int64_t value)
         c->kp.chain_k_drop = (int32_t)value;
         invalidate_graphs(c);
         return CMCG_OK;
+    case CMCG_HOOK_ORBIT_VARIANT:
+        if (value < 0 || value >= orbit_variant_count()) return CMCG_E_ARG;
+        c->orb_variant = (int)value;
+        invalidate_graphs(c);
+        return CMCG_OK;
     default:
         return CMCG_E_ARG;
     }

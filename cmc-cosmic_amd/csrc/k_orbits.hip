@@ -631,7 +631,9 @@ __global__ void coarse_build_kernel(const double2 *__restrict__ nodes, const Dev
 // Launch shapes of the orbit sampler that the autotuner chooses from: threads per
 // block, LDS samples of the (r, phi) table, and the most lanes one star's tries get
 // in a round (64: all idle lanes; fewer: less speculative work).  Every variant gives
-// the same stars; only the block order of the escape sums differs.
+// the same stars; only the block order of the escape sums differs.  (Asking the
+// compiler for 5 waves per SIMD instead of 4 spills about 100 VGPRs and ran 40%
+// slower.)
 using OrbFn = void (*)(DevFields, DevScalars *, KParams, const RngTable *, const Rng *, double *, int64_t *,
                        const float *, const MlpNorm *, float, const double2 *, const double2 *, const double2 *,
                        const NibbleTables<ORB_NIB_POWERS> *);

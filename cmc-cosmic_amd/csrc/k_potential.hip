@@ -290,16 +290,21 @@ __device__ __forceinline__ double err_bound(int64_t n, double vmax, double a, in
     return ((double)n * (ldexp(vmax, -51) + ldexp(1.0, 1 - K)) + ldexp(a, -50)) * 1.001;
 }
 
-// regular increment of element j in 2^-K units, or sensitive
-__device__ __forceinline__ bool classify(int64_t j, double x, double sign, double v0, i128 Pp, i128 Pq, int K,
-                                         i128 *R)
+// Binade of the chain value before element j (after elements 0..j-1, fixed-point
+// prefix Pp), or INT32_MIN when its error interval straddles binades; before
+// element 0 the value is v0 exactly.
+__device__ __forceinline__ int start_binade(int64_t j, double v0, i128 Pp, int K)
 {
-    double av0 = fabs(v0);
-    double p = j == 0 ? v0 : unfxk(Pp, K);
-    double q = unfxk(Pq, K);
-    double ep = j == 0 ? 0.0 : err_bound(j, fmax(av0, fabs(p)), fabs(p), K);
-    double eq = err_bound(j + 1, fmax(av0, fabs(q)), fabs(q), K);
-    int bp = interval_binade(p, ep), bq = interval_binade(q, eq);
+    if (j == 0) return interval_binade(v0, 0.0);
+    const double p = unfxk(Pp, K);
+    return interval_binade(p, err_bound(j, fmax(fabs(v0), fabs(p)), fabs(p), K));
+}
+
+// regular increment of element j in 2^-K units, or sensitive: bp, bq are the binades
+// before and after the element (bq = start_binade(j + 1, ...), so a thread carries
+// it on to its next element)
+__device__ __forceinline__ bool classify(double x, double sign, int bp, int bq, int K, i128 *R)
+{
     if (bp == INT32_MIN || bp != bq || bp < 52 - K || bp > 52 + 60 - K) return true;
     double y = ldexp(sign * x, 52 - bp);         // exact: power-of-two scaling
     double fy = floor(y);
@@ -331,11 +336,18 @@ __device__ void tile_classify(const EsaArgs &a, int64_t tile, int64_t L, double 
     i128 tot;
     i128 P = fxk(v0, a.K) + a.tpreP[tile] + block_excl_scan<CT>(acc, lds, &tot);
     it.sens = 0;
+    int bp = start_binade(j0, v0, P, a.K);
     for (int u = 0; u < CI; u++) {
         int64_t j = j0 + u;
         i128 Pq = P + fxs[u];
         it.R[u] = 0;
-        if (j >= L || classify(j, it.xv[u], a.sign, v0, P, Pq, a.K, &it.R[u])) {
+        bool sens = j >= L;
+        if (!sens) {
+            const int bq = start_binade(j + 1, v0, Pq, a.K);
+            sens = classify(it.xv[u], a.sign, bp, bq, a.K, &it.R[u]);
+            bp = bq;
+        }
+        if (sens) {
             it.sens |= 1u << u;
             it.R[u] = 0;
         }

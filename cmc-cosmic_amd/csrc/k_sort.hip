@@ -131,6 +131,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
             __syncthreads();
             const int64_t cnt = n - i0 < RS_THREADS ? n - i0 : RS_THREADS;
             uint64_t *dst = aos + (int64_t)AOS_W * i0;
+            // whole 128-byte lines (unused words included): partial-line writes cost more
             for (int w = t; w < cnt * AOS_W; w += RS_THREADS) dst[w] = recs[AOS_LD * (w / AOS_W) + w % AOS_W];
             __syncthreads();
         }

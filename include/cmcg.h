@@ -267,6 +267,9 @@ int cmcg_central_calculate(cmcg_ctx *ctx, cmcg_central *out);
  * paths (list overflow, repair round, sequential completion); the potential must
  * still equal the reference's recurrence bit for bit, only slower. */
 #define CMCG_HOOK_CHAIN_K_DROP 1
+/* CMCG_HOOK_ORBIT_VARIANT: run the orbit sampler with launch variant `value`
+ * (0 .. count-1 of cmcg_orbit_variants), as cmcg_autotune would choose it. */
+#define CMCG_HOOK_ORBIT_VARIANT 2
 int cmcg_test_hook(cmcg_ctx *ctx, int32_t hook, int64_t value);
 
 int cmcg_orbit_variants(cmcg_ctx *ctx, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse,

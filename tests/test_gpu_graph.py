@@ -99,3 +99,35 @@ def test_gpu_autotuned_shape_gives_identical_stars(gpu):
     finally:
         a.close()
         b.close()
+
+
+def test_gpu_every_orbit_variant_gives_identical_stars(gpu):
+    """Every launch variant of the orbit sampler (threads per block, LDS samples, tries
+    cap), run from the same state, writes bit-identical new radii,
+    velocities and apsides."""
+    cm = gpu
+    say("all orbit variants, King W0=7 N=3e5")
+    cl = cm.make_king(300000, 7.0, seed=6)
+    with cm.Context(cl.n, cm.default_params(cl.n, orbit_r=cl.rtid, seed=2)) as ctx:
+        ctx.upload(cl)
+        ctx.init_cluster()
+        ctx.energy_conservation1()
+        state = ctx.download()
+        sc = ctx.scalars()
+        nvar = len(ctx.orbit_variants())
+        ref = None
+        for v in range(nvar):
+            ctx.upload(state)
+            ctx.set_scalars(**{k: sc[k] for k in ("n_max", "n_max_new", "Mtotal", "Rtidal", "Dt", "TotalTime")})
+            ctx.test_hook(cm.HOOK_ORBIT_VARIANT, v)
+            st = ctx.orbits(step=3)
+            d = ctx.download()
+            say("variant", v, "tries", st["tries_total"], "removed", st["n_removed_E"])
+            if ref is None:
+                ref = (d, st)
+                continue
+            n = sc["n_max"]
+            for k in ("rnew", "vrnew", "vtnew", "r_peri", "r_apo", "ntry", "flags"):
+                assert np.array_equal(d[k][1:n + 1], ref[0][k][1:n + 1]), (v, k)
+        with pytest.raises(cm.CmcgError):
+            ctx.test_hook(cm.HOOK_ORBIT_VARIANT, nvar)

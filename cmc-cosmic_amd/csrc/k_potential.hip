@@ -644,13 +644,6 @@ __global__ void sentinel_kernel(double *__restrict__ r, double *__restrict__ phi
     phi[n + 1] = 0.0;
 }
 
-__global__ void prep_a_kernel(const double *__restrict__ m, DevScalars *s, double *__restrict__ x)
-{
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t n = s->n_max;
-    if (j < n) x[j] = m[j + 1];
-}
-
 // after chain A: Mtotal, Rtidal (cmc_utils.c:703-707), NaN check (:688-691)
 __global__ void total_kernel(DevScalars *s, KParams P, const double *__restrict__ ga, PotWork w)
 {
@@ -668,12 +661,6 @@ __global__ void prep_b_kernel(const double *__restrict__ m, DevScalars *s, doubl
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
     if (j < n - 1) x[j] = m[n - j] / nstar;           // star_m[k] / clus.N_STAR, k = N - j
-}
-
-__global__ void save_mk_kernel(DevScalars *s, const double *__restrict__ g, double *__restrict__ Mk)
-{
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < s->n_max - 1) Mk[j] = g[j];
 }
 
 // p_k = M_k * (1/r_k - 1/r_{k+1}) with the reference's roundings; chain C runs on
@@ -881,24 +868,22 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     if (n_ub > 0) hipLaunchKernelGGL(pot_count_kernel, dim3(g1), dim3(256), 0, st, f.r, s);
     hipLaunchKernelGGL(pot_setup_kernel, dim3(1), dim3(1), 0, st, s, w);
     hipLaunchKernelGGL(sentinel_kernel, dim3(1), dim3(1), 0, st, f.r, f.phi, s);
-    // chain A: total mass
-    hipLaunchKernelGGL(prep_a_kernel, dim3(g1), dim3(256), 0, st, f.m, s, w.xa);
+    // chain A: total mass, straight over star_m[1..N]
     // fixed-point scales 2^-70 (stored masses are O(1)) and 2^-100; a test hook may
     // coarsen them, which sends most elements through the fallback paths
     const int KA = 70 - P.chain_k_drop < 8 ? 8 : 70 - P.chain_k_drop;
     const int KB = 100 - P.chain_k_drop < 8 ? 8 : 100 - P.chain_k_drop;
-    ChainArgs a{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 0, w.tsum, w.tpre, w.bnd, w.len + 0, KA};
+    ChainArgs a{f.m + 1, w.ga, 1.0, w.scal + 0, w.fbad + 0, w.tsum, w.tpre, w.bnd, w.len + 0, KA};
     EsaArgs ae = e;
-    ae.x = w.xa; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0; ae.K = KA;
+    ae.x = f.m + 1; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0; ae.K = KA;
     if (hipError_t err = run_chain(a, ae, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(total_kernel, dim3(1), dim3(1), 0, st, s, P, (const double *)w.ga, w);
-    // chain B: enclosed mass
+    // chain B: enclosed mass, iterated directly in Mk (chain C's input)
     hipLaunchKernelGGL(prep_b_kernel, dim3(g1), dim3(256), 0, st, f.m, s, (double)P.n_star, w.xa);
-    ChainArgs b{w.xa, w.ga, -1.0, w.scal + 1, w.fbad + 2, w.tsum, w.tpre, w.bnd, w.len + 1, KB};
+    ChainArgs b{w.xa, w.Mk, -1.0, w.scal + 1, w.fbad + 2, w.tsum, w.tpre, w.bnd, w.len + 1, KB};
     EsaArgs be = e;
-    be.x = w.xa; be.g = w.ga; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1; be.K = KB;
+    be.x = w.xa; be.g = w.Mk; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1; be.K = KB;
     if (hipError_t err = run_chain(b, be, REPAIR_ROUNDS, n_ub, st)) return err;
-    hipLaunchKernelGGL(save_mk_kernel, dim3(g1), dim3(256), 0, st, s, (const double *)w.ga, w.Mk);
     // chain C: w = -phi
     hipLaunchKernelGGL(prep_c_kernel, dim3(g1), dim3(256), 0, st, f.r, s, (const double *)w.Mk, w.xa);
     ChainArgs c{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 4, w.tsum, w.tpre, w.bnd, w.len + 2, KB};

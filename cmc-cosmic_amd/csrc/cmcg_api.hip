@@ -49,6 +49,22 @@ struct cmcg_ctx {
     void *cen_buf = nullptr;     // cmcg_central_calculate results (allocated on first use)
 };
 
+// t[b][c][q][v] = XOR of the columns of matrix power b0 + b selected by nibble v at
+// bits 4q..4q+3 of state word c (the device's build_nibble_tables, done once here)
+template <int NB>
+static void fill_nibble_tables(const RngTable &ht, int b0, NibbleTables<NB> &nt)
+{
+    for (int b = 0; b < NB; b++)
+        for (int cc = 0; cc < 4; cc++)
+            for (int q = 0; q < 8; q++)
+                for (int v = 0; v < 16; v++) {
+                    uint32_t o = 0;
+                    for (int i = 0; i < 4; i++)
+                        if ((v >> i) & 1) o ^= ht.col[b0 + b][cc][4 * q + i];
+                    nt.t[b][cc][q][v] = o;
+                }
+}
+
 static void invalidate_graphs(cmcg_ctx *c)
 {
     for (auto &g : c->graph)
@@ -354,7 +370,8 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)SORT_LONG_RUNS)) ||
         (e = dalloc(c, &sc.sort_nlong, 1)) ||
         (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
-        (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
+        (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) ||
+        (e = dalloc(c, (char **)&sc.relax_nib, sizeof(NibbleTables<8>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_mm, 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
@@ -376,18 +393,15 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     // the orbit sampler's per-lane jumps by l 2^20 (l < 64): nibble tables of the
     // matrix powers 2^20 .. 2^25, t[b][c][q][v] = column XOR for nibble v at bits 4q..
     auto *nib = new NibbleTables<ORB_NIB_POWERS>;
-    for (int b = 0; b < ORB_NIB_POWERS; b++)
-        for (int cc = 0; cc < 4; cc++)
-            for (int q = 0; q < 8; q++)
-                for (int v = 0; v < 16; v++) {
-                    uint32_t o = 0;
-                    for (int i = 0; i < 4; i++)
-                        if ((v >> i) & 1) o ^= ht->col[ORB_NIB_B0 + b][cc][4 * q + i];
-                    nib->t[b][cc][q][v] = o;
-                }
+    fill_nibble_tables(*ht, ORB_NIB_B0, *nib);
+    // the relaxation's per-lane jumps by 3 l (l < 64): matrix powers 2^0 .. 2^7
+    auto *rnib = new NibbleTables<8>;
+    fill_nibble_tables(*ht, 0, *rnib);
     bool ok = hipMemcpy(c->T, ht, sizeof(RngTable), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(sc.rng_seeded, &c->seeded, sizeof(Rng), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(sc.orb_nib, nib, sizeof(*nib), hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(sc.orb_nib, nib, sizeof(*nib), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(sc.relax_nib, rnib, sizeof(*rnib), hipMemcpyHostToDevice) == hipSuccess;
+    delete rnib;
     delete nib;
     delete ht;
     if (!ok) { cmcg_destroy(c); return CMCG_E_HIP; }

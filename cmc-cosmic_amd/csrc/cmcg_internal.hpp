@@ -137,6 +137,18 @@ __device__ void build_nibble_tables(const RngTable *__restrict__ T, int b0, Nibb
     __syncthreads();
 }
 
+// copy of tables built once per context (same contents as build_nibble_tables): 16-byte
+// loads, no column XORs per block
+template <int NB>
+__device__ void copy_nibble_tables(const NibbleTables<NB> *__restrict__ src, NibbleTables<NB> &lt)
+{
+    constexpr int NQ = (int)(sizeof(NibbleTables<NB>) / sizeof(uint4));
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    uint4 *d4 = reinterpret_cast<uint4 *>(&lt);
+    for (int q = threadIdx.x; q < NQ; q += blockDim.x) d4[q] = s4[q];
+    __syncthreads();
+}
+
 __device__ __forceinline__ uint32_t nib_matvec(const uint32_t (&tab)[8][16], uint32_t v)
 {
     uint32_t o = 0;

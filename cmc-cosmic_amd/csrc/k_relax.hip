@@ -98,8 +98,10 @@ __global__ void dtrel_init_kernel(DevScalars *s) { s->dtrel = __longlong_as_doub
 // wave) stages its windows' contiguous range in LDS with coalesced loads, as the
 // per-element terms m*madhoc and m*madhoc*(vr^2 + vt^2) (the same roundings as the
 // reference's loop); windows of more than SR_LDS elements read global memory.
-constexpr int SR_T = 64, SR_LDS = 2560;
-__global__ __launch_bounds__(SR_T) void simul_relax_kernel(const double *__restrict__ r, const double *__restrict__ m,
+// SR_T windows at most per block, one per lane of the first wave; all SR_B threads
+// stage the block's elements (more loads in flight than one wave issues alone).
+constexpr int SR_T = 64, SR_B = 256, SR_LDS = 2560;
+__global__ __launch_bounds__(SR_B) void simul_relax_kernel(const double *__restrict__ r, const double *__restrict__ m,
                                                            const double *__restrict__ vr,
                                                            const double *__restrict__ vt, DevScalars *s, KParams P)
 {
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(SR_T) void simul_relax_kernel(const double *__restr
     const int64_t w0 = (int64_t)blockIdx.x * G;
     const int64_t e0 = 1 + w0 * len;
     if (staged) {
-        for (int64_t q = threadIdx.x; q < G * len; q += SR_T) {
+        for (int64_t q = threadIdx.x; q < G * len; q += SR_B) {
             const int64_t k = e0 + q;
             if (k <= N) {
                 const double tmp = m[k] * P.madhoc;
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(SR_T) void simul_relax_kernel(const double *__restr
         unsigned long long other = __shfl_xor(mine, o, 64);
         mine = other < mine ? other : mine;
     }
-    if (threadIdx.x == 0 && mine != kInf) atomicMin((unsigned long long *)&s->dtrel, mine);
+    if (threadIdx.x == 0 && mine != kInf) atomicMin((unsigned long long *)&s->dtrel, mine);   // windows: wave 0 only
 }
 
 // calc_timestep (cmc_utils.c:2132-2148) and GetTimeStep's fallback (cmc_evolution_thr.c:430-432)
@@ -237,10 +239,11 @@ __global__ void sigma_kernel(const double *__restrict__ m, const double *__restr
 // Pair pr draws from offset 3 pr; a wave's first pair starts at the per-wave state
 // and lane l jumps 3 l < 2^8 steps through the LDS tables.
 __global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, KParams P,
-                                                    const RngTable *__restrict__ T, const Rng *__restrict__ wave_base)
+                                                    const Rng *__restrict__ wave_base,
+                                                    const NibbleTables<8> *__restrict__ nib)
 {
     __shared__ NibbleTables<8> lt;
-    build_nibble_tables<8>(T, 0, lt);
+    copy_nibble_tables<8>(nib, lt);
     int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t N = s->n_max;
     if (pr >= N / 2) return;            // pairs while si <= N - N%2 - 1 (cmc_dynamics.c:203)
@@ -380,7 +383,7 @@ hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &
     const int64_t nw = n_ub / (2 * P.avekernel) + 1;
     const int64_t len = 2 * P.avekernel;
     const int64_t G = len <= SR_LDS ? (SR_LDS / len < SR_T ? SR_LDS / len : SR_T) : SR_T;   // as in the kernel
-    hipLaunchKernelGGL(simul_relax_kernel, dim3((unsigned)((nw + G - 1) / G)), dim3(SR_T), 0, st, f.r, f.m, f.vr,
+    hipLaunchKernelGGL(simul_relax_kernel, dim3((unsigned)((nw + G - 1) / G)), dim3(SR_B), 0, st, f.r, f.m, f.vr,
                        f.vt, s, P);
     return hipGetLastError();
 }
@@ -397,7 +400,7 @@ hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, con
     int64_t npairs = n_ub / 2;
     if (npairs <= 0) return hipSuccess;
     if (hipError_t e = launch_rng_wave_bases(T, sc, 0, npairs, st)) return e;
-    hipLaunchKernelGGL(relax_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P, T,
-                       (const Rng *)sc.rng_wave);
+    hipLaunchKernelGGL(relax_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P,
+                       (const Rng *)sc.rng_wave, (const NibbleTables<8> *)sc.relax_nib);
     return hipGetLastError();
 }

@@ -51,6 +51,7 @@ struct Scratch {
     double *u_old;              // new potential at the old sorted radii (set_velocities3)
     double2 *coarse;            // orbit sampler's (r, phi) samples, copied into LDS by every block
     void *orb_nib;              // orbit sampler's GF(2) nibble tables (constant per context)
+    void *relax_nib;            // relaxation's nibble tables, powers 2^0 .. 2^7 (constant per context)
     double *red2;               // second level of the block-partial reductions (RED2_BLOCKS x 16)
     int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event

@@ -371,7 +371,8 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.sort_nlong, 1)) ||
         (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
         (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) ||
-        (e = dalloc(c, (char **)&sc.relax_nib, sizeof(NibbleTables<8>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
+        (e = dalloc(c, (char **)&sc.relax_nib, sizeof(NibbleTables<8>))) ||
+        (e = dalloc(c, (char **)&sc.wb_nib, 2 * sizeof(NibbleTables<RNG_WB_POWERS>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
         (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_mm, 4)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
@@ -397,10 +398,16 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     // the relaxation's per-lane jumps by 3 l (l < 64): matrix powers 2^0 .. 2^7
     auto *rnib = new NibbleTables<8>;
     fill_nibble_tables(*ht, 0, *rnib);
+    // the wave-base jumps (offsets 3 t 2^6 and t 2^26 within a block)
+    auto *wnib = new NibbleTables<RNG_WB_POWERS>[2];
+    fill_nibble_tables(*ht, 6, wnib[0]);
+    fill_nibble_tables(*ht, 26, wnib[1]);
     bool ok = hipMemcpy(c->T, ht, sizeof(RngTable), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(sc.rng_seeded, &c->seeded, sizeof(Rng), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(sc.orb_nib, nib, sizeof(*nib), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(sc.relax_nib, rnib, sizeof(*rnib), hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(sc.relax_nib, rnib, sizeof(*rnib), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(sc.wb_nib, wnib, 2 * sizeof(NibbleTables<RNG_WB_POWERS>), hipMemcpyHostToDevice) == hipSuccess;
+    delete[] wnib;
     delete rnib;
     delete nib;
     delete ht;

@@ -58,16 +58,17 @@ __global__ void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s, 
 // splits into a block part, jumped once per block from the global tables, and
 // 3 t 2^6 (phase 0) or t 2^26 (phase 1), jumped through LDS nibble tables of
 // WB_NB consecutive powers; jumps by powers of one matrix commute.
-constexpr int WB_T = 256, WB_NB = 10;
+constexpr int WB_T = 256, WB_NB = RNG_WB_POWERS;
 __global__ __launch_bounds__(WB_T) void rng_wave_bases_kernel(const RngTable *__restrict__ T,
                                                               const Rng *__restrict__ base, int phase,
-                                                              int64_t nwaves, Rng *__restrict__ out, int jb1)
+                                                              int64_t nwaves, Rng *__restrict__ out, int jb1,
+                                                              const NibbleTables<WB_NB> *__restrict__ nib)
 {
     __shared__ NibbleTables<WB_NB> lt;
     __shared__ Rng bb;
     const int t = threadIdx.x;
     const uint64_t b = blockIdx.x;
-    build_nibble_tables<WB_NB>(T, phase == 0 ? 6 : 26, lt);
+    copy_nibble_tables<WB_NB>(nib, lt);     // powers 2^6.. (phase 0) or 2^26.. (phase 1), built on the host
     if (t == 0)
         bb = phase == 0 ? rng_jump(T, base[0], 49152ull * b, 0, jb1)
                         : rng_jump(T, base[1], (1ull << 20) + (b << 34), 20, jb1);
@@ -330,8 +331,9 @@ hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase
     if (nwaves <= 0) return hipSuccess;
     if (nwaves > sc.max_waves) return hipErrorInvalidValue;
     int jb1 = phase == 0 ? bits_needed(192ull * (uint64_t)nwaves) : 20 + bits_needed((uint64_t)(64 * nwaves + 1));
+    const NibbleTables<WB_NB> *nib = (const NibbleTables<WB_NB> *)sc.wb_nib + (phase == 0 ? 0 : 1);
     hipLaunchKernelGGL(rng_wave_bases_kernel, dim3((unsigned)((nwaves + WB_T - 1) / WB_T)), dim3(WB_T), 0, st, T,
-                       (const Rng *)sc.rng_base, phase, nwaves, sc.rng_wave, jb1);
+                       (const Rng *)sc.rng_base, phase, nwaves, sc.rng_wave, jb1, nib);
     return hipGetLastError();
 }
 

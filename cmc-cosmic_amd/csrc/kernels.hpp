@@ -16,6 +16,8 @@ constexpr int SORT_LONG_RUNS = 4096;
 constexpr int RED2_BLOCKS = 64;
 constexpr int ORB_NIB_B0 = 20, ORB_NIB_POWERS = 6;
 
+constexpr int RNG_WB_POWERS = 10;   // nibble-table powers of the wave-base jumps
+
 struct Scratch {
     // potential chains: per-tile int128 sums and prefixes plus a boundary value
     // (48 B per tile), chain increments / iterates / enclosed masses per star
@@ -52,6 +54,7 @@ struct Scratch {
     double2 *coarse;            // orbit sampler's (r, phi) samples, copied into LDS by every block
     void *orb_nib;              // orbit sampler's GF(2) nibble tables (constant per context)
     void *relax_nib;            // relaxation's nibble tables, powers 2^0 .. 2^7 (constant per context)
+    void *wb_nib;               // wave-base nibble tables: powers 2^6 .. 2^15 and 2^26 .. 2^35
     double *red2;               // second level of the block-partial reductions (RED2_BLOCKS x 16)
     int64_t sort_aos_cap;
     // set_velocities3 excess chain: per-star excess, ordered event list, per-event

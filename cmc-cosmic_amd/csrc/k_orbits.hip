@@ -554,22 +554,25 @@ __global__ __launch_bounds__(RED2_BLOCKS * RED_STRIDE) void orbits_end_kernel(De
                                                                               const double *__restrict__ dpartials)
 {
     __shared__ double sv[RED2_BLOCKS * RED_STRIDE];
+    __shared__ double v[NRED];
     sv[threadIdx.x] = red2[threadIdx.x];       // all loads at once
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double v[NRED];
-        for (int k = 0; k < NRED; k++) {
-            double a = sv[k];
-            for (int b = 1; b < RED2_BLOCKS; b++)
-                a = red_is_max(k) ? (a >= sv[RED_STRIDE * b + k] ? a : sv[RED_STRIDE * b + k])
-                                  : a + sv[RED_STRIDE * b + k];
-            v[k] = a;
-        }
-        int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
+    // thread k combines column k: the records in block order, then the deferred stars'
+    // in star order (the same order for every k as a single thread would take)
+    const int k = threadIdx.x;
+    if (k < NRED) {
+        double a = sv[k];
+        for (int b = 1; b < RED2_BLOCKS; b++)
+            a = red_is_max(k) ? (a >= sv[RED_STRIDE * b + k] ? a : sv[RED_STRIDE * b + k]) : a + sv[RED_STRIDE * b + k];
+        const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
         for (int64_t d = 0; d < nd; d++) {
-            const double *p = dpartials + RED_STRIDE * d;
-            for (int k = 0; k < NRED; k++) v[k] = red_is_max(k) ? (v[k] >= p[k] ? v[k] : p[k]) : v[k] + p[k];
+            const double p = dpartials[RED_STRIDE * d + k];
+            a = red_is_max(k) ? (a >= p ? a : p) : a + p;
         }
+        v[k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         s->max_r = v[0];
         s->Eescaped += v[1];
         s->Jescaped += v[2];

@@ -109,3 +109,18 @@ def test_gpu_dyn_line_through_reference_names(gpu, O):
         assert float(col[k - 1]) == pytest.approx(v, rel=1e-7), k
     for k in (13, 14, 15, 17, 18, 26, 27):
         assert float(col[k - 1]) == 0.0
+
+
+def test_gpu_central_reports_the_reference_exit(gpu):
+    """N_STAR <= 2J (J = 6): the reference exits in central_calculate; the device
+    returns CMCG_E_STATE and leaves the context usable."""
+    cm = gpu
+    cl = cm.make_plummer(12, seed=3)
+    say("central_calculate with N = 12")
+    with cm.Context(cl.n, cm.default_params(cl.n, orbit_r=cl.rtid)) as ctx:
+        ctx.upload(cl)
+        ctx.init_cluster()
+        with pytest.raises(cm.CmcgError) as e:
+            ctx.central_calculate()
+        assert e.value.code == cm.E_STATE
+        ctx.step()

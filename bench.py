@@ -127,7 +127,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported "
-                                   "(multi-GPU sharding is a design in DESIGN.md only)"}))
+                                   "(sharding the path across GPUs is designed, not built)"}))
         sys.exit(2)
 
     cm = load_package()

@@ -626,7 +626,9 @@ struct PotWork {
     double *bnd;
 };
 
-__global__ void pot_setup_kernel(DevScalars *s, PotWork w)
+// N_MAX from the count, the chain lengths, and the sentinels before the recurrence
+// reads r[N+1] (cmc_utils.c:713-714)
+__global__ void pot_setup_kernel(DevScalars *s, PotWork w, double *__restrict__ r, double *__restrict__ phi)
 {
     int64_t n = s->first_inf - 1;
     s->n_max = n;
@@ -634,18 +636,12 @@ __global__ void pot_setup_kernel(DevScalars *s, PotWork w)
     w.len[1] = n > 1 ? n - 1 : 0;       // B: k = N..2 gives M_{N-1}..M_1
     w.len[2] = n;                       // C: k = N..1
     w.scal[0] = 0.0;
-}
-
-// sentinels before the recurrence reads r[N+1] (cmc_utils.c:713-714)
-__global__ void sentinel_kernel(double *__restrict__ r, double *__restrict__ phi, DevScalars *s)
-{
-    int64_t n = s->n_max;
     r[n + 1] = CMCG_SF_INFINITY;
     phi[n + 1] = 0.0;
 }
 
 // after chain A: Mtotal, Rtidal (cmc_utils.c:703-707), NaN check (:688-691)
-__global__ void total_kernel(DevScalars *s, KParams P, const double *__restrict__ ga, PotWork w)
+__device__ __forceinline__ void pot_total(DevScalars *s, const KParams &P, const double *__restrict__ ga, PotWork &w)
 {
     int64_t n = s->n_max;
     double mprev = n > 0 ? ga[n - 1] : 0.0;
@@ -656,11 +652,15 @@ __global__ void total_kernel(DevScalars *s, KParams P, const double *__restrict_
     w.scal[1] = Mtotal;
 }
 
-__global__ void prep_b_kernel(const double *__restrict__ m, DevScalars *s, double nstar, double *__restrict__ x)
+// chain B increments (star_m[k] / clus.N_STAR, k = N - j); its first thread also
+// finishes chain A (Mtotal, the start value of chain B), which the increments do not use
+__global__ void prep_b_kernel(const double *__restrict__ m, DevScalars *s, double nstar, double *__restrict__ x,
+                              KParams P, const double *__restrict__ ga, PotWork w)
 {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) pot_total(s, P, ga, w);
     int64_t n = s->n_max;
-    if (j < n - 1) x[j] = m[n - j] / nstar;           // star_m[k] / clus.N_STAR, k = N - j
+    if (j < n - 1) x[j] = m[n - j] / nstar;
 }
 
 // p_k = M_k * (1/r_k - 1/r_{k+1}) with the reference's roundings; chain C runs on
@@ -681,16 +681,12 @@ __global__ void phi_out_kernel(const double *__restrict__ r, double *__restrict_
 {
     int64_t k = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
+    if (n == 0 && k == 1) phi[0] = phi[1] + P.cenma_m * P.madhoc / r[1];    // no stars: phi[1] is the sentinel
     if (k > n) return;
     double p = -g[n - k];
     phi[k] = p;
     if (isnan(p)) dev_set_error(s, CMCG_E_NAN, k);
     if (k == 1) phi[0] = p + P.cenma_m * P.madhoc / r[1];    // cmc_utils.c:729
-}
-
-__global__ void phi0_empty_kernel(const double *__restrict__ r, double *__restrict__ phi, DevScalars *s, KParams P)
-{
-    if (s->n_max == 0) phi[0] = phi[1] + P.cenma_m * P.madhoc / r[1];
 }
 
 __global__ void pot_at_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
@@ -866,8 +862,7 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
     hipLaunchKernelGGL(pot_init_kernel, dim3(1), dim3(1), 0, st, s);
     if (n_ub > 0) hipLaunchKernelGGL(pot_count_kernel, dim3(g1), dim3(256), 0, st, f.r, s);
-    hipLaunchKernelGGL(pot_setup_kernel, dim3(1), dim3(1), 0, st, s, w);
-    hipLaunchKernelGGL(sentinel_kernel, dim3(1), dim3(1), 0, st, f.r, f.phi, s);
+    hipLaunchKernelGGL(pot_setup_kernel, dim3(1), dim3(1), 0, st, s, w, f.r, f.phi);
     // chain A: total mass, straight over star_m[1..N]
     // fixed-point scales 2^-70 (stored masses are O(1)) and 2^-100; a test hook may
     // coarsen them, which sends most elements through the fallback paths
@@ -877,9 +872,9 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     EsaArgs ae = e;
     ae.x = f.m + 1; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0; ae.K = KA;
     if (hipError_t err = run_chain(a, ae, REPAIR_ROUNDS, n_ub, st)) return err;
-    hipLaunchKernelGGL(total_kernel, dim3(1), dim3(1), 0, st, s, P, (const double *)w.ga, w);
     // chain B: enclosed mass, iterated directly in Mk (chain C's input)
-    hipLaunchKernelGGL(prep_b_kernel, dim3(g1), dim3(256), 0, st, f.m, s, (double)P.n_star, w.xa);
+    hipLaunchKernelGGL(prep_b_kernel, dim3(g1), dim3(256), 0, st, f.m, s, (double)P.n_star, w.xa, P,
+                       (const double *)w.ga, w);
     ChainArgs b{w.xa, w.Mk, -1.0, w.scal + 1, w.fbad + 2, w.tsum, w.tpre, w.bnd, w.len + 1, KB};
     EsaArgs be = e;
     be.x = w.xa; be.g = w.Mk; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1; be.K = KB;
@@ -891,7 +886,6 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2; ce.K = KB;
     if (hipError_t err = run_chain(c, ce, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(phi_out_kernel, dim3(g1), dim3(256), 0, st, f.r, f.phi, s, P, (const double *)w.ga);
-    hipLaunchKernelGGL(phi0_empty_kernel, dim3(1), dim3(1), 0, st, f.r, f.phi, s, P);
     return hipGetLastError();
 }
 

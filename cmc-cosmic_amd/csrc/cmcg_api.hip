@@ -898,19 +898,18 @@ int cmcg_init_cluster(cmcg_ctx *c)
 }
 
 // Enqueues one Henon step (src/cmc/cmc.c:240-407 order).  With `graph` the tidal
-// stripping loop is a fixed number of guarded passes (no host round trip) so the
-// whole step can be captured; nub bounds the grids (the kernels read N_MAX on the
-// device).
-constexpr int STRIP_GRAPH_PASSES = 16;
+// stripping loop is a few guarded grid passes and a one-block tail that finishes a
+// longer loop (no host round trip), so the whole step can be captured; nub bounds
+// the grids (the kernels read N_MAX on the device).
+constexpr int STRIP_GRAPH_PASSES = 3;     // grid passes; a one-block tail kernel finishes rarer longer loops
 
 static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
 {
     {
         PhaseTimer t(c, "timestep");
-        LAUNCH(launch_step_begin(c->s, c->stream));
-        LAUNCH(launch_simul_relax(c->f, c->s, c->kp, nub, c->stream));
-        LAUNCH(launch_timestep_finalize(c->s, c->stream));
-        LAUNCH(launch_rng_bases(c->T, c->s, c->sc, c->stream));
+        LAUNCH(launch_step_begin(c->s, c->stream));                 // also starts the time step's minimum
+        LAUNCH(launch_simul_relax(c->f, c->s, c->kp, nub, c->stream, false));
+        LAUNCH(launch_timestep_rng_bases(c->T, c->s, c->sc, c->stream));
     }
     {
         PhaseTimer t(c, "relax");
@@ -939,7 +938,7 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
     }
     if (c->params.tidal_strip) {
         if (graph) {
-            LAUNCH(launch_strip_check(c->s, c->sc, c->stream));
+            LAUNCH(launch_strip_tail(c->f, c->s, c->kp, c->sc, c->stream));
         } else {
             // the stripping loop normally ends within two passes; otherwise continue it here
             for (int guard = 0; guard < 100000; guard++) {

@@ -54,6 +54,7 @@ __global__ void step_begin_kernel(DevScalars *s)
     s->Etidal_old = s->Etidal;
     s->Eescaped = 0.0; s->Jescaped = 0.0; s->TidalMassLoss = 0.0; s->Etidal = 0.0;
     s->n_max_new = s->n_max;
+    s->dtrel = __longlong_as_double(0x7ff0000000000000ll);     // the time step's minimum starts at +inf
 }
 
 __global__ void after_dynamics_kernel(DevScalars *s) { s->Prev_Dt = s->Dt; }   // cmc.c:297
@@ -105,15 +106,16 @@ __global__ void strip_begin_kernel(DevScalars *s, int32_t *done)
 // number of passes can be enqueued (or captured in a graph) without a host round
 // trip.
 constexpr int STRIP_BLOCKS = 1024;
+constexpr int STRIP_TAIL_PASSES = 64;
 
-__global__ __launch_bounds__(GT) void strip_fused_kernel(DevFields f, DevScalars *s, KParams P, int32_t *done,
-                                                         double *__restrict__ partials, unsigned *__restrict__ arrive)
+// The pass as run by block bi of nb blocks (nb = 1: one block does the whole pass).
+__device__ void strip_pass(DevFields &f, DevScalars *s, const KParams &P, int32_t *done, double *__restrict__ partials,
+                           unsigned *__restrict__ arrive, unsigned nb, unsigned bi)
 {
-    if (*done) return;
     const double Rt = P.orbit_r * pow(s->Mtotal - (s->TidalMassLoss - s->OldTidalMassLoss), 1.0 / 3.0);
     const int64_t n = s->n_max;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = 1 + (int64_t)bi * blockDim.x + threadIdx.x; i <= n; i += (int64_t)nb * blockDim.x) {
         if (f.r_apo[i] > Rt && f.rnew[i] < 1000000) {
             double m = f.m[i], E = f.E[i], J = f.J[i];
             f.rnew[i] = CMCG_SF_INFINITY; f.vrnew[i] = 0.0; f.vtnew[i] = 0.0;
@@ -130,17 +132,17 @@ __global__ __launch_bounds__(GT) void strip_fused_kernel(DevFields f, DevScalars
             f.flags[i] |= CMCG_SF_STRIPPED;
         }
     }
-    block_sum_store<4>(v, partials + 8 * blockIdx.x);
+    block_sum_store<4>(v, partials + 8 * bi);
     __shared__ bool last;
     if (threadIdx.x == 0) {
         __threadfence();                                   // publish this block's partials
-        last = atomicAdd(arrive, 1u) == gridDim.x - 1;
+        last = atomicAdd(arrive, 1u) == nb - 1;
     }
     __syncthreads();
     if (!last) return;
     __threadfence();                                       // see every block's partials
     double res[4];
-    final_sum<4>(partials, gridDim.x, res);
+    final_sum<4>(partials, nb, res);
     if (threadIdx.x == 0) {
         s->Rtidal = Rt;
         s->Eescaped += res[0];
@@ -153,10 +155,30 @@ __global__ __launch_bounds__(GT) void strip_fused_kernel(DevFields f, DevScalars
     }
 }
 
-// after a fixed number of passes: the loop must have ended
-__global__ void strip_check_kernel(DevScalars *s, const int32_t *done)
+__global__ __launch_bounds__(GT) void strip_fused_kernel(DevFields f, DevScalars *s, KParams P, int32_t *done,
+                                                         double *__restrict__ partials, unsigned *__restrict__ arrive)
 {
-    if (!*done) dev_set_error(s, CMCG_E_STRIP, 0);
+    if (*done) return;
+    strip_pass(f, s, P, done, partials, arrive, gridDim.x, blockIdx.x);
+}
+
+// The rest of the loop in one block, for a captured step: after the grid passes the
+// loop has nearly always ended and this returns at once; otherwise it runs further
+// passes here (the escape sums then add in one block's order), at most
+// STRIP_TAIL_PASSES, after which the step reports CMCG_E_STRIP.
+__global__ __launch_bounds__(GT) void strip_tail_kernel(DevFields f, DevScalars *s, KParams P, int32_t *done,
+                                                        double *__restrict__ partials, unsigned *__restrict__ arrive)
+{
+    __shared__ int32_t sdone;
+    for (int pass = 0; pass < STRIP_TAIL_PASSES; pass++) {
+        if (threadIdx.x == 0) sdone = *(volatile int32_t *)done;
+        __syncthreads();
+        if (sdone) return;
+        strip_pass(f, s, P, done, partials, arrive, 1, 0);
+        __threadfence();
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && !*(volatile int32_t *)done) dev_set_error(s, CMCG_E_STRIP, 0);
 }
 
 // ComputeIntermediateEnergy, first loop (EI).
@@ -532,9 +554,10 @@ hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, con
     return hipGetLastError();
 }
 
-hipError_t launch_strip_check(DevScalars *s, const Scratch &sc, hipStream_t st)
+hipError_t launch_strip_tail(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(strip_check_kernel, dim3(1), dim3(1), 0, st, s, (const int32_t *)sc.strip_done);
+    hipLaunchKernelGGL(strip_tail_kernel, dim3(1), dim3(GT), 0, st, f, s, P, sc.strip_done, sc.partials,
+                       sc.strip_arrive);
     return hipGetLastError();
 }
 hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,

@@ -514,13 +514,6 @@ __global__ __launch_bounds__(ORB_THREADS) void defer_tail_kernel(DevFields f, De
     }
 }
 
-__global__ void orbits_begin_kernel(DevScalars *s)
-{
-    // new_orbits_calculate (cmc_utils.c:2198) and counters of this call
-    s->OldTidalMassLoss = s->TidalMassLoss_old;
-    s->n_defer = 0;
-}
-
 // First level of the orbit phase's reduction: block B of RED2_BLOCKS sums the partial
 // records of sampler blocks [B C, (B + 1) C), C = ceil(nblocks / RED2_BLOCKS); 16
 // consecutive threads read one record (coalesced), its 64 groups take records
@@ -620,10 +613,15 @@ __global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count,
 
 // The LDS samples of the (r, phi) table for a COARSE-sample launch: node
 // min(s * stride, n + 1) for s = 0..cs, from the packed nodes.
-__global__ void coarse_build_kernel(const double2 *__restrict__ nodes, const DevScalars *s, int cs,
+__global__ void coarse_build_kernel(const double2 *__restrict__ nodes, DevScalars *s, int cs,
                                     double2 *__restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        // new_orbits_calculate (cmc_utils.c:2198) and the counters of this call
+        s->OldTidalMassLoss = s->TidalMassLoss_old;
+        s->n_defer = 0;
+    }
     if (i > cs) return;
     const int64_t n = s->n_max;
     const int64_t st = coarse_stride(n, cs);
@@ -666,12 +664,11 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     if (nblocks < 1) nblocks = 1;
     if (nblocks > sc.max_blocks) return hipErrorInvalidValue;
     int jb1 = 20 + bits_needed((uint64_t)n_ub);
-    hipLaunchKernelGGL(orbits_begin_kernel, dim3(1), dim3(1), 0, st, s);
     if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     if (V.coarse > ORB_MAX_COARSE) return hipErrorInvalidValue;
     hipLaunchKernelGGL(coarse_build_kernel, dim3((unsigned)((V.coarse + 256) / 256)), dim3(256), 0, st,
-                       (const double2 *)sc.nodes, (const DevScalars *)s, V.coarse, sc.coarse);
+                       (const double2 *)sc.nodes, s, V.coarse, sc.coarse);
     if (ev_a) hipEventRecord(ev_a, st);
     hipLaunchKernelGGL(V.fn, dim3((unsigned)nblocks), dim3(V.threads), 0, st, f, s, P, T, (const Rng *)sc.rng_wave,
                        sc.partials, sc.defer_list, theta, nrm, margin, (const double2 *)sc.mid,

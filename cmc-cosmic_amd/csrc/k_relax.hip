@@ -28,8 +28,8 @@ __device__ __forceinline__ double calc_n_local(const double *__restrict__ r, int
     return (2.0 * ((double)p)) * 3.0 / (4.0 * CMCG_PI * (cub(r[kmax]) - cub(r[kmin])));
 }
 
-__global__ void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s, const Rng *__restrict__ seeded,
-                                 Rng *__restrict__ base, int64_t step_override)
+__device__ __forceinline__ void rng_bases(const RngTable *__restrict__ T, DevScalars *s, const Rng *__restrict__ seeded,
+                                          Rng *__restrict__ base, int64_t step_override)
 {
     uint64_t step = step_override >= 0 ? (uint64_t)step_override : s->step;
     // offset step*2^72: bits 72.. of the 128-bit count, i.e. bits 8.. of the high word
@@ -51,6 +51,11 @@ __global__ void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s, 
     b.z2 = gf2_matvec(T->col[70][2], b.z2);
     b.z3 = gf2_matvec(T->col[70][3], b.z3);
     base[1] = b;
+}
+__global__ void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s, const Rng *__restrict__ seeded,
+                                 Rng *__restrict__ base, int64_t step_override)
+{
+    rng_bases(T, s, seeded, base, step_override);
 }
 
 // state at the first pair (phase 0: offset 3 * 64 w) or slot (phase 1: offset
@@ -165,7 +170,7 @@ __global__ __launch_bounds__(SR_B) void simul_relax_kernel(const double *__restr
 }
 
 // calc_timestep (cmc_utils.c:2132-2148) and GetTimeStep's fallback (cmc_evolution_thr.c:430-432)
-__global__ void timestep_finalize_kernel(DevScalars *s)
+__device__ __forceinline__ void timestep_finalize(DevScalars *s)
 {
     double Dt = s->dtrel;
     if (Dt == __longlong_as_double(0x7ff0000000000000ll)) Dt = 0.001;
@@ -176,6 +181,14 @@ __global__ void timestep_finalize_kernel(DevScalars *s)
     }
     s->Dt = Dt;
     s->TotalTime += Dt;
+}
+__global__ void timestep_finalize_kernel(DevScalars *s) { timestep_finalize(s); }
+// the step's time step, then its RNG substream bases (one thread does both)
+__global__ void timestep_rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s,
+                                          const Rng *__restrict__ seeded, Rng *__restrict__ base)
+{
+    timestep_finalize(s);
+    rng_bases(T, s, seeded, base, -1);
 }
 
 // calc_sigma_r at one rank: sigma over the clamped window [si-p, si+p-1]; the
@@ -379,9 +392,10 @@ hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int
     return hipGetLastError();
 }
 
-hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st)
+hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st,
+                              bool init_min)
 {
-    hipLaunchKernelGGL(dtrel_init_kernel, dim3(1), dim3(1), 0, st, s);
+    if (init_min) hipLaunchKernelGGL(dtrel_init_kernel, dim3(1), dim3(1), 0, st, s);
     const int64_t nw = n_ub / (2 * P.avekernel) + 1;
     const int64_t len = 2 * P.avekernel;
     const int64_t G = len <= SR_LDS ? (SR_LDS / len < SR_T ? SR_LDS / len : SR_T) : SR_T;   // as in the kernel
@@ -390,6 +404,11 @@ hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &
     return hipGetLastError();
 }
 
+hipError_t launch_timestep_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(timestep_rng_bases_kernel, dim3(1), dim3(1), 0, st, T, s, sc.rng_seeded, sc.rng_base);
+    return hipGetLastError();
+}
 hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st)
 {
     hipLaunchKernelGGL(timestep_finalize_kernel, dim3(1), dim3(1), 0, st, s);

@@ -43,9 +43,18 @@ struct GatherList {
     uint64_t *dst[RS_MAX_FIELDS];
 };
 
-__global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n)
+// zeroes the digit totals; its first thread also resets the sort's scalars
+__global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n, DevScalars *s, int64_t *n_long,
+                               unsigned long long *mm)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        s->sort_n = 0;
+        s->sort_in = s->n_max_new;
+        *n_long = 0;
+        mm[0] = ~0ull;
+        mm[1] = 0ull;
+    }
     if (i < n) a[i] = 0;
 }
 
@@ -53,7 +62,7 @@ __device__ __forceinline__ uint32_t word_slot(uint64_t w) { return (uint32_t)w; 
 __device__ __forceinline__ uint32_t word_z(uint64_t w) { return (uint32_t)(w >> RS_SHIFT0); }
 
 // Minimum and maximum order-preserving key of the live stars (r below SF_INFINITY):
-// mm[0] = min, mm[1] = max (set to ~0 and 0 by rs_begin_kernel).
+// mm[0] = min, mm[1] = max (set to ~0 and 0 by rs_zero_kernel).
 __global__ __launch_bounds__(256) void rs_minmax_kernel(const double *__restrict__ keysrc, DevScalars *s,
                                                         uint64_t kinf, unsigned long long *__restrict__ mm)
 {
@@ -462,14 +471,6 @@ __global__ void rs_finish_kernel(DevScalars *s)
     s->n_max_new = s->sort_n;
 }
 
-__global__ void rs_begin_kernel(DevScalars *s, int64_t *n_long, unsigned long long *mm)
-{
-    s->sort_n = 0;
-    s->sort_in = s->n_max_new;
-    *n_long = 0;
-    mm[0] = ~0ull;
-    mm[1] = 0ull;
-}
 } // namespace
 
 // Sort slots 1..n_max_new by (key field, id).  src/dst are parallel lists of payload
@@ -490,8 +491,8 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     if (ntiles > sc.max_sort_tiles) return hipErrorInvalidValue;
     const uint64_t kinf = 0x8000000000000000ull | 0x4202a05f20000000ull;   // sortable(1.0e10)
     unsigned long long *mm = (unsigned long long *)sc.sort_mm;
-    hipLaunchKernelGGL(rs_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sort_nlong, mm);
-    hipLaunchKernelGGL(rs_zero_kernel, dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256));
+    hipLaunchKernelGGL(rs_zero_kernel, dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256), s,
+                       sc.sort_nlong, mm);
     const unsigned gm = (unsigned)(ntiles * 4 < 2048 ? ntiles * 4 : 2048);
     hipLaunchKernelGGL(rs_minmax_kernel, dim3(gm), dim3(256), 0, st, key_src, s, kinf, mm);
     // an even number of passes: the sorted words end in keys_a

@@ -91,7 +91,9 @@ hipError_t launch_potential_at(const DevFields &f, DevScalars *s, const KParams 
                                const double *x, double *out, int64_t count, hipStream_t st);
 
 hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
-hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
+hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st,
+                              bool init_min = true);
+hipError_t launch_timestep_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st);
 hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st);
 hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
                         const Scratch &sc, int64_t n_ub, hipStream_t st);
@@ -128,7 +130,7 @@ hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const 
 hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
 hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,
                         int64_t n_ub, int passes, bool begin, hipStream_t st);
-hipError_t launch_strip_check(DevScalars *s, const Scratch &sc, hipStream_t st);
+hipError_t launch_strip_tail(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, hipStream_t st);
 hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
 hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, hipStream_t st);
 // r_old_sorted: the radii the sort of this step read (old sorted order), or nullptr

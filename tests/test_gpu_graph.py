@@ -131,3 +131,37 @@ def test_gpu_every_orbit_variant_gives_identical_stars(gpu):
                 assert np.array_equal(d[k][1:n + 1], ref[0][k][1:n + 1]), (v, k)
         with pytest.raises(cm.CmcgError):
             ctx.test_hook(cm.HOOK_ORBIT_VARIANT, nvar)
+
+
+def test_gpu_long_stripping_cascade_graph_matches_eager(gpu):
+    """A tidal radius inside the cluster (Plummer 1e5, orbit_r = 1.5): the stripping
+    loop of the first step runs about 8 passes (the orbit phase removes ~30% of the
+    stars, and the mass loss keeps shrinking Rtidal).  The captured step does 3 grid
+    passes and finishes in its one-block tail; the eager step loops on the host.
+    Every star is the same; the escape sums of the tail's passes add in another order
+    (within 1e-12)."""
+    cm = gpu
+    say("stripping cascade: graph tail vs eager loop")
+    cl = cm.make_plummer(100000, seed=5)
+    ctxs = []
+    try:
+        for _ in range(2):
+            c = cm.Context(cl.n, cm.default_params(cl.n, orbit_r=1.5, seed=7))
+            c.upload(cl)
+            c.init_cluster()
+            ctxs.append(c)
+        eager, graph = ctxs
+        eager.step()
+        graph.step_graph()
+        sa, sb = eager.scalars(), graph.scalars()
+        say("n_max", sa["n_max"], sb["n_max"], "TidalMassLoss", sa["TidalMassLoss"], sb["TidalMassLoss"])
+        assert sa["n_max"] == sb["n_max"] and sa["n_max"] < 0.7 * cl.n
+        for k in ("TidalMassLoss", "Eescaped", "Jescaped", "Rtidal", "Mtotal", "Etot"):
+            assert sa[k] == pytest.approx(sb[k], rel=1e-12), k
+        da, db = eager.download(), graph.download()
+        n = sa["n_max"]
+        for k in ("r", "m", "vr", "vt", "id", "phi", "E", "J"):
+            assert np.array_equal(da[k][1:n + 1], db[k][1:n + 1]), k
+    finally:
+        for c in ctxs:
+            c.close()

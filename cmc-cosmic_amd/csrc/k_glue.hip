@@ -223,7 +223,7 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
 // k with r_k < x, as in potential_at (the search predicate is monotone, so any bracket
 // holding that k gives the same k), and the interpolation is the same expression, so
 // the values equal potential_at(x) bit for bit.
-constexpr int MERGE_T = 256, MERGE_PER = 4, MERGE_BLK = MERGE_T * MERGE_PER, MERGE_LDS = 2048;
+constexpr int MERGE_T = 256, MERGE_PER = 2, MERGE_BLK = MERGE_T * MERGE_PER, MERGE_LDS = 1024;
 
 __global__ __launch_bounds__(MERGE_T) void old_node_potential_kernel(const double *__restrict__ r_old,
                                                                      const double2 *__restrict__ nodes,

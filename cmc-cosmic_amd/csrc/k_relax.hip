@@ -196,7 +196,7 @@ __global__ void timestep_rng_bases_kernel(const RngTable *__restrict__ T, DevSca
 // The same window sums with the block's range [first - 2p, last + 2p] staged in
 // LDS (every window, including the shifted ones at both ends, lies inside it), in
 // the same order and expression as sigma_kernel.
-constexpr int SIG_T = 256, SIG_MAXP = 448, SIG_LDS = SIG_T + 4 * SIG_MAXP + 1;
+constexpr int SIG_T = 256, SIG_MAXP = 64, SIG_LDS = SIG_T + 4 * SIG_MAXP + 1;   // 8 KB: eight blocks per CU
 
 __global__ __launch_bounds__(SIG_T) void sigma_lds_kernel(const double *__restrict__ m, const double *__restrict__ vr,
                                                           const double *__restrict__ vt, double *__restrict__ sigma,

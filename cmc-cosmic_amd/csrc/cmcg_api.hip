@@ -373,7 +373,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) ||
         (e = dalloc(c, (char **)&sc.relax_nib, sizeof(NibbleTables<8>))) ||
         (e = dalloc(c, (char **)&sc.wb_nib, 2 * sizeof(NibbleTables<RNG_WB_POWERS>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
-        (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_mm, 4)) ||
+        (e = dalloc(c, &sc.vals_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_mm, 2 * SORT_MM_PARTS)) ||
         (e = dalloc(c, &sc.block_hist, (size_t)sc.max_sort_tiles * 256)) || (e = dalloc(c, &sc.digit_totals, 8 * 256)) ||
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
         (e = dalloc(c, &sc.sv3_wend, (size_t)n_cap + 2)) || (e = dalloc(c, &sc.sv3_merged, (size_t)n_cap + 2)) ||

@@ -44,16 +44,13 @@ struct GatherList {
 };
 
 // zeroes the digit totals; its first thread also resets the sort's scalars
-__global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n, DevScalars *s, int64_t *n_long,
-                               unsigned long long *mm)
+__global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n, DevScalars *s, int64_t *n_long)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         s->sort_n = 0;
         s->sort_in = s->n_max_new;
         *n_long = 0;
-        mm[0] = ~0ull;
-        mm[1] = 0ull;
     }
     if (i < n) a[i] = 0;
 }
@@ -61,21 +58,30 @@ __global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n, DevScalars *
 __device__ __forceinline__ uint32_t word_slot(uint64_t w) { return (uint32_t)w; }
 __device__ __forceinline__ uint32_t word_z(uint64_t w) { return (uint32_t)(w >> RS_SHIFT0); }
 
-// Minimum and maximum order-preserving key of the live stars (r below SF_INFINITY):
-// mm[0] = min, mm[1] = max (set to ~0 and 0 by rs_zero_kernel).
+// Minimum and maximum order-preserving key of the live stars (r below SF_INFINITY), one
+// pair per block: mm[2b] = min, mm[2b + 1] = max (~0 and 0 for a block that saw none).
+// rs_prep_kernel combines the pairs; partials per block replace thousands of device-scope
+// atomics on one address arriving from every XCD.
 __global__ __launch_bounds__(256) void rs_minmax_kernel(const double *__restrict__ keysrc, DevScalars *s,
                                                         uint64_t kinf, unsigned long long *__restrict__ mm)
 {
     __shared__ unsigned long long rmin[4], rmax[4];
     const int64_t n = s->n_max_new;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     unsigned long long lo = ~0ull, hi = 0ull;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = sortable(keysrc[i + 1]);
+    auto take = [&](uint64_t k) {
         if (k < kinf) {
             lo = k < lo ? k : lo;
             hi = k > hi ? k : hi;
         }
+    };
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {       // four independent loads in flight
+        const uint64_t k0 = sortable(keysrc[i + 1]), k1 = sortable(keysrc[i + stride + 1]);
+        const uint64_t k2 = sortable(keysrc[i + 2 * stride + 1]), k3 = sortable(keysrc[i + 3 * stride + 1]);
+        take(k0); take(k1); take(k2); take(k3);
     }
+    for (; i < n; i += stride) take(sortable(keysrc[i + 1]));
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long a = __shfl_down(lo, off, 64), b = __shfl_down(hi, off, 64);
         lo = a < lo ? a : lo;
@@ -89,9 +95,15 @@ __global__ __launch_bounds__(256) void rs_minmax_kernel(const double *__restrict
             lo = rmin[q] < lo ? rmin[q] : lo;
             hi = rmax[q] > hi ? rmax[q] : hi;
         }
-        if (lo != ~0ull) atomicMin(&mm[0], lo);
-        if (hi != 0ull) atomicMax(&mm[1], hi);
+        mm[2 * blockIdx.x] = lo;
+        mm[2 * blockIdx.x + 1] = hi;
     }
+}
+
+// Blocks of rs_minmax_kernel for a sort over `ntiles` tiles (the same count on both sides).
+__host__ __device__ __forceinline__ int minmax_parts(int64_t ntiles)
+{
+    return ntiles * 4 < SORT_MM_PARTS ? (int)(ntiles * 4) : SORT_MM_PARTS;
 }
 
 // Words from slots 1..n plus the four global digit histograms; with `aos` the payload
@@ -108,9 +120,29 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
     __shared__ uint64_t recs[RS_THREADS * AOS_LD];     // one chunk of packed records
     const int t = threadIdx.x;
     for (int p = 0; p < RS_PASSES; p++) h[p][t] = 0;
+    // combine the per-block (min, max) pairs of rs_minmax_kernel
+    __shared__ unsigned long long red[2][RS_THREADS / 64];
+    {
+        unsigned long long lo = ~0ull, hi = 0ull;
+        for (int q = t; q < minmax_parts(gridDim.x); q += RS_THREADS) {
+            const unsigned long long a = mm[2 * q], b = mm[2 * q + 1];
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long a = __shfl_down(lo, off, 64), b = __shfl_down(hi, off, 64);
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+        }
+        if ((t & 63) == 0) { red[0][t >> 6] = lo; red[1][t >> 6] = hi; }
+    }
+    __syncthreads();
     // z = (key - min) >> S with S chosen so the live range fits 31 bits
-    uint64_t kmin = mm[0];
-    const uint64_t kmax = mm[1];
+    uint64_t kmin = red[0][0], kmax = red[1][0];
+    for (int q = 1; q < RS_THREADS / 64; q++) {
+        kmin = red[0][q] < kmin ? red[0][q] : kmin;
+        kmax = red[1][q] > kmax ? red[1][q] : kmax;
+    }
     int S = 0;
     if (kmax >= kmin) {
         const uint64_t range = kmax - kmin;
@@ -492,9 +524,9 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     const uint64_t kinf = 0x8000000000000000ull | 0x4202a05f20000000ull;   // sortable(1.0e10)
     unsigned long long *mm = (unsigned long long *)sc.sort_mm;
     hipLaunchKernelGGL(rs_zero_kernel, dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256), s,
-                       sc.sort_nlong, mm);
-    const unsigned gm = (unsigned)(ntiles * 4 < 2048 ? ntiles * 4 : 2048);
-    hipLaunchKernelGGL(rs_minmax_kernel, dim3(gm), dim3(256), 0, st, key_src, s, kinf, mm);
+                       sc.sort_nlong);
+    hipLaunchKernelGGL(rs_minmax_kernel, dim3((unsigned)minmax_parts(ntiles)), dim3(256), 0, st, key_src, s, kinf,
+                       mm);
     // an even number of passes: the sorted words end in keys_a
     uint64_t *kin = sc.keys_a, *kout = sc.keys_b;
     hipLaunchKernelGGL(rs_prep_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,

@@ -13,6 +13,7 @@ constexpr int ORB_MAX_COARSE = 2048;
 // sort payload records: 16 words (128 B, one cache line) per star
 constexpr int SORT_AOS_WORDS = 16;
 constexpr int SORT_LONG_RUNS = 4096;
+constexpr int SORT_MM_PARTS = 1024;   // blocks of the sort's min/max pass, one (min, max) pair each
 constexpr int RED2_BLOCKS = 64;
 constexpr int ORB_NIB_B0 = 20, ORB_NIB_POWERS = 6;
 
@@ -41,7 +42,7 @@ struct Scratch {
     // sort
     uint64_t *keys_a, *keys_b;  // sort words: 32-bit normalised key of r above the 32-bit slot
     uint32_t *vals_a;           // sorted permutation: slot of the star now at each position
-    uint64_t *sort_mm;          // live key minimum and maximum
+    uint64_t *sort_mm;          // live key (min, max) per block of the min/max pass: 2 * SORT_MM_PARTS
     uint32_t *block_hist;       // 256 * max_sort_tiles
     uint32_t *digit_totals;     // 8 * 256
     int64_t max_sort_tiles;

@@ -137,6 +137,8 @@ _sigs = {
     "cmcg_calc_pericenter_and_apocenter": [_ctx, C.c_int64, C.c_int64, _P(C.c_double), _P(C.c_double),
                                            _P(C.c_double), _P(C.c_double), _P(C.c_double), _P(C.c_double),
                                            _P(C.c_int64), _P(C.c_int64), _P(C.c_int32)],
+    "cmcg_find_root_vr": [_ctx, C.c_int64, _P(C.c_int64), _P(C.c_int64), _P(C.c_double), _P(C.c_double),
+                          _P(C.c_double)],
     "cmcg_find_zero_Q": [_ctx, C.c_int64, _P(C.c_int64), _P(C.c_int64), _P(C.c_int64), _P(C.c_double),
                          _P(C.c_double), _P(C.c_int64)],
     "cmcg_energy_conservation1": [_ctx],
@@ -467,6 +469,15 @@ class Context:
         _chk(lib.cmcg_find_zero_Q(self.h, j.size, _ptr(j, C.c_int64), _ptr(kmin, C.c_int64), _ptr(kmax, C.c_int64),
                                   _ptr(E, C.c_double), _ptr(J, C.c_double), _ptr(out, C.c_int64)),
              "cmcg_find_zero_Q")
+        return out
+
+    def find_root_vr(self, j, k, E, J):
+        """find_root_vr (cmc_nr.c:500-603) for arrays of (star j, interval k, E, J)."""
+        j, k = (np.ascontiguousarray(x, dtype=np.int64) for x in (j, k))
+        E, J = (np.ascontiguousarray(x, dtype=np.float64) for x in (E, J))
+        out = np.zeros(j.size, np.float64)
+        _chk(lib.cmcg_find_root_vr(self.h, j.size, _ptr(j, C.c_int64), _ptr(k, C.c_int64), _ptr(E, C.c_double),
+                                   _ptr(J, C.c_double), _ptr(out, C.c_double)), "cmcg_find_root_vr")
         return out
 
     def energy_conservation1(self):

@@ -364,6 +364,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.esa_max = 1 << 16;
     sc.max_waves = (n_cap + 2) / 64 + 2;
     sc.max_blocks = (n_cap + 127) / 128 + 2;      // the smallest block of any launch shape: 128 threads
+    if (sc.max_blocks < 1024) sc.max_blocks = 1024;   // the stripping passes' fixed grid
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
     sc.sort_aos_cap = n_cap + 4;
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
@@ -831,6 +832,31 @@ int cmcg_find_zero_Q(cmcg_ctx *c, int64_t count, const int64_t *j, const int64_t
         hipMemcpyAsync(out, dout, count * 8, hipMemcpyDeviceToHost, c->stream) || sync(c))
         rc = CMCG_E_HIP;
     hipFree(dj); hipFree(dmin); hipFree(dmax); hipFree(dout); hipFree(dE); hipFree(dJ);
+    if (rc) return rc;
+    return check_error(c);
+}
+
+int cmcg_find_root_vr(cmcg_ctx *c, int64_t count, const int64_t *j, const int64_t *k, const double *E,
+                      const double *J, double *out)
+{
+    if (!c || count < 0 || (count > 0 && (!j || !k || !E || !J || !out))) return CMCG_E_ARG;
+    if (count == 0) return CMCG_OK;
+    for (int64_t i = 0; i < count; i++)
+        if (j[i] < 1 || j[i] > c->n_ub || k[i] < 0 || k[i] > c->n_ub) return CMCG_E_ARG;
+    int64_t *dj, *dk;
+    double *dE, *dJ, *dout;
+    if (hipMalloc(&dj, count * 8) || hipMalloc(&dk, count * 8) || hipMalloc(&dE, count * 8) ||
+        hipMalloc(&dJ, count * 8) || hipMalloc(&dout, count * 8))
+        return CMCG_E_NOMEM;
+    int rc = CMCG_OK;
+    if (hipMemcpyAsync(dj, j, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dk, k, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dE, E, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dJ, J, count * 8, hipMemcpyHostToDevice, c->stream) ||
+        launch_find_root_vr(c->f, c->s, c->kp, count, dj, dk, dE, dJ, dout, c->stream) ||
+        hipMemcpyAsync(out, dout, count * 8, hipMemcpyDeviceToHost, c->stream) || sync(c))
+        rc = CMCG_E_HIP;
+    hipFree(dj); hipFree(dk); hipFree(dE); hipFree(dJ); hipFree(dout);
     if (rc) return rc;
     return check_error(c);
 }

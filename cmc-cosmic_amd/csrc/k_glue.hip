@@ -108,12 +108,11 @@ __global__ void strip_begin_kernel(DevScalars *s, int32_t *done)
 constexpr int STRIP_BLOCKS = 1024;
 constexpr int STRIP_TAIL_PASSES = 64;
 
-// The pass as run by block bi of nb blocks (nb = 1: one block does the whole pass).
-__device__ void strip_pass(DevFields &f, DevScalars *s, const KParams &P, int32_t *done, double *__restrict__ partials,
-                           unsigned *__restrict__ arrive, unsigned nb, unsigned bi)
+// Block bi's share of one pass (stars 1 + bi*GT + t, stride nb*GT): strip them and
+// store the block's partial sums (Eescaped, Jescaped, DTidalMassLoss, Etidal).
+__device__ void strip_block(DevFields &f, const KParams &P, double Rt, int64_t n, double *__restrict__ partials,
+                            unsigned nb, unsigned bi)
 {
-    const double Rt = P.orbit_r * pow(s->Mtotal - (s->TidalMassLoss - s->OldTidalMassLoss), 1.0 / 3.0);
-    const int64_t n = s->n_max;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int64_t i = 1 + (int64_t)bi * blockDim.x + threadIdx.x; i <= n; i += (int64_t)nb * blockDim.x) {
         if (f.r_apo[i] > Rt && f.rnew[i] < 1000000) {
@@ -133,14 +132,18 @@ __device__ void strip_pass(DevFields &f, DevScalars *s, const KParams &P, int32_
         }
     }
     block_sum_store<4>(v, partials + 8 * bi);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        __threadfence();                                   // publish this block's partials
-        last = atomicAdd(arrive, 1u) == nb - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();                                       // see every block's partials
+}
+
+__device__ __forceinline__ double strip_rtidal(const DevScalars *s, const KParams &P)
+{
+    return P.orbit_r * pow(s->Mtotal - (s->TidalMassLoss - s->OldTidalMassLoss), 1.0 / 3.0);
+}
+
+// The end of a pass: the block partials summed in block order, the scalars updated,
+// the loop's exit decided (all threads of the calling block take part).
+__device__ void strip_finish(DevScalars *s, double Rt, int32_t *done, const double *__restrict__ partials,
+                             unsigned nb)
+{
     double res[4];
     final_sum<4>(partials, nb, res);
     if (threadIdx.x == 0) {
@@ -151,7 +154,6 @@ __device__ void strip_pass(DevFields &f, DevScalars *s, const KParams &P, int32_
         s->TidalMassLoss += res[2];
         s->DTidalMassLoss = res[2];
         if (!(res[2] > 0)) *done = 1;
-        *arrive = 0;
     }
 }
 
@@ -159,22 +161,40 @@ __global__ __launch_bounds__(GT) void strip_fused_kernel(DevFields f, DevScalars
                                                          double *__restrict__ partials, unsigned *__restrict__ arrive)
 {
     if (*done) return;
-    strip_pass(f, s, P, done, partials, arrive, gridDim.x, blockIdx.x);
+    const double Rt = strip_rtidal(s, P);
+    strip_block(f, P, Rt, s->n_max, partials, gridDim.x, blockIdx.x);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        __threadfence();                                   // publish this block's partials
+        last = atomicAdd(arrive, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();                                       // see every block's partials
+    strip_finish(s, Rt, done, partials, gridDim.x);
+    if (threadIdx.x == 0) *arrive = 0;
 }
 
 // The rest of the loop in one block, for a captured step: after the grid passes the
 // loop has nearly always ended and this returns at once; otherwise it runs further
-// passes here (the escape sums then add in one block's order), at most
-// STRIP_TAIL_PASSES, after which the step reports CMCG_E_STRIP.
+// passes here, at most STRIP_TAIL_PASSES (then the step reports CMCG_E_STRIP).  The
+// block plays the nb blocks of a grid pass one after another and writes the same
+// partials, so the sums add in the same order as in the eager loop.
 __global__ __launch_bounds__(GT) void strip_tail_kernel(DevFields f, DevScalars *s, KParams P, int32_t *done,
-                                                        double *__restrict__ partials, unsigned *__restrict__ arrive)
+                                                        double *__restrict__ partials, unsigned nb)
 {
     __shared__ int32_t sdone;
     for (int pass = 0; pass < STRIP_TAIL_PASSES; pass++) {
         if (threadIdx.x == 0) sdone = *(volatile int32_t *)done;
         __syncthreads();
         if (sdone) return;
-        strip_pass(f, s, P, done, partials, arrive, 1, 0);
+        const double Rt = strip_rtidal(s, P);
+        const int64_t n = s->n_max;
+        __syncthreads();
+        for (unsigned bi = 0; bi < nb; bi++) strip_block(f, P, Rt, n, partials, nb, bi);
+        __threadfence_block();
+        __syncthreads();
+        strip_finish(s, Rt, done, partials, nb);
         __threadfence();
         __syncthreads();
     }
@@ -546,7 +566,8 @@ hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, con
 {
     (void)n_ub;
     // a fixed grid (grid-stride over the stars) that the per-block partials hold
-    const unsigned nb = (unsigned)(STRIP_BLOCKS < sc.max_blocks ? STRIP_BLOCKS : sc.max_blocks);
+    const unsigned nb = STRIP_BLOCKS;              // the scratch holds at least STRIP_BLOCKS partials
+    if (sc.max_blocks < STRIP_BLOCKS) return hipErrorInvalidValue;
     if (begin) hipLaunchKernelGGL(strip_begin_kernel, dim3(1), dim3(1), 0, st, s, done);
     for (int it = 0; it < passes; it++)
         hipLaunchKernelGGL(strip_fused_kernel, dim3(nb), dim3(GT), 0, st, f, s, P, done, sc.partials,
@@ -556,8 +577,9 @@ hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, con
 
 hipError_t launch_strip_tail(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, hipStream_t st)
 {
+    if (sc.max_blocks < STRIP_BLOCKS) return hipErrorInvalidValue;
     hipLaunchKernelGGL(strip_tail_kernel, dim3(1), dim3(GT), 0, st, f, s, P, sc.strip_done, sc.partials,
-                       sc.strip_arrive);
+                       (unsigned)STRIP_BLOCKS);
     return hipGetLastError();
 }
 hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,

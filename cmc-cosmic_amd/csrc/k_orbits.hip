@@ -611,6 +611,18 @@ __global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count,
     out[i] = k;
 }
 
+__global__ void frv_kernel(DevFields f, DevScalars *s, KParams P, int64_t count, const int64_t *__restrict__ jj,
+                           const int64_t *__restrict__ kk, const double *__restrict__ E,
+                           const double *__restrict__ J, double *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t j = jj[i];
+    bool bad = false;
+    out[i] = find_root_vr(f.r, f.phi, f.r[j], f.m[j], P, kk[i], E[i], J[i], &bad);
+    if (bad) dev_set_error(s, CMCG_E_BRENT, j);
+}
+
 // The LDS samples of the (r, phi) table for a COARSE-sample launch: node
 // min(s * stride, n + 1) for s = 0..cs, from the packed nodes.
 __global__ void coarse_build_kernel(const double2 *__restrict__ nodes, DevScalars *s, int cs,
@@ -702,6 +714,16 @@ hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, i
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(apsides_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
                        count, E, J, out4, kout, circ);
+    return hipGetLastError();
+}
+
+hipError_t launch_find_root_vr(const DevFields &f, DevScalars *s, const KParams &P, int64_t count,
+                               const int64_t *j, const int64_t *k, const double *E, const double *J, double *out,
+                               hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(frv_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, count, j, k, E,
+                       J, out);
     return hipGetLastError();
 }
 

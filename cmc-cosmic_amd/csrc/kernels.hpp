@@ -117,6 +117,9 @@ void orbit_variant_shape(int v, int *threads, int *coarse, int *cap = nullptr);
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
                           const double *E, const double *J, double *out4, int64_t *kout, int32_t *circ,
                           hipStream_t st);
+hipError_t launch_find_root_vr(const DevFields &f, DevScalars *s, const KParams &P, int64_t count,
+                               const int64_t *j, const int64_t *k, const double *E, const double *J, double *out,
+                               hipStream_t st);
 hipError_t launch_find_zero_Q(const DevFields &f, DevScalars *s, const KParams &P, int64_t count,
                               const int64_t *j, const int64_t *kmin, const int64_t *kmax,
                               const double *E, const double *J, int64_t *out, hipStream_t st);

@@ -298,6 +298,14 @@ long find_zero_Q(long j, long kmin, long kmax, long double E, long double J)
     return (long)out;
 }
 
+double find_root_vr(long index, long k, double E, double J)
+{
+    int64_t jj = index, kk = k;
+    double out = 0.0;
+    check(cmcg_find_root_vr(g_ctx, 1, &jj, &kk, &E, &J, &out), "find_root_vr");
+    return out;
+}
+
 int cmc_hotpath_step(cmcg_orbit_stats *stats)
 {
     int rc = cmcg_step(g_ctx, stats);

@@ -178,6 +178,14 @@ int cmcg_calc_pericenter_and_apocenter(cmcg_ctx *ctx, int64_t first, int64_t cou
 int cmcg_find_zero_Q(cmcg_ctx *ctx, int64_t count, const int64_t *j, const int64_t *kmin,
                      const int64_t *kmax, const double *E, const double *J, int64_t *out);
 
+/* a8: find_root_vr (src/cmc/cmc_nr.c:500-603, with calc_vr_within_interval :393-403 and
+ *     calc_pot_within_interval :239-269) for a batch of (index j, interval k, E, J):
+ *     the root of v_r^2 on [r_k, r_{k+1}] by the Brent-Dekker method to the reference's
+ *     APSIDES_PRECISION, then its residual fix-up.  CMCG_E_BRENT where the reference
+ *     exits (same signs at the ends, no convergence, negative residual). */
+int cmcg_find_root_vr(cmcg_ctx *ctx, int64_t count, const int64_t *j, const int64_t *k, const double *E,
+                      const double *J, double *out);
+
 /* ------------------------------------------------------------------ step glue */
 int cmcg_energy_conservation1(cmcg_ctx *ctx);        /* cmc_utils.c:2153-2172 */
 int cmcg_energy_conservation2(cmcg_ctx *ctx);        /* cmc_utils.c:2217-2227 */
@@ -191,8 +199,9 @@ int cmcg_init_cluster(cmcg_ctx *ctx);
 int cmcg_step(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
 /* The same step replayed from a HIP graph (captured on first use; the step
  * alternates between two buffer states, so two graphs are kept).  The tidal
- * stripping loop runs as 16 guarded passes inside the graph (CMCG_E_STRIP if it had
- * not ended).  Results are identical to cmcg_step.  No counterpart in the reference:
+ * stripping loop runs inside the graph as three guarded grid passes and a one-block
+ * tail that plays the grid's blocks in turn, so the sums add in the eager loop's
+ * order (CMCG_E_STRIP if it had not ended after 64 further passes).  Results are identical to cmcg_step.  No counterpart in the reference:
  * for callers that step many small clusters, where launch overhead dominates. */
 int cmcg_step_graph(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
 /* One step of each of `count` clusters (contexts on one device): every cluster's

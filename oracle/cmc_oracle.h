@@ -46,6 +46,7 @@ orc_t113 orc_t113_jump(orc_t113 st, uint64_t lo, uint64_t hi);
  * Offsets are counted from the seeded state (after the 2^80*rank jump of
  * set_rng_states, cmc_utils.c:2364-2374; rank 0 here).  Step 0's relaxation draws
  * therefore coincide with the reference's single sequential stream. */
+orc_t113 orc_t113_jump_pow2(orc_t113 st, int b);
 orc_t113 orc_stream_relax(const orc_t113 *seeded, uint64_t step, uint64_t pair);
 orc_t113 orc_stream_orbit(const orc_t113 *seeded, uint64_t step, uint64_t slot);
 

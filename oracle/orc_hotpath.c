@@ -383,6 +383,7 @@ static void destroy_obj(orc_state *s, int64_t j)
 double orc_get_positions(orc_state *s)
 {
     double max_rad = 0.0;
+    orc_t113 cur = {{0, 0, 0, 0}};     /* substream of slot j: slot 1's, jumped 2^20 per slot */
     for (int64_t j = 1; j <= s->N_MAX_NEW; j++) {
         double E, J, rmin, rmax, dQdr_min, dQdr_max, g1, g2, F, s0, g0, drds, pot, Q;
         double r = 0.0, vr = 0.0, X = 0.0;
@@ -393,6 +394,8 @@ double orc_get_positions(orc_state *s)
         if (s->ntry) s->ntry[j] = 0;
         if (s->flags) s->flags[j] = 0;
         if (s->pot_rnew) s->pot_rnew[j] = 0.0;
+        if (s->rng_mode != ORC_RNG_REF_SEQ)
+            cur = j == 1 ? orc_stream_orbit(&s->seeded, s->step, 1) : orc_t113_jump_pow2(cur, 20);
         E = s->E[j] + phi_s(s, s->r[j], j);
         J = s->J[j];
 
@@ -431,7 +434,7 @@ double orc_get_positions(orc_state *s)
         F = 1.2 * OMAX(g1, g2);
 
         if (s->rng_mode == ORC_RNG_REF_SEQ) rng = &s->seq;
-        else { local = orc_stream_orbit(&s->seeded, s->step, (uint64_t)j); rng = &local; }
+        else { local = cur; rng = &local; }
 
         for (k = 1; k <= ORC_N_TRY; k++) {
             X = orc_t113_dbl(rng);
@@ -567,6 +570,8 @@ static double o_cos(const orc_state *s, double x)
 void orc_dynamics_apply(orc_state *s, double dt)
 {
     int64_t p = s->avekernel, N = s->N_MAX, si = 1, pair = 0;
+    /* pair p's substream starts 3p draws into the step's stream: walk it */
+    orc_t113 cur = orc_stream_relax(&s->seeded, s->step, 0);
     orc_calc_sigma_r(s);
     while (si <= N - N % 2 - 1) {
         int64_t k = si, kp = si + 1;
@@ -576,7 +581,10 @@ void orc_dynamics_apply(orc_state *s, double dt)
         int j;
         si += 2;
         if (s->rng_mode == ORC_RNG_REF_SEQ) rng = &s->seq;
-        else { local = orc_stream_relax(&s->seeded, s->step, (uint64_t)pair); rng = &local; }
+        else {
+            local = cur; rng = &local;
+            orc_t113_int(&cur); orc_t113_int(&cur); orc_t113_int(&cur);
+        }
         pair++;
 
         /* calc_encounter_dyns, cmc_dynamics_helper.c:831-871 */

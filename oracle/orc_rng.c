@@ -109,6 +109,15 @@ orc_t113 orc_t113_jump(orc_t113 st, uint64_t lo, uint64_t hi)
     return st;
 }
 
+/* T^(2^b): the jump by one power of two (used to walk a layout whose substreams are
+ * 2^b apart one substream at a time). */
+orc_t113 orc_t113_jump_pow2(orc_t113 st, int b)
+{
+    build_powers();
+    for (int j = 0; j < 4; j++) st.z[j] = matvec(g_pow[b][j], st.z[j]);
+    return st;
+}
+
 orc_t113 orc_stream_relax(const orc_t113 *seeded, uint64_t step, uint64_t pair)
 {
     /* offset = step*2^72 + 3*pair  (3*pair < 2^64 for any realistic N) */

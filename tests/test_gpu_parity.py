@@ -310,6 +310,69 @@ def test_gpu_apsides_and_find_zero_Q(gpu, O):
         assert kq[i] == O.lib.orc_find_zero_Q(C.byref(o.s), int(j), 0, int(j), E[i], J[i])
 
 
+def test_gpu_find_root_vr_batch(gpu, O):
+    """find_root_vr (cmc_nr.c:500-603) on the apsis intervals of ordinary stars and
+    of near-tangent stars: the device's Brent iteration returns the oracle's root bit
+    for bit (the oracle's root is pinned against the exact root of the interval's
+    quadratic in tests/test_oracle_hotpath.py)."""
+    from cases import near_tangent_stars
+    cm = gpu
+    say("find_root_vr batch")
+    cl = cm.make_plummer(20000, seed=21, kroupa=True)
+    o = oracle_for(O, cl, q_mode=O.Q_FP64)
+    o.init()
+    near_tangent_stars(o, list(range(5, o.N - 1, 53)))
+    js, ks, Es, Js = [], [], [], []
+    for j in range(2, o.N, 29):
+        E = o["E"][j] + o["m"][j] / o["r"][j] / o.s.N_STAR
+        J = o["J"][j]
+        res = o.apsides(j, E, J)
+        if res["circular"]:
+            continue
+        for k in (res["kmin"], res["kmax"]):
+            js.append(j); ks.append(k); Es.append(E); Js.append(J)
+    want = np.array([O.lib.orc_find_root_vr(C.byref(o.s), j, k, E, J) for j, k, E, J in zip(js, ks, Es, Js)])
+    assert o.s.error == 0
+    with ctx_for(cm, cl) as ctx:
+        upload_state(ctx, cl, o, extra=("vr", "vt"))
+        got = ctx.find_root_vr(js, ks, Es, Js)
+    say("intervals", len(js), "bit-equal", int(np.sum(got == want)))
+    assert len(js) > 500
+    assert np.array_equal(got, want)
+
+
+def test_gpu_brent_near_tangent_orbits(gpu, O):
+    """The orbit phase with near-tangent stars (cases.near_tangent_stars), where the
+    reference's calc_orbit_new falls back to find_root_vr (cmc_orbit.c:459-474,
+    485-500): the device takes the same path for the same stars (n_brent equal and
+    > 0) and every star's sampled orbit is bit-identical to the oracle's, including
+    where its narrowed searches meet Q within rounding of zero."""
+    from cases import near_tangent_stars
+    cm = gpu
+    say("orbits with near-tangent stars (Brent path)")
+    cl = cm.make_plummer(100000, seed=22)
+    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64)
+    o.init()
+    js = near_tangent_stars(o, list(range(3, o.N - 1, 41)))
+    with ctx_for(cm, cl) as ctx:
+        upload_state(ctx, cl, o, extra=("vr", "vt"))
+        st = ctx.orbits(step=0)
+        d = ctx.download()
+    o.s.step = 0
+    o.orbits()
+    N = o.N
+    say("near-tangent stars", len(js), "brent gpu", st["n_brent"], "oracle", o.s.n_brent, "circular",
+        st["n_circular"], o.sc("n_circular"))
+    assert st["error"] == 0 and o.s.error == 0
+    assert o.s.n_brent > 50
+    assert st["n_brent"] == o.s.n_brent
+    assert st["n_circular"] == o.sc("n_circular")
+    for k in ("rnew", "vrnew", "vtnew", "r_peri", "r_apo", "X"):
+        same = int(np.sum(d[k][1:N + 1] == o[k][1:N + 1]))
+        assert same == N, (k, same, N)
+    assert np.array_equal(d["flags"][1:N + 1] & 16, o["flags"][1:N + 1] & 16)
+
+
 # ------------------------------------------------------------------ glue
 def test_gpu_glue_parity(gpu, O):
     cm = gpu

@@ -183,6 +183,149 @@ def test_longdouble_vs_fp64_index_flips_are_rare(cm, O):
     assert worst < 1e-12
 
 
+# ------------------------------------------------------------------ find_root_vr
+def _exact_interval_root(o, j, k, E, J, dps=50, with_conditioning=False):
+    """Root of v_r^2(r) = Q(r) on [r_k, r_{k+1}] in 50-digit arithmetic, from the
+    double inputs: inside one interval the potential is the Henon interpolant
+    phi_k + (phi_{k+1} - phi_k)(1/r_k - 1/r)/(1/r_k - 1/r_{k+1}) (phi_0 - cenma/r below
+    r_1, cmc_nr.c:239-269) and the star's own term is m_j/r or m_j/r_j (r_j is a
+    node), so Q is a quadratic in u = 1/r, Q = A + B u + C u^2, solved exactly."""
+    mp = mpmath.mp
+    mp.dps = dps
+    f = lambda x: mpmath.mpf(float(x))
+    N_STAR = o.s.N_STAR
+    rk, rk1 = f(o["r"][k]), f(o["r"][k + 1])
+    rj, mj = f(o["r"][j]), f(o["m"][j])
+    if k == 0:
+        # pot = phi_0 - cenma madhoc / r  -> constant part phi_0, 1/r part -cenma/N
+        p0, p1 = f(o["phi"][0]), -f(o.s.cenma_m) / N_STAR
+    else:
+        pk, pk1 = f(o["phi"][k]), f(o["phi"][k + 1])
+        slope = (pk1 - pk) / (1 / rk - 1 / rk1)         # pot = pk + slope (1/rk - u)
+        p0, p1 = pk + slope / rk, -slope
+    if k + 1 <= j:       # r <= r_{k+1} <= r_j: own term m_j / r_j
+        s0, s1 = mj / rj / N_STAR, mpmath.mpf(0)
+    else:                # r >= r_k >= r_j: own term m_j / r
+        s0, s1 = mpmath.mpf(0), mj / N_STAR
+    # Q(u) = 2(E - p0 - s0) - 2(p1 + s1) u - J^2 u^2
+    A, B, Cq = 2 * (f(E) - p0 - s0), -2 * (p1 + s1), -f(J) ** 2
+    disc = B * B - 4 * A * Cq
+    assert disc >= 0, "no real root"
+    roots = [(-B + sg * mpmath.sqrt(disc)) / (2 * Cq) for sg in (1, -1)] if Cq != 0 else [-A / B]
+    rs = [1 / u for u in roots if u > 0 and rk <= 1 / u <= rk1]
+    assert len(rs) == 1, (k, rs)
+    if not with_conditioning:
+        return rs[0]
+    # how far rounding of the double evaluation of Q moves its sign change: each of
+    # its terms is rounded a few times, |dQ| <~ 2^-48 (|2E| + |2 pot| + |2 self| +
+    # (J/r)^2), and near the root dr = dQ / |dQ/dr|
+    u = 1 / rs[0]
+    dqdr = abs((B + 2 * Cq * u) * u * u)
+    scale = 2 * abs(f(E)) + 2 * abs(p0 + p1 * u) + 2 * abs(s0 + s1 * u) + f(J) ** 2 * u * u
+    return rs[0], float(scale * mpmath.mpf(2) ** -48 / dqdr) if dqdr > 0 else float("inf")
+
+
+def _brent_tol(x):
+    # gsl_root_test_interval(lo, hi, APSIDES_PRECISION, APSIDES_PRECISION) at
+    # cmc_nr.c:540 with APSIDES_PRECISION = 1e-11: |hi - lo| < 1e-11 + 1e-11 min(|lo|, |hi|)
+    return 1e-11 + 1e-11 * abs(x)
+
+
+def test_find_root_vr_matches_exact_root(cm, O):
+    """find_root_vr (cmc_nr.c:500-603) on the apsis intervals of ordinary stars: the
+    Brent root equals the exact root of the interval's quadratic to the reference's
+    convergence tolerance, and v_r^2 at the returned apsis is >= 0 (:587-600)."""
+    cl = small_cluster(cm, 3000, seed=21, kroupa=True)
+    o = make_oracle(O, cl, q_mode=1)
+    o.init()
+    checked = 0
+    for j in range(3, o.N + 1, 97):
+        E = o["E"][j] + o["m"][j] / o["r"][j] / o.s.N_STAR
+        J = o["J"][j]
+        res = o.apsides(j, E, J)
+        if res["circular"]:
+            continue
+        for k in (res["kmin"], res["kmax"]):
+            nb = o.s.n_brent
+            got = O.lib.orc_find_root_vr(C.byref(o.s), j, k, E, J)
+            assert o.s.n_brent == nb + 1 and o.s.error == 0
+            want = _exact_interval_root(o, j, k, E, J)
+            assert abs(got - float(want)) <= _brent_tol(float(want)), (j, k, got, want)
+            q = O.lib.orc_function_q(C.byref(o.s), j, got, o.potential(got), E, J)
+            assert q >= 0.0 or abs(got - o["r"][k]) < 2.3e-16 or abs(got - o["r"][k + 1]) < 2.3e-16
+            checked += 1
+        # the closed form of calc_orbit_new agrees with the exact root too
+        # (a and b of set_a_b carry the cancellation of (U_k1 - U_k) / (1/r_k1 - 1/r_k))
+        assert abs(res["rp"] - float(_exact_interval_root(o, j, res["kmin"], E, J))) <= 1e-10 * res["rp"]
+    assert checked > 40
+
+
+def test_find_root_vr_reached_by_near_tangent_orbits(cm, O):
+    """calc_orbit_new reaches find_root_vr where rounding makes inside_sqrt < 0 and
+    the double root -J^2/a falls outside the apsis interval (cmc_orbit.c:459-474,
+    485-500): nearly circular orbits whose tangency sits on a node
+    (cases.near_tangent_stars).  Each Brent apsis equals the exact root of the
+    interval's quadratic to the reference's tolerance, or, where that quadratic has
+    no real root, lies inside the interval with v_r^2 >= 0 there (:587-600)."""
+    from cases import near_tangent_stars
+    cl = small_cluster(cm, 5000, seed=22)
+    o = make_oracle(O, cl, q_mode=1)
+    o.init()
+    js = near_tangent_stars(o, list(range(2, o.N - 1, 7)))
+    n0 = o.s.n_brent
+    brent_stars = exact_checked = tight = 0
+    for j in js:
+        E = o["E"][j] + o["m"][j] / o["r"][j] / o.s.N_STAR
+        J = o["J"][j]
+        nb = o.s.n_brent
+        o.s.flags[j] = 0
+        res = o.apsides(j, E, J)
+        assert o.s.error == 0
+        if o.s.n_brent == nb:
+            continue
+        brent_stars += 1
+        assert o["flags"][j] & 16
+        used = 0
+        for key, k in (("rp", res["kmin"]), ("ra", res["kmax"])):
+            if res["circular"]:
+                used += 1
+                break
+            x = res[key]
+            if x != O.lib.orc_find_root_vr(C.byref(o.s), j, k, E, J):
+                continue                               # this apsis came from the closed form
+            used += 1
+            assert o["r"][k] - 2.3e-16 <= x <= o["r"][k + 1] + 2.3e-16
+            try:
+                want, dr = _exact_interval_root(o, j, k, E, J, with_conditioning=True)
+            except AssertionError:
+                continue                               # the quadratic has no real root here
+            want = float(want)
+            # a near-double root is ill-conditioned: the bound grows as 1/|dQ/dr|
+            assert abs(x - want) <= _brent_tol(want) + dr, (j, key, x, want, dr)
+            exact_checked += 1
+            tight += dr < 0.05 * (o["r"][k + 1] - o["r"][k])   # the check still beats the interval
+        assert used >= 1
+    assert brent_stars > 100 and o.s.n_brent - n0 >= brent_stars
+    assert exact_checked > 50 and tight > exact_checked // 2
+
+
+def test_find_root_vr_catches_a_loose_solver(cm, O):
+    """The exact-root check is tight enough to reject a solver that stops early:
+    the interval midpoint, or the end of a bracket 1e-9 wide, fails it."""
+    cl = small_cluster(cm, 3000, seed=21, kroupa=True)
+    o = make_oracle(O, cl, q_mode=1)
+    o.init()
+    j = 1500
+    E = o["E"][j] + o["m"][j] / o["r"][j] / o.s.N_STAR
+    J = o["J"][j]
+    res = o.apsides(j, E, J)
+    k = res["kmax"]
+    want = float(_exact_interval_root(o, j, k, E, J))
+    mid = 0.5 * (o["r"][k] + o["r"][k + 1])
+    assert abs(mid - want) > _brent_tol(want)
+    assert abs((want + 1e-9 * want) - want) > _brent_tol(want)
+
+
 # ------------------------------------------------------------------ relaxation
 def test_relaxation_pair_invariants(cm, O):
     cl = cm.make_plummer(2000, seed=6, kroupa=True)
@@ -206,37 +349,141 @@ def test_relaxation_pair_invariants(cm, O):
         assert o["vr"][N] == vr0[N]                # unpaired last star untouched
 
 
-def test_relaxation_deflection_angle(O):
-    """One pair in isolation: the relative velocity keeps its length and turns by
-    exactly beta = (pi/2) sqrt(dt/Trel12) (cmc_dynamics.c:429-430,467-469)."""
-    n = 44
-    r = np.linspace(1.0, 2.0, n)
-    rng = np.random.default_rng(1)
-    vr, vt = rng.normal(size=n) * 0.3, np.abs(rng.normal(size=n)) * 0.3
-    arr = {"r": np.concatenate([[1e-20], r, [1e10]]), "m": np.ones(n + 2), "vr": np.concatenate([[0], vr, [0]]),
-           "vt": np.concatenate([[0], vt, [0]])}
+def _kick_setup(n, seed, unequal):
+    """A shell of n stars between r = 1 and 2 with random velocities (and masses)."""
+    rng = np.random.default_rng(seed)
+    r = np.sort(1.0 + rng.random(n))
+    vr, vt = rng.normal(size=n) * 0.3, np.abs(rng.normal(size=n)) * 0.3 + 0.05
+    m = rng.uniform(0.2, 3.0, n) if unequal else np.ones(n)
+    m *= n / m.sum()                                 # stored masses are m * N_STAR
+    pad = lambda x, a, b: np.concatenate([[a], x, [b]])
+    return {"r": pad(r, 1e-20, 1e10), "m": pad(m, 0.0, 0.0), "vr": pad(vr, 0.0, 0.0), "vt": pad(vt, 0.0, 0.0)}
+
+
+def _applied_angles(arr, o, k, kp):
+    """The angle by which the oracle turned the pair's relative velocity w, recovered
+    from its outputs alone: w_new's radial part is vr'_kp - vr'_k; its transverse part
+    follows from the new transverse speeds and the conserved centre-of-mass velocity
+    up to a reflection about that velocity, so both candidates are returned (with the
+    two angles, |w_new| and the conditioning of the reflection)."""
+    m_k, m_kp = arr["m"][k], arr["m"][kp]
+    mu, mup = m_k / (m_k + m_kp), m_kp / (m_k + m_kp)
+    ph = o["Y"][k] * 2.0 * np.pi
+    vt_k, vr_k, vt_kp, vr_kp = arr["vt"][k], arr["vr"][k], arr["vt"][kp], arr["vr"][kp]
+    v_t, vp_t = np.array([vt_k, 0.0]), np.array([vt_kp * np.cos(ph), vt_kp * np.sin(ph)])
+    w = np.array([vp_t[0] - v_t[0], vp_t[1] - v_t[1], vr_kp - vr_k])
+    W = np.linalg.norm(w)
+    wn3 = o["vr"][kp] - o["vr"][k]
+    c = mu * v_t + mup * vp_t                        # centre-of-mass transverse velocity
+    wt2 = W * W - wn3 * wn3                          # |w_new_t|^2 if |w_new| = W
+    cw = (o["vt"][kp] ** 2 - o["vt"][k] ** 2 - (mu - mup) * wt2) / 2.0
+    c2 = c @ c
+    a = cw / np.sqrt(c2)
+    b2 = wt2 - a * a
+    chat = c / np.sqrt(c2)
+    cperp = np.array([-chat[1], chat[0]])
+    out = []
+    for sgn in (1.0, -1.0):
+        wt = a * chat + sgn * np.sqrt(max(b2, 0.0)) * cperp
+        wn = np.array([wt[0], wt[1], wn3])
+        out.append(np.arctan2(np.linalg.norm(np.cross(w, wn)), w @ wn))
+    # |w_new| from the first new speed (an independent check of |w_new| = |w|)
+    wt_from_k = c - mup * (a * chat + np.sqrt(max(b2, 0.0)) * cperp)
+    return out, W, b2 / max(wt2, 1e-300), (o["vt"][k], np.linalg.norm(wt_from_k))
+
+
+def _beta_expected(arr, n, k, kp, W, dt, p=20):
+    """beta = (pi/2) sqrt(dt/Trel12), Trel12 = (pi/32) W^3 / (N_STAR n_local
+    ((m_k + m_kp) madhoc)^2), n_local = 2p (3/4pi) / (r_kmax^3 - r_kmin^3) over the
+    window k-p .. k+p of calc_n_local (cmc_dynamics.c:429-430,444-446,
+    cmc_dynamics_helper.c:768-786)."""
+    kmin, kmax = k - p, k + p
+    if kmin < 0:
+        kmin, kmax = 0, 2 * p + 1
+    elif kmax > n:
+        kmin, kmax = n - 2 * p - 1, n
+    r = arr["r"]
+    nloc = 2.0 * p * 3.0 / (4.0 * np.pi * (r[kmax] ** 3 - r[kmin] ** 3))
+    mm = (arr["m"][k] + arr["m"][kp]) / n
+    trel = (np.pi / 32.0) * W ** 3 / (n * nloc * mm * mm)
+    return min(np.pi / 2, (np.pi / 2) * np.sqrt(dt / trel))
+
+
+@pytest.mark.parametrize("unequal", [False, True])
+def test_relaxation_kick_angle(O, unequal):
+    """The oracle's relaxation turns each pair's relative velocity w by the angle
+    beta of cmc_dynamics.c:429-446 and keeps |w| and the pair momentum: the angle is
+    measured between w and the w_new recovered from the oracle's output velocities,
+    and compared with beta computed here from the pair's W, masses and the density
+    window.  Then beta^2 is proportional to dt (same draws, dt x 4 doubles every
+    unclamped angle) and a very long dt clamps every angle at pi/2 exactly."""
+    n = 400
+    arr = _kick_setup(n, 7 + unequal, unequal)
+    runs = {}
+    for dt in (2e-4, 8e-4, 1e6):
+        o = O.Oracle(arr, n)
+        o.potential_calculate()
+        o.relax(dt)
+        assert o.s.error == 0
+        runs[dt] = o
+    checked = 0
+    ratios = []
+    for k in range(1, n, 2):
+        kp = k + 1
+        m_k, m_kp = arr["m"][k], arr["m"][kp]
+        # radial momentum of the pair (exactly conserved up to rounding)
+        for o in runs.values():
+            p0 = m_k * arr["vr"][k] + m_kp * arr["vr"][kp]
+            p1 = m_k * o["vr"][k] + m_kp * o["vr"][kp]
+            assert p1 == pytest.approx(p0, rel=1e-12, abs=1e-13 * (m_k + m_kp))
+        got = {}
+        for dt, o in runs.items():
+            cands, W, cond, (vtk_out, vtk_rec) = _applied_angles(arr, o, k, kp)
+            beta = _beta_expected(arr, n, k, kp, W, dt)
+            if cond < 1e-4:
+                continue                              # reflection ill-conditioned: skip the pair
+            err = min(abs(c - beta) for c in cands)
+            assert err <= 1e-9 * max(1.0, beta), (k, dt, cands, beta)
+            got[dt] = beta
+        if len(got) == 3:
+            checked += 1
+            assert got[1e6] == np.pi / 2
+            if got[8e-4] < np.pi / 2:
+                ratios.append(got[8e-4] / got[2e-4])
+    assert checked > n // 4
+    assert len(ratios) > 10 and np.allclose(ratios, 2.0, rtol=1e-12)
+    # the clamp: w_new is perpendicular to w for every pair
+    o = runs[1e6]
+    for k in range(1, n, 2):
+        cands, W, cond, _ = _applied_angles(arr, o, k, k + 1)
+        if cond >= 1e-4:
+            assert min(abs(c - np.pi / 2) for c in cands) <= 1e-9
+
+
+def test_relaxation_kick_angle_catches_a_wrong_constant(O):
+    """The check above fails for a beta that is off by a plausible mistake (Trel12
+    without the pi/32, or sqrt(dt/Trel) replaced by dt/Trel): the recovered angles
+    match only the correct formula."""
+    n = 400
+    arr = _kick_setup(n, 9, True)
     o = O.Oracle(arr, n)
     o.potential_calculate()
-    dt = 1e-4
+    dt = 2e-4
     o.relax(dt)
-    k, kp = 21, 22
-    Y = o["Y"][k]
-    ph = Y * 2 * np.pi
-    w0 = np.array([vt[kp - 1] * np.cos(ph) - vt[k - 1], vt[kp - 1] * np.sin(ph), vr[kp - 1] - vr[k - 1]])
-    W = np.linalg.norm(w0)
-    # after: velocities of k and kp are v -/+ (1/2)(w_new - w) for equal masses; |w_new| = W
-    dv = np.hypot(o["vr"][kp] - o["vr"][k], 0)  # radial part of w_new
-    p = 20
-    nloc = (2.0 * p) * 3.0 / (4.0 * np.pi * (r[k + p - 1] ** 3 - r[k - p - 1] ** 3))
-    Trel = (np.pi / 32) * W ** 3 / (n * nloc * (2.0 / n) ** 2)
-    beta = min(np.pi / 2, (np.pi / 2) * np.sqrt(dt / Trel))
-    # kinetic energy in the centre-of-mass frame is W^2/4 per unit mass both before and after
-    vk2 = o["vr"][k] ** 2 + o["vt"][k] ** 2
-    vkp2 = o["vr"][kp] ** 2 + o["vt"][kp] ** 2
-    v02 = vr[k - 1] ** 2 + vt[k - 1] ** 2 + vr[kp - 1] ** 2 + vt[kp - 1] ** 2
-    assert vk2 + vkp2 == pytest.approx(v02, rel=1e-12)
-    assert 0 < beta < np.pi / 2
-    assert abs(dv) <= W * (1 + 1e-12)
+    miss_const = miss_power = 0
+    for k in range(1, n, 2):
+        cands, W, cond, _ = _applied_angles(arr, o, k, k + 1)
+        if cond < 1e-4:
+            continue
+        beta = _beta_expected(arr, n, k, k + 1, W, dt)
+        trel = (np.pi / 2) ** 2 * dt / beta ** 2 if beta < np.pi / 2 else None
+        if trel is None:
+            continue
+        wrong_const = min(np.pi / 2, (np.pi / 2) * np.sqrt(dt / (trel * 32.0 / np.pi)))
+        wrong_power = min(np.pi / 2, (np.pi / 2) * dt / trel)
+        miss_const += min(abs(c - wrong_const) for c in cands) > 1e-6
+        miss_power += min(abs(c - wrong_power) for c in cands) > 1e-6
+    assert miss_const > 50 and miss_power > 50
 
 
 # ------------------------------------------------------------------ sampling
@@ -327,6 +574,31 @@ def test_step_energy_conservation(cm, O):
     for _ in range(3):
         assert o.step() == 0
     assert abs((o.sc("Etot") + o.sc("Eoops") - e0) / e0) < 2e-5
+
+
+def test_per_star_streams_match_reference_sequence_statistically(cm, O):
+    """The device's random-number layout (one substream per pair and per star) against
+    the reference's single sequential stream (REF_SEQ): after three steps of the same
+    Plummer 1e5 cluster the distributions of r, v_r and v_t agree (two-sample KS),
+    and so do the energy changes."""
+    cl = cm.make_plummer(100000, seed=1)
+    runs = {}
+    for mode in (O.RNG_REF_SEQ, O.RNG_PER_STAR):
+        o = make_oracle(O, cl, rng_mode=mode, q_mode=O.Q_FP64)
+        o.init()
+        e0 = o.sc("Etot")
+        for _ in range(3):
+            assert o.step() == 0
+        runs[mode] = (o, (o.sc("Etot") + o.sc("Eoops") - e0) / abs(e0))
+    a, da = runs[O.RNG_REF_SEQ]
+    b, db = runs[O.RNG_PER_STAR]
+    for k in ("r", "vr", "vt"):
+        x, y = a[k][1:a.N + 1], b[k][1:b.N + 1]
+        res = stats.ks_2samp(x, y)
+        assert res.pvalue > 1e-3, (k, res)
+        # and the two runs are different realisations, not the same draws
+        assert not np.array_equal(np.sort(x), np.sort(y))
+    assert abs(da - db) < 1e-4, (da, db)
 
 
 def test_set_velocities3_excess_chain(O):

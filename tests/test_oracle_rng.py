@@ -79,3 +79,30 @@ def test_orbit_streams_disjoint_windows(O):
     a2 = O.lib.orc_t113_jump(a, 1 << 20, 0)
     assert _draw_ints(O, a2, 4) == _draw_ints(O, b, 4)
     assert 2 * 50000 + 1 < (1 << 20)
+
+
+def test_phases_draw_from_their_layout_substreams(cm, O):
+    """The oracle's relaxation and orbit phases walk the substream layout (3 draws per
+    pair, 2^20 per orbit slot) instead of jumping from the seed for every star; the
+    draws they use are the layout's: Y of pair p is the first draw of
+    orc_stream_relax(step, p), and X of a star accepted on its first try is the first
+    draw of orc_stream_orbit(step, slot)."""
+    cl = cm.make_plummer(3001, seed=4)
+    o = O.Oracle(cl.a, cl.n, orbit_r=cl.rtid, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64, seed=99)
+    o.init()
+    o.s.step = 5
+    dt = O.lib.orc_simul_relax_new(C.byref(o.s))
+    o.relax(dt)
+    seeded = O.seed_state(99)
+    for p in list(range(0, 40)) + list(range(1400, 1500)):
+        st = O.lib.orc_stream_relax(C.byref(seeded), 5, p)
+        assert o["Y"][2 * p + 1] == O.lib.orc_t113_dbl(C.byref(st)), p
+    o.s.step = 6
+    o.orbits()
+    first = 0
+    for j in range(1, o.N + 1):
+        if o["ntry"][j] == 1:
+            st = O.lib.orc_stream_orbit(C.byref(seeded), 6, j)
+            assert o["X"][j] == O.lib.orc_t113_dbl(C.byref(st)), j
+            first += 1
+    assert first > 1000

@@ -14,8 +14,9 @@ harvested during the warm-up steps.  roofline: the dominant kernel (the orbit
 sampler), its algorithmic bytes per launch over its HIP-event time on the
 launching stream; traffic = measured HBM bytes per launch when a counter summary
 exists in profiles/.  cpu_baseline: the CPU oracle (a restatement of the
-reference's serial path, reference RNG order, x87 long-double Q) timed on a
-bounded sample on this host.
+reference's path, reference RNG order, x87 long-double Q) timed for one step of the
+same cluster on this host, on as many threads as the reference would run MPI ranks
+(up to 16) and, beside it, on one core.
 """
 import argparse
 import importlib.util
@@ -49,20 +50,46 @@ def load_package():
     return mod
 
 
-def cpu_baseline(cl):
-    """One Henon step of the oracle (serial C restatement of the reference path,
-    reference RNG order, x87 long-double Q) on the benchmark cluster itself."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cl, ranks):
+    """One Henon step of the CPU oracle (a C restatement of the reference's path, its
+    RNG order, x87 long-double Q) on the benchmark cluster itself, twice: on `ranks`
+    host threads laid out as the reference's MPI ranks (findLimits chunks, per-rank
+    streams 2^80 apart, sample sort; oracle/orc_mpi.c), and on one core (the serial
+    path).  The multi-rank figure is the headline."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
-    o = O.Oracle(cl.a, cl.n, orbit_r=cl.rtid, rng_mode=O.RNG_REF_SEQ, q_mode=O.Q_LONG_DOUBLE)
-    o.init()
-    n0 = o.N
-    t = time.perf_counter()
-    o.step()
-    dt = time.perf_counter() - t
-    return {"value": n0 / dt, "unit": "stars/s", "cores": 1, "kind": "port",
-            "sample": f"one Henon step of the same {cl.n:.0e}-star King W0=7 cluster on one host core "
-                      f"(CPU oracle: serial, reference RNG order, long-double Q), {dt:.1f} s"}
+    model = _cpu_model()
+    res = {}
+    for P in (ranks, 1):
+        o = O.Oracle(cl.a, cl.n, orbit_r=cl.rtid, rng_mode=O.RNG_REF_SEQ, q_mode=O.Q_LONG_DOUBLE)
+        o.init()
+        n0 = o.N
+        R = o.ranks(P)
+        t = time.perf_counter()
+        R.step()
+        dt = time.perf_counter() - t
+        res[P] = (n0 / dt, dt)
+        log(f"cpu baseline, {P} rank(s): {n0 / dt:.3e} stars/s ({dt:.2f} s)")
+        del R, o
+    v, dt = res[ranks]
+    v1, dt1 = res[1]
+    return {"value": v, "unit": "stars/s", "cores": ranks, "kind": "port",
+            "sample": f"one Henon step of the same {cl.n:.0e}-star King W0=7 cluster on {ranks} host threads "
+                      f"({model}) laid out as the reference's MPI ranks (CPU oracle: reference RNG order per rank, "
+                      f"long-double Q), {dt:.2f} s",
+            "serial": {"value": v1, "cores": 1, "seconds": dt1,
+                       "sample": "the same step on one core (the reference's serial path)"}}
 
 
 def ensemble_bench(cm, steps, seed=7):
@@ -122,6 +149,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-ranks", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads for the multi-rank CPU baseline (the GPU box's CPU share is 16)")
     ap.add_argument("--train-iters", type=int, default=1500)
     ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
@@ -249,7 +278,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle step on the same {cl.n} stars")
-        cpu = cpu_baseline(cl)
+        cpu = cpu_baseline(cl, args.cpu_ranks)
         log(f"cpu baseline {cpu['value']:.3e} stars/s")
     out = {
         "metric": "stars/s per Henon step at N=4e6 (plain vs learned-bracket), % HBM peak",

@@ -95,6 +95,11 @@ typedef struct orc_state {
     int64_t n_excess_events;     /* set_velocities3 stars with vnew2 <= 0 (last call) */
     int64_t n_excess_dumps;      /* stars that absorbed excess energy (last call) */
     int32_t libm_mode;           /* ORC_LIBM_*: sin/cos of the relaxation */
+    /* one rank's share (orc_step_ranks): the relaxation's pairs and the orbit phase's
+     * stars are restricted to slots range_lo..range_hi when range_hi > 0, and the
+     * relaxation skips calc_sigma_r when skip_sigma is set */
+    int64_t range_lo, range_hi;
+    int32_t skip_sigma;
 } orc_state;
 
 enum {
@@ -165,6 +170,26 @@ void    orc_compute_energy(orc_state *s);                      /* cmc_utils.c:60
 double  orc_calc_timestep(orc_state *s);                       /* cmc_utils.c:2132-2148 + GetTimeStep */
 void    orc_init(orc_state *s);                                /* cmc.c:151-165 subset: potential, sigma, energy */
 void    orc_step(orc_state *s);                                /* one Henon step, cmc.c:240-407 */
+
+/* ---------------------------------------------------------------- P ranks (CPU baseline)
+ * The reference's MPI layout on P threads of one host: findLimits chunks
+ * (cmc_utils.c:2299-2333, MIN_CHUNK_SIZE = 20), rank i drawing from its own sequential
+ * stream jumped i * 2^80 from the seeded state (set_rng_states, :2364-2374), the
+ * relaxation and orbit phases run by the ranks on their chunks, the sort as a sample
+ * sort, and the rest as at one rank.  At P = 1 it is orc_step in REF_SEQ mode. */
+#define ORC_MAX_RANKS 256
+typedef struct orc_ranks {
+    orc_state *s;
+    int P;
+    orc_t113 st[ORC_MAX_RANKS];  /* each rank's curr_st */
+    int64_t start[ORC_MAX_RANKS], end[ORC_MAX_RANKS];
+} orc_ranks;
+void orc_find_limits(int64_t N, int64_t blk, int P, int64_t *start, int64_t *end);
+int  orc_ranks_init(orc_ranks *R, orc_state *s, int P);
+void orc_step_ranks(orc_ranks *R);
+int64_t orc_sort_by_r_ranks(orc_state *s, int P);
+orc_ranks *orc_ranks_new(orc_state *s, int P);      /* heap-allocated orc_ranks_init */
+void orc_ranks_free(orc_ranks *R);
 
 #ifdef __cplusplus
 }

@@ -44,7 +44,8 @@ class State(C.Structure):
                 + [(k, C.c_int64) for k in ["n_brent", "n_circular", "n_removed_E", "n_removed_tidal",
                                             "n_removed_mass", "n_linscan"]]
                 + [("error", C.c_int32), ("error_index", C.c_int64), ("n_excess_events", C.c_int64),
-                   ("n_excess_dumps", C.c_int64), ("libm_mode", C.c_int32)])
+                   ("n_excess_dumps", C.c_int64), ("libm_mode", C.c_int32), ("range_lo", C.c_int64),
+                   ("range_hi", C.c_int64), ("skip_sigma", C.c_int32)])
 
 
 class OrbitRS(C.Structure):
@@ -88,6 +89,10 @@ _sigs = {
     "orc_comp_mass_percent": (None, [C.c_void_p, DPTR, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]),
     "orc_central_calculate": (C.c_int, [C.c_void_p, C.c_void_p]),
     "orc_step": (None, [_SP]),
+    "orc_ranks_new": (C.c_void_p, [_SP, C.c_int]),
+    "orc_ranks_free": (None, [C.c_void_p]),
+    "orc_step_ranks": (None, [C.c_void_p]),
+    "orc_sort_by_r_ranks": (C.c_int64, [_SP, C.c_int]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(lib, _n)
@@ -181,6 +186,10 @@ class Oracle:
         lib.orc_step(C.byref(self.s))
         return self.s.error
 
+    def ranks(self, P):
+        """The reference's MPI layout on P threads (orc_step_ranks): returns a stepper."""
+        return Ranks(self, P)
+
     def potential_calculate(self):
         return lib.orc_potential_calculate(C.byref(self.s))
 
@@ -221,6 +230,25 @@ class Oracle:
         o = lib.orc_calc_orbit_new(C.byref(self.s), int(j), float(E), float(J))
         return dict(rp=o.rp, ra=o.ra, dQdrp=o.dQdrp, dQdra=o.dQdra, kmin=o.kmin, kmax=o.kmax,
                     circular=o.circular_flag)
+
+
+class Ranks:
+    """P-rank step of an Oracle (its REF_SEQ stream seeds rank 0; rank i is jumped i*2^80)."""
+
+    def __init__(self, o, P):
+        self.o = o
+        self.h = lib.orc_ranks_new(C.byref(o.s), int(P))
+        if not self.h:
+            raise ValueError("bad rank count")
+
+    def step(self):
+        lib.orc_step_ranks(self.h)
+        return self.o.s.error
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_ranks_free(self.h)
+            self.h = None
 
 
 def dd_sincos(x):

@@ -384,7 +384,9 @@ double orc_get_positions(orc_state *s)
 {
     double max_rad = 0.0;
     orc_t113 cur = {{0, 0, 0, 0}};     /* substream of slot j: slot 1's, jumped 2^20 per slot */
-    for (int64_t j = 1; j <= s->N_MAX_NEW; j++) {
+    const int64_t j_lo = s->range_hi > 0 ? s->range_lo : 1;
+    const int64_t j_hi = s->range_hi > 0 ? s->range_hi : s->N_MAX_NEW;
+    for (int64_t j = j_lo; j <= j_hi; j++) {
         double E, J, rmin, rmax, dQdr_min, dQdr_max, g1, g2, F, s0, g0, drds, pot, Q;
         double r = 0.0, vr = 0.0, X = 0.0;
         orc_orbit_rs o;
@@ -395,7 +397,7 @@ double orc_get_positions(orc_state *s)
         if (s->flags) s->flags[j] = 0;
         if (s->pot_rnew) s->pot_rnew[j] = 0.0;
         if (s->rng_mode != ORC_RNG_REF_SEQ)
-            cur = j == 1 ? orc_stream_orbit(&s->seeded, s->step, 1) : orc_t113_jump_pow2(cur, 20);
+            cur = j == j_lo ? orc_stream_orbit(&s->seeded, s->step, (uint64_t)j) : orc_t113_jump_pow2(cur, 20);
         E = s->E[j] + phi_s(s, s->r[j], j);
         J = s->J[j];
 
@@ -571,9 +573,16 @@ void orc_dynamics_apply(orc_state *s, double dt)
 {
     int64_t p = s->avekernel, N = s->N_MAX, si = 1, pair = 0;
     /* pair p's substream starts 3p draws into the step's stream: walk it */
-    orc_t113 cur = orc_stream_relax(&s->seeded, s->step, 0);
-    orc_calc_sigma_r(s);
-    while (si <= N - N % 2 - 1) {
+    int64_t si_end = N - N % 2 - 1;
+    orc_t113 cur;
+    if (s->range_hi > 0) {                   /* one rank's pairs: its chunk holds whole pairs */
+        si = s->range_lo;
+        if (s->range_hi < si_end + 1) si_end = s->range_hi - 1;
+        pair = (si - 1) / 2;
+    }
+    cur = orc_stream_relax(&s->seeded, s->step, (uint64_t)pair);
+    if (!s->skip_sigma) orc_calc_sigma_r(s);
+    while (si <= si_end) {
         int64_t k = si, kp = si + 1;
         double v[4], vp[4], w[4], w1[4], w2[4], w_new[4], v_new[4], vp_new[4];
         double W, phi, n_local, mass_k, mass_kp, P_enc, Trel12, beta, wp, psi;

@@ -175,6 +175,7 @@ _sigs = {
 MLP_NPARAM, MLP_NNORM = 4738, 16
 HOOK_CHAIN_K_DROP = 1
 HOOK_ORBIT_VARIANT = 2
+HOOK_MLP_MFMA = 3
 for _name, _args in _sigs.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args
@@ -450,16 +451,18 @@ class Context:
         return st.as_dict()
 
     def apsides(self, first, E, J):
+        """calc_orbit_new for slots first.. (cmcg_calc_pericenter_and_apocenter)."""
         E = np.ascontiguousarray(E, dtype=np.float64)
         J = np.ascontiguousarray(J, dtype=np.float64)
         n = E.size
         rp, ra, dp, da = (np.zeros(n) for _ in range(4))
         kmin, kmax = np.zeros(n, np.int64), np.zeros(n, np.int64)
         circ = np.zeros(n, np.int32)
-        _chk(lib.cmcg_calc_pericenter_and_apocenter(
+        name = "cmcg_calc_pericenter_and_apocenter"
+        _chk(getattr(lib, name)(
             self.h, int(first), n, _ptr(E, C.c_double), _ptr(J, C.c_double), _ptr(rp, C.c_double),
             _ptr(ra, C.c_double), _ptr(dp, C.c_double), _ptr(da, C.c_double), _ptr(kmin, C.c_int64),
-            _ptr(kmax, C.c_int64), _ptr(circ, C.c_int32)), "cmcg_calc_pericenter_and_apocenter")
+            _ptr(kmax, C.c_int64), _ptr(circ, C.c_int32)), name)
         return dict(rp=rp, ra=ra, dQdrp=dp, dQdra=da, kmin=kmin, kmax=kmax, circular=circ)
 
     def find_zero_Q(self, j, kmin, kmax, E, J):

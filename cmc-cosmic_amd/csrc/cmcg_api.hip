@@ -35,6 +35,8 @@ struct cmcg_ctx {
     // learned brackets
     MlpState mlp{};
     bool mlp_ready = false, harvest = false, learned = false;
+    bool mlp_mfma = false;       // cmcg_mlp_predict on the matrix cores (test hook)
+    bool pred_fixed = false;     // learned brackets from fixed proposals (test hook)
     float margin = 0.05f;        // half-width of the learned brackets in log r
     int64_t adam_t = 0;
     float *io = nullptr;
@@ -367,9 +369,15 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     if (sc.max_blocks < 1024) sc.max_blocks = 1024;   // the stripping passes' fixed grid
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
     sc.sort_aos_cap = n_cap + 4;
+    // about one radial-index bucket per star: 2^10 .. 2^22 buckets
+    sc.ridx_bits = 10;
+    while (sc.ridx_bits < 22 && ((int64_t)1 << sc.ridx_bits) < n_cap + 2) sc.ridx_bits++;
+    if ((e = dalloc(c, &sc.ridx, (size_t)ridx_entries(sc.ridx_bits))) ||
+        (e = dalloc(c, &sc.ridx_tb, (size_t)ridx_tiles(sc.ridx_bits) + 1))) { cmcg_destroy(c); return e; }
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
-        (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)SORT_LONG_RUNS)) ||
-        (e = dalloc(c, &sc.sort_nlong, 1)) ||
+        (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)(sc.sort_long_cap = n_cap / (SORT_RUN_SHORT + 1) + 2))) ||
+        (e = dalloc(c, &sc.sort_huge, (size_t)(sc.sort_huge_cap = n_cap / (SORT_RUN_LONG + 1) + 2))) ||
+        (e = dalloc(c, &sc.sort_nlong, 2)) ||
         (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
         (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) ||
         (e = dalloc(c, (char **)&sc.relax_nib, sizeof(NibbleTables<8>))) ||
@@ -761,8 +769,8 @@ int cmcg_orbits(cmcg_ctx *c, uint64_t step, cmcg_orbit_stats *stats)
         LAUNCH(launch_rng_bases(c->T, c->s, c->sc, c->stream));
         hipEvent_t ka = nullptr, kb = nullptr;
         kernel_events(c, "orbits_kernel", ka, kb);
-        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream, c->learned ? c->mlp.theta : nullptr,
-                             c->mlp.norm, c->margin, ka, kb, c->orb_variant));
+        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream, c->learned ? &c->mlp : nullptr,
+                             c->margin, ka, kb, c->orb_variant, c->pred_fixed));
         if (c->harvest) LAUNCH(launch_mlp_harvest(c->f, c->s, c->kp, c->mlp, c->n_ub, c->stream));
     }
     DevScalars h;
@@ -951,8 +959,8 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
         PhaseTimer t(c, "orbits");
         hipEvent_t ka = nullptr, kb = nullptr;
         kernel_events(c, "orbits_kernel", ka, kb);
-        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream, c->learned ? c->mlp.theta : nullptr,
-                             c->mlp.norm, c->margin, ka, kb, c->orb_variant));
+        LAUNCH(launch_orbits(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream, c->learned ? &c->mlp : nullptr,
+                             c->margin, ka, kb, c->orb_variant, c->pred_fixed));
         if (c->harvest) LAUNCH(launch_mlp_harvest(c->f, c->s, c->kp, c->mlp, nub, c->stream));
     }
     {
@@ -1127,7 +1135,8 @@ int cmcg_mlp_init(cmcg_ctx *c, uint64_t seed, int64_t max_rows)
             (e = dalloc(c, &m.m, MLP_NPARAM)) || (e = dalloc(c, &m.v, MLP_NPARAM)) || (e = dalloc(c, &m.norm, 1)) ||
             (e = dalloc(c, &m.gpart, (size_t)MLP_MAX_BLOCKS * MLP_NPARAM)) || (e = dalloc(c, &m.lpart, MLP_MAX_BLOCKS)) ||
             (e = dalloc(c, &m.loss, 1)) || (e = dalloc(c, &m.rows, (size_t)max_rows * 8)) ||
-            (e = dalloc(c, &m.counter, 1)) || (e = dalloc(c, &m.colsum, 256 * 16)))
+            (e = dalloc(c, &m.counter, 1)) || (e = dalloc(c, &m.colsum, 256 * 16)) ||
+            (e = dalloc(c, &c->sc.pred, (size_t)c->n_cap + 2)))
             return e;
         m.cap = max_rows;
         c->mlp_ready = true;
@@ -1256,7 +1265,8 @@ int cmcg_mlp_predict(cmcg_ctx *c, const float *x, int64_t n, float *y)
     if (!c->mlp_ready) return CMCG_E_STATE;
     if (int e = io_reserve(c, 8 * n)) return e;
     H2D(c->io, x, MLP_NIN * n);
-    LAUNCH(launch_mlp_predict(c->mlp, c->io, n, c->io + MLP_NIN * n, c->stream));
+    if (c->mlp_mfma) LAUNCH(launch_mlp_infer_rows(c->mlp, c->io, n, c->io + MLP_NIN * n, c->stream));
+    else LAUNCH(launch_mlp_predict(c->mlp, c->io, n, c->io + MLP_NIN * n, c->stream));
     D2H(y, c->io + MLP_NIN * n, MLP_NOUT * n);
     return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
 }
@@ -1291,8 +1301,8 @@ int cmcg_autotune(cmcg_ctx *c, int reps, int32_t *variant, int32_t *threads, int
         for (int it = 0; it <= reps; it++) {      // the first run warms up and is not timed
             hipEvent_t ea = it ? a : nullptr, eb = it ? b : nullptr;
             if (launch_rng_bases(c->T, c->s, c->sc, c->stream) != hipSuccess ||
-                launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream, c->learned ? c->mlp.theta : nullptr,
-                              c->mlp.norm, c->margin, ea, eb, v) != hipSuccess ||
+                launch_orbits(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream, c->learned ? &c->mlp : nullptr,
+                              c->margin, ea, eb, v, c->pred_fixed) != hipSuccess ||
                 sync(c) != hipSuccess) { rc = CMCG_E_HIP; break; }
             if (it) {
                 float ms = 0.f;
@@ -1396,6 +1406,17 @@ int cmcg_test_hook(cmcg_ctx *c, int32_t hook, int64_t value)
         if (value < 0 || value >= orbit_variant_count()) return CMCG_E_ARG;
         c->orb_variant = (int)value;
         invalidate_graphs(c);
+        return CMCG_OK;
+    case CMCG_HOOK_PERFECT_BRACKETS:
+        if (value < 0 || value > 1) return CMCG_E_ARG;
+        if (value && !c->mlp_ready) return CMCG_E_STATE;
+        if (value) LAUNCH(launch_pred_from_apsides(c->f, c->s, c->sc, c->n_ub, c->stream));
+        c->pred_fixed = value != 0;
+        invalidate_graphs(c);
+        return sync(c) == hipSuccess ? CMCG_OK : CMCG_E_HIP;
+    case CMCG_HOOK_MLP_MFMA:
+        if (value < 0 || value > 1) return CMCG_E_ARG;
+        c->mlp_mfma = value != 0;
         return CMCG_OK;
     default:
         return CMCG_E_ARG;

@@ -450,6 +450,48 @@ __device__ __noinline__ double find_root_vr(const double *__restrict__ r, const 
 
 // Top levels of the sorted-potential search tree staged in LDS: cr[s] = r[k(s)],
 // cphi[s] = phi[k(s)] with k(s) = min(s * stride, n + 1), s = 0..cs (k(cs) = n + 1).
+// Radial index of the sorted radii: bucket b of a radius x > 0 is
+// (bits(x) - bits(r_1)) >> shift (the IEEE bits of a positive double order like the
+// values), clamped to nb; t[b] = number of nodes k in 1..n whose bucket is below b
+// (t has nb + 2 entries).  Every node up to t[b] lies below x and every node after
+// t[b + 1] above it, so [t[b], t[b + 1] + 1] brackets the largest k with r_k < x.
+struct RIdx {
+    const uint32_t *t;
+    uint64_t lo;
+    int shift;
+    uint32_t nb;
+};
+__device__ __forceinline__ int ridx_shift(uint64_t lo, uint64_t hi, int bits)
+{
+    const uint64_t d = hi > lo ? hi - lo : 0;
+    const int len = d ? 64 - __clzll((long long)d) : 0;
+    return len > bits ? len - bits : 0;
+}
+__device__ __forceinline__ uint32_t ridx_bucket(const RIdx &x, double r)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(r);
+    const uint64_t d = u > x.lo ? (u - x.lo) >> x.shift : 0;
+    return d < x.nb ? (uint32_t)d : x.nb;
+}
+// the index of nodes 1..n (packed (r, phi), n >= 1) with 2^bits buckets
+__device__ __forceinline__ RIdx ridx_of(const uint32_t *t, const double2 *__restrict__ nodes, int64_t n, int bits)
+{
+    RIdx x;
+    x.t = t;
+    x.nb = 1u << bits;
+    x.lo = (uint64_t)__double_as_longlong(nodes[1].x);
+    x.shift = ridx_shift(x.lo, (uint64_t)__double_as_longlong(nodes[n].x), bits);
+    return x;
+}
+
+// [lo, hi] with r_lo < x <= r_hi (x >= r_1, x <= r_{n+1})
+__device__ __forceinline__ void ridx_bracket(const RIdx &x, double r, int32_t &lo, int32_t &hi)
+{
+    const uint32_t b = ridx_bucket(x, r);
+    lo = (int32_t)x.t[b];
+    hi = (int32_t)x.t[b + 1] + 1;
+}
+
 struct Coarse {
     const double *cr, *cphi;
     int64_t stride, n1;          // stride is a multiple of MID_STEP
@@ -459,6 +501,8 @@ struct Coarse {
     // learned bracket proposal: predicted log r_peri, log r_apo and the half-width
     bool hint;
     float lrp, lra, margin;
+    // radial index (t == nullptr: not built)
+    RIdx ri;
 };
 
 constexpr int MID_STEP = 16;
@@ -589,6 +633,22 @@ __device__ __forceinline__ double potential_at_coarse(const double *__restrict__
     return potential_at_coarse_n(SplitNodes{r, phi}, n_max, x, lo, hi, c, bad);
 }
 
+// potential(x) with the index search started from the radial index: one bucket read
+// gives a bracket of a few nodes (the answer is the unique largest k with r_k < x,
+// so any valid bracket gives the full-range search's index), then the caller's
+// bracket [lo, hi] (valid when r_lo < x <= r_hi) narrows it further.
+template <class N>
+__device__ __forceinline__ double potential_at_ridx_n(const N &nd, int64_t n_max, double x, int64_t lo, int64_t hi,
+                                                      const RIdx &ri, bool *bad)
+{
+    if (x == 0.0 || x < nd.R(1)) return potential_at_n(nd, n_max, x, lo, hi, bad);
+    int32_t a, b;
+    ridx_bracket(ri, x, a, b);
+    if (a > lo) lo = a;
+    if (b < hi) hi = b;
+    return potential_at_n(nd, n_max, x, lo, hi, bad);
+}
+
 // Narrow a find_zero_Q bracket [lo, hi] with pred(Q(lo)) true and pred(Q(hi)) false
 // (pred monotone over the bracket) to the mid-node interval holding the sign change.
 // q_lo, q_hi: Q at the bracket ends when known (they guide the first probes only).
@@ -611,6 +671,85 @@ __device__ __forceinline__ void mid_narrow_q(const Coarse &c, double rj, double 
     if (t < t0) { hi = MID_STEP * t0; return; }
     lo = MID_STEP * t;
     if (MID_STEP * (t + 1) < hi) hi = MID_STEP * (t + 1);
+}
+
+// A guess of the radius where Q changes sign between nodes (ra, pa) and (rb, pb),
+// ra < rb: the potential between them taken as the Henon interpolant A/r + B through
+// the two nodes (exactly the reference's interpolation when they are adjacent) plus
+// the star's own term (m_j/r_j below r_j, m_j/r above; judged at ra), so that
+// Q r^2 = 2(E - B - s0) r^2 - 2(A + s1) r - J^2.  The smaller positive root for the
+// pericentre, the larger for the apocentre; -1 when there is none.  Only a guess:
+// every index it leads to is verified by the signs of Q.
+__device__ __forceinline__ double interp_root(double ra, double pa, double rb, double pb, double rj, double mj,
+                                              double inv, double E, double J, bool peri)
+{
+    const double ia = 1.0 / ra, ib = 1.0 / rb;
+    const double A = (pb - pa) / (ib - ia), B = pa - A * ia;
+    double s0 = 0.0, s1 = 0.0;
+    if (ra >= rj) s1 = mj * inv;
+    else s0 = mj * inv / rj;
+    const double a2 = 2.0 * (E - B - s0), a1 = -2.0 * (A + s1), a0 = -J * J;
+    const double disc = a1 * a1 - 4.0 * a2 * a0;
+    if (!(disc >= 0.0) || a2 == 0.0) return -1.0;
+    const double sq = sqrt(disc);
+    const double q = -0.5 * (a1 + (a1 >= 0.0 ? sq : -sq));
+    if (q == 0.0) return -1.0;
+    double x1 = q / a2, x2 = a0 / q;
+    if (x1 > x2) { const double t = x1; x1 = x2; x2 = t; }
+    if (x2 <= 0.0) return -1.0;
+    if (x1 <= 0.0) return x2;
+    return peri ? x1 : x2;
+}
+
+// find_zero_Q's answer on a bracket [lo, hi] whose ends have the bisection's
+// predicate true and false (POSITIVE: the apocentre, predicate Q > 0; otherwise the
+// pericentre, Q < 0), for a Q with one sign change on the bracket: interpolation
+// steps (interp_root on the current bracket ends, turned into a node bracket by the
+// radial index and probed at its two ends) narrow it, then a bisection in the nodes
+// finishes.  Every index kept is a probed node, so the answer is the unique sign
+// change, the index the full-range bisection returns.  (rl, pl), (rh, ph): nodes lo
+// and hi.
+template <bool POSITIVE, class N>
+__device__ __forceinline__ int64_t ridx_zero_Q(const N &nd, const RIdx &ri, double rj, double mj, double inv,
+                                               double E, double J, int64_t lo, int64_t hi, double rl, double pl,
+                                               double rh, double ph)
+{
+    auto pred = [&](double r, double p) {
+        const int sg = q_sign(rj, mj, inv, r, p, E, J);
+        return POSITIVE ? q_gt0(sg) : q_lt0(sg);
+    };
+    int32_t l = (int32_t)lo, h = (int32_t)hi;
+    for (int it = 0; it < 3 && h - l > 8; it++) {
+        const double rg = interp_root(rl, pl, rh, ph, rj, mj, inv, E, J, !POSITIVE);
+        int32_t a, b;
+        if (rg > rl && rg < rh) {
+            ridx_bracket(ri, rg, a, b);
+            a = a < l ? l : (a > h ? h : a);
+            b = b > h ? h : (b < l ? l : b);
+        } else {
+            a = b = l + ((h - l) >> 1);        // no usable guess: a bisection step
+        }
+        if (a > l && a < h) {
+            double r, p;
+            nd.RP(a, r, p);
+            if (pred(r, p)) { l = a; rl = r; pl = p; }
+            else { h = a; rh = r; ph = p; }
+        }
+        if (b > l && b < h) {
+            double r, p;
+            nd.RP(b, r, p);
+            if (pred(r, p)) { l = b; rl = r; pl = p; }
+            else { h = b; rh = r; ph = p; }
+        }
+    }
+    while (h - l > 1) {
+        const int32_t m = l + ((h - l) >> 1);
+        double r, p;
+        nd.RP(m, r, p);
+        if (pred(r, p)) l = m;
+        else h = m;
+    }
+    return l;
 }
 
 struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; int brent; int linscan; int err;
@@ -643,8 +782,18 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
         // log r), padded by 2, accepted only when Q has the bisection's signs at its ends
         const double inv = P.inv_nstar;
         const float ep = __expf(C->margin);
-        int64_t lo = coarse_index_estimate(*C, (double)(__expf(C->lrp) / ep)) - 2;
-        int64_t hi = coarse_index_estimate(*C, (double)(__expf(C->lrp) * ep)) + 2;
+        // node index just below / at or above radius x: exact bounds from the radial
+        // index, estimates padded by two nodes from the LDS samples otherwise
+        auto below = [&](double x) -> int64_t {
+            if (C->ri.t) { int32_t a, b; ridx_bracket(C->ri, x, a, b); return a; }
+            return coarse_index_estimate(*C, x) - 2;
+        };
+        auto above = [&](double x) -> int64_t {
+            if (C->ri.t) { int32_t a, b; ridx_bracket(C->ri, x, a, b); return b; }
+            return coarse_index_estimate(*C, x) + 2;
+        };
+        int64_t lo = below((double)(__expf(C->lrp) / ep));
+        int64_t hi = above((double)(__expf(C->lrp) * ep));
         lo = lo < 0 ? 0 : lo;
         hi = hi > ktemp ? ktemp : hi;
         if (lo < hi && q_lt0(q_sign(rj, mj, inv, nd.R(lo), nd.P(lo), E, J)) &&
@@ -653,8 +802,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
         } else {
             o.hint_misses++;
         }
-        lo = coarse_index_estimate(*C, (double)(__expf(C->lra) / ep)) - 2;
-        hi = coarse_index_estimate(*C, (double)(__expf(C->lra) * ep)) + 2;
+        lo = below((double)(__expf(C->lra) / ep));
+        hi = above((double)(__expf(C->lra) * ep));
         lo = lo < ktemp ? ktemp : lo;
         hi = hi > n_max + 1 ? n_max + 1 : hi;
         if (lo < hi && (lo == ktemp || q_gt0(q_sign(rj, mj, inv, nd.R(lo), nd.P(lo), E, J))) &&
@@ -699,10 +848,28 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
             }
         }
     }
-    if (vp && C) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi, qp_lo, qp_hi);
-    if (va && C) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi, qa_lo, qa_hi);
-    int64_t kmin = find_zero_Q_n(nd, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
-    int64_t kmax = find_zero_Q_n(nd, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
+    // verified brackets: interpolation search through the radial index; otherwise the
+    // middle table and a bisection (which also settles the direction itself)
+    const bool ri = C && C->ri.t;
+    int64_t kmin, kmax;
+    if (ri && vp) {
+        double rl, pl, rh, ph;
+        nd.RP(kp_lo, rl, pl);
+        nd.RP(kp_hi, rh, ph);
+        kmin = ridx_zero_Q<false>(nd, C->ri, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi, rl, pl, rh, ph);
+    } else {
+        if (vp && C) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi, qp_lo, qp_hi);
+        kmin = find_zero_Q_n(nd, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
+    }
+    if (ri && va) {
+        double rl, pl, rh, ph;
+        nd.RP(ka_lo, rl, pl);
+        nd.RP(ka_hi, rh, ph);
+        kmax = ridx_zero_Q<true>(nd, C->ri, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi, rl, pl, rh, ph);
+    } else {
+        if (va && C) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi, qa_lo, qa_hi);
+        kmax = find_zero_Q_n(nd, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
+    }
     if (kmin < 0 || kmax < 0) {
         o.err = CMCG_E_BRACKET;
         o.rp = o.ra = rj; o.kmin = o.kmax = j; o.dQdrp = o.dQdra = 0.0; o.circular = 1;

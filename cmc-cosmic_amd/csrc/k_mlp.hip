@@ -289,6 +289,141 @@ __global__ __launch_bounds__(256) void mlp_harvest_kernel(DevFields f, DevScalar
     row[NIN] = logf((float)f.r_peri[j]);
     row[NIN + 1] = logf((float)f.r_apo[j]);
 }
+
+// ------------------------------------------------------------------ matrix-core inference
+// The network on the matrix cores: a wave evaluates 16 stars at a time as transposed
+// products H1^T = SiLU(W1 X^T + b1), H2^T = SiLU(W2 H1^T + b2), Y^T = W3 H2^T + b3 with
+// 16x16x16 f16 MFMAs (f32 accumulation).  In that orientation the accumulator of one
+// layer (lane l: rows 4(l/16)..+3 of a 16-neuron block, column = star l % 16) is
+// exactly the B operand of the next layer's 16-wide K chunk, so activations never
+// leave the registers; the weights stay in registers as A operands for all of the
+// wave's tiles.  f16 keeps ~3 decimal digits: a bracket proposal only.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+struct MfmaNet {
+    half4_t a1[4], a2[4][4], a3[4];
+    float b1[4][4], b2[4][4], b3[2];
+};
+
+__device__ __forceinline__ void mfma_net_load(const float *__restrict__ theta, MfmaNet &w)
+{
+    const int lane = threadIdx.x & 63, row = lane & 15, quad = lane >> 4;
+    const float *W1 = theta, *B1 = W1 + NH * NIN, *W2 = B1 + NH, *B2 = W2 + NH * NH, *W3 = B2 + NH,
+                *B3 = W3 + NOUT * NH;
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int k = 4 * quad + v;
+            w.a1[ob][v] = (_Float16)(k < NIN ? W1[(16 * ob + row) * NIN + k] : 0.f);
+            w.a3[ob][v] = (_Float16)(row < NOUT ? W3[row * NH + 16 * ob + k] : 0.f);
+            w.b1[ob][v] = B1[16 * ob + k];
+            w.b2[ob][v] = B2[16 * ob + k];
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) w.a2[ob][kc][v] = (_Float16)W2[(16 * ob + row) * NH + 16 * kc + k];
+        }
+    }
+    w.b3[0] = B3[0];
+    w.b3[1] = B3[1];
+}
+
+// xb: this lane's B operand of the standardised inputs (k = 4(l/16) + v of star l % 16,
+// zero beyond the 6 inputs).  Returns the standardised outputs in lanes 0..15.
+__device__ __forceinline__ void mfma_net_forward(const MfmaNet &w, half4_t xb, float &y0, float &y1)
+{
+    half4_t h1[4], h2[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) {
+        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a1[ob], xb, acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) h1[ob][v] = (_Float16)silu(acc[v] + w.b1[ob][v]);
+    }
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) {
+        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a2[ob][kc], h1[kc], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) h2[ob][v] = (_Float16)silu(acc[v] + w.b2[ob][v]);
+    }
+    float4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a3[kc], h2[kc], acc, 0, 0, 0);
+    y0 = acc[0] + w.b3[0];
+    y1 = acc[1] + w.b3[1];
+}
+
+// The standardised inputs of one star as the lane's B operand (lanes with l / 16 = 0
+// hold inputs 0..3, l / 16 = 1 inputs 4..5).
+__device__ __forceinline__ half4_t mfma_inputs(const float (&x)[NIN], const MlpNorm *__restrict__ nrm, bool valid)
+{
+    const int quad = (threadIdx.x & 63) >> 4;
+    half4_t xb = {(_Float16)0.f, (_Float16)0.f, (_Float16)0.f, (_Float16)0.f};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int k = 4 * quad + v;
+        if (valid && k < NIN) xb[v] = (_Float16)((x[k] - nrm->xm[k]) / nrm->xs[k]);
+    }
+    return xb;
+}
+
+// Predictions for every star 1..n of the current state (the orbit phase's proposals):
+// pred[j] = (log r_peri, log r_apo).  Persistent waves over 16-star tiles.
+__global__ __launch_bounds__(256) void mlp_infer_stars_kernel(DevFields f, DevScalars *s, KParams P,
+                                                              const float *__restrict__ theta,
+                                                              const MlpNorm *__restrict__ nrm,
+                                                              float2 *__restrict__ pred)
+{
+    MfmaNet w;
+    mfma_net_load(theta, w);
+    const int lane = threadIdx.x & 63, row = lane & 15, quad = lane >> 4;
+    const int64_t n = s->n_max;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const float ym0 = nrm->ym[0], ys0 = nrm->ys[0], ym1 = nrm->ym[1], ys1 = nrm->ys[1];
+    for (int64_t t = wave; 16 * t < n; t += nwaves) {
+        const int64_t j = 1 + 16 * t + row;
+        float x[NIN];
+        const bool valid = j <= n && quad < 2;
+        if (valid) {
+            const double rj = f.r[j], mj = f.m[j];
+            const double E = f.E[j] + phi_s_own(rj, mj, P.inv_nstar, rj);
+            mlp_features(f.r, f.phi, n, j, rj, mj, E, f.J[j], x);
+        }
+        float y0, y1;
+        mfma_net_forward(w, mfma_inputs(x, nrm, valid), y0, y1);
+        if (quad == 0 && j <= n) pred[j] = make_float2(y0 * ys0 + ym0, y1 * ys1 + ym1);
+    }
+}
+
+// The same network on given raw inputs x[6n] -> y[2n] (tests compare it with the
+// per-thread FP32 path).
+__global__ __launch_bounds__(256) void mlp_infer_rows_kernel(const float *__restrict__ theta,
+                                                             const MlpNorm *__restrict__ nrm,
+                                                             const float *__restrict__ xin, int64_t n,
+                                                             float *__restrict__ y)
+{
+    MfmaNet w;
+    mfma_net_load(theta, w);
+    const int lane = threadIdx.x & 63, row = lane & 15, quad = lane >> 4;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t t = wave; 16 * t < n; t += nwaves) {
+        const int64_t i = 16 * t + row;
+        float x[NIN];
+        const bool valid = i < n && quad < 2;
+        if (valid)
+            for (int c = 0; c < NIN; c++) x[c] = xin[NIN * i + c];
+        float y0, y1;
+        mfma_net_forward(w, mfma_inputs(x, nrm, valid), y0, y1);
+        if (quad == 0 && i < n) {
+            y[NOUT * i] = y0 * nrm->ys[0] + nrm->ym[0];
+            y[NOUT * i + 1] = y1 * nrm->ys[1] + nrm->ym[1];
+        }
+    }
+}
 } // namespace
 
 int mlp_grid_blocks(int64_t batch)
@@ -319,6 +454,29 @@ hipError_t launch_mlp_predict(const MlpState &m, const float *x, int64_t n, floa
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(mlp_predict_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       (const float *)m.theta, (const MlpNorm *)m.norm, x, n, y);
+    return hipGetLastError();
+}
+
+static unsigned mfma_blocks(int64_t tiles)
+{
+    const int64_t b = (tiles + 3) / 4;                // four waves per block
+    return (unsigned)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+hipError_t launch_mlp_infer_stars(const DevFields &f, DevScalars *s, const KParams &P, const MlpState &m,
+                                  float2 *pred, int64_t n_ub, hipStream_t st)
+{
+    if (n_ub <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_infer_stars_kernel, dim3(mfma_blocks((n_ub + 15) / 16)), dim3(256), 0, st, f, s, P,
+                       (const float *)m.theta, (const MlpNorm *)m.norm, pred);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer_rows(const MlpState &m, const float *x, int64_t n, float *y, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_infer_rows_kernel, dim3(mfma_blocks((n + 15) / 16)), dim3(256), 0, st,
                        (const float *)m.theta, (const MlpNorm *)m.norm, x, n, y);
     return hipGetLastError();
 }

@@ -240,7 +240,9 @@ __device__ __forceinline__ bool one_try(const N &nd, const Coarse *C, const KPar
     int64_t lo = tc.lo0, hi = tc.hi0;
     if (!(tc.r_lo0 < rr)) lo = 1;
     if (!(rr <= tc.r_hi0)) hi = n + 1;
-    potr = C ? potential_at_coarse_n(nd, n, rr, lo, hi, *C, &bad) : potential_at_n(nd, n, rr, lo, hi, &bad);
+    potr = !C ? potential_at_n(nd, n, rr, lo, hi, &bad)
+              : C->ri.t ? potential_at_ridx_n(nd, n, rr, lo, hi, C->ri, &bad)
+                        : potential_at_coarse_n(nd, n, rr, lo, hi, *C, &bad);
     const double pot = potr + phi_s_own(tc.rj, tc.mj, P.inv_nstar, rr);
     const double drds = 0.25 * (tc.rmax - tc.rmin) * (3.0 - 3.0 * s0 * s0);
     const double Q = 2.0 * tc.E - 2.0 * pot - tc.J * tc.J / rr / rr;
@@ -342,12 +344,12 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
                                                              const Rng *__restrict__ wave_base,
                                                              double *__restrict__ partials,
                                                              int64_t *__restrict__ defer_list,
-                                                             const float *__restrict__ theta,
-                                                             const MlpNorm *__restrict__ nrm, float margin,
+                                                             const float2 *__restrict__ pred, float margin,
                                                              const double2 *__restrict__ mid,
                                                              const double2 *__restrict__ nodes,
                                                              const double2 *__restrict__ coarse,
-                                                             const NibbleTables<ORB_NIB_POWERS> *__restrict__ nib)
+                                                             const NibbleTables<ORB_NIB_POWERS> *__restrict__ nib,
+                                                             const uint32_t *__restrict__ ridx, int ridx_bits)
 {
     // the jump tables are needed only for the lanes' first states, the try constants
     // only after the apsides: one LDS area serves both
@@ -362,6 +364,8 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
     C.stride = coarse_stride(n, COARSE);
     C.hint = false; C.lrp = C.lra = 0.f; C.margin = margin;
     C.mid = mid;
+    C.ri.t = nullptr;
+    if (ridx && n >= 1) C.ri = ridx_of(ridx, nodes, n, ridx_bits);
     // both tables are built once per phase: contiguous copies
     for (int sI = threadIdx.x; sI <= COARSE; sI += blockDim.x) {
         const double2 v = coarse[sI];
@@ -403,13 +407,11 @@ __global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars
                 if ((int64_t)slot < MAX_DEFER) defer_list[slot] = j;
                 else dev_set_error(s, CMCG_E_STATE, j);
             } else {
-                if (theta) {
-                    float x[MLP_NIN], y[MLP_NOUT];
-                    mlp_features(f.r, f.phi, n, j, rj, mj, E, J, x);
-                    mlp_forward(theta, nrm, x, y);
+                if (pred) {
+                    const float2 y = pred[j];
                     C.hint = true;
-                    C.lrp = y[0];
-                    C.lra = y[1];
+                    C.lrp = y.x;
+                    C.lra = y.y;
                 }
                 need = prepare_star(PackedNodes{nodes}, f, s, P, &C, n, j, rj, mj, E, J, j, flags, hits, misses,
                                     sh.ctx[threadIdx.x]);
@@ -623,6 +625,74 @@ __global__ void frv_kernel(DevFields f, DevScalars *s, KParams P, int64_t count,
     if (bad) dev_set_error(s, CMCG_E_BRENT, j);
 }
 
+__global__ void pred_from_apsides_kernel(DevFields f, DevScalars *s, float2 *__restrict__ pred)
+{
+    const int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > s->n_max) return;
+    const double rp = f.r_peri[j], ra = f.r_apo[j];
+    pred[j] = make_float2(rp > 0.0 ? (float)log(rp) : 0.f, ra > 0.0 ? (float)log(ra) : 0.f);
+}
+
+// ------------------------------------------------------------------ radial index
+__global__ void ridx_clear_kernel(uint32_t *__restrict__ t, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = 0;
+}
+
+// Node k (0..n; node 0 stands for "below every bucket"): the last node of each bucket
+// writes its index at that bucket, and every multiple of RIDX_TILE in
+// (bucket(r_k), bucket(r_{k+1})] gets k in the tile table.
+__global__ void ridx_mark_kernel(const double2 *__restrict__ nodes, DevScalars *s, uint32_t *__restrict__ t,
+                                 uint32_t *__restrict__ tb, int bits)
+{
+    const int64_t n = s->n_max;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    const int64_t nb = (int64_t)1 << bits;
+    int64_t bk = -1, bn = nb + 1;
+    if (n >= 1) {
+        const RIdx x = ridx_of(t, nodes, n, bits);
+        if (k >= 1) bk = ridx_bucket(x, nodes[k].x);
+        if (k < n) bn = ridx_bucket(x, nodes[k + 1].x);
+        if (k >= 1 && bn != bk) t[bk] = (uint32_t)k;
+    }
+    for (int64_t m = (bk + RIDX_TILE) / RIDX_TILE; m * RIDX_TILE <= bn; m++) tb[m] = (uint32_t)k;
+}
+
+// t[b] = the largest marked index below b in its tile, or the tile's entry value:
+// an exclusive running maximum, in place (16 buckets per thread).
+__global__ __launch_bounds__(256) void ridx_scan_kernel(uint32_t *__restrict__ t, const uint32_t *__restrict__ tb,
+                                                        int bits)
+{
+    __shared__ uint32_t pre[256];
+    const int64_t entries = ridx_entries(bits);
+    const int64_t i0 = (int64_t)blockIdx.x * RIDX_TILE + 16 * threadIdx.x;
+    uint32_t v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = i0 + q < entries ? t[i0 + q] : 0u;
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const uint32_t e = v[q];
+        v[q] = m;
+        m = e > m ? e : m;
+    }
+    pre[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {          // inclusive max-scan of the thread totals
+        const uint32_t o = threadIdx.x >= (unsigned)off ? pre[threadIdx.x - off] : 0u;
+        __syncthreads();
+        pre[threadIdx.x] = o > pre[threadIdx.x] ? o : pre[threadIdx.x];
+        __syncthreads();
+    }
+    uint32_t start = tb[blockIdx.x];
+    if (threadIdx.x > 0 && pre[threadIdx.x - 1] > start) start = pre[threadIdx.x - 1];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        if (i0 + q < entries) t[i0 + q] = v[q] > start ? v[q] : start;
+}
+
 // The LDS samples of the (r, phi) table for a COARSE-sample launch: node
 // min(s * stride, n + 1) for s = 0..cs, from the packed nodes.
 __global__ void coarse_build_kernel(const double2 *__restrict__ nodes, DevScalars *s, int cs,
@@ -648,8 +718,8 @@ __global__ void coarse_build_kernel(const double2 *__restrict__ nodes, DevScalar
 // compiler for 5 waves per SIMD instead of 4 spills about 100 VGPRs and ran 40%
 // slower.)
 using OrbFn = void (*)(DevFields, DevScalars *, KParams, const RngTable *, const Rng *, double *, int64_t *,
-                       const float *, const MlpNorm *, float, const double2 *, const double2 *, const double2 *,
-                       const NibbleTables<ORB_NIB_POWERS> *);
+                       const float2 *, float, const double2 *, const double2 *, const double2 *,
+                       const NibbleTables<ORB_NIB_POWERS> *, const uint32_t *, int);
 struct OrbVariant { OrbFn fn; int threads, coarse, cap; };
 const OrbVariant kOrbVariants[] = {
     {orbits_kernel<256, 1024, 64>, 256, 1024, 64}, {orbits_kernel<256, 512, 64>, 256, 512, 64},
@@ -668,8 +738,8 @@ int bits_needed(uint64_t x)
 } // namespace
 
 hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
-                         const Scratch &sc, int64_t n_ub, hipStream_t st, const float *theta, const MlpNorm *nrm,
-                         float margin, hipEvent_t ev_a, hipEvent_t ev_b, int variant)
+                         const Scratch &sc, int64_t n_ub, hipStream_t st, const MlpState *mlp,
+                         float margin, hipEvent_t ev_a, hipEvent_t ev_b, int variant, bool pred_fixed)
 {
     const OrbVariant &V = kOrbVariants[variant >= 0 && variant < ORB_NVARIANTS ? variant : 0];
     int64_t nblocks = (n_ub + V.threads - 1) / V.threads;
@@ -678,14 +748,22 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     int jb1 = 20 + bits_needed((uint64_t)n_ub);
     if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
     if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
+    if (hipError_t e = launch_ridx_build(s, sc, n_ub, st)) return e;
     if (V.coarse > ORB_MAX_COARSE) return hipErrorInvalidValue;
     hipLaunchKernelGGL(coarse_build_kernel, dim3((unsigned)((V.coarse + 256) / 256)), dim3(256), 0, st,
                        (const double2 *)sc.nodes, s, V.coarse, sc.coarse);
+    const float2 *pred = nullptr;
+    if (mlp) {
+        if (!sc.pred) return hipErrorInvalidValue;
+        if (!pred_fixed)
+            if (hipError_t e = launch_mlp_infer_stars(f, s, P, *mlp, sc.pred, n_ub, st)) return e;
+        pred = sc.pred;
+    }
     if (ev_a) hipEventRecord(ev_a, st);
     hipLaunchKernelGGL(V.fn, dim3((unsigned)nblocks), dim3(V.threads), 0, st, f, s, P, T, (const Rng *)sc.rng_wave,
-                       sc.partials, sc.defer_list, theta, nrm, margin, (const double2 *)sc.mid,
+                       sc.partials, sc.defer_list, pred, margin, (const double2 *)sc.mid,
                        (const double2 *)sc.nodes, (const double2 *)sc.coarse,
-                       (const NibbleTables<ORB_NIB_POWERS> *)sc.orb_nib);
+                       (const NibbleTables<ORB_NIB_POWERS> *)sc.orb_nib, (const uint32_t *)sc.ridx, sc.ridx_bits);
     if (ev_b) hipEventRecord(ev_b, st);
     hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
@@ -696,6 +774,27 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
                        sc.red2);
     hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(RED2_BLOCKS * RED_STRIDE), 0, st, s, (const double *)sc.red2,
                        (const double *)sc.defer_partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_pred_from_apsides(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub,
+                                    hipStream_t st)
+{
+    if (!sc.pred) return hipErrorInvalidValue;
+    if (n_ub > 0)
+        hipLaunchKernelGGL(pred_from_apsides_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f, s,
+                           sc.pred);
+    return hipGetLastError();
+}
+
+hipError_t launch_ridx_build(DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st)
+{
+    const int64_t entries = ridx_entries(sc.ridx_bits);
+    hipLaunchKernelGGL(ridx_clear_kernel, dim3(512), dim3(256), 0, st, sc.ridx, entries);
+    hipLaunchKernelGGL(ridx_mark_kernel, dim3((unsigned)((n_ub + 1 + 256) / 256)), dim3(256), 0, st,
+                       (const double2 *)sc.nodes, s, sc.ridx, sc.ridx_tb, sc.ridx_bits);
+    hipLaunchKernelGGL(ridx_scan_kernel, dim3((unsigned)ridx_tiles(sc.ridx_bits)), dim3(256), 0, st, sc.ridx,
+                       (const uint32_t *)sc.ridx_tb, sc.ridx_bits);
     return hipGetLastError();
 }
 

@@ -11,7 +11,9 @@
 // histogram, per-digit tile scan, stable scatter).  The scatter ranks each 4096-word
 // tile with 64-lane ballot multisplit (stable in tile order), stages the sorted tile
 // in LDS and writes it out digit-run by digit-run, so global writes are coalesced.
-// Runs of equal z are then put in (r, id) order, and one gather moves the payload.
+// Runs of equal z are then put in (r, id) order (in place up to 64 words, by a bitonic
+// sort in LDS up to 4096, by a merge sort in global memory beyond: any input sorts),
+// and one gather moves the payload.
 #include "kernels.hpp"
 
 namespace {
@@ -26,8 +28,8 @@ constexpr int RS_PASSES = 4;
 constexpr int RS_SHIFT0 = 64 - 8 * RS_PASSES;     // z = word >> RS_SHIFT0
 constexpr uint32_t Z_TAIL = 0xffffffffu;          // z of the SF_INFINITY tail; live z < 2^31
 static_assert(RS_PASSES % 2 == 0, "the sorted words must end in keys_a");
-constexpr int RUN_SHORT = 64;
-constexpr int RUN_LONG = 4096;
+constexpr int RUN_SHORT = SORT_RUN_SHORT;
+constexpr int RUN_LONG = SORT_RUN_LONG;
 constexpr int AOS_W = SORT_AOS_WORDS;   // words per packed payload record (one 128-B line)
 constexpr int AOS_LD = AOS_W + 1;        // odd LDS stride per record: no bank conflicts
 
@@ -50,7 +52,8 @@ __global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n, DevScalars *
     if (i == 0) {
         s->sort_n = 0;
         s->sort_in = s->n_max_new;
-        *n_long = 0;
+        n_long[0] = 0;
+        n_long[1] = 0;
     }
     if (i < n) a[i] = 0;
 }
@@ -330,7 +333,7 @@ __device__ __forceinline__ bool key_id_less(uint64_t ka, int64_t ia, uint64_t kb
 
 __global__ void rs_runs_kernel(uint64_t *__restrict__ words, const double *__restrict__ keysrc,
                                const int64_t *__restrict__ id, DevScalars *s, int64_t *__restrict__ long_runs,
-                               int64_t *__restrict__ n_long)
+                               int64_t *__restrict__ n_long, int64_t long_cap)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max_new;
@@ -343,8 +346,8 @@ __global__ void rs_runs_kernel(uint64_t *__restrict__ words, const double *__res
     while (e + 1 < n && word_z(words[e + 1]) == z && e - i < RUN_SHORT) e++;
     if (e + 1 < n && word_z(words[e + 1]) == z) {
         unsigned long long slot = atomicAdd((unsigned long long *)n_long, 1ull);
-        if ((int64_t)slot < SORT_LONG_RUNS) long_runs[slot] = i;
-        else dev_set_error(s, CMCG_E_TIES, i + 1);
+        if ((int64_t)slot < long_cap) long_runs[slot] = i;          // the list holds n / 65 runs
+        else dev_set_error(s, CMCG_E_STATE, i + 1);
         return;
     }
     for (int64_t a = i + 1; a <= e; a++) {
@@ -368,17 +371,59 @@ __global__ void rs_runs_kernel(uint64_t *__restrict__ words, const double *__res
 // in LDS, padded to a power of two with all-ones entries.  Every word of a run has
 // the run's z, so only the slots travel.
 constexpr int LR_T = 256;
+// Bitonic sort by (r, id) of words[i .. i + len) (len <= RUN_LONG, one z) in LDS.
+__device__ void lds_sort_run(uint64_t *__restrict__ words, const double *__restrict__ keysrc,
+                             const int64_t *__restrict__ id, int64_t i, int64_t len, uint32_t z,
+                             uint64_t *sk, int64_t *sid, uint32_t *sv)
+{
+    const int t = threadIdx.x;
+    int P = 1;
+    while (P < len) P <<= 1;
+    for (int q = t; q < P; q += LR_T) {
+        if (q < len) {
+            const uint32_t v = word_slot(words[i + q]);
+            sk[q] = sortable(keysrc[v]);
+            sv[q] = v;
+            sid[q] = id[v];
+        } else {
+            sk[q] = ~0ull;
+            sid[q] = INT64_MAX;
+            sv[q] = 0u;
+        }
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= P; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int q = t; q < P; q += LR_T) {
+                const int x = q ^ j;
+                if (x > q) {
+                    const bool up = (q & kk) == 0;
+                    const bool gt = key_id_less(sk[x], sid[x], sk[q], sid[q]);
+                    if (gt == up) {
+                        uint64_t tk = sk[q]; sk[q] = sk[x]; sk[x] = tk;
+                        int64_t ti = sid[q]; sid[q] = sid[x]; sid[x] = ti;
+                        uint32_t tv = sv[q]; sv[q] = sv[x]; sv[x] = tv;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int q = t; q < len; q += LR_T) words[i + q] = ((uint64_t)z << RS_SHIFT0) | sv[q];
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict__ words,
                                                             const double *__restrict__ keysrc,
                                                             const int64_t *__restrict__ id, DevScalars *s,
                                                             const int64_t *__restrict__ long_runs,
-                                                            const int64_t *__restrict__ n_long)
+                                                            int64_t *__restrict__ n_long, int64_t long_cap,
+                                                            int64_t *__restrict__ huge_runs, int64_t huge_cap)
 {
     __shared__ uint64_t sk[RUN_LONG];
     __shared__ int64_t sid[RUN_LONG];
     __shared__ uint32_t sv[RUN_LONG];
     __shared__ int64_t len_s;
-    const int64_t nl = *n_long < SORT_LONG_RUNS ? *n_long : SORT_LONG_RUNS;
+    const int64_t nl = n_long[0] < long_cap ? n_long[0] : long_cap;
     const int64_t n = s->n_max_new;
     const int t = threadIdx.x;
     for (int64_t r = blockIdx.x; r < nl; r += gridDim.x) {
@@ -388,7 +433,7 @@ __global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict
             // words are ordered by z: the last index with the run's z by bisection
             int64_t lo = i, hi = i + RUN_LONG < n ? i + RUN_LONG : n - 1;
             if (word_z(words[hi]) == z) {
-                len_s = hi == i + RUN_LONG ? -1 : hi - i + 1;     // RUN_LONG + 1 words do not fit
+                len_s = hi == i + RUN_LONG ? -1 : hi - i + 1;     // more than RUN_LONG words
             } else {
                 while (hi - lo > 1) {
                     const int64_t m = lo + ((hi - lo) >> 1);
@@ -397,44 +442,88 @@ __global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict
                 }
                 len_s = lo - i + 1;
             }
-            if (len_s < 0) dev_set_error(s, CMCG_E_TIES, i + 1);
+            if (len_s < 0) {                                       // left to the merge sort
+                unsigned long long slot = atomicAdd((unsigned long long *)(n_long + 1), 1ull);
+                if ((int64_t)slot < huge_cap) huge_runs[slot] = i;
+                else dev_set_error(s, CMCG_E_STATE, i + 1);
+            }
         }
         __syncthreads();
         const int64_t len = len_s;
         __syncthreads();
         if (len < 0) continue;
-        int P = 1;
-        while (P < len) P <<= 1;
-        for (int q = t; q < P; q += LR_T) {
-            if (q < len) {
-                const uint32_t v = word_slot(words[i + q]);
-                sk[q] = sortable(keysrc[v]);
-                sv[q] = v;
-                sid[q] = id[v];
-            } else {
-                sk[q] = ~0ull;
-                sid[q] = INT64_MAX;
-                sv[q] = 0u;
+        lds_sort_run(words, keysrc, id, i, len, z, sk, sid, sv);
+    }
+}
+
+// Runs longer than RUN_LONG (one block per run): chunks of RUN_LONG sorted in LDS,
+// then bottom-up merges between words and tmp (the same positions of the second key
+// buffer), each merge split among the threads along its merge path.
+__device__ __forceinline__ bool word_less(uint64_t wa, uint64_t wb, const double *__restrict__ keysrc,
+                                          const int64_t *__restrict__ id)
+{
+    const uint32_t va = word_slot(wa), vb = word_slot(wb);
+    return key_id_less(sortable(keysrc[va]), id[va], sortable(keysrc[vb]), id[vb]);
+}
+
+__global__ __launch_bounds__(LR_T) void rs_huge_runs_kernel(uint64_t *__restrict__ words, uint64_t *__restrict__ tmp,
+                                                            const double *__restrict__ keysrc,
+                                                            const int64_t *__restrict__ id, DevScalars *s,
+                                                            const int64_t *__restrict__ huge_runs,
+                                                            const int64_t *__restrict__ n_long, int64_t huge_cap)
+{
+    __shared__ uint64_t sk[RUN_LONG];
+    __shared__ int64_t sid[RUN_LONG];
+    __shared__ uint32_t sv[RUN_LONG];
+    __shared__ int64_t len_s;
+    const int64_t nh = n_long[1] < huge_cap ? n_long[1] : huge_cap;
+    const int64_t n = s->n_max_new;
+    const int t = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < nh; r += gridDim.x) {
+        const int64_t i = huge_runs[r];
+        const uint32_t z = word_z(words[i]);
+        if (t == 0) {
+            int64_t lo = i, hi = n - 1;                // last index with the run's z
+            if (word_z(words[hi]) == z) lo = hi;
+            while (hi - lo > 1) {
+                const int64_t m = lo + ((hi - lo) >> 1);
+                if (word_z(words[m]) == z) lo = m;
+                else hi = m;
             }
+            len_s = lo - i + 1;
         }
         __syncthreads();
-        for (int kk = 2; kk <= P; kk <<= 1)
-            for (int j = kk >> 1; j > 0; j >>= 1) {
-                for (int q = t; q < P; q += LR_T) {
-                    const int x = q ^ j;
-                    if (x > q) {
-                        const bool up = (q & kk) == 0;
-                        const bool gt = key_id_less(sk[x], sid[x], sk[q], sid[q]);
-                        if (gt == up) {
-                            uint64_t tk = sk[q]; sk[q] = sk[x]; sk[x] = tk;
-                            int64_t ti = sid[q]; sid[q] = sid[x]; sid[x] = ti;
-                            uint32_t tv = sv[q]; sv[q] = sv[x]; sv[x] = tv;
-                        }
-                    }
+        const int64_t len = len_s;
+        __syncthreads();
+        for (int64_t c = 0; c < len; c += RUN_LONG)
+            lds_sort_run(words, keysrc, id, i + c, len - c < RUN_LONG ? len - c : RUN_LONG, z, sk, sid, sv);
+        uint64_t *src = words + i, *dst = tmp + i;
+        for (int64_t w = RUN_LONG; w < len; w <<= 1) {
+            for (int64_t p = 0; p < len; p += 2 * w) {
+                const int64_t la = len - p < w ? len - p : w;
+                const int64_t lb = len - p - la < w ? len - p - la : w;
+                const uint64_t *A = src + p, *B = src + p + la;
+                const int64_t L = la + lb;
+                const int64_t d0 = L * t / LR_T, d1 = L * (t + 1) / LR_T;
+                // merge path: the first d0 outputs hold A[0, ia) and B[0, d0 - ia); A
+                // goes first on equal keys.  A[m] is among them iff A[m] <= B[d0 - 1 - m].
+                int64_t a_lo = d0 > lb ? d0 - lb : 0, a_hi = d0 < la ? d0 : la;
+                while (a_lo < a_hi) {
+                    const int64_t m = (a_lo + a_hi) >> 1;
+                    if (!word_less(B[d0 - 1 - m], A[m], keysrc, id)) a_lo = m + 1;
+                    else a_hi = m;
                 }
-                __syncthreads();
+                int64_t ia = a_lo, ib = d0 - a_lo;
+                for (int64_t d = d0; d < d1; d++) {
+                    const bool takeA = ib >= lb || (ia < la && !word_less(B[ib], A[ia], keysrc, id));
+                    dst[p + d] = takeA ? A[ia++] : B[ib++];
+                }
             }
-        for (int q = t; q < len; q += LR_T) words[i + q] = ((uint64_t)z << RS_SHIFT0) | sv[q];
+            __syncthreads();
+            uint64_t *x = src; src = dst; dst = x;
+        }
+        if (src != words + i)
+            for (int64_t q = t; q < len; q += LR_T) words[i + q] = src[q];
         __syncthreads();
     }
 }
@@ -543,9 +632,12 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     }
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
     hipLaunchKernelGGL(rs_runs_kernel, dim3(g1), dim3(256), 0, st, kin, key_src, id_src, s, sc.sort_long,
-                       sc.sort_nlong);
+                       sc.sort_nlong, sc.sort_long_cap);
     hipLaunchKernelGGL(rs_long_runs_kernel, dim3(64), dim3(LR_T), 0, st, kin, key_src, id_src, s,
-                       (const int64_t *)sc.sort_long, (const int64_t *)sc.sort_nlong);
+                       (const int64_t *)sc.sort_long, sc.sort_nlong, sc.sort_long_cap, sc.sort_huge,
+                       sc.sort_huge_cap);
+    hipLaunchKernelGGL(rs_huge_runs_kernel, dim3(16), dim3(LR_T), 0, st, kin, kout, key_src, id_src, s,
+                       (const int64_t *)sc.sort_huge, (const int64_t *)sc.sort_nlong, sc.sort_huge_cap);
     hipLaunchKernelGGL(rs_count_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s);
     if (aos)
         hipLaunchKernelGGL(rs_gather_aos_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,

@@ -12,12 +12,16 @@
 constexpr int ORB_MAX_COARSE = 2048;
 // sort payload records: 16 words (128 B, one cache line) per star
 constexpr int SORT_AOS_WORDS = 16;
-constexpr int SORT_LONG_RUNS = 4096;
+constexpr int SORT_RUN_SHORT = 64;   // runs of equal sort keys up to this length are ordered in place
+constexpr int SORT_RUN_LONG = 4096;  // ... up to this length in LDS; longer ones by a merge sort
 constexpr int SORT_MM_PARTS = 1024;   // blocks of the sort's min/max pass, one (min, max) pair each
 constexpr int RED2_BLOCKS = 64;
 constexpr int ORB_NIB_B0 = 20, ORB_NIB_POWERS = 6;
 
 constexpr int RNG_WB_POWERS = 10;   // nibble-table powers of the wave-base jumps
+constexpr int RIDX_TILE = 4096;     // buckets per block of the radial index's scan
+__host__ __device__ constexpr int64_t ridx_entries(int bits) { return ((int64_t)1 << bits) + 2; }
+__host__ __device__ constexpr int64_t ridx_tiles(int bits) { return (ridx_entries(bits) + RIDX_TILE - 1) / RIDX_TILE; }
 
 struct Scratch {
     // potential chains: per-tile int128 sums and prefixes plus a boundary value
@@ -47,10 +51,16 @@ struct Scratch {
     uint32_t *digit_totals;     // 8 * 256
     int64_t max_sort_tiles;
     uint64_t *sort_aos;         // packed payload records, SORT_AOS_WORDS words per star
-    int64_t *sort_long;         // starts of long runs of equal key prefixes (SORT_LONG_RUNS)
-    int64_t *sort_nlong;        // their count
+    int64_t *sort_long;         // starts of runs of equal key prefixes longer than SORT_RUN_SHORT
+    int64_t *sort_huge;         // starts of runs longer than SORT_RUN_LONG
+    int64_t *sort_nlong;        // [0] long runs, [1] huge runs
+    int64_t sort_long_cap, sort_huge_cap;
     double2 *mid;               // (r, phi) at every 16th node (search middle level)
     double2 *nodes;             // packed (r, phi) of every node, for the searches
+    float2 *pred;               // learned apsides proposals (log r_peri, log r_apo) per star
+    uint32_t *ridx;             // radial index: nodes below each bucket of r (ridx_nb + 2 entries)
+    uint32_t *ridx_tb;          // its value at every RIDX_TILE-th bucket
+    int ridx_bits;              // log2 of the bucket count
     double *u_old;              // new potential at the old sorted radii (set_velocities3)
     double2 *coarse;            // orbit sampler's (r, phi) samples, copied into LDS by every block
     void *orb_nib;              // orbit sampler's GF(2) nibble tables (constant per context)
@@ -103,15 +113,23 @@ hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch
 // substream start of every 64-lane wave: relaxation (phase 0, pairs 64w..) or orbits
 // (phase 1, slots 1 + 64w..), for `count` pairs / slots
 hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase, int64_t count, hipStream_t st);
+// radial index of the current sorted radii (after launch_mid_build, from the packed nodes)
+hipError_t launch_ridx_build(DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st);
 // middle level of the (r, phi) search tables, from the current arrays
 hipError_t launch_mid_build(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st);
 hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64_t first, int64_t count,
                             int ndraw, double *out, hipStream_t st);
 
+// mlp != nullptr: learned brackets (the network's proposals for every star first, on
+// the matrix cores, into sc.pred)
 hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
-                         const Scratch &sc, int64_t n_ub, hipStream_t st, const float *theta = nullptr,
-                         const MlpNorm *nrm = nullptr, float margin = 0.f, hipEvent_t ev_a = nullptr,
-                         hipEvent_t ev_b = nullptr, int variant = 0);
+                         const Scratch &sc, int64_t n_ub, hipStream_t st, const MlpState *mlp = nullptr,
+                         float margin = 0.f, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr, int variant = 0,
+                         bool pred_fixed = false);
+// proposals equal to the apsides the last orbit phase wrote (log r_peri, log r_apo): a
+// perfect predictor, for measuring what any learned proposal could save
+hipError_t launch_pred_from_apsides(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub,
+                                    hipStream_t st);
 int orbit_variant_count();
 void orbit_variant_shape(int v, int *threads, int *coarse, int *cap = nullptr);
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
@@ -149,6 +167,10 @@ hipError_t launch_mlp_loss_grad(const MlpState &m, const float *rows, int64_t co
                                 uint64_t mult, uint64_t off, hipStream_t st);
 hipError_t launch_mlp_adam(const MlpState &m, int64_t step, float lr, hipStream_t st);
 hipError_t launch_mlp_predict(const MlpState &m, const float *x, int64_t n, float *y, hipStream_t st);
+// the network on the matrix cores: every star's proposal (pred[j], j = 1..n), or given rows
+hipError_t launch_mlp_infer_stars(const DevFields &f, DevScalars *s, const KParams &P, const MlpState &m,
+                                  float2 *pred, int64_t n_ub, hipStream_t st);
+hipError_t launch_mlp_infer_rows(const MlpState &m, const float *x, int64_t n, float *y, hipStream_t st);
 hipError_t launch_mlp_fit_norm(const MlpState &m, int64_t count, hipStream_t st);
 hipError_t launch_mlp_harvest(const DevFields &f, DevScalars *s, const KParams &P, const MlpState &m, int64_t n_ub,
                               hipStream_t st);

@@ -40,7 +40,7 @@ extern "C" {
 #define CMCG_E_WZERO      -4   /* W == 0 in calc_encounter_dyns (cmc_dynamics_helper.c:858-863) */
 #define CMCG_E_WPZERO     -5   /* wp == 0 in dynamics_apply (cmc_dynamics.c:450-453) */
 #define CMCG_E_BRENT      -6   /* find_root_vr failed (cmc_nr.c:510-517,577-584,597-600) */
-#define CMCG_E_TIES       -7   /* a run of equal radii too long for the tie-break pass */
+#define CMCG_E_TIES       -7   /* (no longer returned: the sort orders runs of equal radii of any length) */
 #define CMCG_E_HIP       -10   /* HIP runtime error */
 #define CMCG_E_ARG       -11   /* invalid argument */
 #define CMCG_E_CAPACITY  -12   /* more stars than the context capacity */
@@ -279,6 +279,16 @@ int cmcg_central_calculate(cmcg_ctx *ctx, cmcg_central *out);
 /* CMCG_HOOK_ORBIT_VARIANT: run the orbit sampler with launch variant `value`
  * (0 .. count-1 of cmcg_orbit_variants), as cmcg_autotune would choose it. */
 #define CMCG_HOOK_ORBIT_VARIANT 2
+/* CMCG_HOOK_MLP_MFMA: value 1 makes cmcg_mlp_predict run the network on the matrix
+ * cores in f16 (the kernel that proposes the orbit phase's learned brackets), 0 the
+ * per-thread FP32 kernel. */
+#define CMCG_HOOK_MLP_MFMA 3
+/* CMCG_HOOK_PERFECT_BRACKETS: value 1 copies the apsides that the last orbit phase
+ * wrote (r_peri, r_apo) into the learned-bracket proposals and keeps them (no network
+ * inference) for the following orbit phases with learned brackets on: a perfect
+ * predictor, to measure the most any learned proposal could save.  0 returns to the
+ * network.  Needs cmcg_mlp_init. */
+#define CMCG_HOOK_PERFECT_BRACKETS 4
 int cmcg_test_hook(cmcg_ctx *ctx, int32_t hook, int64_t value);
 
 int cmcg_orbit_variants(cmcg_ctx *ctx, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse,

@@ -30,3 +30,24 @@ def near_tangent_stars(o, js, rel_j=(0.0, 1e-16, 4e-16, 1e-15), vr_choices=(0.0,
         o["J"][j] = rj * vt                       # set_star_EJ, cmc_dynamics_helper.c:878-882
         out.append(j)
     return out
+
+
+def near_tangent_sampleable(O, cl, js, max_drop=64, **kw):
+    """near_tangent_stars on a fresh oracle state of cl, leaving out the stars whose
+    orbit the reference cannot sample (its get_positions stops with N_TRY exceeded on
+    some of these degenerate orbits): returns the initialised oracle (orbit phase not
+    run), the constructed stars and the ones left out."""
+    drop = []
+    for _ in range(max_drop):
+        o = O.Oracle(cl.a, cl.n, orbit_r=cl.rtid, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64, **kw)
+        o.init()
+        used = near_tangent_stars(o, [j for j in js if j not in drop])
+        probe = O.Oracle(cl.a, cl.n, orbit_r=cl.rtid, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64, **kw)
+        probe.init()
+        near_tangent_stars(probe, [j for j in js if j not in drop])
+        probe.s.step = 0
+        probe.orbits()
+        if probe.s.error == 0:
+            return o, used, drop
+        drop.append(int(probe.s.error_index))
+    raise RuntimeError("too many near-tangent stars the reference cannot sample")

@@ -175,14 +175,15 @@ def test_gpu_plummer_kroupa_1e6_ten_steps(gpu, O):
 def test_gpu_energy_drift_100_steps_more_seeds(gpu, O, seed):
     """100 steps of Plummer 2e4 at further seeds: the drift agrees with the oracle's
     (device sin/cos) to 1e-6, and with the oracle run on the reference's C-library
-    sin/cos within the spread that the reference's own drift shows between its
-    C-library and long-double sin/cos."""
+    sin/cos within the spread of the reference's own drift over the math libraries it
+    may be built with (C-library sin/cos, sincos from an optimising compiler, long
+    double): last-bit differences of sin/cos move the drift by ~1e-5."""
     cm = gpu
     steps, n = 100, 20000
     say("energy drift, 100 steps, Plummer 2e4, seed", seed)
     cl = cm.make_plummer(n, seed=seed)
     drift = {}
-    for lm in (O.LIBM_DD, O.LIBM_SYSTEM, O.LIBM_LONG_DOUBLE):
+    for lm in (O.LIBM_DD, O.LIBM_SYSTEM, O.LIBM_SINCOS, O.LIBM_LONG_DOUBLE):
         o = O.Oracle(cl.a, cl.n, orbit_r=cl.rtid, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64, libm_mode=lm)
         o.init()
         e0 = o.sc("Etot")
@@ -197,8 +198,9 @@ def test_gpu_energy_drift_100_steps_more_seeds(gpu, O, seed):
             assert ctx.step()["error"] == 0
         sc = ctx.scalars()
     dg = (sc["Etot"] + sc["Eoops"] - g0) / abs(g0)
-    spread = abs(drift[O.LIBM_SYSTEM] - drift[O.LIBM_LONG_DOUBLE])
-    say("seed", seed, "drift gpu", dg, "oracle dd", drift[O.LIBM_DD], "system", drift[O.LIBM_SYSTEM],
-        "long double", drift[O.LIBM_LONG_DOUBLE])
+    ref = [drift[O.LIBM_SYSTEM], drift[O.LIBM_SINCOS], drift[O.LIBM_LONG_DOUBLE]]
+    spread = max(ref) - min(ref)
+    say("seed", seed, "drift gpu", dg, "oracle dd", drift[O.LIBM_DD], "reference libm variants", ref,
+        "spread", spread)
     assert abs(dg - drift[O.LIBM_DD]) < 1e-6
-    assert abs(dg - drift[O.LIBM_SYSTEM]) <= max(1e-6, 3.0 * spread)
+    assert abs(dg - drift[O.LIBM_SYSTEM]) <= max(1e-6, spread)

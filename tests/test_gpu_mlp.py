@@ -80,6 +80,35 @@ def test_gpu_mlp_predict_matches_oracle(gpu):
     assert err <= 1e-4 * (1 + np.max(np.abs(yo)))
 
 
+def test_gpu_mlp_mfma_inference_matches_fp32(gpu):
+    """The matrix-core (f16 operands, f32 accumulation) forward pass that proposes the
+    orbit phase's learned brackets against the per-thread FP32 pass and the float64
+    oracle: within 1e-2 of each output's standard deviation, on ragged row counts."""
+    cm = gpu
+    say("mlp inference on the matrix cores")
+    rng = np.random.default_rng(6)
+    cl = cm.make_plummer(1000, seed=1)
+    nr = _norm(rng)
+    with ctx_for(cm, cl) as ctx:
+        ctx.mlp_init(seed=9, max_rows=1024)
+        th, _ = ctx.mlp_params()
+        ctx.mlp_set_params(th, nr)
+        for n in (1, 15, 16, 17, 1000, 4099):
+            xr = (nr[:6] + nr[6:12] * rng.normal(size=(n, 6))).astype(np.float32)
+            ctx.test_hook(cm.HOOK_MLP_MFMA, 0)
+            y32 = ctx.mlp_predict(xr)
+            ctx.test_hook(cm.HOOK_MLP_MFMA, 1)
+            y16 = ctx.mlp_predict(xr)
+            xs = (xr.astype(np.float64) - nr[:6]) / nr[6:12]
+            yo, _ = M.forward(th.astype(np.float64), xs)
+            yo = yo * nr[14:16] + nr[12:14]
+            e16 = np.max(np.abs(y16 - yo) / nr[14:16])
+            e32 = np.max(np.abs(y32 - yo) / nr[14:16])
+            say("rows", n, "max |y16 - oracle| / std", e16, "fp32", e32)
+            assert e16 <= 1e-2 and e32 <= 1e-4
+        ctx.test_hook(cm.HOOK_MLP_MFMA, 0)
+
+
 def test_gpu_mlp_harvested_rows_match_state(gpu):
     """Rows harvested by one orbit phase: one per sampled star, inputs computed from
     the star's state and targets equal to log of the apsides the phase wrote."""

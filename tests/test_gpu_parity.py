@@ -151,12 +151,14 @@ def test_gpu_potential_tail_and_count(gpu, O):
 
 
 # ------------------------------------------------------------------ sort
-@pytest.mark.parametrize("n,ties", [(100003, "short"), (300007, "long"), (4000000, None)])
+@pytest.mark.parametrize("n,ties", [(100003, "short"), (300007, "long"), (600011, "huge"), (4000000, None)])
 def test_gpu_sort_bit_exact(gpu, O, n, ties):
-    """Bit-exact (r, id) order against the oracle.  The device sorts on the top 40 bits
-    of the key and orders runs of equal prefixes by (key, id): "short" has many runs of
-    equal radii; "long" has runs of 3000 and 500 equal radii and a run of 1200 radii
-    that agree in the top 40 bits but differ below."""
+    """Bit-exact (r, id) order against the oracle.  The device sorts on a 31-bit
+    range-normalised key and orders runs of equal keys by (r, id): "short" has many
+    runs of equal radii; "long" has runs of 3000 and 500 equal radii and a run of 1200
+    radii that agree in the top 40 bits but differ below; "huge" has runs of 20000 and
+    70000 equal radii (beyond one block's LDS sort: merge-sorted) and 6000 runs of 60
+    to 100 equal radii."""
     cm = gpu
     say("sort n", n, "ties", ties)
     rng = np.random.default_rng(n)
@@ -169,6 +171,16 @@ def test_gpu_sort_bit_exact(gpu, O, n, ties):
         r[rng.choice(n, 500, replace=False)] = 2.5
         near = rng.choice(n, 1200, replace=False)
         r[near] = 1.3 * (1.0 + rng.integers(0, 4000, 1200) * 2.0 ** -52)
+    elif ties == "huge":
+        r = np.exp(rng.normal(size=n))
+        perm = rng.permutation(n)
+        r[perm[:20000]] = 0.9
+        r[perm[20000:90000]] = 1.7
+        pos = 90000
+        for q in range(6000):
+            ln = int(rng.integers(60, 101))
+            r[perm[pos:pos + ln]] = 2.0 + q * 1e-4
+            pos += ln
     else:
         r = np.exp(rng.normal(size=n) * 2)
     r[rng.choice(n, n // 200, replace=False)] = 1e10
@@ -346,14 +358,17 @@ def test_gpu_brent_near_tangent_orbits(gpu, O):
     reference's calc_orbit_new falls back to find_root_vr (cmc_orbit.c:459-474,
     485-500): the device takes the same path for the same stars (n_brent equal and
     > 0) and every star's sampled orbit is bit-identical to the oracle's, including
-    where its narrowed searches meet Q within rounding of zero."""
-    from cases import near_tangent_stars
+    where its narrowed searches meet Q within rounding of zero.  On some of these
+    degenerate orbits the reference's rejection loop runs out of tries and stops
+    (N_TRY, cmc_evolution_thr.c:890-893); the device reports the same error there."""
+    from cases import near_tangent_sampleable, near_tangent_stars
     cm = gpu
     say("orbits with near-tangent stars (Brent path)")
     cl = cm.make_plummer(100000, seed=22)
-    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64)
-    o.init()
-    js = near_tangent_stars(o, list(range(3, o.N - 1, 41)))
+    js0 = list(range(3, cl.n - 1, 41))
+    o, js, dropped = near_tangent_sampleable(O, cl, js0)
+    say("near-tangent stars", len(js), "left out (the reference stops on them)", dropped)
+    assert dropped
     with ctx_for(cm, cl) as ctx:
         upload_state(ctx, cl, o, extra=("vr", "vt"))
         st = ctx.orbits(step=0)
@@ -361,8 +376,7 @@ def test_gpu_brent_near_tangent_orbits(gpu, O):
     o.s.step = 0
     o.orbits()
     N = o.N
-    say("near-tangent stars", len(js), "brent gpu", st["n_brent"], "oracle", o.s.n_brent, "circular",
-        st["n_circular"], o.sc("n_circular"))
+    say("brent gpu", st["n_brent"], "oracle", o.s.n_brent, "circular", st["n_circular"], o.sc("n_circular"))
     assert st["error"] == 0 and o.s.error == 0
     assert o.s.n_brent > 50
     assert st["n_brent"] == o.s.n_brent
@@ -371,7 +385,17 @@ def test_gpu_brent_near_tangent_orbits(gpu, O):
         same = int(np.sum(d[k][1:N + 1] == o[k][1:N + 1]))
         assert same == N, (k, same, N)
     assert np.array_equal(d["flags"][1:N + 1] & 16, o["flags"][1:N + 1] & 16)
-
+    # with the first of the left-out stars back in, both stop with N_TRY exceeded
+    o2 = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64)
+    o2.init()
+    near_tangent_stars(o2, [j for j in js0 if j not in dropped[1:]])
+    with ctx_for(cm, cl) as ctx:
+        upload_state(ctx, cl, o2, extra=("vr", "vt"))
+        with pytest.raises(cm.CmcgError) as ei:
+            ctx.orbits(step=0)
+    o2.s.step = 0
+    o2.orbits()
+    assert ei.value.code == cm.E_NTRY == o2.s.error
 
 # ------------------------------------------------------------------ glue
 def test_gpu_glue_parity(gpu, O):

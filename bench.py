@@ -92,6 +92,40 @@ def cpu_baseline(cl, ranks):
                        "sample": "the same step on one core (the reference's serial path)"}}
 
 
+def orbit_ceiling(cm, ctx, margin, steps=3):
+    """The orbit sampler re-run from one state three ways: plain searches, the network's
+    proposals, and perfect proposals (the apsides the plain run found, a test hook):
+    the last is the most any learned proposal could save.  HIP-event kernel times."""
+    sc = ctx.scalars()
+    keep = {k: sc[k] for k in ("n_max", "n_max_new", "Mtotal", "Rtidal", "Dt", "Prev_Dt", "TotalTime",
+                               "TidalMassLoss", "TidalMassLoss_old", "OldTidalMassLoss", "Eescaped", "Jescaped",
+                               "Etidal", "step")}
+    out = {}
+    for mode in ("plain", "learned", "perfect"):
+        ctx.set_learned_brackets(mode != "plain", margin=margin if mode == "learned" else 1e-6)
+        if mode == "perfect":
+            ctx.test_hook(cm.HOOK_PERFECT_BRACKETS, 1)
+        ctx.set_scalars(**keep)
+        ctx.orbits(step=int(keep["step"]))                # warm-up
+        ctx.timer_reset()
+        ctx.timer_enable(True)
+        hit = 0
+        for _ in range(steps):
+            ctx.set_scalars(**keep)
+            st = ctx.orbits(step=int(keep["step"]))
+            hit = st["n_bracket_hit"]
+        ctx.sync()
+        ctx.timer_enable(False)
+        ms, calls = ctx.timer_get("orbits_kernel")
+        out[mode] = {"orbits_kernel_ms": ms / max(calls, 1), "brackets_accepted": hit}
+        if mode == "perfect":
+            ctx.test_hook(cm.HOOK_PERFECT_BRACKETS, 0)
+    ctx.set_learned_brackets(False)
+    ctx.set_scalars(**keep)
+    log("orbit sampler from one state: " + ", ".join(f"{k} {v['orbits_kernel_ms']:.3f} ms" for k, v in out.items()))
+    return out
+
+
 def ensemble_bench(cm, steps, seed=7):
     """64 independent clusters stepped together: N log-uniform on [1e4, 2e6] (seeded),
     32 Kroupa Plummer and 32 Kroupa King models with W0 = 3, 5, 7, 9, one context each.
@@ -151,7 +185,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ranks", type=int, default=min(16, os.cpu_count() or 1),
                     help="host threads for the multi-rank CPU baseline (the GPU box's CPU share is 16)")
-    ap.add_argument("--train-iters", type=int, default=1500)
+    ap.add_argument("--train-iters", type=int, default=6000)
     ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -178,7 +212,8 @@ def main():
     log(f"autotune: orbit sampler {tune['threads']} threads x {tune['coarse']} LDS samples "
         f"({tune['ms_best']:.3f} ms vs default {tune['ms_default']:.3f} ms)")
     for v in tune["variants"]:
-        log(f"  variant threads {v['threads']:4d} samples {v['coarse']:5d} tries-cap {v['cap']:3d}: {v['ms']:.3f} ms")
+        log(f"  variant threads {v['threads']:4d} samples {v['coarse']:5d} tries-cap {v['cap']:3d} "
+            f"min blocks/CU {v['min_blocks']}: {v['ms']:.3f} ms")
 
     def timed(n_steps):
         ctx.timer_reset()
@@ -256,13 +291,15 @@ def main():
     ctx.set_learned_brackets(False)
     lstars = float(np.sum(llive))
     log(f"timed {args.steps} learned steps in {ltotal:.3f} s")
+    ceiling = orbit_ceiling(cm, ctx, margin, steps=3)
     learned = {"value": lstars / ltotal, "unit": "stars/s", "ms_per_step": ltotal / args.steps * 1e3,
                "median_ms_per_step": float(np.median(ltimes) * 1e3),
                "speedup_vs_plain": (lstars / ltotal) / (stars / total),
                "orbits_kernel_ms": lphases.get("orbits_kernel", {}).get("ms"),
                "brackets_accepted": lst["n_bracket_hit"], "brackets_rejected": lst["n_bracket_fallback"],
                "train": {"rows": int(rows.shape[0]), "adam_steps": args.train_iters, "batch": 4096, "seconds": train_s,
-                         "rms_log_r_error": err, "margin_log_r": margin}}
+                         "rms_log_r_error": err, "margin_log_r": margin},
+               "orbit_phase_ceiling": ceiling}
 
     dom = "orbits_kernel"
     d = phases[dom]

@@ -154,7 +154,7 @@ _sigs = {
     "cmcg_test_hook": [_ctx, C.c_int32, C.c_int64],
     "cmcg_comp_mass_percent": [_ctx, _P(C.c_double), C.c_int32, C.c_int32, C.c_double, C.c_double, _P(MassPc)],
     "cmcg_central_calculate": [_ctx, _P(Central)],
-    "cmcg_orbit_variants": [_ctx, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32),
+    "cmcg_orbit_variants": [_ctx, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32),
                             _P(C.c_double)],
     "cmcg_ensemble_step": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
     "cmcg_rng_draws": [_ctx, C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int, _P(C.c_double)],
@@ -176,6 +176,7 @@ MLP_NPARAM, MLP_NNORM = 4738, 16
 HOOK_CHAIN_K_DROP = 1
 HOOK_ORBIT_VARIANT = 2
 HOOK_MLP_MFMA = 3
+HOOK_PERFECT_BRACKETS = 4
 for _name, _args in _sigs.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args
@@ -548,13 +549,15 @@ class Context:
     def orbit_variants(self):
         """Launch variants of the orbit sampler with their last autotune times."""
         cnt = C.c_int32()
-        _chk(lib.cmcg_orbit_variants(self.h, 0, C.byref(cnt), None, None, None, None), "cmcg_orbit_variants")
+        _chk(lib.cmcg_orbit_variants(self.h, 0, C.byref(cnt), None, None, None, None, None), "cmcg_orbit_variants")
         n = cnt.value
-        th, cs, cp = (np.zeros(n, np.int32) for _ in range(3))
+        th, cs, cp, mb = (np.zeros(n, np.int32) for _ in range(4))
         ms = np.zeros(n, np.float64)
         _chk(lib.cmcg_orbit_variants(self.h, n, C.byref(cnt), _ptr(th, C.c_int32), _ptr(cs, C.c_int32),
-                                     _ptr(cp, C.c_int32), _ptr(ms, C.c_double)), "cmcg_orbit_variants")
-        return [{"threads": int(th[i]), "coarse": int(cs[i]), "cap": int(cp[i]), "ms": float(ms[i])} for i in range(n)]
+                                     _ptr(cp, C.c_int32), _ptr(mb, C.c_int32), _ptr(ms, C.c_double)),
+             "cmcg_orbit_variants")
+        return [{"threads": int(th[i]), "coarse": int(cs[i]), "cap": int(cp[i]), "min_blocks": int(mb[i]),
+                 "ms": float(ms[i])} for i in range(n)]
 
     def step_graph(self, check=True):
         st = OrbitStats()

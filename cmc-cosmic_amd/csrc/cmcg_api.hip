@@ -1424,17 +1424,18 @@ int cmcg_test_hook(cmcg_ctx *c, int32_t hook, int64_t value)
 }
 
 int cmcg_orbit_variants(cmcg_ctx *c, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse, int32_t *cap,
-                        double *ms_last)
+                        int32_t *min_blocks, double *ms_last)
 {
     if (!c || !count || max < 0) return CMCG_E_ARG;
     const int nv = orbit_variant_count();
     *count = nv;
     for (int v = 0; v < nv && v < max; v++) {
-        int th, cs, cp;
-        orbit_variant_shape(v, &th, &cs, &cp);
+        int th, cs, cp, mb;
+        orbit_variant_shape(v, &th, &cs, &cp, &mb);
         if (threads) threads[v] = th;
         if (coarse) coarse[v] = cs;
         if (cap) cap[v] = cp;
+        if (min_blocks) min_blocks[v] = mb;
         if (ms_last) ms_last[v] = v < (int)c->tune_ms.size() ? c->tune_ms[v] : 0.0;
     }
     return CMCG_OK;

@@ -815,6 +815,23 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
     }
     if (C) {
         const double inv = P.inv_nstar;
+        // a verified learned bracket is narrowed by the LDS samples inside it: the
+        // sample bisection runs over those samples only
+        auto narrow = [&](int64_t &lo, int64_t &hi, bool positive) {
+            const int s0 = (int)(lo / C->stride) + 1, s1 = (int)((hi - 1) / C->stride);
+            if (s0 > s1 || s1 > C->cs) return;
+            int a = s0 - 1, b = s1;                    // sample a: predicate true (a = s0 - 1: node lo)
+            while (a < b) {
+                const int m = (a + b + 1) >> 1;
+                const int sg = q_sign(rj, mj, inv, C->cr[m], C->cphi[m], E, J);
+                if (positive ? q_gt0(sg) : q_lt0(sg)) a = m;
+                else b = m - 1;
+            }
+            if (a >= s0) lo = coarse_k(*C, a);
+            if (a < s1) hi = coarse_k(*C, a + 1);
+        };
+        if (got_p) narrow(kp_lo, kp_hi, false);
+        if (got_a) narrow(ka_lo, ka_hi, true);
         if (!got_p && ktemp >= 1 && q_lt0(q_sign(rj, mj, inv, C->cr[0], C->cphi[0], E, J))) {
             int a = 0, b = (int)((ktemp - 1) / C->stride);
             const int top = b;

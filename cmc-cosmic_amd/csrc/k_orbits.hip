@@ -338,8 +338,10 @@ __device__ void wave_tries(const N &nd, const Coarse *C, const KParams &P, int64
 // Slot j draws from offset j 2^20; the wave's first slot starts at the per-wave state
 // and lane l jumps l 2^20 (l < 2^6) through the LDS tables.  With learned brackets
 // the network's prediction proposes both find_zero_Q brackets.
-template <int THREADS, int COARSE, int CAP>
-__global__ __launch_bounds__(THREADS) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
+// MINB: blocks per CU the register allocation must allow (1: no constraint; 1024 /
+// THREADS: four waves per SIMD, at the cost of some spilled registers)
+template <int THREADS, int COARSE, int CAP, int MINB>
+__global__ __launch_bounds__(THREADS, MINB) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
                                                              const RngTable *__restrict__ T,
                                                              const Rng *__restrict__ wave_base,
                                                              double *__restrict__ partials,
@@ -720,13 +722,15 @@ __global__ void coarse_build_kernel(const double2 *__restrict__ nodes, DevScalar
 using OrbFn = void (*)(DevFields, DevScalars *, KParams, const RngTable *, const Rng *, double *, int64_t *,
                        const float2 *, float, const double2 *, const double2 *, const double2 *,
                        const NibbleTables<ORB_NIB_POWERS> *, const uint32_t *, int);
-struct OrbVariant { OrbFn fn; int threads, coarse, cap; };
+struct OrbVariant { OrbFn fn; int threads, coarse, cap, minb; };
 const OrbVariant kOrbVariants[] = {
-    {orbits_kernel<256, 1024, 64>, 256, 1024, 64}, {orbits_kernel<256, 512, 64>, 256, 512, 64},
-    {orbits_kernel<256, 2048, 64>, 256, 2048, 64}, {orbits_kernel<128, 1024, 64>, 128, 1024, 64},
-    {orbits_kernel<512, 1024, 64>, 512, 1024, 64}, {orbits_kernel<128, 512, 64>, 128, 512, 64},
-    {orbits_kernel<256, 1024, 2>, 256, 1024, 2},   {orbits_kernel<256, 1024, 4>, 256, 1024, 4},
-    {orbits_kernel<256, 1024, 8>, 256, 1024, 8},   {orbits_kernel<256, 512, 4>, 256, 512, 4}};
+    {orbits_kernel<256, 1024, 64, 1>, 256, 1024, 64, 1}, {orbits_kernel<256, 512, 64, 1>, 256, 512, 64, 1},
+    {orbits_kernel<256, 2048, 64, 1>, 256, 2048, 64, 1}, {orbits_kernel<128, 1024, 64, 1>, 128, 1024, 64, 1},
+    {orbits_kernel<512, 1024, 64, 1>, 512, 1024, 64, 1}, {orbits_kernel<128, 512, 64, 1>, 128, 512, 64, 1},
+    {orbits_kernel<256, 1024, 2, 1>, 256, 1024, 2, 1},   {orbits_kernel<256, 1024, 4, 1>, 256, 1024, 4, 1},
+    {orbits_kernel<256, 1024, 8, 1>, 256, 1024, 8, 1},   {orbits_kernel<256, 512, 4, 1>, 256, 512, 4, 1},
+    {orbits_kernel<256, 1024, 4, 4>, 256, 1024, 4, 4},   {orbits_kernel<256, 1024, 8, 4>, 256, 1024, 8, 4},
+    {orbits_kernel<256, 512, 4, 4>, 256, 512, 4, 4}};
 constexpr int ORB_NVARIANTS = sizeof(kOrbVariants) / sizeof(kOrbVariants[0]);
 
 int bits_needed(uint64_t x)
@@ -799,11 +803,12 @@ hipError_t launch_ridx_build(DevScalars *s, const Scratch &sc, int64_t n_ub, hip
 }
 
 int orbit_variant_count() { return ORB_NVARIANTS; }
-void orbit_variant_shape(int v, int *threads, int *coarse, int *cap)
+void orbit_variant_shape(int v, int *threads, int *coarse, int *cap, int *minb)
 {
     *threads = kOrbVariants[v].threads;
     *coarse = kOrbVariants[v].coarse;
     if (cap) *cap = kOrbVariants[v].cap;
+    if (minb) *minb = kOrbVariants[v].minb;
 }
 
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,

@@ -131,7 +131,7 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
 hipError_t launch_pred_from_apsides(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub,
                                     hipStream_t st);
 int orbit_variant_count();
-void orbit_variant_shape(int v, int *threads, int *coarse, int *cap = nullptr);
+void orbit_variant_shape(int v, int *threads, int *coarse, int *cap = nullptr, int *minb = nullptr);
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
                           const double *E, const double *J, double *out4, int64_t *kout, int32_t *circ,
                           hipStream_t st);

@@ -228,7 +228,8 @@ int cmcg_rng_draws(cmcg_ctx *ctx, int phase, uint64_t step, int64_t first, int64
 int cmcg_autotune(cmcg_ctx *ctx, int reps, int32_t *variant, int32_t *threads, int32_t *coarse,
                   double *ms_best, double *ms_default);
 /* The orbit sampler's launch variants (threads per block, LDS samples, most lanes one
- * star's rejection tries get per round) and their times in the last cmcg_autotune
+ * star's rejection tries get per round, blocks per CU its register allocation must
+ * allow: 1 = unconstrained) and their times in the last cmcg_autotune
  * (0 before one ran); fills at most max entries, *count = number of variants.
  * No counterpart in the reference. */
 /* comp_mass_percent (src/cmc/cmc_utils.c:872-982) on one device: for the mass
@@ -292,7 +293,7 @@ int cmcg_central_calculate(cmcg_ctx *ctx, cmcg_central *out);
 int cmcg_test_hook(cmcg_ctx *ctx, int32_t hook, int64_t value);
 
 int cmcg_orbit_variants(cmcg_ctx *ctx, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse,
-                        int32_t *cap, double *ms_last);
+                        int32_t *cap, int32_t *min_blocks, double *ms_last);
 
 /* ------------------------------------------------------------------ timing */
 /* Average device time (ms) of the named kernel family over the calls since the last

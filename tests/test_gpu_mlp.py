@@ -184,9 +184,25 @@ def test_gpu_mlp_learns_apsides_and_keeps_orbits_exact(gpu):
         ctx.energy_conservation1()
         learned_stats = ctx.orbits(step=7)
         learned = ctx.download()
+        # perfect proposals (the apsides just found) with a tiny margin: nearly every
+        # search starts from a bracket of a few nodes
+        ctx.test_hook(cm.HOOK_PERFECT_BRACKETS, 1)
+        ctx.upload(state)
+        ctx.set_scalars(**{k: sc[k] for k in ("n_max", "n_max_new", "Mtotal", "Rtidal", "Dt", "Prev_Dt",
+                                             "TotalTime", "TidalMassLoss", "TidalMassLoss_old",
+                                             "OldTidalMassLoss")})
+        ctx.set_learned_brackets(True, margin=1e-6)
+        ctx.energy_conservation1()
+        perfect_stats = ctx.orbits(step=7)
+        perfect = ctx.download()
+        ctx.test_hook(cm.HOOK_PERFECT_BRACKETS, 0)
     N = plain.n
     for k in ("rnew", "vrnew", "vtnew", "r_peri", "r_apo", "X"):
         assert np.array_equal(plain[k][1:N + 1], learned[k][1:N + 1]), k
+        assert np.array_equal(plain[k][1:N + 1], perfect[k][1:N + 1]), k
+    ph, pm = perfect_stats["n_bracket_hit"], perfect_stats["n_bracket_fallback"]
+    say("perfect proposals accepted", ph, "rejected", pm)
+    assert ph > 0.95 * (ph + pm)
     hit, miss = learned_stats["n_bracket_hit"], learned_stats["n_bracket_fallback"]
     say("learned brackets accepted", hit, "rejected", miss)
     assert hit > 0.8 * (hit + miss)

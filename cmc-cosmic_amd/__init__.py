@@ -137,6 +137,9 @@ _sigs = {
     "cmcg_calc_pericenter_and_apocenter": [_ctx, C.c_int64, C.c_int64, _P(C.c_double), _P(C.c_double),
                                            _P(C.c_double), _P(C.c_double), _P(C.c_double), _P(C.c_double),
                                            _P(C.c_int64), _P(C.c_int64), _P(C.c_int32)],
+    "cmcg_calc_pericenter_and_apocenter_noalloc": [_ctx, C.c_int64, C.c_int64, _P(C.c_double), _P(C.c_double),
+                                                   _P(C.c_double), _P(C.c_double), _P(C.c_double),
+                                                   _P(C.c_double), _P(C.c_int64), _P(C.c_int64), _P(C.c_int32)],
     "cmcg_find_root_vr": [_ctx, C.c_int64, _P(C.c_int64), _P(C.c_int64), _P(C.c_double), _P(C.c_double),
                           _P(C.c_double)],
     "cmcg_find_zero_Q": [_ctx, C.c_int64, _P(C.c_int64), _P(C.c_int64), _P(C.c_int64), _P(C.c_double),
@@ -451,15 +454,18 @@ class Context:
             _chk(rc, "cmcg_orbits")
         return st.as_dict()
 
-    def apsides(self, first, E, J):
-        """calc_orbit_new for slots first.. (cmcg_calc_pericenter_and_apocenter)."""
+    APSIDES_NOALLOC_MAX = 32768
+
+    def apsides(self, first, E, J, noalloc=False):
+        """calc_orbit_new for slots first.. (cmcg_calc_pericenter_and_apocenter, or its
+        variant that allocates nothing when noalloc is set)."""
         E = np.ascontiguousarray(E, dtype=np.float64)
         J = np.ascontiguousarray(J, dtype=np.float64)
         n = E.size
         rp, ra, dp, da = (np.zeros(n) for _ in range(4))
         kmin, kmax = np.zeros(n, np.int64), np.zeros(n, np.int64)
         circ = np.zeros(n, np.int32)
-        name = "cmcg_calc_pericenter_and_apocenter"
+        name = "cmcg_calc_pericenter_and_apocenter" + ("_noalloc" if noalloc else "")
         _chk(getattr(lib, name)(
             self.h, int(first), n, _ptr(E, C.c_double), _ptr(J, C.c_double), _ptr(rp, C.c_double),
             _ptr(ra, C.c_double), _ptr(dp, C.c_double), _ptr(da, C.c_double), _ptr(kmin, C.c_int64),

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -336,6 +337,35 @@ const char *cmcg_error_string(int code)
     }
 }
 
+// ---- staging of cmcg_calc_pericenter_and_apocenter_noalloc ----
+// One file-scope device array, `scratch`, that each GPU holds until the process
+// ends: E and J in, the seven outputs back, for at most CMCG_APSIDES_NOALLOC_MAX
+// stars.  Its host-side name is not a device address; host code uses only the
+// address hipGetSymbolAddress returns on each device (looked up once per device
+// when a context is created there).  Calls that overlap on one GPU share it and race.
+constexpr size_t NOALLOC_SLOT = (size_t)CMCG_APSIDES_NOALLOC_MAX * 8;   // one 8-byte array
+constexpr size_t NOALLOC_BYTES = 8 * NOALLOC_SLOT + (size_t)CMCG_APSIDES_NOALLOC_MAX * 4;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 256;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+int g_scratch_rc[NOALLOC_MAX_DEVICES];          // 0 = not looked up, 1 = ready, CMCG_E_HIP = failed
+
+// With `device` current: look up this device's copy of `scratch` once.
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_rc[device] != 0) return;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = ptr;
+    g_scratch_rc[device] = (e == hipSuccess && ptr) ? 1 : CMCG_E_HIP;
+}
+}  // namespace
+
 int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *params)
 {
     if (!out || !params || n_cap < 1 || params->n_star < 1 || params->avekernel < 1) return CMCG_E_ARG;
@@ -347,6 +377,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     c->params = *params;
     if (hipSetDevice(device) != hipSuccess) { delete c; return CMCG_E_HIP; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CMCG_E_HIP; }
+    noalloc_setup(device);
     KParams &P = c->kp;
     P.n_star = params->n_star;
     P.inv_nstar = 1.0 / params->n_star;
@@ -814,6 +845,41 @@ int cmcg_calc_pericenter_and_apocenter(cmcg_ctx *c, int64_t first, int64_t count
         if (kmax) kmax[i] = hk[2 * i + 1];
         if (circular) circular[i] = hc[i];
     }
+    return check_error(c);
+}
+
+int cmcg_calc_pericenter_and_apocenter_noalloc(cmcg_ctx *c, int64_t first, int64_t count, const double *E,
+                                               const double *J, double *rp, double *ra, double *dQdrp,
+                                               double *dQdra, int64_t *kmin, int64_t *kmax, int32_t *circular)
+{
+    if (!c || count < 0 || !E || !J) return CMCG_E_ARG;
+    if (count == 0) return CMCG_OK;
+    if (first < 1 || first + count - 1 > c->n_ub) return CMCG_E_ARG;
+    if (count > CMCG_APSIDES_NOALLOC_MAX) return CMCG_E_CAPACITY;
+    if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES || g_scratch_rc[c->device] != 1) return CMCG_E_HIP;
+    char *base = (char *)g_scratch_ptr[c->device];
+    double *dE = (double *)(base + 0 * NOALLOC_SLOT), *dJ = (double *)(base + 1 * NOALLOC_SLOT);
+    ApsidesOut o;
+    o.rp = (double *)(base + 2 * NOALLOC_SLOT);
+    o.ra = (double *)(base + 3 * NOALLOC_SLOT);
+    o.dQdrp = (double *)(base + 4 * NOALLOC_SLOT);
+    o.dQdra = (double *)(base + 5 * NOALLOC_SLOT);
+    o.kmin = (int64_t *)(base + 6 * NOALLOC_SLOT);
+    o.kmax = (int64_t *)(base + 7 * NOALLOC_SLOT);
+    o.circular = (int32_t *)(base + 8 * NOALLOC_SLOT);
+    const size_t b8 = (size_t)count * 8;
+    hipStream_t st = c->stream;
+    bool bad = hipMemcpyAsync(dE, E, b8, hipMemcpyHostToDevice, st) ||
+               hipMemcpyAsync(dJ, J, b8, hipMemcpyHostToDevice, st) ||
+               launch_apsides_soa(c->f, c->s, c->kp, first, count, dE, dJ, o, st);
+    if (!bad && rp) bad = hipMemcpyAsync(rp, o.rp, b8, hipMemcpyDeviceToHost, st);
+    if (!bad && ra) bad = hipMemcpyAsync(ra, o.ra, b8, hipMemcpyDeviceToHost, st);
+    if (!bad && dQdrp) bad = hipMemcpyAsync(dQdrp, o.dQdrp, b8, hipMemcpyDeviceToHost, st);
+    if (!bad && dQdra) bad = hipMemcpyAsync(dQdra, o.dQdra, b8, hipMemcpyDeviceToHost, st);
+    if (!bad && kmin) bad = hipMemcpyAsync(kmin, o.kmin, b8, hipMemcpyDeviceToHost, st);
+    if (!bad && kmax) bad = hipMemcpyAsync(kmax, o.kmax, b8, hipMemcpyDeviceToHost, st);
+    if (!bad && circular) bad = hipMemcpyAsync(circular, o.circular, (size_t)count * 4, hipMemcpyDeviceToHost, st);
+    if (sync(c) || bad) return CMCG_E_HIP;
     return check_error(c);
 }
 

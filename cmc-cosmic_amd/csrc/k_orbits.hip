@@ -603,6 +603,21 @@ __global__ void apsides_kernel(DevFields f, DevScalars *s, KParams P, int64_t fi
     circ[i] = o.circular;
 }
 
+// The same per-star apsides written to one array per output (for callers that copy
+// each output straight into its own host array).
+__global__ void apsides_soa_kernel(DevFields f, DevScalars *s, KParams P, int64_t first, int64_t count,
+                                   const double *__restrict__ E, const double *__restrict__ J, ApsidesOut o)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t j = first + i, n = s->n_max;
+    OrbitRS r = calc_orbit(f.r, f.phi, n, j, f.r[j], f.m[j], P, E[i], J[i]);
+    if (r.err) dev_set_error(s, r.err, j);
+    o.rp[i] = r.rp; o.ra[i] = r.ra; o.dQdrp[i] = r.dQdrp; o.dQdra[i] = r.dQdra;
+    o.kmin[i] = r.kmin; o.kmax[i] = r.kmax;
+    o.circular[i] = r.circular;
+}
+
 __global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count, const int64_t *__restrict__ jj,
                            const int64_t *__restrict__ kmin, const int64_t *__restrict__ kmax,
                            const double *__restrict__ E, const double *__restrict__ J, int64_t *__restrict__ out)
@@ -818,6 +833,15 @@ hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, i
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(apsides_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
                        count, E, J, out4, kout, circ);
+    return hipGetLastError();
+}
+
+hipError_t launch_apsides_soa(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
+                              const double *E, const double *J, const ApsidesOut &o, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(apsides_soa_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
+                       count, E, J, o);
     return hipGetLastError();
 }
 

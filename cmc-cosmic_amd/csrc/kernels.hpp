@@ -132,6 +132,14 @@ hipError_t launch_pred_from_apsides(const DevFields &f, DevScalars *s, const Scr
                                     hipStream_t st);
 int orbit_variant_count();
 void orbit_variant_shape(int v, int *threads, int *coarse, int *cap = nullptr, int *minb = nullptr);
+// Device arrays receiving calc_orbit_new's results, one per output.
+struct ApsidesOut {
+    double *rp, *ra, *dQdrp, *dQdra;
+    int64_t *kmin, *kmax;
+    int32_t *circular;
+};
+hipError_t launch_apsides_soa(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
+                              const double *E, const double *J, const ApsidesOut &o, hipStream_t st);
 hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, int64_t first, int64_t count,
                           const double *E, const double *J, double *out4, int64_t *kout, int32_t *circ,
                           hipStream_t st);

@@ -174,6 +174,20 @@ int cmcg_calc_pericenter_and_apocenter(cmcg_ctx *ctx, int64_t first, int64_t cou
                                        double *rp, double *ra, double *dQdrp, double *dQdra,
                                        int64_t *kmin, int64_t *kmax, int32_t *circular);
 
+/* The same as cmcg_calc_pericenter_and_apocenter, without any device or host
+ * allocation: no counterpart in the reference.  For callers that cannot allocate
+ * (tight loops that query apsides, code that must not call hipMalloc).  Its inputs
+ * and outputs are staged in one fixed device array per GPU (about 2.4 MB, held until
+ * the process ends), so count may be at most CMCG_APSIDES_NOALLOC_MAX
+ * (CMCG_E_CAPACITY beyond), and calls that overlap on one GPU race for it.  Returns
+ * CMCG_E_HIP if the array could not be located when the first context was created
+ * on that device. */
+#define CMCG_APSIDES_NOALLOC_MAX 32768
+int cmcg_calc_pericenter_and_apocenter_noalloc(cmcg_ctx *ctx, int64_t first, int64_t count,
+                                               const double *E, const double *J,
+                                               double *rp, double *ra, double *dQdrp, double *dQdra,
+                                               int64_t *kmin, int64_t *kmax, int32_t *circular);
+
 /* a7: find_zero_Q (src/cmc/cmc_nr.c:316-383) for a batch of (j, kmin, kmax, E, J). */
 int cmcg_find_zero_Q(cmcg_ctx *ctx, int64_t count, const int64_t *j, const int64_t *kmin,
                      const int64_t *kmax, const double *E, const double *J, int64_t *out);

@@ -367,18 +367,18 @@ __device__ __forceinline__ void set_a_b(const double *__restrict__ r, const doub
     set_a_b_n(SplitNodes{r, phi}, rj, mj, P, k, a, b);
 }
 
-// cmc_nr.c:239-269 + :393-403: v_r^2 inside [r_k, r_{k+1}]
+// cmc_nr.c:239-269 + :393-403: v_r^2 inside [r_k, r_{k+1}]; cen = cenma.m * madhoc
 __device__ __forceinline__ double vr2_in_interval(const double *__restrict__ r, const double *__restrict__ phi,
-                                                  double rj, double mj, const KParams &P, int64_t k,
+                                                  double rj, double mj, double inv_nstar, double cen, int64_t k,
                                                   double E, double J, double x)
 {
     const double eps = 2.220446049250313e-16;
     double pot;
     if (fabs(x - r[k]) < eps) pot = phi[k];
     else if (fabs(x - r[k + 1]) < eps) pot = phi[k + 1];
-    else if (x < r[1]) pot = phi[0] - P.cenma_m * P.madhoc / x;
+    else if (x < r[1]) pot = phi[0] - cen / x;
     else pot = (phi[k] + (phi[k + 1] - phi[k]) * (1.0 / r[k] - 1.0 / x) / (1.0 / r[k] - 1.0 / r[k + 1]));
-    return function_q(rj, mj, P.inv_nstar, x, pot, E, J);
+    return function_q(rj, mj, inv_nstar, x, pot, E, J);
 }
 
 __device__ __forceinline__ bool interval_converged(double lo, double hi, double epsabs, double epsrel)
@@ -390,17 +390,20 @@ __device__ __forceinline__ bool interval_converged(double lo, double hi, double 
 }
 
 // cmc_nr.c:500-603 find_root_vr by the Brent-Dekker method (Brent 1973, ch. 4),
-// the same iteration as the CPU oracle's.  Rare fallback path.
-__device__ __noinline__ double find_root_vr(const double *__restrict__ r, const double *__restrict__ phi,
-                                            double rj, double mj, const KParams &P, int64_t k,
-                                            double E, double J, bool *bad)
+// the same iteration as the CPU oracle's.  Rare fallback path; scalar arguments and a
+// returned pair keep its caller's objects out of private memory.
+struct RootVr { double root; int bad; };
+__device__ __noinline__ RootVr find_root_vr(const double *__restrict__ r, const double *__restrict__ phi,
+                                            double rj, double mj, double inv_nstar, double cen, int64_t k,
+                                            double E, double J)
 {
+    bool bad_ = false, *bad = &bad_;
     const double prec = 1e-11, deps = 2.220446049250313e-16;
     int iter = 100;
     double a = r[k], b = r[k + 1];
-    double fa = vr2_in_interval(r, phi, rj, mj, P, k, E, J, a);
-    double fb = vr2_in_interval(r, phi, rj, mj, P, k, E, J, b);
-    if ((fa > 0) == (fb > 0) && (fa < 0) == (fb < 0)) { *bad = true; return 0.0; }
+    double fa = vr2_in_interval(r, phi, rj, mj, inv_nstar, cen, k, E, J, a);
+    double fb = vr2_in_interval(r, phi, rj, mj, inv_nstar, cen, k, E, J, b);
+    if ((fa > 0) == (fb > 0) && (fa < 0) == (fb < 0)) return RootVr{0.0, 1};
     double c = a, fc = fa, d = b - a, e = d;
     double lo = a < b ? a : b, hi = a < b ? b : a;
     bool converged = false;
@@ -429,7 +432,7 @@ __device__ __noinline__ double find_root_vr(const double *__restrict__ r, const 
         a = b; fa = fb;
         if (fabs(d) > tol) b += d;
         else b += (mm > 0 ? tol : -tol);
-        fb = vr2_in_interval(r, phi, rj, mj, P, k, E, J, b);
+        fb = vr2_in_interval(r, phi, rj, mj, inv_nstar, cen, k, E, J, b);
         lo = b < c ? b : c; hi = b < c ? c : b;
         if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) { lo = a < b ? a : b; hi = a < b ? b : a; }
         converged = interval_converged(lo, hi, prec, prec);
@@ -440,12 +443,12 @@ __device__ __noinline__ double find_root_vr(const double *__restrict__ r, const 
     }
     if (!converged) *bad = true;
     double apsis = b;
-    if (vr2_in_interval(r, phi, rj, mj, P, k, E, J, apsis) < 0.) {
-        if (vr2_in_interval(r, phi, rj, mj, P, k, E, J, lo) < 0.) apsis = hi;
-        if (vr2_in_interval(r, phi, rj, mj, P, k, E, J, hi) < 0.) apsis = lo;
+    if (vr2_in_interval(r, phi, rj, mj, inv_nstar, cen, k, E, J, apsis) < 0.) {
+        if (vr2_in_interval(r, phi, rj, mj, inv_nstar, cen, k, E, J, lo) < 0.) apsis = hi;
+        if (vr2_in_interval(r, phi, rj, mj, inv_nstar, cen, k, E, J, hi) < 0.) apsis = lo;
     }
-    if (vr2_in_interval(r, phi, rj, mj, P, k, E, J, apsis) < 0.) *bad = true;
-    return apsis;
+    if (vr2_in_interval(r, phi, rj, mj, inv_nstar, cen, k, E, J, apsis) < 0.) *bad = true;
+    return RootVr{apsis, bad_ ? 1 : 0};
 }
 
 // Top levels of the sorted-potential search tree staged in LDS: cr[s] = r[k(s)],
@@ -757,11 +760,15 @@ struct OrbitRS { double rp, ra, dQdrp, dQdra; int64_t kmin, kmax; int circular; 
 
 // cmc_orbit.c:413-540 calc_orbit_new after get_positive_Q_index has produced ktemp
 // (an index with Q > 0; ktemp < 0 means "no positive Q": circular orbit).
-template <class N>
-__device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *__restrict__ r,
-                                                     const double *__restrict__ phi, int64_t n_max, int64_t j,
-                                                     double rj, double mj, const KParams &P, double E, double J,
-                                                     int64_t ktemp, const Coarse *C)
+// HAS_C: the LDS samples, radial index and learned proposal of Cr are available.  (A
+// template flag rather than a null test of a pointer: comparing the address of a
+// private object with null is not folded on this target and keeps the object in
+// scratch memory.)
+template <class N, bool HAS_C>
+__device__ __forceinline__ OrbitRS calc_orbit_core(const N &nd, const double *__restrict__ r,
+                                                   const double *__restrict__ phi, int64_t n_max, int64_t j,
+                                                   double rj, double mj, const KParams &P, double E, double J,
+                                                   int64_t ktemp, const Coarse &Cr)
 {
     OrbitRS o;
     o.brent = 0; o.linscan = 0; o.err = 0; o.hint_hits = 0; o.hint_misses = 0;
@@ -777,23 +784,23 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
     bool got_p = false, got_a = false;
     bool vp = false, va = false;       // bracket ends checked: Q(lo), Q(hi) have the bisection's signs
     double qp_lo = 0.0, qp_hi = 0.0, qa_lo = 0.0, qa_hi = 0.0;   // Q at sample ends (search guidance only)
-    if (C && C->hint) {
+    if (HAS_C && Cr.hint) {
         // learned proposals: the index range of the predicted apsis +- margin (in
         // log r), padded by 2, accepted only when Q has the bisection's signs at its ends
         const double inv = P.inv_nstar;
-        const float ep = __expf(C->margin);
+        const float ep = __expf(Cr.margin);
         // node index just below / at or above radius x: exact bounds from the radial
         // index, estimates padded by two nodes from the LDS samples otherwise
         auto below = [&](double x) -> int64_t {
-            if (C->ri.t) { int32_t a, b; ridx_bracket(C->ri, x, a, b); return a; }
-            return coarse_index_estimate(*C, x) - 2;
+            if (Cr.ri.t) { int32_t a, b; ridx_bracket(Cr.ri, x, a, b); return a; }
+            return coarse_index_estimate(Cr, x) - 2;
         };
         auto above = [&](double x) -> int64_t {
-            if (C->ri.t) { int32_t a, b; ridx_bracket(C->ri, x, a, b); return b; }
-            return coarse_index_estimate(*C, x) + 2;
+            if (Cr.ri.t) { int32_t a, b; ridx_bracket(Cr.ri, x, a, b); return b; }
+            return coarse_index_estimate(Cr, x) + 2;
         };
-        int64_t lo = below((double)(__expf(C->lrp) / ep));
-        int64_t hi = above((double)(__expf(C->lrp) * ep));
+        int64_t lo = below((double)(__expf(Cr.lrp) / ep));
+        int64_t hi = above((double)(__expf(Cr.lrp) * ep));
         lo = lo < 0 ? 0 : lo;
         hi = hi > ktemp ? ktemp : hi;
         if (lo < hi && q_lt0(q_sign(rj, mj, inv, nd.R(lo), nd.P(lo), E, J)) &&
@@ -802,8 +809,8 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
         } else {
             o.hint_misses++;
         }
-        lo = below((double)(__expf(C->lra) / ep));
-        hi = above((double)(__expf(C->lra) * ep));
+        lo = below((double)(__expf(Cr.lra) / ep));
+        hi = above((double)(__expf(Cr.lra) * ep));
         lo = lo < ktemp ? ktemp : lo;
         hi = hi > n_max + 1 ? n_max + 1 : hi;
         if (lo < hi && (lo == ktemp || q_gt0(q_sign(rj, mj, inv, nd.R(lo), nd.P(lo), E, J))) &&
@@ -813,78 +820,78 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
             o.hint_misses++;
         }
     }
-    if (C) {
+    if (HAS_C) {
         const double inv = P.inv_nstar;
         // a verified learned bracket is narrowed by the LDS samples inside it: the
         // sample bisection runs over those samples only
         auto narrow = [&](int64_t &lo, int64_t &hi, bool positive) {
-            const int s0 = (int)(lo / C->stride) + 1, s1 = (int)((hi - 1) / C->stride);
-            if (s0 > s1 || s1 > C->cs) return;
+            const int s0 = (int)(lo / Cr.stride) + 1, s1 = (int)((hi - 1) / Cr.stride);
+            if (s0 > s1 || s1 > Cr.cs) return;
             int a = s0 - 1, b = s1;                    // sample a: predicate true (a = s0 - 1: node lo)
             while (a < b) {
                 const int m = (a + b + 1) >> 1;
-                const int sg = q_sign(rj, mj, inv, C->cr[m], C->cphi[m], E, J);
+                const int sg = q_sign(rj, mj, inv, Cr.cr[m], Cr.cphi[m], E, J);
                 if (positive ? q_gt0(sg) : q_lt0(sg)) a = m;
                 else b = m - 1;
             }
-            if (a >= s0) lo = coarse_k(*C, a);
-            if (a < s1) hi = coarse_k(*C, a + 1);
+            if (a >= s0) lo = coarse_k(Cr, a);
+            if (a < s1) hi = coarse_k(Cr, a + 1);
         };
         if (got_p) narrow(kp_lo, kp_hi, false);
         if (got_a) narrow(ka_lo, ka_hi, true);
-        if (!got_p && ktemp >= 1 && q_lt0(q_sign(rj, mj, inv, C->cr[0], C->cphi[0], E, J))) {
-            int a = 0, b = (int)((ktemp - 1) / C->stride);
+        if (!got_p && ktemp >= 1 && q_lt0(q_sign(rj, mj, inv, Cr.cr[0], Cr.cphi[0], E, J))) {
+            int a = 0, b = (int)((ktemp - 1) / Cr.stride);
             const int top = b;
             while (a < b) {
                 int m = (a + b + 1) >> 1;
-                if (q_lt0(q_sign(rj, mj, inv, C->cr[m], C->cphi[m], E, J))) a = m;
+                if (q_lt0(q_sign(rj, mj, inv, Cr.cr[m], Cr.cphi[m], E, J))) a = m;
                 else b = m - 1;
             }
-            kp_lo = coarse_k(*C, a);
-            kp_hi = a < top ? coarse_k(*C, a + 1) : ktemp;
+            kp_lo = coarse_k(Cr, a);
+            kp_hi = a < top ? coarse_k(Cr, a + 1) : ktemp;
             vp = true;
-            (void)q_sign(rj, mj, inv, C->cr[a], C->cphi[a], E, J, &qp_lo);
-            if (a < top) (void)q_sign(rj, mj, inv, C->cr[a + 1], C->cphi[a + 1], E, J, &qp_hi);
+            (void)q_sign(rj, mj, inv, Cr.cr[a], Cr.cphi[a], E, J, &qp_lo);
+            if (a < top) (void)q_sign(rj, mj, inv, Cr.cr[a + 1], Cr.cphi[a + 1], E, J, &qp_hi);
         }
-        const int sb = (int)(ktemp / C->stride) + 1;
-        if (!got_a && sb <= C->cs && !q_gt0(q_sign(rj, mj, inv, C->cr[C->cs], C->cphi[C->cs], E, J))) {
+        const int sb = (int)(ktemp / Cr.stride) + 1;
+        if (!got_a && sb <= Cr.cs && !q_gt0(q_sign(rj, mj, inv, Cr.cr[Cr.cs], Cr.cphi[Cr.cs], E, J))) {
             va = true;
-            if (!q_gt0(q_sign(rj, mj, inv, C->cr[sb], C->cphi[sb], E, J))) {
-                ka_hi = coarse_k(*C, sb);
+            if (!q_gt0(q_sign(rj, mj, inv, Cr.cr[sb], Cr.cphi[sb], E, J))) {
+                ka_hi = coarse_k(Cr, sb);
             } else {
-                int a = sb, b = C->cs - 1;
+                int a = sb, b = Cr.cs - 1;
                 while (a < b) {
                     int m = (a + b + 1) >> 1;
-                    if (q_gt0(q_sign(rj, mj, inv, C->cr[m], C->cphi[m], E, J))) a = m;
+                    if (q_gt0(q_sign(rj, mj, inv, Cr.cr[m], Cr.cphi[m], E, J))) a = m;
                     else b = m - 1;
                 }
-                ka_lo = coarse_k(*C, a);
-                ka_hi = coarse_k(*C, a + 1);
-                (void)q_sign(rj, mj, inv, C->cr[a], C->cphi[a], E, J, &qa_lo);
-                (void)q_sign(rj, mj, inv, C->cr[a + 1], C->cphi[a + 1], E, J, &qa_hi);
+                ka_lo = coarse_k(Cr, a);
+                ka_hi = coarse_k(Cr, a + 1);
+                (void)q_sign(rj, mj, inv, Cr.cr[a], Cr.cphi[a], E, J, &qa_lo);
+                (void)q_sign(rj, mj, inv, Cr.cr[a + 1], Cr.cphi[a + 1], E, J, &qa_hi);
             }
         }
     }
     // verified brackets: interpolation search through the radial index; otherwise the
     // middle table and a bisection (which also settles the direction itself)
-    const bool ri = C && C->ri.t;
+    const bool ri = HAS_C && Cr.ri.t;
     int64_t kmin, kmax;
     if (ri && vp) {
         double rl, pl, rh, ph;
         nd.RP(kp_lo, rl, pl);
         nd.RP(kp_hi, rh, ph);
-        kmin = ridx_zero_Q<false>(nd, C->ri, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi, rl, pl, rh, ph);
+        kmin = ridx_zero_Q<false>(nd, Cr.ri, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi, rl, pl, rh, ph);
     } else {
-        if (vp && C) mid_narrow_q<false>(*C, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi, qp_lo, qp_hi);
+        if (vp && HAS_C) mid_narrow_q<false>(Cr, rj, mj, P.inv_nstar, E, J, kp_lo, kp_hi, qp_lo, qp_hi);
         kmin = find_zero_Q_n(nd, rj, mj, P.inv_nstar, kp_lo, kp_hi, E, J);
     }
     if (ri && va) {
         double rl, pl, rh, ph;
         nd.RP(ka_lo, rl, pl);
         nd.RP(ka_hi, rh, ph);
-        kmax = ridx_zero_Q<true>(nd, C->ri, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi, rl, pl, rh, ph);
+        kmax = ridx_zero_Q<true>(nd, Cr.ri, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi, rl, pl, rh, ph);
     } else {
-        if (va && C) mid_narrow_q<true>(*C, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi, qa_lo, qa_hi);
+        if (va && HAS_C) mid_narrow_q<true>(Cr, rj, mj, P.inv_nstar, E, J, ka_lo, ka_hi, qa_lo, qa_hi);
         kmax = find_zero_Q_n(nd, rj, mj, P.inv_nstar, ka_lo, ka_hi, E, J);
     }
     if (kmin < 0 || kmax < 0) {
@@ -899,7 +906,9 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
     if (inside_sqrt < 0.0) {
         rmin = -1.0 * J * J / a;
         if ((rmin < nd.R(kmin)) || (rmin > nd.R(kmin + 1))) {
-            rmin = find_root_vr(r, phi, rj, mj, P, kmin, E, J, &bad);
+            const RootVr rv = find_root_vr(r, phi, rj, mj, P.inv_nstar, P.cenma_m * P.madhoc, kmin, E, J);
+            rmin = rv.root;
+            bad |= rv.bad != 0;
             o.brent = 1;
         }
     } else {
@@ -911,7 +920,9 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
     if (inside_sqrt < 0.0) {
         rmax = -a / (2.0 * (b - E));
         if ((rmax < nd.R(kmax)) || (rmax > nd.R(kmax + 1))) {
-            rmax = find_root_vr(r, phi, rj, mj, P, kmax, E, J, &bad);
+            const RootVr rv = find_root_vr(r, phi, rj, mj, P.inv_nstar, P.cenma_m * P.madhoc, kmax, E, J);
+            rmax = rv.root;
+            bad |= rv.bad != 0;
             o.brent = 1;
         }
     } else {
@@ -928,12 +939,21 @@ __device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *
     return o;
 }
 
+template <class N>
+__device__ __forceinline__ OrbitRS calc_orbit_from_n(const N &nd, const double *__restrict__ r,
+                                                     const double *__restrict__ phi, int64_t n_max, int64_t j,
+                                                     double rj, double mj, const KParams &P, double E, double J,
+                                                     int64_t ktemp)
+{
+    const Coarse none{};
+    return calc_orbit_core<N, false>(nd, r, phi, n_max, j, rj, mj, P, E, J, ktemp, none);
+}
+
 __device__ __forceinline__ OrbitRS calc_orbit_from(const double *__restrict__ r, const double *__restrict__ phi,
                                                    int64_t n_max, int64_t j, double rj, double mj,
-                                                   const KParams &P, double E, double J, int64_t ktemp,
-                                                   const Coarse *C = nullptr)
+                                                   const KParams &P, double E, double J, int64_t ktemp)
 {
-    return calc_orbit_from_n(SplitNodes{r, phi}, r, phi, n_max, j, rj, mj, P, E, J, ktemp, C);
+    return calc_orbit_from_n(SplitNodes{r, phi}, r, phi, n_max, j, rj, mj, P, E, J, ktemp);
 }
 
 // get_positive_Q_index (cmc_orbit.c:30-59) with its linear scan, then calc_orbit_from.

@@ -81,14 +81,14 @@ __device__ __forceinline__ void removed(double (&red)[NRED], double Ej, double J
 // The part of get_positions_loop after the positive-Q index is known
 // (cmc_evolution_thr.c:832-934).
 template <class N>
-__device__ void sample_star(const N &nd, const DevFields &f, DevScalars *s, const KParams &P, Rng rng, const Coarse *C,
+__device__ void sample_star(const N &nd, const DevFields &f, DevScalars *s, const KParams &P, Rng rng,
                             int64_t n, int64_t j, double rj, double mj, double E,
                             double Ej, double J, int64_t ktemp, int &flags, int &ntry, int &hits, int &misses,
                             double (&red)[NRED])
 {
     const double *__restrict__ r = f.r;
     const double *__restrict__ phi = f.phi;
-    OrbitRS o = calc_orbit_from_n(nd, r, phi, n, j, rj, mj, P, E, J, ktemp, C);
+    OrbitRS o = calc_orbit_from_n(nd, r, phi, n, j, rj, mj, P, E, J, ktemp);
     if (o.err) dev_set_error(s, o.err, j);
     hits = o.hint_hits;
     misses = o.hint_misses;
@@ -124,7 +124,7 @@ __device__ void sample_star(const N &nd, const DevFields &f, DevScalars *s, cons
         int64_t lo = lo0, hi = hi0;
         if (!(r_lo0 < rr)) lo = 1;
         if (!(rr <= r_hi0)) hi = n + 1;
-        potr = C ? potential_at_coarse_n(nd, n, rr, lo, hi, *C, &bad) : potential_at_n(nd, n, rr, lo, hi, &bad);
+        potr = potential_at_n(nd, n, rr, lo, hi, &bad);
         double pot = potr + phi_s_own(rj, mj, P.inv_nstar, rr);
         double drds = 0.25 * (rmax - rmin) * (3.0 - 3.0 * s0 * s0);
         double Q = 2.0 * E - 2.0 * pot - J * J / rr / rr;
@@ -193,11 +193,11 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int q)
 // orbits, and the loop's constants.  Returns true if the star samples a new radius.
 template <class N>
 __device__ __forceinline__ bool prepare_star(const N &nd, const DevFields &f, DevScalars *s, const KParams &P,
-                                             const Coarse *C, int64_t n, int64_t j, double rj, double mj,
+                                             const Coarse &C, int64_t n, int64_t j, double rj, double mj,
                                              double E, double J, int64_t ktemp, int &flags, int &hits, int &misses,
                                              TryCtx &tc)
 {
-    OrbitRS o = calc_orbit_from_n(nd, f.r, f.phi, n, j, rj, mj, P, E, J, ktemp, C);
+    OrbitRS o = calc_orbit_core<N, true>(nd, f.r, f.phi, n, j, rj, mj, P, E, J, ktemp, C);
     if (o.err) dev_set_error(s, o.err, j);
     hits = o.hint_hits;
     misses = o.hint_misses;
@@ -229,7 +229,7 @@ __device__ __forceinline__ bool prepare_star(const N &nd, const DevFields &f, De
 // One try (cmc_evolution_thr.c:862-886) from the state at its first draw; on return
 // st is the state after the try's two draws.
 template <class N>
-__device__ __forceinline__ bool one_try(const N &nd, const Coarse *C, const KParams &P, int64_t n, const TryCtx &tc,
+__device__ __forceinline__ bool one_try(const N &nd, const Coarse &C, const KParams &P, int64_t n, const TryCtx &tc,
                                         Rng &st, double &X, double &rr, double &vr, double &potr, bool &bad,
                                         bool &nan)
 {
@@ -240,9 +240,8 @@ __device__ __forceinline__ bool one_try(const N &nd, const Coarse *C, const KPar
     int64_t lo = tc.lo0, hi = tc.hi0;
     if (!(tc.r_lo0 < rr)) lo = 1;
     if (!(rr <= tc.r_hi0)) hi = n + 1;
-    potr = !C ? potential_at_n(nd, n, rr, lo, hi, &bad)
-              : C->ri.t ? potential_at_ridx_n(nd, n, rr, lo, hi, C->ri, &bad)
-                        : potential_at_coarse_n(nd, n, rr, lo, hi, *C, &bad);
+    potr = C.ri.t ? potential_at_ridx_n(nd, n, rr, lo, hi, C.ri, &bad)
+                  : potential_at_coarse_n(nd, n, rr, lo, hi, C, &bad);
     const double pot = potr + phi_s_own(tc.rj, tc.mj, P.inv_nstar, rr);
     const double drds = 0.25 * (tc.rmax - tc.rmin) * (3.0 - 3.0 * s0 * s0);
     const double Q = 2.0 * tc.E - 2.0 * pot - tc.J * tc.J / rr / rr;
@@ -259,7 +258,7 @@ __device__ __forceinline__ bool one_try(const N &nd, const Coarse *C, const KPar
 // samples; wctx: the wave's 64 try constants in LDS (slot = lane, written before a
 // block barrier); rng: this lane's substream state before its star's first try.
 template <int CAP, class N>
-__device__ void wave_tries(const N &nd, const Coarse *C, const KParams &P, int64_t n, bool need,
+__device__ void wave_tries(const N &nd, const Coarse &C, const KParams &P, int64_t n, bool need,
                            const TryCtx *__restrict__ wctx, Rng rng, TryOut &out)
 {
     const int lane = threadIdx.x & 63;
@@ -415,7 +414,7 @@ __global__ __launch_bounds__(THREADS, MINB) void orbits_kernel(DevFields f, DevS
                     C.lrp = y.x;
                     C.lra = y.y;
                 }
-                need = prepare_star(PackedNodes{nodes}, f, s, P, &C, n, j, rj, mj, E, J, j, flags, hits, misses,
+                need = prepare_star(PackedNodes{nodes}, f, s, P, C, n, j, rj, mj, E, J, j, flags, hits, misses,
                                     sh.ctx[threadIdx.x]);
             }
         }
@@ -423,7 +422,7 @@ __global__ __launch_bounds__(THREADS, MINB) void orbits_kernel(DevFields f, DevS
     __syncthreads();                           // try constants visible to the wave
     // every lane of the wave takes part, stars or not
     TryOut to;
-    wave_tries<CAP>(PackedNodes{nodes}, &C, P, n, need, sh.ctx + (threadIdx.x & ~63), rng, to);
+    wave_tries<CAP>(PackedNodes{nodes}, C, P, n, need, sh.ctx + (threadIdx.x & ~63), rng, to);
     double red[NRED];
     for (int i = 0; i < NRED; i++) red[i] = 0.0;
     if (j <= n) {
@@ -509,7 +508,7 @@ __global__ __launch_bounds__(ORB_THREADS) void defer_tail_kernel(DevFields f, De
         double red[NRED];
         for (int i = 0; i < NRED; i++) red[i] = 0.0;
         int flags = CMCG_SF_LINSCAN, ntry = 0, hits = 0, misses = 0;
-        sample_star(SplitNodes{f.r, f.phi}, f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), nullptr, n, j,
+        sample_star(SplitNodes{f.r, f.phi}, f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), n, j,
                     rj, mj, E, Ej, J, kt, flags, ntry, hits, misses, red);
         f.flags[j] = flags;
         f.ntry[j] = ntry;
@@ -637,9 +636,9 @@ __global__ void frv_kernel(DevFields f, DevScalars *s, KParams P, int64_t count,
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     int64_t j = jj[i];
-    bool bad = false;
-    out[i] = find_root_vr(f.r, f.phi, f.r[j], f.m[j], P, kk[i], E[i], J[i], &bad);
-    if (bad) dev_set_error(s, CMCG_E_BRENT, j);
+    const RootVr rv = find_root_vr(f.r, f.phi, f.r[j], f.m[j], P.inv_nstar, P.cenma_m * P.madhoc, kk[i], E[i], J[i]);
+    out[i] = rv.root;
+    if (rv.bad) dev_set_error(s, CMCG_E_BRENT, j);
 }
 
 __global__ void pred_from_apsides_kernel(DevFields f, DevScalars *s, float2 *__restrict__ pred)

@@ -16,12 +16,14 @@
  *  - find_zero_Q / calc_orbit_new: brute-force Q over all k on tiny N gives
  *    kmin/kmax; Kepler special case (point mass) gives a(1-e), a(1+e)
  *    [test_oracle_apsides_*].
- *  - find_root_vr (GSL Brent in the reference): PARITY UNPINNED — GSL is not in
- *    the reference tree nor in this image; the Brent-Dekker iteration below is
- *    written from the published method, and only the residual post-conditions of
- *    cmc_nr.c:587-600 are checked.
- *  - dynamics_apply relaxation: pair momentum and kinetic energy conservation,
- *    |w_new| = |w|, angle(w, w_new) = beta [test_oracle_relaxation_invariants].
+ *  - find_root_vr (GSL Brent in the reference; GSL is not in the reference tree
+ *    nor in this image, so the Brent-Dekker iteration below is written from the
+ *    published method): the root equals the exact root of the interval's quadratic
+ *    in 1/r (50-digit) to the reference's tolerance, also on near-tangent orbits
+ *    that reach it through calc_orbit_new [test_find_root_vr_*].
+ *  - dynamics_apply relaxation: pair momentum and kinetic energy conservation;
+ *    the angle between w and w_new recovered from the outputs equals beta, beta^2
+ *    scales with dt, the clamp at pi/2 [test_relaxation_kick_angle*].
  *  - get_positions sampling: fixed-draw closed forms + KS test of accepted r
  *    against the 1/|v_r| density [test_oracle_sampling_ks].
  *  - sort: numpy.lexsort on (id, r) [test_oracle_sort].
@@ -225,8 +227,9 @@ static int interval_converged(double lo, double hi, double epsabs, double epsrel
 /* cmc_nr.c:500-603 find_root_vr: root of v_r^2(r) in [r_k, r_{k+1}] by the
  * Brent-Dekker method (R. P. Brent, Algorithms for Minimization without
  * Derivatives, 1973, ch. 4), APSIDES_PRECISION = 1e-11, APSIDES_MAX_ITER = 100
- * (cmc_io.c:1752-1754), then the residual fix-up of cmc_nr.c:587-600.
- * PARITY UNPINNED against GSL's iterate path (GSL is absent). */
+ * (cmc_io.c:1752-1754), then the residual fix-up of cmc_nr.c:587-600.  GSL's own
+ * iterate path is not reproduced (GSL is absent); the root is pinned to the exact
+ * root within the reference's tolerance. */
 double orc_find_root_vr(orc_state *s, int64_t index, int64_t k, double E, double J)
 {
     const double prec = 1e-11;

@@ -11,21 +11,24 @@
 //
 // * If v_{j-1} and v_j lie in one binade with ulp u, then fl(v_{j-1} + s x_j) =
 //   v_{j-1} + RN_u(s x_j) unless s x_j / u is a tie: the increment does not depend
-//   on v_{j-1}.  Such "regular" elements are recognised from the exact prefix sum
-//   of the x (2^-K fixed point, int128) and a rigorous bound on the accumulated
-//   rounding error; the rest ("sensitive": binade crossings, ties, tiny values) are
-//   few, about one per binade.
-// * Segment and walk: an exact prefix sum of the regular increments, then one
-//   thread walks the sensitive elements in order, each getting
-//   fl(v_prev + s x) from the exact value before it (the run of regular elements
-//   in between stays in one binade, so adding the run's sum is exact), then every
-//   element is rebuilt in parallel from the last sensitive value before it.
-// * Every element is verified against fl(v_{j-1} + s x_j).  A violation (only when
-//   the error bound or the sensitive-list capacity is exceeded) is repaired by
-//   Jacobi rounds in which each bracket fl(g_{i-1} + s x_i) - g_{i-1} of the
-//   previous iterate g is an exact int128 and each round restarts at the first
-//   violation, and finally by a sequential pass.  The result equals the sequential
-//   chain exactly.
+//   on v_{j-1}.  Such "regular" elements are recognised from a double estimate of
+//   the prefix sums of the x and a rigorous bound on its error and on the chain's
+//   own accumulated rounding; the rest ("sensitive": binade crossings, ties, tiny
+//   values) are few, about one per binade.
+// * Segment and walk: a regular increment is an integer R in units of its binade's
+//   ulp, and a run of them between two sensitive elements stays in one binade, so
+//   an int64 scan that restarts at every sensitive element gives each run's exact
+//   sum.  One thread walks the sensitive elements in order, each getting
+//   fl(v_prev + s x) from the exact value before it (the previous one plus the run
+//   in between), then every element is rebuilt in parallel from the last sensitive
+//   value before it.  Four passes: tile sums (fused into the kernel that produces
+//   the increments), classification, walk, rebuild.
+// * Every element is verified against fl(v_{j-1} + s x_j), so the estimate's bound
+//   decides only the speed, never the result.  A violation (only when the bound or
+//   the sensitive-list capacity is exceeded) is repaired by Jacobi rounds in which
+//   each bracket fl(g_{i-1} + s x_i) - g_{i-1} of the previous iterate g is an exact
+//   int128 (2^-K fixed point) and each round restarts at the first violation, and
+//   finally by a sequential pass.  The result equals the sequential chain exactly.
 #include "kernels.hpp"
 
 namespace {
@@ -35,6 +38,8 @@ constexpr int CTILE = CT * CI;
 constexpr int SCAN_T = 1024;
 constexpr int REPAIR_ROUNDS = 1;      // Jacobi rounds after the segment-and-walk pass (then sequential)
 constexpr int RAW_CAP = 2048;         // sensitive elements sorted in LDS by the walk
+constexpr int WALK_T = 1024;          // threads of the one-block kernels
+constexpr int WALK_CHUNK = 256;       // sensitive elements staged per round of the walk
 
 typedef __int128 i128;
 
@@ -243,27 +248,102 @@ __global__ void chain_finish_kernel(ChainArgs a)
 }
 
 // ---------------------------------------------------------------- segment and walk
+// The classification needs only the binade of each chain value, so it runs on a
+// double estimate of the prefix sums with a rigorous error bound.  A regular element
+// adds RN_u(s x) = R u, u = 2^(b-52) the ulp of its binade b, and all regular
+// elements between two sensitive ones share one binade: a sum of the integers R that
+// restarts at every sensitive element (a segmented scan) is exact in int64, and times
+// u exact in double.
 struct EsaArgs {
     const double *x;
     double *g;
     double sign;
     const double *v0p;
-    int64_t *first_bad;         // as in ChainArgs, initialised here
-    i128 *tsumP, *tpreP;        // per tile: fixed-point sums of s*x and exclusive prefixes
-    i128 *tsumR, *tpreR;        // per tile: regular-increment sums and prefixes
-    i128 *tcnt, *toff;          // per tile: sensitive counts and prefixes
-    i128 *tsum;                 // per tile: next Jacobi increments (for the repair rounds)
-    int64_t *s_idx;             // sensitive list: element index
-    i128 *s_racc;               //   regular-increment prefix before the element
-    double *s_v;                //   exact chain value at the element
-    int64_t *s_n;               // number of sensitive elements (uncapped)
-    int64_t smax;               // list capacity
-    int64_t *raw_n;             // sensitive elements appended by the classification, unordered:
-    int64_t *raw_idx;           //   index
-    i128 *raw_part;             //   regular-increment prefix inside its tile
+    int64_t *first_bad;         // as in ChainArgs, initialised with the tile sums
     const int64_t *Lp;
-    int K;
+    int slack;                  // error bound widened by 2^slack (0; a test hook raises it)
+    // per tile
+    double *tsumP, *tpreP;      // estimated sums of s*x and their exclusive prefixes
+    int64_t *tcnt, *ttail;      // sensitive count; regular sum after the tile's last sensitive element
+    int64_t *toff, *tcarry;     // sensitive elements before the tile; regular sum of the segment open at its start
+    uint32_t *arrive;           // [1]: blocks of the ordered compaction arrived (zero between launches)
+    // the sensitive elements in chain order
+    int64_t *s_idx;
+    double *s_pre;              // regular sum between the previous sensitive element and this one
+    double *s_v;                // chain value after the element
+    int64_t *s_n;               // count (uncapped)
+    int64_t smax;               // capacity
+    // appended by the classification in any order (RAW_CAP entries): index, regular sum
+    // since the previous sensitive element of its tile (or the tile start), binade
+    // before it
+    int64_t *raw_n, *raw_idx, *raw_part;
+    int32_t *raw_bp;
+    double *raw_sx;                         // s x of the element
 };
+
+// A run of elements: regular sum after its last sensitive element (all of it when it
+// has none) and its number of sensitive elements.
+struct Seg {
+    int64_t sum;
+    int64_t cnt;
+};
+// a followed by b
+__device__ __forceinline__ Seg seg_then(Seg a, Seg b) { return Seg{b.cnt ? b.sum : a.sum + b.sum, a.cnt + b.cnt}; }
+__device__ __forceinline__ Seg shfl_up_seg(Seg s, int d)
+{
+    return Seg{__shfl_up(s.sum, d, 64), __shfl_up(s.cnt, d, 64)};
+}
+
+// Block-wide exclusive scans (every thread of the block calls; lds holds blockDim/64
+// entries).  The wave totals are folded in wave order, so the results do not depend
+// on timing.
+__device__ double block_excl_sum(double v, double *lds, double *total)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6;
+    double x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    double ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0.0;
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    double off = 0.0, tot = 0.0;
+    for (int q = 0; q < nw; q++) {
+        const double s = lds[q];
+        if (q < w) off += s;
+        tot += s;
+    }
+    *total = tot;
+    __syncthreads();
+    return off + ex;
+}
+
+__device__ Seg block_excl_seg(Seg v, Seg *lds, Seg *total)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6;
+    Seg x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const Seg y = shfl_up_seg(x, d);
+        if (lane >= d) x = seg_then(y, x);
+    }
+    Seg ex = shfl_up_seg(x, 1);
+    if (lane == 0) ex = Seg{0, 0};
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    Seg off{0, 0}, tot{0, 0};
+    for (int q = 0; q < nw; q++) {
+        const Seg s = lds[q];
+        if (q < w) off = seg_then(off, s);
+        tot = seg_then(tot, s);
+    }
+    *total = tot;
+    __syncthreads();
+    return seg_then(off, ex);
+}
 
 __device__ __forceinline__ int binade_of(double v)
 {
@@ -282,261 +362,403 @@ __device__ __forceinline__ int interval_binade(double v, double e)
     return b1 == b2 ? b1 : INT32_MIN;
 }
 
-// Bound on |v - estimate| after n additions: n roundings of at most half an ulp of
-// values below 2 vmax (monotone chain), n truncations to 2^-K, and the conversion of
-// the fixed-point estimate to double.
-__device__ __forceinline__ double err_bound(int64_t n, double vmax, double a, int K)
+// Bound on |v - P| for the estimate P of the chain value v after j elements: the
+// chain's own j roundings (at most 2^-53 |v| each, |v| <= vmax for a monotone chain)
+// and the roundings of the estimate's sums in any order (at most (j + 2) 2^-53
+// (|v0| + sum |s x|) <= (j + 2) 2^-52 vmax), with a factor 2 to spare.  A chain for
+// which it fails only takes the repair path: every element is verified.
+__device__ __forceinline__ double err_bound(int64_t j, double vmax, int slack)
 {
-    return ((double)n * (ldexp(vmax, -51) + ldexp(1.0, 1 - K)) + ldexp(a, -50)) * 1.001;
+    return ldexp(((double)j + 2.0) * vmax, slack - 50) * 1.001;
 }
 
-// Binade of the chain value before element j (after elements 0..j-1, fixed-point
-// prefix Pp), or INT32_MIN when its error interval straddles binades; before
-// element 0 the value is v0 exactly.
-__device__ __forceinline__ int start_binade(int64_t j, double v0, i128 Pp, int K)
+// binade of the chain value before element j, or INT32_MIN when its error interval
+// straddles binades; before element 0 the value is v0 exactly
+__device__ __forceinline__ int start_binade(int64_t j, double v0, double P, int slack)
 {
     if (j == 0) return interval_binade(v0, 0.0);
-    const double p = unfxk(Pp, K);
-    return interval_binade(p, err_bound(j, fmax(fabs(v0), fabs(p)), fabs(p), K));
+    return interval_binade(P, err_bound(j, fmax(fabs(v0), fabs(P)), slack));
 }
 
-// regular increment of element j in 2^-K units, or sensitive: bp, bq are the binades
-// before and after the element (bq = start_binade(j + 1, ...), so a thread carries
-// it on to its next element)
-__device__ __forceinline__ bool classify(double x, double sign, int bp, int bq, int K, i128 *R)
+// regular increment of an element in units of 2^(bp-52), or sensitive: bp, bq are the
+// binades before and after it
+__device__ __forceinline__ bool classify(double sx, int bp, int bq, int64_t *R)
 {
-    if (bp == INT32_MIN || bp != bq || bp < 52 - K || bp > 52 + 60 - K) return true;
-    double y = ldexp(sign * x, 52 - bp);         // exact: power-of-two scaling
-    double fy = floor(y);
+    if (bp == INT32_MIN || bp != bq) return true;
+    const double y = ldexp(sx, 52 - bp);      // exact unless it underflows, far below 1/2
+    const double fy = floor(y);
     if (y - fy == 0.5 || !(fabs(y) <= 9007199254740992.0)) return true;
-    double ry = rint(y);
-    *R = (i128)(int64_t)ry * ((i128)1 << (bp - 52 + K));
+    *R = (int64_t)rint(y);
     return false;
 }
 
-// Classification of one tile's items (CI consecutive per thread).  Needs tpreP.
+// One tile's items, CI consecutive per thread.
 struct TileItems {
-    double xv[CI];
-    i128 R[CI];
-    uint32_t sens;      // bit u: item u sensitive (or past the end)
+    double sx[CI];      // s x_j (exact: s = +-1)
+    double pa[CI];      // estimate of the chain value after the item
+    int64_t R[CI];      // regular increment in units of 2^(bp - 52), 0 if sensitive
+    int bp[CI];         // binade before the item
+    uint32_t sens;      // bit u: item u sensitive
 };
 
-__device__ void tile_classify(const EsaArgs &a, int64_t tile, int64_t L, double v0, TileItems &it, i128 *lds)
+// A tile staged through LDS so that global memory is read and written in coalesced
+// rows (element i of the tile by thread i mod CT) while each thread works on CI
+// consecutive elements; one pad double per 8 keeps the per-thread reads free of bank
+// conflicts.
+constexpr int TILE_LDS = CTILE + CTILE / 8;
+__device__ __forceinline__ int tile_slot(int i) { return i + (i >> 3); }
+
+__device__ __forceinline__ void load_tile(const double *__restrict__ x, double sign, int64_t base, int64_t L,
+                                          double *tbuf, double (&sx)[CI])
 {
     const int t = threadIdx.x;
-    const int64_t j0 = tile * CTILE + (int64_t)t * CI;
-    i128 fxs[CI];
-    i128 acc = 0;
+#pragma unroll
     for (int u = 0; u < CI; u++) {
-        int64_t j = j0 + u;
-        it.xv[u] = j < L ? a.x[j] : 0.0;
-        fxs[u] = j < L ? fxk(a.sign * it.xv[u], a.K) : (i128)0;
-        acc += fxs[u];
+        const int i = u * CT + t;
+        const int64_t j = base + i;
+        tbuf[tile_slot(i)] = j < L ? sign * x[j] : 0.0;
     }
-    i128 tot;
-    i128 P = fxk(v0, a.K) + a.tpreP[tile] + block_excl_scan<CT>(acc, lds, &tot);
-    it.sens = 0;
-    int bp = start_binade(j0, v0, P, a.K);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CI; u++) sx[u] = tbuf[tile_slot(t * CI + u)];
+}
+
+// the reverse; tbuf must be free (every thread past its last read of it)
+__device__ __forceinline__ void store_tile(double *__restrict__ g, int64_t base, int64_t L, double *tbuf,
+                                           const double (&v)[CI])
+{
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < CI; u++) tbuf[tile_slot(t * CI + u)] = v[u];
+    __syncthreads();
+#pragma unroll
     for (int u = 0; u < CI; u++) {
-        int64_t j = j0 + u;
-        i128 Pq = P + fxs[u];
-        it.R[u] = 0;
-        bool sens = j >= L;
-        if (!sens) {
-            const int bq = start_binade(j + 1, v0, Pq, a.K);
-            sens = classify(it.xv[u], a.sign, bp, bq, a.K, &it.R[u]);
-            bp = bq;
+        const int i = u * CT + t;
+        const int64_t j = base + i;
+        if (j < L) g[j] = tbuf[tile_slot(i)];
+    }
+}
+
+// Classification of one tile (items past the end are regular with R = 0).  Needs
+// tpreP.  Every kernel classifies through this function on the same inputs, so they
+// agree element for element; the binade at a tile start is taken from the same
+// expression in both tiles that meet there.  Nearly always a thread's whole range of
+// values lies in one binade, checked once with the bound at its last element: then
+// only the ties remain to be found.
+__device__ void tile_classify(const EsaArgs &a, int64_t tile, int64_t L, double v0, TileItems &it, double *lds,
+                              double *tbuf)
+{
+    const int64_t j0 = tile * CTILE + (int64_t)threadIdx.x * CI;
+    load_tile(a.x, a.sign, tile * CTILE, L, tbuf, it.sx);
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < CI; u++) acc += it.sx[u];
+    double tot;
+    const double E = block_excl_sum(acc, lds, &tot);
+    const double P0 = v0 + (a.tpreP[tile] + E);
+    double P = P0, Pmin = P0, Pmax = P0;
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+        P = P + it.sx[u];
+        it.pa[u] = P;
+        Pmin = fmin(Pmin, P);
+        Pmax = fmax(Pmax, P);
+    }
+    it.sens = 0;
+    const int64_t jl = j0 + CI < L ? j0 + CI : L;           // one past the thread's last element
+    const bool edge = jl < L && jl % CTILE == 0;             // the next element opens the next tile
+    const double Pn = edge ? v0 + (a.tpreP[tile + 1] + 0.0) : 0.0;
+    // fast path: every value before and after the thread's elements in one binade
+    int bt = INT32_MIN;
+    if (j0 < L) {
+        const double vmax = fmax(fabs(v0), fmax(fabs(Pmin), fabs(Pmax)));
+        const double err = err_bound(jl, vmax, a.slack);
+        const double lo = fmin(Pmin, j0 == 0 ? v0 : Pmin) - err, hi = fmax(Pmax, j0 == 0 ? v0 : Pmax) + err;
+        if (lo > 0.0 || hi < 0.0) {
+            const int b1 = binade_of(lo), b2 = binade_of(hi);
+            if (b1 == b2) bt = b1;
         }
-        if (sens) {
-            it.sens |= 1u << u;
+        if (j0 == 0 && bt != interval_binade(v0, 0.0)) bt = INT32_MIN;
+        if (edge && bt != start_binade(jl, v0, Pn, a.slack)) bt = INT32_MIN;
+    }
+    if (bt != INT32_MIN) {
+#pragma unroll
+        for (int u = 0; u < CI; u++) {
             it.R[u] = 0;
+            it.bp[u] = bt;
+            if (j0 + u < L && classify(it.sx[u], bt, bt, &it.R[u])) {
+                it.sens |= 1u << u;
+                it.R[u] = 0;
+            }
+        }
+        return;
+    }
+    P = P0;
+    int bp = start_binade(j0, v0, P, a.slack);
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+        const int64_t j = j0 + u;
+        const double Pq = it.pa[u];
+        it.R[u] = 0;
+        it.bp[u] = bp;
+        if (j < L) {
+            const double Pe = edge && j + 1 == jl ? Pn : Pq;
+            const int bq = start_binade(j + 1, v0, Pe, a.slack);
+            if (classify(it.sx[u], bp, bq, &it.R[u])) {
+                it.sens |= 1u << u;
+                it.R[u] = 0;
+            }
+            bp = bq;
         }
         P = Pq;
     }
 }
 
-__global__ __launch_bounds__(CT) void esa_sum_kernel(EsaArgs a)
+__device__ __forceinline__ Seg items_seg(const TileItems &it)
 {
-    __shared__ i128 lds[CT];
-    const int64_t L = *a.Lp;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { a.first_bad[0] = 0; a.first_bad[1] = L; *a.raw_n = 0; }
-    const int64_t base = (int64_t)blockIdx.x * CTILE;
-    if (base >= L) return;
-    i128 acc = 0;
+    Seg s{0, 0};
+#pragma unroll
     for (int u = 0; u < CI; u++) {
-        int64_t j = base + (int64_t)threadIdx.x * CI + u;
-        if (j < L) acc += fxk(a.sign * a.x[j], a.K);
+        if ((it.sens >> u) & 1) {
+            s.sum = 0;
+            s.cnt++;
+        } else {
+            s.sum += it.R[u];
+        }
     }
-    i128 tot;
-    block_excl_scan<CT>(acc, lds, &tot);
-    if (threadIdx.x == 0) a.tsumP[blockIdx.x] = tot;
+    return s;
 }
 
-// exclusive scan of up to two per-tile int128 arrays (one block); the total of the
-// second goes to *total
-__global__ __launch_bounds__(SCAN_T) void esa_tile_scan_kernel(const i128 *in1, i128 *out1, const i128 *in2,
-                                                               i128 *out2, int64_t *total, const int64_t *Lp)
+__device__ __forceinline__ double seg_value(int64_t sum, int bp)
 {
-    __shared__ i128 lds[SCAN_T];
-    const int64_t L = *Lp;
-    const int64_t ntiles = (L + CTILE - 1) / CTILE;
-    const int64_t chunk = (ntiles + SCAN_T - 1) / SCAN_T;
-    const int64_t b0 = (int64_t)threadIdx.x * chunk;
-    for (int pass = 0; pass < (in2 ? 2 : 1); pass++) {
-        const i128 *in = pass ? in2 : in1;
-        i128 *out = pass ? out2 : out1;
-        i128 acc = 0;
-        for (int64_t b = b0; b < b0 + chunk && b < ntiles; b++) acc += in[b];
-        i128 tot;
-        i128 run = block_excl_scan<SCAN_T>(acc, lds, &tot);
-        for (int64_t b = b0; b < b0 + chunk && b < ntiles; b++) {
-            out[b] = run;
-            run += in[b];
+    return sum == 0 || bp == INT32_MIN ? 0.0 : ldexp((double)sum, bp - 52);
+}
+
+// Tile sums of s*x (acc: this thread's share); block 0 also resets the chain's flags.
+__device__ void esa_tile_sums(const EsaArgs &a, int64_t L, double acc, double *lds)
+{
+    double tot;
+    block_excl_sum(acc, lds, &tot);
+    if (threadIdx.x == 0) {
+        a.tsumP[blockIdx.x] = tot;
+        if (blockIdx.x == 0) {
+            a.first_bad[0] = 0;
+            a.first_bad[1] = L;
+            *a.raw_n = 0;
         }
-        if (pass == 1 && threadIdx.x == 0) *total = (int64_t)tot;
     }
+}
+
+// one block: exclusive prefixes of the nt tile sums, in tile order
+__global__ __launch_bounds__(WALK_T) void esa_tile_scan_kernel(EsaArgs a, int64_t nt)
+{
+    __shared__ double lds[WALK_T / 64];
+    const int64_t chunk = (nt + WALK_T - 1) / WALK_T;
+    const int64_t b0 = (int64_t)threadIdx.x * chunk;
+    double s = 0.0;
+    for (int64_t b = b0; b < b0 + chunk && b < nt; b++) s += a.tsumP[b];
+    double tot;
+    double run = block_excl_sum(s, lds, &tot);
+    for (int64_t b = b0; b < b0 + chunk && b < nt; b++) {
+        a.tpreP[b] = run;
+        run += a.tsumP[b];
+    }
+}
+
+__global__ __launch_bounds__(CT) void esa_sum_kernel(EsaArgs a)
+{
+    __shared__ double lds[CT / 64];
+    const int64_t L = *a.Lp;
+    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    double acc = 0.0;
+    for (int u = 0; u < CI; u++) {
+        const int64_t j = base + u * CT + threadIdx.x;
+        if (j < L) acc += a.sign * a.x[j];
+    }
+    esa_tile_sums(a, L, acc, lds);
 }
 
 __global__ __launch_bounds__(CT) void esa_classify_kernel(EsaArgs a)
 {
-    __shared__ i128 lds[CT];
+    __shared__ double ldsd[CT / 64];
+    __shared__ Seg ldss[CT / 64];
+    __shared__ double tbuf[TILE_LDS];
     const int64_t L = *a.Lp;
     const int64_t tile = blockIdx.x;
     if (tile * CTILE >= L) return;
-    const double v0 = *a.v0p;
     TileItems it;
-    tile_classify(a, tile, L, v0, it, lds);
-    i128 racc = 0, cnt = 0;
-    for (int u = 0; u < CI; u++) {
-        racc += it.R[u];
-        int64_t j = tile * CTILE + (int64_t)threadIdx.x * CI + u;
-        if (j < L && ((it.sens >> u) & 1)) cnt += 1;
+    tile_classify(a, tile, L, *a.v0p, it, ldsd, tbuf);
+    Seg tot;
+    const Seg pre = block_excl_seg(items_seg(it), ldss, &tot);
+    if (threadIdx.x == 0) {
+        a.tcnt[tile] = tot.cnt;
+        a.ttail[tile] = tot.sum;
     }
-    i128 tr, tc;
-    i128 R = block_excl_scan<CT>(racc, lds, &tr);
-    block_excl_scan<CT>(cnt, lds, &tc);
-    if (threadIdx.x == 0) { a.tsumR[tile] = tr; a.tcnt[tile] = tc; }
-    // the few sensitive elements, unordered, with their in-tile regular prefix
-    for (int u = 0; u < CI; u++) {
-        int64_t j = tile * CTILE + (int64_t)threadIdx.x * CI + u;
-        if (j >= L) break;
-        if ((it.sens >> u) & 1) {
-            unsigned long long slot = atomicAdd((unsigned long long *)a.raw_n, 1ull);
-            if (slot < (unsigned long long)RAW_CAP) { a.raw_idx[slot] = j; a.raw_part[slot] = R; }
-        }
-        R += it.R[u];
-    }
-}
-
-// in-tile exclusive prefixes of the regular increments and sensitive counts
-__device__ void tile_prefixes(const EsaArgs &a, int64_t tile, int64_t L, const TileItems &it, i128 *lds,
-                              i128 *Racc, int64_t *kpos)
-{
-    i128 racc = 0, cnt = 0;
+    if (!it.sens) return;
+    int64_t cur = pre.sum;
     const int64_t j0 = tile * CTILE + (int64_t)threadIdx.x * CI;
     for (int u = 0; u < CI; u++) {
-        racc += it.R[u];
-        if (j0 + u < L && ((it.sens >> u) & 1)) cnt += 1;
-    }
-    i128 tr, tc;
-    *Racc = a.tpreR[tile] + block_excl_scan<CT>(racc, lds, &tr);
-    *kpos = (int64_t)(a.toff[tile] + block_excl_scan<CT>(cnt, lds, &tc));
-}
-
-// Ordered compaction, used only when more than RAW_CAP elements are sensitive.
-__global__ __launch_bounds__(CT) void esa_compact_kernel(EsaArgs a)
-{
-    __shared__ i128 lds[CT];
-    if (*a.s_n <= RAW_CAP) return;
-    const int64_t L = *a.Lp;
-    const int64_t tile = blockIdx.x;
-    if (tile * CTILE >= L) return;
-    const double v0 = *a.v0p;
-    TileItems it;
-    tile_classify(a, tile, L, v0, it, lds);
-    i128 R;
-    int64_t k;
-    tile_prefixes(a, tile, L, it, lds, &R, &k);
-    const int64_t j0 = tile * CTILE + (int64_t)threadIdx.x * CI;
-    for (int u = 0; u < CI; u++) {
-        int64_t j = j0 + u;
-        if (j >= L) break;
         if ((it.sens >> u) & 1) {
-            if (k < a.smax) { a.s_idx[k] = j; a.s_racc[k] = R; }
-            k++;
+            const unsigned long long slot = atomicAdd((unsigned long long *)a.raw_n, 1ull);
+            if (slot < (unsigned long long)RAW_CAP) {
+                a.raw_idx[slot] = j0 + u;
+                a.raw_part[slot] = cur;
+                a.raw_bp[slot] = it.bp[u];
+                a.raw_sx[slot] = it.sx[u];
+            }
+            cur = 0;
+        } else {
+            cur += it.R[u];
         }
-        R += it.R[u];
     }
 }
 
-// One thread walks the sensitive elements in order; the block stages the list
-// through LDS.  Between two sensitive elements the regular run stays in one binade,
-// so v_prev + (run sum) is exact in double.
-__global__ __launch_bounds__(256) void esa_walk_kernel(EsaArgs a)
+// The sequential part: one thread walks the sensitive elements in order from the
+// chain's start value, each getting fl(value before it + s x), the value before it
+// being the previous one plus the regular sum in between (exact: the run stays in one
+// binade).  The list is in global memory (staged through LDS in rounds) or, from the
+// one-block kernel, already in LDS (lpre, lsx non-null).  A list beyond its capacity
+// leaves every element after the last one kept to the repair.
+__device__ void esa_walk_block(const EsaArgs &a, int64_t ns, const double *lpre, const double *lsx)
 {
-    __shared__ i128 sr[256];
-    __shared__ double sx[256], sv[256];
+    __shared__ double sp[WALK_CHUNK], sx[WALK_CHUNK], sv[WALK_CHUNK];
     __shared__ double Vd;
-    __shared__ i128 Rl;
     const int t = threadIdx.x;
-    int64_t ns = *a.s_n;
-    if (ns <= RAW_CAP) {
-        // rank sort of the classification's list by index (indices are distinct);
-        // the regular prefix gains its tile's offset
-        __shared__ int64_t ridx[RAW_CAP];
-        for (int64_t e = t; e < ns; e += blockDim.x) ridx[e] = a.raw_idx[e];
-        __syncthreads();
-        for (int64_t e = t; e < ns; e += blockDim.x) {
-            const int64_t j = ridx[e];
-            int64_t rank = 0;
-            for (int64_t q = 0; q < ns; q++) rank += ridx[q] < j;
-            a.s_idx[rank] = j;
-            a.s_racc[rank] = a.raw_part[e] + a.tpreR[j / CTILE];
-        }
-        __threadfence();
-        __syncthreads();
+    if (ns > a.smax) {
+        if (t == 0)
+            atomicMin((unsigned long long *)&a.first_bad[1], (unsigned long long)(a.s_idx[a.smax - 1] + 1));
+        ns = a.smax;
     }
-    if (ns > a.smax) ns = a.smax;
-    if (t == 0) { Vd = *a.v0p; Rl = 0; }
+    if (lpre) {
+        if (t == 0) {
+            double vd = *a.v0p;
+            for (int64_t k = 0; k < ns; k++) {
+                const double pred = lpre[k] == 0.0 ? vd : vd + lpre[k];
+                vd = pred + lsx[k];
+                a.s_v[k] = vd;
+            }
+        }
+        return;
+    }
+    if (t == 0) Vd = *a.v0p;
     __syncthreads();
-    for (int64_t c0 = 0; c0 < ns; c0 += 256) {
-        int64_t k = c0 + t;
-        if (k < ns) { sr[t] = a.s_racc[k]; sx[t] = a.x[a.s_idx[k]]; }
+    for (int64_t c0 = 0; c0 < ns; c0 += WALK_CHUNK) {
+        const int64_t k = c0 + t;
+        if (t < WALK_CHUNK && k < ns) {
+            sp[t] = a.s_pre[k];
+            sx[t] = a.sign * a.x[a.s_idx[k]];
+        }
         __syncthreads();
         if (t == 0) {
             double vd = Vd;
-            i128 rl = Rl;
-            int cnt = (int)(ns - c0 < 256 ? ns - c0 : 256);
+            const int cnt = (int)(ns - c0 < WALK_CHUNK ? ns - c0 : WALK_CHUNK);
             for (int u = 0; u < cnt; u++) {
-                i128 ra = sr[u];
-                double pred = ra == rl ? vd : vd + unfxk(ra - rl, a.K);
-                vd = pred + a.sign * sx[u];
-                rl = ra;
+                const double pred = sp[u] == 0.0 ? vd : vd + sp[u];
+                vd = pred + sx[u];
                 sv[u] = vd;
             }
             Vd = vd;
-            Rl = rl;
         }
         __syncthreads();
-        if (k < ns) a.s_v[k] = sv[t];
+        if (t < WALK_CHUNK && k < ns) a.s_v[k] = sv[t];
         __syncthreads();
     }
 }
 
-// value of an element from the last sensitive element kk at or before it
-__device__ __forceinline__ double esa_value(const EsaArgs &a, int64_t kk, int64_t ns_eff, double v0, i128 Rincl)
+// One block: the tiles' sensitive offsets and open-segment carries (a segmented scan
+// of the tile aggregates), then, when the classification's list holds every
+// sensitive element, its order by index and the walk.
+__global__ __launch_bounds__(WALK_T) void esa_walk_kernel(EsaArgs a)
 {
-    if (kk >= ns_eff) kk = ns_eff - 1;
-    if (kk < 0) return v0 + unfxk(Rincl, a.K);
-    return a.s_v[kk] + unfxk(Rincl - a.s_racc[kk], a.K);
+    __shared__ Seg lds[WALK_T / 64];
+    __shared__ int64_t ridx[RAW_CAP];
+    __shared__ double lpre[RAW_CAP], lsx[RAW_CAP];
+    const int t = threadIdx.x;
+    const int64_t L = *a.Lp;
+    const int64_t nt = (L + CTILE - 1) / CTILE;
+    const int64_t chunk = (nt + WALK_T - 1) / WALK_T;
+    const int64_t b0 = (int64_t)t * chunk;
+    Seg acc{0, 0};
+    for (int64_t b = b0; b < b0 + chunk && b < nt; b++) acc = seg_then(acc, Seg{a.ttail[b], a.tcnt[b]});
+    Seg tot;
+    Seg run = block_excl_seg(acc, lds, &tot);
+    for (int64_t b = b0; b < b0 + chunk && b < nt; b++) {
+        a.toff[b] = run.cnt;
+        a.tcarry[b] = run.sum;
+        run = seg_then(run, Seg{a.ttail[b], a.tcnt[b]});
+    }
+    const int64_t ns = tot.cnt;
+    if (t == 0) *a.s_n = ns;
+    if (ns > RAW_CAP) return;                       // the ordered compaction takes over
+    for (int64_t e = t; e < ns; e += WALK_T) ridx[e] = a.raw_idx[e];
+    __syncthreads();                                // (also makes the block's global writes visible to it)
+    for (int64_t e = t; e < ns; e += WALK_T) {
+        const int64_t j = ridx[e];
+        int64_t rank = 0;
+        for (int64_t q = 0; q < ns; q++) rank += ridx[q] < j;      // the indices are distinct
+        const int64_t tile = j / CTILE;
+        const int64_t pre = a.raw_part[e] + (rank == a.toff[tile] ? a.tcarry[tile] : 0);
+        a.s_idx[rank] = j;
+        lpre[rank] = seg_value(pre, a.raw_bp[e]);
+        lsx[rank] = a.raw_sx[e];
+    }
+    __syncthreads();
+    esa_walk_block(a, ns, lpre, lsx);
 }
 
-// Parallel rebuild of every element, verification of the recurrence, and the
-// tile sums of the next Jacobi increments.
+// Only when more than RAW_CAP elements are sensitive: the ordered list from a second
+// classification, and the walk in the last block to arrive.
+__global__ __launch_bounds__(CT) void esa_compact_kernel(EsaArgs a)
+{
+    __shared__ double ldsd[CT / 64];
+    __shared__ Seg ldss[CT / 64];
+    __shared__ double tbuf[TILE_LDS];
+    __shared__ bool last;
+    const int64_t ns = *a.s_n;
+    if (ns <= RAW_CAP) return;
+    const int64_t L = *a.Lp;
+    const int64_t tile = blockIdx.x;
+    if (tile * CTILE < L) {
+        TileItems it;
+        tile_classify(a, tile, L, *a.v0p, it, ldsd, tbuf);
+        Seg tot;
+        const Seg pre = block_excl_seg(items_seg(it), ldss, &tot);
+        int64_t k = a.toff[tile] + pre.cnt;
+        int64_t cur = pre.sum + (pre.cnt ? 0 : a.tcarry[tile]);
+        const int64_t j0 = tile * CTILE + (int64_t)threadIdx.x * CI;
+        for (int u = 0; u < CI; u++) {
+            if ((it.sens >> u) & 1) {
+                if (k < a.smax) {
+                    a.s_idx[k] = j0 + u;
+                    a.s_pre[k] = seg_value(cur, it.bp[u]);
+                }
+                k++;
+                cur = 0;
+            } else {
+                cur += it.R[u];
+            }
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&a.arrive[1], 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) a.arrive[1] = 0;
+    esa_walk_block(a, ns, nullptr, nullptr);
+}
+
+// Parallel rebuild of every element from the last sensitive value before it,
+// verification of the recurrence against the values actually written (the value
+// before a tile is rebuilt exactly as the previous tile wrote it), and the first
+// violation.
 __global__ __launch_bounds__(CT) void esa_rebuild_kernel(EsaArgs a)
 {
-    __shared__ i128 lds[CT];
+    __shared__ double ldsd[CT / 64];
+    __shared__ Seg ldss[CT / 64];
     __shared__ double lastv[CT];
+    __shared__ double tbuf[TILE_LDS];
     const int64_t L = *a.Lp;
     const int64_t tile = blockIdx.x;
     const int64_t base = tile * CTILE;
@@ -546,43 +768,59 @@ __global__ __launch_bounds__(CT) void esa_rebuild_kernel(EsaArgs a)
     int64_t ns_eff = *a.s_n;
     if (ns_eff > a.smax) ns_eff = a.smax;
     TileItems it;
-    tile_classify(a, tile, L, v0, it, lds);
-    i128 R;
-    int64_t k;
-    tile_prefixes(a, tile, L, it, lds, &R, &k);
+    tile_classify(a, tile, L, v0, it, ldsd, tbuf);
+    Seg tot;
+    const Seg pre = block_excl_seg(items_seg(it), ldss, &tot);
+    const int64_t toff = a.toff[tile], tcarry = a.tcarry[tile];
+    int64_t k = toff + pre.cnt;                       // rank of the next sensitive element
+    int64_t cur = pre.sum + (pre.cnt ? 0 : tcarry);
     const int64_t j0 = base + (int64_t)t * CI;
     double nv[CI];
+#pragma unroll
     for (int u = 0; u < CI; u++) {
-        int64_t j = j0 + u;
-        nv[u] = 0.0;
-        if (j >= L) continue;
-        bool sens = (it.sens >> u) & 1;
-        R += it.R[u];
-        if (sens && k < ns_eff) nv[u] = a.s_v[k];
-        else nv[u] = esa_value(a, sens ? k : k - 1, ns_eff, v0, R);
-        if (sens) k++;
+        if ((it.sens >> u) & 1) {
+            nv[u] = k < ns_eff ? a.s_v[k] : it.pa[u];
+            k++;
+            cur = 0;
+        } else {
+            cur += it.R[u];
+            const int64_t kk = k - 1;                 // the last sensitive element before
+            if (kk >= ns_eff) {
+                nv[u] = it.pa[u];
+            } else {
+                const double b = kk >= 0 ? a.s_v[kk] : v0;
+                nv[u] = cur == 0 ? b : b + seg_value(cur, it.bp[u]);
+            }
+        }
     }
     lastv[t] = nv[CI - 1];
     __syncthreads();
     double before;
-    if (t > 0) before = lastv[t - 1];
-    else if (tile == 0) before = v0;
-    else {
-        int64_t kk = (int64_t)a.toff[tile] - 1;
-        if (kk >= 0 && kk < ns_eff && a.s_idx[kk] == base - 1) before = a.s_v[kk];
-        else before = esa_value(a, kk, ns_eff, v0, a.tpreR[tile]);
+    if (t > 0) {
+        before = lastv[t - 1];
+    } else if (tile == 0) {
+        before = v0;
+    } else {
+        const int64_t kk = toff - 1;
+        if (kk >= ns_eff) before = it.pa[0] - it.sx[0];        // past the list: already left to the repair
+        else if (kk >= 0 && a.s_idx[kk] == base - 1) before = a.s_v[kk];
+        else {
+            const double b = kk >= 0 ? a.s_v[kk] : v0;
+            before = tcarry == 0 ? b : b + seg_value(tcarry, it.bp[0]);
+        }
     }
     int64_t bad = L;
+#pragma unroll
     for (int u = 0; u < CI; u++) {
-        int64_t j = j0 + u;
+        const int64_t j = j0 + u;
         if (j < L) {
-            double want = before + a.sign * it.xv[u];
+            const double want = before + it.sx[u];
             if (want != nv[u] && j < bad) bad = j;
-            a.g[j] = nv[u];
         }
         before = nv[u];
     }
     if (bad < L) atomicMin((unsigned long long *)&a.first_bad[1], (unsigned long long)bad);
+    store_tile(a.g, base, L, tbuf, nv);
 }
 
 // Only when the rebuild found a violation: per-tile sums of the Jacobi increments
@@ -652,28 +890,50 @@ __device__ __forceinline__ void pot_total(DevScalars *s, const KParams &P, const
     w.scal[1] = Mtotal;
 }
 
-// chain B increments (star_m[k] / clus.N_STAR, k = N - j); its first thread also
-// finishes chain A (Mtotal, the start value of chain B), which the increments do not use
-__global__ void prep_b_kernel(const double *__restrict__ m, DevScalars *s, double nstar, double *__restrict__ x,
-                              KParams P, const double *__restrict__ ga, PotWork w)
+// chain B increments (star_m[k] / clus.N_STAR, k = N - j) and their tile sums; the
+// first thread also finishes chain A (Mtotal, the start value of chain B), which the
+// increments do not use
+__global__ __launch_bounds__(CT) void prep_b_kernel(const double *__restrict__ m, DevScalars *s, double nstar,
+                                                    double *__restrict__ x, KParams P, const double *__restrict__ ga,
+                                                    PotWork w, EsaArgs e)
 {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) pot_total(s, P, ga, w);
-    int64_t n = s->n_max;
-    if (j < n - 1) x[j] = m[n - j] / nstar;
+    __shared__ double lds[CT / 64];
+    if (blockIdx.x == 0 && threadIdx.x == 0) pot_total(s, P, ga, w);
+    const int64_t n = s->n_max;
+    const int64_t L = n > 1 ? n - 1 : 0;
+    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    double acc = 0.0;
+    for (int u = 0; u < CI; u++) {
+        const int64_t j = base + u * CT + threadIdx.x;
+        if (j < L) {
+            const double v = m[n - j] / nstar;
+            x[j] = v;
+            acc += e.sign * v;
+        }
+    }
+    esa_tile_sums(e, L, acc, lds);
 }
 
-// p_k = M_k * (1/r_k - 1/r_{k+1}) with the reference's roundings; chain C runs on
-// w = -phi, w_k = fl(w_{k+1} + p_k)
-__global__ void prep_c_kernel(const double *__restrict__ r, DevScalars *s, const double *__restrict__ Mk,
-                              double *__restrict__ x)
+// p_k = M_k * (1/r_k - 1/r_{k+1}) with the reference's roundings, and their tile sums;
+// chain C runs on w = -phi, w_k = fl(w_{k+1} + p_k)
+__global__ __launch_bounds__(CT) void prep_c_kernel(const double *__restrict__ r, DevScalars *s,
+                                                    const double *__restrict__ Mk, double *__restrict__ x, EsaArgs e)
 {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t n = s->n_max;
-    if (j >= n) return;
-    int64_t k = n - j;
-    double M = k == n ? s->Mtotal : Mk[n - 1 - k];
-    x[j] = M * (1.0 / r[k] - 1.0 / r[k + 1]);
+    __shared__ double lds[CT / 64];
+    const int64_t n = s->n_max;
+    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    double acc = 0.0;
+    for (int u = 0; u < CI; u++) {
+        const int64_t j = base + u * CT + threadIdx.x;
+        if (j < n) {
+            const int64_t k = n - j;
+            const double M = k == n ? s->Mtotal : Mk[n - 1 - k];
+            const double v = M * (1.0 / r[k] - 1.0 / r[k + 1]);
+            x[j] = v;
+            acc += e.sign * v;
+        }
+    }
+    esa_tile_sums(e, n, acc, lds);
 }
 
 __global__ void phi_out_kernel(const double *__restrict__ r, double *__restrict__ phi, DevScalars *s, KParams P,
@@ -701,19 +961,18 @@ __global__ void pot_at_kernel(const double *__restrict__ r, const double *__rest
     out[i] = v;
 }
 
-hipError_t run_chain(ChainArgs a, EsaArgs e, int rounds, int64_t n_ub, hipStream_t st)
+// The chain's increments must be in e.x; `summed`: their tile sums are already formed
+// (by a producer that calls esa_tile_sums over the same grid).
+hipError_t run_chain(ChainArgs a, EsaArgs e, int rounds, int64_t n_ub, hipStream_t st, bool summed = false)
 {
     int64_t ntiles = (n_ub + CTILE - 1) / CTILE;
     if (ntiles < 1) ntiles = 1;
     dim3 gt((unsigned)ntiles), bt(CT);
-    hipLaunchKernelGGL(esa_sum_kernel, gt, bt, 0, st, e);
-    hipLaunchKernelGGL(esa_tile_scan_kernel, dim3(1), dim3(SCAN_T), 0, st, (const i128 *)e.tsumP, e.tpreP,
-                       (const i128 *)nullptr, (i128 *)nullptr, (int64_t *)nullptr, e.Lp);
+    if (!summed) hipLaunchKernelGGL(esa_sum_kernel, gt, bt, 0, st, e);
+    hipLaunchKernelGGL(esa_tile_scan_kernel, dim3(1), dim3(WALK_T), 0, st, e, ntiles);
     hipLaunchKernelGGL(esa_classify_kernel, gt, bt, 0, st, e);
-    hipLaunchKernelGGL(esa_tile_scan_kernel, dim3(1), dim3(SCAN_T), 0, st, (const i128 *)e.tsumR, e.tpreR,
-                       (const i128 *)e.tcnt, e.toff, e.s_n, e.Lp);
+    hipLaunchKernelGGL(esa_walk_kernel, dim3(1), dim3(WALK_T), 0, st, e);
     hipLaunchKernelGGL(esa_compact_kernel, gt, bt, 0, st, e);
-    hipLaunchKernelGGL(esa_walk_kernel, dim3(1), dim3(256), 0, st, e);
     hipLaunchKernelGGL(esa_rebuild_kernel, gt, bt, 0, st, e);
     hipLaunchKernelGGL(chain_seed_from_g_kernel, gt, bt, 0, st, a);
     for (int it = 0; it < rounds; it++) {
@@ -722,6 +981,36 @@ hipError_t run_chain(ChainArgs a, EsaArgs e, int rounds, int64_t n_ub, hipStream
     }
     hipLaunchKernelGGL(chain_finish_kernel, dim3(1), dim3(1), 0, st, a);
     return hipGetLastError();
+}
+
+// The chains' per-tile and list buffers in the context's scratch.  Per tile, in int128
+// units of tile_sums: [0,T) [T,2T) the repair rounds' sums and prefixes, [2T,3T) the
+// estimated sums and prefixes (doubles), [3T,4T) sensitive counts and tail sums,
+// [4T,5T) offsets and carries (int64), [8T,8.5T) the repair's boundary values.
+EsaArgs esa_args(const Scratch &sc, int slack)
+{
+    i128 *tb = (i128 *)sc.tile_sums;
+    const int64_t T = sc.max_tiles;
+    EsaArgs e{};
+    e.slack = slack;
+    e.tsumP = (double *)(tb + 2 * T);
+    e.tpreP = e.tsumP + T;
+    e.tcnt = (int64_t *)(tb + 3 * T);
+    e.ttail = e.tcnt + T;
+    e.toff = (int64_t *)(tb + 4 * T);
+    e.tcarry = e.toff + T;
+    e.arrive = (uint32_t *)(sc.esa_n + 2);
+    e.s_idx = sc.esa_idx;
+    e.s_pre = sc.esa_racc;
+    e.s_v = sc.esa_v;
+    e.s_n = sc.esa_n;
+    e.smax = sc.esa_max;
+    e.raw_n = sc.esa_n + 1;
+    e.raw_idx = sc.esa_raw_idx;
+    e.raw_part = (int64_t *)sc.esa_raw_part;
+    e.raw_bp = (int32_t *)(sc.esa_raw_part + RAW_CAP);
+    e.raw_sx = sc.esa_raw_part + 2 * RAW_CAP;
+    return e;
 }
 } // namespace
 
@@ -841,50 +1130,40 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     w.scal = sc.chain_scal;
     w.len = sc.chain_len;
     w.fbad = sc.chain_fbad;
-    // per tile: tsum tpre tsumP tpreP tsumR tpreR tcnt toff (int128) and bnd (double)
     i128 *tb = (i128 *)sc.tile_sums;
     const int64_t T = sc.max_tiles;
     w.tsum = tb;
     w.tpre = tb + T;
     w.bnd = (double *)(tb + 8 * T);
-    EsaArgs e{};
-    e.tsumP = tb + 2 * T; e.tpreP = tb + 3 * T; e.tsumR = tb + 4 * T; e.tpreR = tb + 5 * T;
-    e.tcnt = tb + 6 * T; e.toff = tb + 7 * T;
-    e.tsum = w.tsum;
-    e.s_idx = sc.esa_idx;
-    e.s_racc = (i128 *)sc.esa_racc;
-    e.s_v = sc.esa_v;
-    e.s_n = sc.esa_n;
-    e.smax = sc.esa_max;
-    e.raw_n = sc.esa_n + 1;
-    e.raw_idx = sc.esa_raw_idx;
-    e.raw_part = (i128 *)sc.esa_raw_part;
-    unsigned g1 = (unsigned)((n_ub + 256) / 256);
+    // a test hook widens the classification's error bound and coarsens the repair's
+    // fixed-point scales (2^-70: stored masses are O(1), and 2^-100), which sends most
+    // elements through the fallback paths
+    const EsaArgs e = esa_args(sc, P.chain_k_drop);
+    const unsigned g1 = (unsigned)((n_ub + 256) / 256);
+    const dim3 gt((unsigned)(n_ub > 0 ? (n_ub + CTILE - 1) / CTILE : 1));
     hipLaunchKernelGGL(pot_init_kernel, dim3(1), dim3(1), 0, st, s);
     if (n_ub > 0) hipLaunchKernelGGL(pot_count_kernel, dim3(g1), dim3(256), 0, st, f.r, s);
     hipLaunchKernelGGL(pot_setup_kernel, dim3(1), dim3(1), 0, st, s, w, f.r, f.phi);
-    // chain A: total mass, straight over star_m[1..N]
-    // fixed-point scales 2^-70 (stored masses are O(1)) and 2^-100; a test hook may
-    // coarsen them, which sends most elements through the fallback paths
     const int KA = 70 - P.chain_k_drop < 8 ? 8 : 70 - P.chain_k_drop;
     const int KB = 100 - P.chain_k_drop < 8 ? 8 : 100 - P.chain_k_drop;
+    // chain A: total mass, straight over star_m[1..N]
     ChainArgs a{f.m + 1, w.ga, 1.0, w.scal + 0, w.fbad + 0, w.tsum, w.tpre, w.bnd, w.len + 0, KA};
     EsaArgs ae = e;
-    ae.x = f.m + 1; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0; ae.K = KA;
+    ae.x = f.m + 1; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0;
     if (hipError_t err = run_chain(a, ae, REPAIR_ROUNDS, n_ub, st)) return err;
     // chain B: enclosed mass, iterated directly in Mk (chain C's input)
-    hipLaunchKernelGGL(prep_b_kernel, dim3(g1), dim3(256), 0, st, f.m, s, (double)P.n_star, w.xa, P,
-                       (const double *)w.ga, w);
     ChainArgs b{w.xa, w.Mk, -1.0, w.scal + 1, w.fbad + 2, w.tsum, w.tpre, w.bnd, w.len + 1, KB};
     EsaArgs be = e;
-    be.x = w.xa; be.g = w.Mk; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1; be.K = KB;
-    if (hipError_t err = run_chain(b, be, REPAIR_ROUNDS, n_ub, st)) return err;
+    be.x = w.xa; be.g = w.Mk; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1;
+    hipLaunchKernelGGL(prep_b_kernel, gt, dim3(CT), 0, st, f.m, s, (double)P.n_star, w.xa, P, (const double *)w.ga, w,
+                       be);
+    if (hipError_t err = run_chain(b, be, REPAIR_ROUNDS, n_ub, st, true)) return err;
     // chain C: w = -phi
-    hipLaunchKernelGGL(prep_c_kernel, dim3(g1), dim3(256), 0, st, f.r, s, (const double *)w.Mk, w.xa);
     ChainArgs c{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 4, w.tsum, w.tpre, w.bnd, w.len + 2, KB};
     EsaArgs ce = e;
-    ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2; ce.K = KB;
-    if (hipError_t err = run_chain(c, ce, REPAIR_ROUNDS, n_ub, st)) return err;
+    ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2;
+    hipLaunchKernelGGL(prep_c_kernel, gt, dim3(CT), 0, st, f.r, s, (const double *)w.Mk, w.xa, ce);
+    if (hipError_t err = run_chain(c, ce, REPAIR_ROUNDS, n_ub, st, true)) return err;
     hipLaunchKernelGGL(phi_out_kernel, dim3(g1), dim3(256), 0, st, f.r, f.phi, s, P, (const double *)w.ga);
     return hipGetLastError();
 }
@@ -909,18 +1188,7 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
     if (ntiles > sc.max_tiles) return hipErrorInvalidValue;
     i128 *tb = (i128 *)sc.tile_sums;
     const int64_t T = sc.max_tiles;
-    EsaArgs e{};
-    e.tsumP = tb + 2 * T; e.tpreP = tb + 3 * T; e.tsumR = tb + 4 * T; e.tpreR = tb + 5 * T;
-    e.tcnt = tb + 6 * T; e.toff = tb + 7 * T;
-    e.tsum = tb;
-    e.s_idx = sc.esa_idx;
-    e.s_racc = (i128 *)sc.esa_racc;
-    e.s_v = sc.esa_v;
-    e.s_n = sc.esa_n;
-    e.smax = sc.esa_max;
-    e.raw_n = sc.esa_n + 1;
-    e.raw_idx = sc.esa_raw_idx;
-    e.raw_part = (i128 *)sc.esa_raw_part;
+    EsaArgs e = esa_args(sc, P.chain_k_drop);
     e.x = sc.chain_x; e.g = sc.chain_g; e.sign = 1.0; e.first_bad = sc.chain_fbad + 6;
     e.Lp = sc.chain_len + 3;
     double *v0 = sc.chain_scal + 2;              // start values: [2] mprev0, [3] 0
@@ -934,8 +1202,8 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
     hipLaunchKernelGGL(mpc_prep_m_kernel, dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
                        sc.chain_x, sc.chain_len);
     EsaArgs em = e;
-    em.v0p = v0; em.K = Kof(100);
-    ChainArgs cm{sc.chain_x, sc.chain_g, 1.0, v0, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, em.K};
+    em.v0p = v0;
+    ChainArgs cm{sc.chain_x, sc.chain_g, 1.0, v0, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, Kof(100)};
     if (hipError_t err = run_chain(cm, em, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(mpc_assign_kernel, dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
                        (const double *)sc.chain_g, pc, count, bh_include, mprev0, itm, minimum_r, o);
@@ -944,8 +1212,9 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
         hipLaunchKernelGGL(mpc_prep_v_kernel, dim3(g1), dim3(256), 0, st, f, (const DevScalars *)s, P.madhoc, term,
                            sc.chain_x, sc.chain_len);
         EsaArgs ev = e;
-        ev.v0p = v0 + 1; ev.K = Kof(term < 2 ? 100 : 80);
-        ChainArgs cv{sc.chain_x, sc.chain_g, 1.0, v0 + 1, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, ev.K};
+        ev.v0p = v0 + 1;
+        ChainArgs cv{sc.chain_x, sc.chain_g, 1.0, v0 + 1, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp,
+                     Kof(term < 2 ? 100 : 80)};
         if (hipError_t err = run_chain(cv, ev, REPAIR_ROUNDS, n_ub, st)) return err;
         hipLaunchKernelGGL(mpc_take_kernel, dim3(1), dim3(MPC_MAX), 0, st, (const double *)sc.chain_g, o, term, count);
     }
@@ -1145,18 +1414,7 @@ hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, i
     if (ntiles > sc.max_tiles) return hipErrorInvalidValue;
     i128 *tb = (i128 *)sc.tile_sums;
     const int64_t T = sc.max_tiles;
-    EsaArgs e{};
-    e.tsumP = tb + 2 * T; e.tpreP = tb + 3 * T; e.tsumR = tb + 4 * T; e.tpreR = tb + 5 * T;
-    e.tcnt = tb + 6 * T; e.toff = tb + 7 * T;
-    e.tsum = tb;
-    e.s_idx = sc.esa_idx;
-    e.s_racc = (i128 *)sc.esa_racc;
-    e.s_v = sc.esa_v;
-    e.s_n = sc.esa_n;
-    e.smax = sc.esa_max;
-    e.raw_n = sc.esa_n + 1;
-    e.raw_idx = sc.esa_raw_idx;
-    e.raw_part = (i128 *)sc.esa_raw_part;
+    EsaArgs e = esa_args(sc, P.chain_k_drop);
     e.x = sc.chain_x; e.g = sc.chain_g; e.sign = 1.0; e.first_bad = sc.chain_fbad + 6;
     e.Lp = sc.chain_len + 3;
     double *v0 = sc.chain_scal + 2;              // both start values 0
@@ -1169,7 +1427,7 @@ hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, i
     hipLaunchKernelGGL(mpc_prep_m_kernel, dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
                        sc.chain_x, sc.chain_len);
     EsaArgs em = e;
-    em.v0p = v0; em.K = K;
+    em.v0p = v0;
     ChainArgs cmass{sc.chain_x, sc.chain_g, 1.0, v0, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, K};
     if (hipError_t err = run_chain(cmass, em, REPAIR_ROUNDS, n_ub, st)) return err;
     hipLaunchKernelGGL(cen_nave_kernel, dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
@@ -1180,7 +1438,7 @@ hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, i
     for (int t = 0; t < CEN_TERMS; t++) {
         hipLaunchKernelGGL(cen_prep_kernel, dim3(g1), dim3(256), 0, st, f, P.madhoc, t, o, sc.chain_x);
         EsaArgs es = e;
-        es.v0p = v0 + 1; es.K = K;
+        es.v0p = v0 + 1;
         ChainArgs cs{sc.chain_x, sc.chain_g, 1.0, v0 + 1, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, K};
         if (hipError_t err = run_chain(cs, es, REPAIR_ROUNDS, n_ub, st)) return err;
         hipLaunchKernelGGL(cen_take_kernel, dim3(1), dim3(1), 0, st, (const double *)sc.chain_g, o, t);

@@ -34,11 +34,11 @@ struct Scratch {
     int64_t *chain_fbad;        // 6
     // potential chains: list of sensitive elements (index, int128 prefix, value)
     int64_t *esa_idx;
-    double *esa_racc;           // int128 per entry
+    double *esa_racc;           // regular sum before each listed element
     double *esa_v;
-    int64_t *esa_n;             // [0] sensitive count, [1] appended count
+    int64_t *esa_n;             // [0] sensitive count, [1] appended count, [2] two arrival counters
     int64_t *esa_raw_idx;       // appended sensitive elements (unordered), 2048
-    double *esa_raw_part;       // their in-tile prefixes (int128 each)
+    double *esa_raw_part;       // their in-tile regular sums (int64), binades (int32) and s x
     int64_t esa_max;
     // deterministic block partials: 8 doubles per block
     double *partials;

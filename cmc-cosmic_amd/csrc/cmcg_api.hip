@@ -405,7 +405,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     while (sc.ridx_bits < 22 && ((int64_t)1 << sc.ridx_bits) < n_cap + 2) sc.ridx_bits++;
     if ((e = dalloc(c, &sc.ridx, (size_t)ridx_entries(sc.ridx_bits))) ||
         (e = dalloc(c, &sc.ridx_tb, (size_t)ridx_tiles(sc.ridx_bits) + 1))) { cmcg_destroy(c); return e; }
-    if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 18)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
+    if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 6)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)(sc.sort_long_cap = n_cap / (SORT_RUN_SHORT + 1) + 2))) ||
         (e = dalloc(c, &sc.sort_huge, (size_t)(sc.sort_huge_cap = n_cap / (SORT_RUN_LONG + 1) + 2))) ||
         (e = dalloc(c, &sc.sort_nlong, 2)) ||

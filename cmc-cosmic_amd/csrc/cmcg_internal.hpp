@@ -70,7 +70,7 @@ struct KParams {
     double theta_se_max;
     double gamma;
     int64_t avekernel;
-    int32_t chain_k_drop;  // test hook: coarser fixed-point scale in the potential chains
+    int32_t chain_k_drop;  // test hook: potential chains' error bound widened by 2^chain_k_drop
     int32_t pad0;
 };
 

@@ -25,227 +25,18 @@
 //   the increments), classification, walk, rebuild.
 // * Every element is verified against fl(v_{j-1} + s x_j), so the estimate's bound
 //   decides only the speed, never the result.  A violation (only when the bound or
-//   the sensitive-list capacity is exceeded) is repaired by Jacobi rounds in which
-//   each bracket fl(g_{i-1} + s x_i) - g_{i-1} of the previous iterate g is an exact
-//   int128 (2^-K fixed point) and each round restarts at the first violation, and
-//   finally by a sequential pass.  The result equals the sequential chain exactly.
+//   the sensitive-list capacity is exceeded) is repaired by running the recurrence
+//   itself from the first violating element.  The result equals the sequential
+//   chain exactly.
 #include "kernels.hpp"
 
 namespace {
 constexpr int CT = 256;             // threads per tile
 constexpr int CI = 8;               // items per thread
 constexpr int CTILE = CT * CI;
-constexpr int SCAN_T = 1024;
-constexpr int REPAIR_ROUNDS = 1;      // Jacobi rounds after the segment-and-walk pass (then sequential)
 constexpr int RAW_CAP = 2048;         // sensitive elements sorted in LDS by the walk
 constexpr int WALK_T = 1024;          // threads of the one-block kernels
 constexpr int WALK_CHUNK = 256;       // sensitive elements staged per round of the walk
-
-typedef __int128 i128;
-
-struct ChainArgs {
-    const double *x;      // increments in processing order, x[0..L-1]
-    double *g;            // iterate, updated in place, g[0..L-1]
-    double sign;          // +1 or -1
-    const double *v0p;    // start value v_{-1} (device scalar)
-    int64_t *first_bad;   // [0] restart index of this round, [1] first violation found
-    i128 *tsum;           // per-tile increment sums for the next round
-    i128 *tpre;           // per-tile exclusive prefixes
-    double *bnd;          // per-tile last value of the previous iterate
-    const int64_t *Lp;    // chain length (device scalar)
-    int K;                // fixed-point scale 2^-K
-};
-
-__device__ __forceinline__ i128 fxk(double d, int K)
-{
-    // d * 2^K as an integer, exact when d is a multiple of 2^-K (truncated otherwise)
-    if (d == 0.0) return 0;
-    bool neg = d < 0.0;
-    double a = neg ? -d : d;
-    int e;
-    double mfrac = frexp(a, &e);
-    uint64_t M = (uint64_t)ldexp(mfrac, 53);
-    int sh = e - 53 + K;
-    i128 v;
-    if (sh >= 0) v = sh >= 74 ? ((i128)1 << 126) : ((i128)M << sh);
-    else v = sh <= -64 ? (i128)0 : (i128)(M >> (-sh));
-    return neg ? -v : v;
-}
-
-__device__ __forceinline__ double unfxk(i128 v, int K)
-{
-    bool neg = v < 0;
-    unsigned __int128 u = neg ? (unsigned __int128)(-v) : (unsigned __int128)v;
-    uint64_t hi = (uint64_t)(u >> 64), lo = (uint64_t)u;
-    double d = ldexp((double)hi, 64) + (double)lo;    // exact when the value is a double
-    d = ldexp(d, -K);
-    return neg ? -d : d;
-}
-
-__device__ __forceinline__ i128 incr(double prev, double x, double sign, int K)
-{
-    double s = prev + sign * x;
-    return fxk(s, K) - fxk(prev, K);
-}
-
-__device__ __forceinline__ i128 shfl_up_i128(i128 v, int d)
-{
-    const unsigned __int128 u = (unsigned __int128)v;
-    unsigned long long lo = (unsigned long long)u, hi = (unsigned long long)(u >> 64);
-    lo = __shfl_up(lo, d, 64);
-    hi = __shfl_up(hi, d, 64);
-    return (i128)(((unsigned __int128)hi << 64) | lo);
-}
-
-// block-wide exclusive scan of int128: inclusive scans inside each 64-lane wave by
-// shuffles, then the wave totals through LDS (lds needs NT / 64 entries); every
-// thread of the block must call it
-template <int NT>
-__device__ i128 block_excl_scan(i128 v, i128 *lds, i128 *total)
-{
-    constexpr int NW = NT / 64;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    i128 x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const i128 y = shfl_up_i128(x, d);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) lds[w] = x;
-    __syncthreads();
-    i128 off = 0, tot = 0;
-#pragma unroll
-    for (int q = 0; q < NW; q++) {
-        const i128 s = lds[q];
-        if (q < w) off += s;
-        tot += s;
-    }
-    *total = tot;
-    __syncthreads();
-    return off + x - v;
-}
-
-// One block: takes the restart index found by the previous round, snapshots the
-// tile-boundary values of the current iterate and scans the tile sums from the
-// restart tile on.
-__global__ __launch_bounds__(SCAN_T) void chain_scan_kernel(ChainArgs a)
-{
-    __shared__ i128 lds[SCAN_T];
-    __shared__ int64_t fbs;
-    const int64_t L = *a.Lp;
-    if (threadIdx.x == 0) {
-        int64_t fb = a.first_bad[1];
-        if (a.first_bad[0] >= L) fb = L;          // already converged
-        a.first_bad[0] = fb;
-        a.first_bad[1] = L;
-        fbs = fb;
-    }
-    __syncthreads();
-    const int64_t fb = fbs;
-    if (fb >= L) return;
-    const int64_t t_lo = fb / CTILE;
-    const int64_t ntiles = (L + CTILE - 1) / CTILE;
-    for (int64_t b = (t_lo > 0 ? t_lo - 1 : 0) + threadIdx.x; b < ntiles; b += SCAN_T) {
-        int64_t last = (b + 1) * CTILE - 1;
-        a.bnd[b] = a.g[last < L ? last : L - 1];
-    }
-    const int64_t nt = ntiles - t_lo;
-    const int64_t chunk = (nt + SCAN_T - 1) / SCAN_T;
-    const int64_t b0 = t_lo + (int64_t)threadIdx.x * chunk;
-    i128 acc = 0;
-    for (int64_t b = b0; b < b0 + chunk && b < ntiles; b++) acc += a.tsum[b];
-    i128 tot;
-    i128 run = block_excl_scan<SCAN_T>(acc, lds, &tot);
-    for (int64_t b = b0; b < b0 + chunk && b < ntiles; b++) {
-        a.tpre[b] = run;
-        run += a.tsum[b];
-    }
-}
-
-// One round, in place: new iterate from exact prefix sums of the previous
-// iterate's increments, verification of the recurrence, next increments.
-__global__ __launch_bounds__(CT) void chain_round_kernel(ChainArgs a)
-{
-    __shared__ i128 lds[CT];
-    __shared__ double lastv[CT];
-    const int64_t L = *a.Lp;
-    const int64_t fb = a.first_bad[0];
-    if (fb >= L) return;
-    const int64_t t_lo = fb / CTILE;
-    const int64_t tile = t_lo + blockIdx.x;
-    const int64_t base = tile * CTILE;
-    if (base >= L) return;
-    const double v0 = *a.v0p;
-    const int t = threadIdx.x;
-    const int64_t j0 = base + (int64_t)t * CI;
-    // value before the restart tile is final
-    const double vstart = t_lo == 0 ? v0 : a.bnd[t_lo - 1];
-    const i128 Vstart = fxk(vstart, a.K);
-    double xv[CI], gv[CI];
-    for (int u = 0; u < CI; u++) {
-        int64_t j = j0 + u;
-        xv[u] = j < L ? a.x[j] : 0.0;
-        gv[u] = j < L ? a.g[j] : 0.0;
-    }
-    lastv[t] = gv[CI - 1];
-    __syncthreads();
-    // predecessor of the first item under the previous iterate
-    double prev = t > 0 ? lastv[t - 1] : (tile == 0 ? v0 : a.bnd[tile - 1]);
-    i128 inc[CI];
-    i128 acc = 0;
-    for (int u = 0; u < CI; u++) {
-        int64_t j = j0 + u;
-        inc[u] = 0;
-        if (j < L) {
-            inc[u] = incr(j == 0 ? v0 : prev, xv[u], a.sign, a.K);
-            acc += inc[u];
-        }
-        prev = gv[u];
-    }
-    i128 tot;
-    i128 run = Vstart + a.tpre[tile] + block_excl_scan<CT>(acc, lds, &tot);
-    double nv[CI];
-    for (int u = 0; u < CI; u++) {
-        run += inc[u];
-        nv[u] = unfxk(run, a.K);
-    }
-    __syncthreads();
-    lastv[t] = nv[CI - 1];
-    __syncthreads();
-    double before = t > 0 ? lastv[t - 1] : (tile == t_lo ? vstart : unfxk(Vstart + a.tpre[tile], a.K));
-    i128 nacc = 0;
-    int64_t bad = L;
-    for (int u = 0; u < CI; u++) {
-        int64_t j = j0 + u;
-        if (j < L) {
-            double p = j == 0 ? v0 : before;
-            double want = p + a.sign * xv[u];
-            if (want != nv[u] && j < bad) bad = j;
-            nacc += incr(p, xv[u], a.sign, a.K);
-            a.g[j] = nv[u];
-        }
-        before = nv[u];
-    }
-    if (bad < L) atomicMin((unsigned long long *)&a.first_bad[1], (unsigned long long)bad);
-    block_excl_scan<CT>(nacc, lds, &tot);
-    if (t == 0) a.tsum[tile] = tot;
-}
-
-// Bounded sequential completion from the first element that still violates the
-// recurrence after the last round (normally none).
-__global__ void chain_finish_kernel(ChainArgs a)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int64_t L = *a.Lp;
-    int64_t j = a.first_bad[0] >= L ? L : a.first_bad[1];
-    if (j >= L) return;
-    double v = j == 0 ? *a.v0p : a.g[j - 1];
-    for (; j < L; j++) {
-        v = v + a.sign * a.x[j];
-        a.g[j] = v;
-    }
-    a.first_bad[0] = L;
-}
 
 // ---------------------------------------------------------------- segment and walk
 // The classification needs only the binade of each chain value, so it runs on a
@@ -259,7 +50,7 @@ struct EsaArgs {
     double *g;
     double sign;
     const double *v0p;
-    int64_t *first_bad;         // as in ChainArgs, initialised with the tile sums
+    int64_t *first_bad;         // [1] first element that violates the recurrence (L: none)
     const int64_t *Lp;
     int slack;                  // error bound widened by 2^slack (0; a test hook raises it)
     // per tile
@@ -823,24 +614,36 @@ __global__ __launch_bounds__(CT) void esa_rebuild_kernel(EsaArgs a)
     store_tile(a.g, base, L, tbuf, nv);
 }
 
-// Only when the rebuild found a violation: per-tile sums of the Jacobi increments
-// fl(g_{j-1} + s x_j) - g_{j-1} of the rebuilt chain, which the repair rounds scan.
-__global__ __launch_bounds__(CT) void chain_seed_from_g_kernel(ChainArgs a)
+// The repair, only after a violation (the estimate's bound or the list's capacity
+// exceeded, which a monotone chain of ordinary size never does): the recurrence
+// itself from the first violating element on, one thread over chunks that the block
+// stages through LDS in coalesced rows.
+constexpr int FIN_CHUNK = 4 * WALK_T;
+__global__ __launch_bounds__(WALK_T) void chain_finish_kernel(EsaArgs a)
 {
-    __shared__ i128 lds[CT];
+    __shared__ double buf[FIN_CHUNK];
+    __shared__ double vlast;
     const int64_t L = *a.Lp;
-    if (a.first_bad[1] >= L) return;
-    const int64_t base = (int64_t)blockIdx.x * CTILE;
-    if (base >= L) return;
-    const double v0 = *a.v0p;
-    i128 acc = 0;
-    for (int u = 0; u < CI; u++) {
-        int64_t j = base + (int64_t)threadIdx.x * CI + u;
-        if (j < L) acc += incr(j == 0 ? v0 : a.g[j - 1], a.x[j], a.sign, a.K);
+    const int64_t j0 = a.first_bad[1];
+    if (j0 >= L) return;
+    const int t = threadIdx.x;
+    if (t == 0) vlast = j0 == 0 ? *a.v0p : a.g[j0 - 1];
+    for (int64_t c0 = j0; c0 < L; c0 += FIN_CHUNK) {
+        const int cnt = (int)(L - c0 < FIN_CHUNK ? L - c0 : FIN_CHUNK);
+        for (int i = t; i < cnt; i += WALK_T) buf[i] = a.sign * a.x[c0 + i];
+        __syncthreads();
+        if (t == 0) {
+            double v = vlast;
+            for (int i = 0; i < cnt; i++) {
+                v = v + buf[i];
+                buf[i] = v;
+            }
+            vlast = v;
+        }
+        __syncthreads();
+        for (int i = t; i < cnt; i += WALK_T) a.g[c0 + i] = buf[i];
+        __syncthreads();
     }
-    i128 tot;
-    block_excl_scan<CT>(acc, lds, &tot);
-    if (threadIdx.x == 0) a.tsum[blockIdx.x] = tot;
 }
 
 // ---------------------------------------------------------------- the three chains
@@ -855,13 +658,11 @@ __global__ void pot_count_kernel(const double *__restrict__ r, DevScalars *s)
 }
 
 struct PotWork {
-    double *xa, *ga, *gb;      // chain buffers (x, iterate, scratch iterate)
+    double *xa, *ga;           // chain buffers (increments, values)
     double *Mk;                // chain B result: M_{N-1-j}
-    double *scal;              // [0] 0.0  [1] Mtotal  [2] chain lengths as doubles (unused)
+    double *scal;              // [0] 0.0  [1] Mtotal
     int64_t *len;              // [0] L_A  [1] L_B  [2] L_C
     int64_t *fbad;             // 2 per chain
-    i128 *tsum, *tpre;
-    double *bnd;
 };
 
 // N_MAX from the count, the chain lengths, and the sentinels before the recurrence
@@ -963,7 +764,7 @@ __global__ void pot_at_kernel(const double *__restrict__ r, const double *__rest
 
 // The chain's increments must be in e.x; `summed`: their tile sums are already formed
 // (by a producer that calls esa_tile_sums over the same grid).
-hipError_t run_chain(ChainArgs a, EsaArgs e, int rounds, int64_t n_ub, hipStream_t st, bool summed = false)
+hipError_t run_chain(const EsaArgs &e, int64_t n_ub, hipStream_t st, bool summed = false)
 {
     int64_t ntiles = (n_ub + CTILE - 1) / CTILE;
     if (ntiles < 1) ntiles = 1;
@@ -974,31 +775,24 @@ hipError_t run_chain(ChainArgs a, EsaArgs e, int rounds, int64_t n_ub, hipStream
     hipLaunchKernelGGL(esa_walk_kernel, dim3(1), dim3(WALK_T), 0, st, e);
     hipLaunchKernelGGL(esa_compact_kernel, gt, bt, 0, st, e);
     hipLaunchKernelGGL(esa_rebuild_kernel, gt, bt, 0, st, e);
-    hipLaunchKernelGGL(chain_seed_from_g_kernel, gt, bt, 0, st, a);
-    for (int it = 0; it < rounds; it++) {
-        hipLaunchKernelGGL(chain_scan_kernel, dim3(1), dim3(SCAN_T), 0, st, a);
-        hipLaunchKernelGGL(chain_round_kernel, gt, bt, 0, st, a);
-    }
-    hipLaunchKernelGGL(chain_finish_kernel, dim3(1), dim3(1), 0, st, a);
+    hipLaunchKernelGGL(chain_finish_kernel, dim3(1), dim3(WALK_T), 0, st, e);
     return hipGetLastError();
 }
 
-// The chains' per-tile and list buffers in the context's scratch.  Per tile, in int128
-// units of tile_sums: [0,T) [T,2T) the repair rounds' sums and prefixes, [2T,3T) the
-// estimated sums and prefixes (doubles), [3T,4T) sensitive counts and tail sums,
-// [4T,5T) offsets and carries (int64), [8T,8.5T) the repair's boundary values.
+// The chains' per-tile and list buffers in the context's scratch (tile_sums: six
+// 8-byte words per tile).
 EsaArgs esa_args(const Scratch &sc, int slack)
 {
-    i128 *tb = (i128 *)sc.tile_sums;
+    double *tb = sc.tile_sums;
     const int64_t T = sc.max_tiles;
     EsaArgs e{};
     e.slack = slack;
-    e.tsumP = (double *)(tb + 2 * T);
-    e.tpreP = e.tsumP + T;
-    e.tcnt = (int64_t *)(tb + 3 * T);
-    e.ttail = e.tcnt + T;
+    e.tsumP = tb;
+    e.tpreP = tb + T;
+    e.tcnt = (int64_t *)(tb + 2 * T);
+    e.ttail = (int64_t *)(tb + 3 * T);
     e.toff = (int64_t *)(tb + 4 * T);
-    e.tcarry = e.toff + T;
+    e.tcarry = (int64_t *)(tb + 5 * T);
     e.arrive = (uint32_t *)(sc.esa_n + 2);
     e.s_idx = sc.esa_idx;
     e.s_pre = sc.esa_racc;
@@ -1125,45 +919,33 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     PotWork w;
     w.xa = sc.chain_x;
     w.ga = sc.chain_g;
-    w.gb = nullptr;
     w.Mk = sc.chain_mk;
     w.scal = sc.chain_scal;
     w.len = sc.chain_len;
     w.fbad = sc.chain_fbad;
-    i128 *tb = (i128 *)sc.tile_sums;
-    const int64_t T = sc.max_tiles;
-    w.tsum = tb;
-    w.tpre = tb + T;
-    w.bnd = (double *)(tb + 8 * T);
-    // a test hook widens the classification's error bound and coarsens the repair's
-    // fixed-point scales (2^-70: stored masses are O(1), and 2^-100), which sends most
-    // elements through the fallback paths
+    // a test hook widens the classification's error bound, which sends most elements
+    // through the fallback paths
     const EsaArgs e = esa_args(sc, P.chain_k_drop);
     const unsigned g1 = (unsigned)((n_ub + 256) / 256);
     const dim3 gt((unsigned)(n_ub > 0 ? (n_ub + CTILE - 1) / CTILE : 1));
     hipLaunchKernelGGL(pot_init_kernel, dim3(1), dim3(1), 0, st, s);
     if (n_ub > 0) hipLaunchKernelGGL(pot_count_kernel, dim3(g1), dim3(256), 0, st, f.r, s);
     hipLaunchKernelGGL(pot_setup_kernel, dim3(1), dim3(1), 0, st, s, w, f.r, f.phi);
-    const int KA = 70 - P.chain_k_drop < 8 ? 8 : 70 - P.chain_k_drop;
-    const int KB = 100 - P.chain_k_drop < 8 ? 8 : 100 - P.chain_k_drop;
     // chain A: total mass, straight over star_m[1..N]
-    ChainArgs a{f.m + 1, w.ga, 1.0, w.scal + 0, w.fbad + 0, w.tsum, w.tpre, w.bnd, w.len + 0, KA};
     EsaArgs ae = e;
     ae.x = f.m + 1; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0;
-    if (hipError_t err = run_chain(a, ae, REPAIR_ROUNDS, n_ub, st)) return err;
+    if (hipError_t err = run_chain(ae, n_ub, st)) return err;
     // chain B: enclosed mass, iterated directly in Mk (chain C's input)
-    ChainArgs b{w.xa, w.Mk, -1.0, w.scal + 1, w.fbad + 2, w.tsum, w.tpre, w.bnd, w.len + 1, KB};
     EsaArgs be = e;
     be.x = w.xa; be.g = w.Mk; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1;
     hipLaunchKernelGGL(prep_b_kernel, gt, dim3(CT), 0, st, f.m, s, (double)P.n_star, w.xa, P, (const double *)w.ga, w,
                        be);
-    if (hipError_t err = run_chain(b, be, REPAIR_ROUNDS, n_ub, st, true)) return err;
+    if (hipError_t err = run_chain(be, n_ub, st, true)) return err;
     // chain C: w = -phi
-    ChainArgs c{w.xa, w.ga, 1.0, w.scal + 0, w.fbad + 4, w.tsum, w.tpre, w.bnd, w.len + 2, KB};
     EsaArgs ce = e;
     ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2;
     hipLaunchKernelGGL(prep_c_kernel, gt, dim3(CT), 0, st, f.r, s, (const double *)w.Mk, w.xa, ce);
-    if (hipError_t err = run_chain(c, ce, REPAIR_ROUNDS, n_ub, st, true)) return err;
+    if (hipError_t err = run_chain(ce, n_ub, st, true)) return err;
     hipLaunchKernelGGL(phi_out_kernel, dim3(g1), dim3(256), 0, st, f.r, f.phi, s, P, (const double *)w.ga);
     return hipGetLastError();
 }
@@ -1186,8 +968,6 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
     if (count < 0 || count > MPC_MAX) return hipErrorInvalidValue;
     int64_t ntiles = (n_ub + CTILE - 1) / CTILE + 1;
     if (ntiles > sc.max_tiles) return hipErrorInvalidValue;
-    i128 *tb = (i128 *)sc.tile_sums;
-    const int64_t T = sc.max_tiles;
     EsaArgs e = esa_args(sc, P.chain_k_drop);
     e.x = sc.chain_x; e.g = sc.chain_g; e.sign = 1.0; e.first_bad = sc.chain_fbad + 6;
     e.Lp = sc.chain_len + 3;
@@ -1196,15 +976,12 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
     hipLaunchKernelGGL(mpc_init_kernel, dim3(1), dim3(1), 0, st, v0, mprev0);
     MassPcOut *o = (MassPcOut *)out;
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
-    const int KD = P.chain_k_drop;
-    auto Kof = [&](int k) { return k - KD < 8 ? 8 : k - KD; };
     // chain M
     hipLaunchKernelGGL(mpc_prep_m_kernel, dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
                        sc.chain_x, sc.chain_len);
     EsaArgs em = e;
     em.v0p = v0;
-    ChainArgs cm{sc.chain_x, sc.chain_g, 1.0, v0, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, Kof(100)};
-    if (hipError_t err = run_chain(cm, em, REPAIR_ROUNDS, n_ub, st)) return err;
+    if (hipError_t err = run_chain(em, n_ub, st)) return err;
     hipLaunchKernelGGL(mpc_assign_kernel, dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
                        (const double *)sc.chain_g, pc, count, bh_include, mprev0, itm, minimum_r, o);
     // the four running sums
@@ -1213,9 +990,7 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
                            sc.chain_x, sc.chain_len);
         EsaArgs ev = e;
         ev.v0p = v0 + 1;
-        ChainArgs cv{sc.chain_x, sc.chain_g, 1.0, v0 + 1, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp,
-                     Kof(term < 2 ? 100 : 80)};
-        if (hipError_t err = run_chain(cv, ev, REPAIR_ROUNDS, n_ub, st)) return err;
+        if (hipError_t err = run_chain(ev, n_ub, st)) return err;
         hipLaunchKernelGGL(mpc_take_kernel, dim3(1), dim3(MPC_MAX), 0, st, (const double *)sc.chain_g, o, term, count);
     }
     return hipGetLastError();
@@ -1228,9 +1003,9 @@ size_t mass_percent_out_bytes() { return sizeof(MassPcOut); }
 // half-mass index nave comes from the enclosed-mass chain of comp_mass_percent
 // (star_m[k] / N_STAR, from 0); the six density-weighted sums over i = 1..nave are
 // sequential recurrences solved by the exact chain emulation.  Each sum's terms are
-// first scaled by a power of two that brings nave * max(term) to at most 1: binary
-// rounding is invariant under such a scaling, so the scaled chain rounds exactly as
-// the unscaled one and fits the chain's fixed-point range.
+// first scaled by a power of two that brings nave * max(term) to at most 1, so no
+// partial sum can overflow; binary rounding is invariant under such a scaling, so the
+// scaled chain rounds exactly as the unscaled one.
 constexpr int CEN_TERMS = 6;     // rhoj, rhoj^2, rhoj v^2, rhoj r, (rhoj r)^2, rhoj m madhoc
 enum { CEN_OK = 0, CEN_E_STATE = 1, CEN_E_RANGE = 2 };
 
@@ -1412,24 +1187,19 @@ hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, i
 {
     int64_t ntiles = (n_ub + CTILE - 1) / CTILE + 1;
     if (ntiles > sc.max_tiles) return hipErrorInvalidValue;
-    i128 *tb = (i128 *)sc.tile_sums;
-    const int64_t T = sc.max_tiles;
     EsaArgs e = esa_args(sc, P.chain_k_drop);
     e.x = sc.chain_x; e.g = sc.chain_g; e.sign = 1.0; e.first_bad = sc.chain_fbad + 6;
     e.Lp = sc.chain_len + 3;
     double *v0 = sc.chain_scal + 2;              // both start values 0
     CentralOut *o = (CentralOut *)out;
     const unsigned g1 = (unsigned)((n_ub + 256) / 256);
-    const int KD = P.chain_k_drop;
-    const int K = 100 - KD < 8 ? 8 : 100 - KD;
     hipLaunchKernelGGL(cen_init_kernel, dim3(1), dim3(1), 0, st, o, v0);
     // enclosed mass, star_m[k] / N_STAR from 0
     hipLaunchKernelGGL(mpc_prep_m_kernel, dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
                        sc.chain_x, sc.chain_len);
     EsaArgs em = e;
     em.v0p = v0;
-    ChainArgs cmass{sc.chain_x, sc.chain_g, 1.0, v0, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, K};
-    if (hipError_t err = run_chain(cmass, em, REPAIR_ROUNDS, n_ub, st)) return err;
+    if (hipError_t err = run_chain(em, n_ub, st)) return err;
     hipLaunchKernelGGL(cen_nave_kernel, dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
                        (const double *)sc.chain_g, (int64_t)P.n_star, o, sc.chain_len);
     const unsigned gmax = g1 < 1024u ? g1 : 1024u;
@@ -1439,8 +1209,7 @@ hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, i
         hipLaunchKernelGGL(cen_prep_kernel, dim3(g1), dim3(256), 0, st, f, P.madhoc, t, o, sc.chain_x);
         EsaArgs es = e;
         es.v0p = v0 + 1;
-        ChainArgs cs{sc.chain_x, sc.chain_g, 1.0, v0 + 1, e.first_bad, tb, tb + T, (double *)(tb + 8 * T), e.Lp, K};
-        if (hipError_t err = run_chain(cs, es, REPAIR_ROUNDS, n_ub, st)) return err;
+        if (hipError_t err = run_chain(es, n_ub, st)) return err;
         hipLaunchKernelGGL(cen_take_kernel, dim3(1), dim3(1), 0, st, (const double *)sc.chain_g, o, t);
     }
     hipLaunchKernelGGL(cen_final_kernel, dim3(1), dim3(1), 0, st, o);

@@ -26,7 +26,7 @@ __host__ __device__ constexpr int64_t ridx_tiles(int bits) { return (ridx_entrie
 struct Scratch {
     // potential chains: per-tile int128 sums and prefixes plus a boundary value
     // (48 B per tile), chain increments / iterates / enclosed masses per star
-    double *tile_sums;         // 18 doubles per tile (potential chains)
+    double *tile_sums;         // 6 words per tile (potential chains)
     int64_t max_tiles;
     double *chain_x, *chain_g, *chain_mk;
     double *chain_scal;         // 4 doubles

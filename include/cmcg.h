@@ -286,10 +286,12 @@ typedef struct cmcg_central {
 int cmcg_central_calculate(cmcg_ctx *ctx, cmcg_central *out);
 
 /* Test hooks (no counterpart in the reference; for tests that force rare paths).
- * CMCG_HOOK_CHAIN_K_DROP: coarsen the fixed-point scale of the potential chains by
- * `value` bits (0..92; 0 = normal).  Nearly every element then takes the fallback
- * paths (list overflow, repair round, sequential completion); the potential must
- * still equal the reference's recurrence bit for bit, only slower. */
+ * CMCG_HOOK_CHAIN_K_DROP: widen the error bound with which the exact-chain emulation
+ * (potential, comp_mass_percent, central_calculate) classifies its elements by 2^value
+ * (0..92; 0 = normal).  At 30 or more nearly every element is then treated as
+ * sensitive and takes the fallback paths (ordered compaction, list overflow,
+ * sequential repair); the results must still equal the reference's recurrences bit
+ * for bit, only slower. */
 #define CMCG_HOOK_CHAIN_K_DROP 1
 /* CMCG_HOOK_ORBIT_VARIANT: run the orbit sampler with launch variant `value`
  * (0 .. count-1 of cmcg_orbit_variants), as cmcg_autotune would choose it. */

@@ -4,8 +4,8 @@
 derived values use the same correctly rounded divisions and square roots).
 
 Cases: Plummer 1e5 with a Kroupa spectrum and King W0=7 1e6 (segregated) straight
-from their generators; King 1e6 again with the chains' fixed-point scale coarsened by
-60 bits (test hook), which sends most elements through the fallback paths; the
+from their generators; King 1e6 again with the chains' classification bound widened
+by 2^60 (test hook), which sends most elements through the fallback paths; the
 reference's CI input after 3 steps on both sides.
 """
 import os

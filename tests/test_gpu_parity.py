@@ -109,11 +109,11 @@ def test_gpu_potential_parity(gpu, O, case):
 
 @pytest.mark.parametrize("drop", [30, 60])
 def test_gpu_potential_fallback_paths(gpu, O, drop):
-    """A test hook coarsens the chains' fixed-point scale by `drop` bits, so that most
-    elements are classified sensitive: the raw list overflows (ordered compaction),
-    at drop 60 the compacted list overflows too, the verification fails and the
-    repair round and the sequential completion finish the chains.  The potential
-    must still be the reference's recurrence bit for bit."""
+    """A test hook widens the chains' classification bound by 2^drop, so that most
+    elements are classified sensitive: the appended list overflows (ordered
+    compaction), the compacted list (65536 entries) overflows too, and the
+    sequential repair finishes each chain from the first element after the list.
+    The potential must still be the reference's recurrence bit for bit."""
     cm = gpu
     say("potential fallback paths, scale drop", drop)
     cl = cm.make_plummer(100000, seed=3, kroupa=True)

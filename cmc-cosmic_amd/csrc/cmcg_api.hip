@@ -1012,14 +1012,11 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
         LAUNCH(launch_timestep_rng_bases(c->T, c->s, c->sc, c->stream));
     }
     {
+        // relaxation with energy_conservation1 fused in
         PhaseTimer t(c, "relax");
         LAUNCH(launch_sigma(c->f, c->s, c->kp, nub, c->stream));
-        LAUNCH(launch_relax(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream));
+        LAUNCH(launch_relax(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream, true));
         LAUNCH(launch_after_dynamics(c->s, c->stream));
-    }
-    {
-        PhaseTimer t(c, "ec1");
-        LAUNCH(launch_energy_conservation1(c->f, c->s, c->kp, nub, c->stream));
     }
     {
         PhaseTimer t(c, "orbits");
@@ -1030,8 +1027,9 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
         if (c->harvest) LAUNCH(launch_mlp_harvest(c->f, c->s, c->kp, c->mlp, nub, c->stream));
     }
     {
+        // energy_conservation2 with ComputeIntermediateEnergy's EI fused in
         PhaseTimer t(c, "glue_pre_sort");
-        LAUNCH(launch_energy_conservation2(c->f, c->s, c->kp, nub, c->stream));
+        LAUNCH(launch_energy_conservation2(c->f, c->s, c->kp, nub, c->stream, true));
         if (c->params.tidal_strip)
             LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, graph ? STRIP_GRAPH_PASSES : 2, true,
                                 c->stream));
@@ -1049,10 +1047,6 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
                 LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, 1, false, c->stream));
             }
         }
-    }
-    {
-        PhaseTimer t(c, "ie");
-        LAUNCH(launch_intermediate_energy(c->f, c->s, c->kp, nub, c->stream));
     }
     {
         PhaseTimer t(c, "sort");

@@ -92,6 +92,31 @@ __global__ void ec2_kernel(DevFields f, DevScalars *s, KParams P)
     f.Uoldrnew[i] = pot + phi_s_own(f.r[i], f.m[i], P.inv_nstar, x);
 }
 
+// The step's form of energy_conservation2 followed by ComputeIntermediateEnergy's
+// first loop (cmc_utils.c:335-351): the stripping between them leaves every field
+// that EI reads unchanged for the stars it keeps and sets EI = 0 for the stars it
+// strips, so EI can be formed here, before it, with the same result.
+__global__ void ec2_ie_kernel(DevFields f, DevScalars *s, KParams P)
+{
+    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t n = s->n_max;
+    if (i > n) return;
+    const double x = f.rnew[i];
+    double pot;
+    const int fl = f.flags[i];
+    if (fl & 256) {
+        pot = f.pot_rnew[i];
+    } else {
+        bool bad = false;
+        pot = potential_at(f.r, f.phi, n, x, 1, n + 1, &bad);
+        if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
+        f.pot_rnew[i] = pot;
+        f.flags[i] = fl | 256;
+    }
+    f.Uoldrnew[i] = pot + phi_s_own(f.r[i], f.m[i], P.inv_nstar, x);
+    if (x < 1.0e6) f.EI[i] = sqr(f.vr[i]) + sqr(f.vt[i]) + f.phi[i] - pot;
+}
+
 __global__ void strip_begin_kernel(DevScalars *s, int32_t *done)
 {
     s->Etidal = 0.0;        // cmc_evolution_thr.c:449
@@ -556,9 +581,10 @@ hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const 
     return hipGetLastError();
 }
 hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
-                                       hipStream_t st)
+                                       hipStream_t st, bool with_ie)
 {
-    hipLaunchKernelGGL(ec2_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
+    if (with_ie) hipLaunchKernelGGL(ec2_ie_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
+    else hipLaunchKernelGGL(ec2_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
     return hipGetLastError();
 }
 hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,

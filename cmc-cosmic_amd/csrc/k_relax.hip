@@ -252,6 +252,19 @@ __global__ void sigma_kernel(const double *__restrict__ m, const double *__restr
 
 // Pair pr draws from offset 3 pr; a wave's first pair starts at the per-wave state
 // and lane l jumps 3 l < 2^8 steps through the LDS tables.
+// energy_conservation1 (cmc_utils.c:227-236) for star i from its values after the
+// relaxation: vtold, vrold and Uoldrold = phi + MPI_PHI_S(r, i)
+__device__ __forceinline__ void ec1_star(DevFields &f, const KParams &P, int64_t i, double r, double m, double phi,
+                                         double vr, double vt)
+{
+    f.vtold[i] = vt;
+    f.vrold[i] = vr;
+    f.Uoldrold[i] = phi + phi_s_own(r, m, P.inv_nstar, r);
+}
+
+// EC1: energy_conservation1 fused into the pass (the step's form; the entry point of
+// that name keeps its own kernel).
+template <bool EC1>
 __global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, KParams P,
                                                     const Rng *__restrict__ wave_base,
                                                     const NibbleTables<8> *__restrict__ nib)
@@ -260,6 +273,10 @@ __global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, 
     copy_nibble_tables<8>(nib, lt);
     int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t N = s->n_max;
+    if (EC1 && pr == N / 2 && (N & 1)) {    // the unpaired last star keeps its velocities
+        ec1_star(f, P, N, f.r[N], f.m[N], f.phi[N], f.vr[N], f.vt[N]);
+        return;
+    }
     if (pr >= N / 2) return;            // pairs while si <= N - N%2 - 1 (cmc_dynamics.c:203)
     const int64_t k = 2 * pr + 1, kp = k + 1;
     const double dt = s->Dt;
@@ -312,10 +329,15 @@ __global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, 
     f.vr[k] = nvr_k; f.vt[k] = nvt_k;
     f.vr[kp] = nvr_kp; f.vt[kp] = nvt_kp;
     // set_star_EJ (cmc_dynamics_helper.c:878-882), old phi and r
-    f.E[k] = f.phi[k] + 0.5 * (sqr(nvr_k) + sqr(nvt_k));
-    f.J[k] = f.r[k] * nvt_k;
-    f.E[kp] = f.phi[kp] + 0.5 * (sqr(nvr_kp) + sqr(nvt_kp));
-    f.J[kp] = f.r[kp] * nvt_kp;
+    const double phik = f.phi[k], phikp = f.phi[kp], rk = f.r[k], rkp = f.r[kp];
+    f.E[k] = phik + 0.5 * (sqr(nvr_k) + sqr(nvt_k));
+    f.J[k] = rk * nvt_k;
+    f.E[kp] = phikp + 0.5 * (sqr(nvr_kp) + sqr(nvt_kp));
+    f.J[kp] = rkp * nvt_kp;
+    if (EC1) {
+        ec1_star(f, P, k, rk, mass_k, phik, nvr_k, nvt_k);
+        ec1_star(f, P, kp, rkp, mass_kp, phikp, nvr_kp, nvt_kp);
+    }
 }
 
 int bits_needed(uint64_t x)
@@ -416,12 +438,22 @@ hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st)
 }
 
 hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
-                        const Scratch &sc, int64_t n_ub, hipStream_t st)
+                        const Scratch &sc, int64_t n_ub, hipStream_t st, bool with_ec1)
 {
     int64_t npairs = n_ub / 2;
+    if (with_ec1) {
+        // one thread more for an unpaired last star
+        const int64_t nth = (n_ub + 1) / 2;
+        if (nth <= 0) return hipSuccess;
+        if (npairs > 0)
+            if (hipError_t e = launch_rng_wave_bases(T, sc, 0, npairs, st)) return e;
+        hipLaunchKernelGGL(relax_kernel<true>, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, st, f, s, P,
+                           (const Rng *)sc.rng_wave, (const NibbleTables<8> *)sc.relax_nib);
+        return hipGetLastError();
+    }
     if (npairs <= 0) return hipSuccess;
     if (hipError_t e = launch_rng_wave_bases(T, sc, 0, npairs, st)) return e;
-    hipLaunchKernelGGL(relax_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P,
+    hipLaunchKernelGGL(relax_kernel<false>, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P,
                        (const Rng *)sc.rng_wave, (const NibbleTables<8> *)sc.relax_nib);
     return hipGetLastError();
 }

@@ -107,7 +107,8 @@ hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &
 hipError_t launch_timestep_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st);
 hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st);
 hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
-                        const Scratch &sc, int64_t n_ub, hipStream_t st);
+                        const Scratch &sc, int64_t n_ub, hipStream_t st,
+                        bool with_ec1 = false);
 hipError_t launch_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st);
 hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch &sc, int64_t step, hipStream_t st);
 // substream start of every 64-lane wave: relaxation (phase 0, pairs 64w..) or orbits
@@ -157,7 +158,8 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
 hipError_t launch_step_begin(DevScalars *s, hipStream_t st);
 hipError_t launch_after_dynamics(DevScalars *s, hipStream_t st);
 hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
-hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
+hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st,
+                                       bool with_ie = false);
 hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,
                         int64_t n_ub, int passes, bool begin, hipStream_t st);
 hipError_t launch_strip_tail(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, hipStream_t st);

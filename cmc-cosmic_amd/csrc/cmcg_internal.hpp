@@ -116,6 +116,44 @@ __device__ __forceinline__ Rng rng_jump(const RngTable *__restrict__ T, Rng s, u
     return s;
 }
 
+// Block-cooperative jump: s advanced by 2^(row0 + k) steps for every k < nbits (<= 64)
+// with bit k of `bits` set.  Every thread of the block calls it (blockDim.x >= 128).
+// The block first stages the columns of those matrix powers in LDS (cols: 128 * nbits
+// words), then thread (c, i) = (t / 32, t % 32), t < 128, keeps word c of the state
+// and contributes column i of a power when bit i of the word is set: an XOR over the
+// 32 lanes is the product, and the chain of products never waits on memory.  The
+// four words advance independently.  res: an LDS slot for the result.
+__device__ __forceinline__ Rng rng_jump_block(const RngTable *__restrict__ T, Rng s, uint64_t bits, int row0,
+                                              int nbits, uint32_t *cols, Rng *res)
+{
+    const int t = threadIdx.x;
+    const uint32_t *src = &T->col[row0][0][0];
+    for (int e = t; e < nbits * 128; e += blockDim.x)
+        if ((bits >> (e >> 7)) & 1ull) cols[e] = src[e];
+    __syncthreads();
+    if (t < 128) {
+        const int c = t >> 5, i = t & 31;
+        uint32_t z = c == 0 ? s.z0 : c == 1 ? s.z1 : c == 2 ? s.z2 : s.z3;
+        for (int k = 0; k < nbits; k++) {
+            if (!((bits >> k) & 1ull)) continue;
+            uint32_t v = cols[k * 128 + t] & (0u - ((z >> i) & 1u));
+#pragma unroll
+            for (int d = 16; d > 0; d >>= 1) v ^= __shfl_xor(v, d, 32);
+            z = v;
+        }
+        if (i == 0) {
+            if (c == 0) res->z0 = z;
+            else if (c == 1) res->z1 = z;
+            else if (c == 2) res->z2 = z;
+            else res->z3 = z;
+        }
+    }
+    __syncthreads();
+    const Rng out = *res;
+    __syncthreads();          // cols and res free again
+    return out;
+}
+
 // In-wave jumps through LDS: for the NB matrix powers T^(2^b), b = b0..b0+NB-1, and
 // each state word, tab[b][c][q][v] is the product of the matrix with the nibble v
 // placed at bits 4q..4q+3, so a 32x32 GF(2) product is 8 table lookups.  Lanes of a

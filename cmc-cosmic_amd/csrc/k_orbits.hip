@@ -677,16 +677,27 @@ __global__ void ridx_mark_kernel(const double2 *__restrict__ nodes, DevScalars *
 }
 
 // t[b] = the largest marked index below b in its tile, or the tile's entry value:
-// an exclusive running maximum, in place (16 buckets per thread).
+// an exclusive running maximum, in place.  16 buckets per thread (four 16-byte loads),
+// the threads' maxima scanned by wave shuffles and the four wave totals through LDS.
 __global__ __launch_bounds__(256) void ridx_scan_kernel(uint32_t *__restrict__ t, const uint32_t *__restrict__ tb,
                                                         int bits)
 {
-    __shared__ uint32_t pre[256];
+    __shared__ uint32_t wtot[4];
     const int64_t entries = ridx_entries(bits);
     const int64_t i0 = (int64_t)blockIdx.x * RIDX_TILE + 16 * threadIdx.x;
+    const bool whole = i0 + 16 <= entries;
     uint32_t v[16];
+    if (whole) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(t + i0);
 #pragma unroll
-    for (int q = 0; q < 16; q++) v[q] = i0 + q < entries ? t[i0 + q] : 0u;
+        for (int q = 0; q < 4; q++) {
+            const uint4 x = p[q];
+            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = i0 + q < entries ? t[i0 + q] : 0u;
+    }
     uint32_t m = 0;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -694,19 +705,31 @@ __global__ __launch_bounds__(256) void ridx_scan_kernel(uint32_t *__restrict__ t
         v[q] = m;
         m = e > m ? e : m;
     }
-    pre[threadIdx.x] = m;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {          // inclusive max-scan of the thread totals
-        const uint32_t o = threadIdx.x >= (unsigned)off ? pre[threadIdx.x - off] : 0u;
-        __syncthreads();
-        pre[threadIdx.x] = o > pre[threadIdx.x] ? o : pre[threadIdx.x];
-        __syncthreads();
-    }
-    uint32_t start = tb[blockIdx.x];
-    if (threadIdx.x > 0 && pre[threadIdx.x - 1] > start) start = pre[threadIdx.x - 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = m;                                   // inclusive max-scan over the wave
 #pragma unroll
-    for (int q = 0; q < 16; q++)
-        if (i0 + q < entries) t[i0 + q] = v[q] > start ? v[q] : start;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc = o > inc ? o : inc;
+    }
+    uint32_t ex = __shfl_up(inc, 1, 64);
+    if (lane == 0) ex = 0u;
+    if (lane == 63) wtot[w] = inc;
+    __syncthreads();
+    uint32_t start = tb[blockIdx.x];
+    for (int q = 0; q < w; q++) start = wtot[q] > start ? wtot[q] : start;
+    start = ex > start ? ex : start;
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = v[q] > start ? v[q] : start;
+    if (whole) {
+        uint4 *p = reinterpret_cast<uint4 *>(t + i0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) p[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            if (i0 + q < entries) t[i0 + q] = v[q];
+    }
 }
 
 // The LDS samples of the (r, phi) table for a COARSE-sample launch: node

@@ -28,32 +28,24 @@ __device__ __forceinline__ double calc_n_local(const double *__restrict__ r, int
     return (2.0 * ((double)p)) * 3.0 / (4.0 * CMCG_PI * (cub(r[kmax]) - cub(r[kmin])));
 }
 
+// The step's substream bases: base[0] = seeded state advanced by step * 2^72 (the
+// step's relaxation pairs), base[1] = base[0] + 2^70 (its orbit slots).  A block of
+// RB_T threads (rng_jump_block); the offset's bits 72..127 are the step's bits 0..55.
+constexpr int RB_T = 128;
 __device__ __forceinline__ void rng_bases(const RngTable *__restrict__ T, DevScalars *s, const Rng *__restrict__ seeded,
                                           Rng *__restrict__ base, int64_t step_override)
 {
-    uint64_t step = step_override >= 0 ? (uint64_t)step_override : s->step;
-    // offset step*2^72: bits 72.. of the 128-bit count, i.e. bits 8.. of the high word
-    Rng b = *seeded;
-    uint64_t hi = step << 8;
-    for (int bit = 0; bit < 64; bit++) {
-        if (!((hi >> bit) & 1ull)) continue;
-        int row = 64 + bit;
-        if (row >= 128) break;
-        b.z0 = gf2_matvec(T->col[row][0], b.z0);
-        b.z1 = gf2_matvec(T->col[row][1], b.z1);
-        b.z2 = gf2_matvec(T->col[row][2], b.z2);
-        b.z3 = gf2_matvec(T->col[row][3], b.z3);
-    }
-    base[0] = b;
-    // orbits: + 2^70
-    b.z0 = gf2_matvec(T->col[70][0], b.z0);
-    b.z1 = gf2_matvec(T->col[70][1], b.z1);
-    b.z2 = gf2_matvec(T->col[70][2], b.z2);
-    b.z3 = gf2_matvec(T->col[70][3], b.z3);
-    base[1] = b;
+    __shared__ uint32_t cols[56 * 128];
+    __shared__ Rng res;
+    const uint64_t step = step_override >= 0 ? (uint64_t)step_override : s->step;
+    Rng b = rng_jump_block(T, *seeded, step, 72, 56, cols, &res);
+    if (threadIdx.x == 0) base[0] = b;
+    b = rng_jump_block(T, b, 1ull, 70, 1, cols, &res);
+    if (threadIdx.x == 0) base[1] = b;
 }
-__global__ void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s, const Rng *__restrict__ seeded,
-                                 Rng *__restrict__ base, int64_t step_override)
+__global__ __launch_bounds__(RB_T) void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s,
+                                                         const Rng *__restrict__ seeded, Rng *__restrict__ base,
+                                                         int64_t step_override)
 {
     rng_bases(T, s, seeded, base, step_override);
 }
@@ -70,14 +62,13 @@ __global__ __launch_bounds__(WB_T) void rng_wave_bases_kernel(const RngTable *__
                                                               const NibbleTables<WB_NB> *__restrict__ nib)
 {
     __shared__ NibbleTables<WB_NB> lt;
-    __shared__ Rng bb;
+    __shared__ uint32_t cols[64 * 128];
+    __shared__ Rng res;
     const int t = threadIdx.x;
     const uint64_t b = blockIdx.x;
     copy_nibble_tables<WB_NB>(nib, lt);     // powers 2^6.. (phase 0) or 2^26.. (phase 1), built on the host
-    if (t == 0)
-        bb = phase == 0 ? rng_jump(T, base[0], 49152ull * b, 0, jb1)
-                        : rng_jump(T, base[1], (1ull << 20) + (b << 34), 20, jb1);
-    __syncthreads();
+    const Rng bb = phase == 0 ? rng_jump_block(T, base[0], 49152ull * b, 0, jb1, cols, &res)
+                              : rng_jump_block(T, base[1], 1ull + (b << 14), 20, jb1 - 20, cols, &res);
     const int64_t w = (int64_t)b * WB_T + t;
     if (w >= nwaves) return;
     out[w] = rng_jump_lds<WB_NB>(lt, bb, phase == 0 ? 3u * (uint32_t)t : (uint32_t)t);
@@ -183,11 +174,12 @@ __device__ __forceinline__ void timestep_finalize(DevScalars *s)
     s->TotalTime += Dt;
 }
 __global__ void timestep_finalize_kernel(DevScalars *s) { timestep_finalize(s); }
-// the step's time step, then its RNG substream bases (one thread does both)
-__global__ void timestep_rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s,
-                                          const Rng *__restrict__ seeded, Rng *__restrict__ base)
+// the step's time step (thread 0), then its RNG substream bases
+__global__ __launch_bounds__(RB_T) void timestep_rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s,
+                                                                  const Rng *__restrict__ seeded,
+                                                                  Rng *__restrict__ base)
 {
-    timestep_finalize(s);
+    if (threadIdx.x == 0) timestep_finalize(s);
     rng_bases(T, s, seeded, base, -1);
 }
 
@@ -350,13 +342,13 @@ int bits_needed(uint64_t x)
 
 hipError_t launch_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(rng_bases_kernel, dim3(1), dim3(1), 0, st, T, s, sc.rng_seeded, sc.rng_base, (int64_t)-1);
+    hipLaunchKernelGGL(rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base, (int64_t)-1);
     return hipGetLastError();
 }
 
 hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch &sc, int64_t step, hipStream_t st)
 {
-    hipLaunchKernelGGL(rng_bases_kernel, dim3(1), dim3(1), 0, st, T, s, sc.rng_seeded, sc.rng_base, step);
+    hipLaunchKernelGGL(rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base, step);
     return hipGetLastError();
 }
 
@@ -428,7 +420,7 @@ hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &
 
 hipError_t launch_timestep_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(timestep_rng_bases_kernel, dim3(1), dim3(1), 0, st, T, s, sc.rng_seeded, sc.rng_base);
+    hipLaunchKernelGGL(timestep_rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base);
     return hipGetLastError();
 }
 hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st)

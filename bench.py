@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 
 # algorithmic bytes per live star for each phase (8-byte doubles and ids)
 ALG_BYTES = {
-    "sort": 160.0,        # key + 72 B payload, read once and written once
+    "sort": 128.0,        # key + 56 B payload (m, vr, vt, D, r_peri, r_apo, id), read once and written once
     "potential": 24.0,    # read r, m; write phi
     "relax": 72.0,        # read r, m, phi, vr, vt; write vr, vt, E, J
     "orbits": 96.0,       # read E, J, r, m, vr, vt; write rnew, vrnew, vtnew, r_peri, r_apo, X

@@ -237,6 +237,8 @@ struct SortSpec {
     int64_t **swap_ia = nullptr, **swap_ib = nullptr;
 };
 
+// fa, fb: the live/alternate pointer pair swapped after the sort (null for a scratch
+// array that the sort writes into a fixed destination)
 void add_field(SortSpec &sp, double *src, double *dst, double **fa, double **fb)
 {
     sp.src[sp.nf] = (const uint64_t *)src;
@@ -253,14 +255,14 @@ static void sort_spec(cmcg_ctx *c, bool fused, SortSpec &sp)
     if (fused) {
         // ComputeIntermediateEnergy's transfer (cmc_utils.c:382-390) fused into the gather:
         // r <- rnew (the key), rOld <- r, vr <- vrnew, vt <- vtnew.
+        // The step's own scratch is not carried: set_velocities3 reads rOld through the
+        // permutation and, of vtold, vrold, Uoldrold and Uoldrnew, only their
+        // combination D, which energy_conservation2 formed and the sort moves (8-word
+        // records).
         add_field(sp, f.m, a.m, &f.m, &a.m);
         add_field(sp, f.vrnew, a.vr, &f.vr, &a.vr);
         add_field(sp, f.vtnew, a.vt, &f.vt, &a.vt);
-        add_field(sp, f.r, a.rOld, &f.rOld, &a.rOld);
-        add_field(sp, f.Uoldrold, a.Uoldrold, &f.Uoldrold, &a.Uoldrold);
-        add_field(sp, f.Uoldrnew, a.Uoldrnew, &f.Uoldrnew, &a.Uoldrnew);
-        add_field(sp, f.vrold, a.vrold, &f.vrold, &a.vrold);
-        add_field(sp, f.vtold, a.vtold, &f.vtold, &a.vtold);
+        add_field(sp, c->sc.sv3_dpre, c->sc.sv3_dpost, nullptr, nullptr);
         add_field(sp, f.r_apo, a.r_apo, &f.r_apo, &a.r_apo);
         add_field(sp, f.r_peri, a.r_peri, &f.r_peri, &a.r_peri);
     } else {
@@ -280,7 +282,8 @@ static void sort_spec(cmcg_ctx *c, bool fused, SortSpec &sp)
 static void swap_sorted(cmcg_ctx *c, const SortSpec &sp)
 {
     std::swap(c->f.r, c->alt.r);
-    for (int i = 0; i < sp.nf - 1; i++) std::swap(*sp.swap_a[i], *sp.swap_b[i]);
+    for (int i = 0; i < sp.nf - 1; i++)
+        if (sp.swap_a[i]) std::swap(*sp.swap_a[i], *sp.swap_b[i]);
     std::swap(c->f.id, c->alt.id);
 }
 
@@ -409,7 +412,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)(sc.sort_long_cap = n_cap / (SORT_RUN_SHORT + 1) + 2))) ||
         (e = dalloc(c, &sc.sort_huge, (size_t)(sc.sort_huge_cap = n_cap / (SORT_RUN_LONG + 1) + 2))) ||
         (e = dalloc(c, &sc.sort_nlong, 2)) ||
-        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
+        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.sv3_dpre, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.sv3_dpost, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
         (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) ||
         (e = dalloc(c, (char **)&sc.relax_nib, sizeof(NibbleTables<8>))) ||
         (e = dalloc(c, (char **)&sc.wb_nib, 2 * sizeof(NibbleTables<RNG_WB_POWERS>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
@@ -1029,7 +1032,7 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
     {
         // energy_conservation2 with ComputeIntermediateEnergy's EI fused in
         PhaseTimer t(c, "glue_pre_sort");
-        LAUNCH(launch_energy_conservation2(c->f, c->s, c->kp, nub, c->stream, true));
+        LAUNCH(launch_energy_conservation2(c->f, c->s, c->kp, nub, c->stream, true, c->sc.sv3_dpre));
         if (c->params.tidal_strip)
             LAUNCH(launch_strip(c->f, c->s, c->kp, c->sc, c->sc.strip_done, nub, graph ? STRIP_GRAPH_PASSES : 2, true,
                                 c->stream));
@@ -1064,7 +1067,7 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
     {
         PhaseTimer t(c, "glue_post_sort");
         // the sort above read the old sorted radii from what is now the alternate r
-        LAUNCH(launch_set_velocities3(c->f, c->s, c->kp, c->sc, nub, c->stream, c->alt.r));
+        LAUNCH(launch_set_velocities3(c->f, c->s, c->kp, c->sc, nub, c->stream, c->alt.r, c->sc.sv3_dpost));
         LAUNCH(launch_compute_energy(c->f, c->s, c->kp, c->sc, nub, c->stream));
         hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->s);
     }

@@ -96,7 +96,10 @@ __global__ void ec2_kernel(DevFields f, DevScalars *s, KParams P)
 // first loop (cmc_utils.c:335-351): the stripping between them leaves every field
 // that EI reads unchanged for the stars it keeps and sets EI = 0 for the stars it
 // strips, so EI can be formed here, before it, with the same result.
-__global__ void ec2_ie_kernel(DevFields f, DevScalars *s, KParams P)
+// It also forms D = vold^2 + 2 (1 - q) (Uoldrold - Uoldrnew), the part of
+// set_velocities3's vnew^2 (cmc_utils.c:242-247) that the sort then carries in place of
+// its four inputs: the same operations in the same order, so the same double.
+__global__ void ec2_ie_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ dpre)
 {
     int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
@@ -113,8 +116,13 @@ __global__ void ec2_ie_kernel(DevFields f, DevScalars *s, KParams P)
         f.pot_rnew[i] = pot;
         f.flags[i] = fl | 256;
     }
-    f.Uoldrnew[i] = pot + phi_s_own(f.r[i], f.m[i], P.inv_nstar, x);
+    const double Uoldrnew = pot + phi_s_own(f.r[i], f.m[i], P.inv_nstar, x);
+    f.Uoldrnew[i] = Uoldrnew;
     if (x < 1.0e6) f.EI[i] = sqr(f.vr[i]) + sqr(f.vt[i]) + f.phi[i] - pot;
+    const double q = 0.5;
+    const double vtold = f.vtold[i], vrold = f.vrold[i];
+    const double vold2 = vtold * vtold + vrold * vrold;
+    dpre[i] = vold2 + 2.0 * (1.0 - q) * (f.Uoldrold[i] - Uoldrnew);
 }
 
 __global__ void strip_begin_kernel(DevScalars *s, int32_t *done)
@@ -272,7 +280,7 @@ constexpr int MERGE_T = 256, MERGE_PER = 2, MERGE_BLK = MERGE_T * MERGE_PER, MER
 
 __global__ __launch_bounds__(MERGE_T) void old_node_potential_kernel(const double *__restrict__ r_old,
                                                                      const double2 *__restrict__ nodes,
-                                                                     DevScalars *s, double *__restrict__ u_old)
+                                                                     DevScalars *s, double2 *__restrict__ u_old)
 {
     __shared__ double2 L[MERGE_LDS];
     __shared__ int64_t ends[2];
@@ -328,16 +336,19 @@ __global__ __launch_bounds__(MERGE_T) void old_node_potential_kernel(const doubl
             if (i > n) { dev_set_error(s, CMCG_E_BRACKET, sl); v = 0.0; }
             else v = pi + (pi1 - pi) * (1.0 / ri - 1.0 / x) / (1.0 / ri - 1.0 / ri1);
         }
-        u_old[sl] = v;
+        u_old[sl] = make_double2(v, x);       // with the radius: set_velocities3 reads both at once
     }
 }
 
 constexpr int SV3_COARSE = 1024;      // LDS samples of r for potential(rOld)
 
+// With u_old (the step): potential(rOld) and rOld itself through the sort's permutation,
+// and D (dpost) in place of vtold, vrold, Uoldrold and Uoldrnew.  Without: the
+// reference's fields and a search (the entry point of the reference's name).
 __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
                                                  int64_t *__restrict__ blk_cnt, const double2 *__restrict__ mid,
-                                                 const double2 *__restrict__ nodes, const double *__restrict__ u_old,
-                                                 const uint32_t *__restrict__ perm)
+                                                 const double2 *__restrict__ nodes, const double2 *__restrict__ u_old,
+                                                 const uint32_t *__restrict__ perm, const double *__restrict__ dpost)
 {
     __shared__ int cnt;
     __shared__ double cr[SV3_COARSE + 1];
@@ -354,17 +365,23 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
     __syncthreads();
     if (i <= n) {
         double m = f.m[i], ri = f.r[i];
-        double rO = f.rOld[i];
-        bool bad = false;
-        // potential(rOld): merged values through the sort's permutation when this step
-        // sorted the stars, otherwise a search
-        double Unewrold = (u_old ? u_old[perm[i - 1]] : potential_at_coarse_n(PackedNodes{nodes}, n, rO, 1, n + 1, C, &bad)) +
-                          phi_s_own(ri, m, P.inv_nstar, rO);
-        if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
+        double Unewrold, D;
+        if (u_old) {
+            const double2 ux = u_old[perm[i - 1]];
+            Unewrold = ux.x + phi_s_own(ri, m, P.inv_nstar, ux.y);
+            D = dpost[i];
+        } else {
+            const double rO = f.rOld[i];
+            bool bad = false;
+            Unewrold = potential_at_coarse_n(PackedNodes{nodes}, n, rO, 1, n + 1, C, &bad) +
+                       phi_s_own(ri, m, P.inv_nstar, rO);
+            if (bad) dev_set_error(s, CMCG_E_BRACKET, i);
+            const double vtold = f.vtold[i], vrold = f.vrold[i];
+            const double vold2 = vtold * vtold + vrold * vrold;
+            D = vold2 + 2.0 * (1.0 - q) * (f.Uoldrold[i] - f.Uoldrnew[i]);
+        }
         double Unewrnew = f.phi[i] + phi_s_own(ri, m, P.inv_nstar, ri);
-        double vtold = f.vtold[i], vrold = f.vrold[i];
-        double vold2 = vtold * vtold + vrold * vrold;
-        double vnew2 = vold2 + 2.0 * (1.0 - q) * (f.Uoldrold[i] - f.Uoldrnew[i]) + 2.0 * q * (Unewrold - Unewrnew);
+        double vnew2 = D + 2.0 * q * (Unewrold - Unewrnew);
         double vr = f.vr[i], vt = f.vt[i];
         if (vnew2 <= 0.0) {
             excess_c[i] = 0.5 * (sqr(vr) + sqr(vt) - vnew2) * m;
@@ -581,9 +598,10 @@ hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const 
     return hipGetLastError();
 }
 hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
-                                       hipStream_t st, bool with_ie)
+                                       hipStream_t st, bool with_ie, double *dpre)
 {
-    if (with_ie) hipLaunchKernelGGL(ec2_ie_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
+    if (with_ie && !dpre) return hipErrorInvalidValue;
+    if (with_ie) hipLaunchKernelGGL(ec2_ie_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P, dpre);
     else hipLaunchKernelGGL(ec2_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
     return hipGetLastError();
 }
@@ -621,8 +639,9 @@ hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, h
     return hipGetLastError();
 }
 hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
-                                  int64_t n_ub, hipStream_t st, const double *r_old_sorted)
+                                  int64_t n_ub, hipStream_t st, const double *r_old_sorted, const double *dpost)
 {
+    if ((r_old_sorted != nullptr) != (dpost != nullptr)) return hipErrorInvalidValue;
     int64_t nb = grid_for(n_ub);
     if (nb > sc.max_blocks) return hipErrorInvalidValue;
     unsigned ge = grid_for(n_ub);
@@ -634,7 +653,7 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     }
     hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
                        (const double2 *)sc.mid, (const double2 *)sc.nodes,
-                       r_old_sorted ? (const double *)sc.u_old : nullptr, (const uint32_t *)sc.vals_a);
+                       r_old_sorted ? (const double2 *)sc.u_old : nullptr, (const uint32_t *)sc.vals_a, dpost);
     hipLaunchKernelGGL(sv3_scan_kernel, dim3(1), dim3(1024), 0, st, s, sc.sv3_blk, nb);
     hipLaunchKernelGGL(sv3_compact_kernel, dim3((unsigned)nb), dim3(GT), 0, st, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.sv3_blk, sc.excess_list);

@@ -30,8 +30,9 @@ constexpr uint32_t Z_TAIL = 0xffffffffu;          // z of the SF_INFINITY tail; 
 static_assert(RS_PASSES % 2 == 0, "the sorted words must end in keys_a");
 constexpr int RUN_SHORT = SORT_RUN_SHORT;
 constexpr int RUN_LONG = SORT_RUN_LONG;
-constexpr int AOS_W = SORT_AOS_WORDS;   // words per packed payload record (one 128-B line)
-constexpr int AOS_LD = AOS_W + 1;        // odd LDS stride per record: no bank conflicts
+// Packed payload records: W = 8 words (64 B) when the fields and r fit, else 16 (one
+// 128-B line); an odd LDS stride per record keeps the packing free of bank conflicts.
+constexpr int AOS_MAX = SORT_AOS_WORDS;
 
 __device__ __forceinline__ uint64_t sortable(double x)
 {
@@ -112,6 +113,7 @@ __host__ __device__ __forceinline__ int minmax_parts(int64_t ntiles)
 // Words from slots 1..n plus the four global digit histograms; with `aos` the payload
 // fields of each slot, and r itself after them, are packed into one 128-byte record,
 // so the final gather reads one record per star instead of one cache line per field.
+template <int W>
 __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__restrict__ keysrc, DevScalars *s,
                                                              uint64_t *__restrict__ words,
                                                              uint32_t *__restrict__ digit_totals,
@@ -120,7 +122,8 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
 {
     constexpr int P0 = 8 - RS_PASSES;      // first byte sorted on
     __shared__ uint32_t h[RS_PASSES][256];
-    __shared__ uint64_t recs[RS_THREADS * AOS_LD];     // one chunk of packed records
+    constexpr int LD = W + 1;
+    __shared__ uint64_t recs[RS_THREADS * LD];         // one chunk of packed records
     const int t = threadIdx.x;
     for (int p = 0; p < RS_PASSES; p++) h[p][t] = 0;
     // combine the per-block (min, max) pairs of rs_minmax_kernel
@@ -169,14 +172,14 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
         }
         if (aos && i0 < n) {        // block-uniform: pack the chunk through LDS, write it contiguously
             if (i < n) {
-                for (int f = 0; f < g.nf; f++) recs[AOS_LD * t + f] = g.src[f][i + 1];
-                recs[AOS_LD * t + g.nf] = (uint64_t)__double_as_longlong(keysrc[i + 1]);
+                for (int f = 0; f < g.nf; f++) recs[LD * t + f] = g.src[f][i + 1];
+                recs[LD * t + g.nf] = (uint64_t)__double_as_longlong(keysrc[i + 1]);
             }
             __syncthreads();
             const int64_t cnt = n - i0 < RS_THREADS ? n - i0 : RS_THREADS;
-            uint64_t *dst = aos + (int64_t)AOS_W * i0;
-            // whole 128-byte lines (unused words included): partial-line writes cost more
-            for (int w = t; w < cnt * AOS_W; w += RS_THREADS) dst[w] = recs[AOS_LD * (w / AOS_W) + w % AOS_W];
+            uint64_t *dst = aos + (int64_t)W * i0;
+            // whole records (unused words included): partial writes cost more
+            for (int w = t; w < cnt * W; w += RS_THREADS) dst[w] = recs[LD * (w / W) + w % W];
             __syncthreads();
         }
     }
@@ -558,6 +561,7 @@ __global__ __launch_bounds__(256) void rs_gather_kernel(const uint64_t *__restri
     for (int f = 0; f < g.nf; f++) g.dst[f][i + 1] = g.src[f][src];
 }
 
+template <int W>
 __global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__restrict__ words, DevScalars *s,
                                                             double *__restrict__ r_out, uint32_t *__restrict__ perm,
                                                             const uint64_t *__restrict__ aos, GatherList g)
@@ -571,16 +575,16 @@ __global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__re
     }
     const uint32_t src = word_slot(words[i]);
     perm[i] = src;
-    const ulonglong2 *rec = (const ulonglong2 *)(aos + (int64_t)AOS_W * (src - 1));
-    uint64_t w[AOS_W];
+    const ulonglong2 *rec = (const ulonglong2 *)(aos + (int64_t)W * (src - 1));
+    uint64_t w[W];
 #pragma unroll
-    for (int q = 0; q < AOS_W / 2; q++) {
+    for (int q = 0; q < W / 2; q++) {
         ulonglong2 v = 2 * q <= g.nf ? rec[q] : ulonglong2{0ull, 0ull};    // fields, then r in word nf
         w[2 * q] = v.x;
         w[2 * q + 1] = v.y;
     }
 #pragma unroll
-    for (int f = 0; f < AOS_W; f++) {
+    for (int f = 0; f < W; f++) {
         if (f < g.nf) g.dst[f][i + 1] = w[f];
         if (f == g.nf) r_out[i + 1] = __longlong_as_double((long long)w[f]);
     }
@@ -606,7 +610,8 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     for (int f = 0; f < nf; f++) { g.src[f] = src[f]; g.dst[f] = dst[f]; }
     for (int f = nf; f < RS_MAX_FIELDS; f++) { g.src[f] = nullptr; g.dst[f] = nullptr; }
     // packed records hold the fields and r after them
-    uint64_t *aos = (nf < AOS_W && sc.sort_aos && n_ub <= sc.sort_aos_cap) ? sc.sort_aos : nullptr;
+    uint64_t *aos = (nf < AOS_MAX && sc.sort_aos && n_ub <= sc.sort_aos_cap) ? sc.sort_aos : nullptr;
+    const bool narrow = nf < 8;          // fields and r in 8 words
     int64_t ntiles = (n_ub + RS_TILE - 1) / RS_TILE;
     if (ntiles < 1) ntiles = 1;
     if (ntiles > sc.max_sort_tiles) return hipErrorInvalidValue;
@@ -618,8 +623,12 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
                        mm);
     // an even number of passes: the sorted words end in keys_a
     uint64_t *kin = sc.keys_a, *kout = sc.keys_b;
-    hipLaunchKernelGGL(rs_prep_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
-                       sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm);
+    if (narrow)
+        hipLaunchKernelGGL(rs_prep_kernel<8>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
+                           sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm);
+    else
+        hipLaunchKernelGGL(rs_prep_kernel<AOS_MAX>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
+                           sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm);
     for (int pass = 8 - RS_PASSES; pass < 8; pass++) {
         int shift = 8 * pass;
         hipLaunchKernelGGL(rs_hist_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, s, shift,
@@ -639,8 +648,11 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     hipLaunchKernelGGL(rs_huge_runs_kernel, dim3(16), dim3(LR_T), 0, st, kin, kout, key_src, id_src, s,
                        (const int64_t *)sc.sort_huge, (const int64_t *)sc.sort_nlong, sc.sort_huge_cap);
     hipLaunchKernelGGL(rs_count_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s);
-    if (aos)
-        hipLaunchKernelGGL(rs_gather_aos_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
+    if (aos && narrow)
+        hipLaunchKernelGGL(rs_gather_aos_kernel<8>, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
+                           sc.vals_a, (const uint64_t *)aos, g);
+    else if (aos)
+        hipLaunchKernelGGL(rs_gather_aos_kernel<AOS_MAX>, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
                            sc.vals_a, (const uint64_t *)aos, g);
     else
         hipLaunchKernelGGL(rs_gather_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, key_src, r_dst,

@@ -61,7 +61,8 @@ struct Scratch {
     uint32_t *ridx;             // radial index: nodes below each bucket of r (ridx_nb + 2 entries)
     uint32_t *ridx_tb;          // its value at every RIDX_TILE-th bucket
     int ridx_bits;              // log2 of the bucket count
-    double *u_old;              // new potential at the old sorted radii (set_velocities3)
+    double2 *u_old;             // (new potential, radius) at each old sorted radius (set_velocities3)
+    double *sv3_dpre, *sv3_dpost; // vold^2 + 2(1-q)(Uoldrold - Uoldrnew) per star, before and after the step's sort
     double2 *coarse;            // orbit sampler's (r, phi) samples, copied into LDS by every block
     void *orb_nib;              // orbit sampler's GF(2) nibble tables (constant per context)
     void *relax_nib;            // relaxation's nibble tables, powers 2^0 .. 2^7 (constant per context)
@@ -159,7 +160,7 @@ hipError_t launch_step_begin(DevScalars *s, hipStream_t st);
 hipError_t launch_after_dynamics(DevScalars *s, hipStream_t st);
 hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st);
 hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st,
-                                       bool with_ie = false);
+                                       bool with_ie = false, double *dpre = nullptr);
 hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,
                         int64_t n_ub, int passes, bool begin, hipStream_t st);
 hipError_t launch_strip_tail(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, hipStream_t st);
@@ -167,7 +168,8 @@ hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const K
 hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, hipStream_t st);
 // r_old_sorted: the radii the sort of this step read (old sorted order), or nullptr
 hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
-                                  int64_t n_ub, hipStream_t st, const double *r_old_sorted = nullptr);
+                                  int64_t n_ub, hipStream_t st, const double *r_old_sorted = nullptr,
+                                  const double *dpost = nullptr);
 hipError_t launch_compute_energy(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
                                  int64_t n_ub, hipStream_t st);
 

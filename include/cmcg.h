@@ -209,7 +209,12 @@ int cmcg_set_velocities3(cmcg_ctx *ctx);             /* cmc_utils.c:254-354 */
 int cmcg_compute_energy(cmcg_ctx *ctx);              /* cmc_utils.c:604-653, 2017-2028 */
 /* Initial potential, sigma and energies (cmc.c:151-165). */
 int cmcg_init_cluster(cmcg_ctx *ctx);
-/* One whole Henon step in the order of src/cmc/cmc.c:240-407. */
+/* One whole Henon step in the order of src/cmc/cmc.c:240-407.  Every field the
+ * reference carries from one step to the next (r, m, phi, vr, vt, E, J, id, r_peri,
+ * r_apo) ends in the reference's order and values.  The step's own scratch, which the
+ * next step rewrites before any read (rnew, vrnew, vtnew, X, Y, rOld, Uoldrold,
+ * Uoldrnew, vrold, vtold, EI, sigma and the per-star counters), is not carried
+ * through the step's sort and is left in the order the step began with. */
 int cmcg_step(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
 /* The same step replayed from a HIP graph (captured on first use; the step
  * alternates between two buffer states, so two graphs are kept).  The tidal

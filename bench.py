@@ -29,6 +29,7 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+VALU_ISSUE_PEAK = 256 * 2.4e9      # wave instructions per second: one per cycle per CU
 
 # algorithmic bytes per live star for each phase (8-byte doubles and ids)
 ALG_BYTES = {
@@ -90,6 +91,40 @@ def cpu_baseline(cl, ranks):
                       f"long-double Q), {dt:.2f} s",
             "serial": {"value": v1, "cores": 1, "seconds": dt1,
                        "sample": "the same step on one core (the reference's serial path)"}}
+
+
+def orbit_roofline(d, live):
+    """The dominant kernel against the HBM roofline (algorithmic bytes per launch over
+    its HIP-event launch time), with what actually limits it: the committed counter
+    summary gives its measured L2-miss traffic and its VALU instructions per launch,
+    set here against the same launch time (MI355X: 8 TB/s HBM; one wave instruction
+    per cycle per CU, 256 CUs at 2.4 GHz)."""
+    prof = {}
+    pfile = os.path.join(REPO, "profiles", "orbits_kernel_king4e6_counters.json")
+    if os.path.exists(pfile):
+        with open(pfile) as f:
+            prof = json.load(f)
+    secs = d["ms"] * 1e-3
+    traffic = prof.get("hbm_bytes_per_launch")
+    secondary = {}
+    if traffic:
+        secondary["measured_traffic"] = {"achieved": traffic / secs / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": traffic / secs / 1e9 / HBM_PEAK_GBS,
+                                         "note": "FETCH_SIZE + WRITE_SIZE (L2 misses, MALL hits included), "
+                                                 "copy-calibrated, per launch"}
+    if prof.get("valu_insts_per_launch"):
+        rate = prof["valu_insts_per_launch"] / secs
+        secondary["valu_issue"] = {"achieved": rate / 1e9, "peak": VALU_ISSUE_PEAK / 1e9,
+                                   "unit": "G wave-instructions/s", "frac": rate / VALU_ISSUE_PEAK,
+                                   "fp64_share": prof.get("fp64_insts_per_wave", 0.0) * prof.get("waves_per_launch", 0.0)
+                                   / prof["valu_insts_per_launch"]}
+    return {"kernel": "orbits_kernel (orbit sampling, k_orbits.hip)", "bound": "hbm", "achieved": d["alg_GBs"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+            "alg_bytes_per_launch": ALG_BYTES["orbits"] * live, "avg_launch_ms": d["ms"],
+            "limiter": "VALU issue and dependent-gather latency: the exact FP64 apsides and tries (divisions, "
+                       "square roots) and the node lookups, each a load whose address depends on the previous one; "
+                       "HBM carries a few percent of its peak",
+            "secondary": secondary, "counters": "profiles/orbits_kernel_king4e6_counters.json" if prof else None}
 
 
 def orbit_ceiling(cm, ctx, margin, steps=3):
@@ -301,16 +336,7 @@ def main():
                          "rms_log_r_error": err, "margin_log_r": margin},
                "orbit_phase_ceiling": ceiling}
 
-    dom = "orbits_kernel"
-    d = phases[dom]
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", "orbits_kernel_king4e6_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    roofline = {"kernel": "orbits_kernel (orbit sampling, k_orbits.hip)", "bound": "hbm", "achieved": d["alg_GBs"],
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                "alg_bytes_per_launch": ALG_BYTES["orbits"] * float(np.mean(live)), "avg_launch_ms": d["ms"]}
+    roofline = orbit_roofline(phases["orbits_kernel"], float(np.mean(live)))
     ens = None if args.no_ensemble else ensemble_bench(cm, 10)
     cpu = None
     if not args.no_cpu_baseline:

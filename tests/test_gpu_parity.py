@@ -634,3 +634,57 @@ def test_gpu_host_shim_reference_names(gpu, O):
     for x in [0.0, 1e-3, 0.5, 1.0, 3.0]:
         assert h.potential(x) == pytest.approx(o.potential(x), rel=1e-12)
     h.cmc_hotpath_finalize()
+
+
+STEP_FIELDS = ("r", "m", "phi", "vr", "vt", "E", "J", "id", "r_peri", "r_apo")
+STEP_SCALARS = ("n_max", "Mtotal", "Rtidal", "max_r", "Dt", "TotalTime", "TidalMassLoss", "Eescaped", "Jescaped",
+                "Etidal", "Etot", "Ekin", "Epot")
+
+
+def test_gpu_shim_step_in_reference_order_equals_cmcg_step(gpu):
+    """One step driven through the reference-named entry points in the order of the
+    reference's main loop (src/cmc/cmc.c:268-383: calc_timestep, dynamics_apply,
+    energy_conservation1, new_orbits_calculate, energy_conservation2,
+    tidally_strip_stars, ComputeIntermediateEnergy, qsorts_new, post_sort_comm,
+    calc_potential_new, energy_conservation3, compute_energy_new) leaves the same
+    cluster as cmcg_step, which fuses several of them: every field carried across
+    steps and the scalars, bit for bit."""
+    cm = gpu
+    say("reference-ordered step through the host shim vs cmcg_step")
+    cl = cm.make_king(60000, 7.0, seed=3)
+    h = cm.hostlib
+    errors = []
+    handler_t = C.CFUNCTYPE(None, C.c_int, C.c_int64, C.c_char_p)
+    handler = handler_t(lambda code, idx, where: errors.append((code, idx, where)))
+    h.cmc_hotpath_set_error_handler(handler)
+    p = cm.default_params(cl.n, orbit_r=cl.rtid, seed=7)
+    assert h.cmc_hotpath_init(C.byref(p), cl.n, C.byref(cl.soa()), 0) == 0
+    h.cmc_hotpath_ctx.restype = C.c_void_p
+    shim = cm.Context.__new__(cm.Context)
+    shim.h = C.c_void_p(h.cmc_hotpath_ctx())
+    shim.init_cluster()
+    h.dynamics_apply.argtypes = [C.c_double, C.c_void_p]
+    h.calc_timestep.argtypes = [C.c_void_p]
+    h.calc_timestep(None)
+    h.dynamics_apply(shim.scalars()["Dt"], None)
+    for name in ("energy_conservation1", "new_orbits_calculate", "energy_conservation2", "tidally_strip_stars",
+                 "ComputeIntermediateEnergy", "qsorts_new", "post_sort_comm", "calc_potential_new",
+                 "energy_conservation3", "compute_energy_new"):
+        getattr(h, name).restype = None
+        getattr(h, name)()
+    assert not errors, errors
+    a, sa = shim.download(), shim.scalars()
+    shim.h = C.c_void_p()               # the shim owns that context
+    h.cmc_hotpath_set_error_handler(None)
+    h.cmc_hotpath_finalize()
+    with cm.Context(cl.n, p) as ctx:
+        ctx.upload(cl)
+        ctx.init_cluster()
+        ctx.step()
+        b, sb = ctx.download(), ctx.scalars()
+    n = sb["n_max"]
+    say("stars after the step", n, "shim", sa["n_max"])
+    for k in STEP_SCALARS:
+        assert sa[k] == sb[k], (k, sa[k], sb[k])
+    for k in STEP_FIELDS:
+        assert np.array_equal(a[k][1:n + 1], b[k][1:n + 1]), k

@@ -412,7 +412,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)(sc.sort_long_cap = n_cap / (SORT_RUN_SHORT + 1) + 2))) ||
         (e = dalloc(c, &sc.sort_huge, (size_t)(sc.sort_huge_cap = n_cap / (SORT_RUN_LONG + 1) + 2))) ||
         (e = dalloc(c, &sc.sort_nlong, 2)) ||
-        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.sv3_dpre, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.sv3_dpost, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
+        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.sv3_dpre, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.sv3_dpost, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.dt_part, (size_t)64 * 16)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
         (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) ||
         (e = dalloc(c, (char **)&sc.relax_nib, sizeof(NibbleTables<8>))) ||
         (e = dalloc(c, (char **)&sc.wb_nib, 2 * sizeof(NibbleTables<RNG_WB_POWERS>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||
@@ -1011,13 +1011,12 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
     {
         PhaseTimer t(c, "timestep");
         LAUNCH(launch_step_begin(c->s, c->stream));                 // also starts the time step's minimum
-        LAUNCH(launch_simul_relax(c->f, c->s, c->kp, nub, c->stream, false));
-        LAUNCH(launch_timestep_rng_bases(c->T, c->s, c->sc, c->stream));
+        // calc_sigma_r with simul_relax_new, then Dt and the RNG bases
+        LAUNCH(launch_timestep_sigma(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream));
     }
     {
         // relaxation with energy_conservation1 fused in
         PhaseTimer t(c, "relax");
-        LAUNCH(launch_sigma(c->f, c->s, c->kp, nub, c->stream));
         LAUNCH(launch_relax(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream, true));
         LAUNCH(launch_after_dynamics(c->s, c->stream));
     }

@@ -399,29 +399,56 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
     if (threadIdx.x == 0) blk_cnt[blockIdx.x] = cnt;
 }
 
+// Exclusive scan of the per-block event counts, in place, and their total.  Each thread
+// takes a contiguous chunk, held in registers when it has at most SV3_CHUNK entries (so
+// the write-back waits on no load), the chunk sums are scanned by wave shuffles and the
+// 16 wave totals through LDS.
+constexpr int SV3_CHUNK = 16;
 __global__ __launch_bounds__(1024) void sv3_scan_kernel(DevScalars *s, int64_t *__restrict__ blk_cnt, int64_t nb)
 {
-    __shared__ int64_t lds[1024];
-    const int t = threadIdx.x;
+    __shared__ int64_t wtot[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t chunk = (nb + 1023) / 1024;
     const int64_t b0 = (int64_t)t * chunk;
+    const bool regs = chunk <= SV3_CHUNK;
+    int64_t v[SV3_CHUNK];
     int64_t acc = 0;
-#pragma unroll 8
-    for (int64_t b = b0; b < b0 + chunk && b < nb; b++) acc += blk_cnt[b];
-    lds[t] = acc;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int64_t v = t >= off ? lds[t - off] : 0;
-        __syncthreads();
-        lds[t] += v;
-        __syncthreads();
+    if (regs) {
+#pragma unroll
+        for (int q = 0; q < SV3_CHUNK; q++) {
+            v[q] = q < chunk && b0 + q < nb ? blk_cnt[b0 + q] : 0;
+            acc += v[q];
+        }
+    } else {
+        for (int64_t b = b0; b < b0 + chunk && b < nb; b++) acc += blk_cnt[b];
     }
-    int64_t run = lds[t] - acc;
-    if (t == 1023) s->n_excess = lds[1023];
-    for (int64_t b = b0; b < b0 + chunk && b < nb; b++) {
-        int64_t c = blk_cnt[b];
-        blk_cnt[b] = run;
-        run += c;
+    int64_t inc = acc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) wtot[w] = inc;
+    __syncthreads();
+    int64_t run = inc - acc, tot = 0;
+    for (int q = 0; q < 16; q++) {
+        if (q < w) run += wtot[q];
+        tot += wtot[q];
+    }
+    if (t == 1023) s->n_excess = tot;
+    if (regs) {
+#pragma unroll
+        for (int q = 0; q < SV3_CHUNK; q++)
+            if (q < chunk && b0 + q < nb) {
+                blk_cnt[b0 + q] = run;
+                run += v[q];
+            }
+    } else {
+        for (int64_t b = b0; b < b0 + chunk && b < nb; b++) {
+            const int64_t c = blk_cnt[b];
+            blk_cnt[b] = run;
+            run += c;
+        }
     }
 }
 
@@ -646,7 +673,9 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     if (nb > sc.max_blocks) return hipErrorInvalidValue;
     unsigned ge = grid_for(n_ub);
     hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
-    if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
+    // the step calls this right after the potential, which left the node table current
+    if (!dpost)
+        if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     if (r_old_sorted) {
         hipLaunchKernelGGL(old_node_potential_kernel, dim3((unsigned)((n_ub + MERGE_BLK - 1) / MERGE_BLK)),
                            dim3(MERGE_T), 0, st, r_old_sorted, (const double2 *)sc.nodes, s, sc.u_old);

@@ -656,21 +656,33 @@ __global__ void ridx_clear_kernel(uint32_t *__restrict__ t, int64_t count)
         t[i] = 0;
 }
 
-// Node k (0..n; node 0 stands for "below every bucket"): the last node of each bucket
-// writes its index at that bucket, and every multiple of RIDX_TILE in
-// (bucket(r_k), bucket(r_{k+1})] gets k in the tile table.
-__global__ void ridx_mark_kernel(const double2 *__restrict__ nodes, DevScalars *s, uint32_t *__restrict__ t,
-                                 uint32_t *__restrict__ tb, int bits)
+// The packed (r, phi) node table with its every MID_STEP-th entry (nodes 0..n+1), and
+// the radial index's marks (nodes 0..n), in one pass over r and phi: node k (node 0
+// stands for "below every bucket") writes its index at its bucket when it is the
+// bucket's last node, and k at every multiple of RIDX_TILE in (bucket(r_k),
+// bucket(r_{k+1})] of the tile table.  The index array must be cleared before.
+__global__ void node_index_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
+                                  double2 *__restrict__ nodes, double2 *__restrict__ mid, uint32_t *__restrict__ t,
+                                  uint32_t *__restrict__ tb, int bits)
 {
     const int64_t n = s->n_max;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n + 1) return;
+    const double rk = r[k];
+    const double2 v = make_double2(rk, phi[k]);
+    nodes[k] = v;
+    if (k % MID_STEP == 0) mid[k / MID_STEP] = v;
     if (k > n) return;
     const int64_t nb = (int64_t)1 << bits;
     int64_t bk = -1, bn = nb + 1;
     if (n >= 1) {
-        const RIdx x = ridx_of(t, nodes, n, bits);
-        if (k >= 1) bk = ridx_bucket(x, nodes[k].x);
-        if (k < n) bn = ridx_bucket(x, nodes[k + 1].x);
+        RIdx x;
+        x.t = t;
+        x.nb = 1u << bits;
+        x.lo = (uint64_t)__double_as_longlong(r[1]);
+        x.shift = ridx_shift(x.lo, (uint64_t)__double_as_longlong(r[n]), bits);
+        if (k >= 1) bk = ridx_bucket(x, rk);
+        if (k < n) bn = ridx_bucket(x, r[k + 1]);
         if (k >= 1 && bn != bk) t[bk] = (uint32_t)k;
     }
     for (int64_t m = (bk + RIDX_TILE) / RIDX_TILE; m * RIDX_TILE <= bn; m++) tb[m] = (uint32_t)k;
@@ -788,8 +800,7 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     if (nblocks > sc.max_blocks) return hipErrorInvalidValue;
     int jb1 = 20 + bits_needed((uint64_t)n_ub);
     if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
-    if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
-    if (hipError_t e = launch_ridx_build(s, sc, n_ub, st)) return e;
+    if (hipError_t e = launch_node_index(f, s, sc, n_ub, st)) return e;
     if (V.coarse > ORB_MAX_COARSE) return hipErrorInvalidValue;
     hipLaunchKernelGGL(coarse_build_kernel, dim3((unsigned)((V.coarse + 256) / 256)), dim3(256), 0, st,
                        (const double2 *)sc.nodes, s, V.coarse, sc.coarse);
@@ -828,12 +839,12 @@ hipError_t launch_pred_from_apsides(const DevFields &f, DevScalars *s, const Scr
     return hipGetLastError();
 }
 
-hipError_t launch_ridx_build(DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st)
+hipError_t launch_node_index(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st)
 {
     const int64_t entries = ridx_entries(sc.ridx_bits);
     hipLaunchKernelGGL(ridx_clear_kernel, dim3(512), dim3(256), 0, st, sc.ridx, entries);
-    hipLaunchKernelGGL(ridx_mark_kernel, dim3((unsigned)((n_ub + 1 + 256) / 256)), dim3(256), 0, st,
-                       (const double2 *)sc.nodes, s, sc.ridx, sc.ridx_tb, sc.ridx_bits);
+    hipLaunchKernelGGL(node_index_kernel, dim3((unsigned)((n_ub + 2 + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s,
+                       (double2 *)sc.nodes, (double2 *)sc.mid, sc.ridx, sc.ridx_tb, sc.ridx_bits);
     hipLaunchKernelGGL(ridx_scan_kernel, dim3((unsigned)ridx_tiles(sc.ridx_bits)), dim3(256), 0, st, sc.ridx,
                        (const uint32_t *)sc.ridx_tb, sc.ridx_bits);
     return hipGetLastError();

@@ -737,17 +737,30 @@ __global__ __launch_bounds__(CT) void prep_c_kernel(const double *__restrict__ r
     esa_tile_sums(e, n, acc, lds);
 }
 
+// phi[k] = -g[n-k] (k = 1..n) and phi[0] (cmc_utils.c:729); thread t handles node
+// k = t for k = 0..n+1 and also writes the packed (r, phi) node table and its every
+// MID_STEP-th entry, which the searches of set_velocities3 and the orbit phase read.
 __global__ void phi_out_kernel(const double *__restrict__ r, double *__restrict__ phi, DevScalars *s, KParams P,
-                               const double *__restrict__ g)
+                               const double *__restrict__ g, double2 *__restrict__ nodes, double2 *__restrict__ mid)
 {
-    int64_t k = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t n = s->n_max;
-    if (n == 0 && k == 1) phi[0] = phi[1] + P.cenma_m * P.madhoc / r[1];    // no stars: phi[1] is the sentinel
-    if (k > n) return;
-    double p = -g[n - k];
-    phi[k] = p;
-    if (isnan(p)) dev_set_error(s, CMCG_E_NAN, k);
-    if (k == 1) phi[0] = p + P.cenma_m * P.madhoc / r[1];    // cmc_utils.c:729
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s->n_max;
+    if (k > n + 1) return;
+    double p;
+    if (k == 0) {
+        // no stars: phi[1] is the sentinel 0
+        p = (n == 0 ? 0.0 : -g[n - 1]) + P.cenma_m * P.madhoc / r[1];
+        phi[0] = p;
+    } else if (k <= n) {
+        p = -g[n - k];
+        phi[k] = p;
+        if (isnan(p)) dev_set_error(s, CMCG_E_NAN, k);
+    } else {
+        p = 0.0;                                  // phi[n + 1], set with r[n + 1] by pot_setup
+    }
+    const double2 v = make_double2(r[k], p);
+    nodes[k] = v;
+    if (k % MID_STEP == 0) mid[k / MID_STEP] = v;
 }
 
 __global__ void pot_at_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
@@ -946,7 +959,8 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2;
     hipLaunchKernelGGL(prep_c_kernel, gt, dim3(CT), 0, st, f.r, s, (const double *)w.Mk, w.xa, ce);
     if (hipError_t err = run_chain(ce, n_ub, st, true)) return err;
-    hipLaunchKernelGGL(phi_out_kernel, dim3(g1), dim3(256), 0, st, f.r, f.phi, s, P, (const double *)w.ga);
+    hipLaunchKernelGGL(phi_out_kernel, dim3((unsigned)((n_ub + 2 + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s, P,
+                       (const double *)w.ga, (double2 *)sc.nodes, (double2 *)sc.mid);
     return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@ __device__ __forceinline__ double calc_n_local(const double *__restrict__ r, int
 // step's relaxation pairs), base[1] = base[0] + 2^70 (its orbit slots).  A block of
 // RB_T threads (rng_jump_block); the offset's bits 72..127 are the step's bits 0..55.
 constexpr int RB_T = 128;
+constexpr int DT_SLOTS = 64;     // the step's time-step minimum: slots of sigma_dt_kernel (<= RB_T)
 __device__ __forceinline__ void rng_bases(const RngTable *__restrict__ T, DevScalars *s, const Rng *__restrict__ seeded,
                                           Rng *__restrict__ base, int64_t step_override)
 {
@@ -174,11 +175,34 @@ __device__ __forceinline__ void timestep_finalize(DevScalars *s)
     s->TotalTime += Dt;
 }
 __global__ void timestep_finalize_kernel(DevScalars *s) { timestep_finalize(s); }
-// the step's time step (thread 0), then its RNG substream bases
+// the step's time step (thread 0; first the minimum over sigma_dt_kernel's slots, which
+// are reset for the next step, when dt_part is given), then its RNG substream bases
 __global__ __launch_bounds__(RB_T) void timestep_rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s,
                                                                   const Rng *__restrict__ seeded,
-                                                                  Rng *__restrict__ base)
+                                                                  Rng *__restrict__ base,
+                                                                  unsigned long long *__restrict__ dt_part)
 {
+    __shared__ unsigned long long wmin[RB_T / 64];
+    if (dt_part) {
+        unsigned long long mine = 0x7ff0000000000000ull;
+        if (threadIdx.x < DT_SLOTS) {
+            const unsigned long long c = ~dt_part[threadIdx.x * 16];
+            dt_part[threadIdx.x * 16] = 0ull;
+            mine = c < mine ? c : mine;
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const unsigned long long other = __shfl_xor(mine, o, 64);
+            mine = other < mine ? other : mine;
+        }
+        if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = mine;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 1; q < RB_T / 64; q++) mine = wmin[q] < mine ? wmin[q] : mine;
+            const unsigned long long cur = (unsigned long long)__double_as_longlong(s->dtrel);
+            s->dtrel = __longlong_as_double((long long)(mine < cur ? mine : cur));
+        }
+    }
     if (threadIdx.x == 0) timestep_finalize(s);
     rng_bases(T, s, seeded, base, -1);
 }
@@ -240,6 +264,87 @@ __global__ void sigma_kernel(const double *__restrict__ m, const double *__restr
         Mave += mk;
     }
     sigma[si] = sqrt(Mv2ave / Mave);
+}
+
+// calc_sigma_r and simul_relax_new in one pass (the step's form): each block stages
+// m*madhoc and m*madhoc*(vr^2 + vt^2) for its SIG_T stars and 2p on either side (the
+// roundings of both references' loops, as sigma_lds_kernel and simul_relax_kernel
+// take them), writes sigma of its stars, evaluates the relaxation-time windows that
+// start among its stars (window w = [1 + 2pw, 1 + 2p(w+1)), the sums in the
+// reference's order) and folds the block's smallest dt into one of DT_SLOTS slots
+// (128 bytes apart) as an atomicMax of the complemented bit pattern (non-negative
+// doubles order like their patterns; 0 is the identity, so the slots start and are
+// left at 0); the one-block time-step kernel takes the minimum over the slots.  Needs
+// p <= SIG_MAXP.
+__global__ __launch_bounds__(SIG_T) void sigma_dt_kernel(const double *__restrict__ r, const double *__restrict__ m,
+                                                         const double *__restrict__ vr, const double *__restrict__ vt,
+                                                         double *__restrict__ sigma, DevScalars *s, KParams P,
+                                                         unsigned long long *__restrict__ dt_part)
+{
+    __shared__ double a[SIG_LDS], b[SIG_LDS];
+    __shared__ unsigned long long wmin[SIG_T / 64];
+    const unsigned long long kInf = 0x7ff0000000000000ull;
+    const int64_t N = s->n_max, p = P.avekernel, len = 2 * p;
+    const int64_t first = 1 + (int64_t)blockIdx.x * SIG_T;
+    const int64_t last = first + SIG_T - 1 < N ? first + SIG_T - 1 : N;
+    const int64_t lo = first - 2 * p > 1 ? first - 2 * p : 1;
+    const int64_t hi = last + 2 * p < N ? last + 2 * p : N;
+    if (first <= N)
+        for (int64_t k = lo + threadIdx.x; k <= hi; k += SIG_T) {
+            const double mk = m[k] * P.madhoc;
+            a[k - lo] = mk;
+            b[k - lo] = mk * (sqr(vr[k]) + sqr(vt[k]));
+        }
+    __syncthreads();
+    unsigned long long mine = kInf;
+    if (first <= N) {
+        const int64_t si = first + threadIdx.x;
+        if (si <= N) {
+            int64_t simin = si - p, simax = simin + (2 * p - 1);
+            if (simin < 1) { simin = 1; simax = simin + (2 * p - 1); }
+            else if (simax > N) { simax = N; simin = simax - (2 * p - 1); }
+            if (simin < 1) simin = 1;
+            if (simax > N) simax = N;
+            double Mv2ave = 0.0, Mave = 0.0;
+            for (int64_t k = simin; k <= simax; k++) {
+                Mv2ave += b[k - lo];
+                Mave += a[k - lo];
+            }
+            sigma[si] = sqrt(Mv2ave / Mave);
+        }
+        // the windows starting at 1 + 2pw in [first, last]; each lies inside [lo, hi]
+        const int64_t w = (first - 1 + len - 1) / len + threadIdx.x;
+        const int64_t wstart = 1 + w * len, wsi = wstart + p;
+        if (wstart <= last && wsi < N - p) {
+            double Mv2ave = 0.0, Mave = 0.0, M2ave = 0.0;
+            for (int64_t k = wstart; k < wstart + len; k++) {
+                const double tmp = a[k - lo];
+                Mv2ave += b[k - lo];
+                Mave += tmp;
+                M2ave += sqr(tmp);
+            }
+            Mv2ave /= (double)(2 * p);
+            Mave /= (double)(2 * p);
+            M2ave /= (double)(2 * p);
+            const double sg = sqrt(Mv2ave / Mave);
+            const double W = 4.0 * sg / sqrt(3.0 * CMCG_PI);
+            const double n_local = calc_n_local(r, wsi, p, N);
+            const double dt = sqr(2.0 * P.theta_se_max / CMCG_PI) * (CMCG_PI / 32.0) * cub(W) /
+                              (((double)P.n_star) * n_local * (4.0 * M2ave));
+            if (dt >= 0.0) mine = (unsigned long long)__double_as_longlong(dt);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long other = __shfl_xor(mine, o, 64);
+        mine = other < mine ? other : mine;
+    }
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < SIG_T / 64; q++) mine = wmin[q] < mine ? wmin[q] : mine;
+        if (mine != kInf) atomicMax(&dt_part[(blockIdx.x % DT_SLOTS) * 16], ~mine);
+    }
 }
 
 // Pair pr draws from offset 3 pr; a wave's first pair starts at the per-wave state
@@ -420,7 +525,27 @@ hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &
 
 hipError_t launch_timestep_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(timestep_rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base);
+    hipLaunchKernelGGL(timestep_rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base,
+                       (unsigned long long *)nullptr);
+    return hipGetLastError();
+}
+
+// The step's time-step phase: calc_sigma_r and simul_relax_new in one pass (or, for a
+// window longer than the pass stages, their own kernels), then Dt and the RNG bases.
+hipError_t launch_timestep_sigma(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
+                                 const Scratch &sc, int64_t n_ub, hipStream_t st)
+{
+    if (P.avekernel > SIG_MAXP || n_ub <= 0) {
+        if (hipError_t e = launch_simul_relax(f, s, P, n_ub, st, false)) return e;
+        if (hipError_t e = launch_sigma(f, s, P, n_ub, st)) return e;
+        return launch_timestep_rng_bases(T, s, sc, st);
+    }
+    const int64_t nb = (n_ub + SIG_T - 1) / SIG_T;
+    if (nb > sc.max_blocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(sigma_dt_kernel, dim3((unsigned)nb), dim3(SIG_T), 0, st, f.r, f.m, f.vr, f.vt, f.sigma, s, P,
+                       sc.dt_part);
+    hipLaunchKernelGGL(timestep_rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base,
+                       sc.dt_part);
     return hipGetLastError();
 }
 hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st)

@@ -63,6 +63,7 @@ struct Scratch {
     int ridx_bits;              // log2 of the bucket count
     double2 *u_old;             // (new potential, radius) at each old sorted radius (set_velocities3)
     double *sv3_dpre, *sv3_dpost; // vold^2 + 2(1-q)(Uoldrold - Uoldrnew) per star, before and after the step's sort
+    unsigned long long *dt_part;  // the step's time step: 64 slots, 128 bytes apart, of complemented dt minima
     double2 *coarse;            // orbit sampler's (r, phi) samples, copied into LDS by every block
     void *orb_nib;              // orbit sampler's GF(2) nibble tables (constant per context)
     void *relax_nib;            // relaxation's nibble tables, powers 2^0 .. 2^7 (constant per context)
@@ -106,6 +107,8 @@ hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int
 hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st,
                               bool init_min = true);
 hipError_t launch_timestep_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st);
+hipError_t launch_timestep_sigma(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
+                                 const Scratch &sc, int64_t n_ub, hipStream_t st);
 hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st);
 hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
                         const Scratch &sc, int64_t n_ub, hipStream_t st,
@@ -115,8 +118,8 @@ hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch
 // substream start of every 64-lane wave: relaxation (phase 0, pairs 64w..) or orbits
 // (phase 1, slots 1 + 64w..), for `count` pairs / slots
 hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase, int64_t count, hipStream_t st);
-// radial index of the current sorted radii (after launch_mid_build, from the packed nodes)
-hipError_t launch_ridx_build(DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st);
+// the packed node table and the radial index in one pass over r and phi
+hipError_t launch_node_index(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st);
 // middle level of the (r, phi) search tables, from the current arrays
 hipError_t launch_mid_build(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st);
 hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64_t first, int64_t count,

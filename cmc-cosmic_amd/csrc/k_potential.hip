@@ -57,7 +57,6 @@ struct EsaArgs {
     double *tsumP, *tpreP;      // estimated sums of s*x and their exclusive prefixes
     int64_t *tcnt, *ttail;      // sensitive count; regular sum after the tile's last sensitive element
     int64_t *toff, *tcarry;     // sensitive elements before the tile; regular sum of the segment open at its start
-    uint32_t *arrive;           // [1]: blocks of the ordered compaction arrived (zero between launches)
     // the sensitive elements in chain order
     int64_t *s_idx;
     double *s_pre;              // regular sum between the previous sensitive element and this one
@@ -230,14 +229,15 @@ __device__ __forceinline__ void store_tile(double *__restrict__ g, int64_t base,
     }
 }
 
-// Classification of one tile (items past the end are regular with R = 0).  Needs
-// tpreP.  Every kernel classifies through this function on the same inputs, so they
-// agree element for element; the binade at a tile start is taken from the same
-// expression in both tiles that meet there.  Nearly always a thread's whole range of
+// Classification of one tile (items past the end are regular with R = 0), from the
+// estimated prefixes before the tile and before the next one.  Every kernel
+// classifies through this function on the same inputs, so they agree element for
+// element; the binade at a tile start is taken from the same expression in both tiles
+// that meet there.  Nearly always a thread's whole range of
 // values lies in one binade, checked once with the bound at its last element: then
 // only the ties remain to be found.
 __device__ void tile_classify(const EsaArgs &a, int64_t tile, int64_t L, double v0, TileItems &it, double *lds,
-                              double *tbuf)
+                              double *tbuf, double pre_t, double pre_t1)
 {
     const int64_t j0 = tile * CTILE + (int64_t)threadIdx.x * CI;
     load_tile(a.x, a.sign, tile * CTILE, L, tbuf, it.sx);
@@ -246,7 +246,7 @@ __device__ void tile_classify(const EsaArgs &a, int64_t tile, int64_t L, double 
     for (int u = 0; u < CI; u++) acc += it.sx[u];
     double tot;
     const double E = block_excl_sum(acc, lds, &tot);
-    const double P0 = v0 + (a.tpreP[tile] + E);
+    const double P0 = v0 + (pre_t + E);
     double P = P0, Pmin = P0, Pmax = P0;
 #pragma unroll
     for (int u = 0; u < CI; u++) {
@@ -258,7 +258,7 @@ __device__ void tile_classify(const EsaArgs &a, int64_t tile, int64_t L, double 
     it.sens = 0;
     const int64_t jl = j0 + CI < L ? j0 + CI : L;           // one past the thread's last element
     const bool edge = jl < L && jl % CTILE == 0;             // the next element opens the next tile
-    const double Pn = edge ? v0 + (a.tpreP[tile + 1] + 0.0) : 0.0;
+    const double Pn = edge ? v0 + (pre_t1 + 0.0) : 0.0;
     // fast path: every value before and after the thread's elements in one binade
     int bt = INT32_MIN;
     if (j0 < L) {
@@ -369,7 +369,35 @@ __global__ __launch_bounds__(CT) void esa_sum_kernel(EsaArgs a)
     esa_tile_sums(a, L, acc, lds);
 }
 
-__global__ __launch_bounds__(CT) void esa_classify_kernel(EsaArgs a)
+// The estimated prefixes of tile t and t + 1 (the sums of tsumP before them), by one
+// block: thread i sums the tiles i, i + CT, ... in order, and the threads' sums are
+// reduced by a fixed tree.  Tile t + 1's own block forms the same sums term for term
+// (its thread t mod CT ends with tile t), so both blocks get the same bits.
+constexpr int64_t SELF_PREFIX_MAX = 16384;    // tiles; beyond, a one-block scan forms them
+__device__ __forceinline__ double tree_sum(double v, double *lds)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double r = lds[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) r += lds[q];
+    __syncthreads();
+    return r;
+}
+__device__ void tile_prefixes(const double *__restrict__ tsumP, int64_t t, double *lds, double *pre_t, double *pre_t1)
+{
+    double p = 0.0;
+    for (int64_t q = threadIdx.x; q < t; q += CT) p += tsumP[q];
+    const double p1 = (int64_t)threadIdx.x == t % CT ? p + tsumP[t] : p;
+    *pre_t = tree_sum(p, lds);
+    *pre_t1 = tree_sum(p1, lds);
+}
+
+// self_prefix: form the tile prefixes here (and store this tile's in tpreP for the
+// later passes); otherwise read them from tpreP (esa_tile_scan_kernel).
+__global__ __launch_bounds__(CT) void esa_classify_kernel(EsaArgs a, int self_prefix)
 {
     __shared__ double ldsd[CT / 64];
     __shared__ Seg ldss[CT / 64];
@@ -377,8 +405,16 @@ __global__ __launch_bounds__(CT) void esa_classify_kernel(EsaArgs a)
     const int64_t L = *a.Lp;
     const int64_t tile = blockIdx.x;
     if (tile * CTILE >= L) return;
+    double pre_t, pre_t1;
+    if (self_prefix) {
+        tile_prefixes(a.tsumP, tile, ldsd, &pre_t, &pre_t1);
+        if (threadIdx.x == 0) a.tpreP[tile] = pre_t;
+    } else {
+        pre_t = a.tpreP[tile];
+        pre_t1 = a.tpreP[tile + 1];
+    }
     TileItems it;
-    tile_classify(a, tile, L, *a.v0p, it, ldsd, tbuf);
+    tile_classify(a, tile, L, *a.v0p, it, ldsd, tbuf, pre_t, pre_t1);
     Seg tot;
     const Seg pre = block_excl_seg(items_seg(it), ldss, &tot);
     if (threadIdx.x == 0) {
@@ -457,17 +493,20 @@ __device__ void esa_walk_block(const EsaArgs &a, int64_t ns, const double *lpre,
 }
 
 // One block: the tiles' sensitive offsets and open-segment carries (a segmented scan
-// of the tile aggregates), then, when the classification's list holds every
-// sensitive element, its order by index and the walk.
-__global__ __launch_bounds__(WALK_T) void esa_walk_kernel(EsaArgs a)
+// of the tile aggregates), then the ordered list of the sensitive elements and the
+// walk.  The list is the classification's appended one put in order by index when it
+// holds every sensitive element; otherwise (more than RAW_CAP, never for an ordinary
+// cluster) this block classifies the tiles again one after another and lists them in
+// order, slowly but exactly.
+__global__ __launch_bounds__(CT) void esa_walk_kernel(EsaArgs a)
 {
-    __shared__ Seg lds[WALK_T / 64];
+    __shared__ Seg lds[CT / 64];
     __shared__ int64_t ridx[RAW_CAP];
     __shared__ double lpre[RAW_CAP], lsx[RAW_CAP];
     const int t = threadIdx.x;
     const int64_t L = *a.Lp;
     const int64_t nt = (L + CTILE - 1) / CTILE;
-    const int64_t chunk = (nt + WALK_T - 1) / WALK_T;
+    const int64_t chunk = (nt + CT - 1) / CT;
     const int64_t b0 = (int64_t)t * chunk;
     Seg acc{0, 0};
     for (int64_t b = b0; b < b0 + chunk && b < nt; b++) acc = seg_then(acc, Seg{a.ttail[b], a.tcnt[b]});
@@ -480,10 +519,38 @@ __global__ __launch_bounds__(WALK_T) void esa_walk_kernel(EsaArgs a)
     }
     const int64_t ns = tot.cnt;
     if (t == 0) *a.s_n = ns;
-    if (ns > RAW_CAP) return;                       // the ordered compaction takes over
-    for (int64_t e = t; e < ns; e += WALK_T) ridx[e] = a.raw_idx[e];
     __syncthreads();                                // (also makes the block's global writes visible to it)
-    for (int64_t e = t; e < ns; e += WALK_T) {
+    if (ns > RAW_CAP) {
+        __shared__ double ldsd[CT / 64];
+        __shared__ double tbuf[TILE_LDS];
+        for (int64_t tile = 0; tile < nt; tile++) {
+            TileItems it;
+            tile_classify(a, tile, L, *a.v0p, it, ldsd, tbuf, a.tpreP[tile], a.tpreP[tile + 1]);
+            Seg ttot;
+            const Seg pre = block_excl_seg(items_seg(it), lds, &ttot);
+            int64_t k = a.toff[tile] + pre.cnt;
+            int64_t cur = pre.sum + (pre.cnt ? 0 : a.tcarry[tile]);
+            const int64_t j0 = tile * CTILE + (int64_t)t * CI;
+            for (int u = 0; u < CI; u++) {
+                if ((it.sens >> u) & 1) {
+                    if (k < a.smax) {
+                        a.s_idx[k] = j0 + u;
+                        a.s_pre[k] = seg_value(cur, it.bp[u]);
+                    }
+                    k++;
+                    cur = 0;
+                } else {
+                    cur += it.R[u];
+                }
+            }
+            __syncthreads();
+        }
+        esa_walk_block(a, ns, nullptr, nullptr);
+        return;
+    }
+    for (int64_t e = t; e < ns; e += CT) ridx[e] = a.raw_idx[e];
+    __syncthreads();
+    for (int64_t e = t; e < ns; e += CT) {
         const int64_t j = ridx[e];
         int64_t rank = 0;
         for (int64_t q = 0; q < ns; q++) rank += ridx[q] < j;      // the indices are distinct
@@ -495,49 +562,6 @@ __global__ __launch_bounds__(WALK_T) void esa_walk_kernel(EsaArgs a)
     }
     __syncthreads();
     esa_walk_block(a, ns, lpre, lsx);
-}
-
-// Only when more than RAW_CAP elements are sensitive: the ordered list from a second
-// classification, and the walk in the last block to arrive.
-__global__ __launch_bounds__(CT) void esa_compact_kernel(EsaArgs a)
-{
-    __shared__ double ldsd[CT / 64];
-    __shared__ Seg ldss[CT / 64];
-    __shared__ double tbuf[TILE_LDS];
-    __shared__ bool last;
-    const int64_t ns = *a.s_n;
-    if (ns <= RAW_CAP) return;
-    const int64_t L = *a.Lp;
-    const int64_t tile = blockIdx.x;
-    if (tile * CTILE < L) {
-        TileItems it;
-        tile_classify(a, tile, L, *a.v0p, it, ldsd, tbuf);
-        Seg tot;
-        const Seg pre = block_excl_seg(items_seg(it), ldss, &tot);
-        int64_t k = a.toff[tile] + pre.cnt;
-        int64_t cur = pre.sum + (pre.cnt ? 0 : a.tcarry[tile]);
-        const int64_t j0 = tile * CTILE + (int64_t)threadIdx.x * CI;
-        for (int u = 0; u < CI; u++) {
-            if ((it.sens >> u) & 1) {
-                if (k < a.smax) {
-                    a.s_idx[k] = j0 + u;
-                    a.s_pre[k] = seg_value(cur, it.bp[u]);
-                }
-                k++;
-                cur = 0;
-            } else {
-                cur += it.R[u];
-            }
-        }
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&a.arrive[1], 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x == 0) a.arrive[1] = 0;
-    esa_walk_block(a, ns, nullptr, nullptr);
 }
 
 // Parallel rebuild of every element from the last sensitive value before it,
@@ -559,7 +583,7 @@ __global__ __launch_bounds__(CT) void esa_rebuild_kernel(EsaArgs a)
     int64_t ns_eff = *a.s_n;
     if (ns_eff > a.smax) ns_eff = a.smax;
     TileItems it;
-    tile_classify(a, tile, L, v0, it, ldsd, tbuf);
+    tile_classify(a, tile, L, v0, it, ldsd, tbuf, a.tpreP[tile], a.tpreP[tile + 1]);
     Seg tot;
     const Seg pre = block_excl_seg(items_seg(it), ldss, &tot);
     const int64_t toff = a.toff[tile], tcarry = a.tcarry[tile];
@@ -783,10 +807,10 @@ hipError_t run_chain(const EsaArgs &e, int64_t n_ub, hipStream_t st, bool summed
     if (ntiles < 1) ntiles = 1;
     dim3 gt((unsigned)ntiles), bt(CT);
     if (!summed) hipLaunchKernelGGL(esa_sum_kernel, gt, bt, 0, st, e);
-    hipLaunchKernelGGL(esa_tile_scan_kernel, dim3(1), dim3(WALK_T), 0, st, e, ntiles);
-    hipLaunchKernelGGL(esa_classify_kernel, gt, bt, 0, st, e);
-    hipLaunchKernelGGL(esa_walk_kernel, dim3(1), dim3(WALK_T), 0, st, e);
-    hipLaunchKernelGGL(esa_compact_kernel, gt, bt, 0, st, e);
+    const bool self_prefix = ntiles <= SELF_PREFIX_MAX;
+    if (!self_prefix) hipLaunchKernelGGL(esa_tile_scan_kernel, dim3(1), dim3(WALK_T), 0, st, e, ntiles);
+    hipLaunchKernelGGL(esa_classify_kernel, gt, bt, 0, st, e, (int)self_prefix);
+    hipLaunchKernelGGL(esa_walk_kernel, dim3(1), dim3(CT), 0, st, e);
     hipLaunchKernelGGL(esa_rebuild_kernel, gt, bt, 0, st, e);
     hipLaunchKernelGGL(chain_finish_kernel, dim3(1), dim3(WALK_T), 0, st, e);
     return hipGetLastError();
@@ -806,7 +830,6 @@ EsaArgs esa_args(const Scratch &sc, int slack)
     e.ttail = (int64_t *)(tb + 3 * T);
     e.toff = (int64_t *)(tb + 4 * T);
     e.tcarry = (int64_t *)(tb + 5 * T);
-    e.arrive = (uint32_t *)(sc.esa_n + 2);
     e.s_idx = sc.esa_idx;
     e.s_pre = sc.esa_racc;
     e.s_v = sc.esa_v;

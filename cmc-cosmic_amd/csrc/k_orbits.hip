@@ -451,20 +451,44 @@ __global__ __launch_bounds__(THREADS, MINB) void orbits_kernel(DevFields f, DevS
     block_reduce_store<THREADS>(red, partials + RED_STRIDE * xcd_block());
 }
 
-// Deferred stars: order the list (for a deterministic reduction order) and reset
-// the first-positive-index slots.
-__global__ void defer_prepare_kernel(DevScalars *s, int64_t *__restrict__ list, int64_t *__restrict__ kfirst)
+// Deferred stars (one block of DEFER_T threads): order the list by star index, so
+// the reduction of their sums does not depend on the order in which they were
+// appended, and reset the first-positive-index slots.  A short list is sorted by one
+// thread; a longer one by a bitonic sort over the next power of two (the list's
+// capacity MAX_DEFER is one), padded with INT64_MAX, in global memory.
+constexpr int DEFER_T = 1024;
+__global__ __launch_bounds__(DEFER_T) void defer_prepare_kernel(DevScalars *s, int64_t *__restrict__ list,
+                                                                int64_t *__restrict__ kfirst)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
-    // (more than 4096 deferred stars: keep append order; only the order of the
-    // escaped-energy sums depends on it)
-    for (int64_t a = 1; a < (nd <= 4096 ? nd : 0); a++) {
-        int64_t v = list[a], b = a - 1;
-        while (b >= 0 && list[b] > v) { list[b + 1] = list[b]; b--; }
-        list[b + 1] = v;
+    const int t = threadIdx.x;
+    const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
+    if (nd <= 64) {
+        if (t == 0)
+            for (int64_t a = 1; a < nd; a++) {
+                const int64_t v = list[a];
+                int64_t b = a - 1;
+                while (b >= 0 && list[b] > v) { list[b + 1] = list[b]; b--; }
+                list[b + 1] = v;
+            }
+    } else {
+        int64_t P = 1;
+        while (P < nd) P <<= 1;
+        for (int64_t i = nd + t; i < P; i += DEFER_T) list[i] = INT64_MAX;
+        __syncthreads();
+        for (int64_t k = 2; k <= P; k <<= 1)
+            for (int64_t j = k >> 1; j > 0; j >>= 1) {
+                for (int64_t i = t; i < P; i += DEFER_T) {
+                    const int64_t l = i ^ j;
+                    if (l > i) {
+                        const int64_t a = list[i], b = list[l];
+                        if ((a > b) == ((i & k) == 0)) { list[i] = b; list[l] = a; }
+                    }
+                }
+                __syncthreads();
+            }
     }
-    for (int64_t d = 0; d < nd; d++) kfirst[d] = s->n_max + 2;
+    const int64_t kf = s->n_max + 2;
+    for (int64_t d = t; d < nd; d += DEFER_T) kfirst[d] = kf;
 }
 
 // Grid-wide equivalent of the linear scan: the first k in [0, N_MAX+1] with Q > 0.
@@ -817,7 +841,7 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
                        (const double2 *)sc.nodes, (const double2 *)sc.coarse,
                        (const NibbleTables<ORB_NIB_POWERS> *)sc.orb_nib, (const uint32_t *)sc.ridx, sc.ridx_bits);
     if (ev_b) hipEventRecord(ev_b, st);
-    hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(1), 0, st, s, sc.defer_list, sc.defer_kfirst);
+    hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(DEFER_T), 0, st, s, sc.defer_list, sc.defer_kfirst);
     hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
                        sc.defer_kfirst);
     hipLaunchKernelGGL(defer_tail_kernel, dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,

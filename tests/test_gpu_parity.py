@@ -617,6 +617,35 @@ def test_gpu_edge_cases(gpu, O):
     assert ei.value.code == cm.E_CAPACITY
 
 
+def test_gpu_many_linear_scan_stars(gpu, O):
+    """8000 stars with v_r = 0 (Q = 0 at their own radius: about 60% of them take the
+    reference's linear-scan case, cmc_orbit.c:40-56), more than the one-thread sort of
+    the deferred list takes:
+    the list is put in index order by a bitonic sort, so the phase's sums do not depend
+    on the order of appending.  Bit-exact against the oracle, twice in a row."""
+    cm = gpu
+    say("8000 stars with v_r = 0: linear-scan stars")
+    cl = cm.make_plummer(20000, seed=11)
+    rng = np.random.default_rng(11)
+    cl["vr"][1 + rng.choice(20000, 8000, replace=False)] = 0.0
+    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64)
+    o.init()
+    runs = []
+    for _ in range(2):
+        with ctx_for(cm, cl) as ctx:
+            upload_state(ctx, cl, o)
+            st = ctx.orbits(step=0)
+            runs.append((st, ctx.download()))
+    o.orbits()
+    N = o.N
+    say("linear-scan stars gpu", runs[0][0]["n_linscan"], "oracle", o.sc("n_linscan"))
+    assert runs[0][0]["n_linscan"] == o.sc("n_linscan") > 4096
+    for st, d in runs:
+        for k in ("rnew", "vrnew", "vtnew", "r_apo", "r_peri", "X"):
+            assert np.array_equal(d[k][1:N + 1], o[k][1:N + 1]), k
+    assert runs[0][0] == runs[1][0]
+
+
 def test_gpu_host_shim_reference_names(gpu, O):
     """The reference-named C entry points drive the same kernels."""
     cm = gpu

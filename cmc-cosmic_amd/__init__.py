@@ -160,6 +160,8 @@ _sigs = {
     "cmcg_orbit_variants": [_ctx, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32),
                             _P(C.c_double)],
     "cmcg_ensemble_step": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
+    "cmcg_ensemble_step_concurrent": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
+    "cmcg_ensemble_stats": [_P(C.c_int64), _P(C.c_int64)],
     "cmcg_rng_draws": [_ctx, C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int, _P(C.c_double)],
     "cmcg_timer_enable": [_ctx, C.c_int],
     "cmcg_timer_get": [_ctx, C.c_char_p, _P(C.c_double), _P(C.c_int64)],
@@ -645,13 +647,22 @@ class Context:
         _chk(lib.cmcg_set_learned_brackets(self.h, int(on), float(margin)), "cmcg_set_learned_brackets")
 
 
-def ensemble_step(ctxs, check=True):
-    """One step of every context's cluster, all launched before any wait."""
+def ensemble_step(ctxs, check=True, concurrent=False):
+    """One step of every context's cluster: merged into one segmented launch per
+    kernel (default), or each cluster's step graph on its own stream (concurrent)."""
     n = len(ctxs)
     arr = (_ctx * n)(*[c.h for c in ctxs])
     stats = (OrbitStats * n)()
     errs = (C.c_int32 * n)()
-    rc = lib.cmcg_ensemble_step(arr, n, stats, errs)
+    name = "cmcg_ensemble_step_concurrent" if concurrent else "cmcg_ensemble_step"
+    rc = getattr(lib, name)(arr, n, stats, errs)
     if check:
-        _chk(rc, "cmcg_ensemble_step")
+        _chk(rc, name)
     return [stats[i].as_dict() for i in range(n)], [int(errs[i]) for i in range(n)]
+
+
+def ensemble_stats():
+    """(merged launches, recorded launches) of the last merged ensemble step."""
+    a, b = C.c_int64(), C.c_int64()
+    _chk(lib.cmcg_ensemble_stats(C.byref(a), C.byref(b)), "cmcg_ensemble_stats")
+    return int(a.value), int(b.value)

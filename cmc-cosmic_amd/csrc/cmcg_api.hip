@@ -4,7 +4,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -46,6 +49,8 @@ struct cmcg_ctx {
     // two pointer states; graph[p] is the step captured in state p
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int parity = 0;
+    uint64_t serial = 0;         // unique per context (ensemble plans name their clusters by it)
+    uint64_t gen = 0;            // bumped whenever the graphs are dropped
     int orb_variant = 0;         // launch shape of the orbit sampler (autotuned)
     std::vector<double> tune_ms; // per-variant time of the last cmcg_autotune
     void *mpc_buf = nullptr;     // cmcg_comp_mass_percent results (allocated on first use)
@@ -73,6 +78,7 @@ static void invalidate_graphs(cmcg_ctx *c)
     for (auto &g : c->graph)
         if (g) { hipGraphExecDestroy(g); g = nullptr; }
     c->parity = 0;
+    c->gen++;
 }
 
 namespace {
@@ -220,12 +226,15 @@ void collect_timers(cmcg_ctx *c)
     c->pending.clear();
 }
 
-__global__ void set_dt_step_kernel(DevScalars *s, double dt, int64_t step)
+struct set_dt_step_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, double dt, int64_t step)
 {
     if (dt == dt) s->Dt = dt;
     if (step >= 0) s->step = (uint64_t)step;
 }
-__global__ void step_end_kernel(DevScalars *s) { s->step += 1; }
+};
+struct step_end_kernel { static constexpr int LB = 1024, MB = 0; static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s) { s->step += 1; } };
 
 // payload field lists for the sort
 struct SortSpec {
@@ -316,6 +325,10 @@ int read_scalars(cmcg_ctx *c, DevScalars &h)
 
 } // namespace
 
+thread_local SegRecorder *g_seg_rec = nullptr;
+
+static void ens_forget(uint64_t serial);
+
 // ======================================================================= C ABI
 extern "C" {
 
@@ -374,7 +387,9 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     if (!out || !params || n_cap < 1 || params->n_star < 1 || params->avekernel < 1) return CMCG_E_ARG;
     if (n_cap > (int64_t)0x7fffffff) return CMCG_E_CAPACITY;
     *out = nullptr;
+    static std::atomic<uint64_t> serials{0};
     cmcg_ctx *c = new cmcg_ctx();
+    c->serial = ++serials;
     c->device = device;
     c->n_cap = n_cap;
     c->params = *params;
@@ -464,6 +479,8 @@ int cmcg_destroy(cmcg_ctx *c)
     if (!c) return CMCG_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    ens_forget(c->serial);
+    hipSetDevice(c->device);
     invalidate_graphs(c);
     for (auto &p : c->pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
     for (auto e : c->event_pool) hipEventDestroy(e);
@@ -775,7 +792,7 @@ int cmcg_relax(cmcg_ctx *c, double dt, uint64_t step)
     if (!c) return CMCG_E_ARG;
     {
         PhaseTimer t(c, "relax");
-        hipLaunchKernelGGL(set_dt_step_kernel, dim3(1), dim3(1), 0, c->stream, c->s, dt, (int64_t)step);
+        kl<set_dt_step_kernel>(dim3(1), dim3(1), 0, c->stream, c->s, dt, (int64_t)step);
         LAUNCH(launch_rng_bases(c->T, c->s, c->sc, c->stream));
         LAUNCH(launch_sigma(c->f, c->s, c->kp, c->n_ub, c->stream));
         LAUNCH(launch_relax(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream));
@@ -799,7 +816,7 @@ int cmcg_orbits(cmcg_ctx *c, uint64_t step, cmcg_orbit_stats *stats)
     if (!c) return CMCG_E_ARG;
     {
         PhaseTimer t(c, "orbits");
-        hipLaunchKernelGGL(set_dt_step_kernel, dim3(1), dim3(1), 0, c->stream, c->s, __builtin_nan(""), (int64_t)step);
+        kl<set_dt_step_kernel>(dim3(1), dim3(1), 0, c->stream, c->s, __builtin_nan(""), (int64_t)step);
         LAUNCH(launch_rng_bases(c->T, c->s, c->sc, c->stream));
         hipEvent_t ka = nullptr, kb = nullptr;
         kernel_events(c, "orbits_kernel", ka, kb);
@@ -1068,7 +1085,7 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
         // the sort above read the old sorted radii from what is now the alternate r
         LAUNCH(launch_set_velocities3(c->f, c->s, c->kp, c->sc, nub, c->stream, c->alt.r, c->sc.sv3_dpost));
         LAUNCH(launch_compute_energy(c->f, c->s, c->kp, c->sc, nub, c->stream));
-        hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, c->stream, c->s);
+        kl<step_end_kernel>(dim3(1), dim3(1), 0, c->stream, c->s);
     }
     c->parity ^= 1;
     return CMCG_OK;
@@ -1130,7 +1147,7 @@ int cmcg_step_graph(cmcg_ctx *c, cmcg_orbit_stats *stats)
     return finish_step(c, stats);
 }
 
-int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors)
+int cmcg_ensemble_step_concurrent(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors)
 {
     if (!ctxs || count < 1) return CMCG_E_ARG;
     for (int i = 0; i < count; i++)
@@ -1151,6 +1168,284 @@ int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int3
         if (e && !first_err) first_err = e;
     }
     return first_err;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------ merged ensemble
+// One plan per pointer state of a set of clusters: every cluster's step is recorded
+// (not launched), the recorded launches of all clusters are merged into one
+// segmented launch per kernel, and the merged sequence is captured in one graph.
+namespace {
+
+struct EnsGroup {
+    SegLaunchFn fn;
+    uint32_t blocks, gy;
+    dim3 block;
+    size_t lds;
+    size_t tab_off, arg_off;     // into the plan's device buffer
+};
+
+struct EnsPlan {
+    std::vector<uint64_t> key;   // (serial, gen, parity) per cluster
+    int device = 0;
+    void *dmem = nullptr;        // block tables and argument bytes of every merged launch
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;  // one per cluster: that cluster's stream has reached the step
+    hipEvent_t done = nullptr;
+    int64_t launches = 0, recorded = 0;
+    ~EnsPlan()
+    {
+        hipSetDevice(device);
+        if (exec) hipGraphExecDestroy(exec);
+        for (hipEvent_t e : ev)
+            if (e) hipEventDestroy(e);
+        if (done) hipEventDestroy(done);
+        if (dmem) hipFree(dmem);
+    }
+};
+
+constexpr size_t ENS_MAX_PLANS = 4;     // both pointer states of two ensembles
+std::mutex g_ens_mu;
+std::vector<std::unique_ptr<EnsPlan>> g_ens_plans;   // most recently used last
+int64_t g_ens_last_launches = 0, g_ens_last_recorded = 0;
+
+std::vector<uint64_t> ens_key(cmcg_ctx **ctxs, int count)
+{
+    std::vector<uint64_t> k;
+    k.reserve(3 * (size_t)count);
+    for (int i = 0; i < count; i++) {
+        k.push_back(ctxs[i]->serial);
+        k.push_back(ctxs[i]->gen);
+        k.push_back((uint64_t)ctxs[i]->parity);
+    }
+    return k;
+}
+
+// the host-side pointer swap and parity flip the step leaves behind
+void apply_step_swap(cmcg_ctx *c)
+{
+    SortSpec sp;
+    sort_spec(c, true, sp);
+    swap_sorted(c, sp);
+    c->parity ^= 1;
+}
+
+struct CtxPointers { DevFields f, alt; Scratch sc; int parity; };
+
+// Merges the recorded launches and captures the merged sequence in the plan's graph.
+int ens_assemble(EnsPlan *plan, const std::vector<SegRecorder> &rec, cmcg_ctx **ctxs, int count)
+{
+    // merge: each round takes the kernel that most clusters have next (ties: the one
+    // of the cluster with most launches left) and launches it for all of them
+    std::vector<size_t> cur(count, 0);
+    std::vector<std::vector<std::pair<int, size_t>>> members;   // (cluster, its record) per merged launch
+    auto same = [](const SegRec &a, const SegRec &b) {
+        return a.fn == b.fn && a.gy == b.gy && a.block.x == b.block.x && a.block.y == b.block.y &&
+               a.block.z == b.block.z && a.lds == b.lds && a.args.size() == b.args.size();
+    };
+    for (int i = 0; i < count; i++) plan->recorded += (int64_t)rec[i].recs.size();
+    for (;;) {
+        int best = -1, best_n = 0;
+        size_t best_left = 0;
+        for (int i = 0; i < count; i++) {
+            if (cur[i] >= rec[i].recs.size()) continue;
+            const SegRec &r = rec[i].recs[cur[i]];
+            int n = 0;
+            for (int j = 0; j < count; j++)
+                if (cur[j] < rec[j].recs.size() && same(rec[j].recs[cur[j]], r)) n++;
+            const size_t left = rec[i].recs.size() - cur[i];
+            if (n > best_n || (n == best_n && left > best_left)) { best = i; best_n = n; best_left = left; }
+        }
+        if (best < 0) break;
+        const SegRec &r0 = rec[best].recs[cur[best]];
+        std::vector<std::pair<int, size_t>> m;
+        for (int j = 0; j < count; j++)
+            if (cur[j] < rec[j].recs.size() && same(rec[j].recs[cur[j]], r0)) m.emplace_back(j, cur[j]);
+        for (auto &jm : m) cur[jm.first]++;
+        members.push_back(std::move(m));
+    }
+    // device buffer: per merged launch a block table (each cluster's grid starting at
+    // a multiple of 8 blocks) and the members' argument bytes, each 256-byte aligned
+    std::vector<unsigned char> host;
+    auto reserve = [&](size_t bytes) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        return off;
+    };
+    std::vector<EnsGroup> groups;
+    for (const auto &m : members) {
+        const SegRec &r0 = rec[m[0].first].recs[m[0].second];
+        const size_t asz = r0.args.size();
+        EnsGroup g{r0.fn, 0, r0.gy, r0.block, r0.lds, 0, 0};
+        g.tab_off = reserve(8 * (1 + m.size()));
+        g.arg_off = reserve(asz * m.size());
+        uint64_t off = 0;
+        for (size_t k = 0; k < m.size(); k++) {
+            const SegRec &r = rec[m[k].first].recs[m[k].second];
+            const uint32_t e[2] = {(uint32_t)off, r.gx};
+            memcpy(host.data() + g.tab_off + 8 * (1 + k), e, 8);
+            memcpy(host.data() + g.arg_off + asz * k, r.args.data(), asz);
+            off = (off + r.gx + 7) & ~(uint64_t)7;
+            if (k + 1 == m.size()) off = e[0] + (uint64_t)r.gx;
+        }
+        if (off > 0x7fffffffull) return CMCG_E_ARG;
+        const uint32_t h[2] = {(uint32_t)m.size(), 0u};
+        memcpy(host.data() + g.tab_off, h, 8);
+        g.blocks = (uint32_t)off;
+        groups.push_back(g);
+    }
+    plan->launches = (int64_t)groups.size();
+    hipStream_t st = ctxs[0]->stream;
+    if (hipMalloc(&plan->dmem, host.size() ? host.size() : 256) != hipSuccess) return CMCG_E_NOMEM;
+    if (hipMemcpy(plan->dmem, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) return CMCG_E_HIP;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return CMCG_E_HIP;
+    for (const EnsGroup &g : groups)
+        g.fn(g.blocks, g.gy, g.block, g.lds, st, (const uint2 *)((char *)plan->dmem + g.tab_off),
+             (char *)plan->dmem + g.arg_off);
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (ce != hipSuccess || !graph) {
+        if (graph) hipGraphDestroy(graph);
+        return CMCG_E_HIP;
+    }
+    hipError_t ie = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) { plan->exec = nullptr; return CMCG_E_HIP; }
+    plan->ev.assign(count, nullptr);
+    for (int i = 0; i < count; i++)
+        if (hipEventCreateWithFlags(&plan->ev[i], hipEventDisableTiming) != hipSuccess) return CMCG_E_HIP;
+    if (hipEventCreateWithFlags(&plan->done, hipEventDisableTiming) != hipSuccess) return CMCG_E_HIP;
+    return CMCG_OK;
+}
+
+
+// Records each cluster's step and builds its plan.  On success the clusters are left
+// in the pointer state after the step (as enqueue_step leaves them); on failure in
+// the state they had.
+int ens_build(cmcg_ctx **ctxs, int count, std::unique_ptr<EnsPlan> &out)
+{
+    auto plan = std::make_unique<EnsPlan>();
+    plan->key = ens_key(ctxs, count);
+    plan->device = ctxs[0]->device;
+    std::vector<SegRecorder> rec(count);
+    std::vector<CtxPointers> saved(count);
+    int err = CMCG_OK, done_ctx = 0;
+    for (int i = 0; i < count && !err; i++, done_ctx++) {
+        cmcg_ctx *c = ctxs[i];
+        saved[i] = CtxPointers{c->f, c->alt, c->sc, c->parity};
+        const bool timing = c->timing;
+        c->timing = false;
+        // the capture proves the step issued nothing but kernel launches (all recorded)
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            c->timing = timing;
+            err = CMCG_E_HIP;
+            break;
+        }
+        g_seg_rec = &rec[i];
+        int e = enqueue_step(c, c->n_cap, true);
+        g_seg_rec = nullptr;
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        c->timing = timing;
+        size_t nodes = 0;
+        if (g) {
+            if (hipGraphGetNodes(g, nullptr, &nodes) != hipSuccess) ce = hipErrorUnknown;
+            hipGraphDestroy(g);
+        }
+        if (e) err = e;
+        else if (ce != hipSuccess || nodes != 0 || rec[i].bad) err = CMCG_E_HIP;
+    }
+    if (!err) err = ens_assemble(plan.get(), rec, ctxs, count);
+    if (err) {
+        for (int i = 0; i < done_ctx; i++) {
+            ctxs[i]->f = saved[i].f;
+            ctxs[i]->alt = saved[i].alt;
+            ctxs[i]->sc = saved[i].sc;
+            ctxs[i]->parity = saved[i].parity;
+        }
+        return err;
+    }
+    out = std::move(plan);
+    return CMCG_OK;
+}
+
+// Launches a plan on the first cluster's stream, ordered after every cluster's
+// earlier work and before its later work.
+int ens_launch(EnsPlan &p, cmcg_ctx **ctxs, int count)
+{
+    hipStream_t st = ctxs[0]->stream;
+    for (int i = 1; i < count; i++) {
+        if (hipEventRecord(p.ev[i], ctxs[i]->stream) != hipSuccess) return CMCG_E_HIP;
+        if (hipStreamWaitEvent(st, p.ev[i], 0) != hipSuccess) return CMCG_E_HIP;
+    }
+    if (hipGraphLaunch(p.exec, st) != hipSuccess) return CMCG_E_HIP;
+    if (hipEventRecord(p.done, st) != hipSuccess) return CMCG_E_HIP;
+    for (int i = 1; i < count; i++)
+        if (hipStreamWaitEvent(ctxs[i]->stream, p.done, 0) != hipSuccess) return CMCG_E_HIP;
+    return CMCG_OK;
+}
+
+} // namespace
+
+// drops the ensemble plans that step a cluster (before its context goes away)
+static void ens_forget(uint64_t serial)
+{
+    std::lock_guard<std::mutex> lk(g_ens_mu);
+    auto has = [serial](const std::unique_ptr<EnsPlan> &p) {
+        for (size_t i = 0; i < p->key.size(); i += 3)
+            if (p->key[i] == serial) return true;
+        return false;
+    };
+    g_ens_plans.erase(std::remove_if(g_ens_plans.begin(), g_ens_plans.end(), has), g_ens_plans.end());
+}
+
+extern "C" {
+
+int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors)
+{
+    if (!ctxs || count < 1) return CMCG_E_ARG;
+    for (int i = 0; i < count; i++) {
+        if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) return CMCG_E_ARG;
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return CMCG_E_ARG;
+    }
+    hipSetDevice(ctxs[0]->device);
+    {
+        std::lock_guard<std::mutex> lk(g_ens_mu);
+        const std::vector<uint64_t> key = ens_key(ctxs, count);
+        auto it = std::find_if(g_ens_plans.begin(), g_ens_plans.end(),
+                               [&](const std::unique_ptr<EnsPlan> &p) { return p->key == key; });
+        std::unique_ptr<EnsPlan> plan;
+        if (it != g_ens_plans.end()) {
+            plan = std::move(*it);
+            g_ens_plans.erase(it);
+            for (int i = 0; i < count; i++) apply_step_swap(ctxs[i]);   // the state the plan's step leaves
+        } else if (int e = ens_build(ctxs, count, plan)) {
+            return e;
+        }
+        int e = ens_launch(*plan, ctxs, count);
+        g_ens_last_launches = plan->launches;
+        g_ens_last_recorded = plan->recorded;
+        g_ens_plans.push_back(std::move(plan));
+        if (g_ens_plans.size() > ENS_MAX_PLANS) g_ens_plans.erase(g_ens_plans.begin());
+        if (e) return e;
+    }
+    int first_err = CMCG_OK;
+    for (int i = 0; i < count; i++) {
+        int e = finish_step(ctxs[i], stats ? &stats[i] : nullptr);
+        if (errors) errors[i] = e;
+        if (e && !first_err) first_err = e;
+    }
+    return first_err;
+}
+
+int cmcg_ensemble_stats(int64_t *launches, int64_t *recorded)
+{
+    std::lock_guard<std::mutex> lk(g_ens_mu);
+    if (launches) *launches = g_ens_last_launches;
+    if (recorded) *recorded = g_ens_last_recorded;
+    return CMCG_OK;
 }
 
 int cmcg_rng_draws(cmcg_ctx *c, int phase, uint64_t step, int64_t first, int64_t count, int ndraw, double *out)

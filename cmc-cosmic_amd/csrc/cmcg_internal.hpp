@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "../../include/cmcg.h"
+#include "seg.hpp"
 
 #define CMCG_SF_INFINITY 1.0e10
 #define CMCG_ZERO 1.0e-20
@@ -212,9 +213,9 @@ __device__ __forceinline__ Rng rng_jump_lds(const NibbleTables<NB> &lt, Rng s, u
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  This
 // bijection gives XCD x a contiguous range of logical blocks, so the blocks that
 // share an L2 work on neighbouring stars and their search paths overlap.
-__device__ __forceinline__ int64_t xcd_block()
+__device__ __forceinline__ int64_t xcd_block(const KGrid &kg)
 {
-    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t nb = kg.nb, b = kg.bid;
     const int64_t x = b & 7, local = b >> 3, q = nb >> 3, rem = nb & 7;
     return x * q + (x < rem ? x : rem) + local;
 }

@@ -46,7 +46,9 @@ __device__ void final_sum(const double *__restrict__ partials, int64_t nblocks, 
 }
 
 // cmc.c:246-258, 278: per-step resets
-__global__ void step_begin_kernel(DevScalars *s)
+struct step_begin_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s)
 {
     s->Eescaped_old = s->Eescaped;
     s->Jescaped_old = s->Jescaped;
@@ -56,25 +58,31 @@ __global__ void step_begin_kernel(DevScalars *s)
     s->n_max_new = s->n_max;
     s->dtrel = __longlong_as_double(0x7ff0000000000000ll);     // the time step's minimum starts at +inf
 }
+};
 
-__global__ void after_dynamics_kernel(DevScalars *s) { s->Prev_Dt = s->Dt; }   // cmc.c:297
+struct after_dynamics_kernel { static constexpr int LB = 1024, MB = 0; static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s) { s->Prev_Dt = s->Dt; } };   // cmc.c:297
 
-__global__ void ec1_kernel(DevFields f, DevScalars *s, KParams P)
+struct ec1_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P)
 {
-    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i > s->n_max) return;
     double ri = f.r[i];
     f.vtold[i] = f.vt[i];
     f.vrold[i] = f.vr[i];
     f.Uoldrold[i] = f.phi[i] + phi_s_own(ri, f.m[i], P.inv_nstar, ri);
 }
+};
 
 // energy_conservation2: Uoldrnew = potential(rnew) + MPI_PHI_S(rnew, i).  The orbit
 // phase already evaluated potential(rnew) for sampled stars (flag 256) on the same
 // arrays, so only circular / removed stars search here.
-__global__ void ec2_kernel(DevFields f, DevScalars *s, KParams P)
+struct ec2_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P)
 {
-    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
     if (i > n) return;
     double x = f.rnew[i];
@@ -91,6 +99,7 @@ __global__ void ec2_kernel(DevFields f, DevScalars *s, KParams P)
     }
     f.Uoldrnew[i] = pot + phi_s_own(f.r[i], f.m[i], P.inv_nstar, x);
 }
+};
 
 // The step's form of energy_conservation2 followed by ComputeIntermediateEnergy's
 // first loop (cmc_utils.c:335-351): the stripping between them leaves every field
@@ -99,9 +108,11 @@ __global__ void ec2_kernel(DevFields f, DevScalars *s, KParams P)
 // It also forms D = vold^2 + 2 (1 - q) (Uoldrold - Uoldrnew), the part of
 // set_velocities3's vnew^2 (cmc_utils.c:242-247) that the sort then carries in place of
 // its four inputs: the same operations in the same order, so the same double.
-__global__ void ec2_ie_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ dpre)
+struct ec2_ie_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, double *__restrict__ dpre)
 {
-    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
     if (i > n) return;
     const double x = f.rnew[i];
@@ -124,12 +135,16 @@ __global__ void ec2_ie_kernel(DevFields f, DevScalars *s, KParams P, double *__r
     const double vold2 = vtold * vtold + vrold * vrold;
     dpre[i] = vold2 + 2.0 * (1.0 - q) * (f.Uoldrold[i] - Uoldrnew);
 }
+};
 
-__global__ void strip_begin_kernel(DevScalars *s, int32_t *done)
+struct strip_begin_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, int32_t *done)
 {
     s->Etidal = 0.0;        // cmc_evolution_thr.c:449
     *done = 0;
 }
+};
 
 // One pass of the stripping loop (cmc_evolution_thr.c:447-645) in one kernel: every
 // block derives Rtidal from the scalars left by the previous pass, strips its stars
@@ -190,30 +205,35 @@ __device__ void strip_finish(DevScalars *s, double Rt, int32_t *done, const doub
     }
 }
 
-__global__ __launch_bounds__(GT) void strip_fused_kernel(DevFields f, DevScalars *s, KParams P, int32_t *done,
+struct strip_fused_kernel {
+    static constexpr int LB = GT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, int32_t *done,
                                                          double *__restrict__ partials, unsigned *__restrict__ arrive)
 {
     if (*done) return;
     const double Rt = strip_rtidal(s, P);
-    strip_block(f, P, Rt, s->n_max, partials, gridDim.x, blockIdx.x);
+    strip_block(f, P, Rt, s->n_max, partials, kg_.nb, kg_.bid);
     __shared__ bool last;
     if (threadIdx.x == 0) {
         __threadfence();                                   // publish this block's partials
-        last = atomicAdd(arrive, 1u) == gridDim.x - 1;
+        last = atomicAdd(arrive, 1u) == kg_.nb - 1;
     }
     __syncthreads();
     if (!last) return;
     __threadfence();                                       // see every block's partials
-    strip_finish(s, Rt, done, partials, gridDim.x);
+    strip_finish(s, Rt, done, partials, kg_.nb);
     if (threadIdx.x == 0) *arrive = 0;
 }
+};
 
 // The rest of the loop in one block, for a captured step: after the grid passes the
 // loop has nearly always ended and this returns at once; otherwise it runs further
 // passes here, at most STRIP_TAIL_PASSES (then the step reports CMCG_E_STRIP).  The
 // block plays the nb blocks of a grid pass one after another and writes the same
 // partials, so the sums add in the same order as in the eager loop.
-__global__ __launch_bounds__(GT) void strip_tail_kernel(DevFields f, DevScalars *s, KParams P, int32_t *done,
+struct strip_tail_kernel {
+    static constexpr int LB = GT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, int32_t *done,
                                                         double *__restrict__ partials, unsigned nb)
 {
     __shared__ int32_t sdone;
@@ -233,11 +253,14 @@ __global__ __launch_bounds__(GT) void strip_tail_kernel(DevFields f, DevScalars 
     }
     if (threadIdx.x == 0 && !*(volatile int32_t *)done) dev_set_error(s, CMCG_E_STRIP, 0);
 }
+};
 
 // ComputeIntermediateEnergy, first loop (EI).
-__global__ void ie_kernel(DevFields f, DevScalars *s)
+struct ie_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s)
 {
-    int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t j = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
     if (j > n) return;
     double x = f.rnew[j];
@@ -252,17 +275,21 @@ __global__ void ie_kernel(DevFields f, DevScalars *s)
         f.EI[j] = sqr(f.vr[j]) + sqr(f.vt[j]) + f.phi[j] - pot;
     }
 }
+};
 
 // ComputeIntermediateEnergy, second loop (transfer of the new orbit variables).
-__global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
+struct ie_transfer_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s)
 {
-    int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t j = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (j > s->n_max) return;
     f.rOld[j] = f.r[j];
     f.r[j] = f.rnew[j];
     f.vr[j] = f.vrnew[j];
     f.vt[j] = f.vtnew[j];
 }
+};
 
 // set_velocities3 per-star part.  Stars with vnew2 <= 0 ("events") carry excess
 // energy that the following stars absorb (cmc_utils.c:268-310); they are counted
@@ -278,7 +305,9 @@ __global__ void ie_transfer_kernel(DevFields f, DevScalars *s)
 // the values equal potential_at(x) bit for bit.
 constexpr int MERGE_T = 256, MERGE_PER = 2, MERGE_BLK = MERGE_T * MERGE_PER, MERGE_LDS = 1024;
 
-__global__ __launch_bounds__(MERGE_T) void old_node_potential_kernel(const double *__restrict__ r_old,
+struct old_node_potential_kernel {
+    static constexpr int LB = MERGE_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r_old,
                                                                      const double2 *__restrict__ nodes,
                                                                      DevScalars *s, double2 *__restrict__ u_old)
 {
@@ -286,7 +315,7 @@ __global__ __launch_bounds__(MERGE_T) void old_node_potential_kernel(const doubl
     __shared__ int64_t ends[2];
     __shared__ double xends[2];
     const int64_t n_old = s->sort_in, n = s->n_max;
-    const int64_t first = 1 + (int64_t)blockIdx.x * MERGE_BLK;
+    const int64_t first = 1 + (int64_t)kg_.bid * MERGE_BLK;
     if (first > n_old) return;
     const int64_t last = first + MERGE_BLK - 1 <= n_old ? first + MERGE_BLK - 1 : n_old;
     const PackedNodes nd{nodes};
@@ -339,13 +368,16 @@ __global__ __launch_bounds__(MERGE_T) void old_node_potential_kernel(const doubl
         u_old[sl] = make_double2(v, x);       // with the radius: set_velocities3 reads both at once
     }
 }
+};
 
 constexpr int SV3_COARSE = 1024;      // LDS samples of r for potential(rOld)
 
 // With u_old (the step): potential(rOld) and rOld itself through the sort's permutation,
 // and D (dpost) in place of vtold, vrold, Uoldrold and Uoldrnew.  Without: the
 // reference's fields and a search (the entry point of the reference's name).
-__global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
+struct sv3_kernel {
+    static constexpr int LB = GT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, double *__restrict__ excess_c,
                                                  int64_t *__restrict__ blk_cnt, const double2 *__restrict__ mid,
                                                  const double2 *__restrict__ nodes, const double2 *__restrict__ u_old,
                                                  const uint32_t *__restrict__ perm, const double *__restrict__ dpost)
@@ -353,7 +385,7 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
     __shared__ int cnt;
     __shared__ double cr[SV3_COARSE + 1];
     const double q = 0.5;
-    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
     Coarse C;
     C.cr = cr; C.cphi = nullptr; C.cs = SV3_COARSE; C.n1 = n + 1; C.hint = false;
@@ -396,15 +428,18 @@ __global__ __launch_bounds__(GT) void sv3_kernel(DevFields f, DevScalars *s, KPa
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) blk_cnt[blockIdx.x] = cnt;
+    if (threadIdx.x == 0) blk_cnt[kg_.bid] = cnt;
 }
+};
 
 // Exclusive scan of the per-block event counts, in place, and their total.  Each thread
 // takes a contiguous chunk, held in registers when it has at most SV3_CHUNK entries (so
 // the write-back waits on no load), the chunk sums are scanned by wave shuffles and the
 // 16 wave totals through LDS.
 constexpr int SV3_CHUNK = 16;
-__global__ __launch_bounds__(1024) void sv3_scan_kernel(DevScalars *s, int64_t *__restrict__ blk_cnt, int64_t nb)
+struct sv3_scan_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, int64_t *__restrict__ blk_cnt, int64_t nb)
 {
     __shared__ int64_t wtot[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -451,24 +486,28 @@ __global__ __launch_bounds__(1024) void sv3_scan_kernel(DevScalars *s, int64_t *
         }
     }
 }
+};
 
 // ordered list of the event stars
-__global__ __launch_bounds__(GT) void sv3_compact_kernel(DevScalars *s, const double *__restrict__ excess_c,
+struct sv3_compact_kernel {
+    static constexpr int LB = GT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, const double *__restrict__ excess_c,
                                                          const int64_t *__restrict__ blk_off,
                                                          int64_t *__restrict__ events)
 {
     __shared__ int wcnt[GT / 64];
-    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     bool ev = i <= s->n_max && excess_c[i] >= 0.0;
     uint64_t bal = __ballot(ev);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) wcnt[w] = __popcll(bal);
     __syncthreads();
-    int64_t pos = blk_off[blockIdx.x];
+    int64_t pos = blk_off[kg_.bid];
     for (int w2 = 0; w2 < w; w2++) pos += wcnt[w2];
     pos += __popcll(bal & ((1ull << lane) - 1ull));
     if (ev) events[pos] = i;
 }
+};
 
 constexpr int64_t SPEC_WALK_MAX = 4096;
 
@@ -507,28 +546,33 @@ __device__ int64_t excess_walk(const DevFields &f, const double *__restrict__ ex
 // Fresh (no carried excess) bounded walk of every event.  joins[d] = 1 when that
 // walk reaches event d + 1 or was cut off, so event d + 1 is presumed to lie inside
 // the same episode.  An episode starting at a true head is exactly its fresh walk.
-__global__ void sv3_spec_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
+struct sv3_spec_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, const double *__restrict__ excess_c,
                                 const int64_t *__restrict__ events, int64_t *__restrict__ wend,
                                 int32_t *__restrict__ joins)
 {
-    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t ne = s->n_excess;
     if (d >= ne) return;
     int64_t e = excess_walk(f, excess_c, events[d], s->n_max, false, SPEC_WALK_MAX);
     wend[d] = e;
     joins[d] = (e < 0) || (d + 1 < ne && e >= events[d + 1]);
 }
+};
 
 // Presumed heads are d = 0 and every d with joins[d-1] = 0; the presumed group of
 // head d runs to the first q >= d with joins[q] = 0.  If every presumed head's walk
 // ends inside its group's span [events[q], events[q+1]), then by induction from
 // event 0 (which starts with no carried excess) each presumed head is a true head
 // and its walk is the true episode.  Otherwise the heads are relinked exactly.
-__global__ void sv3_verify_kernel(DevScalars *s, const int64_t *__restrict__ events,
+struct sv3_verify_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, const int64_t *__restrict__ events,
                                   const int32_t *__restrict__ joins, const int64_t *__restrict__ wend,
                                   int64_t *__restrict__ relink)
 {
-    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t ne = s->n_excess;
     if (d >= ne || (d > 0 && joins[d - 1])) return;
     int64_t q = d;
@@ -537,16 +581,19 @@ __global__ void sv3_verify_kernel(DevScalars *s, const int64_t *__restrict__ eve
     if (e < 0 || e < events[q] || (q + 1 < ne && e >= events[q + 1]))
         atomicMin((unsigned long long *)relink, (unsigned long long)d);
 }
+};
 
 // Exact head chain in event order: the next head is the first event after the end
 // of the current episode.  Runs only when a presumed head failed the check, and
 // starts at the first one that did: every presumed head before it passed, so by
 // induction it is a true head and the groups before it are exact.
-__global__ void sv3_relink_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
+struct sv3_relink_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, const double *__restrict__ excess_c,
                                   const int64_t *__restrict__ events, const int64_t *__restrict__ wend,
                                   int32_t *__restrict__ joins, const int64_t *__restrict__ relink)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (threadIdx.x != 0 || kg_.bid != 0) return;
     int64_t n = s->n_max, ne = s->n_excess;
     int64_t d = *relink;
     if (d >= ne) return;
@@ -559,23 +606,29 @@ __global__ void sv3_relink_kernel(DevFields f, DevScalars *s, const double *__re
         d = d2;
     }
 }
+};
 
 // every head walks its episode and writes the dumped velocities
-__global__ void sv3_heads_kernel(DevFields f, DevScalars *s, const double *__restrict__ excess_c,
+struct sv3_heads_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, const double *__restrict__ excess_c,
                                  const int64_t *__restrict__ events, const int32_t *__restrict__ joins)
 {
-    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t ne = s->n_excess;
     if (d >= ne || (d > 0 && joins[d - 1])) return;
     excess_walk(f, excess_c, events[d], s->n_max, true, INT64_MAX);
 }
+};
 
-__global__ void sv3_begin_kernel(DevScalars *s, int64_t *relink) { s->n_excess = 0; *relink = INT64_MAX; }
+struct sv3_begin_kernel { static constexpr int LB = 1024, MB = 0; static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, int64_t *relink) { s->n_excess = 0; *relink = INT64_MAX; } };
 
-__global__ __launch_bounds__(GT) void energy_kernel(DevFields f, DevScalars *s, KParams P,
+struct energy_kernel {
+    static constexpr int LB = GT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
                                                     double *__restrict__ partials)
 {
-    int64_t i = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     int64_t n = s->n_max;
     if (i <= n) {
@@ -585,11 +638,14 @@ __global__ __launch_bounds__(GT) void energy_kernel(DevFields f, DevScalars *s, 
         v[0] = 0.5 * (sqr(vr) + sqr(vt)) * m * P.madhoc;
         v[1] = ph * m * P.madhoc + f.phi[0] * P.cenma_m * P.madhoc / n;
     }
-    block_sum_store<2>(v, partials + 8 * blockIdx.x);
+    block_sum_store<2>(v, partials + 8 * kg_.bid);
 }
+};
 
 // compute_energy_new (cmc_utils.c:2017-2028) + the totals of ComputeEnergy
-__global__ void energy_end_kernel(DevScalars *s, const double *__restrict__ partials, int64_t nblocks)
+struct energy_end_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, const double *__restrict__ partials, int64_t nblocks)
 {
     double res[2];
     final_sum<2>(partials, nblocks, res);
@@ -604,32 +660,33 @@ __global__ void energy_end_kernel(DevScalars *s, const double *__restrict__ part
         s->Etot += 0.0 + s->Eescaped + 0.0 + 0.0;
     }
 }
+};
 
 unsigned grid_for(int64_t n) { return (unsigned)((n + GT - 1) / GT > 0 ? (n + GT - 1) / GT : 1); }
 } // namespace
 
 hipError_t launch_step_begin(DevScalars *s, hipStream_t st)
 {
-    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(1), 0, st, s);
+    kl<step_begin_kernel>(dim3(1), dim3(1), 0, st, s);
     return hipGetLastError();
 }
 hipError_t launch_after_dynamics(DevScalars *s, hipStream_t st)
 {
-    hipLaunchKernelGGL(after_dynamics_kernel, dim3(1), dim3(1), 0, st, s);
+    kl<after_dynamics_kernel>(dim3(1), dim3(1), 0, st, s);
     return hipGetLastError();
 }
 hipError_t launch_energy_conservation1(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
                                        hipStream_t st)
 {
-    hipLaunchKernelGGL(ec1_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
+    kl<ec1_kernel>(dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
     return hipGetLastError();
 }
 hipError_t launch_energy_conservation2(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
                                        hipStream_t st, bool with_ie, double *dpre)
 {
     if (with_ie && !dpre) return hipErrorInvalidValue;
-    if (with_ie) hipLaunchKernelGGL(ec2_ie_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P, dpre);
-    else hipLaunchKernelGGL(ec2_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
+    if (with_ie) kl<ec2_ie_kernel>(dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P, dpre);
+    else kl<ec2_kernel>(dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s, P);
     return hipGetLastError();
 }
 hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, int32_t *done,
@@ -639,9 +696,9 @@ hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, con
     // a fixed grid (grid-stride over the stars) that the per-block partials hold
     const unsigned nb = STRIP_BLOCKS;              // the scratch holds at least STRIP_BLOCKS partials
     if (sc.max_blocks < STRIP_BLOCKS) return hipErrorInvalidValue;
-    if (begin) hipLaunchKernelGGL(strip_begin_kernel, dim3(1), dim3(1), 0, st, s, done);
+    if (begin) kl<strip_begin_kernel>(dim3(1), dim3(1), 0, st, s, done);
     for (int it = 0; it < passes; it++)
-        hipLaunchKernelGGL(strip_fused_kernel, dim3(nb), dim3(GT), 0, st, f, s, P, done, sc.partials,
+        kl<strip_fused_kernel>(dim3(nb), dim3(GT), 0, st, f, s, P, done, sc.partials,
                            sc.strip_arrive);
     return hipGetLastError();
 }
@@ -649,7 +706,7 @@ hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, con
 hipError_t launch_strip_tail(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc, hipStream_t st)
 {
     if (sc.max_blocks < STRIP_BLOCKS) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(strip_tail_kernel, dim3(1), dim3(GT), 0, st, f, s, P, sc.strip_done, sc.partials,
+    kl<strip_tail_kernel>(dim3(1), dim3(GT), 0, st, f, s, P, sc.strip_done, sc.partials,
                        (unsigned)STRIP_BLOCKS);
     return hipGetLastError();
 }
@@ -657,12 +714,12 @@ hipError_t launch_intermediate_energy(const DevFields &f, DevScalars *s, const K
                                       hipStream_t st)
 {
     (void)P;
-    hipLaunchKernelGGL(ie_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s);
+    kl<ie_kernel>(dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s);
     return hipGetLastError();
 }
 hipError_t launch_ie_transfer(const DevFields &f, DevScalars *s, int64_t n_ub, hipStream_t st)
 {
-    hipLaunchKernelGGL(ie_transfer_kernel, dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s);
+    kl<ie_transfer_kernel>(dim3(grid_for(n_ub)), dim3(GT), 0, st, f, s);
     return hipGetLastError();
 }
 hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc,
@@ -672,28 +729,28 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     int64_t nb = grid_for(n_ub);
     if (nb > sc.max_blocks) return hipErrorInvalidValue;
     unsigned ge = grid_for(n_ub);
-    hipLaunchKernelGGL(sv3_begin_kernel, dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
+    kl<sv3_begin_kernel>(dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
     // the step calls this right after the potential, which left the node table current
     if (!dpost)
         if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     if (r_old_sorted) {
-        hipLaunchKernelGGL(old_node_potential_kernel, dim3((unsigned)((n_ub + MERGE_BLK - 1) / MERGE_BLK)),
+        kl<old_node_potential_kernel>(dim3((unsigned)((n_ub + MERGE_BLK - 1) / MERGE_BLK)),
                            dim3(MERGE_T), 0, st, r_old_sorted, (const double2 *)sc.nodes, s, sc.u_old);
     }
-    hipLaunchKernelGGL(sv3_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
+    kl<sv3_kernel>(dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
                        (const double2 *)sc.mid, (const double2 *)sc.nodes,
                        r_old_sorted ? (const double2 *)sc.u_old : nullptr, (const uint32_t *)sc.vals_a, dpost);
-    hipLaunchKernelGGL(sv3_scan_kernel, dim3(1), dim3(1024), 0, st, s, sc.sv3_blk, nb);
-    hipLaunchKernelGGL(sv3_compact_kernel, dim3((unsigned)nb), dim3(GT), 0, st, s, (const double *)sc.excess_c,
+    kl<sv3_scan_kernel>(dim3(1), dim3(1024), 0, st, s, sc.sv3_blk, nb);
+    kl<sv3_compact_kernel>(dim3((unsigned)nb), dim3(GT), 0, st, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.sv3_blk, sc.excess_list);
-    hipLaunchKernelGGL(sv3_spec_kernel, dim3(ge), dim3(GT), 0, st, f, s, (const double *)sc.excess_c,
+    kl<sv3_spec_kernel>(dim3(ge), dim3(GT), 0, st, f, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.excess_list, sc.sv3_wend, sc.sv3_merged);
-    hipLaunchKernelGGL(sv3_verify_kernel, dim3(ge), dim3(GT), 0, st, s, (const int64_t *)sc.excess_list,
+    kl<sv3_verify_kernel>(dim3(ge), dim3(GT), 0, st, s, (const int64_t *)sc.excess_list,
                        (const int32_t *)sc.sv3_merged, (const int64_t *)sc.sv3_wend, sc.sv3_fallback);
-    hipLaunchKernelGGL(sv3_relink_kernel, dim3(1), dim3(64), 0, st, f, s, (const double *)sc.excess_c,
+    kl<sv3_relink_kernel>(dim3(1), dim3(64), 0, st, f, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.excess_list, (const int64_t *)sc.sv3_wend, sc.sv3_merged,
                        (const int64_t *)sc.sv3_fallback);
-    hipLaunchKernelGGL(sv3_heads_kernel, dim3(ge), dim3(GT), 0, st, f, s, (const double *)sc.excess_c,
+    kl<sv3_heads_kernel>(dim3(ge), dim3(GT), 0, st, f, s, (const double *)sc.excess_c,
                        (const int64_t *)sc.excess_list, (const int32_t *)sc.sv3_merged);
     return hipGetLastError();
 }
@@ -702,7 +759,7 @@ hipError_t launch_compute_energy(const DevFields &f, DevScalars *s, const KParam
 {
     int64_t nb = grid_for(n_ub);
     if (nb > sc.max_blocks) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(energy_kernel, dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.partials);
-    hipLaunchKernelGGL(energy_end_kernel, dim3(1), dim3(GT), 0, st, s, (const double *)sc.partials, nb);
+    kl<energy_kernel>(dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.partials);
+    kl<energy_end_kernel>(dim3(1), dim3(GT), 0, st, s, (const double *)sc.partials, nb);
     return hipGetLastError();
 }

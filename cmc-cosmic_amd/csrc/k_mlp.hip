@@ -37,7 +37,9 @@ __device__ __forceinline__ int64_t shuffled(int64_t i, int64_t count, uint64_t m
 
 // One block: tiles blockIdx.x, blockIdx.x + gridDim.x, ... of the batch.  Writes the
 // block's gradient partial (MLP_NPARAM floats) and its loss partial.
-__global__ __launch_bounds__(FT) void mlp_fwdbwd_kernel(const float *__restrict__ theta, const MlpNorm *__restrict__ nrm,
+struct mlp_fwdbwd_kernel {
+    static constexpr int LB = FT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const float *__restrict__ theta, const MlpNorm *__restrict__ nrm,
                                                         const float *__restrict__ rows, int64_t count, int64_t first,
                                                         int64_t batch, uint64_t mult, uint64_t off, float inv_b,
                                                         float *__restrict__ gpart, double *__restrict__ lpart)
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(FT) void mlp_fwdbwd_kernel(const float *__restrict_
     double loss = 0.0;
     __syncthreads();
     const int64_t ntiles = (batch + TILE - 1) / TILE;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int64_t tile = kg_.bid; tile < ntiles; tile += kg_.nb) {
         // 1. inputs and targets (standardised); samples past the batch get zero weight
         if (t < TILE) {
             int64_t i = tile * TILE + t;
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(FT) void mlp_fwdbwd_kernel(const float *__restrict_
         }
         __syncthreads();
     }
-    float *gp = gpart + (int64_t)blockIdx.x * MLP_NPARAM;
+    float *gp = gpart + (int64_t)kg_.bid * MLP_NPARAM;
     for (int q = 0; q < 16; q++) gp[OFF_W2 + t + FT * q] = gW2[q];
     for (int k = 0; k < 2; k++)
         if (t + FT * k < NH * NIN) gp[OFF_W1 + t + FT * k] = gW1[k];
@@ -175,15 +177,18 @@ __global__ __launch_bounds__(FT) void mlp_fwdbwd_kernel(const float *__restrict_
     if (t == 0) {
         double a = 0.0;
         for (int i = 0; i < FT; i++) a += lred[i];
-        lpart[blockIdx.x] = a;
+        lpart[kg_.bid] = a;
     }
 }
+};
 
 // gradient = sum of the block partials in block order; loss likewise (mean over batch)
-__global__ void mlp_reduce_kernel(const float *__restrict__ gpart, const double *__restrict__ lpart, int nblocks,
+struct mlp_reduce_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const float *__restrict__ gpart, const double *__restrict__ lpart, int nblocks,
                                   float *__restrict__ grad, double *__restrict__ loss, double inv_b)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = kg_.bid * blockDim.x + threadIdx.x;
     if (p < MLP_NPARAM) {
         float a = 0.f;
         for (int b = 0; b < nblocks; b++) a += gpart[(int64_t)b * MLP_NPARAM + p];
@@ -195,12 +200,15 @@ __global__ void mlp_reduce_kernel(const float *__restrict__ gpart, const double 
         *loss = a * inv_b;
     }
 }
+};
 
 // Adam (Kingma and Ba 2015, Algorithm 1), step t counted from 1
-__global__ void mlp_adam_kernel(float *__restrict__ theta, const float *__restrict__ grad, float *__restrict__ m,
+struct mlp_adam_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, float *__restrict__ theta, const float *__restrict__ grad, float *__restrict__ m,
                                 float *__restrict__ v, int64_t step, float lr)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = kg_.bid * blockDim.x + threadIdx.x;
     if (p >= MLP_NPARAM) return;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     float g = grad[p];
@@ -212,27 +220,33 @@ __global__ void mlp_adam_kernel(float *__restrict__ theta, const float *__restri
     float vh = vv / (1.f - powf(b2, (float)step));
     theta[p] -= lr * mh / (sqrtf(vh) + eps);
 }
+};
 
 // per-thread inference; weights are wave-uniform reads
-__global__ void mlp_predict_kernel(const float *__restrict__ theta, const MlpNorm *__restrict__ nrm,
+struct mlp_predict_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const float *__restrict__ theta, const MlpNorm *__restrict__ nrm,
                                    const float *__restrict__ x, int64_t n, float *__restrict__ y)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float in[NIN], out[NOUT];
     for (int c = 0; c < NIN; c++) in[c] = x[NIN * i + c];
     mlp_forward(theta, nrm, in, out);
     for (int o = 0; o < NOUT; o++) y[NOUT * i + o] = out[o];
 }
+};
 
 // column sums of the harvested rows (mean and variance of every input and target)
-__global__ __launch_bounds__(256) void mlp_colsum_kernel(const float *__restrict__ rows, int64_t count,
+struct mlp_colsum_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const float *__restrict__ rows, int64_t count,
                                                          double *__restrict__ part)
 {
     __shared__ double lds[16][256];
     double acc[16];
     for (int c = 0; c < 16; c++) acc[c] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x; i < count; i += (int64_t)kg_.nb * blockDim.x)
         for (int c = 0; c < 8; c++) {
             double v = rows[8 * i + c];
             acc[c] += v;
@@ -243,11 +257,14 @@ __global__ __launch_bounds__(256) void mlp_colsum_kernel(const float *__restrict
     if (threadIdx.x < 16) {
         double a = 0.0;
         for (int k = 0; k < 256; k++) a += lds[threadIdx.x][k];
-        part[16 * blockIdx.x + threadIdx.x] = a;
+        part[16 * kg_.bid + threadIdx.x] = a;
     }
 }
+};
 
-__global__ void mlp_norm_kernel(const double *__restrict__ part, int nblocks, int64_t count, MlpNorm *__restrict__ nrm)
+struct mlp_norm_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ part, int nblocks, int64_t count, MlpNorm *__restrict__ nrm)
 {
     int c = threadIdx.x;
     if (c >= 8) return;
@@ -259,15 +276,18 @@ __global__ void mlp_norm_kernel(const double *__restrict__ part, int nblocks, in
     if (c < NIN) { nrm->xm[c] = (float)mean; nrm->xs[c] = sd; }
     else { nrm->ym[c - NIN] = (float)mean; nrm->ys[c - NIN] = sd; }
 }
+};
 
 // Training rows from the orbit phase just run: every sampled star (flag 256, set
 // only by the orbit sampler before energy_conservation2) appends its features and
 // exact apsides to the ring buffer.
-__global__ __launch_bounds__(256) void mlp_harvest_kernel(DevFields f, DevScalars *s, KParams P,
+struct mlp_harvest_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
                                                           float *__restrict__ rows, int64_t cap,
                                                           unsigned long long *__restrict__ counter)
 {
-    int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t j = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t n = s->n_max;
     bool take = j <= n && (f.flags[j] & 256) && !(f.flags[j] & (CMCG_SF_LINSCAN | CMCG_SF_BRENT));
     float x[NIN];
@@ -289,6 +309,7 @@ __global__ __launch_bounds__(256) void mlp_harvest_kernel(DevFields f, DevScalar
     row[NIN] = logf((float)f.r_peri[j]);
     row[NIN + 1] = logf((float)f.r_apo[j]);
 }
+};
 
 // ------------------------------------------------------------------ matrix-core inference
 // The network on the matrix cores: a wave evaluates 16 stars at a time as transposed
@@ -371,7 +392,9 @@ __device__ __forceinline__ half4_t mfma_inputs(const float (&x)[NIN], const MlpN
 
 // Predictions for every star 1..n of the current state (the orbit phase's proposals):
 // pred[j] = (log r_peri, log r_apo).  Persistent waves over 16-star tiles.
-__global__ __launch_bounds__(256) void mlp_infer_stars_kernel(DevFields f, DevScalars *s, KParams P,
+struct mlp_infer_stars_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
                                                               const float *__restrict__ theta,
                                                               const MlpNorm *__restrict__ nrm,
                                                               float2 *__restrict__ pred)
@@ -380,8 +403,8 @@ __global__ __launch_bounds__(256) void mlp_infer_stars_kernel(DevFields f, DevSc
     mfma_net_load(theta, w);
     const int lane = threadIdx.x & 63, row = lane & 15, quad = lane >> 4;
     const int64_t n = s->n_max;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)kg_.bid * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)kg_.nb * blockDim.x) >> 6;
     const float ym0 = nrm->ym[0], ys0 = nrm->ys[0], ym1 = nrm->ym[1], ys1 = nrm->ys[1];
     for (int64_t t = wave; 16 * t < n; t += nwaves) {
         const int64_t j = 1 + 16 * t + row;
@@ -397,10 +420,13 @@ __global__ __launch_bounds__(256) void mlp_infer_stars_kernel(DevFields f, DevSc
         if (quad == 0 && j <= n) pred[j] = make_float2(y0 * ys0 + ym0, y1 * ys1 + ym1);
     }
 }
+};
 
 // The same network on given raw inputs x[6n] -> y[2n] (tests compare it with the
 // per-thread FP32 path).
-__global__ __launch_bounds__(256) void mlp_infer_rows_kernel(const float *__restrict__ theta,
+struct mlp_infer_rows_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const float *__restrict__ theta,
                                                              const MlpNorm *__restrict__ nrm,
                                                              const float *__restrict__ xin, int64_t n,
                                                              float *__restrict__ y)
@@ -408,8 +434,8 @@ __global__ __launch_bounds__(256) void mlp_infer_rows_kernel(const float *__rest
     MfmaNet w;
     mfma_net_load(theta, w);
     const int lane = threadIdx.x & 63, row = lane & 15, quad = lane >> 4;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)kg_.bid * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)kg_.nb * blockDim.x) >> 6;
     for (int64_t t = wave; 16 * t < n; t += nwaves) {
         const int64_t i = 16 * t + row;
         float x[NIN];
@@ -424,6 +450,7 @@ __global__ __launch_bounds__(256) void mlp_infer_rows_kernel(const float *__rest
         }
     }
 }
+};
 } // namespace
 
 int mlp_grid_blocks(int64_t batch)
@@ -436,16 +463,16 @@ hipError_t launch_mlp_loss_grad(const MlpState &m, const float *rows, int64_t co
                                 uint64_t mult, uint64_t off, hipStream_t st)
 {
     int nb = mlp_grid_blocks(batch);
-    hipLaunchKernelGGL(mlp_fwdbwd_kernel, dim3(nb), dim3(FT), 0, st, (const float *)m.theta, (const MlpNorm *)m.norm,
+    kl<mlp_fwdbwd_kernel>(dim3(nb), dim3(FT), 0, st, (const float *)m.theta, (const MlpNorm *)m.norm,
                        rows, count, first, batch, mult, off, 1.0f / (float)batch, m.gpart, m.lpart);
-    hipLaunchKernelGGL(mlp_reduce_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, (const float *)m.gpart,
+    kl<mlp_reduce_kernel>(dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, (const float *)m.gpart,
                        (const double *)m.lpart, nb, m.grad, m.loss, 1.0 / (double)batch);
     return hipGetLastError();
 }
 
 hipError_t launch_mlp_adam(const MlpState &m, int64_t step, float lr, hipStream_t st)
 {
-    hipLaunchKernelGGL(mlp_adam_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, m.theta,
+    kl<mlp_adam_kernel>(dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, m.theta,
                        (const float *)m.grad, m.m, m.v, step, lr);
     return hipGetLastError();
 }
@@ -453,7 +480,7 @@ hipError_t launch_mlp_adam(const MlpState &m, int64_t step, float lr, hipStream_
 hipError_t launch_mlp_predict(const MlpState &m, const float *x, int64_t n, float *y, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_predict_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+    kl<mlp_predict_kernel>(dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                        (const float *)m.theta, (const MlpNorm *)m.norm, x, n, y);
     return hipGetLastError();
 }
@@ -468,7 +495,7 @@ hipError_t launch_mlp_infer_stars(const DevFields &f, DevScalars *s, const KPara
                                   float2 *pred, int64_t n_ub, hipStream_t st)
 {
     if (n_ub <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_infer_stars_kernel, dim3(mfma_blocks((n_ub + 15) / 16)), dim3(256), 0, st, f, s, P,
+    kl<mlp_infer_stars_kernel>(dim3(mfma_blocks((n_ub + 15) / 16)), dim3(256), 0, st, f, s, P,
                        (const float *)m.theta, (const MlpNorm *)m.norm, pred);
     return hipGetLastError();
 }
@@ -476,7 +503,7 @@ hipError_t launch_mlp_infer_stars(const DevFields &f, DevScalars *s, const KPara
 hipError_t launch_mlp_infer_rows(const MlpState &m, const float *x, int64_t n, float *y, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_infer_rows_kernel, dim3(mfma_blocks((n + 15) / 16)), dim3(256), 0, st,
+    kl<mlp_infer_rows_kernel>(dim3(mfma_blocks((n + 15) / 16)), dim3(256), 0, st,
                        (const float *)m.theta, (const MlpNorm *)m.norm, x, n, y);
     return hipGetLastError();
 }
@@ -485,8 +512,8 @@ hipError_t launch_mlp_fit_norm(const MlpState &m, int64_t count, hipStream_t st)
 {
     if (count <= 0) return hipErrorInvalidValue;
     int nb = 256;
-    hipLaunchKernelGGL(mlp_colsum_kernel, dim3(nb), dim3(256), 0, st, (const float *)m.rows, count, m.colsum);
-    hipLaunchKernelGGL(mlp_norm_kernel, dim3(1), dim3(64), 0, st, (const double *)m.colsum, nb, count, m.norm);
+    kl<mlp_colsum_kernel>(dim3(nb), dim3(256), 0, st, (const float *)m.rows, count, m.colsum);
+    kl<mlp_norm_kernel>(dim3(1), dim3(64), 0, st, (const double *)m.colsum, nb, count, m.norm);
     return hipGetLastError();
 }
 
@@ -494,7 +521,7 @@ hipError_t launch_mlp_harvest(const DevFields &f, DevScalars *s, const KParams &
                               hipStream_t st)
 {
     if (n_ub <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_harvest_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f, s, P, m.rows,
+    kl<mlp_harvest_kernel>(dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f, s, P, m.rows,
                        m.cap, m.counter);
     return hipGetLastError();
 }

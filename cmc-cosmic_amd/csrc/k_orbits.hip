@@ -340,7 +340,9 @@ __device__ void wave_tries(const N &nd, const Coarse &C, const KParams &P, int64
 // MINB: blocks per CU the register allocation must allow (1: no constraint; 1024 /
 // THREADS: four waves per SIMD, at the cost of some spilled registers)
 template <int THREADS, int COARSE, int CAP, int MINB>
-__global__ __launch_bounds__(THREADS, MINB) void orbits_kernel(DevFields f, DevScalars *s, KParams P,
+struct orbits_kernel {
+    static constexpr int LB = THREADS, MB = MINB;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
                                                              const RngTable *__restrict__ T,
                                                              const Rng *__restrict__ wave_base,
                                                              double *__restrict__ partials,
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(THREADS, MINB) void orbits_kernel(DevFields f, DevS
         for (int q = threadIdx.x; q < NQ; q += blockDim.x) dst[q] = src[q];
     }
     __syncthreads();
-    const int64_t j = 1 + xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t j = 1 + xcd_block(kg_) * blockDim.x + threadIdx.x;
     Rng rng{};
     if (j <= n) rng = rng_jump_lds<ORB_NIB_POWERS>(sh.lt, wave_base[(j - 1) >> 6], (uint32_t)((j - 1) & 63));
     __syncthreads();                           // the tables are dead from here
@@ -448,8 +450,9 @@ __global__ __launch_bounds__(THREADS, MINB) void orbits_kernel(DevFields f, DevS
         f.ntry[j] = ntry;
         count_star(red, flags, ntry, hits, misses, true);
     }
-    block_reduce_store<THREADS>(red, partials + RED_STRIDE * xcd_block());
+    block_reduce_store<THREADS>(red, partials + RED_STRIDE * xcd_block(kg_));
 }
+};
 
 // Deferred stars (one block of DEFER_T threads): order the list by star index, so
 // the reduction of their sums does not depend on the order in which they were
@@ -457,7 +460,9 @@ __global__ __launch_bounds__(THREADS, MINB) void orbits_kernel(DevFields f, DevS
 // thread; a longer one by a bitonic sort over the next power of two (the list's
 // capacity MAX_DEFER is one), padded with INT64_MAX, in global memory.
 constexpr int DEFER_T = 1024;
-__global__ __launch_bounds__(DEFER_T) void defer_prepare_kernel(DevScalars *s, int64_t *__restrict__ list,
+struct defer_prepare_kernel {
+    static constexpr int LB = DEFER_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, int64_t *__restrict__ list,
                                                                 int64_t *__restrict__ kfirst)
 {
     const int t = threadIdx.x;
@@ -490,30 +495,36 @@ __global__ __launch_bounds__(DEFER_T) void defer_prepare_kernel(DevScalars *s, i
     const int64_t kf = s->n_max + 2;
     for (int64_t d = t; d < nd; d += DEFER_T) kfirst[d] = kf;
 }
+};
 
 // Grid-wide equivalent of the linear scan: the first k in [0, N_MAX+1] with Q > 0.
-__global__ __launch_bounds__(256) void defer_search_kernel(DevFields f, DevScalars *s, KParams P,
+struct defer_search_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
                                                            const int64_t *__restrict__ list,
                                                            int64_t *__restrict__ kfirst)
 {
     const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
     if (nd == 0) return;
     const int64_t n = s->n_max;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t d = blockIdx.y; d < nd; d += gridDim.y) {
+    const int64_t stride = (int64_t)kg_.nb * blockDim.x;
+    for (int64_t d = kg_.by; d < nd; d += kg_.nby) {
         const int64_t j = list[d];
         const double rj = f.r[j], mj = f.m[j];
         const double E = f.E[j] + phi_s_own(rj, mj, P.inv_nstar, rj);
         const double J = f.J[j];
-        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n + 1; k += stride) {
+        for (int64_t k = (int64_t)kg_.bid * blockDim.x + threadIdx.x; k <= n + 1; k += stride) {
             if (k > (int64_t)kfirst[d]) break;
             if (function_q(rj, mj, P.inv_nstar, f.r[k], f.phi[k], E, J) > 0.0)
                 atomicMin((unsigned long long *)&kfirst[d], (unsigned long long)k);
         }
     }
 }
+};
 
-__global__ __launch_bounds__(ORB_THREADS) void defer_tail_kernel(DevFields f, DevScalars *s, KParams P,
+struct defer_tail_kernel {
+    static constexpr int LB = ORB_THREADS, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
                                                                  const RngTable *__restrict__ T,
                                                                  const Rng *__restrict__ base, int jb1,
                                                                  const int64_t *__restrict__ list,
@@ -540,18 +551,21 @@ __global__ __launch_bounds__(ORB_THREADS) void defer_tail_kernel(DevFields f, De
         for (int i = 0; i < NRED; i++) dpartials[RED_STRIDE * d + i] = red[i];
     }
 }
+};
 
 // First level of the orbit phase's reduction: block B of RED2_BLOCKS sums the partial
 // records of sampler blocks [B C, (B + 1) C), C = ceil(nblocks / RED2_BLOCKS); 16
 // consecutive threads read one record (coalesced), its 64 groups take records
 // g, g + 64, ... of the range, and the groups are combined in group order.
-__global__ __launch_bounds__(1024) void orbits_part_kernel(const double *__restrict__ partials, int64_t nblocks,
+struct orbits_part_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ partials, int64_t nblocks,
                                                            double *__restrict__ out)
 {
     __shared__ double gsum[64][RED_STRIDE];
     const int t = threadIdx.x, i = t & 15, g = t >> 4;
     const int64_t C = (nblocks + RED2_BLOCKS - 1) / RED2_BLOCKS;
-    const int64_t b0 = (int64_t)blockIdx.x * C, b1 = b0 + C < nblocks ? b0 + C : nblocks;
+    const int64_t b0 = (int64_t)kg_.bid * C, b1 = b0 + C < nblocks ? b0 + C : nblocks;
     double acc = 0.0;
     if (i < NRED)
         for (int64_t b = b0 + g; b < b1; b += 64) {
@@ -563,13 +577,16 @@ __global__ __launch_bounds__(1024) void orbits_part_kernel(const double *__restr
     if (t < NRED) {
         double a = gsum[0][t];
         for (int g2 = 1; g2 < 64; g2++) a = red_is_max(t) ? (a >= gsum[g2][t] ? a : gsum[g2][t]) : a + gsum[g2][t];
-        out[RED_STRIDE * blockIdx.x + t] = a;
+        out[RED_STRIDE * kg_.bid + t] = a;
     }
 }
+};
 
 // Sums the first level's RED2_BLOCKS records in block order, then the deferred stars'
 // in star order, and finishes new_orbits_calculate (a fixed order: deterministic sums).
-__global__ __launch_bounds__(RED2_BLOCKS * RED_STRIDE) void orbits_end_kernel(DevScalars *s,
+struct orbits_end_kernel {
+    static constexpr int LB = RED2_BLOCKS * RED_STRIDE, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s,
                                                                               const double *__restrict__ red2,
                                                                               const double *__restrict__ dpartials)
 {
@@ -611,12 +628,15 @@ __global__ __launch_bounds__(RED2_BLOCKS * RED_STRIDE) void orbits_end_kernel(De
         s->n_bracket_fallback = (int64_t)v[14];
     }
 }
+};
 
-__global__ void apsides_kernel(DevFields f, DevScalars *s, KParams P, int64_t first, int64_t count,
+struct apsides_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, int64_t first, int64_t count,
                                const double *__restrict__ E, const double *__restrict__ J,
                                double *__restrict__ out4, int64_t *__restrict__ kout, int32_t *__restrict__ circ)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i >= count) return;
     int64_t j = first + i, n = s->n_max;
     OrbitRS o = calc_orbit(f.r, f.phi, n, j, f.r[j], f.m[j], P, E[i], J[i]);
@@ -625,13 +645,16 @@ __global__ void apsides_kernel(DevFields f, DevScalars *s, KParams P, int64_t fi
     kout[2 * i + 0] = o.kmin; kout[2 * i + 1] = o.kmax;
     circ[i] = o.circular;
 }
+};
 
 // The same per-star apsides written to one array per output (for callers that copy
 // each output straight into its own host array).
-__global__ void apsides_soa_kernel(DevFields f, DevScalars *s, KParams P, int64_t first, int64_t count,
+struct apsides_soa_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, int64_t first, int64_t count,
                                    const double *__restrict__ E, const double *__restrict__ J, ApsidesOut o)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i >= count) return;
     int64_t j = first + i, n = s->n_max;
     OrbitRS r = calc_orbit(f.r, f.phi, n, j, f.r[j], f.m[j], P, E[i], J[i]);
@@ -640,57 +663,72 @@ __global__ void apsides_soa_kernel(DevFields f, DevScalars *s, KParams P, int64_
     o.kmin[i] = r.kmin; o.kmax[i] = r.kmax;
     o.circular[i] = r.circular;
 }
+};
 
-__global__ void fzq_kernel(DevFields f, DevScalars *s, KParams P, int64_t count, const int64_t *__restrict__ jj,
+struct fzq_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, int64_t count, const int64_t *__restrict__ jj,
                            const int64_t *__restrict__ kmin, const int64_t *__restrict__ kmax,
                            const double *__restrict__ E, const double *__restrict__ J, int64_t *__restrict__ out)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i >= count) return;
     int64_t j = jj[i];
     int64_t k = find_zero_Q(f.r, f.phi, f.r[j], f.m[j], P.inv_nstar, kmin[i], kmax[i], E[i], J[i]);
     if (k < 0) dev_set_error(s, CMCG_E_BRACKET, j);
     out[i] = k;
 }
+};
 
-__global__ void frv_kernel(DevFields f, DevScalars *s, KParams P, int64_t count, const int64_t *__restrict__ jj,
+struct frv_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, int64_t count, const int64_t *__restrict__ jj,
                            const int64_t *__restrict__ kk, const double *__restrict__ E,
                            const double *__restrict__ J, double *__restrict__ out)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i >= count) return;
     int64_t j = jj[i];
     const RootVr rv = find_root_vr(f.r, f.phi, f.r[j], f.m[j], P.inv_nstar, P.cenma_m * P.madhoc, kk[i], E[i], J[i]);
     out[i] = rv.root;
     if (rv.bad) dev_set_error(s, CMCG_E_BRENT, j);
 }
+};
 
-__global__ void pred_from_apsides_kernel(DevFields f, DevScalars *s, float2 *__restrict__ pred)
+struct pred_from_apsides_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, float2 *__restrict__ pred)
 {
-    const int64_t j = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (j > s->n_max) return;
     const double rp = f.r_peri[j], ra = f.r_apo[j];
     pred[j] = make_float2(rp > 0.0 ? (float)log(rp) : 0.f, ra > 0.0 ? (float)log(ra) : 0.f);
 }
+};
 
 // ------------------------------------------------------------------ radial index
-__global__ void ridx_clear_kernel(uint32_t *__restrict__ t, int64_t count)
+struct ridx_clear_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, uint32_t *__restrict__ t, int64_t count)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x; i < count; i += (int64_t)kg_.nb * blockDim.x)
         t[i] = 0;
 }
+};
 
 // The packed (r, phi) node table with its every MID_STEP-th entry (nodes 0..n+1), and
 // the radial index's marks (nodes 0..n), in one pass over r and phi: node k (node 0
 // stands for "below every bucket") writes its index at its bucket when it is the
 // bucket's last node, and k at every multiple of RIDX_TILE in (bucket(r_k),
 // bucket(r_{k+1})] of the tile table.  The index array must be cleared before.
-__global__ void node_index_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
+struct node_index_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
                                   double2 *__restrict__ nodes, double2 *__restrict__ mid, uint32_t *__restrict__ t,
                                   uint32_t *__restrict__ tb, int bits)
 {
     const int64_t n = s->n_max;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (k > n + 1) return;
     const double rk = r[k];
     const double2 v = make_double2(rk, phi[k]);
@@ -711,16 +749,19 @@ __global__ void node_index_kernel(const double *__restrict__ r, const double *__
     }
     for (int64_t m = (bk + RIDX_TILE) / RIDX_TILE; m * RIDX_TILE <= bn; m++) tb[m] = (uint32_t)k;
 }
+};
 
 // t[b] = the largest marked index below b in its tile, or the tile's entry value:
 // an exclusive running maximum, in place.  16 buckets per thread (four 16-byte loads),
 // the threads' maxima scanned by wave shuffles and the four wave totals through LDS.
-__global__ __launch_bounds__(256) void ridx_scan_kernel(uint32_t *__restrict__ t, const uint32_t *__restrict__ tb,
+struct ridx_scan_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, uint32_t *__restrict__ t, const uint32_t *__restrict__ tb,
                                                         int bits)
 {
     __shared__ uint32_t wtot[4];
     const int64_t entries = ridx_entries(bits);
-    const int64_t i0 = (int64_t)blockIdx.x * RIDX_TILE + 16 * threadIdx.x;
+    const int64_t i0 = (int64_t)kg_.bid * RIDX_TILE + 16 * threadIdx.x;
     const bool whole = i0 + 16 <= entries;
     uint32_t v[16];
     if (whole) {
@@ -752,7 +793,7 @@ __global__ __launch_bounds__(256) void ridx_scan_kernel(uint32_t *__restrict__ t
     if (lane == 0) ex = 0u;
     if (lane == 63) wtot[w] = inc;
     __syncthreads();
-    uint32_t start = tb[blockIdx.x];
+    uint32_t start = tb[kg_.bid];
     for (int q = 0; q < w; q++) start = wtot[q] > start ? wtot[q] : start;
     start = ex > start ? ex : start;
 #pragma unroll
@@ -767,13 +808,16 @@ __global__ __launch_bounds__(256) void ridx_scan_kernel(uint32_t *__restrict__ t
             if (i0 + q < entries) t[i0 + q] = v[q];
     }
 }
+};
 
 // The LDS samples of the (r, phi) table for a COARSE-sample launch: node
 // min(s * stride, n + 1) for s = 0..cs, from the packed nodes.
-__global__ void coarse_build_kernel(const double2 *__restrict__ nodes, DevScalars *s, int cs,
+struct coarse_build_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double2 *__restrict__ nodes, DevScalars *s, int cs,
                                     double2 *__restrict__ out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = kg_.bid * blockDim.x + threadIdx.x;
     if (i == 0) {
         // new_orbits_calculate (cmc_utils.c:2198) and the counters of this call
         s->OldTidalMassLoss = s->TidalMassLoss_old;
@@ -785,6 +829,7 @@ __global__ void coarse_build_kernel(const double2 *__restrict__ nodes, DevScalar
     const int64_t k = (int64_t)i * st;
     out[i] = nodes[k < n + 1 ? k : n + 1];
 }
+};
 
 // Launch shapes of the orbit sampler that the autotuner chooses from: threads per
 // block, LDS samples of the (r, phi) table, and the most lanes one star's tries get
@@ -792,18 +837,26 @@ __global__ void coarse_build_kernel(const double2 *__restrict__ nodes, DevScalar
 // the same stars; only the block order of the escape sums differs.  (Asking the
 // compiler for 5 waves per SIMD instead of 4 spills about 100 VGPRs and ran 40%
 // slower.)
-using OrbFn = void (*)(DevFields, DevScalars *, KParams, const RngTable *, const Rng *, double *, int64_t *,
-                       const float2 *, float, const double2 *, const double2 *, const double2 *,
+using OrbFn = void (*)(dim3, dim3, hipStream_t, DevFields, DevScalars *, KParams, const RngTable *, const Rng *,
+                       double *, int64_t *, const float2 *, float, const double2 *, const double2 *, const double2 *,
                        const NibbleTables<ORB_NIB_POWERS> *, const uint32_t *, int);
+template <int THREADS, int COARSE, int CAP, int MINB>
+void orbits_launch(dim3 g, dim3 b, hipStream_t st, DevFields f, DevScalars *s, KParams P, const RngTable *T,
+                   const Rng *wb, double *partials, int64_t *defer, const float2 *pred, float margin,
+                   const double2 *mid, const double2 *nodes, const double2 *coarse,
+                   const NibbleTables<ORB_NIB_POWERS> *nib, const uint32_t *ridx, int ridx_bits)
+{
+    kl<orbits_kernel<THREADS, COARSE, CAP, MINB>>(g, b, 0, st, f, s, P, T, wb, partials, defer, pred, margin, mid, nodes,
+                                                  coarse, nib, ridx, ridx_bits);
+}
 struct OrbVariant { OrbFn fn; int threads, coarse, cap, minb; };
+#define ORB_V(T, C, K, M) {orbits_launch<T, C, K, M>, T, C, K, M}
 const OrbVariant kOrbVariants[] = {
-    {orbits_kernel<256, 1024, 64, 1>, 256, 1024, 64, 1}, {orbits_kernel<256, 512, 64, 1>, 256, 512, 64, 1},
-    {orbits_kernel<256, 2048, 64, 1>, 256, 2048, 64, 1}, {orbits_kernel<128, 1024, 64, 1>, 128, 1024, 64, 1},
-    {orbits_kernel<512, 1024, 64, 1>, 512, 1024, 64, 1}, {orbits_kernel<128, 512, 64, 1>, 128, 512, 64, 1},
-    {orbits_kernel<256, 1024, 2, 1>, 256, 1024, 2, 1},   {orbits_kernel<256, 1024, 4, 1>, 256, 1024, 4, 1},
-    {orbits_kernel<256, 1024, 8, 1>, 256, 1024, 8, 1},   {orbits_kernel<256, 512, 4, 1>, 256, 512, 4, 1},
-    {orbits_kernel<256, 1024, 4, 4>, 256, 1024, 4, 4},   {orbits_kernel<256, 1024, 8, 4>, 256, 1024, 8, 4},
-    {orbits_kernel<256, 512, 4, 4>, 256, 512, 4, 4}};
+    ORB_V(256, 1024, 64, 1), ORB_V(256, 512, 64, 1), ORB_V(256, 2048, 64, 1), ORB_V(128, 1024, 64, 1),
+    ORB_V(512, 1024, 64, 1), ORB_V(128, 512, 64, 1), ORB_V(256, 1024, 2, 1),  ORB_V(256, 1024, 4, 1),
+    ORB_V(256, 1024, 8, 1),  ORB_V(256, 512, 4, 1),  ORB_V(256, 1024, 4, 4),  ORB_V(256, 1024, 8, 4),
+    ORB_V(256, 512, 4, 4)};
+#undef ORB_V
 constexpr int ORB_NVARIANTS = sizeof(kOrbVariants) / sizeof(kOrbVariants[0]);
 
 int bits_needed(uint64_t x)
@@ -826,7 +879,7 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     if (hipError_t e = launch_rng_wave_bases(T, sc, 1, n_ub, st)) return e;
     if (hipError_t e = launch_node_index(f, s, sc, n_ub, st)) return e;
     if (V.coarse > ORB_MAX_COARSE) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(coarse_build_kernel, dim3((unsigned)((V.coarse + 256) / 256)), dim3(256), 0, st,
+    kl<coarse_build_kernel>(dim3((unsigned)((V.coarse + 256) / 256)), dim3(256), 0, st,
                        (const double2 *)sc.nodes, s, V.coarse, sc.coarse);
     const float2 *pred = nullptr;
     if (mlp) {
@@ -836,19 +889,19 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
         pred = sc.pred;
     }
     if (ev_a) hipEventRecord(ev_a, st);
-    hipLaunchKernelGGL(V.fn, dim3((unsigned)nblocks), dim3(V.threads), 0, st, f, s, P, T, (const Rng *)sc.rng_wave,
+    V.fn(dim3((unsigned)nblocks), dim3(V.threads), st, f, s, P, T, (const Rng *)sc.rng_wave,
                        sc.partials, sc.defer_list, pred, margin, (const double2 *)sc.mid,
                        (const double2 *)sc.nodes, (const double2 *)sc.coarse,
                        (const NibbleTables<ORB_NIB_POWERS> *)sc.orb_nib, (const uint32_t *)sc.ridx, sc.ridx_bits);
     if (ev_b) hipEventRecord(ev_b, st);
-    hipLaunchKernelGGL(defer_prepare_kernel, dim3(1), dim3(DEFER_T), 0, st, s, sc.defer_list, sc.defer_kfirst);
-    hipLaunchKernelGGL(defer_search_kernel, dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
+    kl<defer_prepare_kernel>(dim3(1), dim3(DEFER_T), 0, st, s, sc.defer_list, sc.defer_kfirst);
+    kl<defer_search_kernel>(dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
                        sc.defer_kfirst);
-    hipLaunchKernelGGL(defer_tail_kernel, dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,
+    kl<defer_tail_kernel>(dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,
                        (const int64_t *)sc.defer_list, (const int64_t *)sc.defer_kfirst, sc.defer_partials);
-    hipLaunchKernelGGL(orbits_part_kernel, dim3(RED2_BLOCKS), dim3(1024), 0, st, (const double *)sc.partials, nblocks,
+    kl<orbits_part_kernel>(dim3(RED2_BLOCKS), dim3(1024), 0, st, (const double *)sc.partials, nblocks,
                        sc.red2);
-    hipLaunchKernelGGL(orbits_end_kernel, dim3(1), dim3(RED2_BLOCKS * RED_STRIDE), 0, st, s, (const double *)sc.red2,
+    kl<orbits_end_kernel>(dim3(1), dim3(RED2_BLOCKS * RED_STRIDE), 0, st, s, (const double *)sc.red2,
                        (const double *)sc.defer_partials);
     return hipGetLastError();
 }
@@ -858,7 +911,7 @@ hipError_t launch_pred_from_apsides(const DevFields &f, DevScalars *s, const Scr
 {
     if (!sc.pred) return hipErrorInvalidValue;
     if (n_ub > 0)
-        hipLaunchKernelGGL(pred_from_apsides_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f, s,
+        kl<pred_from_apsides_kernel>(dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f, s,
                            sc.pred);
     return hipGetLastError();
 }
@@ -866,10 +919,10 @@ hipError_t launch_pred_from_apsides(const DevFields &f, DevScalars *s, const Scr
 hipError_t launch_node_index(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st)
 {
     const int64_t entries = ridx_entries(sc.ridx_bits);
-    hipLaunchKernelGGL(ridx_clear_kernel, dim3(512), dim3(256), 0, st, sc.ridx, entries);
-    hipLaunchKernelGGL(node_index_kernel, dim3((unsigned)((n_ub + 2 + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s,
+    kl<ridx_clear_kernel>(dim3(512), dim3(256), 0, st, sc.ridx, entries);
+    kl<node_index_kernel>(dim3((unsigned)((n_ub + 2 + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s,
                        (double2 *)sc.nodes, (double2 *)sc.mid, sc.ridx, sc.ridx_tb, sc.ridx_bits);
-    hipLaunchKernelGGL(ridx_scan_kernel, dim3((unsigned)ridx_tiles(sc.ridx_bits)), dim3(256), 0, st, sc.ridx,
+    kl<ridx_scan_kernel>(dim3((unsigned)ridx_tiles(sc.ridx_bits)), dim3(256), 0, st, sc.ridx,
                        (const uint32_t *)sc.ridx_tb, sc.ridx_bits);
     return hipGetLastError();
 }
@@ -888,7 +941,7 @@ hipError_t launch_apsides(const DevFields &f, DevScalars *s, const KParams &P, i
                           hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(apsides_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
+    kl<apsides_kernel>(dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
                        count, E, J, out4, kout, circ);
     return hipGetLastError();
 }
@@ -897,7 +950,7 @@ hipError_t launch_apsides_soa(const DevFields &f, DevScalars *s, const KParams &
                               const double *E, const double *J, const ApsidesOut &o, hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(apsides_soa_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
+    kl<apsides_soa_kernel>(dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, first,
                        count, E, J, o);
     return hipGetLastError();
 }
@@ -907,7 +960,7 @@ hipError_t launch_find_root_vr(const DevFields &f, DevScalars *s, const KParams 
                                hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(frv_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, count, j, k, E,
+    kl<frv_kernel>(dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, count, j, k, E,
                        J, out);
     return hipGetLastError();
 }
@@ -917,7 +970,7 @@ hipError_t launch_find_zero_Q(const DevFields &f, DevScalars *s, const KParams &
                               const double *E, const double *J, int64_t *out, hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fzq_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, count, j, kmin,
+    kl<fzq_kernel>(dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f, s, P, count, j, kmin,
                        kmax, E, J, out);
     return hipGetLastError();
 }

@@ -326,13 +326,13 @@ __device__ __forceinline__ double seg_value(int64_t sum, int bp)
 }
 
 // Tile sums of s*x (acc: this thread's share); block 0 also resets the chain's flags.
-__device__ void esa_tile_sums(const EsaArgs &a, int64_t L, double acc, double *lds)
+__device__ void esa_tile_sums(const EsaArgs &a, int64_t L, double acc, double *lds, uint32_t tile)
 {
     double tot;
     block_excl_sum(acc, lds, &tot);
     if (threadIdx.x == 0) {
-        a.tsumP[blockIdx.x] = tot;
-        if (blockIdx.x == 0) {
+        a.tsumP[tile] = tot;
+        if (tile == 0) {
             a.first_bad[0] = 0;
             a.first_bad[1] = L;
             *a.raw_n = 0;
@@ -341,7 +341,9 @@ __device__ void esa_tile_sums(const EsaArgs &a, int64_t L, double acc, double *l
 }
 
 // one block: exclusive prefixes of the nt tile sums, in tile order
-__global__ __launch_bounds__(WALK_T) void esa_tile_scan_kernel(EsaArgs a, int64_t nt)
+struct esa_tile_scan_kernel {
+    static constexpr int LB = WALK_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, EsaArgs a, int64_t nt)
 {
     __shared__ double lds[WALK_T / 64];
     const int64_t chunk = (nt + WALK_T - 1) / WALK_T;
@@ -355,19 +357,23 @@ __global__ __launch_bounds__(WALK_T) void esa_tile_scan_kernel(EsaArgs a, int64_
         run += a.tsumP[b];
     }
 }
+};
 
-__global__ __launch_bounds__(CT) void esa_sum_kernel(EsaArgs a)
+struct esa_sum_kernel {
+    static constexpr int LB = CT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, EsaArgs a)
 {
     __shared__ double lds[CT / 64];
     const int64_t L = *a.Lp;
-    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    const int64_t base = (int64_t)kg_.bid * CTILE;
     double acc = 0.0;
     for (int u = 0; u < CI; u++) {
         const int64_t j = base + u * CT + threadIdx.x;
         if (j < L) acc += a.sign * a.x[j];
     }
-    esa_tile_sums(a, L, acc, lds);
+    esa_tile_sums(a, L, acc, lds, kg_.bid);
 }
+};
 
 // The estimated prefixes of tile t and t + 1 (the sums of tsumP before them), by one
 // block: thread i sums the tiles i, i + CT, ... in order, and the threads' sums are
@@ -397,13 +403,15 @@ __device__ void tile_prefixes(const double *__restrict__ tsumP, int64_t t, doubl
 
 // self_prefix: form the tile prefixes here (and store this tile's in tpreP for the
 // later passes); otherwise read them from tpreP (esa_tile_scan_kernel).
-__global__ __launch_bounds__(CT) void esa_classify_kernel(EsaArgs a, int self_prefix)
+struct esa_classify_kernel {
+    static constexpr int LB = CT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, EsaArgs a, int self_prefix)
 {
     __shared__ double ldsd[CT / 64];
     __shared__ Seg ldss[CT / 64];
     __shared__ double tbuf[TILE_LDS];
     const int64_t L = *a.Lp;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = kg_.bid;
     if (tile * CTILE >= L) return;
     double pre_t, pre_t1;
     if (self_prefix) {
@@ -439,6 +447,7 @@ __global__ __launch_bounds__(CT) void esa_classify_kernel(EsaArgs a, int self_pr
         }
     }
 }
+};
 
 // The sequential part: one thread walks the sensitive elements in order from the
 // chain's start value, each getting fl(value before it + s x), the value before it
@@ -498,7 +507,9 @@ __device__ void esa_walk_block(const EsaArgs &a, int64_t ns, const double *lpre,
 // holds every sensitive element; otherwise (more than RAW_CAP, never for an ordinary
 // cluster) this block classifies the tiles again one after another and lists them in
 // order, slowly but exactly.
-__global__ __launch_bounds__(CT) void esa_walk_kernel(EsaArgs a)
+struct esa_walk_kernel {
+    static constexpr int LB = CT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, EsaArgs a)
 {
     __shared__ Seg lds[CT / 64];
     __shared__ int64_t ridx[RAW_CAP];
@@ -563,19 +574,22 @@ __global__ __launch_bounds__(CT) void esa_walk_kernel(EsaArgs a)
     __syncthreads();
     esa_walk_block(a, ns, lpre, lsx);
 }
+};
 
 // Parallel rebuild of every element from the last sensitive value before it,
 // verification of the recurrence against the values actually written (the value
 // before a tile is rebuilt exactly as the previous tile wrote it), and the first
 // violation.
-__global__ __launch_bounds__(CT) void esa_rebuild_kernel(EsaArgs a)
+struct esa_rebuild_kernel {
+    static constexpr int LB = CT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, EsaArgs a)
 {
     __shared__ double ldsd[CT / 64];
     __shared__ Seg ldss[CT / 64];
     __shared__ double lastv[CT];
     __shared__ double tbuf[TILE_LDS];
     const int64_t L = *a.Lp;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = kg_.bid;
     const int64_t base = tile * CTILE;
     if (base >= L) return;
     const double v0 = *a.v0p;
@@ -637,13 +651,16 @@ __global__ __launch_bounds__(CT) void esa_rebuild_kernel(EsaArgs a)
     if (bad < L) atomicMin((unsigned long long *)&a.first_bad[1], (unsigned long long)bad);
     store_tile(a.g, base, L, tbuf, nv);
 }
+};
 
 // The repair, only after a violation (the estimate's bound or the list's capacity
 // exceeded, which a monotone chain of ordinary size never does): the recurrence
 // itself from the first violating element on, one thread over chunks that the block
 // stages through LDS in coalesced rows.
 constexpr int FIN_CHUNK = 4 * WALK_T;
-__global__ __launch_bounds__(WALK_T) void chain_finish_kernel(EsaArgs a)
+struct chain_finish_kernel {
+    static constexpr int LB = WALK_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, EsaArgs a)
 {
     __shared__ double buf[FIN_CHUNK];
     __shared__ double vlast;
@@ -669,17 +686,21 @@ __global__ __launch_bounds__(WALK_T) void chain_finish_kernel(EsaArgs a)
         __syncthreads();
     }
 }
+};
 
 // ---------------------------------------------------------------- the three chains
-__global__ void pot_init_kernel(DevScalars *s) { s->first_inf = s->n_max + 1; }
+struct pot_init_kernel { static constexpr int LB = 1024, MB = 0; static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s) { s->first_inf = s->n_max + 1; } };
 
 // cmc_utils.c:685-696: the count stops at the first k with !(r[k] < SF_INFINITY).
-__global__ void pot_count_kernel(const double *__restrict__ r, DevScalars *s)
+struct pot_count_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, DevScalars *s)
 {
-    int64_t k = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t k = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (k > s->n_max) return;
     if (!(r[k] < CMCG_SF_INFINITY)) atomicMin((unsigned long long *)&s->first_inf, (unsigned long long)k);
 }
+};
 
 struct PotWork {
     double *xa, *ga;           // chain buffers (increments, values)
@@ -691,7 +712,9 @@ struct PotWork {
 
 // N_MAX from the count, the chain lengths, and the sentinels before the recurrence
 // reads r[N+1] (cmc_utils.c:713-714)
-__global__ void pot_setup_kernel(DevScalars *s, PotWork w, double *__restrict__ r, double *__restrict__ phi)
+struct pot_setup_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, PotWork w, double *__restrict__ r, double *__restrict__ phi)
 {
     int64_t n = s->first_inf - 1;
     s->n_max = n;
@@ -702,6 +725,7 @@ __global__ void pot_setup_kernel(DevScalars *s, PotWork w, double *__restrict__ 
     r[n + 1] = CMCG_SF_INFINITY;
     phi[n + 1] = 0.0;
 }
+};
 
 // after chain A: Mtotal, Rtidal (cmc_utils.c:703-707), NaN check (:688-691)
 __device__ __forceinline__ void pot_total(DevScalars *s, const KParams &P, const double *__restrict__ ga, PotWork &w)
@@ -718,15 +742,17 @@ __device__ __forceinline__ void pot_total(DevScalars *s, const KParams &P, const
 // chain B increments (star_m[k] / clus.N_STAR, k = N - j) and their tile sums; the
 // first thread also finishes chain A (Mtotal, the start value of chain B), which the
 // increments do not use
-__global__ __launch_bounds__(CT) void prep_b_kernel(const double *__restrict__ m, DevScalars *s, double nstar,
+struct prep_b_kernel {
+    static constexpr int LB = CT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ m, DevScalars *s, double nstar,
                                                     double *__restrict__ x, KParams P, const double *__restrict__ ga,
                                                     PotWork w, EsaArgs e)
 {
     __shared__ double lds[CT / 64];
-    if (blockIdx.x == 0 && threadIdx.x == 0) pot_total(s, P, ga, w);
+    if (kg_.bid == 0 && threadIdx.x == 0) pot_total(s, P, ga, w);
     const int64_t n = s->n_max;
     const int64_t L = n > 1 ? n - 1 : 0;
-    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    const int64_t base = (int64_t)kg_.bid * CTILE;
     double acc = 0.0;
     for (int u = 0; u < CI; u++) {
         const int64_t j = base + u * CT + threadIdx.x;
@@ -736,17 +762,20 @@ __global__ __launch_bounds__(CT) void prep_b_kernel(const double *__restrict__ m
             acc += e.sign * v;
         }
     }
-    esa_tile_sums(e, L, acc, lds);
+    esa_tile_sums(e, L, acc, lds, kg_.bid);
 }
+};
 
 // p_k = M_k * (1/r_k - 1/r_{k+1}) with the reference's roundings, and their tile sums;
 // chain C runs on w = -phi, w_k = fl(w_{k+1} + p_k)
-__global__ __launch_bounds__(CT) void prep_c_kernel(const double *__restrict__ r, DevScalars *s,
+struct prep_c_kernel {
+    static constexpr int LB = CT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, DevScalars *s,
                                                     const double *__restrict__ Mk, double *__restrict__ x, EsaArgs e)
 {
     __shared__ double lds[CT / 64];
     const int64_t n = s->n_max;
-    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    const int64_t base = (int64_t)kg_.bid * CTILE;
     double acc = 0.0;
     for (int u = 0; u < CI; u++) {
         const int64_t j = base + u * CT + threadIdx.x;
@@ -758,16 +787,19 @@ __global__ __launch_bounds__(CT) void prep_c_kernel(const double *__restrict__ r
             acc += e.sign * v;
         }
     }
-    esa_tile_sums(e, n, acc, lds);
+    esa_tile_sums(e, n, acc, lds, kg_.bid);
 }
+};
 
 // phi[k] = -g[n-k] (k = 1..n) and phi[0] (cmc_utils.c:729); thread t handles node
 // k = t for k = 0..n+1 and also writes the packed (r, phi) node table and its every
 // MID_STEP-th entry, which the searches of set_velocities3 and the orbit phase read.
-__global__ void phi_out_kernel(const double *__restrict__ r, double *__restrict__ phi, DevScalars *s, KParams P,
+struct phi_out_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, double *__restrict__ phi, DevScalars *s, KParams P,
                                const double *__restrict__ g, double2 *__restrict__ nodes, double2 *__restrict__ mid)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max;
     if (k > n + 1) return;
     double p;
@@ -786,11 +818,14 @@ __global__ void phi_out_kernel(const double *__restrict__ r, double *__restrict_
     nodes[k] = v;
     if (k % MID_STEP == 0) mid[k / MID_STEP] = v;
 }
+};
 
-__global__ void pot_at_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
+struct pot_at_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
                               const double *__restrict__ x, double *__restrict__ out, int64_t count)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i >= count) return;
     int64_t n = s->n_max;
     bool bad = false;
@@ -798,6 +833,7 @@ __global__ void pot_at_kernel(const double *__restrict__ r, const double *__rest
     if (bad) { dev_set_error(s, CMCG_E_BRACKET, i); v = 0.0; }
     out[i] = v;
 }
+};
 
 // The chain's increments must be in e.x; `summed`: their tile sums are already formed
 // (by a producer that calls esa_tile_sums over the same grid).
@@ -806,13 +842,13 @@ hipError_t run_chain(const EsaArgs &e, int64_t n_ub, hipStream_t st, bool summed
     int64_t ntiles = (n_ub + CTILE - 1) / CTILE;
     if (ntiles < 1) ntiles = 1;
     dim3 gt((unsigned)ntiles), bt(CT);
-    if (!summed) hipLaunchKernelGGL(esa_sum_kernel, gt, bt, 0, st, e);
+    if (!summed) kl<esa_sum_kernel>(gt, bt, 0, st, e);
     const bool self_prefix = ntiles <= SELF_PREFIX_MAX;
-    if (!self_prefix) hipLaunchKernelGGL(esa_tile_scan_kernel, dim3(1), dim3(WALK_T), 0, st, e, ntiles);
-    hipLaunchKernelGGL(esa_classify_kernel, gt, bt, 0, st, e, (int)self_prefix);
-    hipLaunchKernelGGL(esa_walk_kernel, dim3(1), dim3(CT), 0, st, e);
-    hipLaunchKernelGGL(esa_rebuild_kernel, gt, bt, 0, st, e);
-    hipLaunchKernelGGL(chain_finish_kernel, dim3(1), dim3(WALK_T), 0, st, e);
+    if (!self_prefix) kl<esa_tile_scan_kernel>(dim3(1), dim3(WALK_T), 0, st, e, ntiles);
+    kl<esa_classify_kernel>(gt, bt, 0, st, e, (int)self_prefix);
+    kl<esa_walk_kernel>(dim3(1), dim3(CT), 0, st, e);
+    kl<esa_rebuild_kernel>(gt, bt, 0, st, e);
+    kl<chain_finish_kernel>(dim3(1), dim3(WALK_T), 0, st, e);
     return hipGetLastError();
 }
 
@@ -857,20 +893,25 @@ struct MassPcOut {
     int64_t reached;             // fractions assigned
 };
 
-__global__ void mpc_prep_m_kernel(const double *__restrict__ m, const DevScalars *s, double nstar,
+struct mpc_prep_m_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ m, const DevScalars *s, double nstar,
                                   double *__restrict__ x, int64_t *__restrict__ len)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max;
     if (j == 0) len[3] = n;
     if (j < n) x[j] = m[j + 1] / nstar;                   // star_m[k] / clus.N_STAR, k = j + 1
 }
+};
 
 // term 0: 0.5 m madhoc vr vr, 1: 0.5 m madhoc vt vt, 2: vr vr, 3: vt vt (left to right)
-__global__ void mpc_prep_v_kernel(const DevFields f, const DevScalars *s, double madhoc, int term,
+struct mpc_prep_v_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const DevFields f, const DevScalars *s, double madhoc, int term,
                                   double *__restrict__ x, int64_t *__restrict__ len)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max_new;
     if (j == 0) len[3] = n;
     if (j >= n) return;
@@ -884,6 +925,7 @@ __global__ void mpc_prep_v_kernel(const DevFields f, const DevScalars *s, double
     }
     x[j] = v;
 }
+};
 
 // x^3 from x*x exactly as a double-double, times x: within 2^-50 ulp of the correctly
 // rounded cube (pow(x, 3) of a correctly rounding libm)
@@ -896,11 +938,13 @@ __device__ __forceinline__ double cube_rn(double x)
 
 // the fractions' indices (one fraction per star at most, as the reference's loop),
 // from the enclosed-mass chain g[k-1] = mprev after star k
-__global__ void mpc_assign_kernel(const double *__restrict__ r, const DevScalars *s, const double *__restrict__ g,
+struct mpc_assign_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, const DevScalars *s, const double *__restrict__ g,
                                   const double *__restrict__ pc, int count, int bh_include, double mprev0,
                                   double itm, double minimum_r, MassPcOut *o)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (threadIdx.x != 0 || kg_.bid != 0) return;
     const double PI = 3.14159265358979323;                  // include/cmc/cmc.h:62
     const double Mtotal = s->Mtotal;
     const int64_t n = s->n_max;
@@ -932,20 +976,27 @@ __global__ void mpc_assign_kernel(const double *__restrict__ r, const DevScalars
     }
     o->reached = mc;
 }
+};
 
-__global__ void mpc_init_kernel(double *v0, double mprev0)
+struct mpc_init_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, double *v0, double mprev0)
 {
     v0[0] = mprev0;
     v0[1] = 0.0;
 }
+};
 
-__global__ void mpc_take_kernel(const double *__restrict__ g, MassPcOut *o, int term, int count)
+struct mpc_take_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ g, MassPcOut *o, int term, int count)
 {
     const int i = threadIdx.x;
     if (i >= count || i >= o->reached) return;
     const int64_t k = o->k[i];
     o->v[3 + term][i] = k > 0 ? g[k - 1] : 0.0;
 }
+};
 
 hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub,
                             const Scratch &sc, hipStream_t st)
@@ -964,9 +1015,9 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     const EsaArgs e = esa_args(sc, P.chain_k_drop);
     const unsigned g1 = (unsigned)((n_ub + 256) / 256);
     const dim3 gt((unsigned)(n_ub > 0 ? (n_ub + CTILE - 1) / CTILE : 1));
-    hipLaunchKernelGGL(pot_init_kernel, dim3(1), dim3(1), 0, st, s);
-    if (n_ub > 0) hipLaunchKernelGGL(pot_count_kernel, dim3(g1), dim3(256), 0, st, f.r, s);
-    hipLaunchKernelGGL(pot_setup_kernel, dim3(1), dim3(1), 0, st, s, w, f.r, f.phi);
+    kl<pot_init_kernel>(dim3(1), dim3(1), 0, st, s);
+    if (n_ub > 0) kl<pot_count_kernel>(dim3(g1), dim3(256), 0, st, f.r, s);
+    kl<pot_setup_kernel>(dim3(1), dim3(1), 0, st, s, w, f.r, f.phi);
     // chain A: total mass, straight over star_m[1..N]
     EsaArgs ae = e;
     ae.x = f.m + 1; ae.g = w.ga; ae.sign = 1.0; ae.v0p = w.scal + 0; ae.first_bad = w.fbad + 0; ae.Lp = w.len + 0;
@@ -974,15 +1025,15 @@ hipError_t launch_potential(const DevFields &f, DevScalars *s, const KParams &P,
     // chain B: enclosed mass, iterated directly in Mk (chain C's input)
     EsaArgs be = e;
     be.x = w.xa; be.g = w.Mk; be.sign = -1.0; be.v0p = w.scal + 1; be.first_bad = w.fbad + 2; be.Lp = w.len + 1;
-    hipLaunchKernelGGL(prep_b_kernel, gt, dim3(CT), 0, st, f.m, s, (double)P.n_star, w.xa, P, (const double *)w.ga, w,
+    kl<prep_b_kernel>(gt, dim3(CT), 0, st, f.m, s, (double)P.n_star, w.xa, P, (const double *)w.ga, w,
                        be);
     if (hipError_t err = run_chain(be, n_ub, st, true)) return err;
     // chain C: w = -phi
     EsaArgs ce = e;
     ce.x = w.xa; ce.g = w.ga; ce.sign = 1.0; ce.v0p = w.scal + 0; ce.first_bad = w.fbad + 4; ce.Lp = w.len + 2;
-    hipLaunchKernelGGL(prep_c_kernel, gt, dim3(CT), 0, st, f.r, s, (const double *)w.Mk, w.xa, ce);
+    kl<prep_c_kernel>(gt, dim3(CT), 0, st, f.r, s, (const double *)w.Mk, w.xa, ce);
     if (hipError_t err = run_chain(ce, n_ub, st, true)) return err;
-    hipLaunchKernelGGL(phi_out_kernel, dim3((unsigned)((n_ub + 2 + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s, P,
+    kl<phi_out_kernel>(dim3((unsigned)((n_ub + 2 + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s, P,
                        (const double *)w.ga, (double2 *)sc.nodes, (double2 *)sc.mid);
     return hipGetLastError();
 }
@@ -992,7 +1043,7 @@ hipError_t launch_potential_at(const DevFields &f, DevScalars *s, const KParams 
 {
     (void)P;
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pot_at_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s, x,
+    kl<pot_at_kernel>(dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s, x,
                        out, count);
     return hipGetLastError();
 }
@@ -1010,25 +1061,25 @@ hipError_t launch_mass_percent(const DevFields &f, DevScalars *s, const KParams 
     e.Lp = sc.chain_len + 3;
     double *v0 = sc.chain_scal + 2;              // start values: [2] mprev0, [3] 0
     const double mprev0 = bh_include ? P.cenma_m * P.madhoc : 0.0;   // cmc_utils.c:878
-    hipLaunchKernelGGL(mpc_init_kernel, dim3(1), dim3(1), 0, st, v0, mprev0);
+    kl<mpc_init_kernel>(dim3(1), dim3(1), 0, st, v0, mprev0);
     MassPcOut *o = (MassPcOut *)out;
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
     // chain M
-    hipLaunchKernelGGL(mpc_prep_m_kernel, dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
+    kl<mpc_prep_m_kernel>(dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
                        sc.chain_x, sc.chain_len);
     EsaArgs em = e;
     em.v0p = v0;
     if (hipError_t err = run_chain(em, n_ub, st)) return err;
-    hipLaunchKernelGGL(mpc_assign_kernel, dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
+    kl<mpc_assign_kernel>(dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
                        (const double *)sc.chain_g, pc, count, bh_include, mprev0, itm, minimum_r, o);
     // the four running sums
     for (int term = 0; term < 4; term++) {
-        hipLaunchKernelGGL(mpc_prep_v_kernel, dim3(g1), dim3(256), 0, st, f, (const DevScalars *)s, P.madhoc, term,
+        kl<mpc_prep_v_kernel>(dim3(g1), dim3(256), 0, st, f, (const DevScalars *)s, P.madhoc, term,
                            sc.chain_x, sc.chain_len);
         EsaArgs ev = e;
         ev.v0p = v0 + 1;
         if (hipError_t err = run_chain(ev, n_ub, st)) return err;
-        hipLaunchKernelGGL(mpc_take_kernel, dim3(1), dim3(MPC_MAX), 0, st, (const double *)sc.chain_g, o, term, count);
+        kl<mpc_take_kernel>(dim3(1), dim3(MPC_MAX), 0, st, (const double *)sc.chain_g, o, term, count);
     }
     return hipGetLastError();
 }
@@ -1083,9 +1134,11 @@ __device__ __forceinline__ double cen_term(const DevFields &f, double madhoc, in
     }
 }
 
-__global__ void cen_init_kernel(CentralOut *o, double *v0)
+struct cen_init_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, CentralOut *o, double *v0)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (threadIdx.x != 0 || kg_.bid != 0) return;
     o->nave = 0;
     o->error = CEN_OK;
     for (int t = 0; t < CEN_TERMS; t++) {
@@ -1096,13 +1149,16 @@ __global__ void cen_init_kernel(CentralOut *o, double *v0)
     v0[0] = 0.0;
     v0[1] = 0.0;
 }
+};
 
 // nave and rh from the chain g[k-1] = m after star k (cmc_utils.c:1247-1257, 1620-1623);
 // the chain length for the sums becomes nave
-__global__ void cen_nave_kernel(const double *__restrict__ r, const DevScalars *s, const double *__restrict__ g,
+struct cen_nave_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, const DevScalars *s, const double *__restrict__ g,
                                 int64_t n_star, CentralOut *o, int64_t *len)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (threadIdx.x != 0 || kg_.bid != 0) return;
     const int64_t n = s->n_max;
     const double thr = 0.5 * s->Mtotal;
     int64_t lo = 1, hi = n + 1;                 // smallest k with !(g[k-1] < thr); g non-decreasing
@@ -1123,13 +1179,16 @@ __global__ void cen_nave_kernel(const double *__restrict__ r, const DevScalars *
     o->rh = r[nave];
     len[3] = nave;
 }
+};
 
-__global__ __launch_bounds__(256) void cen_max_kernel(const DevFields f, double madhoc, CentralOut *o)
+struct cen_max_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const DevFields f, double madhoc, CentralOut *o)
 {
     __shared__ unsigned long long red[CEN_TERMS][256 / 64];
     const int64_t nave = o->nave;
     unsigned long long mx[CEN_TERMS] = {0, 0, 0, 0, 0, 0};
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nave; j += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t j = (int64_t)kg_.bid * blockDim.x + threadIdx.x; j < nave; j += (int64_t)kg_.nb * blockDim.x)
         for (int t = 0; t < CEN_TERMS; t++) {
             const unsigned long long b = (unsigned long long)__double_as_longlong(cen_term(f, madhoc, j + 1, t));
             mx[t] = b > mx[t] ? b : mx[t];      // a negative or NaN term has the largest patterns
@@ -1150,10 +1209,13 @@ __global__ __launch_bounds__(256) void cen_max_kernel(const DevFields f, double 
         if (v) atomicMax(&o->mx[threadIdx.x], v);
     }
 }
+};
 
-__global__ void cen_scale_kernel(CentralOut *o)
+struct cen_scale_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, CentralOut *o)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || o->error) return;
+    if (threadIdx.x != 0 || kg_.bid != 0 || o->error) return;
     for (int t = 0; t < CEN_TERMS; t++) {
         if (o->mx[t] >= 0x7ff0000000000000ull) {   // infinite, NaN or negative
             o->error = CEN_E_RANGE;
@@ -1168,10 +1230,13 @@ __global__ void cen_scale_kernel(CentralOut *o)
         }
     }
 }
+};
 
-__global__ void cen_prep_kernel(const DevFields f, double madhoc, int t, CentralOut *o, double *__restrict__ x)
+struct cen_prep_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const DevFields f, double madhoc, int t, CentralOut *o, double *__restrict__ x)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t nave = o->nave;
     if (o->error) {
         if (j == 0) x[0] = 0.0;
@@ -1182,17 +1247,23 @@ __global__ void cen_prep_kernel(const DevFields f, double madhoc, int t, Central
     if (v != 0.0 && v < 2.2250738585072014e-308) o->error = CEN_E_RANGE;   // scaled into subnormals
     x[j] = v;
 }
+};
 
-__global__ void cen_take_kernel(const double *__restrict__ g, CentralOut *o, int t)
+struct cen_take_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ g, CentralOut *o, int t)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || o->error) return;
+    if (threadIdx.x != 0 || kg_.bid != 0 || o->error) return;
     o->S[t] = g[o->nave - 1] * o->unscale[t];
 }
+};
 
 // the averages and derived quantities, cmc_utils.c:1323-1345
-__global__ void cen_final_kernel(CentralOut *o)
+struct cen_final_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, CentralOut *o)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || o->error) return;
+    if (threadIdx.x != 0 || kg_.bid != 0 || o->error) return;
     const double PI = 3.14159265358979323;
     const double rhojsum = o->S[0], rhoj2sum = o->S[1];
     double rho = o->S[1], v_rms = o->S[2], rc = o->S[3], rc_nb = o->S[4], m_ave = o->S[5];
@@ -1218,6 +1289,7 @@ __global__ void cen_final_kernel(CentralOut *o)
     o->m_ave = m_ave;
     o->n = n;
 }
+};
 
 hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, const Scratch &sc,
                           void *out, hipStream_t st)
@@ -1230,26 +1302,26 @@ hipError_t launch_central(const DevFields &f, DevScalars *s, const KParams &P, i
     double *v0 = sc.chain_scal + 2;              // both start values 0
     CentralOut *o = (CentralOut *)out;
     const unsigned g1 = (unsigned)((n_ub + 256) / 256);
-    hipLaunchKernelGGL(cen_init_kernel, dim3(1), dim3(1), 0, st, o, v0);
+    kl<cen_init_kernel>(dim3(1), dim3(1), 0, st, o, v0);
     // enclosed mass, star_m[k] / N_STAR from 0
-    hipLaunchKernelGGL(mpc_prep_m_kernel, dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
+    kl<mpc_prep_m_kernel>(dim3(g1), dim3(256), 0, st, f.m, (const DevScalars *)s, (double)P.n_star,
                        sc.chain_x, sc.chain_len);
     EsaArgs em = e;
     em.v0p = v0;
     if (hipError_t err = run_chain(em, n_ub, st)) return err;
-    hipLaunchKernelGGL(cen_nave_kernel, dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
+    kl<cen_nave_kernel>(dim3(1), dim3(1), 0, st, (const double *)f.r, (const DevScalars *)s,
                        (const double *)sc.chain_g, (int64_t)P.n_star, o, sc.chain_len);
     const unsigned gmax = g1 < 1024u ? g1 : 1024u;
-    hipLaunchKernelGGL(cen_max_kernel, dim3(gmax), dim3(256), 0, st, f, P.madhoc, o);
-    hipLaunchKernelGGL(cen_scale_kernel, dim3(1), dim3(1), 0, st, o);
+    kl<cen_max_kernel>(dim3(gmax), dim3(256), 0, st, f, P.madhoc, o);
+    kl<cen_scale_kernel>(dim3(1), dim3(1), 0, st, o);
     for (int t = 0; t < CEN_TERMS; t++) {
-        hipLaunchKernelGGL(cen_prep_kernel, dim3(g1), dim3(256), 0, st, f, P.madhoc, t, o, sc.chain_x);
+        kl<cen_prep_kernel>(dim3(g1), dim3(256), 0, st, f, P.madhoc, t, o, sc.chain_x);
         EsaArgs es = e;
         es.v0p = v0 + 1;
         if (hipError_t err = run_chain(es, n_ub, st)) return err;
-        hipLaunchKernelGGL(cen_take_kernel, dim3(1), dim3(1), 0, st, (const double *)sc.chain_g, o, t);
+        kl<cen_take_kernel>(dim3(1), dim3(1), 0, st, (const double *)sc.chain_g, o, t);
     }
-    hipLaunchKernelGGL(cen_final_kernel, dim3(1), dim3(1), 0, st, o);
+    kl<cen_final_kernel>(dim3(1), dim3(1), 0, st, o);
     return hipGetLastError();
 }
 

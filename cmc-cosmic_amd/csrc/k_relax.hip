@@ -44,12 +44,15 @@ __device__ __forceinline__ void rng_bases(const RngTable *__restrict__ T, DevSca
     b = rng_jump_block(T, b, 1ull, 70, 1, cols, &res);
     if (threadIdx.x == 0) base[1] = b;
 }
-__global__ __launch_bounds__(RB_T) void rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s,
+struct rng_bases_kernel {
+    static constexpr int LB = RB_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const RngTable *__restrict__ T, DevScalars *s,
                                                          const Rng *__restrict__ seeded, Rng *__restrict__ base,
                                                          int64_t step_override)
 {
     rng_bases(T, s, seeded, base, step_override);
 }
+};
 
 // state at the first pair (phase 0: offset 3 * 64 w) or slot (phase 1: offset
 // (1 + 64 w) 2^20) of wave w.  With w = 256 b + t (block b, thread t) the offset
@@ -57,7 +60,9 @@ __global__ __launch_bounds__(RB_T) void rng_bases_kernel(const RngTable *__restr
 // 3 t 2^6 (phase 0) or t 2^26 (phase 1), jumped through LDS nibble tables of
 // WB_NB consecutive powers; jumps by powers of one matrix commute.
 constexpr int WB_T = 256, WB_NB = RNG_WB_POWERS;
-__global__ __launch_bounds__(WB_T) void rng_wave_bases_kernel(const RngTable *__restrict__ T,
+struct rng_wave_bases_kernel {
+    static constexpr int LB = WB_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const RngTable *__restrict__ T,
                                                               const Rng *__restrict__ base, int phase,
                                                               int64_t nwaves, Rng *__restrict__ out, int jb1,
                                                               const NibbleTables<WB_NB> *__restrict__ nib)
@@ -66,7 +71,7 @@ __global__ __launch_bounds__(WB_T) void rng_wave_bases_kernel(const RngTable *__
     __shared__ uint32_t cols[64 * 128];
     __shared__ Rng res;
     const int t = threadIdx.x;
-    const uint64_t b = blockIdx.x;
+    const uint64_t b = kg_.bid;
     copy_nibble_tables<WB_NB>(nib, lt);     // powers 2^6.. (phase 0) or 2^26.. (phase 1), built on the host
     const Rng bb = phase == 0 ? rng_jump_block(T, base[0], 49152ull * b, 0, jb1, cols, &res)
                               : rng_jump_block(T, base[1], 1ull + (b << 14), 20, jb1 - 20, cols, &res);
@@ -74,11 +79,14 @@ __global__ __launch_bounds__(WB_T) void rng_wave_bases_kernel(const RngTable *__
     if (w >= nwaves) return;
     out[w] = rng_jump_lds<WB_NB>(lt, bb, phase == 0 ? 3u * (uint32_t)t : (uint32_t)t);
 }
+};
 
-__global__ void rng_draws_kernel(const RngTable *__restrict__ T, const Rng *__restrict__ base, int phase,
+struct rng_draws_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const RngTable *__restrict__ T, const Rng *__restrict__ base, int phase,
                                  int64_t first, int64_t count, int ndraw, double *__restrict__ out, int jb1)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i >= count) return;
     int64_t idx = first + i;
     Rng st;
@@ -86,8 +94,9 @@ __global__ void rng_draws_kernel(const RngTable *__restrict__ T, const Rng *__re
     else st = rng_jump(T, base[1], (uint64_t)idx << 20, 20, jb1);
     for (int d = 0; d < ndraw; d++) out[i * ndraw + d] = rng_dbl(st);
 }
+};
 
-__global__ void dtrel_init_kernel(DevScalars *s) { s->dtrel = __longlong_as_double(0x7ff0000000000000ll); }
+struct dtrel_init_kernel { static constexpr int LB = 1024, MB = 0; static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s) { s->dtrel = __longlong_as_double(0x7ff0000000000000ll); } };
 
 // simul_relax_new: one thread per window si = 1+p, 1+3p, ... (si < N - p), the
 // window sums taken in the reference's order, then an order-independent min: the
@@ -99,7 +108,9 @@ __global__ void dtrel_init_kernel(DevScalars *s) { s->dtrel = __longlong_as_doub
 // SR_T windows at most per block, one per lane of the first wave; all SR_B threads
 // stage the block's elements (more loads in flight than one wave issues alone).
 constexpr int SR_T = 64, SR_B = 256, SR_LDS = 2560;
-__global__ __launch_bounds__(SR_B) void simul_relax_kernel(const double *__restrict__ r, const double *__restrict__ m,
+struct simul_relax_kernel {
+    static constexpr int LB = SR_B, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, const double *__restrict__ m,
                                                            const double *__restrict__ vr,
                                                            const double *__restrict__ vt, DevScalars *s, KParams P)
 {
@@ -109,7 +120,7 @@ __global__ __launch_bounds__(SR_B) void simul_relax_kernel(const double *__restr
     const int64_t p = P.avekernel, N = s->n_max, len = 2 * p;
     const bool staged = len <= SR_LDS;
     const int G = staged ? (SR_LDS / len < SR_T ? (int)(SR_LDS / len) : SR_T) : SR_T;   // windows per block
-    const int64_t w0 = (int64_t)blockIdx.x * G;
+    const int64_t w0 = (int64_t)kg_.bid * G;
     const int64_t e0 = 1 + w0 * len;
     if (staged) {
         for (int64_t q = threadIdx.x; q < G * len; q += SR_B) {
@@ -160,6 +171,7 @@ __global__ __launch_bounds__(SR_B) void simul_relax_kernel(const double *__restr
     }
     if (threadIdx.x == 0 && mine != kInf) atomicMin((unsigned long long *)&s->dtrel, mine);   // windows: wave 0 only
 }
+};
 
 // calc_timestep (cmc_utils.c:2132-2148) and GetTimeStep's fallback (cmc_evolution_thr.c:430-432)
 __device__ __forceinline__ void timestep_finalize(DevScalars *s)
@@ -174,10 +186,12 @@ __device__ __forceinline__ void timestep_finalize(DevScalars *s)
     s->Dt = Dt;
     s->TotalTime += Dt;
 }
-__global__ void timestep_finalize_kernel(DevScalars *s) { timestep_finalize(s); }
+struct timestep_finalize_kernel { static constexpr int LB = 1024, MB = 0; static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s) { timestep_finalize(s); } };
 // the step's time step (thread 0; first the minimum over sigma_dt_kernel's slots, which
 // are reset for the next step, when dt_part is given), then its RNG substream bases
-__global__ __launch_bounds__(RB_T) void timestep_rng_bases_kernel(const RngTable *__restrict__ T, DevScalars *s,
+struct timestep_rng_bases_kernel {
+    static constexpr int LB = RB_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const RngTable *__restrict__ T, DevScalars *s,
                                                                   const Rng *__restrict__ seeded,
                                                                   Rng *__restrict__ base,
                                                                   unsigned long long *__restrict__ dt_part)
@@ -206,6 +220,7 @@ __global__ __launch_bounds__(RB_T) void timestep_rng_bases_kernel(const RngTable
     if (threadIdx.x == 0) timestep_finalize(s);
     rng_bases(T, s, seeded, base, -1);
 }
+};
 
 // calc_sigma_r at one rank: sigma over the clamped window [si-p, si+p-1]; the
 // reference slides running sums, here each window is summed directly.
@@ -214,13 +229,15 @@ __global__ __launch_bounds__(RB_T) void timestep_rng_bases_kernel(const RngTable
 // the same order and expression as sigma_kernel.
 constexpr int SIG_T = 256, SIG_MAXP = 64, SIG_LDS = SIG_T + 4 * SIG_MAXP + 1;   // 8 KB: eight blocks per CU
 
-__global__ __launch_bounds__(SIG_T) void sigma_lds_kernel(const double *__restrict__ m, const double *__restrict__ vr,
+struct sigma_lds_kernel {
+    static constexpr int LB = SIG_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ m, const double *__restrict__ vr,
                                                           const double *__restrict__ vt, double *__restrict__ sigma,
                                                           DevScalars *s, KParams P)
 {
     __shared__ double a[SIG_LDS], b[SIG_LDS];
     const int64_t N = s->n_max, p = P.avekernel;
-    const int64_t first = 1 + (int64_t)blockIdx.x * SIG_T;
+    const int64_t first = 1 + (int64_t)kg_.bid * SIG_T;
     if (first > N) return;
     const int64_t last = first + SIG_T - 1 < N ? first + SIG_T - 1 : N;
     const int64_t lo = first - 2 * p > 1 ? first - 2 * p : 1;
@@ -245,11 +262,14 @@ __global__ __launch_bounds__(SIG_T) void sigma_lds_kernel(const double *__restri
     }
     sigma[si] = sqrt(Mv2ave / Mave);
 }
+};
 
-__global__ void sigma_kernel(const double *__restrict__ m, const double *__restrict__ vr,
+struct sigma_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ m, const double *__restrict__ vr,
                              const double *__restrict__ vt, double *__restrict__ sigma, DevScalars *s, KParams P)
 {
-    int64_t si = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t si = 1 + (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t N = s->n_max, p = P.avekernel;
     if (si > N) return;
     int64_t simin = si - p, simax = simin + (2 * p - 1);
@@ -265,6 +285,7 @@ __global__ void sigma_kernel(const double *__restrict__ m, const double *__restr
     }
     sigma[si] = sqrt(Mv2ave / Mave);
 }
+};
 
 // calc_sigma_r and simul_relax_new in one pass (the step's form): each block stages
 // m*madhoc and m*madhoc*(vr^2 + vt^2) for its SIG_T stars and 2p on either side (the
@@ -276,7 +297,9 @@ __global__ void sigma_kernel(const double *__restrict__ m, const double *__restr
 // doubles order like their patterns; 0 is the identity, so the slots start and are
 // left at 0); the one-block time-step kernel takes the minimum over the slots.  Needs
 // p <= SIG_MAXP.
-__global__ __launch_bounds__(SIG_T) void sigma_dt_kernel(const double *__restrict__ r, const double *__restrict__ m,
+struct sigma_dt_kernel {
+    static constexpr int LB = SIG_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, const double *__restrict__ m,
                                                          const double *__restrict__ vr, const double *__restrict__ vt,
                                                          double *__restrict__ sigma, DevScalars *s, KParams P,
                                                          unsigned long long *__restrict__ dt_part)
@@ -285,7 +308,7 @@ __global__ __launch_bounds__(SIG_T) void sigma_dt_kernel(const double *__restric
     __shared__ unsigned long long wmin[SIG_T / 64];
     const unsigned long long kInf = 0x7ff0000000000000ull;
     const int64_t N = s->n_max, p = P.avekernel, len = 2 * p;
-    const int64_t first = 1 + (int64_t)blockIdx.x * SIG_T;
+    const int64_t first = 1 + (int64_t)kg_.bid * SIG_T;
     const int64_t last = first + SIG_T - 1 < N ? first + SIG_T - 1 : N;
     const int64_t lo = first - 2 * p > 1 ? first - 2 * p : 1;
     const int64_t hi = last + 2 * p < N ? last + 2 * p : N;
@@ -343,9 +366,10 @@ __global__ __launch_bounds__(SIG_T) void sigma_dt_kernel(const double *__restric
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int q = 1; q < SIG_T / 64; q++) mine = wmin[q] < mine ? wmin[q] : mine;
-        if (mine != kInf) atomicMax(&dt_part[(blockIdx.x % DT_SLOTS) * 16], ~mine);
+        if (mine != kInf) atomicMax(&dt_part[(kg_.bid % DT_SLOTS) * 16], ~mine);
     }
 }
+};
 
 // Pair pr draws from offset 3 pr; a wave's first pair starts at the per-wave state
 // and lane l jumps 3 l < 2^8 steps through the LDS tables.
@@ -362,13 +386,15 @@ __device__ __forceinline__ void ec1_star(DevFields &f, const KParams &P, int64_t
 // EC1: energy_conservation1 fused into the pass (the step's form; the entry point of
 // that name keeps its own kernel).
 template <bool EC1>
-__global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, KParams P,
+struct relax_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
                                                     const Rng *__restrict__ wave_base,
                                                     const NibbleTables<8> *__restrict__ nib)
 {
     __shared__ NibbleTables<8> lt;
     copy_nibble_tables<8>(nib, lt);
-    int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t pr = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     int64_t N = s->n_max;
     if (EC1 && pr == N / 2 && (N & 1)) {    // the unpaired last star keeps its velocities
         ec1_star(f, P, N, f.r[N], f.m[N], f.phi[N], f.vr[N], f.vt[N]);
@@ -436,6 +462,7 @@ __global__ __launch_bounds__(256) void relax_kernel(DevFields f, DevScalars *s, 
         ec1_star(f, P, kp, rkp, mass_kp, phikp, nvr_kp, nvt_kp);
     }
 }
+};
 
 int bits_needed(uint64_t x)
 {
@@ -447,13 +474,13 @@ int bits_needed(uint64_t x)
 
 hipError_t launch_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base, (int64_t)-1);
+    kl<rng_bases_kernel>(dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base, (int64_t)-1);
     return hipGetLastError();
 }
 
 hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch &sc, int64_t step, hipStream_t st)
 {
-    hipLaunchKernelGGL(rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base, step);
+    kl<rng_bases_kernel>(dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base, step);
     return hipGetLastError();
 }
 
@@ -464,26 +491,29 @@ hipError_t launch_rng_wave_bases(const RngTable *T, const Scratch &sc, int phase
     if (nwaves > sc.max_waves) return hipErrorInvalidValue;
     int jb1 = phase == 0 ? bits_needed(192ull * (uint64_t)nwaves) : 20 + bits_needed((uint64_t)(64 * nwaves + 1));
     const NibbleTables<WB_NB> *nib = (const NibbleTables<WB_NB> *)sc.wb_nib + (phase == 0 ? 0 : 1);
-    hipLaunchKernelGGL(rng_wave_bases_kernel, dim3((unsigned)((nwaves + WB_T - 1) / WB_T)), dim3(WB_T), 0, st, T,
+    kl<rng_wave_bases_kernel>(dim3((unsigned)((nwaves + WB_T - 1) / WB_T)), dim3(WB_T), 0, st, T,
                        (const Rng *)sc.rng_base, phase, nwaves, sc.rng_wave, jb1, nib);
     return hipGetLastError();
 }
 
 // Packed (r, phi) nodes for the searches and the middle table (every MID_STEP-th node).
-__global__ void mid_build_kernel(const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
+struct mid_build_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r, const double *__restrict__ phi, DevScalars *s,
                                  double2 *__restrict__ nodes, double2 *__restrict__ mid)
 {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t k = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (k > s->n_max + 1) return;
     double2 v = make_double2(r[k], phi[k]);
     nodes[k] = v;
     if (k % MID_STEP == 0) mid[k / MID_STEP] = v;
 }
+};
 
 hipError_t launch_mid_build(const DevFields &f, DevScalars *s, const Scratch &sc, int64_t n_ub, hipStream_t st)
 {
     int64_t nn = n_ub + 2;
-    hipLaunchKernelGGL(mid_build_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s,
+    kl<mid_build_kernel>(dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, f.r, f.phi, s,
                        sc.nodes, sc.mid);
     return hipGetLastError();
 }
@@ -494,7 +524,7 @@ hipError_t launch_rng_draws(const RngTable *T, const Rng *base, int phase, int64
     if (count <= 0) return hipSuccess;
     uint64_t last = (uint64_t)(first + count);
     int jb1 = phase == 0 ? bits_needed(3ull * last) : 20 + bits_needed(last);
-    hipLaunchKernelGGL(rng_draws_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, T, base, phase,
+    kl<rng_draws_kernel>(dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, T, base, phase,
                        first, count, ndraw, out, jb1);
     return hipGetLastError();
 }
@@ -503,10 +533,10 @@ hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int
 {
     if (n_ub <= 0) return hipSuccess;
     if (P.avekernel <= SIG_MAXP)
-        hipLaunchKernelGGL(sigma_lds_kernel, dim3((unsigned)((n_ub + SIG_T - 1) / SIG_T)), dim3(SIG_T), 0, st, f.m,
+        kl<sigma_lds_kernel>(dim3((unsigned)((n_ub + SIG_T - 1) / SIG_T)), dim3(SIG_T), 0, st, f.m,
                            f.vr, f.vt, f.sigma, s, P);
     else
-        hipLaunchKernelGGL(sigma_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f.m, f.vr, f.vt,
+        kl<sigma_kernel>(dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, st, f.m, f.vr, f.vt,
                            f.sigma, s, P);
     return hipGetLastError();
 }
@@ -514,18 +544,18 @@ hipError_t launch_sigma(const DevFields &f, DevScalars *s, const KParams &P, int
 hipError_t launch_simul_relax(const DevFields &f, DevScalars *s, const KParams &P, int64_t n_ub, hipStream_t st,
                               bool init_min)
 {
-    if (init_min) hipLaunchKernelGGL(dtrel_init_kernel, dim3(1), dim3(1), 0, st, s);
+    if (init_min) kl<dtrel_init_kernel>(dim3(1), dim3(1), 0, st, s);
     const int64_t nw = n_ub / (2 * P.avekernel) + 1;
     const int64_t len = 2 * P.avekernel;
     const int64_t G = len <= SR_LDS ? (SR_LDS / len < SR_T ? SR_LDS / len : SR_T) : SR_T;   // as in the kernel
-    hipLaunchKernelGGL(simul_relax_kernel, dim3((unsigned)((nw + G - 1) / G)), dim3(SR_B), 0, st, f.r, f.m, f.vr,
+    kl<simul_relax_kernel>(dim3((unsigned)((nw + G - 1) / G)), dim3(SR_B), 0, st, f.r, f.m, f.vr,
                        f.vt, s, P);
     return hipGetLastError();
 }
 
 hipError_t launch_timestep_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(timestep_rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base,
+    kl<timestep_rng_bases_kernel>(dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base,
                        (unsigned long long *)nullptr);
     return hipGetLastError();
 }
@@ -542,15 +572,15 @@ hipError_t launch_timestep_sigma(const DevFields &f, DevScalars *s, const KParam
     }
     const int64_t nb = (n_ub + SIG_T - 1) / SIG_T;
     if (nb > sc.max_blocks) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(sigma_dt_kernel, dim3((unsigned)nb), dim3(SIG_T), 0, st, f.r, f.m, f.vr, f.vt, f.sigma, s, P,
+    kl<sigma_dt_kernel>(dim3((unsigned)nb), dim3(SIG_T), 0, st, f.r, f.m, f.vr, f.vt, f.sigma, s, P,
                        sc.dt_part);
-    hipLaunchKernelGGL(timestep_rng_bases_kernel, dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base,
+    kl<timestep_rng_bases_kernel>(dim3(1), dim3(RB_T), 0, st, T, s, sc.rng_seeded, sc.rng_base,
                        sc.dt_part);
     return hipGetLastError();
 }
 hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st)
 {
-    hipLaunchKernelGGL(timestep_finalize_kernel, dim3(1), dim3(1), 0, st, s);
+    kl<timestep_finalize_kernel>(dim3(1), dim3(1), 0, st, s);
     return hipGetLastError();
 }
 
@@ -564,13 +594,13 @@ hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, con
         if (nth <= 0) return hipSuccess;
         if (npairs > 0)
             if (hipError_t e = launch_rng_wave_bases(T, sc, 0, npairs, st)) return e;
-        hipLaunchKernelGGL(relax_kernel<true>, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, st, f, s, P,
+        kl<relax_kernel<true>>(dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, st, f, s, P,
                            (const Rng *)sc.rng_wave, (const NibbleTables<8> *)sc.relax_nib);
         return hipGetLastError();
     }
     if (npairs <= 0) return hipSuccess;
     if (hipError_t e = launch_rng_wave_bases(T, sc, 0, npairs, st)) return e;
-    hipLaunchKernelGGL(relax_kernel<false>, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P,
+    kl<relax_kernel<false>>(dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P,
                        (const Rng *)sc.rng_wave, (const NibbleTables<8> *)sc.relax_nib);
     return hipGetLastError();
 }

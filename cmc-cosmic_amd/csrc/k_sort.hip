@@ -47,9 +47,11 @@ struct GatherList {
 };
 
 // zeroes the digit totals; its first thread also resets the sort's scalars
-__global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n, DevScalars *s, int64_t *n_long)
+struct rs_zero_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, uint32_t *__restrict__ a, int64_t n, DevScalars *s, int64_t *n_long)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     if (i == 0) {
         s->sort_n = 0;
         s->sort_in = s->n_max_new;
@@ -58,6 +60,7 @@ __global__ void rs_zero_kernel(uint32_t *__restrict__ a, int64_t n, DevScalars *
     }
     if (i < n) a[i] = 0;
 }
+};
 
 __device__ __forceinline__ uint32_t word_slot(uint64_t w) { return (uint32_t)w; }
 __device__ __forceinline__ uint32_t word_z(uint64_t w) { return (uint32_t)(w >> RS_SHIFT0); }
@@ -66,12 +69,14 @@ __device__ __forceinline__ uint32_t word_z(uint64_t w) { return (uint32_t)(w >> 
 // pair per block: mm[2b] = min, mm[2b + 1] = max (~0 and 0 for a block that saw none).
 // rs_prep_kernel combines the pairs; partials per block replace thousands of device-scope
 // atomics on one address arriving from every XCD.
-__global__ __launch_bounds__(256) void rs_minmax_kernel(const double *__restrict__ keysrc, DevScalars *s,
+struct rs_minmax_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ keysrc, DevScalars *s,
                                                         uint64_t kinf, unsigned long long *__restrict__ mm)
 {
     __shared__ unsigned long long rmin[4], rmax[4];
     const int64_t n = s->n_max_new;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t stride = (int64_t)kg_.nb * blockDim.x;
     unsigned long long lo = ~0ull, hi = 0ull;
     auto take = [&](uint64_t k) {
         if (k < kinf) {
@@ -79,7 +84,7 @@ __global__ __launch_bounds__(256) void rs_minmax_kernel(const double *__restrict
             hi = k > hi ? k : hi;
         }
     };
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     for (; i + 3 * stride < n; i += 4 * stride) {       // four independent loads in flight
         const uint64_t k0 = sortable(keysrc[i + 1]), k1 = sortable(keysrc[i + stride + 1]);
         const uint64_t k2 = sortable(keysrc[i + 2 * stride + 1]), k3 = sortable(keysrc[i + 3 * stride + 1]);
@@ -99,10 +104,11 @@ __global__ __launch_bounds__(256) void rs_minmax_kernel(const double *__restrict
             lo = rmin[q] < lo ? rmin[q] : lo;
             hi = rmax[q] > hi ? rmax[q] : hi;
         }
-        mm[2 * blockIdx.x] = lo;
-        mm[2 * blockIdx.x + 1] = hi;
+        mm[2 * kg_.bid] = lo;
+        mm[2 * kg_.bid + 1] = hi;
     }
 }
+};
 
 // Blocks of rs_minmax_kernel for a sort over `ntiles` tiles (the same count on both sides).
 __host__ __device__ __forceinline__ int minmax_parts(int64_t ntiles)
@@ -114,7 +120,9 @@ __host__ __device__ __forceinline__ int minmax_parts(int64_t ntiles)
 // fields of each slot, and r itself after them, are packed into one 128-byte record,
 // so the final gather reads one record per star instead of one cache line per field.
 template <int W>
-__global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__restrict__ keysrc, DevScalars *s,
+struct rs_prep_kernel {
+    static constexpr int LB = RS_THREADS, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ keysrc, DevScalars *s,
                                                              uint64_t *__restrict__ words,
                                                              uint32_t *__restrict__ digit_totals,
                                                              uint64_t *__restrict__ aos, GatherList g, uint64_t kinf,
@@ -130,7 +138,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
     __shared__ unsigned long long red[2][RS_THREADS / 64];
     {
         unsigned long long lo = ~0ull, hi = 0ull;
-        for (int q = t; q < minmax_parts(gridDim.x); q += RS_THREADS) {
+        for (int q = t; q < minmax_parts(kg_.nb); q += RS_THREADS) {
             const unsigned long long a = mm[2 * q], b = mm[2 * q + 1];
             lo = a < lo ? a : lo;
             hi = b > hi ? b : hi;
@@ -159,7 +167,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
     }
     __syncthreads();
     const int64_t n = s->n_max_new;
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = (int64_t)kg_.bid * RS_TILE;
     for (int u = 0; u < RS_ITEMS; u++) {
         const int64_t i0 = base + u * RS_THREADS;
         const int64_t i = i0 + t;
@@ -187,8 +195,11 @@ __global__ __launch_bounds__(RS_THREADS) void rs_prep_kernel(const double *__res
     for (int p = P0; p < 8; p++)
         if (h[p - P0][t]) atomicAdd(&digit_totals[p * 256 + t], h[p - P0][t]);
 }
+};
 
-__global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint64_t *__restrict__ keys, DevScalars *s,
+struct rs_hist_kernel {
+    static constexpr int LB = RS_THREADS, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const uint64_t *__restrict__ keys, DevScalars *s,
                                                              int shift, uint32_t *__restrict__ block_hist,
                                                              int64_t ntiles)
 {
@@ -197,23 +208,26 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint64_t *__r
     h[t] = 0;
     __syncthreads();
     const int64_t n = s->n_max_new;
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = (int64_t)kg_.bid * RS_TILE;
     for (int u = 0; u < RS_ITEMS; u++) {
         int64_t i = base + u * RS_THREADS + t;
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
     }
     __syncthreads();
-    block_hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
+    block_hist[(int64_t)t * ntiles + kg_.bid] = h[t];
 }
+};
 
 // One block per digit: exclusive scan of that digit's tile counts, plus the digit's
 // global base (sum of the lower digits' totals for this pass).
-__global__ __launch_bounds__(RS_THREADS) void rs_scan_kernel(uint32_t *__restrict__ block_hist, int64_t ntiles,
+struct rs_scan_kernel {
+    static constexpr int LB = RS_THREADS, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, uint32_t *__restrict__ block_hist, int64_t ntiles,
                                                              const uint32_t *__restrict__ totals)
 {
     __shared__ uint32_t red[RS_THREADS];
     const int t = threadIdx.x;
-    const int d = blockIdx.x;
+    const int d = kg_.bid;
     red[t] = t < d ? totals[t] : 0u;
     __syncthreads();
     for (int off = RS_THREADS / 2; off > 0; off >>= 1) {
@@ -243,6 +257,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scan_kernel(uint32_t *__restric
         run += c;
     }
 }
+};
 
 // Stable scatter of one 4096-word tile.  Wave w ranks the contiguous sub-tile
 // [w*1024, (w+1)*1024) on its own (ballot matching of the 8 digit bits, a per-wave
@@ -250,7 +265,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scan_kernel(uint32_t *__restric
 // item loop needs no block barrier; the (digit, wave) counts then give each wave's
 // offset inside the tile's digit runs.  Index order (wave, item, lane) is the tile's
 // index order, so equal digits keep their order.
-__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *__restrict__ words_in,
+struct rs_scatter_kernel {
+    static constexpr int LB = RS_THREADS, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const uint64_t *__restrict__ words_in,
                                                                 uint64_t *__restrict__ words_out, DevScalars *s,
                                                                 int shift, const uint32_t *__restrict__ tile_off,
                                                                 int64_t ntiles)
@@ -261,7 +278,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
     __shared__ uint32_t dstart[256];       // 37 KB in all: four blocks per CU
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t n = s->n_max_new;
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = (int64_t)kg_.bid * RS_TILE;
     const int64_t wbase = base + (int64_t)w * (64 * RS_ITEMS);
     for (int d = lane; d < 256; d += 64) wcnt[w][d] = 0;
     uint64_t k[RS_ITEMS];
@@ -319,11 +336,12 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t *
         if (p < cnt_tile) {
             uint64_t word = swords[p];
             uint32_t d = (uint32_t)((word >> shift) & 255);
-            int64_t g = (int64_t)tile_off[(int64_t)d * ntiles + blockIdx.x] + (p - (int64_t)dstart[d]);
+            int64_t g = (int64_t)tile_off[(int64_t)d * ntiles + kg_.bid] + (p - (int64_t)dstart[d]);
             words_out[g] = word;
         }
     }
 }
+};
 
 // Runs of words equal in z (live stars only: the tail has z = Z_TAIL) are put in
 // (r, id) order, r read back through the slot.  Run starts sort runs of up to
@@ -334,11 +352,13 @@ __device__ __forceinline__ bool key_id_less(uint64_t ka, int64_t ia, uint64_t kb
     return ka < kb || (ka == kb && ia < ib);
 }
 
-__global__ void rs_runs_kernel(uint64_t *__restrict__ words, const double *__restrict__ keysrc,
+struct rs_runs_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, uint64_t *__restrict__ words, const double *__restrict__ keysrc,
                                const int64_t *__restrict__ id, DevScalars *s, int64_t *__restrict__ long_runs,
                                int64_t *__restrict__ n_long, int64_t long_cap)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max_new;
     if (i + 1 >= n) return;
     const uint32_t z = word_z(words[i]);
@@ -369,6 +389,7 @@ __global__ void rs_runs_kernel(uint64_t *__restrict__ words, const double *__res
         words[b + 1] = wa;
     }
 }
+};
 
 // Long runs: one block per run (grid-stride over the list), bitonic sort by (r, id)
 // in LDS, padded to a power of two with all-ones entries.  Every word of a run has
@@ -415,7 +436,9 @@ __device__ void lds_sort_run(uint64_t *__restrict__ words, const double *__restr
     __syncthreads();
 }
 
-__global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict__ words,
+struct rs_long_runs_kernel {
+    static constexpr int LB = LR_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, uint64_t *__restrict__ words,
                                                             const double *__restrict__ keysrc,
                                                             const int64_t *__restrict__ id, DevScalars *s,
                                                             const int64_t *__restrict__ long_runs,
@@ -429,7 +452,7 @@ __global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict
     const int64_t nl = n_long[0] < long_cap ? n_long[0] : long_cap;
     const int64_t n = s->n_max_new;
     const int t = threadIdx.x;
-    for (int64_t r = blockIdx.x; r < nl; r += gridDim.x) {
+    for (int64_t r = kg_.bid; r < nl; r += kg_.nb) {
         const int64_t i = long_runs[r];
         const uint32_t z = word_z(words[i]);
         if (t == 0) {
@@ -458,6 +481,7 @@ __global__ __launch_bounds__(LR_T) void rs_long_runs_kernel(uint64_t *__restrict
         lds_sort_run(words, keysrc, id, i, len, z, sk, sid, sv);
     }
 }
+};
 
 // Runs longer than RUN_LONG (one block per run): chunks of RUN_LONG sorted in LDS,
 // then bottom-up merges between words and tmp (the same positions of the second key
@@ -469,7 +493,9 @@ __device__ __forceinline__ bool word_less(uint64_t wa, uint64_t wb, const double
     return key_id_less(sortable(keysrc[va]), id[va], sortable(keysrc[vb]), id[vb]);
 }
 
-__global__ __launch_bounds__(LR_T) void rs_huge_runs_kernel(uint64_t *__restrict__ words, uint64_t *__restrict__ tmp,
+struct rs_huge_runs_kernel {
+    static constexpr int LB = LR_T, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, uint64_t *__restrict__ words, uint64_t *__restrict__ tmp,
                                                             const double *__restrict__ keysrc,
                                                             const int64_t *__restrict__ id, DevScalars *s,
                                                             const int64_t *__restrict__ huge_runs,
@@ -482,7 +508,7 @@ __global__ __launch_bounds__(LR_T) void rs_huge_runs_kernel(uint64_t *__restrict
     const int64_t nh = n_long[1] < huge_cap ? n_long[1] : huge_cap;
     const int64_t n = s->n_max_new;
     const int t = threadIdx.x;
-    for (int64_t r = blockIdx.x; r < nh; r += gridDim.x) {
+    for (int64_t r = kg_.bid; r < nh; r += kg_.nb) {
         const int64_t i = huge_runs[r];
         const uint32_t z = word_z(words[i]);
         if (t == 0) {
@@ -530,25 +556,31 @@ __global__ __launch_bounds__(LR_T) void rs_huge_runs_kernel(uint64_t *__restrict
         __syncthreads();
     }
 }
+};
 
 // number of live stars = words below the tail
-__global__ void rs_count_kernel(const uint64_t *__restrict__ words, DevScalars *s)
+struct rs_count_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const uint64_t *__restrict__ words, DevScalars *s)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max_new;
     if (i > n) return;
     bool below_prev = (i == 0) ? true : word_z(words[i - 1]) != Z_TAIL;
     bool below_here = (i == n) ? false : word_z(words[i]) != Z_TAIL;
     if (below_prev && !below_here) s->sort_n = i;
 }
+};
 
 // The sorted permutation goes to perm (slot of the star now at position i), r and the
 // payload fields to their destinations.
-__global__ __launch_bounds__(256) void rs_gather_kernel(const uint64_t *__restrict__ words, DevScalars *s,
+struct rs_gather_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const uint64_t *__restrict__ words, DevScalars *s,
                                                         const double *__restrict__ keysrc, double *__restrict__ r_out,
                                                         uint32_t *__restrict__ perm, GatherList g)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->sort_n;
     if (i > n) return;
     if (i == n) {   // the outer sentinel (potential_calculate sets it again)
@@ -560,13 +592,16 @@ __global__ __launch_bounds__(256) void rs_gather_kernel(const uint64_t *__restri
     r_out[i + 1] = keysrc[src];
     for (int f = 0; f < g.nf; f++) g.dst[f][i + 1] = g.src[f][src];
 }
+};
 
 template <int W>
-__global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__restrict__ words, DevScalars *s,
+struct rs_gather_aos_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const uint64_t *__restrict__ words, DevScalars *s,
                                                             double *__restrict__ r_out, uint32_t *__restrict__ perm,
                                                             const uint64_t *__restrict__ aos, GatherList g)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->sort_n;
     if (i > n) return;
     if (i == n) {
@@ -589,12 +624,16 @@ __global__ __launch_bounds__(256) void rs_gather_aos_kernel(const uint64_t *__re
         if (f == g.nf) r_out[i + 1] = __longlong_as_double((long long)w[f]);
     }
 }
+};
 
-__global__ void rs_finish_kernel(DevScalars *s)
+struct rs_finish_kernel {
+    static constexpr int LB = 1024, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s)
 {
     s->n_max = s->sort_n;
     s->n_max_new = s->sort_n;
 }
+};
 
 } // namespace
 
@@ -617,46 +656,46 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     if (ntiles > sc.max_sort_tiles) return hipErrorInvalidValue;
     const uint64_t kinf = 0x8000000000000000ull | 0x4202a05f20000000ull;   // sortable(1.0e10)
     unsigned long long *mm = (unsigned long long *)sc.sort_mm;
-    hipLaunchKernelGGL(rs_zero_kernel, dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256), s,
+    kl<rs_zero_kernel>(dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256), s,
                        sc.sort_nlong);
-    hipLaunchKernelGGL(rs_minmax_kernel, dim3((unsigned)minmax_parts(ntiles)), dim3(256), 0, st, key_src, s, kinf,
+    kl<rs_minmax_kernel>(dim3((unsigned)minmax_parts(ntiles)), dim3(256), 0, st, key_src, s, kinf,
                        mm);
     // an even number of passes: the sorted words end in keys_a
     uint64_t *kin = sc.keys_a, *kout = sc.keys_b;
     if (narrow)
-        hipLaunchKernelGGL(rs_prep_kernel<8>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
+        kl<rs_prep_kernel<8>>(dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
                            sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm);
     else
-        hipLaunchKernelGGL(rs_prep_kernel<AOS_MAX>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
+        kl<rs_prep_kernel<AOS_MAX>>(dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
                            sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm);
     for (int pass = 8 - RS_PASSES; pass < 8; pass++) {
         int shift = 8 * pass;
-        hipLaunchKernelGGL(rs_hist_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, s, shift,
+        kl<rs_hist_kernel>(dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, s, shift,
                            sc.block_hist, ntiles);
-        hipLaunchKernelGGL(rs_scan_kernel, dim3(256), dim3(RS_THREADS), 0, st, sc.block_hist, ntiles,
+        kl<rs_scan_kernel>(dim3(256), dim3(RS_THREADS), 0, st, sc.block_hist, ntiles,
                            (const uint32_t *)(sc.digit_totals + 256 * pass));
-        hipLaunchKernelGGL(rs_scatter_kernel, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, kout, s, shift,
+        kl<rs_scatter_kernel>(dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, kout, s, shift,
                            (const uint32_t *)sc.block_hist, ntiles);
         uint64_t *tk = kin; kin = kout; kout = tk;
     }
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
-    hipLaunchKernelGGL(rs_runs_kernel, dim3(g1), dim3(256), 0, st, kin, key_src, id_src, s, sc.sort_long,
+    kl<rs_runs_kernel>(dim3(g1), dim3(256), 0, st, kin, key_src, id_src, s, sc.sort_long,
                        sc.sort_nlong, sc.sort_long_cap);
-    hipLaunchKernelGGL(rs_long_runs_kernel, dim3(64), dim3(LR_T), 0, st, kin, key_src, id_src, s,
+    kl<rs_long_runs_kernel>(dim3(64), dim3(LR_T), 0, st, kin, key_src, id_src, s,
                        (const int64_t *)sc.sort_long, sc.sort_nlong, sc.sort_long_cap, sc.sort_huge,
                        sc.sort_huge_cap);
-    hipLaunchKernelGGL(rs_huge_runs_kernel, dim3(16), dim3(LR_T), 0, st, kin, kout, key_src, id_src, s,
+    kl<rs_huge_runs_kernel>(dim3(16), dim3(LR_T), 0, st, kin, kout, key_src, id_src, s,
                        (const int64_t *)sc.sort_huge, (const int64_t *)sc.sort_nlong, sc.sort_huge_cap);
-    hipLaunchKernelGGL(rs_count_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s);
+    kl<rs_count_kernel>(dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s);
     if (aos && narrow)
-        hipLaunchKernelGGL(rs_gather_aos_kernel<8>, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
+        kl<rs_gather_aos_kernel<8>>(dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
                            sc.vals_a, (const uint64_t *)aos, g);
     else if (aos)
-        hipLaunchKernelGGL(rs_gather_aos_kernel<AOS_MAX>, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
+        kl<rs_gather_aos_kernel<AOS_MAX>>(dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
                            sc.vals_a, (const uint64_t *)aos, g);
     else
-        hipLaunchKernelGGL(rs_gather_kernel, dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, key_src, r_dst,
+        kl<rs_gather_kernel>(dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, key_src, r_dst,
                            sc.vals_a, g);
-    hipLaunchKernelGGL(rs_finish_kernel, dim3(1), dim3(1), 0, st, s);
+    kl<rs_finish_kernel>(dim3(1), dim3(1), 0, st, s);
     return hipGetLastError();
 }

@@ -223,11 +223,23 @@ int cmcg_step(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
  * order (CMCG_E_STRIP if it had not ended after 64 further passes).  Results are identical to cmcg_step.  No counterpart in the reference:
  * for callers that step many small clusters, where launch overhead dominates. */
 int cmcg_step_graph(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
-/* One step of each of `count` clusters (contexts on one device): every cluster's
- * step graph is launched on its own stream before any wait, so the clusters run
- * concurrently.  stats/errors (optional) receive one entry per cluster; returns the
- * first error.  No counterpart in the reference (an ensemble driver). */
+/* One step of each of `count` clusters (distinct contexts on one device) as one
+ * batched step: every cluster's step is recorded once per buffer state, the launches
+ * of each kernel are merged over the clusters into one segmented launch (the
+ * clusters' grids laid end to end; a block finds its cluster in a table of block
+ * offsets), and the merged sequence is replayed from one HIP graph on the first
+ * context's stream, ordered after every context's earlier work.  Each cluster's
+ * results are identical to its own cmcg_step.  Plans for the last few cluster sets
+ * and buffer states are kept (rebuilt after anything that drops a context's step
+ * graphs).  stats/errors (optional) receive one entry per cluster; returns the first
+ * error.  No counterpart in the reference (an ensemble driver). */
 int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors);
+/* The same ensemble step with each cluster's own step graph launched on its own
+ * stream before any wait (the clusters run concurrently, kernel by kernel). */
+int cmcg_ensemble_step_concurrent(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors);
+/* Of the last cmcg_ensemble_step: the merged launches its graph holds and the
+ * kernel launches recorded over all its clusters. */
+int cmcg_ensemble_stats(int64_t *launches, int64_t *recorded);
 
 /* ------------------------------------------------------------------ RNG access */
 /* Draw `ndraw` consecutive doubles of the substream of each of `count` consecutive

@@ -1,5 +1,6 @@
-"""The Henon step replayed from HIP graphs, alone and for an ensemble of clusters on
-concurrent streams, against the same step enqueued eagerly: every field and scalar
+"""The Henon step replayed from HIP graphs, alone and for an ensemble of clusters
+(merged into one segmented launch per kernel, or each cluster's graph on its own
+stream), against the same step enqueued eagerly: every field and scalar
 bit-identical after every step (each replay is checked, with host reads and eager
 work between replays)."""
 import numpy as np
@@ -50,22 +51,70 @@ def test_gpu_step_graph_matches_eager(gpu):
 
 
 def test_gpu_ensemble_step_matches_eager(gpu):
+    """Merged ensemble steps (both buffer states, then replays of the kept plans), a
+    step of one cluster alone in between (its buffer state then differs from the
+    others', so a plan for the mixed state is built), and a concurrent-streams step."""
     cm = gpu
     ns = [10000, 30000, 20000, 50000]
-    say("ensemble of", len(ns), "clusters, 3 steps")
+    say("ensemble of", len(ns), "clusters, 5 steps")
     cls = [cm.make_plummer(n, seed=3 + i) if i % 2 == 0 else cm.make_king(n, 5.0, seed=3 + i)
            for i, n in enumerate(ns)]
     eager = [_ctx(cm, cl) for cl in cls]
     ens = [_ctx(cm, cl) for cl in cls]
     try:
+        for k in range(5):
+            if k == 2:
+                eager[0].step()
+                ens[0].step_graph()
+                _same(eager[0], ens[0], "cluster 0 alone")
+            for c in eager:
+                c.step()
+            stats, errs = cm.ensemble_step(ens, concurrent=(k == 4))
+            assert errs == [0] * len(ns)
+            for i in range(len(ns)):
+                _same(eager[i], ens[i], f"cluster {i} step {k}")
+            if k < 4:
+                launches, recorded = cm.ensemble_stats()
+                say("step", k, "merged launches", launches, "recorded", recorded)
+                assert launches * len(ns) == recorded
+            say("step", k, "all clusters identical")
+    finally:
+        for c in eager + ens:
+            c.close()
+
+
+def test_gpu_ensemble_64_clusters_merged(gpu):
+    """64 clusters of uneven N (log-uniform on [2e3, 3e4]), half Plummer and half King
+    W0 = 3, 5, 7, 9, Kroupa masses: three merged steps, every cluster bit-identical to
+    its own eager step after each; each kernel of the step is one launch for all 64."""
+    cm = gpu
+    rng = np.random.default_rng(11)
+    ns = np.exp(rng.uniform(np.log(2e3), np.log(3e4), 64)).astype(np.int64)
+    say("ensemble of 64 clusters,", int(ns.sum()), "stars")
+    eager, ens = [], []
+    try:
+        for i, n in enumerate(ns):
+            n = int(n)
+            if i < 32:
+                cl = cm.make_plummer(n, seed=200 + i, kroupa=True)
+            else:
+                cl = cm.make_king(n, (3.0, 5.0, 7.0, 9.0)[i % 4], seed=200 + i, kroupa=True, segregate=False)
+            for lst in (eager, ens):
+                c = cm.Context(n, cm.default_params(n, orbit_r=cl.rtid, seed=200 + i))
+                c.upload(cl)
+                c.init_cluster()
+                lst.append(c)
         for k in range(3):
             for c in eager:
                 c.step()
             stats, errs = cm.ensemble_step(ens)
-            assert errs == [0] * len(ns)
-            for i in range(len(ns)):
+            assert errs == [0] * 64
+            launches, recorded = cm.ensemble_stats()
+            say("step", k, "merged launches", launches, "recorded", recorded)
+            assert launches * 64 == recorded
+            for i in range(64):
                 _same(eager[i], ens[i], f"cluster {i} step {k}")
-            say("step", k, "all clusters identical")
+            say("step", k, "all 64 clusters identical")
     finally:
         for c in eager + ens:
             c.close()

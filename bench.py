@@ -163,8 +163,8 @@ def orbit_ceiling(cm, ctx, margin, steps=3):
 
 def ensemble_bench(cm, steps, seed=7):
     """64 independent clusters stepped together: N log-uniform on [1e4, 2e6] (seeded),
-    32 Kroupa Plummer and 32 Kroupa King models with W0 = 3, 5, 7, 9, one context each.
-    Times the merged ensemble step (one segmented launch per kernel over all 64
+    32 Kroupa Plummer and 32 Kroupa King models with W0 = 3, 5, 7, 9, one context each,
+    each with its orbit sampler's launch shape autotuned.  Times the merged ensemble step (one segmented launch per kernel over all 64
     clusters, replayed from one graph) against every cluster's step graph on its own
     stream (concurrent) and against stepping the clusters one after another (eager)."""
     rng = np.random.default_rng(seed)
@@ -183,6 +183,10 @@ def ensemble_bench(cm, steps, seed=7):
         ctxs.append(ctx)
         del cl
     log(f"ensemble: 64 clusters, {int(ns.sum())} stars, set up in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    shapes = [c.autotune(reps=2)["variant"] for c in ctxs]   # each cluster's orbit-sampler launch shape
+    log(f"ensemble: autotuned in {time.perf_counter() - t0:.1f} s, "
+        f"{len(set(shapes))} distinct orbit-sampler shapes")
     for _ in range(2):                   # captures both pointer states' graphs
         cm.ensemble_step(ctxs)
     live = lambda: sum(c.scalars()["n_max"] for c in ctxs)
@@ -216,6 +220,7 @@ def ensemble_bench(cm, steps, seed=7):
     out = {"clusters": 64, "stars": int(ns.sum()), "n_min": int(ns.min()), "n_max": int(ns.max()),
            "steps": steps, "value": stars / t_ens, "unit": "stars/s", "ms_per_step": t_ens / steps * 1e3,
            "merged_launches": merged[0], "recorded_launches": merged[1],
+           "orbit_shapes": sorted(set(shapes)),
            "concurrent_graphs": {"value": con_stars / t_con, "ms_per_step": t_con / max(1, steps // 2) * 1e3},
            "sequential_eager": {"value": seq_stars / t_seq, "ms_per_step": t_seq / max(1, steps // 2) * 1e3},
            "note": "R_vir sets physical units only and does not enter the Henon-unit step"}

@@ -436,7 +436,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.excess_c, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.excess_list, (size_t)n_cap + 2)) ||
         (e = dalloc(c, &sc.sv3_wend, (size_t)n_cap + 2)) || (e = dalloc(c, &sc.sv3_merged, (size_t)n_cap + 2)) ||
         (e = dalloc(c, &sc.sv3_blk, (size_t)sc.max_blocks)) || (e = dalloc(c, &sc.sv3_fallback, 1)) ||
-        (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.strip_arrive, 1)) || (e = dalloc(c, &sc.chain_x, (size_t)n_cap + 8)) ||
+        (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.chain_x, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.chain_g, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.chain_mk, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.defer_list, 65536)) || (e = dalloc(c, &sc.defer_kfirst, 65536)) ||
         (e = dalloc(c, &sc.defer_partials, 65536 * 16)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
@@ -1091,13 +1091,18 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
     return CMCG_OK;
 }
 
+static int finish_from(cmcg_ctx *c, const DevScalars &h, cmcg_orbit_stats *stats)
+{
+    c->n_ub = h.n_max;
+    fill_stats(h, stats);
+    return h.error;
+}
+
 static int finish_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
 {
     DevScalars h;
     if (int e = read_scalars(c, h)) return e;
-    c->n_ub = h.n_max;
-    fill_stats(h, stats);
-    return h.error;
+    return finish_from(c, h, stats);
 }
 
 int cmcg_step(cmcg_ctx *c, cmcg_orbit_stats *stats)
@@ -1178,12 +1183,25 @@ int cmcg_ensemble_step_concurrent(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *
 // segmented launch per kernel, and the merged sequence is captured in one graph.
 namespace {
 
+// copies every cluster's scalars into one array (block c: cluster c)
+struct ens_scalars_kernel {
+    static constexpr int LB = 64, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const DevScalars *const *__restrict__ src,
+                                               DevScalars *__restrict__ dst)
+    {
+        static_assert(sizeof(DevScalars) % 8 == 0, "word copy");
+        const uint64_t *a = (const uint64_t *)src[kg_.bid];
+        uint64_t *b = (uint64_t *)(dst + kg_.bid);
+        for (unsigned w = threadIdx.x; w < sizeof(DevScalars) / 8; w += 64) b[w] = a[w];
+    }
+};
+
 struct EnsGroup {
     SegLaunchFn fn;
     uint32_t blocks, gy;
     dim3 block;
     size_t lds;
-    size_t tab_off, arg_off;     // into the plan's device buffer
+    size_t ent_off, chunk_off, arg_off;   // into the plan's device buffer
 };
 
 struct EnsPlan {
@@ -1192,15 +1210,15 @@ struct EnsPlan {
     void *dmem = nullptr;        // block tables and argument bytes of every merged launch
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> ev;  // one per cluster: that cluster's stream has reached the step
-    hipEvent_t done = nullptr;
+    DevScalars *host_scal = nullptr;   // pinned: every cluster's scalars after the step
     int64_t launches = 0, recorded = 0;
     ~EnsPlan()
     {
         hipSetDevice(device);
         if (exec) hipGraphExecDestroy(exec);
+        if (host_scal) hipHostFree(host_scal);
         for (hipEvent_t e : ev)
             if (e) hipEventDestroy(e);
-        if (done) hipEventDestroy(done);
         if (dmem) hipFree(dmem);
     }
 };
@@ -1236,37 +1254,80 @@ struct CtxPointers { DevFields f, alt; Scratch sc; int parity; };
 // Merges the recorded launches and captures the merged sequence in the plan's graph.
 int ens_assemble(EnsPlan *plan, const std::vector<SegRecorder> &rec, cmcg_ctx **ctxs, int count)
 {
-    // merge: each round takes the kernel that most clusters have next (ties: the one
-    // of the cluster with most launches left) and launches it for all of them
-    std::vector<size_t> cur(count, 0);
-    std::vector<std::vector<std::pair<int, size_t>>> members;   // (cluster, its record) per merged launch
+    // Alignment: each cluster's launches are matched to the longest cluster's sequence
+    // (a longest common subsequence of kernel keys).  Launches are issued slot by slot:
+    // slot 2j+1 holds the launches matched to position j of that sequence, slot 2j the
+    // unmatched launches a cluster has before that match (another launch shape of the
+    // orbit sampler, a kernel only some clusters run).  Within a slot, each round takes
+    // the kernel that most clusters have next (ties: the cluster with most launches
+    // left in the slot) and launches it for all of them.  Each cluster's launches keep
+    // their order.
     auto same = [](const SegRec &a, const SegRec &b) {
         return a.fn == b.fn && a.gy == b.gy && a.block.x == b.block.x && a.block.y == b.block.y &&
                a.block.z == b.block.z && a.lds == b.lds && a.args.size() == b.args.size();
     };
-    for (int i = 0; i < count; i++) plan->recorded += (int64_t)rec[i].recs.size();
-    for (;;) {
-        int best = -1, best_n = 0;
-        size_t best_left = 0;
-        for (int i = 0; i < count; i++) {
-            if (cur[i] >= rec[i].recs.size()) continue;
-            const SegRec &r = rec[i].recs[cur[i]];
-            int n = 0;
-            for (int j = 0; j < count; j++)
-                if (cur[j] < rec[j].recs.size() && same(rec[j].recs[cur[j]], r)) n++;
-            const size_t left = rec[i].recs.size() - cur[i];
-            if (n > best_n || (n == best_n && left > best_left)) { best = i; best_n = n; best_left = left; }
-        }
-        if (best < 0) break;
-        const SegRec &r0 = rec[best].recs[cur[best]];
-        std::vector<std::pair<int, size_t>> m;
-        for (int j = 0; j < count; j++)
-            if (cur[j] < rec[j].recs.size() && same(rec[j].recs[cur[j]], r0)) m.emplace_back(j, cur[j]);
-        for (auto &jm : m) cur[jm.first]++;
-        members.push_back(std::move(m));
+    std::vector<std::vector<std::pair<int, size_t>>> members;   // (cluster, its record) per merged launch
+    int refi = 0;
+    for (int i = 0; i < count; i++) {
+        plan->recorded += (int64_t)rec[i].recs.size();
+        if (rec[i].recs.size() > rec[refi].recs.size()) refi = i;
     }
-    // device buffer: per merged launch a block table (each cluster's grid starting at
-    // a multiple of 8 blocks) and the members' argument bytes, each 256-byte aligned
+    const std::vector<SegRec> &R = rec[refi].recs;
+    const size_t L = R.size();
+    std::vector<std::vector<std::vector<size_t>>> slot(count, std::vector<std::vector<size_t>>(2 * L + 1));
+    std::vector<uint32_t> dp;
+    for (int i = 0; i < count; i++) {
+        const std::vector<SegRec> &A = rec[i].recs;
+        const size_t n = A.size(), W = L + 1;
+        dp.assign((n + 1) * W, 0);                       // dp[a][r]: LCS of A[a..] and R[r..]
+        for (size_t x = n; x-- > 0;)
+            for (size_t y = L; y-- > 0;)
+                dp[x * W + y] = same(A[x], R[y]) ? dp[(x + 1) * W + y + 1] + 1
+                                                 : std::max(dp[(x + 1) * W + y], dp[x * W + y + 1]);
+        size_t x = 0, y = 0;
+        while (x < n) {
+            if (y < L && same(A[x], R[y]) && dp[x * W + y] == dp[(x + 1) * W + y + 1] + 1) {
+                slot[i][2 * y + 1].push_back(x);
+                x++, y++;
+            } else if (y < L && dp[x * W + y + 1] >= dp[(x + 1) * W + y]) {
+                y++;
+            } else {
+                slot[i][2 * y].push_back(x);
+                x++;
+            }
+        }
+    }
+    for (size_t sl = 0; sl < 2 * L + 1; sl++) {
+        std::vector<size_t> cur(count, 0);
+        for (;;) {
+            int best = -1, best_n = 0;
+            size_t best_left = 0;
+            for (int i = 0; i < count; i++) {
+                const std::vector<size_t> &li = slot[i][sl];
+                if (cur[i] >= li.size()) continue;
+                const SegRec &r = rec[i].recs[li[cur[i]]];
+                int nm = 0;
+                for (int j = 0; j < count; j++) {
+                    const std::vector<size_t> &lj = slot[j][sl];
+                    if (cur[j] < lj.size() && same(rec[j].recs[lj[cur[j]]], r)) nm++;
+                }
+                const size_t left = li.size() - cur[i];
+                if (nm > best_n || (nm == best_n && left > best_left)) { best = i; best_n = nm; best_left = left; }
+            }
+            if (best < 0) break;
+            const SegRec &r0 = rec[best].recs[slot[best][sl][cur[best]]];
+            std::vector<std::pair<int, size_t>> m;
+            for (int j = 0; j < count; j++) {
+                const std::vector<size_t> &lj = slot[j][sl];
+                if (cur[j] < lj.size() && same(rec[j].recs[lj[cur[j]]], r0)) m.emplace_back(j, lj[cur[j]]);
+            }
+            for (auto &jm : m) cur[jm.first]++;
+            members.push_back(std::move(m));
+        }
+    }
+    // device buffer: per merged launch the members' grids (each starting at a multiple
+    // of 8 blocks), the owner of every 8 blocks and the members' argument bytes, each
+    // 256-byte aligned
     std::vector<unsigned char> host;
     auto reserve = [&](size_t bytes) {
         const size_t off = (host.size() + 255) & ~(size_t)255;
@@ -1277,32 +1338,50 @@ int ens_assemble(EnsPlan *plan, const std::vector<SegRecorder> &rec, cmcg_ctx **
     for (const auto &m : members) {
         const SegRec &r0 = rec[m[0].first].recs[m[0].second];
         const size_t asz = r0.args.size();
-        EnsGroup g{r0.fn, 0, r0.gy, r0.block, r0.lds, 0, 0};
-        g.tab_off = reserve(8 * (1 + m.size()));
+        EnsGroup g{r0.fn, 0, r0.gy, r0.block, r0.lds, 0, 0, 0};
+        // cluster k's grid starts at a multiple of 8 blocks
+        std::vector<uint32_t> first(m.size());
+        uint64_t end = 0;
+        for (size_t k = 0; k < m.size(); k++) {
+            first[k] = (uint32_t)((end + 7) & ~(uint64_t)7);
+            end = first[k] + (uint64_t)rec[m[k].first].recs[m[k].second].gx;
+            if (end > 0x7fffffffull) return CMCG_E_ARG;
+        }
+        const size_t nchunk = (size_t)((end + 7) >> 3);
+        g.ent_off = reserve(8 * m.size());
+        g.chunk_off = reserve(2 * nchunk);
         g.arg_off = reserve(asz * m.size());
-        uint64_t off = 0;
+        uint16_t *chunk = (uint16_t *)(host.data() + g.chunk_off);
         for (size_t k = 0; k < m.size(); k++) {
             const SegRec &r = rec[m[k].first].recs[m[k].second];
-            const uint32_t e[2] = {(uint32_t)off, r.gx};
-            memcpy(host.data() + g.tab_off + 8 * (1 + k), e, 8);
+            const uint32_t e[2] = {first[k], r.gx};
+            memcpy(host.data() + g.ent_off + 8 * k, e, 8);
             memcpy(host.data() + g.arg_off + asz * k, r.args.data(), asz);
-            off = (off + r.gx + 7) & ~(uint64_t)7;
-            if (k + 1 == m.size()) off = e[0] + (uint64_t)r.gx;
+            for (size_t q = first[k] >> 3; q < ((size_t)first[k] + r.gx + 7) >> 3; q++) chunk[q] = (uint16_t)k;
         }
-        if (off > 0x7fffffffull) return CMCG_E_ARG;
-        const uint32_t h[2] = {(uint32_t)m.size(), 0u};
-        memcpy(host.data() + g.tab_off, h, 8);
-        g.blocks = (uint32_t)off;
+        g.blocks = (uint32_t)end;
         groups.push_back(g);
     }
+    const size_t ptr_off = reserve(sizeof(DevScalars *) * (size_t)count);
+    for (int i = 0; i < count; i++) memcpy(host.data() + ptr_off + sizeof(DevScalars *) * i, &ctxs[i]->s, sizeof(DevScalars *));
+    const size_t scal_off = reserve(sizeof(DevScalars) * (size_t)count);
     plan->launches = (int64_t)groups.size();
+    if (hipHostMalloc((void **)&plan->host_scal, sizeof(DevScalars) * (size_t)count, hipHostMallocDefault) != hipSuccess) {
+        plan->host_scal = nullptr;
+        return CMCG_E_NOMEM;
+    }
     hipStream_t st = ctxs[0]->stream;
     if (hipMalloc(&plan->dmem, host.size() ? host.size() : 256) != hipSuccess) return CMCG_E_NOMEM;
     if (hipMemcpy(plan->dmem, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) return CMCG_E_HIP;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return CMCG_E_HIP;
     for (const EnsGroup &g : groups)
-        g.fn(g.blocks, g.gy, g.block, g.lds, st, (const uint2 *)((char *)plan->dmem + g.tab_off),
-             (char *)plan->dmem + g.arg_off);
+        g.fn(g.blocks, g.gy, g.block, g.lds, st, (const uint2 *)((char *)plan->dmem + g.ent_off),
+             (const uint16_t *)((char *)plan->dmem + g.chunk_off), (char *)plan->dmem + g.arg_off);
+    // every cluster's scalars to the host in one copy
+    DevScalars *scal = (DevScalars *)((char *)plan->dmem + scal_off);
+    kl<ens_scalars_kernel>(dim3((unsigned)count), dim3(64), 0, st,
+                           (const DevScalars *const *)((char *)plan->dmem + ptr_off), scal);
+    hipMemcpyAsync(plan->host_scal, scal, sizeof(DevScalars) * (size_t)count, hipMemcpyDeviceToHost, st);
     hipGraph_t graph = nullptr;
     hipError_t ce = hipStreamEndCapture(st, &graph);
     if (ce != hipSuccess || !graph) {
@@ -1315,7 +1394,6 @@ int ens_assemble(EnsPlan *plan, const std::vector<SegRecorder> &rec, cmcg_ctx **
     plan->ev.assign(count, nullptr);
     for (int i = 0; i < count; i++)
         if (hipEventCreateWithFlags(&plan->ev[i], hipEventDisableTiming) != hipSuccess) return CMCG_E_HIP;
-    if (hipEventCreateWithFlags(&plan->done, hipEventDisableTiming) != hipSuccess) return CMCG_E_HIP;
     return CMCG_OK;
 }
 
@@ -1371,18 +1449,18 @@ int ens_build(cmcg_ctx **ctxs, int count, std::unique_ptr<EnsPlan> &out)
 }
 
 // Launches a plan on the first cluster's stream, ordered after every cluster's
-// earlier work and before its later work.
+// earlier work (the caller waits for it before any later work).
 int ens_launch(EnsPlan &p, cmcg_ctx **ctxs, int count)
 {
     hipStream_t st = ctxs[0]->stream;
     for (int i = 1; i < count; i++) {
+        const hipError_t q = hipStreamQuery(ctxs[i]->stream);
+        if (q == hipSuccess) continue;                       // nothing pending there
+        if (q != hipErrorNotReady) return CMCG_E_HIP;
         if (hipEventRecord(p.ev[i], ctxs[i]->stream) != hipSuccess) return CMCG_E_HIP;
         if (hipStreamWaitEvent(st, p.ev[i], 0) != hipSuccess) return CMCG_E_HIP;
     }
     if (hipGraphLaunch(p.exec, st) != hipSuccess) return CMCG_E_HIP;
-    if (hipEventRecord(p.done, st) != hipSuccess) return CMCG_E_HIP;
-    for (int i = 1; i < count; i++)
-        if (hipStreamWaitEvent(ctxs[i]->stream, p.done, 0) != hipSuccess) return CMCG_E_HIP;
     return CMCG_OK;
 }
 
@@ -1404,7 +1482,7 @@ extern "C" {
 
 int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors)
 {
-    if (!ctxs || count < 1) return CMCG_E_ARG;
+    if (!ctxs || count < 1 || count > 65536) return CMCG_E_ARG;   // block owners are 16-bit
     for (int i = 0; i < count; i++) {
         if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) return CMCG_E_ARG;
         for (int j = 0; j < i; j++)
@@ -1430,14 +1508,16 @@ int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int3
         g_ens_plans.push_back(std::move(plan));
         if (g_ens_plans.size() > ENS_MAX_PLANS) g_ens_plans.erase(g_ens_plans.begin());
         if (e) return e;
+        if (hipStreamSynchronize(ctxs[0]->stream) != hipSuccess) return CMCG_E_HIP;
+        const DevScalars *h = g_ens_plans.back()->host_scal;
+        int first_err = CMCG_OK;
+        for (int i = 0; i < count; i++) {
+            int e = finish_from(ctxs[i], h[i], stats ? &stats[i] : nullptr);
+            if (errors) errors[i] = e;
+            if (e && !first_err) first_err = e;
+        }
+        return first_err;
     }
-    int first_err = CMCG_OK;
-    for (int i = 0; i < count; i++) {
-        int e = finish_step(ctxs[i], stats ? &stats[i] : nullptr);
-        if (errors) errors[i] = e;
-        if (e && !first_err) first_err = e;
-    }
-    return first_err;
 }
 
 int cmcg_ensemble_stats(int64_t *launches, int64_t *recorded)

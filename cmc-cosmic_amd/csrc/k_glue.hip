@@ -146,13 +146,12 @@ struct strip_begin_kernel {
 }
 };
 
-// One pass of the stripping loop (cmc_evolution_thr.c:447-645) in one kernel: every
-// block derives Rtidal from the scalars left by the previous pass, strips its stars
-// (grid-stride), and publishes its partial sums; the last block to arrive reduces
-// them in block order, updates the scalars, decides whether the loop ends and
-// resets the arrival counter.  Passes after the end return at once, so a fixed
-// number of passes can be enqueued (or captured in a graph) without a host round
-// trip.
+// One pass of the stripping loop (cmc_evolution_thr.c:447-645) is two kernels: a
+// fixed grid in which every block derives Rtidal from the scalars left by the
+// previous pass, strips its stars (grid-stride) and stores its partial sums, then one
+// block that reduces them in block order, updates the scalars and decides whether the
+// loop ends.  Passes after the end return at once, so a fixed number of passes can be
+// enqueued (or captured in a graph) without a host round trip.
 constexpr int STRIP_BLOCKS = 1024;
 constexpr int STRIP_TAIL_PASSES = 64;
 
@@ -205,25 +204,24 @@ __device__ void strip_finish(DevScalars *s, double Rt, int32_t *done, const doub
     }
 }
 
-struct strip_fused_kernel {
+struct strip_pass_kernel {
     static constexpr int LB = GT, MB = 0;
-    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, int32_t *done,
-                                                         double *__restrict__ partials, unsigned *__restrict__ arrive)
-{
-    if (*done) return;
-    const double Rt = strip_rtidal(s, P);
-    strip_block(f, P, Rt, s->n_max, partials, kg_.nb, kg_.bid);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        __threadfence();                                   // publish this block's partials
-        last = atomicAdd(arrive, 1u) == kg_.nb - 1;
+    static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P, const int32_t *done,
+                                               double *__restrict__ partials)
+    {
+        if (*done) return;
+        strip_block(f, P, strip_rtidal(s, P), s->n_max, partials, kg_.nb, kg_.bid);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();                                       // see every block's partials
-    strip_finish(s, Rt, done, partials, kg_.nb);
-    if (threadIdx.x == 0) *arrive = 0;
-}
+};
+
+struct strip_end_kernel {
+    static constexpr int LB = GT, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, KParams P, int32_t *done,
+                                               const double *__restrict__ partials, unsigned nb)
+    {
+        if (*done) return;
+        strip_finish(s, strip_rtidal(s, P), done, partials, nb);
+    }
 };
 
 // The rest of the loop in one block, for a captured step: after the grid passes the
@@ -552,12 +550,12 @@ struct sv3_spec_kernel {
                                 const int64_t *__restrict__ events, int64_t *__restrict__ wend,
                                 int32_t *__restrict__ joins)
 {
-    int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
-    int64_t ne = s->n_excess;
-    if (d >= ne) return;
-    int64_t e = excess_walk(f, excess_c, events[d], s->n_max, false, SPEC_WALK_MAX);
-    wend[d] = e;
-    joins[d] = (e < 0) || (d + 1 < ne && e >= events[d + 1]);
+    const int64_t ne = s->n_excess;
+    for (int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x; d < ne; d += (int64_t)kg_.nb * blockDim.x) {
+        int64_t e = excess_walk(f, excess_c, events[d], s->n_max, false, SPEC_WALK_MAX);
+        wend[d] = e;
+        joins[d] = (e < 0) || (d + 1 < ne && e >= events[d + 1]);
+    }
 }
 };
 
@@ -572,14 +570,15 @@ struct sv3_verify_kernel {
                                   const int32_t *__restrict__ joins, const int64_t *__restrict__ wend,
                                   int64_t *__restrict__ relink)
 {
-    int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
-    int64_t ne = s->n_excess;
-    if (d >= ne || (d > 0 && joins[d - 1])) return;
-    int64_t q = d;
-    while (q + 1 < ne && joins[q]) q++;
-    int64_t e = wend[d];
-    if (e < 0 || e < events[q] || (q + 1 < ne && e >= events[q + 1]))
-        atomicMin((unsigned long long *)relink, (unsigned long long)d);
+    const int64_t ne = s->n_excess;
+    for (int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x; d < ne; d += (int64_t)kg_.nb * blockDim.x) {
+        if (d > 0 && joins[d - 1]) continue;
+        int64_t q = d;
+        while (q + 1 < ne && joins[q]) q++;
+        int64_t e = wend[d];
+        if (e < 0 || e < events[q] || (q + 1 < ne && e >= events[q + 1]))
+            atomicMin((unsigned long long *)relink, (unsigned long long)d);
+    }
 }
 };
 
@@ -614,10 +613,9 @@ struct sv3_heads_kernel {
     static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, const double *__restrict__ excess_c,
                                  const int64_t *__restrict__ events, const int32_t *__restrict__ joins)
 {
-    int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
-    int64_t ne = s->n_excess;
-    if (d >= ne || (d > 0 && joins[d - 1])) return;
-    excess_walk(f, excess_c, events[d], s->n_max, true, INT64_MAX);
+    const int64_t ne = s->n_excess;
+    for (int64_t d = (int64_t)kg_.bid * blockDim.x + threadIdx.x; d < ne; d += (int64_t)kg_.nb * blockDim.x)
+        if (d == 0 || !joins[d - 1]) excess_walk(f, excess_c, events[d], s->n_max, true, INT64_MAX);
 }
 };
 
@@ -697,9 +695,10 @@ hipError_t launch_strip(const DevFields &f, DevScalars *s, const KParams &P, con
     const unsigned nb = STRIP_BLOCKS;              // the scratch holds at least STRIP_BLOCKS partials
     if (sc.max_blocks < STRIP_BLOCKS) return hipErrorInvalidValue;
     if (begin) kl<strip_begin_kernel>(dim3(1), dim3(1), 0, st, s, done);
-    for (int it = 0; it < passes; it++)
-        kl<strip_fused_kernel>(dim3(nb), dim3(GT), 0, st, f, s, P, done, sc.partials,
-                           sc.strip_arrive);
+    for (int it = 0; it < passes; it++) {
+        kl<strip_pass_kernel>(dim3(nb), dim3(GT), 0, st, f, s, P, (const int32_t *)done, sc.partials);
+        kl<strip_end_kernel>(dim3(1), dim3(GT), 0, st, s, P, done, (const double *)sc.partials, nb);
+    }
     return hipGetLastError();
 }
 
@@ -729,6 +728,7 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     int64_t nb = grid_for(n_ub);
     if (nb > sc.max_blocks) return hipErrorInvalidValue;
     unsigned ge = grid_for(n_ub);
+    if (ge > 256) ge = 256;            // the excess-list kernels stride over the list
     kl<sv3_begin_kernel>(dim3(1), dim3(1), 0, st, s, sc.sv3_fallback);
     // the step calls this right after the potential, which left the node table current
     if (!dpost)

@@ -80,7 +80,6 @@ struct Scratch {
     int64_t *sv3_fallback;      // first presumed head that failed the check
     // tidal stripping loop flag
     int32_t *strip_done;
-    unsigned *strip_arrive;     // arrival counter of the fused stripping pass
     // orbit stars deferred to the linear-scan path (65536 slots)
     int64_t *defer_list, *defer_kfirst;
     double *defer_partials;

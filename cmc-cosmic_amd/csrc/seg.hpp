@@ -9,9 +9,9 @@
 // (kernel, grid, block, LDS bytes, argument bytes) instead of issuing it.  An
 // ensemble then merges the recorded launches of many clusters into one kseg launch
 // per kernel: its blocks are the clusters' grids laid end to end (each starting at a
-// multiple of 8 blocks, so a block keeps the XCD its cluster-local index implies),
-// and a block finds its cluster by a binary search in a table of block offsets and
-// reads that cluster's arguments from device memory.
+// multiple of 8 blocks, so a block keeps the XCD its cluster-local index implies);
+// a block finds its cluster in a table indexed by its block index over 8 and reads
+// that cluster's grid and arguments from device memory.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -77,21 +77,17 @@ __device__ __forceinline__ void seg_apply(KGrid g, const Tup<typename SegArg<P>:
     Body::run(g, SegArg<P>::get(TupAt<I>::get(a))...);
 }
 
-// tab[0].x = clusters; tab[1 + c] = (first block, blocks) of cluster c, first blocks
-// ascending
+// ent[c] = (first block, blocks) of cluster c; chunk[b / 8] = the cluster that owns
+// blocks 8 (b / 8) .. 8 (b / 8) + 7 (cluster grids start at multiples of 8)
 template <typename Body, typename... P>
-__device__ __forceinline__ void seg_run(const uint2 *__restrict__ tab, const Tup<typename SegArg<P>::type...> *__restrict__ a)
+__device__ __forceinline__ void seg_run(const uint2 *__restrict__ ent, const uint16_t *__restrict__ chunk,
+                                        const Tup<typename SegArg<P>::type...> *__restrict__ a)
 {
     const uint32_t b = blockIdx.x;
-    int lo = 0, hi = (int)tab[0].x - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (tab[1 + mid].x <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const uint2 e = tab[1 + lo];
+    const int c = chunk[b >> 3];
+    const uint2 e = ent[c];
     if (b - e.x >= e.y) return;             // alignment padding after the cluster's grid
-    seg_apply<Body, P...>(KGrid{b - e.x, e.y, blockIdx.y, gridDim.y}, a[lo], std::index_sequence_for<P...>{});
+    seg_apply<Body, P...>(KGrid{b - e.x, e.y, blockIdx.y, gridDim.y}, a[c], std::index_sequence_for<P...>{});
 }
 
 template <typename Body, int LB, typename... P>
@@ -102,21 +98,22 @@ __global__ __launch_bounds__(LB, MB) void kone_mb(P... p)
     Body::run(KGrid{blockIdx.x, gridDim.x, blockIdx.y, gridDim.y}, p...);
 }
 template <typename Body, int LB, typename... P>
-__global__ __launch_bounds__(LB) void kseg(const uint2 *__restrict__ tab, const Tup<typename SegArg<P>::type...> *__restrict__ a)
+__global__ __launch_bounds__(LB) void kseg(const uint2 *__restrict__ ent, const uint16_t *__restrict__ chunk,
+                                           const Tup<typename SegArg<P>::type...> *__restrict__ a)
 {
-    seg_run<Body, P...>(tab, a);
+    seg_run<Body, P...>(ent, chunk, a);
 }
 template <typename Body, int LB, int MB, typename... P>
-__global__ __launch_bounds__(LB, MB) void kseg_mb(const uint2 *__restrict__ tab,
+__global__ __launch_bounds__(LB, MB) void kseg_mb(const uint2 *__restrict__ ent, const uint16_t *__restrict__ chunk,
                                                   const Tup<typename SegArg<P>::type...> *__restrict__ a)
 {
-    seg_run<Body, P...>(tab, a);
+    seg_run<Body, P...>(ent, chunk, a);
 }
 
 // ---------------------------------------------------------------- host side
 // Launches a merged kernel: `blocks` blocks in x (the clusters' aligned grids), gy in y.
-using SegLaunchFn = void (*)(uint32_t blocks, uint32_t gy, dim3 block, size_t lds, hipStream_t st, const uint2 *tab,
-                             const void *args);
+using SegLaunchFn = void (*)(uint32_t blocks, uint32_t gy, dim3 block, size_t lds, hipStream_t st, const uint2 *ent,
+                             const uint16_t *chunk, const void *args);
 
 struct SegRec {
     SegLaunchFn fn;                  // also identifies the kernel instantiation
@@ -132,15 +129,15 @@ struct SegRecorder { std::vector<SegRec> recs; bool bad = false; };   // bad: a 
 extern thread_local SegRecorder *g_seg_rec;
 
 template <typename Body, typename... P>
-void seg_launch(uint32_t blocks, uint32_t gy, dim3 block, size_t lds, hipStream_t st, const uint2 *tab,
-                const void *args)
+void seg_launch(uint32_t blocks, uint32_t gy, dim3 block, size_t lds, hipStream_t st, const uint2 *ent,
+                const uint16_t *chunk, const void *args)
 {
     static_assert(sizeof(Tup<typename SegArg<P>::type...>) == sizeof(Tup<P...>), "argument layout");
     using A = const Tup<typename SegArg<P>::type...> *;
     if constexpr (Body::MB > 0)
-        hipLaunchKernelGGL((kseg_mb<Body, Body::LB, Body::MB, P...>), dim3(blocks, gy), block, lds, st, tab, (A)args);
+        hipLaunchKernelGGL((kseg_mb<Body, Body::LB, Body::MB, P...>), dim3(blocks, gy), block, lds, st, ent, chunk, (A)args);
     else
-        hipLaunchKernelGGL((kseg<Body, Body::LB, P...>), dim3(blocks, gy), block, lds, st, tab, (A)args);
+        hipLaunchKernelGGL((kseg<Body, Body::LB, P...>), dim3(blocks, gy), block, lds, st, ent, chunk, (A)args);
 }
 
 template <typename T> inline T seg_conv(T x) { return x; }

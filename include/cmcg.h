@@ -231,8 +231,9 @@ int cmcg_step_graph(cmcg_ctx *ctx, cmcg_orbit_stats *stats);
  * context's stream, ordered after every context's earlier work.  Each cluster's
  * results are identical to its own cmcg_step.  Plans for the last few cluster sets
  * and buffer states are kept (rebuilt after anything that drops a context's step
- * graphs).  stats/errors (optional) receive one entry per cluster; returns the first
- * error.  No counterpart in the reference (an ensemble driver). */
+ * graphs).  At most 65536 clusters.  stats/errors (optional) receive one entry per
+ * cluster; returns the first error.  No counterpart in the reference (an ensemble
+ * driver). */
 int cmcg_ensemble_step(cmcg_ctx **ctxs, int count, cmcg_orbit_stats *stats, int32_t *errors);
 /* The same ensemble step with each cluster's own step graph launched on its own
  * stream before any wait (the clusters run concurrently, kernel by kernel). */

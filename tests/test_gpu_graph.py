@@ -86,7 +86,10 @@ def test_gpu_ensemble_step_matches_eager(gpu):
 def test_gpu_ensemble_64_clusters_merged(gpu):
     """64 clusters of uneven N (log-uniform on [2e3, 3e4]), half Plummer and half King
     W0 = 3, 5, 7, 9, Kroupa masses: three merged steps, every cluster bit-identical to
-    its own eager step after each; each kernel of the step is one launch for all 64."""
+    its own eager step after each; each kernel of the step is one launch for all 64.
+    Then the clusters take three different orbit-sampler launch shapes: two more
+    steps, still identical, with the orbit kernel three launches and every other
+    kernel still one."""
     cm = gpu
     rng = np.random.default_rng(11)
     ns = np.exp(rng.uniform(np.log(2e3), np.log(3e4), 64)).astype(np.int64)
@@ -104,14 +107,23 @@ def test_gpu_ensemble_64_clusters_merged(gpu):
                 c.upload(cl)
                 c.init_cluster()
                 lst.append(c)
-        for k in range(3):
+        base = None
+        for k in range(5):
+            if k == 3:
+                for i in range(64):
+                    for c in (eager[i], ens[i]):
+                        c.test_hook(cm.HOOK_ORBIT_VARIANT, (0, 7, 10)[i % 3])
             for c in eager:
                 c.step()
             stats, errs = cm.ensemble_step(ens)
             assert errs == [0] * 64
             launches, recorded = cm.ensemble_stats()
             say("step", k, "merged launches", launches, "recorded", recorded)
-            assert launches * 64 == recorded
+            if k < 3:
+                assert launches * 64 == recorded
+                base = launches
+            else:
+                assert launches == base + 2
             for i in range(64):
                 _same(eager[i], ens[i], f"cluster {i} step {k}")
             say("step", k, "all 64 clusters identical")

@@ -161,7 +161,7 @@ def orbit_ceiling(cm, ctx, margin, steps=3):
     return out
 
 
-def ensemble_bench(cm, steps, seed=7):
+def ensemble_bench(cm, steps, seed=7, mlp=None):
     """64 independent clusters stepped together: N log-uniform on [1e4, 2e6] (seeded),
     32 Kroupa Plummer and 32 Kroupa King models with W0 = 3, 5, 7, 9, one context each,
     each with its orbit sampler's launch shape autotuned.  Times the merged ensemble step (one segmented launch per kernel over all 64
@@ -207,6 +207,30 @@ def ensemble_bench(cm, steps, seed=7):
         cm.ensemble_step(ctxs, concurrent=True)
         t_con += time.perf_counter() - t
         con_stars += nl
+    learned = None
+    if mlp is not None:
+        # the learned brackets on every cluster: the network trained on the 4e6 King
+        # cluster's rows, verified brackets (results identical to the plain step)
+        theta, norm, margin = mlp
+        for c in ctxs:
+            c.mlp_init(seed=1, max_rows=1 << 12)
+            c.mlp_set_params(theta, norm)
+            c.set_learned_brackets(True, margin=margin)
+        for _ in range(2):
+            cm.ensemble_step(ctxs)
+        l_stars, t_l, hit, miss = 0, 0.0, 0, 0
+        for _ in range(max(1, steps // 2)):
+            nl = live()
+            t = time.perf_counter()
+            st, _ = cm.ensemble_step(ctxs)
+            t_l += time.perf_counter() - t
+            l_stars += nl
+            hit += sum(x["n_bracket_hit"] for x in st)
+            miss += sum(x["n_bracket_fallback"] for x in st)
+        for c in ctxs:
+            c.set_learned_brackets(False)
+        learned = {"value": l_stars / t_l, "ms_per_step": t_l / max(1, steps // 2) * 1e3,
+                   "brackets_accepted": hit, "brackets_rejected": miss}
     seq_stars, t_seq = 0, 0.0
     for _ in range(max(1, steps // 2)):
         nl = live()
@@ -222,11 +246,14 @@ def ensemble_bench(cm, steps, seed=7):
            "merged_launches": merged[0], "recorded_launches": merged[1],
            "orbit_shapes": sorted(set(shapes)),
            "concurrent_graphs": {"value": con_stars / t_con, "ms_per_step": t_con / max(1, steps // 2) * 1e3},
+           "learned": learned,
            "sequential_eager": {"value": seq_stars / t_seq, "ms_per_step": t_seq / max(1, steps // 2) * 1e3},
            "note": "R_vir sets physical units only and does not enter the Henon-unit step"}
     log(f"ensemble: {out['value']:.3e} stars/s ({out['ms_per_step']:.2f} ms per step of all 64, "
         f"{merged[0]} merged launches of {merged[1]}), concurrent graphs "
-        f"{out['concurrent_graphs']['value']:.3e} stars/s, one by one {out['sequential_eager']['value']:.3e} stars/s")
+        f"{out['concurrent_graphs']['value']:.3e} stars/s, one by one {out['sequential_eager']['value']:.3e} stars/s"
+        + (f", learned brackets {learned['value']:.3e} stars/s ({learned['brackets_accepted']} accepted, "
+           f"{learned['brackets_rejected']} rejected)" if learned else ""))
     return out
 
 
@@ -356,7 +383,7 @@ def main():
                "orbit_phase_ceiling": ceiling}
 
     roofline = orbit_roofline(phases["orbits_kernel"], float(np.mean(live)))
-    ens = None if args.no_ensemble else ensemble_bench(cm, 10)
+    ens = None if args.no_ensemble else ensemble_bench(cm, 10, mlp=(*ctx.mlp_params(), margin))
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle step on the same {cl.n} stars")

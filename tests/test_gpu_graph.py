@@ -132,6 +132,57 @@ def test_gpu_ensemble_64_clusters_merged(gpu):
             c.close()
 
 
+def test_gpu_ensemble_learned_brackets_identical(gpu):
+    """The merged ensemble with the learned brackets on (one network, trained on
+    another cluster's harvested rows, set on every context): each cluster stays
+    bit-identical to its own plain eager step, and most proposals are accepted."""
+    cm = gpu
+    say("ensemble with learned brackets, 4 clusters, 3 steps")
+    tr = cm.make_plummer(50000, seed=21, kroupa=True)
+    t = _ctx(cm, tr)
+    try:
+        t.mlp_init(seed=1, max_rows=1 << 20)
+        t.mlp_harvest(True)
+        for _ in range(2):
+            t.step()
+        t.mlp_harvest(False)
+        rows = t.mlp_rows()
+        t.mlp_fit_norm()
+        t.mlp_train(400, batch=4096, lr=2e-3, seed=3)
+        val = rows[::97][:8000]
+        err = float(np.sqrt(np.mean((t.mlp_predict(val[:, :6]) - val[:, 6:]) ** 2)))
+        theta, norm = t.mlp_params()
+    finally:
+        t.close()
+    say("trained on", rows.shape[0], "rows, rms log r error", err)
+    ns = [20000, 40000, 30000, 60000]
+    cls = [cm.make_plummer(n, seed=30 + i, kroupa=True) if i % 2 == 0 else cm.make_king(n, 7.0, seed=30 + i)
+           for i, n in enumerate(ns)]
+    eager = [_ctx(cm, cl) for cl in cls]
+    ens = [_ctx(cm, cl) for cl in cls]
+    try:
+        for c in ens:
+            c.mlp_init(seed=1, max_rows=1 << 12)
+            c.mlp_set_params(theta, norm)
+            c.set_learned_brackets(True, margin=max(3.0 * err, 0.01))
+        hit = miss = 0
+        for k in range(3):
+            for c in eager:
+                c.step()
+            stats, errs = cm.ensemble_step(ens)
+            assert errs == [0] * len(ns)
+            hit += sum(x["n_bracket_hit"] for x in stats)
+            miss += sum(x["n_bracket_fallback"] for x in stats)
+            for i in range(len(ns)):
+                _same(eager[i], ens[i], f"cluster {i} step {k}")
+            say("step", k, "all clusters identical")
+        say("learned brackets accepted", hit, "rejected", miss)
+        assert hit > 0.5 * (hit + miss)
+    finally:
+        for c in eager + ens:
+            c.close()
+
+
 def test_gpu_autotuned_shape_gives_identical_stars(gpu):
     """The autotuner picks one of the orbit sampler's launch shapes; whichever it
     picks, the stars after a step are bit-identical to the default shape's (only the

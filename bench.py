@@ -268,6 +268,7 @@ def main():
                     help="host threads for the multi-rank CPU baseline (the GPU box's CPU share is 16)")
     ap.add_argument("--train-iters", type=int, default=6000)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-cold", action="store_true", help="skip the cold-cache phase timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported "
@@ -328,6 +329,15 @@ def main():
 
     total, live, times, phases, st = timed(args.steps)
     log(f"timed {args.steps} plain steps (eager launches) in {total:.3f} s")
+    # the same phases from cold caches: before each phase a 1 GiB buffer is read and
+    # rewritten (evicting the L2s and the 256 MB MALL), outside the phase's events
+    phases_cold = None
+    if not args.no_cold:
+        ctx.test_hook(cm.HOOK_COLD_PHASES, 1024)
+        _, _, _, phases_cold, _ = timed(3)
+        ctx.test_hook(cm.HOOK_COLD_PHASES, 0)
+        log("cold-cache phases (ms): " + ", ".join(f"{k} {v['ms']:.3f} (warm {phases[k]['ms']:.3f})"
+                                                   for k, v in phases_cold.items() if k in phases))
     stars = float(np.sum(live))
 
     # the same steps replayed from HIP graphs (no host round trip inside a step); the
@@ -411,6 +421,7 @@ def main():
         "phases": phases,
         "learned": learned,
         "autotune": tune,
+        "phases_cold_cache": phases_cold,
         "ensemble": ens,
         "cpu_baseline": cpu,
         "last_step_stats": st,

@@ -182,6 +182,7 @@ HOOK_CHAIN_K_DROP = 1
 HOOK_ORBIT_VARIANT = 2
 HOOK_MLP_MFMA = 3
 HOOK_PERFECT_BRACKETS = 4
+HOOK_COLD_PHASES = 5
 for _name, _args in _sigs.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args

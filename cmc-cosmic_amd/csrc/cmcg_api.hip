@@ -53,6 +53,8 @@ struct cmcg_ctx {
     uint64_t gen = 0;            // bumped whenever the graphs are dropped
     int orb_variant = 0;         // launch shape of the orbit sampler (autotuned)
     std::vector<double> tune_ms; // per-variant time of the last cmcg_autotune
+    uint4 *flush_buf = nullptr;  // CMCG_HOOK_COLD_PHASES: evicted through before each timed phase
+    size_t flush_words = 0;
     void *mpc_buf = nullptr;     // cmcg_comp_mass_percent results (allocated on first use)
     void *cen_buf = nullptr;     // cmcg_central_calculate results (allocated on first use)
 };
@@ -183,13 +185,30 @@ hipEvent_t get_event(cmcg_ctx *c)
     return e;
 }
 
+// reads and rewrites a buffer larger than the L2s and the MALL together
+struct flush_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, uint4 *__restrict__ buf, size_t words)
+    {
+        for (size_t i = (size_t)kg_.bid * 256 + threadIdx.x; i < words; i += (size_t)kg_.nb * 256) {
+            uint4 v = buf[i];
+            v.x += 1u;
+            buf[i] = v;
+        }
+    }
+};
+
 struct PhaseTimer {
     cmcg_ctx *c;
     const char *name;
     hipEvent_t a = nullptr, b = nullptr;
     PhaseTimer(cmcg_ctx *c_, const char *n) : c(c_), name(n)
     {
-        if (c->timing) { a = get_event(c); hipEventRecord(a, c->stream); }
+        if (c->timing) {
+            if (c->flush_buf) kl<flush_kernel>(dim3(8192), dim3(256), 0, c->stream, c->flush_buf, c->flush_words);
+            a = get_event(c);
+            hipEventRecord(a, c->stream);
+        }
     }
     ~PhaseTimer()
     {
@@ -486,6 +505,7 @@ int cmcg_destroy(cmcg_ctx *c)
     for (auto e : c->event_pool) hipEventDestroy(e);
     for (void *p : c->allocs) hipFree(p);
     if (c->io) hipFree(c->io);
+    if (c->flush_buf) hipFree(c->flush_buf);
     if (c->mpc_buf) hipFree(c->mpc_buf);
     if (c->cen_buf) hipFree(c->cen_buf);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1854,6 +1874,17 @@ int cmcg_test_hook(cmcg_ctx *c, int32_t hook, int64_t value)
     case CMCG_HOOK_MLP_MFMA:
         if (value < 0 || value > 1) return CMCG_E_ARG;
         c->mlp_mfma = value != 0;
+        return CMCG_OK;
+    case CMCG_HOOK_COLD_PHASES:
+        if (value < 0 || value > 65536) return CMCG_E_ARG;
+        if (sync(c) != hipSuccess) return CMCG_E_HIP;
+        if (c->flush_buf) { hipFree(c->flush_buf); c->flush_buf = nullptr; c->flush_words = 0; }
+        if (value) {
+            const size_t bytes = (size_t)value << 20;
+            if (hipMalloc((void **)&c->flush_buf, bytes) != hipSuccess) { c->flush_buf = nullptr; return CMCG_E_NOMEM; }
+            if (hipMemset(c->flush_buf, 0, bytes) != hipSuccess) return CMCG_E_HIP;
+            c->flush_words = bytes / sizeof(uint4);
+        }
         return CMCG_OK;
     default:
         return CMCG_E_ARG;

@@ -324,6 +324,11 @@ int cmcg_central_calculate(cmcg_ctx *ctx, cmcg_central *out);
  * predictor, to measure the most any learned proposal could save.  0 returns to the
  * network.  Needs cmcg_mlp_init. */
 #define CMCG_HOOK_PERFECT_BRACKETS 4
+/* CMCG_HOOK_COLD_PHASES: value > 0 makes every timed phase of an eager step start
+ * with cold caches: before the phase's first event a kernel reads and rewrites a
+ * `value` MiB device buffer (allocated here), evicting the step's data from the L2s
+ * and the MALL; the flush lies outside the phase's events.  0 frees the buffer. */
+#define CMCG_HOOK_COLD_PHASES 5
 int cmcg_test_hook(cmcg_ctx *ctx, int32_t hook, int64_t value);
 
 int cmcg_orbit_variants(cmcg_ctx *ctx, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse,

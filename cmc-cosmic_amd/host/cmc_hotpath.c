@@ -2,7 +2,8 @@
  * The reference's hot-path call surface on top of the cmcg C-ABI (one device,
  * one process-wide context).  Each function keeps the reference name, arguments
  * and effect on the cluster state; the state itself lives on the device and the
- * scalar globals are mirrored in cmc_hp after every call.
+ * scalar globals are mirrored in cmc_hp after every call, and written to the
+ * reference's own globals when those are linked in (below).
  */
 #include "../../include/cmc_hotpath.h"
 #include <stdio.h>
@@ -12,6 +13,25 @@
 cmc_hotpath_globals cmc_hp;
 cmc_hotpath_lagrange cmc_lag = {0, 0, 0.0, 1.0, {0}, {0}, {0}, {0}, {0}, {0}, {0}, {0}, {0}};
 cmc_hotpath_central cmc_cen;
+
+/* The reference's own globals that the path updates (include/cmc/cmc_vars.h:11-12,
+ * 32-33,38,188,293; clus_struct_t and Etotal_struct_t, include/cmc/cmc.h:1690-1745).
+ * Weak references: linked into the reference they bind to its definitions, and every
+ * call leaves them as the reference's own functions would, so its main loop reads
+ * them unchanged; linked alone they are null and only cmc_hp is kept. */
+typedef struct { long N_MAX, N_MAX_NEW, N_STAR, N_STAR_NEW, N_BINARY; } cmc_ref_clus;
+typedef struct { double tot, New, ini, K, P, Eint, Eb; } cmc_ref_etotal;
+extern cmc_ref_clus clus __attribute__((weak));
+extern cmc_ref_etotal Etotal __attribute__((weak));
+extern double Mtotal __attribute__((weak)), Rtidal __attribute__((weak)), max_r __attribute__((weak)),
+    Dt __attribute__((weak)), TotalTime __attribute__((weak)), TidalMassLoss __attribute__((weak)),
+    Eescaped __attribute__((weak)), Jescaped __attribute__((weak)), Etidal __attribute__((weak)),
+    Eoops __attribute__((weak));
+
+static void set_ref(double *g, double v)
+{
+    if (g) *g = v;
+}
 
 static cmcg_ctx *g_ctx = NULL;
 static uint64_t g_step = 0;   /* RNG step counter (tcount - 1 in the reference's main loop) */
@@ -44,6 +64,25 @@ static void refresh(void)
     cmc_hp.Ekin = s.Ekin;
     cmc_hp.Epot = s.Epot;
     cmc_hp.Eoops = s.Eoops;
+    if (&clus) {
+        clus.N_MAX = cmc_hp.N_MAX;
+        clus.N_MAX_NEW = cmc_hp.N_MAX_NEW;
+    }
+    if (&Etotal) {
+        Etotal.tot = s.Etot;
+        Etotal.K = s.Ekin;
+        Etotal.P = s.Epot;
+    }
+    set_ref(&Mtotal, s.Mtotal);
+    set_ref(&Rtidal, s.Rtidal);
+    set_ref(&max_r, s.max_r);
+    set_ref(&Dt, s.Dt);
+    set_ref(&TotalTime, s.TotalTime);
+    set_ref(&TidalMassLoss, s.TidalMassLoss);
+    set_ref(&Eescaped, s.Eescaped);
+    set_ref(&Jescaped, s.Jescaped);
+    set_ref(&Etidal, s.Etidal);
+    set_ref(&Eoops, s.Eoops);
 }
 
 static int check(int rc, const char *where)

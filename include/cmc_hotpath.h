@@ -28,7 +28,9 @@ typedef struct {
     int circular_flag;
 } cmc_orbit_rs_t;
 
-/* Mirror of the reference globals the path reads or writes (refreshed after every call). */
+/* Mirror of the reference globals the path reads or writes (refreshed after every call).
+ * When the reference's own globals are linked in (clus, Etotal, Dt, Mtotal, Rtidal ...,
+ * include/cmc/cmc_vars.h), the library writes those too, through weak references. */
 typedef struct {
     long N_STAR, N_MAX, N_MAX_NEW;      /* clus.* (include/cmc/cmc.h:1690-1711) */
     double Mtotal, Rtidal, max_r, Dt, TotalTime, madhoc;

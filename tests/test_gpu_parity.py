@@ -665,6 +665,79 @@ def test_gpu_host_shim_reference_names(gpu, O):
     h.cmc_hotpath_finalize()
 
 
+_GLOBALS_SCRIPT = r"""
+import ctypes as C, json, os, sys
+repo = sys.argv[1]
+g = C.CDLL(os.path.join(repo, "tests", "libref_globals.so"), mode=C.RTLD_GLOBAL)
+sys.path.insert(0, repo)
+import importlib
+cm = importlib.import_module("cmc-cosmic_amd")
+h = cm.hostlib
+cl = cm.make_king(40000, 7.0, seed=5)
+p = cm.default_params(cl.n, orbit_r=cl.rtid, seed=3)
+assert h.cmc_hotpath_init(C.byref(p), cl.n, C.byref(cl.soa()), 0) == 0
+h.cmc_hotpath_ctx.restype = C.c_void_p
+shim = cm.Context.__new__(cm.Context)
+shim.h = C.c_void_p(h.cmc_hotpath_ctx())
+shim.init_cluster()
+h.calc_potential_new.restype = None
+h.calc_potential_new()
+dbl = lambda name: C.c_double.in_dll(g, name).value
+out = {"after_init": {"Mtotal": dbl("Mtotal"), "N_MAX": C.c_long.in_dll(g, "clus").value,
+                      "dev_Mtotal": shim.scalars()["Mtotal"], "dev_n_max": shim.scalars()["n_max"]}}
+h.dynamics_apply.argtypes = [C.c_double, C.c_void_p]
+h.calc_timestep.argtypes = [C.c_void_p]
+h.calc_timestep(None)
+out["Dt_after_calc_timestep"] = [dbl("Dt"), shim.scalars()["Dt"]]
+h.dynamics_apply(dbl("Dt"), None)
+for name in ("energy_conservation1", "new_orbits_calculate", "energy_conservation2", "tidally_strip_stars",
+             "ComputeIntermediateEnergy", "qsorts_new", "post_sort_comm", "calc_potential_new",
+             "energy_conservation3", "compute_energy_new"):
+    getattr(h, name).restype = None
+    getattr(h, name)()
+sc = shim.scalars()
+class Clus(C.Structure):
+    _fields_ = [("N_MAX", C.c_long), ("N_MAX_NEW", C.c_long), ("N_STAR", C.c_long), ("N_STAR_NEW", C.c_long),
+                ("N_BINARY", C.c_long)]
+class Etot(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("tot", "New", "ini", "K", "P", "Eint", "Eb")]
+cs, et = Clus.in_dll(g, "clus"), Etot.in_dll(g, "Etotal")
+out["ref"] = {"N_MAX": cs.N_MAX, "N_MAX_NEW": cs.N_MAX_NEW, "Etot": et.tot, "Ekin": et.K, "Epot": et.P,
+              **{k: dbl(k) for k in ("Mtotal", "Rtidal", "max_r", "Dt", "TotalTime", "TidalMassLoss", "Eescaped",
+                                     "Jescaped", "Etidal", "Eoops")}}
+out["dev"] = {"N_MAX": sc["n_max"], "N_MAX_NEW": sc["n_max_new"], "Etot": sc["Etot"], "Ekin": sc["Ekin"],
+              "Epot": sc["Epot"], **{k: sc[k] for k in ("Mtotal", "Rtidal", "max_r", "Dt", "TotalTime",
+                                                       "TidalMassLoss", "Eescaped", "Jescaped", "Etidal", "Eoops")}}
+shim.h = C.c_void_p()
+h.cmc_hotpath_finalize()
+print("GLOBALS " + json.dumps(out), flush=True)
+"""
+
+
+def test_gpu_shim_writes_reference_globals(gpu):
+    """Linked beside the reference's own globals (a fixture library with the names and
+    layouts of include/cmc/cmc_vars.h, loaded first as the reference executable's
+    definitions would be), the shim's calls leave clus.N_MAX, Dt, Mtotal, Rtidal,
+    Etotal ... equal to the device scalars after each call, so the reference's main
+    loop reads them unchanged.  Runs in a fresh process (the fixture must be loaded
+    before the shim)."""
+    import json
+    import os
+    import subprocess
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    say("shim writes the reference's globals (subprocess)")
+    r = subprocess.run([sys.executable, "-c", _GLOBALS_SCRIPT, repo], capture_output=True, text=True, timeout=300)
+    line = [l for l in r.stdout.splitlines() if l.startswith("GLOBALS ")]
+    assert r.returncode == 0 and line, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    out = json.loads(line[0][len("GLOBALS "):])
+    ai = out["after_init"]
+    assert ai["N_MAX"] == ai["dev_n_max"] > 0 and ai["Mtotal"] == ai["dev_Mtotal"] > 0, ai
+    dt = out["Dt_after_calc_timestep"]
+    assert dt[0] == dt[1] > 0, dt
+    assert out["ref"] == out["dev"], (out["ref"], out["dev"])
+
+
 STEP_FIELDS = ("r", "m", "phi", "vr", "vt", "E", "J", "id", "r_peri", "r_apo")
 STEP_SCALARS = ("n_max", "Mtotal", "Rtidal", "max_r", "Dt", "TotalTime", "TidalMassLoss", "Eescaped", "Jescaped",
                 "Etidal", "Etot", "Ekin", "Epot")

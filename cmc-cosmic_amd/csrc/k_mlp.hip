@@ -29,6 +29,27 @@ __device__ __forceinline__ float dsilu(float z)
     return s * (1.0f + z * (1.0f - s));
 }
 
+typedef float float4_t __attribute__((ext_vector_type(4)));
+constexpr int LDP = NH + 1;         // padded LDS row (conflict-free matrix-core operand reads)
+
+// The wave's 16 x 64 block of A (64 x K) B (K x 64) on the matrix cores (16x16x4 f32
+// MFMAs, f32 accumulation into acc): rows i0 .. i0 + 15 of A, all 64 columns of B in
+// four 16-column blocks.  Lane l supplies A[i0 + l % 16][4 kc + l / 16] and
+// B[4 kc + l / 16][16 cb + l % 16] and holds C[i0 + 4 (l / 16) + v][16 cb + l % 16] in
+// acc[cb][v].
+template <int K, class FA, class FB>
+__device__ __forceinline__ void wave_gemm_16x64(int i0, FA a, FB b, float4_t (&acc)[4])
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+#pragma unroll 4
+    for (int kc = 0; kc < K / 4; kc++) {
+        const float av = a(i0 + r, 4 * kc + q);
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++)
+            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b(4 * kc + q, 16 * cb + r), acc[cb], 0, 0, 0);
+    }
+}
+
 // row r of the training set for minibatch position i: a multiplicative shuffle
 __device__ __forceinline__ int64_t shuffled(int64_t i, int64_t count, uint64_t mult, uint64_t off)
 {
@@ -36,7 +57,10 @@ __device__ __forceinline__ int64_t shuffled(int64_t i, int64_t count, uint64_t m
 }
 
 // One block: tiles blockIdx.x, blockIdx.x + gridDim.x, ... of the batch.  Writes the
-// block's gradient partial (MLP_NPARAM floats) and its loss partial.
+// block's gradient partial (MLP_NPARAM floats) and its loss partial.  The three 64 x
+// 64 x 64 products of a tile (layer 2 forward, layer 2 weight gradient, error at
+// layer 1) run on the matrix cores, each wave taking 16 rows; the 6- and 2-wide
+// layers stay on the vector units.
 struct mlp_fwdbwd_kernel {
     static constexpr int LB = FT, MB = 0;
     static __device__ __forceinline__ void run(KGrid kg_, const float *__restrict__ theta, const MlpNorm *__restrict__ nrm,
@@ -44,24 +68,21 @@ struct mlp_fwdbwd_kernel {
                                                         int64_t batch, uint64_t mult, uint64_t off, float inv_b,
                                                         float *__restrict__ gpart, double *__restrict__ lpart)
 {
-    __shared__ float W1T[NIN][NH], B1[NH], W2[NH][NH], W2T[NH][NH], B2[NH], W3[NOUT][NH], B3[NOUT];
+    __shared__ float W1T[NIN][NH], B1[NH], W2[NH][LDP], B2[NH], W3[NOUT][NH], B3[NOUT];
     __shared__ float X[TILE][NIN], T[TILE][NOUT], D3[TILE][NOUT];
-    __shared__ float Z1[TILE][NH], H1[TILE][NH], Z2[TILE][NH], H2[TILE][NH];
+    __shared__ float Z1[TILE][LDP], H1[TILE][LDP], Z2[TILE][LDP], H2[TILE][LDP];
     __shared__ float xm[NIN], xs[NIN], ym[NOUT], ys[NOUT];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, wv = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
     for (int e = t; e < NH * NIN; e += FT) W1T[e % NIN][e / NIN] = theta[OFF_W1 + e];
-    for (int e = t; e < NH * NH; e += FT) {
-        float w = theta[OFF_W2 + e];
-        W2[e / NH][e % NH] = w;
-        W2T[e % NH][e / NH] = w;
-    }
+    for (int e = t; e < NH * NH; e += FT) W2[e / NH][e % NH] = theta[OFF_W2 + e];
     for (int e = t; e < NOUT * NH; e += FT) W3[e / NH][e % NH] = theta[OFF_W3 + e];
     if (t < NH) { B1[t] = theta[OFF_B1 + t]; B2[t] = theta[OFF_B2 + t]; }
     if (t < NOUT) { B3[t] = theta[OFF_B3 + t]; ym[t] = nrm->ym[t]; ys[t] = nrm->ys[t]; }
     if (t < NIN) { xm[t] = nrm->xm[t]; xs[t] = nrm->xs[t]; }
-    // gradient accumulators owned by this thread
-    float gW2[16], gW1[2] = {0.f, 0.f}, gB = 0.f, gW3 = 0.f, gB3 = 0.f;
-    for (int q = 0; q < 16; q++) gW2[q] = 0.f;
+    // gradient accumulators owned by this thread (gW2: the wave's 16 rows of W2's
+    // gradient in the matrix-core layout)
+    float4_t gW2[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    float gW1[2] = {0.f, 0.f}, gB = 0.f, gW3 = 0.f, gB3 = 0.f;
     double loss = 0.0;
     __syncthreads();
     const int64_t ntiles = (batch + TILE - 1) / TILE;
@@ -85,13 +106,20 @@ struct mlp_fwdbwd_kernel {
             H1[s][u] = silu(z);
         }
         __syncthreads();
-        // 3. layer 2
-        for (int q = 0; q < 16; q++) {
-            int e = t + FT * q, s = e / NH, u = e % NH;
-            float z = B2[u];
-            for (int v = 0; v < NH; v++) z += W2T[v][u] * H1[s][v];
-            Z2[s][u] = z;
-            H2[s][u] = silu(z);
+        // 3. layer 2: Z2[s][u] = sum_v H1[s][v] W2[u][v] + B2[u]
+        {
+            float4_t acc[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+            wave_gemm_16x64<NH>(16 * wv, [&](int s, int v) { return H1[s][v]; },
+                                [&](int v, int u) { return W2[u][v]; }, acc);
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int s = 16 * wv + 4 * lq + v, u = 16 * cb + lr;
+                    const float z = acc[cb][v] + B2[u];
+                    Z2[s][u] = z;
+                    H2[s][u] = silu(z);
+                }
         }
         __syncthreads();
         // 4. output, loss, output error
@@ -126,23 +154,26 @@ struct mlp_fwdbwd_kernel {
             Z2[s][u] = d * dsilu(Z2[s][u]);
         }
         __syncthreads();
-        // 6. layer 2 gradients; error at layer 1 (in place of Z1)
-        for (int q = 0; q < 16; q++) {
-            int e = t + FT * q, u = e / NH, v = e % NH;
-            float g = 0.f;
-            for (int s = 0; s < TILE; s++) g += Z2[s][u] * H1[s][v];
-            gW2[q] += g;
-        }
+        // 6. layer 2 gradients (gW2[u][v] += sum_s Z2[s][u] H1[s][v]); error at layer 1
+        // (in place of Z1: sum_u Z2[s][u] W2[u][v] times SiLU'(Z1[s][v]))
+        wave_gemm_16x64<TILE>(16 * wv, [&](int u, int s) { return Z2[s][u]; },
+                              [&](int s, int v) { return H1[s][v]; }, gW2);
         if (t >= NH && t < 2 * NH) {
             float g = 0.f;
             for (int s = 0; s < TILE; s++) g += Z2[s][t - NH];
             gB += g;
         }
-        for (int q = 0; q < 16; q++) {
-            int e = t + FT * q, s = e / NH, v = e % NH;
-            float d = 0.f;
-            for (int u = 0; u < NH; u++) d += Z2[s][u] * W2[u][v];
-            Z1[s][v] = d * dsilu(Z1[s][v]);
+        {
+            float4_t acc[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+            wave_gemm_16x64<NH>(16 * wv, [&](int s, int u) { return Z2[s][u]; },
+                                [&](int u, int v) { return W2[u][v]; }, acc);
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int s = 16 * wv + 4 * lq + v, c = 16 * cb + lr;
+                    Z1[s][c] = acc[cb][v] * dsilu(Z1[s][c]);
+                }
         }
         __syncthreads();
         // 7. layer 1 gradients
@@ -163,7 +194,10 @@ struct mlp_fwdbwd_kernel {
         __syncthreads();
     }
     float *gp = gpart + (int64_t)kg_.bid * MLP_NPARAM;
-    for (int q = 0; q < 16; q++) gp[OFF_W2 + t + FT * q] = gW2[q];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) gp[OFF_W2 + (16 * wv + 4 * lq + v) * NH + 16 * cb + lr] = gW2[cb][v];
     for (int k = 0; k < 2; k++)
         if (t + FT * k < NH * NIN) gp[OFF_W1 + t + FT * k] = gW1[k];
     if (t < NH) gp[OFF_B1 + t] = gB;
@@ -320,7 +354,6 @@ struct mlp_harvest_kernel {
 // leave the registers; the weights stay in registers as A operands for all of the
 // wave's tiles.  f16 keeps ~3 decimal digits: a bracket proposal only.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
 
 struct MfmaNet {
     half4_t a1[4], a2[4][4], a3[4];

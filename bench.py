@@ -161,7 +161,7 @@ def orbit_ceiling(cm, ctx, margin, steps=3):
     return out
 
 
-def ensemble_bench(cm, steps, seed=7, mlp=None):
+def ensemble_bench(cm, steps, seed=7, mlp=None, concurrent=True):
     """64 independent clusters stepped together: N log-uniform on [1e4, 2e6] (seeded),
     32 Kroupa Plummer and 32 Kroupa King models with W0 = 3, 5, 7, 9, one context each,
     each with its orbit sampler's launch shape autotuned.  Times the merged ensemble step (one segmented launch per kernel over all 64
@@ -198,15 +198,18 @@ def ensemble_bench(cm, steps, seed=7, mlp=None):
         t_ens += time.perf_counter() - t
         stars += nl
     merged = cm.ensemble_stats()
-    for _ in range(2):
-        cm.ensemble_step(ctxs, concurrent=True)
-    con_stars, t_con = 0, 0.0
-    for _ in range(max(1, steps // 2)):
-        nl = live()
-        t = time.perf_counter()
-        cm.ensemble_step(ctxs, concurrent=True)
-        t_con += time.perf_counter() - t
-        con_stars += nl
+    con = None
+    if concurrent:
+        for _ in range(2):
+            cm.ensemble_step(ctxs, concurrent=True)
+        con_stars, t_con = 0, 0.0
+        for _ in range(max(1, steps // 2)):
+            nl = live()
+            t = time.perf_counter()
+            cm.ensemble_step(ctxs, concurrent=True)
+            t_con += time.perf_counter() - t
+            con_stars += nl
+        con = {"value": con_stars / t_con, "ms_per_step": t_con / max(1, steps // 2) * 1e3}
     learned = None
     if mlp is not None:
         # the learned brackets on every cluster: the network trained on the 4e6 King
@@ -245,13 +248,14 @@ def ensemble_bench(cm, steps, seed=7, mlp=None):
            "steps": steps, "value": stars / t_ens, "unit": "stars/s", "ms_per_step": t_ens / steps * 1e3,
            "merged_launches": merged[0], "recorded_launches": merged[1],
            "orbit_shapes": sorted(set(shapes)),
-           "concurrent_graphs": {"value": con_stars / t_con, "ms_per_step": t_con / max(1, steps // 2) * 1e3},
+           "concurrent_graphs": con,
            "learned": learned,
            "sequential_eager": {"value": seq_stars / t_seq, "ms_per_step": t_seq / max(1, steps // 2) * 1e3},
            "note": "R_vir sets physical units only and does not enter the Henon-unit step"}
     log(f"ensemble: {out['value']:.3e} stars/s ({out['ms_per_step']:.2f} ms per step of all 64, "
-        f"{merged[0]} merged launches of {merged[1]}), concurrent graphs "
-        f"{out['concurrent_graphs']['value']:.3e} stars/s, one by one {out['sequential_eager']['value']:.3e} stars/s"
+        f"{merged[0]} merged launches of {merged[1]})"
+        + (f", concurrent graphs {con['value']:.3e} stars/s" if con else "")
+        + f", one by one {out['sequential_eager']['value']:.3e} stars/s"
         + (f", learned brackets {learned['value']:.3e} stars/s ({learned['brackets_accepted']} accepted, "
            f"{learned['brackets_rejected']} rejected)" if learned else ""))
     return out
@@ -269,6 +273,8 @@ def main():
     ap.add_argument("--train-iters", type=int, default=6000)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-cold", action="store_true", help="skip the cold-cache phase timings")
+    ap.add_argument("--no-concurrent", action="store_true",
+                    help="skip the ensemble's concurrent-graphs comparison leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported "
@@ -393,7 +399,8 @@ def main():
                "orbit_phase_ceiling": ceiling}
 
     roofline = orbit_roofline(phases["orbits_kernel"], float(np.mean(live)))
-    ens = None if args.no_ensemble else ensemble_bench(cm, 10, mlp=(*ctx.mlp_params(), margin))
+    ens = None if args.no_ensemble else ensemble_bench(cm, 10, mlp=(*ctx.mlp_params(), margin),
+                                                       concurrent=not args.no_concurrent)
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle step on the same {cl.n} stars")

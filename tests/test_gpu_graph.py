@@ -78,6 +78,18 @@ def test_gpu_ensemble_step_matches_eager(gpu):
                 say("step", k, "merged launches", launches, "recorded", recorded)
                 assert launches * len(ns) == recorded
             say("step", k, "all clusters identical")
+        # one cluster's context closed: the plans that stepped it are dropped, and the
+        # other three step on together
+        ens.pop(1).close()
+        eager.pop(1).close()
+        for k in range(2):
+            for c in eager:
+                c.step()
+            stats, errs = cm.ensemble_step(ens)
+            assert errs == [0] * len(ens)
+            for i in range(len(ens)):
+                _same(eager[i], ens[i], f"after close: cluster {i} step {k}")
+        say("after closing one context: identical")
     finally:
         for c in eager + ens:
             c.close()

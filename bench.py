@@ -185,8 +185,9 @@ def ensemble_bench(cm, steps, seed=7, mlp=None, concurrent=True):
     log(f"ensemble: 64 clusters, {int(ns.sum())} stars, set up in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     shapes = [c.autotune(reps=2)["variant"] for c in ctxs]   # each cluster's orbit-sampler launch shape
+    rshapes = [c.autotune_relax(reps=2)["threads"] for c in ctxs]   # and its relaxation block size
     log(f"ensemble: autotuned in {time.perf_counter() - t0:.1f} s, "
-        f"{len(set(shapes))} distinct orbit-sampler shapes")
+        f"{len(set(shapes))} distinct orbit-sampler shapes, {len(set(rshapes))} relaxation block sizes")
     for _ in range(2):                   # captures both pointer states' graphs
         cm.ensemble_step(ctxs)
     live = lambda: sum(c.scalars()["n_max"] for c in ctxs)
@@ -247,7 +248,7 @@ def ensemble_bench(cm, steps, seed=7, mlp=None, concurrent=True):
     out = {"clusters": 64, "stars": int(ns.sum()), "n_min": int(ns.min()), "n_max": int(ns.max()),
            "steps": steps, "value": stars / t_ens, "unit": "stars/s", "ms_per_step": t_ens / steps * 1e3,
            "merged_launches": merged[0], "recorded_launches": merged[1],
-           "orbit_shapes": sorted(set(shapes)),
+           "orbit_shapes": sorted(set(shapes)), "relax_block_sizes": sorted(set(rshapes)),
            "concurrent_graphs": con,
            "learned": learned,
            "sequential_eager": {"value": seq_stars / t_seq, "ms_per_step": t_seq / max(1, steps // 2) * 1e3},
@@ -302,6 +303,10 @@ def main():
     for v in tune["variants"]:
         log(f"  variant threads {v['threads']:4d} samples {v['coarse']:5d} tries-cap {v['cap']:3d} "
             f"min blocks/CU {v['min_blocks']}: {v['ms']:.3f} ms")
+    rtune = ctx.autotune_relax(reps=3)
+    log(f"autotune: relaxation {rtune['threads']} threads per block ({rtune['ms_best']:.3f} ms vs default "
+        f"{rtune['ms_default']:.3f} ms)")
+    tune["relax"] = rtune
 
     def timed(n_steps):
         ctx.timer_reset()

@@ -162,6 +162,7 @@ _sigs = {
     "cmcg_ensemble_step": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
     "cmcg_ensemble_step_concurrent": [_P(_ctx), C.c_int, _P(OrbitStats), _P(C.c_int32)],
     "cmcg_ensemble_stats": [_P(C.c_int64), _P(C.c_int64)],
+    "cmcg_autotune_relax": [_ctx, C.c_int, _P(C.c_int32), _P(C.c_double), _P(C.c_double)],
     "cmcg_rng_draws": [_ctx, C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int, _P(C.c_double)],
     "cmcg_timer_enable": [_ctx, C.c_int],
     "cmcg_timer_get": [_ctx, C.c_char_p, _P(C.c_double), _P(C.c_int64)],
@@ -183,6 +184,7 @@ HOOK_ORBIT_VARIANT = 2
 HOOK_MLP_MFMA = 3
 HOOK_PERFECT_BRACKETS = 4
 HOOK_COLD_PHASES = 5
+HOOK_RELAX_SHAPE = 6
 for _name, _args in _sigs.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args
@@ -530,6 +532,13 @@ class Context:
                "ms_default": default.value}
         out["variants"] = self.orbit_variants()
         return out
+
+    def autotune_relax(self, reps=3):
+        """cmcg_autotune_relax: the relaxation pass's block size, timed on this state."""
+        th, best, default = C.c_int32(), C.c_double(), C.c_double()
+        _chk(lib.cmcg_autotune_relax(self.h, int(reps), C.byref(th), C.byref(best), C.byref(default)),
+             "cmcg_autotune_relax")
+        return {"threads": th.value, "ms_best": best.value, "ms_default": default.value}
 
     def comp_mass_percent(self, pc, bh_include=0, initial_total_mass=1.0, minimum_r=0.0, fill=np.nan):
         """cmcg_comp_mass_percent: dict of arrays (entries not reached = fill) and the count reached."""

@@ -52,6 +52,7 @@ struct cmcg_ctx {
     uint64_t serial = 0;         // unique per context (ensemble plans name their clusters by it)
     uint64_t gen = 0;            // bumped whenever the graphs are dropped
     int orb_variant = 0;         // launch shape of the orbit sampler (autotuned)
+    int relax_threads = 256;     // block size of the relaxation kernel (autotuned)
     std::vector<double> tune_ms; // per-variant time of the last cmcg_autotune
     uint4 *flush_buf = nullptr;  // CMCG_HOOK_COLD_PHASES: evicted through before each timed phase
     size_t flush_words = 0;
@@ -1054,7 +1055,7 @@ static int enqueue_step(cmcg_ctx *c, int64_t nub, bool graph)
     {
         // relaxation with energy_conservation1 fused in
         PhaseTimer t(c, "relax");
-        LAUNCH(launch_relax(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream, true));
+        LAUNCH(launch_relax(c->f, c->s, c->kp, c->T, c->sc, nub, c->stream, true, c->relax_threads));
         LAUNCH(launch_after_dynamics(c->s, c->stream));
     }
     {
@@ -1875,6 +1876,14 @@ int cmcg_test_hook(cmcg_ctx *c, int32_t hook, int64_t value)
         if (value < 0 || value > 1) return CMCG_E_ARG;
         c->mlp_mfma = value != 0;
         return CMCG_OK;
+    case CMCG_HOOK_RELAX_SHAPE: {
+        bool ok = false;
+        for (int v = 0; v < RELAX_NSHAPES; v++) ok = ok || RELAX_SHAPES[v] == value;
+        if (!ok) return CMCG_E_ARG;
+        if (c->relax_threads != (int)value) invalidate_graphs(c);
+        c->relax_threads = (int)value;
+        return CMCG_OK;
+    }
     case CMCG_HOOK_COLD_PHASES:
         if (value < 0 || value > 65536) return CMCG_E_ARG;
         if (sync(c) != hipSuccess) return CMCG_E_HIP;
@@ -1889,6 +1898,63 @@ int cmcg_test_hook(cmcg_ctx *c, int32_t hook, int64_t value)
     default:
         return CMCG_E_ARG;
     }
+}
+
+int cmcg_autotune_relax(cmcg_ctx *c, int reps, int32_t *threads, double *ms_best, double *ms_default)
+{
+    if (!c || reps < 1) return CMCG_E_ARG;
+    DevScalars saved;
+    if (sync(c) != hipSuccess || hipMemcpy(&saved, c->s, sizeof(saved), hipMemcpyDeviceToHost) != hipSuccess)
+        return CMCG_E_HIP;
+    // the fields the fused relaxation pass rewrites, restored after every trial
+    double *const fields[] = {c->f.vr, c->f.vt, c->f.E, c->f.J, c->f.Y, c->f.vrold, c->f.vtold, c->f.Uoldrold};
+    constexpr int NF = (int)(sizeof(fields) / sizeof(fields[0]));
+    const size_t bytes = (size_t)(c->n_cap + 2) * sizeof(double);
+    double *keep = nullptr;
+    if (hipMalloc(&keep, bytes * NF) != hipSuccess) return CMCG_E_NOMEM;
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc = CMCG_OK;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) rc = CMCG_E_HIP;
+    for (int i = 0; i < NF && !rc; i++)
+        if (hipMemcpyAsync(keep + (c->n_cap + 2) * i, fields[i], bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            rc = CMCG_E_HIP;
+    int best = c->relax_threads;
+    double best_ms = 1e300, def_ms = 0.0;
+    for (int v = 0; v < RELAX_NSHAPES && !rc; v++) {
+        double total = 0.0;
+        for (int it = 0; it <= reps && !rc; it++) {      // the first run warms up and is not timed
+            if (it && hipEventRecord(a, c->stream) != hipSuccess) { rc = CMCG_E_HIP; break; }
+            if (launch_relax(c->f, c->s, c->kp, c->T, c->sc, c->n_ub, c->stream, true, RELAX_SHAPES[v]) != hipSuccess) {
+                rc = CMCG_E_HIP;
+                break;
+            }
+            if (it && hipEventRecord(b, c->stream) != hipSuccess) { rc = CMCG_E_HIP; break; }
+            for (int i = 0; i < NF && !rc; i++)
+                if (hipMemcpyAsync(fields[i], keep + (c->n_cap + 2) * i, bytes, hipMemcpyDeviceToDevice, c->stream) !=
+                    hipSuccess)
+                    rc = CMCG_E_HIP;
+            if (rc || sync(c) != hipSuccess) { rc = CMCG_E_HIP; break; }
+            if (it) {
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, a, b);
+                total += ms;
+            }
+            if (hipMemcpy(c->s, &saved, sizeof(saved), hipMemcpyHostToDevice) != hipSuccess) rc = CMCG_E_HIP;
+        }
+        const double avg = total / reps;
+        if (v == 0) def_ms = avg;
+        if (avg < best_ms) { best_ms = avg; best = RELAX_SHAPES[v]; }
+    }
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    hipFree(keep);
+    if (rc) return rc;
+    if (best != c->relax_threads) invalidate_graphs(c);
+    c->relax_threads = best;
+    if (threads) *threads = best;
+    if (ms_best) *ms_best = best_ms;
+    if (ms_default) *ms_default = def_ms;
+    return CMCG_OK;
 }
 
 int cmcg_orbit_variants(cmcg_ctx *c, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse, int32_t *cap,

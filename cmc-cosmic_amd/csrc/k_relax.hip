@@ -384,10 +384,11 @@ __device__ __forceinline__ void ec1_star(DevFields &f, const KParams &P, int64_t
 }
 
 // EC1: energy_conservation1 fused into the pass (the step's form; the entry point of
-// that name keeps its own kernel).
-template <bool EC1>
+// that name keeps its own kernel).  T: threads per block (an autotuned launch shape;
+// pair p is thread p of the grid whatever T is, so results do not depend on it).
+template <bool EC1, int T>
 struct relax_kernel {
-    static constexpr int LB = 256, MB = 0;
+    static constexpr int LB = T, MB = 0;
     static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
                                                     const Rng *__restrict__ wave_base,
                                                     const NibbleTables<8> *__restrict__ nib)
@@ -584,8 +585,29 @@ hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st)
     return hipGetLastError();
 }
 
+template <bool EC1, int TH>
+void relax_launch(int64_t nth, hipStream_t st, const DevFields &f, DevScalars *s, const KParams &P, const Scratch &sc)
+{
+    kl<relax_kernel<EC1, TH>>(dim3((unsigned)((nth + TH - 1) / TH)), dim3(TH), 0, st, f, s, P,
+                              (const Rng *)sc.rng_wave, (const NibbleTables<8> *)sc.relax_nib);
+}
+
+template <bool EC1>
+hipError_t relax_launch_shape(int threads, int64_t nth, hipStream_t st, const DevFields &f, DevScalars *s,
+                              const KParams &P, const Scratch &sc)
+{
+    switch (threads) {
+    case 128: relax_launch<EC1, 128>(nth, st, f, s, P, sc); break;
+    case 256: relax_launch<EC1, 256>(nth, st, f, s, P, sc); break;
+    case 512: relax_launch<EC1, 512>(nth, st, f, s, P, sc); break;
+    case 1024: relax_launch<EC1, 1024>(nth, st, f, s, P, sc); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
-                        const Scratch &sc, int64_t n_ub, hipStream_t st, bool with_ec1)
+                        const Scratch &sc, int64_t n_ub, hipStream_t st, bool with_ec1, int threads)
 {
     int64_t npairs = n_ub / 2;
     if (with_ec1) {
@@ -594,13 +616,9 @@ hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, con
         if (nth <= 0) return hipSuccess;
         if (npairs > 0)
             if (hipError_t e = launch_rng_wave_bases(T, sc, 0, npairs, st)) return e;
-        kl<relax_kernel<true>>(dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, st, f, s, P,
-                           (const Rng *)sc.rng_wave, (const NibbleTables<8> *)sc.relax_nib);
-        return hipGetLastError();
+        return relax_launch_shape<true>(threads, nth, st, f, s, P, sc);
     }
     if (npairs <= 0) return hipSuccess;
     if (hipError_t e = launch_rng_wave_bases(T, sc, 0, npairs, st)) return e;
-    kl<relax_kernel<false>>(dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, f, s, P,
-                       (const Rng *)sc.rng_wave, (const NibbleTables<8> *)sc.relax_nib);
-    return hipGetLastError();
+    return relax_launch_shape<false>(threads, npairs, st, f, s, P, sc);
 }

@@ -109,9 +109,13 @@ hipError_t launch_timestep_rng_bases(const RngTable *T, DevScalars *s, const Scr
 hipError_t launch_timestep_sigma(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
                                  const Scratch &sc, int64_t n_ub, hipStream_t st);
 hipError_t launch_timestep_finalize(DevScalars *s, hipStream_t st);
+// threads: block size of the relaxation kernel, one of RELAX_SHAPES (an autotuned
+// launch shape; results do not depend on it)
+constexpr int RELAX_SHAPES[] = {256, 128, 512, 1024};
+constexpr int RELAX_NSHAPES = 4;
 hipError_t launch_relax(const DevFields &f, DevScalars *s, const KParams &P, const RngTable *T,
                         const Scratch &sc, int64_t n_ub, hipStream_t st,
-                        bool with_ec1 = false);
+                        bool with_ec1 = false, int threads = 256);
 hipError_t launch_rng_bases(const RngTable *T, DevScalars *s, const Scratch &sc, hipStream_t st);
 hipError_t launch_rng_bases_step(const RngTable *T, DevScalars *s, const Scratch &sc, int64_t step, hipStream_t st);
 // substream start of every 64-lane wave: relaxation (phase 0, pairs 64w..) or orbits

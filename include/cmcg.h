@@ -259,6 +259,12 @@ int cmcg_rng_draws(cmcg_ctx *ctx, int phase, uint64_t step, int64_t first, int64
  * the reference. */
 int cmcg_autotune(cmcg_ctx *ctx, int reps, int32_t *variant, int32_t *threads, int32_t *coarse,
                   double *ms_best, double *ms_default);
+/* The same for the relaxation pass (with energy_conservation1 fused, as in the step):
+ * times its block sizes (128, 256, 512, 1024 threads) on the context's current state,
+ * restoring the fields and scalars it rewrites after every trial, and keeps the
+ * fastest for the following steps; per-pair results do not depend on it.  No
+ * counterpart in the reference. */
+int cmcg_autotune_relax(cmcg_ctx *ctx, int reps, int32_t *threads, double *ms_best, double *ms_default);
 /* The orbit sampler's launch variants (threads per block, LDS samples, most lanes one
  * star's rejection tries get per round, blocks per CU its register allocation must
  * allow: 1 = unconstrained) and their times in the last cmcg_autotune
@@ -329,6 +335,9 @@ int cmcg_central_calculate(cmcg_ctx *ctx, cmcg_central *out);
  * `value` MiB device buffer (allocated here), evicting the step's data from the L2s
  * and the MALL; the flush lies outside the phase's events.  0 frees the buffer. */
 #define CMCG_HOOK_COLD_PHASES 5
+/* CMCG_HOOK_RELAX_SHAPE: run the relaxation pass with `value` threads per block
+ * (128, 256, 512 or 1024), as cmcg_autotune_relax would choose it. */
+#define CMCG_HOOK_RELAX_SHAPE 6
 int cmcg_test_hook(cmcg_ctx *ctx, int32_t hook, int64_t value);
 
 int cmcg_orbit_variants(cmcg_ctx *ctx, int32_t max, int32_t *count, int32_t *threads, int32_t *coarse,

@@ -225,6 +225,40 @@ def test_gpu_autotuned_shape_gives_identical_stars(gpu):
         b.close()
 
 
+def test_gpu_every_relax_shape_gives_identical_stars(gpu):
+    """Every block size of the relaxation pass, from the same state, gives a
+    bit-identical step (eager and graph); the relaxation autotuner restores the state it
+    times on, and the step after it is identical too."""
+    cm = gpu
+    say("relaxation shapes, King W0=7 N=2e5")
+    cl = cm.make_king(200000, 7.0, seed=8)
+    ref = _ctx(cm, cl)
+    try:
+        ref.step()
+        for th in (128, 512, 1024):
+            c = _ctx(cm, cl)
+            try:
+                c.test_hook(cm.HOOK_RELAX_SHAPE, th)
+                c.step()
+                _same(ref, c, f"relax shape {th}, eager")
+            finally:
+                c.close()
+        c = _ctx(cm, cl)
+        try:
+            t = c.autotune_relax(reps=2)
+            say("relax autotune", t)
+            assert t["threads"] in (128, 256, 512, 1024)
+            assert t["ms_best"] <= t["ms_default"] * 1.0001
+            c.step_graph()
+            _same(ref, c, "after relax autotune, graph")
+        finally:
+            c.close()
+        with pytest.raises(cm.CmcgError):
+            ref.test_hook(cm.HOOK_RELAX_SHAPE, 96)
+    finally:
+        ref.close()
+
+
 def test_gpu_every_orbit_variant_gives_identical_stars(gpu):
     """Every launch variant of the orbit sampler (threads per block, LDS samples, tries
     cap), run from the same state, writes bit-identical new radii,

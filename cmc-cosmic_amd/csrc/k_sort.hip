@@ -168,27 +168,60 @@ struct rs_prep_kernel {
     __syncthreads();
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)kg_.bid * RS_TILE;
-    for (int u = 0; u < RS_ITEMS; u++) {
-        const int64_t i0 = base + u * RS_THREADS;
-        const int64_t i = i0 + t;
-        if (i < n) {
-            const uint64_t k = sortable(keysrc[i + 1]);
-            const uint32_t z = k >= kinf ? Z_TAIL : (uint32_t)((k - kmin) >> S);
-            const uint64_t w = ((uint64_t)z << RS_SHIFT0) | (uint64_t)(uint32_t)(i + 1);
-            words[i] = w;
-            for (int p = P0; p < 8; p++) atomicAdd(&h[p - P0][(w >> (8 * p)) & 255], 1u);
-        }
-        if (aos && i0 < n) {        // block-uniform: pack the chunk through LDS, write it contiguously
+    auto key_word = [&](double r, int64_t i) {
+        const uint64_t k = sortable(r);
+        const uint32_t z = k >= kinf ? Z_TAIL : (uint32_t)((k - kmin) >> S);
+        const uint64_t w = ((uint64_t)z << RS_SHIFT0) | (uint64_t)(uint32_t)(i + 1);
+        words[i] = w;
+        for (int p = P0; p < 8; p++) atomicAdd(&h[p - P0][(w >> (8 * p)) & 255], 1u);
+    };
+    if (aos && g.nf == W - 1) {
+        // the usual full record, software-pipelined: the next chunk's field loads are in
+        // flight while this chunk goes through LDS
+        uint64_t v[W - 1];
+        double kr = 0.0;
+        auto load = [&](int64_t i) {
             if (i < n) {
-                for (int f = 0; f < g.nf; f++) recs[LD * t + f] = g.src[f][i + 1];
-                recs[LD * t + g.nf] = (uint64_t)__double_as_longlong(keysrc[i + 1]);
+#pragma unroll
+                for (int f = 0; f < W - 1; f++) v[f] = g.src[f][i + 1];
+                kr = keysrc[i + 1];
             }
+        };
+        load(base + t);
+        for (int u = 0; u < RS_ITEMS; u++) {
+            const int64_t i0 = base + u * RS_THREADS;
+            if (i0 >= n) break;                          // block-uniform
+            const int64_t i = i0 + t;
+            if (i < n) {
+                key_word(kr, i);
+#pragma unroll
+                for (int f = 0; f < W - 1; f++) recs[LD * t + f] = v[f];
+                recs[LD * t + W - 1] = (uint64_t)__double_as_longlong(kr);
+            }
+            if (u + 1 < RS_ITEMS) load(i + RS_THREADS);
             __syncthreads();
             const int64_t cnt = n - i0 < RS_THREADS ? n - i0 : RS_THREADS;
             uint64_t *dst = aos + (int64_t)W * i0;
-            // whole records (unused words included): partial writes cost more
             for (int w = t; w < cnt * W; w += RS_THREADS) dst[w] = recs[LD * (w / W) + w % W];
             __syncthreads();
+        }
+    } else {
+        for (int u = 0; u < RS_ITEMS; u++) {
+            const int64_t i0 = base + u * RS_THREADS;
+            const int64_t i = i0 + t;
+            if (i < n) key_word(keysrc[i + 1], i);
+            if (aos && i0 < n) {        // block-uniform: pack the chunk through LDS, write it contiguously
+                if (i < n) {
+                    for (int f = 0; f < g.nf; f++) recs[LD * t + f] = g.src[f][i + 1];
+                    recs[LD * t + g.nf] = (uint64_t)__double_as_longlong(keysrc[i + 1]);
+                }
+                __syncthreads();
+                const int64_t cnt = n - i0 < RS_THREADS ? n - i0 : RS_THREADS;
+                uint64_t *dst = aos + (int64_t)W * i0;
+                // whole records (unused words included): partial writes cost more
+                for (int w = t; w < cnt * W; w += RS_THREADS) dst[w] = recs[LD * (w / W) + w % W];
+                __syncthreads();
+            }
         }
     }
     __syncthreads();

@@ -227,7 +227,7 @@ struct timestep_rng_bases_kernel {
 // The same window sums with the block's range [first - 2p, last + 2p] staged in
 // LDS (every window, including the shifted ones at both ends, lies inside it), in
 // the same order and expression as sigma_kernel.
-constexpr int SIG_T = 256, SIG_MAXP = 64, SIG_LDS = SIG_T + 4 * SIG_MAXP + 1;   // 8 KB: eight blocks per CU
+constexpr int SIG_T = 512, SIG_MAXP = 64, SIG_LDS = SIG_T + 4 * SIG_MAXP + 1;   // 12 KB
 
 struct sigma_lds_kernel {
     static constexpr int LB = SIG_T, MB = 0;

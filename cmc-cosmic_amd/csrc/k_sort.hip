@@ -242,10 +242,15 @@ struct rs_hist_kernel {
     __syncthreads();
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)kg_.bid * RS_TILE;
+    uint64_t k[RS_ITEMS];                   // every load of the tile in flight before the counting
+#pragma unroll
     for (int u = 0; u < RS_ITEMS; u++) {
-        int64_t i = base + u * RS_THREADS + t;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+        const int64_t i = base + u * RS_THREADS + t;
+        k[u] = i < n ? keys[i] : 0ull;
     }
+#pragma unroll
+    for (int u = 0; u < RS_ITEMS; u++)
+        if (base + u * RS_THREADS + t < n) atomicAdd(&h[(k[u] >> shift) & 255], 1u);
     __syncthreads();
     block_hist[(int64_t)t * ntiles + kg_.bid] = h[t];
 }

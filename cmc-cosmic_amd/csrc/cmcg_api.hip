@@ -437,6 +437,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.max_blocks = (n_cap + 127) / 128 + 2;      // the smallest block of any launch shape: 128 threads
     if (sc.max_blocks < 1024) sc.max_blocks = 1024;   // the stripping passes' fixed grid
     sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
+    sc.merge_ends_cap = 2 * (n_cap / 256 + 4);       // two per merge block of set_velocities3 (>= 256 stars)
     sc.sort_aos_cap = n_cap + 4;
     // about one radial-index bucket per star: 2^10 .. 2^22 buckets
     sc.ridx_bits = 10;
@@ -447,7 +448,8 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)(sc.sort_long_cap = n_cap / (SORT_RUN_SHORT + 1) + 2))) ||
         (e = dalloc(c, &sc.sort_huge, (size_t)(sc.sort_huge_cap = n_cap / (SORT_RUN_LONG + 1) + 2))) ||
         (e = dalloc(c, &sc.sort_nlong, 2)) ||
-        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.sv3_dpre, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.sv3_dpost, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.dt_part, (size_t)64 * 16)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
+        (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) ||
+        (e = dalloc(c, &sc.merge_ends, (size_t)sc.merge_ends_cap)) || (e = dalloc(c, &sc.sv3_dpre, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.sv3_dpost, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.dt_part, (size_t)64 * 16)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
         (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) ||
         (e = dalloc(c, (char **)&sc.relax_nib, sizeof(NibbleTables<8>))) ||
         (e = dalloc(c, (char **)&sc.wb_nib, 2 * sizeof(NibbleTables<RNG_WB_POWERS>))) || (e = dalloc(c, &sc.keys_b, (size_t)n_cap + 4)) ||

@@ -303,15 +303,35 @@ struct ie_transfer_kernel {
 // the values equal potential_at(x) bit for bit.
 constexpr int MERGE_T = 256, MERGE_PER = 2, MERGE_BLK = MERGE_T * MERGE_PER, MERGE_LDS = 1024;
 
+// The node index of both ends of every merge block (slot 2b: the block's first old
+// radius, 2b + 1: its last), one search per thread, so the merge blocks start from
+// their brackets instead of each waiting on two full-range searches.
+struct merge_ends_kernel {
+    static constexpr int LB = 256, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r_old,
+                                               const double2 *__restrict__ nodes, const DevScalars *s,
+                                               int64_t *__restrict__ ends)
+    {
+        const int64_t q = (int64_t)kg_.bid * 256 + threadIdx.x;       // 2 b + (0: first, 1: last)
+        const int64_t n_old = s->sort_in, n = s->n_max;
+        const int64_t first = 1 + (q >> 1) * MERGE_BLK;
+        if (first > n_old) return;
+        const int64_t last = first + MERGE_BLK - 1 <= n_old ? first + MERGE_BLK - 1 : n_old;
+        const PackedNodes nd{nodes};
+        const double x = r_old[(q & 1) ? last : first];
+        // largest k in [1, n+1] with r_k < x (1 if none); monotone in x
+        ends[q] = (x == 0.0 || x < nd.R(1)) ? 1 : find_zero_r_n(nd, 1, n + 1, x);
+    }
+};
+
 struct old_node_potential_kernel {
     static constexpr int LB = MERGE_T, MB = 0;
     static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ r_old,
                                                                      const double2 *__restrict__ nodes,
-                                                                     DevScalars *s, double2 *__restrict__ u_old)
+                                                                     DevScalars *s, double2 *__restrict__ u_old,
+                                                                     const int64_t *__restrict__ ends)
 {
     __shared__ double2 L[MERGE_LDS];
-    __shared__ int64_t ends[2];
-    __shared__ double xends[2];
     const int64_t n_old = s->sort_in, n = s->n_max;
     const int64_t first = 1 + (int64_t)kg_.bid * MERGE_BLK;
     if (first > n_old) return;
@@ -319,15 +339,8 @@ struct old_node_potential_kernel {
     const PackedNodes nd{nodes};
     const double r1 = nd.R(1);
     const int t = threadIdx.x;
-    if (t < 2) {
-        const double x = r_old[t == 0 ? first : last];
-        // largest k in [1, n+1] with r_k < x (1 if none); monotone in x
-        ends[t] = (x == 0.0 || x < r1) ? 1 : find_zero_r_n(nd, 1, n + 1, x);
-        xends[t] = x;
-    }
-    __syncthreads();
-    const int64_t lo = ends[0], hi = ends[1];
-    const double xf = xends[0], xl = xends[1];
+    const int64_t lo = ends[2 * (int64_t)kg_.bid], hi = ends[2 * (int64_t)kg_.bid + 1];
+    const double xf = r_old[first], xl = r_old[last];
     const int64_t span = hi + 2 - lo;             // nodes lo .. hi+1
     const bool staged = span <= MERGE_LDS && hi + 1 <= n + 1;
     if (staged)
@@ -734,8 +747,12 @@ hipError_t launch_set_velocities3(const DevFields &f, DevScalars *s, const KPara
     if (!dpost)
         if (hipError_t e = launch_mid_build(f, s, sc, n_ub, st)) return e;
     if (r_old_sorted) {
-        kl<old_node_potential_kernel>(dim3((unsigned)((n_ub + MERGE_BLK - 1) / MERGE_BLK)),
-                           dim3(MERGE_T), 0, st, r_old_sorted, (const double2 *)sc.nodes, s, sc.u_old);
+        const int64_t mb = (n_ub + MERGE_BLK - 1) / MERGE_BLK;
+        if (2 * mb > sc.merge_ends_cap) return hipErrorInvalidValue;
+        kl<merge_ends_kernel>(dim3((unsigned)((2 * mb + 255) / 256)), dim3(256), 0, st, r_old_sorted,
+                              (const double2 *)sc.nodes, (const DevScalars *)s, sc.merge_ends);
+        kl<old_node_potential_kernel>(dim3((unsigned)mb), dim3(MERGE_T), 0, st, r_old_sorted,
+                                      (const double2 *)sc.nodes, s, sc.u_old, (const int64_t *)sc.merge_ends);
     }
     kl<sv3_kernel>(dim3((unsigned)nb), dim3(GT), 0, st, f, s, P, sc.excess_c, sc.sv3_blk,
                        (const double2 *)sc.mid, (const double2 *)sc.nodes,

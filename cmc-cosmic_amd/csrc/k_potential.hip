@@ -202,12 +202,14 @@ __device__ __forceinline__ void load_tile(const double *__restrict__ x, double s
                                           double *tbuf, double (&sx)[CI])
 {
     const int t = threadIdx.x;
+    double v[CI];                           // every load in flight before the LDS writes
 #pragma unroll
     for (int u = 0; u < CI; u++) {
-        const int i = u * CT + t;
-        const int64_t j = base + i;
-        tbuf[tile_slot(i)] = j < L ? sign * x[j] : 0.0;
+        const int64_t j = base + u * CT + t;
+        v[u] = j < L ? x[j] : 0.0;
     }
+#pragma unroll
+    for (int u = 0; u < CI; u++) tbuf[tile_slot(u * CT + t)] = base + u * CT + t < L ? sign * v[u] : 0.0;
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < CI; u++) sx[u] = tbuf[tile_slot(t * CI + u)];
@@ -366,11 +368,16 @@ struct esa_sum_kernel {
     __shared__ double lds[CT / 64];
     const int64_t L = *a.Lp;
     const int64_t base = (int64_t)kg_.bid * CTILE;
-    double acc = 0.0;
+    double v[CI];
+#pragma unroll
     for (int u = 0; u < CI; u++) {
         const int64_t j = base + u * CT + threadIdx.x;
-        if (j < L) acc += a.sign * a.x[j];
+        v[u] = j < L ? a.x[j] : 0.0;
     }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < CI; u++)
+        if (base + u * CT + threadIdx.x < L) acc += a.sign * v[u];
     esa_tile_sums(a, L, acc, lds, kg_.bid);
 }
 };
@@ -753,11 +760,18 @@ struct prep_b_kernel {
     const int64_t n = s->n_max;
     const int64_t L = n > 1 ? n - 1 : 0;
     const int64_t base = (int64_t)kg_.bid * CTILE;
+    double mv[CI];
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+        const int64_t j = base + u * CT + threadIdx.x;
+        mv[u] = j < L ? m[n - j] : 0.0;
+    }
     double acc = 0.0;
+#pragma unroll
     for (int u = 0; u < CI; u++) {
         const int64_t j = base + u * CT + threadIdx.x;
         if (j < L) {
-            const double v = m[n - j] / nstar;
+            const double v = mv[u] / nstar;
             x[j] = v;
             acc += e.sign * v;
         }
@@ -776,13 +790,25 @@ struct prep_c_kernel {
     __shared__ double lds[CT / 64];
     const int64_t n = s->n_max;
     const int64_t base = (int64_t)kg_.bid * CTILE;
+    double Mv[CI], rk[CI], rk1[CI];
+    const double Mtot = s->Mtotal;
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+        const int64_t j = base + u * CT + threadIdx.x;
+        const int64_t k = n - j;
+        Mv[u] = rk[u] = rk1[u] = 1.0;
+        if (j < n) {
+            Mv[u] = k == n ? Mtot : Mk[n - 1 - k];
+            rk[u] = r[k];
+            rk1[u] = r[k + 1];
+        }
+    }
     double acc = 0.0;
+#pragma unroll
     for (int u = 0; u < CI; u++) {
         const int64_t j = base + u * CT + threadIdx.x;
         if (j < n) {
-            const int64_t k = n - j;
-            const double M = k == n ? s->Mtotal : Mk[n - 1 - k];
-            const double v = M * (1.0 / r[k] - 1.0 / r[k + 1]);
+            const double v = Mv[u] * (1.0 / rk[u] - 1.0 / rk1[u]);
             x[j] = v;
             acc += e.sign * v;
         }

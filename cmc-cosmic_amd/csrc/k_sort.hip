@@ -313,7 +313,7 @@ struct rs_scatter_kernel {
     constexpr int NW = RS_THREADS / 64;
     __shared__ uint64_t swords[RS_TILE];
     __shared__ uint32_t wcnt[NW][256];
-    __shared__ uint32_t dstart[256];       // 37 KB in all: four blocks per CU
+    __shared__ uint32_t dstart[256];       // 38 KB in all: four blocks per CU
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t n = s->n_max_new;
     const int64_t base = (int64_t)kg_.bid * RS_TILE;
@@ -322,10 +322,19 @@ struct rs_scatter_kernel {
     uint64_t k[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
     const uint64_t lt_mask = (1ull << lane) - 1ull;
+    // every load of the wave's sub-tile in flight before the ranking
+#pragma unroll
+    for (int u = 0; u < RS_ITEMS; u++) {
+        const int64_t i = wbase + u * 64 + lane;
+        k[u] = i < n ? words_in[i] : 0ull;
+    }
+    // this tile's output base of every digit
+    __shared__ uint32_t dout[256];
+    dout[t] = tile_off[(int64_t)t * ntiles + kg_.bid];
+#pragma unroll
     for (int u = 0; u < RS_ITEMS; u++) {
         int64_t i = wbase + u * 64 + lane;
         bool valid = i < n;
-        k[u] = valid ? words_in[i] : 0ull;
         uint32_t d = (uint32_t)((k[u] >> shift) & 255);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -374,7 +383,7 @@ struct rs_scatter_kernel {
         if (p < cnt_tile) {
             uint64_t word = swords[p];
             uint32_t d = (uint32_t)((word >> shift) & 255);
-            int64_t g = (int64_t)tile_off[(int64_t)d * ntiles + kg_.bid] + (p - (int64_t)dstart[d]);
+            int64_t g = (int64_t)dout[d] + (p - (int64_t)dstart[d]);
             words_out[g] = word;
         }
     }

@@ -62,6 +62,8 @@ struct Scratch {
     uint32_t *ridx_tb;          // its value at every RIDX_TILE-th bucket
     int ridx_bits;              // log2 of the bucket count
     double2 *u_old;             // (new potential, radius) at each old sorted radius (set_velocities3)
+    int64_t *merge_ends;        // set_velocities3: node index of both ends of each merge block
+    int64_t merge_ends_cap;
     double *sv3_dpre, *sv3_dpost; // vold^2 + 2(1-q)(Uoldrold - Uoldrnew) per star, before and after the step's sort
     unsigned long long *dt_part;  // the step's time step: 64 slots, 128 bytes apart, of complemented dt minima
     double2 *coarse;            // orbit sampler's (r, phi) samples, copied into LDS by every block

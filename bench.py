@@ -161,7 +161,7 @@ def orbit_ceiling(cm, ctx, margin, steps=3):
     return out
 
 
-def ensemble_bench(cm, steps, seed=7, mlp=None, concurrent=True):
+def ensemble_bench(cm, steps, seed=7, mlp=None, concurrent=True, sequential=True):
     """64 independent clusters stepped together: N log-uniform on [1e4, 2e6] (seeded),
     32 Kroupa Plummer and 32 Kroupa King models with W0 = 3, 5, 7, 9, one context each,
     each with its orbit sampler's launch shape autotuned.  Times the merged ensemble step (one segmented launch per kernel over all 64
@@ -235,14 +235,17 @@ def ensemble_bench(cm, steps, seed=7, mlp=None, concurrent=True):
             c.set_learned_brackets(False)
         learned = {"value": l_stars / t_l, "ms_per_step": t_l / max(1, steps // 2) * 1e3,
                    "brackets_accepted": hit, "brackets_rejected": miss}
-    seq_stars, t_seq = 0, 0.0
-    for _ in range(max(1, steps // 2)):
-        nl = live()
-        t = time.perf_counter()
-        for c in ctxs:
-            c.step()
-        t_seq += time.perf_counter() - t
-        seq_stars += nl
+    seq = None
+    if sequential:
+        seq_stars, t_seq = 0, 0.0
+        for _ in range(max(1, steps // 2)):
+            nl = live()
+            t = time.perf_counter()
+            for c in ctxs:
+                c.step()
+            t_seq += time.perf_counter() - t
+            seq_stars += nl
+        seq = {"value": seq_stars / t_seq, "ms_per_step": t_seq / max(1, steps // 2) * 1e3}
     for c in ctxs:
         c.close()
     out = {"clusters": 64, "stars": int(ns.sum()), "n_min": int(ns.min()), "n_max": int(ns.max()),
@@ -251,12 +254,12 @@ def ensemble_bench(cm, steps, seed=7, mlp=None, concurrent=True):
            "orbit_shapes": sorted(set(shapes)), "relax_block_sizes": sorted(set(rshapes)),
            "concurrent_graphs": con,
            "learned": learned,
-           "sequential_eager": {"value": seq_stars / t_seq, "ms_per_step": t_seq / max(1, steps // 2) * 1e3},
+           "sequential_eager": seq,
            "note": "R_vir sets physical units only and does not enter the Henon-unit step"}
     log(f"ensemble: {out['value']:.3e} stars/s ({out['ms_per_step']:.2f} ms per step of all 64, "
         f"{merged[0]} merged launches of {merged[1]})"
         + (f", concurrent graphs {con['value']:.3e} stars/s" if con else "")
-        + f", one by one {out['sequential_eager']['value']:.3e} stars/s"
+        + (f", one by one {seq['value']:.3e} stars/s" if seq else "")
         + (f", learned brackets {learned['value']:.3e} stars/s ({learned['brackets_accepted']} accepted, "
            f"{learned['brackets_rejected']} rejected)" if learned else ""))
     return out
@@ -276,6 +279,8 @@ def main():
     ap.add_argument("--no-cold", action="store_true", help="skip the cold-cache phase timings")
     ap.add_argument("--no-concurrent", action="store_true",
                     help="skip the ensemble's concurrent-graphs comparison leg")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the merged ensemble leg (no comparison legs), its JSON on stdout")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported "
@@ -283,6 +288,9 @@ def main():
         sys.exit(2)
 
     cm = load_package()
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(cm, 10, concurrent=False, sequential=False)}), flush=True)
+        return
     t0 = time.perf_counter()
     cl = cm.make_king(args.n, 7.0, seed=1, kroupa=True, segregate=True)
     log(f"generated King W0=7 N={args.n} in {time.perf_counter() - t0:.1f} s, rtid={cl.rtid:.3f}")

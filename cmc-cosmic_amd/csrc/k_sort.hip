@@ -645,23 +645,43 @@ template <int W>
 struct rs_gather_aos_kernel {
     static constexpr int LB = 256, MB = 0;
     static __device__ __forceinline__ void run(KGrid kg_, const uint64_t *__restrict__ words, DevScalars *s,
-                                                            double *__restrict__ r_out, uint32_t *__restrict__ perm,
-                                                            const uint64_t *__restrict__ aos, GatherList g)
+                                               double *__restrict__ r_out, uint32_t *__restrict__ perm,
+                                               const uint64_t *__restrict__ aos, GatherList g)
 {
-    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
+    // The block's 256 records are read cooperatively, W/2 lanes per record (one
+    // 16-byte piece each, so each load instruction covers whole records), through LDS;
+    // then each thread writes its star's fields, coalesced across the block.
+    constexpr int PC = W / 2;                          // 16-byte pieces per record
+    __shared__ ulonglong2 recs[256 * PC + 256 * PC / 32];   // one pad slot per 32 pieces
+    __shared__ uint32_t ssrc[256];
+    const int t = threadIdx.x;
+    const int64_t i = (int64_t)kg_.bid * 256 + t;
     const int64_t n = s->sort_n;
-    if (i > n) return;
-    if (i == n) {
+    uint32_t src = 0;
+    if (i < n) {
+        src = word_slot(words[i]);
+        perm[i] = src;
+    } else if (i == n) {
         r_out[n + 1] = CMCG_SF_INFINITY;
-        return;
     }
-    const uint32_t src = word_slot(words[i]);
-    perm[i] = src;
-    const ulonglong2 *rec = (const ulonglong2 *)(aos + (int64_t)W * (src - 1));
+    ssrc[t] = src;
+    __syncthreads();
+    const int64_t live = n - (int64_t)kg_.bid * 256;   // stars of this block (may be <= 0)
+#pragma unroll
+    for (int u = 0; u < PC; u++) {
+        const int task = u * 256 + t, sl = task / PC, pc = task % PC;
+        if (sl < live) {
+            const ulonglong2 *rec = (const ulonglong2 *)(aos + (int64_t)W * (ssrc[sl] - 1));
+            recs[task + task / 32] = 2 * pc <= g.nf ? rec[pc] : ulonglong2{0ull, 0ull};    // fields, then r in word nf
+        }
+    }
+    __syncthreads();
+    if (i >= n) return;
     uint64_t w[W];
 #pragma unroll
-    for (int q = 0; q < W / 2; q++) {
-        ulonglong2 v = 2 * q <= g.nf ? rec[q] : ulonglong2{0ull, 0ull};    // fields, then r in word nf
+    for (int q = 0; q < PC; q++) {
+        const int task = t * PC + q;
+        const ulonglong2 v = recs[task + task / 32];
         w[2 * q] = v.x;
         w[2 * q + 1] = v.y;
     }

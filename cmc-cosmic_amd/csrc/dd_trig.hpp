@@ -2,6 +2,9 @@
 // evaluation (about 100 correct bits), so the result is the correctly rounded value
 // except for arguments whose exact sine lies within ~2^-100 relative of a rounding
 // midpoint.  The result therefore does not depend on the device math library.
+// sincos() first tries a shorter evaluation (sincos_fast, error below 2^-71 of the
+// result) and keeps its rounding when that bound decides it, about 99.99% of calls;
+// otherwise it takes the double-double one (sincos_full).  Both give the same bits.
 //
 // Method: Cody-Waite reduction x = k pi/2 + r with pi/2 split into five 33-bit
 // pieces (k * piece is exact for |k| < 2^20, so the reduction is exact up to the
@@ -19,6 +22,7 @@
 // [0, 2 pi]) fall back to the library functions.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "dd_trig_fine.hpp"
 
 namespace ddtrig {
 
@@ -135,7 +139,8 @@ __device__ __constant__ const double kTab[33][4] = {
     {0x1.6a09e667f3bcdp-1, -0x1.bdd3413b26456p-55, 0x1.6a09e667f3bcdp-1, -0x1.bdd3413b26456p-55},
 };
 
-__device__ __forceinline__ void sincos(double x, double *s_out, double *c_out)
+// The double-double evaluation above, rounded once at the end.
+__device__ __forceinline__ void sincos_full(double x, double *s_out, double *c_out)
 {
     if (!(__builtin_fabs(x) < 1.0e5)) {
         *s_out = ::sin(x);
@@ -183,6 +188,105 @@ __device__ __forceinline__ void sincos(double x, double *s_out, double *c_out)
     }
     *s_out = s;
     *c_out = c;
+}
+
+// True when hi is the correctly rounded value of any number within `rel` |hi| of
+// hi + lo (hi + lo normalised, |lo| <= ulp(hi) / 2): hi + lo then lies farther than
+// that from the rounding midpoints next to hi.  Below a power of two (lo pointing
+// toward zero) the midpoint is half as far.
+__device__ __forceinline__ bool rounds_surely(double hi, double lo, double rel)
+{
+    if (hi == 0.0) return lo == 0.0;
+    int e;
+    (void)__builtin_frexp(hi, &e);                 // |hi| in [2^(e-1), 2^e)
+    const bool pow2 = (__double_as_longlong(hi) & 0x000fffffffffffffll) == 0;
+    const bool down = lo != 0.0 && ((lo < 0.0) != (hi < 0.0));
+    const double half_ulp = __builtin_ldexp(1.0, (pow2 && down) ? e - 55 : e - 54);
+    return __builtin_fabs(lo) + rel * __builtin_fabs(hi) < half_ulp;
+}
+
+// The same values by a shorter evaluation, accepted only where its error bound decides
+// the rounding.  The reduction to r is the exact one above; then r = j pi/1024 + t,
+// |j| <= 256, |t| <= pi/2048, with pi/1024 = G1 + G2 + G3 (j times the two 44-bit
+// pieces is exact) and sin, cos of j pi/1024 in double-double (dd_trig_fine.hpp).
+// With z = t_hi^2 rounded and t^2 = z + zl (zl from the fma residual and 2 t_hi t_lo):
+//   sin t = t + sl,          sl = t_lo + t_hi z (-1/3! + z (1/5! - z/7!)) - t_hi zl/3!
+//   cos t = 1 - z/2 + cq,    cq = z^2 (1/4! - z/6!) - zl/2
+// (the omitted terms are below 2^-89).  In
+//   sin r = Sj + Cj t_hi - Sj z/2 + [Sj cq + Cj sl + Cj_lo t_hi + Sj_lo (1 - z/2)]
+//   cos r = Cj - Sj t_hi - Cj z/2 + [Cj cq - Sj sl - Sj_lo t_hi + Cj_lo (1 - z/2)]
+// the three leading terms are summed exactly (two_sum, fma residuals), so only the
+// bracket, below 6.1e-10 + 2^-53 |Sj| in magnitude, is rounded: its few roundings stay
+// below 2^-79 absolute, at most 2^-71 of the result (at worst for sin with j = 1), and
+// sl's own error below 2^-73 |t|.  Returns (hi, lo) of sin r and cos r and k.
+__device__ __forceinline__ void sincos_short_dd(double x, dd *sr_out, dd *cr_out, double *k_out)
+{
+    const double two_over_pi = 0x1.45f306dc9c883p-1;
+    const double P1 = 0x1.921fb544p+0, P2 = 0x1.0b4611a6p-34, P3 = 0x1.3198a2ep-69, P4 = 0x1.b839a252p-104,
+                 P5 = 0x1.27044533p-142;
+    const double k = __builtin_rint(x * two_over_pi);
+    dd r = two_sum(x - k * P1, -(k * P2));
+    r = add_d(r, -(k * P3));
+    r = add_d(r, -(k * P4));
+    r = add_d(r, -(k * P5));
+    const double j = __builtin_rint(r.hi * 0x1.45f306dc9c883p+8);      // 1024/pi
+    dd t = two_sum(r.hi - j * kG1, -(j * kG2));
+    t = add_d(t, r.lo);
+    t = add_d(t, -(j * kG3));
+    const double z = t.hi * t.hi;
+    const double zl = __builtin_fma(t.hi, t.hi, -z) + 2.0 * t.hi * t.lo;
+    const double ps = z * (-kInvFact[3][0] + z * (kInvFact[5][0] - z * kInvFact[7][0]));
+    const double sl = t.lo + t.hi * (ps - kInvFact[3][1] * z) - t.hi * zl * kInvFact[3][0];
+    const double cq = z * z * (kInvFact[4][0] - z * kInvFact[6][0]) - 0.5 * zl;
+    const double ch = -0.5 * z;                                      // exact
+    const int aj = (int)__builtin_fabs(j);
+    double sjh = kTabFine[aj][0], sjl = kTabFine[aj][1];
+    const double cjh = kTabFine[aj][2], cjl = kTabFine[aj][3];
+    if (j < 0) { sjh = -sjh; sjl = -sjl; }
+    // sin r
+    const double a1 = cjh * t.hi, e1 = __builtin_fma(cjh, t.hi, -a1);
+    const double p1 = sjh * ch, f1 = __builtin_fma(sjh, ch, -p1);
+    dd u = two_sum(sjh, a1);
+    dd v = two_sum(u.hi, p1);
+    double lo = u.lo + v.lo + (e1 + f1 + (sjh * cq + cjh * sl + cjl * t.hi + sjl * (1.0 + ch)));
+    *sr_out = fast_two_sum(v.hi, lo);
+    // cos r
+    const double a2 = sjh * t.hi, e2 = __builtin_fma(sjh, t.hi, -a2);
+    const double p2 = cjh * ch, f2 = __builtin_fma(cjh, ch, -p2);
+    u = two_sum(cjh, -a2);
+    v = two_sum(u.hi, p2);
+    lo = u.lo + v.lo + (f2 - e2 + (cjh * cq - sjh * sl - sjl * t.hi + cjl * (1.0 + ch)));
+    *cr_out = fast_two_sum(v.hi, lo);
+    *k_out = k;
+}
+
+// sincos_short_dd rounded, when 2^-68 of the result (8 times its error bound) cannot
+// move the rounding; returns false otherwise.
+__device__ __forceinline__ bool sincos_fast(double x, double *s_out, double *c_out)
+{
+    if (!(__builtin_fabs(x) < 1.0e5)) return false;
+    dd sr, cr;
+    double k;
+    sincos_short_dd(x, &sr, &cr, &k);
+    const double rel = 0x1.0p-68;
+    if (!rounds_surely(sr.hi, sr.lo, rel) || !rounds_surely(cr.hi, cr.lo, rel)) return false;
+    const double srd = sr.hi, crd = cr.hi;
+    int q = (int)((long long)k & 3);
+    double s, c;
+    switch (q) {
+    case 0: s = srd; c = crd; break;
+    case 1: s = crd; c = -srd; break;
+    case 2: s = -srd; c = -crd; break;
+    default: s = -crd; c = srd; break;
+    }
+    *s_out = s;
+    *c_out = c;
+    return true;
+}
+
+__device__ __forceinline__ void sincos(double x, double *s_out, double *c_out)
+{
+    if (!sincos_fast(x, s_out, c_out)) sincos_full(x, s_out, c_out);
 }
 
 } // namespace ddtrig

@@ -46,37 +46,30 @@ struct GatherList {
     uint64_t *dst[RS_MAX_FIELDS];
 };
 
-// zeroes the digit totals; its first thread also resets the sort's scalars
-struct rs_zero_kernel {
-    static constexpr int LB = 1024, MB = 0;
-    static __device__ __forceinline__ void run(KGrid kg_, uint32_t *__restrict__ a, int64_t n, DevScalars *s, int64_t *n_long)
-{
-    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        s->sort_n = 0;
-        s->sort_in = s->n_max_new;
-        n_long[0] = 0;
-        n_long[1] = 0;
-    }
-    if (i < n) a[i] = 0;
-}
-};
-
 __device__ __forceinline__ uint32_t word_slot(uint64_t w) { return (uint32_t)w; }
 __device__ __forceinline__ uint32_t word_z(uint64_t w) { return (uint32_t)(w >> RS_SHIFT0); }
 
 // Minimum and maximum order-preserving key of the live stars (r below SF_INFINITY), one
 // pair per block: mm[2b] = min, mm[2b + 1] = max (~0 and 0 for a block that saw none).
 // rs_prep_kernel combines the pairs; partials per block replace thousands of device-scope
-// atomics on one address arriving from every XCD.
+// atomics on one address arriving from every XCD.  The kernel also zeroes the digit
+// totals (zn words at z) and resets the sort's scalars and long-run counters.
 struct rs_minmax_kernel {
     static constexpr int LB = 256, MB = 0;
     static __device__ __forceinline__ void run(KGrid kg_, const double *__restrict__ keysrc, DevScalars *s,
-                                                        uint64_t kinf, unsigned long long *__restrict__ mm)
+                                                        uint64_t kinf, unsigned long long *__restrict__ mm,
+                                                        uint32_t *__restrict__ z, int64_t zn, int64_t *__restrict__ n_long)
 {
     __shared__ unsigned long long rmin[4], rmax[4];
     const int64_t n = s->n_max_new;
     const int64_t stride = (int64_t)kg_.nb * blockDim.x;
+    for (int64_t q = (int64_t)kg_.bid * blockDim.x + threadIdx.x; q < zn; q += stride) z[q] = 0u;
+    if (kg_.bid == 0 && threadIdx.x == 0) {
+        s->sort_n = 0;
+        s->sort_in = n;
+        n_long[0] = 0;
+        n_long[1] = 0;
+    }
     unsigned long long lo = ~0ull, hi = 0ull;
     auto take = [&](uint64_t k) {
         if (k < kinf) {
@@ -116,7 +109,8 @@ __host__ __device__ __forceinline__ int minmax_parts(int64_t ntiles)
     return ntiles * 4 < SORT_MM_PARTS ? (int)(ntiles * 4) : SORT_MM_PARTS;
 }
 
-// Words from slots 1..n plus the four global digit histograms; with `aos` the payload
+// Words from slots 1..n plus the four global digit histograms, and the first pass's
+// per-tile histogram (a block's words are that pass's tile); with `aos` the payload
 // fields of each slot, and r itself after them, are packed into one 128-byte record,
 // so the final gather reads one record per star instead of one cache line per field.
 template <int W>
@@ -126,7 +120,8 @@ struct rs_prep_kernel {
                                                              uint64_t *__restrict__ words,
                                                              uint32_t *__restrict__ digit_totals,
                                                              uint64_t *__restrict__ aos, GatherList g, uint64_t kinf,
-                                                             const unsigned long long *__restrict__ mm)
+                                                             const unsigned long long *__restrict__ mm,
+                                                             uint32_t *__restrict__ block_hist, int64_t ntiles)
 {
     constexpr int P0 = 8 - RS_PASSES;      // first byte sorted on
     __shared__ uint32_t h[RS_PASSES][256];
@@ -225,6 +220,7 @@ struct rs_prep_kernel {
         }
     }
     __syncthreads();
+    block_hist[(int64_t)t * ntiles + kg_.bid] = h[0][t];
     for (int p = P0; p < 8; p++)
         if (h[p - P0][t]) atomicAdd(&digit_totals[p * 256 + t], h[p - P0][t]);
 }
@@ -405,13 +401,24 @@ struct rs_runs_kernel {
                                const int64_t *__restrict__ id, DevScalars *s, int64_t *__restrict__ long_runs,
                                int64_t *__restrict__ n_long, int64_t long_cap)
 {
-    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->n_max_new;
+    if (i > n || n <= 0) return;       // n <= 0: sort_n stays 0
+    // the words at i - 1, i and i + 1, loaded together (indices clamped into [0, n),
+    // the clamped ones are not compared)
+    const uint64_t wp = words[i > 0 ? i - 1 : 0];
+    const uint64_t wi = words[i < n ? i : n - 1];
+    const uint64_t wn = words[i + 1 < n ? i + 1 : n - 1];
+    // the number of live stars: the position where the words reach the tail's z (runs
+    // reorder words of one z only, so the z sequence read here is final)
+    const bool below_prev = i == 0 || word_z(wp) != Z_TAIL;
+    const bool below_here = i < n && word_z(wi) != Z_TAIL;
+    if (below_prev && !below_here) s->sort_n = i;
     if (i + 1 >= n) return;
-    const uint32_t z = word_z(words[i]);
+    const uint32_t z = word_z(wi);
     if (z == Z_TAIL) return;
-    if (word_z(words[i + 1]) != z) return;
-    if (i > 0 && word_z(words[i - 1]) == z) return;
+    if (word_z(wn) != z) return;
+    if (i > 0 && word_z(wp) == z) return;
     int64_t e = i + 1;
     while (e + 1 < n && word_z(words[e + 1]) == z && e - i < RUN_SHORT) e++;
     if (e + 1 < n && word_z(words[e + 1]) == z) {
@@ -605,20 +612,6 @@ struct rs_huge_runs_kernel {
 }
 };
 
-// number of live stars = words below the tail
-struct rs_count_kernel {
-    static constexpr int LB = 1024, MB = 0;
-    static __device__ __forceinline__ void run(KGrid kg_, const uint64_t *__restrict__ words, DevScalars *s)
-{
-    int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
-    const int64_t n = s->n_max_new;
-    if (i > n) return;
-    bool below_prev = (i == 0) ? true : word_z(words[i - 1]) != Z_TAIL;
-    bool below_here = (i == n) ? false : word_z(words[i]) != Z_TAIL;
-    if (below_prev && !below_here) s->sort_n = i;
-}
-};
-
 // The sorted permutation goes to perm (slot of the star now at position i), r and the
 // payload fields to their destinations.
 struct rs_gather_kernel {
@@ -629,6 +622,7 @@ struct rs_gather_kernel {
 {
     int64_t i = (int64_t)kg_.bid * blockDim.x + threadIdx.x;
     const int64_t n = s->sort_n;
+    if (i == 0) { s->n_max = n; s->n_max_new = n; }     // no block reads them
     if (i > n) return;
     if (i == n) {   // the outer sentinel (potential_calculate sets it again)
         r_out[n + 1] = CMCG_SF_INFINITY;
@@ -657,6 +651,7 @@ struct rs_gather_aos_kernel {
     const int t = threadIdx.x;
     const int64_t i = (int64_t)kg_.bid * 256 + t;
     const int64_t n = s->sort_n;
+    if (i == 0) { s->n_max = n; s->n_max_new = n; }     // no block reads them
     uint32_t src = 0;
     if (i < n) {
         src = word_slot(words[i]);
@@ -693,15 +688,6 @@ struct rs_gather_aos_kernel {
 }
 };
 
-struct rs_finish_kernel {
-    static constexpr int LB = 1024, MB = 0;
-    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s)
-{
-    s->n_max = s->sort_n;
-    s->n_max_new = s->sort_n;
-}
-};
-
 } // namespace
 
 // Sort slots 1..n_max_new by (key field, id).  src/dst are parallel lists of payload
@@ -723,21 +709,20 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     if (ntiles > sc.max_sort_tiles) return hipErrorInvalidValue;
     const uint64_t kinf = 0x8000000000000000ull | 0x4202a05f20000000ull;   // sortable(1.0e10)
     unsigned long long *mm = (unsigned long long *)sc.sort_mm;
-    kl<rs_zero_kernel>(dim3(8), dim3(256), 0, st, sc.digit_totals, (int64_t)(8 * 256), s,
-                       sc.sort_nlong);
     kl<rs_minmax_kernel>(dim3((unsigned)minmax_parts(ntiles)), dim3(256), 0, st, key_src, s, kinf,
-                       mm);
+                       mm, sc.digit_totals, (int64_t)(8 * 256), sc.sort_nlong);
     // an even number of passes: the sorted words end in keys_a
     uint64_t *kin = sc.keys_a, *kout = sc.keys_b;
     if (narrow)
         kl<rs_prep_kernel<8>>(dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
-                           sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm);
+                           sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm, sc.block_hist, ntiles);
     else
         kl<rs_prep_kernel<AOS_MAX>>(dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, key_src, s, kin,
-                           sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm);
+                           sc.digit_totals, aos, g, kinf, (const unsigned long long *)mm, sc.block_hist, ntiles);
     for (int pass = 8 - RS_PASSES; pass < 8; pass++) {
         int shift = 8 * pass;
-        kl<rs_hist_kernel>(dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, s, shift,
+        if (pass > 8 - RS_PASSES)          // the first pass's tile counts come from rs_prep_kernel
+            kl<rs_hist_kernel>(dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, kin, s, shift,
                            sc.block_hist, ntiles);
         kl<rs_scan_kernel>(dim3(256), dim3(RS_THREADS), 0, st, sc.block_hist, ntiles,
                            (const uint32_t *)(sc.digit_totals + 256 * pass));
@@ -753,7 +738,6 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
                        sc.sort_huge_cap);
     kl<rs_huge_runs_kernel>(dim3(16), dim3(LR_T), 0, st, kin, kout, key_src, id_src, s,
                        (const int64_t *)sc.sort_huge, (const int64_t *)sc.sort_nlong, sc.sort_huge_cap);
-    kl<rs_count_kernel>(dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s);
     if (aos && narrow)
         kl<rs_gather_aos_kernel<8>>(dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
                            sc.vals_a, (const uint64_t *)aos, g);
@@ -763,6 +747,5 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     else
         kl<rs_gather_kernel>(dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, key_src, r_dst,
                            sc.vals_a, g);
-    kl<rs_finish_kernel>(dim3(1), dim3(1), 0, st, s);
     return hipGetLastError();
 }

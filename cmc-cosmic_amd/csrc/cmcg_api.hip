@@ -446,8 +446,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.ridx_tb, (size_t)ridx_tiles(sc.ridx_bits) + 1))) { cmcg_destroy(c); return e; }
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 6)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
         (e = dalloc(c, &sc.keys_a, (size_t)n_cap + 4)) || (e = dalloc(c, &sc.sort_aos, (size_t)(n_cap + 4) * SORT_AOS_WORDS)) || (e = dalloc(c, &sc.sort_long, (size_t)(sc.sort_long_cap = n_cap / (SORT_RUN_SHORT + 1) + 2))) ||
-        (e = dalloc(c, &sc.sort_huge, (size_t)(sc.sort_huge_cap = n_cap / (SORT_RUN_LONG + 1) + 2))) ||
-        (e = dalloc(c, &sc.sort_nlong, 2)) ||
+        (e = dalloc(c, &sc.sort_nlong, 1)) ||
         (e = dalloc(c, &sc.mid, (size_t)(n_cap + 2) / MID_STEP + 4)) || (e = dalloc(c, &sc.nodes, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.u_old, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.merge_ends, (size_t)sc.merge_ends_cap)) || (e = dalloc(c, &sc.sv3_dpre, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.sv3_dpost, (size_t)n_cap + 6)) || (e = dalloc(c, &sc.dt_part, (size_t)64 * 16)) || (e = dalloc(c, &sc.coarse, (size_t)ORB_MAX_COARSE + 8)) || (e = dalloc(c, &sc.red2, (size_t)RED2_BLOCKS * 16)) ||
         (e = dalloc(c, (char **)&sc.orb_nib, sizeof(NibbleTables<ORB_NIB_POWERS>))) ||

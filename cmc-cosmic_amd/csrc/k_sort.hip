@@ -23,7 +23,7 @@ constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_MAX_FIELDS = 24;
 // LSD passes over z, the top 32 bits of each word; words equal in z are then put in
 // (r, id) order by the run pass (one thread per run up to RUN_SHORT words, one block
-// sorting in LDS up to RUN_LONG words).
+// per longer run: sorting in LDS up to RUN_LONG words, a merge sort beyond).
 constexpr int RS_PASSES = 4;
 constexpr int RS_SHIFT0 = 64 - 8 * RS_PASSES;     // z = word >> RS_SHIFT0
 constexpr uint32_t Z_TAIL = 0xffffffffu;          // z of the SF_INFINITY tail; live z < 2^31
@@ -68,7 +68,6 @@ struct rs_minmax_kernel {
         s->sort_n = 0;
         s->sort_in = n;
         n_long[0] = 0;
-        n_long[1] = 0;
     }
     unsigned long long lo = ~0ull, hi = 0ull;
     auto take = [&](uint64_t k) {
@@ -490,14 +489,62 @@ __device__ void lds_sort_run(uint64_t *__restrict__ words, const double *__restr
     __syncthreads();
 }
 
+// Runs longer than RUN_LONG: chunks of RUN_LONG sorted in LDS, then bottom-up merges
+// between words and tmp (the same positions of the second key buffer), each merge
+// split among the threads along its merge path.
+__device__ __forceinline__ bool word_less(uint64_t wa, uint64_t wb, const double *__restrict__ keysrc,
+                                          const int64_t *__restrict__ id)
+{
+    const uint32_t va = word_slot(wa), vb = word_slot(wb);
+    return key_id_less(sortable(keysrc[va]), id[va], sortable(keysrc[vb]), id[vb]);
+}
+
+__device__ void merge_sort_run(uint64_t *__restrict__ words, uint64_t *__restrict__ tmp,
+                               const double *__restrict__ keysrc, const int64_t *__restrict__ id, int64_t i,
+                               int64_t len, uint32_t z, uint64_t *sk, int64_t *sid, uint32_t *sv)
+{
+    const int t = threadIdx.x;
+    for (int64_t c = 0; c < len; c += RUN_LONG)
+        lds_sort_run(words, keysrc, id, i + c, len - c < RUN_LONG ? len - c : RUN_LONG, z, sk, sid, sv);
+    uint64_t *src = words + i, *dst = tmp + i;
+    for (int64_t w = RUN_LONG; w < len; w <<= 1) {
+        for (int64_t p = 0; p < len; p += 2 * w) {
+            const int64_t la = len - p < w ? len - p : w;
+            const int64_t lb = len - p - la < w ? len - p - la : w;
+            const uint64_t *A = src + p, *B = src + p + la;
+            const int64_t L = la + lb;
+            const int64_t d0 = L * t / LR_T, d1 = L * (t + 1) / LR_T;
+            // merge path: the first d0 outputs hold A[0, ia) and B[0, d0 - ia); A
+            // goes first on equal keys.  A[m] is among them iff A[m] <= B[d0 - 1 - m].
+            int64_t a_lo = d0 > lb ? d0 - lb : 0, a_hi = d0 < la ? d0 : la;
+            while (a_lo < a_hi) {
+                const int64_t m = (a_lo + a_hi) >> 1;
+                if (!word_less(B[d0 - 1 - m], A[m], keysrc, id)) a_lo = m + 1;
+                else a_hi = m;
+            }
+            int64_t ia = a_lo, ib = d0 - a_lo;
+            for (int64_t d = d0; d < d1; d++) {
+                const bool takeA = ib >= lb || (ia < la && !word_less(B[ib], A[ia], keysrc, id));
+                dst[p + d] = takeA ? A[ia++] : B[ib++];
+            }
+        }
+        __syncthreads();
+        uint64_t *x = src; src = dst; dst = x;
+    }
+    if (src != words + i)
+        for (int64_t q = t; q < len; q += LR_T) words[i + q] = src[q];
+    __syncthreads();
+}
+
+// One block per listed run (grid-stride over the list): its length by bisection (words
+// are ordered by z), then the LDS bitonic sort, or the merge sort beyond RUN_LONG words.
 struct rs_long_runs_kernel {
     static constexpr int LB = LR_T, MB = 0;
-    static __device__ __forceinline__ void run(KGrid kg_, uint64_t *__restrict__ words,
+    static __device__ __forceinline__ void run(KGrid kg_, uint64_t *__restrict__ words, uint64_t *__restrict__ tmp,
                                                             const double *__restrict__ keysrc,
                                                             const int64_t *__restrict__ id, DevScalars *s,
                                                             const int64_t *__restrict__ long_runs,
-                                                            int64_t *__restrict__ n_long, int64_t long_cap,
-                                                            int64_t *__restrict__ huge_runs, int64_t huge_cap)
+                                                            const int64_t *__restrict__ n_long, int64_t long_cap)
 {
     __shared__ uint64_t sk[RUN_LONG];
     __shared__ int64_t sid[RUN_LONG];
@@ -510,63 +557,9 @@ struct rs_long_runs_kernel {
         const int64_t i = long_runs[r];
         const uint32_t z = word_z(words[i]);
         if (t == 0) {
-            // words are ordered by z: the last index with the run's z by bisection
-            int64_t lo = i, hi = i + RUN_LONG < n ? i + RUN_LONG : n - 1;
-            if (word_z(words[hi]) == z) {
-                len_s = hi == i + RUN_LONG ? -1 : hi - i + 1;     // more than RUN_LONG words
-            } else {
-                while (hi - lo > 1) {
-                    const int64_t m = lo + ((hi - lo) >> 1);
-                    if (word_z(words[m]) == z) lo = m;
-                    else hi = m;
-                }
-                len_s = lo - i + 1;
-            }
-            if (len_s < 0) {                                       // left to the merge sort
-                unsigned long long slot = atomicAdd((unsigned long long *)(n_long + 1), 1ull);
-                if ((int64_t)slot < huge_cap) huge_runs[slot] = i;
-                else dev_set_error(s, CMCG_E_STATE, i + 1);
-            }
-        }
-        __syncthreads();
-        const int64_t len = len_s;
-        __syncthreads();
-        if (len < 0) continue;
-        lds_sort_run(words, keysrc, id, i, len, z, sk, sid, sv);
-    }
-}
-};
-
-// Runs longer than RUN_LONG (one block per run): chunks of RUN_LONG sorted in LDS,
-// then bottom-up merges between words and tmp (the same positions of the second key
-// buffer), each merge split among the threads along its merge path.
-__device__ __forceinline__ bool word_less(uint64_t wa, uint64_t wb, const double *__restrict__ keysrc,
-                                          const int64_t *__restrict__ id)
-{
-    const uint32_t va = word_slot(wa), vb = word_slot(wb);
-    return key_id_less(sortable(keysrc[va]), id[va], sortable(keysrc[vb]), id[vb]);
-}
-
-struct rs_huge_runs_kernel {
-    static constexpr int LB = LR_T, MB = 0;
-    static __device__ __forceinline__ void run(KGrid kg_, uint64_t *__restrict__ words, uint64_t *__restrict__ tmp,
-                                                            const double *__restrict__ keysrc,
-                                                            const int64_t *__restrict__ id, DevScalars *s,
-                                                            const int64_t *__restrict__ huge_runs,
-                                                            const int64_t *__restrict__ n_long, int64_t huge_cap)
-{
-    __shared__ uint64_t sk[RUN_LONG];
-    __shared__ int64_t sid[RUN_LONG];
-    __shared__ uint32_t sv[RUN_LONG];
-    __shared__ int64_t len_s;
-    const int64_t nh = n_long[1] < huge_cap ? n_long[1] : huge_cap;
-    const int64_t n = s->n_max_new;
-    const int t = threadIdx.x;
-    for (int64_t r = kg_.bid; r < nh; r += kg_.nb) {
-        const int64_t i = huge_runs[r];
-        const uint32_t z = word_z(words[i]);
-        if (t == 0) {
-            int64_t lo = i, hi = n - 1;                // last index with the run's z
+            // the last index with the run's z: words[lo] has it, words[hi] not (or lo == hi)
+            int64_t lo = i, hi = i + RUN_LONG < n - 1 ? i + RUN_LONG : n - 1;
+            if (word_z(words[hi]) == z) { lo = hi; hi = n - 1; }     // more than RUN_LONG words
             if (word_z(words[hi]) == z) lo = hi;
             while (hi - lo > 1) {
                 const int64_t m = lo + ((hi - lo) >> 1);
@@ -578,36 +571,8 @@ struct rs_huge_runs_kernel {
         __syncthreads();
         const int64_t len = len_s;
         __syncthreads();
-        for (int64_t c = 0; c < len; c += RUN_LONG)
-            lds_sort_run(words, keysrc, id, i + c, len - c < RUN_LONG ? len - c : RUN_LONG, z, sk, sid, sv);
-        uint64_t *src = words + i, *dst = tmp + i;
-        for (int64_t w = RUN_LONG; w < len; w <<= 1) {
-            for (int64_t p = 0; p < len; p += 2 * w) {
-                const int64_t la = len - p < w ? len - p : w;
-                const int64_t lb = len - p - la < w ? len - p - la : w;
-                const uint64_t *A = src + p, *B = src + p + la;
-                const int64_t L = la + lb;
-                const int64_t d0 = L * t / LR_T, d1 = L * (t + 1) / LR_T;
-                // merge path: the first d0 outputs hold A[0, ia) and B[0, d0 - ia); A
-                // goes first on equal keys.  A[m] is among them iff A[m] <= B[d0 - 1 - m].
-                int64_t a_lo = d0 > lb ? d0 - lb : 0, a_hi = d0 < la ? d0 : la;
-                while (a_lo < a_hi) {
-                    const int64_t m = (a_lo + a_hi) >> 1;
-                    if (!word_less(B[d0 - 1 - m], A[m], keysrc, id)) a_lo = m + 1;
-                    else a_hi = m;
-                }
-                int64_t ia = a_lo, ib = d0 - a_lo;
-                for (int64_t d = d0; d < d1; d++) {
-                    const bool takeA = ib >= lb || (ia < la && !word_less(B[ib], A[ia], keysrc, id));
-                    dst[p + d] = takeA ? A[ia++] : B[ib++];
-                }
-            }
-            __syncthreads();
-            uint64_t *x = src; src = dst; dst = x;
-        }
-        if (src != words + i)
-            for (int64_t q = t; q < len; q += LR_T) words[i + q] = src[q];
-        __syncthreads();
+        if (len <= RUN_LONG) lds_sort_run(words, keysrc, id, i, len, z, sk, sid, sv);
+        else merge_sort_run(words, tmp, keysrc, id, i, len, z, sk, sid, sv);
     }
 }
 };
@@ -733,11 +698,8 @@ hipError_t launch_sort_fields(const double *key_src, double *r_dst, const int64_
     unsigned g1 = (unsigned)((n_ub + 256) / 256);
     kl<rs_runs_kernel>(dim3(g1), dim3(256), 0, st, kin, key_src, id_src, s, sc.sort_long,
                        sc.sort_nlong, sc.sort_long_cap);
-    kl<rs_long_runs_kernel>(dim3(64), dim3(LR_T), 0, st, kin, key_src, id_src, s,
-                       (const int64_t *)sc.sort_long, sc.sort_nlong, sc.sort_long_cap, sc.sort_huge,
-                       sc.sort_huge_cap);
-    kl<rs_huge_runs_kernel>(dim3(16), dim3(LR_T), 0, st, kin, kout, key_src, id_src, s,
-                       (const int64_t *)sc.sort_huge, (const int64_t *)sc.sort_nlong, sc.sort_huge_cap);
+    kl<rs_long_runs_kernel>(dim3(64), dim3(LR_T), 0, st, kin, kout, key_src, id_src, s,
+                       (const int64_t *)sc.sort_long, (const int64_t *)sc.sort_nlong, sc.sort_long_cap);
     if (aos && narrow)
         kl<rs_gather_aos_kernel<8>>(dim3(g1), dim3(256), 0, st, (const uint64_t *)kin, s, r_dst,
                            sc.vals_a, (const uint64_t *)aos, g);

@@ -52,9 +52,8 @@ struct Scratch {
     int64_t max_sort_tiles;
     uint64_t *sort_aos;         // packed payload records, SORT_AOS_WORDS words per star
     int64_t *sort_long;         // starts of runs of equal key prefixes longer than SORT_RUN_SHORT
-    int64_t *sort_huge;         // starts of runs longer than SORT_RUN_LONG
-    int64_t *sort_nlong;        // [0] long runs, [1] huge runs
-    int64_t sort_long_cap, sort_huge_cap;
+    int64_t *sort_nlong;        // number of listed runs
+    int64_t sort_long_cap;
     double2 *mid;               // (r, phi) at every 16th node (search middle level)
     double2 *nodes;             // packed (r, phi) of every node, for the searches
     float2 *pred;               // learned apsides proposals (log r_peri, log r_apo) per star

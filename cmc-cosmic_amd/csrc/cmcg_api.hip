@@ -436,7 +436,7 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.max_waves = (n_cap + 2) / 64 + 2;
     sc.max_blocks = (n_cap + 127) / 128 + 2;      // the smallest block of any launch shape: 128 threads
     if (sc.max_blocks < 1024) sc.max_blocks = 1024;   // the stripping passes' fixed grid
-    sc.max_sort_tiles = (n_cap + 4095) / 4096 + 1;
+    sc.max_sort_tiles = (n_cap + SORT_TILE - 1) / SORT_TILE + 1;
     sc.merge_ends_cap = 2 * (n_cap / 256 + 4);       // two per merge block of set_velocities3 (>= 256 stars)
     sc.sort_aos_cap = n_cap + 4;
     // about one radial-index bucket per star: 2^10 .. 2^22 buckets

@@ -18,7 +18,7 @@
 
 namespace {
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = SORT_TILE / RS_THREADS;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_MAX_FIELDS = 24;
 // LSD passes over z, the top 32 bits of each word; words equal in z are then put in

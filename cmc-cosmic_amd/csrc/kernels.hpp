@@ -13,6 +13,7 @@ constexpr int ORB_MAX_COARSE = 2048;
 // sort payload records: 16 words (128 B, one cache line) per star
 constexpr int SORT_AOS_WORDS = 16;
 constexpr int SORT_RUN_SHORT = 64;   // runs of equal sort keys up to this length are ordered in place
+constexpr int SORT_TILE = 4096;      // words per tile of the radix passes (2048: 4% slower)
 constexpr int SORT_RUN_LONG = 4096;  // ... up to this length in LDS; longer ones by a merge sort
 constexpr int SORT_MM_PARTS = 1024;   // blocks of the sort's min/max pass, one (min, max) pair each
 constexpr int RED2_BLOCKS = 64;

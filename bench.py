@@ -273,7 +273,7 @@ def main():
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ranks", type=int, default=min(16, os.cpu_count() or 1),
-                    help="host threads for the multi-rank CPU baseline (the GPU box's CPU share is 16)")
+                    help="host threads for the multi-rank CPU baseline (default: 16, or fewer host cores)")
     ap.add_argument("--train-iters", type=int, default=6000)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-cold", action="store_true", help="skip the cold-cache phase timings")
@@ -283,7 +283,7 @@ def main():
                     help="only the merged ensemble leg (no comparison legs), its JSON on stdout")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported "
+        print(json.dumps({"error": "the hot path runs on one GPU; --gpus above 1 is not supported "
                                    "(sharding the path across GPUs is designed, not built)"}))
         sys.exit(2)
 

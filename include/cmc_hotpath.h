@@ -116,7 +116,7 @@ int    cmc_hotpath_dyn_line(char *buf, size_t len);
 /* The two header lines of that file (src/cmc/cmc_io.c:2346-2347). */
 const char *cmc_hotpath_dyn_header(void);
 cmc_orbit_rs_t calc_orbit_new(long index, double E, double J);                 /* src/cmc/cmc_orbit.c:413 */
-cmc_orbit_rs_t calc_pericenter_and_apocenter(long index, double E, double J);  /* north_star name of the same */
+cmc_orbit_rs_t calc_pericenter_and_apocenter(long index, double E, double J);  /* calc_orbit_new under its descriptive name */
 long   find_zero_Q(long j, long kmin, long kmax, long double E, long double J); /* src/cmc/cmc_nr.c:316 */
 double find_root_vr(long index, long k, double E, double J);                  /* src/cmc/cmc_nr.c:500 */
 /* One Henon step in the order of src/cmc/cmc.c:240-407 (configs' switch set). */

@@ -165,7 +165,7 @@ int cmcg_relax(cmcg_ctx *ctx, double dt, uint64_t step);
  *     src/cmc/cmc_evolution_thr.c:784-1023). */
 int cmcg_orbits(cmcg_ctx *ctx, uint64_t step, cmcg_orbit_stats *stats);
 
-/* a6: calc_orbit_new (src/cmc/cmc_orbit.c:413-540; the north_star's
+/* a6: calc_orbit_new (src/cmc/cmc_orbit.c:413-540; also known as
  *     calc_pericenter_and_apocenter) for slots first..first+count-1 with the given
  *     E (already including the self term, cmc_evolution_thr.c:806) and J.
  *     Outputs may be NULL. */

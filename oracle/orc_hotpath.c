@@ -291,7 +291,7 @@ double orc_find_root_vr(orc_state *s, int64_t index, int64_t k, double E, double
     return apsis;
 }
 
-/* cmc_orbit.c:413-540 calc_orbit_new (north_star name: calc_pericenter_and_apocenter) */
+/* cmc_orbit.c:413-540 calc_orbit_new (also known as calc_pericenter_and_apocenter) */
 orc_orbit_rs orc_calc_orbit_new(orc_state *s, int64_t index, double E, double J)
 {
     orc_orbit_rs o;
@@ -648,8 +648,8 @@ void orc_dynamics_apply(orc_state *s, double dt)
     }
 }
 
-/* cmc_sort.c:245-250 compare_type extended with the id tie-break this build
- * defines (libc qsort is unstable, so the reference leaves equal-r order open). */
+/* cmc_sort.c:245-250 compare_type extended with an id tie-break, which the device sort
+ * also applies (libc qsort is unstable, so the reference leaves equal-r order open). */
 static const orc_state *g_sort_state;
 static int cmp_r_id(const void *pa, const void *pb)
 {

@@ -37,6 +37,7 @@ constexpr int CTILE = CT * CI;
 constexpr int RAW_CAP = 2048;         // sensitive elements sorted in LDS by the walk
 constexpr int WALK_T = 1024;          // threads of the one-block kernels
 constexpr int WALK_CHUNK = 256;       // sensitive elements staged per round of the walk
+constexpr int WALK_REG = 16;          // tiles per walk thread held in registers (N <= 8.4e6 at CT*CI = 2048)
 
 // ---------------------------------------------------------------- segment and walk
 // The classification needs only the binade of each chain value, so it runs on a
@@ -526,14 +527,37 @@ struct esa_walk_kernel {
     const int64_t nt = (L + CTILE - 1) / CTILE;
     const int64_t chunk = (nt + CT - 1) / CT;
     const int64_t b0 = (int64_t)t * chunk;
-    Seg acc{0, 0};
-    for (int64_t b = b0; b < b0 + chunk && b < nt; b++) acc = seg_then(acc, Seg{a.ttail[b], a.tcnt[b]});
     Seg tot;
-    Seg run = block_excl_seg(acc, lds, &tot);
-    for (int64_t b = b0; b < b0 + chunk && b < nt; b++) {
-        a.toff[b] = run.cnt;
-        a.tcarry[b] = run.sum;
-        run = seg_then(run, Seg{a.ttail[b], a.tcnt[b]});
+    if (chunk <= WALK_REG) {
+        // the thread's tile entries in registers, every load issued before the first use
+        Seg v[WALK_REG];
+#pragma unroll
+        for (int u = 0; u < WALK_REG; u++) {
+            const int64_t b = b0 + u;
+            v[u] = u < chunk && b < nt ? Seg{a.ttail[b], a.tcnt[b]} : Seg{0, 0};
+        }
+        Seg acc{0, 0};
+#pragma unroll
+        for (int u = 0; u < WALK_REG; u++) acc = seg_then(acc, v[u]);
+        Seg run = block_excl_seg(acc, lds, &tot);
+#pragma unroll
+        for (int u = 0; u < WALK_REG; u++) {
+            const int64_t b = b0 + u;
+            if (u < chunk && b < nt) {
+                a.toff[b] = run.cnt;
+                a.tcarry[b] = run.sum;
+            }
+            run = seg_then(run, v[u]);
+        }
+    } else {
+        Seg acc{0, 0};
+        for (int64_t b = b0; b < b0 + chunk && b < nt; b++) acc = seg_then(acc, Seg{a.ttail[b], a.tcnt[b]});
+        Seg run = block_excl_seg(acc, lds, &tot);
+        for (int64_t b = b0; b < b0 + chunk && b < nt; b++) {
+            a.toff[b] = run.cnt;
+            a.tcarry[b] = run.sum;
+            run = seg_then(run, Seg{a.ttail[b], a.tcnt[b]});
+        }
     }
     const int64_t ns = tot.cnt;
     if (t == 0) *a.s_n = ns;

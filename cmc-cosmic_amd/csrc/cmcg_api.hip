@@ -425,6 +425,8 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     P.theta_se_max = params->theta_se_max;
     P.gamma = params->gamma;
     P.avekernel = params->avekernel;
+    P.defer_cap = 1;
+    while (P.defer_cap < n_cap + 2) P.defer_cap <<= 1;      // every star could be deferred
     int e;
     if ((e = alloc_fields(c, c->f, true)) || (e = alloc_fields(c, c->alt, false))) { cmcg_destroy(c); return e; }
     c->alt.phi = c->f.phi; c->alt.sigma = c->f.sigma; c->alt.pot_rnew = c->f.pot_rnew;
@@ -459,8 +461,8 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
         (e = dalloc(c, &sc.sv3_blk, (size_t)sc.max_blocks)) || (e = dalloc(c, &sc.sv3_fallback, 1)) ||
         (e = dalloc(c, &sc.strip_done, 1)) || (e = dalloc(c, &sc.chain_x, (size_t)n_cap + 8)) ||
         (e = dalloc(c, &sc.chain_g, (size_t)n_cap + 8)) || (e = dalloc(c, &sc.chain_mk, (size_t)n_cap + 8)) ||
-        (e = dalloc(c, &sc.defer_list, 65536)) || (e = dalloc(c, &sc.defer_kfirst, 65536)) ||
-        (e = dalloc(c, &sc.defer_partials, 65536 * 16)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
+        (e = dalloc(c, &sc.defer_list, (size_t)P.defer_cap)) || (e = dalloc(c, &sc.defer_kfirst, (size_t)P.defer_cap)) ||
+        (e = dalloc(c, &sc.orb_sums, 16)) || (e = dalloc(c, &sc.chain_scal, 4)) || (e = dalloc(c, &sc.chain_len, 4)) || (e = dalloc(c, &sc.chain_fbad, 8)) ||
         (e = dalloc(c, &sc.esa_idx, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_racc, (size_t)sc.esa_max)) ||
         (e = dalloc(c, &sc.esa_v, (size_t)sc.esa_max)) || (e = dalloc(c, &sc.esa_n, 4)) || (e = dalloc(c, &sc.esa_raw_idx, 2048)) || (e = dalloc(c, &sc.esa_raw_part, 3 * 2048)) || (e = dalloc(c, &sc.rng_base, 2)) || (e = dalloc(c, &sc.rng_wave, (size_t)sc.max_waves)) || (e = dalloc(c, &sc.rng_seeded, 1)) ||
         (e = dalloc(c, &c->T, 1))) {

@@ -73,6 +73,7 @@ struct KParams {
     int64_t avekernel;
     int32_t chain_k_drop;  // test hook: potential chains' error bound widened by 2^chain_k_drop
     int32_t pad0;
+    int64_t defer_cap;     // capacity of the deferred-star list: a power of two above the star capacity
 };
 
 __device__ __forceinline__ void dev_set_error(DevScalars *s, int code, int64_t idx)

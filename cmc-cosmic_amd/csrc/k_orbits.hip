@@ -20,7 +20,6 @@
 namespace {
 constexpr int ORB_THREADS = 256;
 constexpr int ORB_COARSE = 1024;     // LDS samples of the (r, phi) table per block
-constexpr int64_t MAX_DEFER = 65536;
 
 // Per-block partials of the orbit phase (fixed-order wave-shuffle tree, then the
 // four waves in order): [0] max_r (max), [1] Eescaped, [2] Jescaped,
@@ -407,7 +406,7 @@ struct orbits_kernel {
                 // linear-scan case: finished by the deferred path
                 flags = CMCG_SF_LINSCAN;
                 unsigned long long slot = atomicAdd((unsigned long long *)&s->n_defer, 1ull);
-                if ((int64_t)slot < MAX_DEFER) defer_list[slot] = j;
+                if ((int64_t)slot < P.defer_cap) defer_list[slot] = j;
                 else dev_set_error(s, CMCG_E_STATE, j);
             } else {
                 if (pred) {
@@ -458,15 +457,15 @@ struct orbits_kernel {
 // the reduction of their sums does not depend on the order in which they were
 // appended, and reset the first-positive-index slots.  A short list is sorted by one
 // thread; a longer one by a bitonic sort over the next power of two (the list's
-// capacity MAX_DEFER is one), padded with INT64_MAX, in global memory.
+// capacity is one, above the number of stars), padded with INT64_MAX, in global memory.
 constexpr int DEFER_T = 1024;
 struct defer_prepare_kernel {
     static constexpr int LB = DEFER_T, MB = 0;
     static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, int64_t *__restrict__ list,
-                                                                int64_t *__restrict__ kfirst)
+                                                                int64_t *__restrict__ kfirst, int64_t cap)
 {
     const int t = threadIdx.x;
-    const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
+    const int64_t nd = s->n_defer < cap ? s->n_defer : cap;
     if (nd <= 64) {
         if (t == 0)
             for (int64_t a = 1; a < nd; a++) {
@@ -504,7 +503,7 @@ struct defer_search_kernel {
                                                            const int64_t *__restrict__ list,
                                                            int64_t *__restrict__ kfirst)
 {
-    const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
+    const int64_t nd = s->n_defer < P.defer_cap ? s->n_defer : P.defer_cap;
     if (nd == 0) return;
     const int64_t n = s->n_max;
     const int64_t stride = (int64_t)kg_.nb * blockDim.x;
@@ -522,6 +521,9 @@ struct defer_search_kernel {
 }
 };
 
+// The deferred stars, ORB_THREADS at a time in list order, and the phase's sums: the
+// sampler blocks' records (the first level's RED2_BLOCKS records, in block order), then
+// each deferred star's in list (= star) order, as a single thread would add them.
 struct defer_tail_kernel {
     static constexpr int LB = ORB_THREADS, MB = 0;
     static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
@@ -529,27 +531,54 @@ struct defer_tail_kernel {
                                                                  const Rng *__restrict__ base, int jb1,
                                                                  const int64_t *__restrict__ list,
                                                                  const int64_t *__restrict__ kfirst,
-                                                                 double *__restrict__ dpartials)
+                                                                 const double *__restrict__ red2,
+                                                                 double *__restrict__ sums)
 {
-    const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
-    const int64_t n = s->n_max;
-    for (int64_t d = threadIdx.x; d < nd; d += blockDim.x) {
-        const int64_t j = list[d];
-        const double rj = f.r[j], mj = f.m[j];
-        const double Ej = f.E[j], J = f.J[j];
-        const double E = Ej + phi_s_own(rj, mj, P.inv_nstar, rj);
-        int64_t kt = kfirst[d];
-        if (kt >= n) kt = -1;                      // cmc_orbit.c:52-56: circular
-        double red[NRED];
-        for (int i = 0; i < NRED; i++) red[i] = 0.0;
-        int flags = CMCG_SF_LINSCAN, ntry = 0, hits = 0, misses = 0;
-        sample_star(SplitNodes{f.r, f.phi}, f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), n, j,
-                    rj, mj, E, Ej, J, kt, flags, ntry, hits, misses, red);
-        f.flags[j] = flags;
-        f.ntry[j] = ntry;
-        count_star(red, flags, ntry, hits, misses, false);
-        for (int i = 0; i < NRED; i++) dpartials[RED_STRIDE * d + i] = red[i];
+    __shared__ double part[ORB_THREADS][NRED];
+    __shared__ double acc[NRED];
+    const int t = threadIdx.x;
+    if (t < NRED) {
+        double a = red2[t];
+        for (int b = 1; b < RED2_BLOCKS; b++) {
+            const double p = red2[RED_STRIDE * b + t];
+            a = red_is_max(t) ? (a >= p ? a : p) : a + p;
+        }
+        acc[t] = a;
     }
+    const int64_t nd = s->n_defer < P.defer_cap ? s->n_defer : P.defer_cap;
+    const int64_t n = s->n_max;
+    for (int64_t c0 = 0; c0 < nd; c0 += ORB_THREADS) {          // block-uniform
+        const int64_t d = c0 + t;
+        if (d < nd) {
+            const int64_t j = list[d];
+            const double rj = f.r[j], mj = f.m[j];
+            const double Ej = f.E[j], J = f.J[j];
+            const double E = Ej + phi_s_own(rj, mj, P.inv_nstar, rj);
+            int64_t kt = kfirst[d];
+            if (kt >= n) kt = -1;                      // cmc_orbit.c:52-56: circular
+            double red[NRED];
+            for (int i = 0; i < NRED; i++) red[i] = 0.0;
+            int flags = CMCG_SF_LINSCAN, ntry = 0, hits = 0, misses = 0;
+            sample_star(SplitNodes{f.r, f.phi}, f, s, P, rng_jump(T, base[1], (uint64_t)j << 20, 20, jb1), n, j,
+                        rj, mj, E, Ej, J, kt, flags, ntry, hits, misses, red);
+            f.flags[j] = flags;
+            f.ntry[j] = ntry;
+            count_star(red, flags, ntry, hits, misses, false);
+            for (int i = 0; i < NRED; i++) part[t][i] = red[i];
+        }
+        __syncthreads();
+        if (t < NRED) {
+            const int cnt = nd - c0 < ORB_THREADS ? (int)(nd - c0) : ORB_THREADS;
+            double a = acc[t];
+            for (int u = 0; u < cnt; u++) {
+                const double p = part[u][t];
+                a = red_is_max(t) ? (a >= p ? a : p) : a + p;
+            }
+            acc[t] = a;
+        }
+        __syncthreads();
+    }
+    if (t < NRED) sums[t] = acc[t];
 }
 };
 
@@ -582,33 +611,11 @@ struct orbits_part_kernel {
 }
 };
 
-// Sums the first level's RED2_BLOCKS records in block order, then the deferred stars'
-// in star order, and finishes new_orbits_calculate (a fixed order: deterministic sums).
+// Finishes new_orbits_calculate from the phase's sums (defer_tail_kernel).
 struct orbits_end_kernel {
-    static constexpr int LB = RED2_BLOCKS * RED_STRIDE, MB = 0;
-    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s,
-                                                                              const double *__restrict__ red2,
-                                                                              const double *__restrict__ dpartials)
+    static constexpr int LB = 64, MB = 0;
+    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, const double *__restrict__ v)
 {
-    __shared__ double sv[RED2_BLOCKS * RED_STRIDE];
-    __shared__ double v[NRED];
-    sv[threadIdx.x] = red2[threadIdx.x];       // all loads at once
-    __syncthreads();
-    // thread k combines column k: the records in block order, then the deferred stars'
-    // in star order (the same order for every k as a single thread would take)
-    const int k = threadIdx.x;
-    if (k < NRED) {
-        double a = sv[k];
-        for (int b = 1; b < RED2_BLOCKS; b++)
-            a = red_is_max(k) ? (a >= sv[RED_STRIDE * b + k] ? a : sv[RED_STRIDE * b + k]) : a + sv[RED_STRIDE * b + k];
-        const int64_t nd = s->n_defer < MAX_DEFER ? s->n_defer : MAX_DEFER;
-        for (int64_t d = 0; d < nd; d++) {
-            const double p = dpartials[RED_STRIDE * d + k];
-            a = red_is_max(k) ? (a >= p ? a : p) : a + p;
-        }
-        v[k] = a;
-    }
-    __syncthreads();
     if (threadIdx.x == 0) {
         s->max_r = v[0];
         s->Eescaped += v[1];
@@ -894,15 +901,15 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
                        (const double2 *)sc.nodes, (const double2 *)sc.coarse,
                        (const NibbleTables<ORB_NIB_POWERS> *)sc.orb_nib, (const uint32_t *)sc.ridx, sc.ridx_bits);
     if (ev_b) hipEventRecord(ev_b, st);
-    kl<defer_prepare_kernel>(dim3(1), dim3(DEFER_T), 0, st, s, sc.defer_list, sc.defer_kfirst);
+    kl<defer_prepare_kernel>(dim3(1), dim3(DEFER_T), 0, st, s, sc.defer_list, sc.defer_kfirst, P.defer_cap);
     kl<defer_search_kernel>(dim3(256, 4), dim3(256), 0, st, f, s, P, (const int64_t *)sc.defer_list,
                        sc.defer_kfirst);
-    kl<defer_tail_kernel>(dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,
-                       (const int64_t *)sc.defer_list, (const int64_t *)sc.defer_kfirst, sc.defer_partials);
     kl<orbits_part_kernel>(dim3(RED2_BLOCKS), dim3(1024), 0, st, (const double *)sc.partials, nblocks,
                        sc.red2);
-    kl<orbits_end_kernel>(dim3(1), dim3(RED2_BLOCKS * RED_STRIDE), 0, st, s, (const double *)sc.red2,
-                       (const double *)sc.defer_partials);
+    kl<defer_tail_kernel>(dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,
+                       (const int64_t *)sc.defer_list, (const int64_t *)sc.defer_kfirst, (const double *)sc.red2,
+                       sc.orb_sums);
+    kl<orbits_end_kernel>(dim3(1), dim3(64), 0, st, s, (const double *)sc.orb_sums);
     return hipGetLastError();
 }
 

@@ -646,6 +646,28 @@ def test_gpu_many_linear_scan_stars(gpu, O):
     assert runs[0][0] == runs[1][0]
 
 
+def test_gpu_linear_scan_stars_beyond_65536(gpu, O):
+    """120000 stars all with v_r = 0: more linear-scan stars in one step than 65536
+    (the deferred list holds as many entries as the context holds stars, so any
+    count is finished).  Bit-exact against the oracle, and the phase's sums too."""
+    cm = gpu
+    say("120000 stars with v_r = 0: linear-scan stars beyond 65536")
+    cl = cm.make_plummer(120000, seed=12)
+    cl["vr"][1:120001] = 0.0
+    o = oracle_for(O, cl, rng_mode=O.RNG_PER_STAR, q_mode=O.Q_FP64)
+    o.init()
+    with ctx_for(cm, cl) as ctx:
+        upload_state(ctx, cl, o)
+        st = ctx.orbits(step=0)
+        d = ctx.download()
+    o.orbits()
+    N = o.N
+    say("linear-scan stars gpu", st["n_linscan"], "oracle", o.sc("n_linscan"))
+    assert st["n_linscan"] == o.sc("n_linscan") > 65536
+    for k in ("rnew", "vrnew", "vtnew", "r_apo", "r_peri", "X"):
+        assert np.array_equal(d[k][1:N + 1], o[k][1:N + 1]), k
+
+
 def test_gpu_host_shim_reference_names(gpu, O):
     """The reference-named C entry points drive the same kernels."""
     cm = gpu

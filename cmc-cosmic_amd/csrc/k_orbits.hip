@@ -523,7 +523,8 @@ struct defer_search_kernel {
 
 // The deferred stars, ORB_THREADS at a time in list order, and the phase's sums: the
 // sampler blocks' records (the first level's RED2_BLOCKS records, in block order), then
-// each deferred star's in list (= star) order, as a single thread would add them.
+// each deferred star's in list (= star) order, as a single thread would add them; then
+// the end of new_orbits_calculate.
 struct defer_tail_kernel {
     static constexpr int LB = ORB_THREADS, MB = 0;
     static __device__ __forceinline__ void run(KGrid kg_, DevFields f, DevScalars *s, KParams P,
@@ -531,8 +532,7 @@ struct defer_tail_kernel {
                                                                  const Rng *__restrict__ base, int jb1,
                                                                  const int64_t *__restrict__ list,
                                                                  const int64_t *__restrict__ kfirst,
-                                                                 const double *__restrict__ red2,
-                                                                 double *__restrict__ sums)
+                                                                 const double *__restrict__ red2)
 {
     __shared__ double part[ORB_THREADS][NRED];
     __shared__ double acc[NRED];
@@ -578,7 +578,25 @@ struct defer_tail_kernel {
         }
         __syncthreads();
     }
-    if (t < NRED) sums[t] = acc[t];
+    __syncthreads();
+    if (t == 0) {                           // new_orbits_calculate's results
+        s->max_r = acc[0];
+        s->Eescaped += acc[1];
+        s->Jescaped += acc[2];
+        s->TidalMassLoss += acc[3];
+        s->Etidal += acc[4];
+        s->TidalMassLoss += s->TidalMassLoss_old;    // cmc_utils.c:2210
+        s->n_circular = (int64_t)acc[5];
+        s->n_removed_E = (int64_t)acc[6];
+        s->n_removed_rt = (int64_t)acc[7];
+        s->n_destroyed = (int64_t)acc[8];
+        s->n_brent = (int64_t)acc[9];
+        s->n_linscan = (int64_t)acc[10];
+        s->tries_total = (int64_t)acc[11];
+        s->tries_max = (int64_t)acc[12];
+        s->n_bracket_hit = (int64_t)acc[13];
+        s->n_bracket_fallback = (int64_t)acc[14];
+    }
 }
 };
 
@@ -607,32 +625,6 @@ struct orbits_part_kernel {
         double a = gsum[0][t];
         for (int g2 = 1; g2 < 64; g2++) a = red_is_max(t) ? (a >= gsum[g2][t] ? a : gsum[g2][t]) : a + gsum[g2][t];
         out[RED_STRIDE * kg_.bid + t] = a;
-    }
-}
-};
-
-// Finishes new_orbits_calculate from the phase's sums (defer_tail_kernel).
-struct orbits_end_kernel {
-    static constexpr int LB = 64, MB = 0;
-    static __device__ __forceinline__ void run(KGrid kg_, DevScalars *s, const double *__restrict__ v)
-{
-    if (threadIdx.x == 0) {
-        s->max_r = v[0];
-        s->Eescaped += v[1];
-        s->Jescaped += v[2];
-        s->TidalMassLoss += v[3];
-        s->Etidal += v[4];
-        s->TidalMassLoss += s->TidalMassLoss_old;    // cmc_utils.c:2210
-        s->n_circular = (int64_t)v[5];
-        s->n_removed_E = (int64_t)v[6];
-        s->n_removed_rt = (int64_t)v[7];
-        s->n_destroyed = (int64_t)v[8];
-        s->n_brent = (int64_t)v[9];
-        s->n_linscan = (int64_t)v[10];
-        s->tries_total = (int64_t)v[11];
-        s->tries_max = (int64_t)v[12];
-        s->n_bracket_hit = (int64_t)v[13];
-        s->n_bracket_fallback = (int64_t)v[14];
     }
 }
 };
@@ -907,9 +899,7 @@ hipError_t launch_orbits(const DevFields &f, DevScalars *s, const KParams &P, co
     kl<orbits_part_kernel>(dim3(RED2_BLOCKS), dim3(1024), 0, st, (const double *)sc.partials, nblocks,
                        sc.red2);
     kl<defer_tail_kernel>(dim3(1), dim3(ORB_THREADS), 0, st, f, s, P, T, sc.rng_base, jb1,
-                       (const int64_t *)sc.defer_list, (const int64_t *)sc.defer_kfirst, (const double *)sc.red2,
-                       sc.orb_sums);
-    kl<orbits_end_kernel>(dim3(1), dim3(64), 0, st, s, (const double *)sc.orb_sums);
+                       (const int64_t *)sc.defer_list, (const int64_t *)sc.defer_kfirst, (const double *)sc.red2);
     return hipGetLastError();
 }
 

@@ -84,7 +84,6 @@ struct Scratch {
     int32_t *strip_done;
     // orbit stars deferred to the linear-scan path (65536 slots)
     int64_t *defer_list, *defer_kfirst;   // KParams::defer_cap entries
-    double *orb_sums;                   // the orbit phase's sums (16)
     // RNG bases of the current step: [0] relaxation, [1] orbits; seeded state
     Rng *rng_base;
     Rng *rng_seeded;

@@ -441,9 +441,11 @@ int cmcg_create(cmcg_ctx **out, int device, int64_t n_cap, const cmcg_params *pa
     sc.max_sort_tiles = (n_cap + SORT_TILE - 1) / SORT_TILE + 1;
     sc.merge_ends_cap = 2 * (n_cap / 256 + 4);       // two per merge block of set_velocities3 (>= 256 stars)
     sc.sort_aos_cap = n_cap + 4;
-    // about one radial-index bucket per star: 2^10 .. 2^22 buckets
+    // about two radial-index buckets per star: 2^10 .. 2^23 buckets (at N = 4e6 one per
+    // star makes the orbit sampler 3% slower; four per star cost more to build than the
+    // sampler gains)
     sc.ridx_bits = 10;
-    while (sc.ridx_bits < 22 && ((int64_t)1 << sc.ridx_bits) < n_cap + 2) sc.ridx_bits++;
+    while (sc.ridx_bits < 23 && ((int64_t)1 << sc.ridx_bits) < 2 * (n_cap + 2)) sc.ridx_bits++;
     if ((e = dalloc(c, &sc.ridx, (size_t)ridx_entries(sc.ridx_bits))) ||
         (e = dalloc(c, &sc.ridx_tb, (size_t)ridx_tiles(sc.ridx_bits) + 1))) { cmcg_destroy(c); return e; }
     if ((e = dalloc(c, &sc.tile_sums, (size_t)sc.max_tiles * 6)) || (e = dalloc(c, &sc.partials, (size_t)sc.max_blocks * 16)) ||
